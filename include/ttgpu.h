@@ -1,0 +1,260 @@
+/*
+ * ttgpu.h -- C ABI of the MI355X (gfx950) implementation of ttcode's per-step
+ * hot path: ADVECT3D, FORGAS3D, COUPLED_ENERGY_EQS (FLD build + red-black SOR),
+ * BOUNDS, SCHRITT and DIFFUSION, on one rank's whole subdomain held in HBM.
+ *
+ * The reference exposes parameterless Fortran subroutines that read and write
+ * module globals (ppmod.f).  This library owns a device-resident copy of those
+ * globals; the host moves arrays across with tt_upload/tt_download in the
+ * reference's own wire format (Fortran column-major, r fastest, with each
+ * array's declared lower/upper bounds, ghosts included), and calls the entry
+ * points below in place of the reference subroutines.
+ *
+ * Conventions
+ *  - Every entry point returns 0 on success.  A positive return is fatal (the
+ *    reference would have called CLEAN_STOP or STOP); a negative return is
+ *    informational (an iteration cap was reached, which the reference treats
+ *    as non-fatal, ppsor.f:161-174, ppenergy.f:750-775).  tt_last_error()
+ *    describes the last non-zero return.
+ *  - One host thread; all device work is ordered on one HIP stream owned by
+ *    the library.  Host-side module arrays are stale between tt_download
+ *    calls.
+ *  - Everything here is plain C: pointers, sizes, ints and doubles.
+ */
+#ifndef TTGPU_H
+#define TTGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------
+ * Run parameters.  Replaces MODULE input_init (ppmod.f:2-33) as read from the
+ * positional control file ipp by readinitfile (ppmod.f:38-78), restricted to
+ * the switches the hot path reads, plus the power-law opacity that ttcode's
+ * KAPPA dispatcher (pprphy.f:1-360) does not have.
+ * Only NCOSYS=2 (spherical polar), MODTYP=2 (planet), IGASTYP=1 (ideal gas),
+ * ICSTYP=0, IRAD=3 (SOR), IVIS=0, IARTVIS=0 and inc_poles=0 are supported.
+ * ------------------------------------------------------------------------ */
+typedef struct tt_params {
+    /* integers (keep the order: a Fortran bind(C) type mirrors it) */
+    int32_t nx, ny, nz;      /* locNX, locNY, locNZ  (r, phi, theta)          */
+    int32_t bcxmin;          /* BOUNDXMIN type at r_min; 12 supported        */
+    int32_t ibdrad;          /* BOUNDXMAX(2) top-T version: 3, 4, 7 or 9      */
+    int32_t iflimtyp;        /* flux limiter: 1 = 1/3, 2 = Levermore-Pomraning */
+    int32_t f_incident;      /* 0 = no irradiation, 2 = implicit cos heating  */
+    int32_t ikaptyp;         /* 1 = constant ZKAPMULT, 26 = power law (new)   */
+    int32_t isigtyp;         /* 0 = no scattering, 1 = 0.4 cm^2/g             */
+    int32_t isoth;           /* 0 = energy equation on                        */
+    int32_t itmax;           /* ITMAX, shared by SOR and Newton (ipp:13)      */
+    int32_t pad0;
+    /* doubles */
+    double xmin, xmax;       /* r range, cm                                   */
+    double ymin, ymax;       /* phi range in units of 2 pi (ipp YMIN,YMAX)    */
+    double zmin, zmax;       /* latitude range in units of pi/2 (ipp ZMIN..)  */
+    double gamma, mu_gas;    /* GAMMA, mu_gas                                 */
+    double zkapmult;         /* IKAPTYP=1 opacity                             */
+    double kap0, dkap0;      /* IKAPTYP=26: kappa = kap0 (rho/kap_rho0)^a (T/kap_t0)^b,
+                                dkappa/dT = dkap0 (rho/kap_rho0)^a (T/kap_t0)^(b-1) */
+    double kap_rho0, kap_t0, kap_a, kap_b;
+    double tirr;             /* TIRR, K                                       */
+    double fbottom;          /* FBottom, bottom radiative flux                */
+    double d_t;              /* D_T, fixed bottom dT/dr for BCXMIN=12         */
+    double omrot;            /* OMROT, frame rotation rate (0 if IROTATE=0)   */
+    double sorparam;         /* SORPARAM (omega)                              */
+    double epsmat;           /* EPSMAT, L1 during-sweep tolerance             */
+    double fdelt, dtmax;     /* FDELT (CFL factor), DTMAX                     */
+} tt_params;
+
+/* ------------------------------------------------------------------------
+ * Field ids for tt_upload / tt_download.  Ids 1..11 follow PLANETSHIFT's
+ * numbering (ppshift.f:1013-1022).  Bounds are the reference's declared
+ * bounds (ppmod.f allocate*var); n = (nx,ny,nz).
+ * ------------------------------------------------------------------------ */
+enum {
+    TT_T = 1,          /* T   (-1:nx+2,-1:ny+2,-1:nz+2)  ppmod.f:322 */
+    TT_RH = 2,         /* RH  (-1:nx+2,-1:ny+2,-1:nz+2)  ppmod.f:325 */
+    TT_V = 3,          /* V   ( 0:nx+2,-1:ny+2,-1:nz+2)  ppmod.f:326 */
+    TT_G = 4,          /* G   (-1:nx+2, 0:ny+2,-1:nz+2)  ppmod.f:327 */
+    TT_H = 5,          /* H   (-1:nx+2,-1:ny+2, 0:nz+2)  ppmod.f:328 */
+    TT_ER = 6,         /* ER  ghosted                    ppmod.f:323 */
+    TT_DELTA_ER = 8,   /* delta_ER ghosted               ppmod.f:543 */
+    TT_GD = 11,        /* GD ghosted                     ppmod.f:329 */
+    TT_PG = 20,        /* PG ghosted */
+    TT_CS = 21,        /* CS ghosted */
+    TT_CV = 22,        /* CV ghosted (constant) */
+    TT_RHQX = 23,      /* RHQX ( 0:nx+2,-1:ny+2,-1:nz+2) */
+    TT_RHQY = 24,      /* RHQY (-1:nx+2, 0:ny+2,-1:nz+2) */
+    TT_RHQZ = 25,      /* RHQZ (-1:nx+2,-1:ny+2, 0:nz+2) */
+    TT_XKAPR = 26, TT_XKAPP = 27, TT_XKAPA = 28,   /* ghosted */
+    TT_DXKAPP_DT = 29, TT_DXKAPA_DT = 30,          /* ghosted */
+    TT_SIG = 31,       /* sig (0:nx+1,0:ny+1,0:nz+1) */
+    TT_GRAVX = 32, TT_GRAVY = 33, TT_GRAVZ = 34,   /* ghosted */
+    TT_STELLARINPUT = 35, /* (1:nx,-1:ny+2,-1:nz+2) */
+    TT_STELLARTAU = 36,   /* (1:nx+1,-1:ny+2,-1:nz+2) */
+    TT_DTAUS = 37,        /* (1:nx+1,-1:ny+2,-1:nz+2) */
+    TT_ER_OLD = 38,       /* ghosted */
+    TT_RHS_ER = 40,       /* (1:nx,1:ny,1:nz) */
+    TT_SOR_A = 41, TT_SOR_B, TT_SOR_C, TT_SOR_D, TT_SOR_E, TT_SOR_F, TT_SOR_G, /* 41..47 (1:n) */
+    TT_EVAL1 = 50, TT_EVAL2, TT_EVAL3, TT_EVAL4, TT_EVAL5, TT_EVAL6,          /* 50..55 (1:n) */
+    TT_EVAL7 = 56, TT_EVAL8 = 57, TT_EVAL_1_6 = 58, TT_JACOB_ETA = 59,
+    TT_FQ = 60, TT_FR = 61,                         /* (1:n) */
+    TT_FLIMX = 62, TT_FLIMY = 63, TT_FLIMZ = 64,    /* (1:nx+1,1:ny,1:nz) etc. */
+    TT_DIFRX = 65, TT_DIFRY = 66, TT_DIFRZ = 67,
+    TT_FLUX_X = 68, TT_FLUX_Y = 69, TT_FLUX_Z = 70,
+    TT_CENTCORX = 71, TT_CENTCORY = 72, TT_CENTCORZ = 73, /* (1:nx+1,1:ny+1,1:nz+1) */
+    TT_UPPERBND = 80,     /* int32 (-1:ny+2,-1:nz+2)  ppmod.f:321 */
+    TT_MU_STAR = 81,      /* (-1:ny+2,-1:nz+2)        ppmod.f:549 */
+    TT_Q = 82,            /* Q of COUPLED_ENERGY_EQS, ghosted (local there) */
+    TT_FIELD_MAX = 90
+};
+
+/* Declared bounds of a field: lo[3], hi[3] (hi[2]=lo[2]=0 and rank 2 for
+ * upperbnd/mu_star); *is_int = 1 for int32 fields.  Returns 0, or 1 for an
+ * unknown id.  Needs tt_init (sizes come from it). */
+int tt_field_info(int field, int32_t lo[3], int32_t hi[3], int32_t* rank, int32_t* is_int);
+
+/* Device set-up.  Replaces the device side of allocate*var (ppmod.f:310-911)
+ * plus GITTER (ppinit.f:1028-1406), GASINIT (ppinit.f:1408) and CALCMUSTAR
+ * (ppsub.f:156): allocates every device array, computes the 1-D metric on the
+ * host and uploads it, sets CV and mu_star.  Device 0 of the visible set. */
+int tt_init(const tt_params* p);
+/* Release all device memory and the stream. */
+void tt_finalize(void);
+const char* tt_last_error(void);
+
+/* Host <-> device transfer of one module array in Fortran order. */
+int tt_upload(int field, const void* host);
+int tt_download(int field, void* host);
+/* 1-D metric override: id per TT_M_* below, data in the declared bounds. */
+int tt_upload_metric(int metric, const double* host);
+int tt_download_metric(int metric, double* host);
+enum {
+    TT_M_XXA = 0, TT_M_XXB, TT_M_DXA, TT_M_DXB, TT_M_XYA, TT_M_XYB, TT_M_DYA, TT_M_DYB,
+    TT_M_XZA, TT_M_XZB, TT_M_DZA, TT_M_DZB, TT_M_DDX0, TT_M_DDX1, TT_M_DDY0, TT_M_DDY1,
+    TT_M_DDZ0, TT_M_DDZ1, TT_M_DCY0, TT_M_DCY1, TT_M_DDX0KEP, TT_M_DDX1KEP,
+    TT_M_SURXA, TT_M_SURXB, TT_M_VOLXA, TT_M_VOLXB, TT_M_CCXA, TT_M_CCXB,
+    TT_M_SURYA, TT_M_SURYB, TT_M_VOLYA, TT_M_VOLYB, TT_M_CCYA, TT_M_CCYB,
+    TT_M_SURZA, TT_M_SURZB, TT_M_VOLZA, TT_M_VOLZB, TT_M_CCZA, TT_M_CCZB,
+    TT_M_GEOXGA, TT_M_GEOXHA, TT_M_GEOXG, TT_M_GEOXH, TT_M_GEOZG, TT_M_GEOZGA,
+    TT_M_COUNT
+};
+/* Declared bounds of a metric array (lo..hi). */
+int tt_metric_info(int metric, int32_t* lo, int32_t* hi);
+
+/* ------------------------------------------------------------------------
+ * Solver options (new: ttcode has one fixed path).  precision 0 is the
+ * reference algorithm: FP64 red-black SOR from delta_ER = 0 (ppsor.f:29) until
+ * the L1 during-sweep ratio <= epsmat (ppsor.f:176).  precision 1 runs FP32
+ * SOR sweeps inside an FP64 defect-correction loop whose FP64 L1 residual
+ * ratio sum|RHS - A d| / sum|RHS| must reach epsmat.
+ * guess 0 = cold (delta_ER = 0), 1 = warm (previous solve's delta_ER times
+ * warm_scale), 2 = learned (CNN prediction); a warm or learned guess is kept
+ * only if its FP64 residual does not exceed the cold start's.
+ * ------------------------------------------------------------------------ */
+typedef struct tt_solver_opts {
+    int32_t precision;       /* 0 = FP64 SOR, 1 = FP32 SOR + FP64 correction */
+    int32_t guess;           /* 0 cold, 1 warm, 2 learned                    */
+    int32_t batch;           /* sweeps queued between host convergence checks */
+    int32_t inner_max;       /* FP32 sweeps per correction (precision 1)     */
+    double inner_tol;        /* FP32 inner L1 ratio target (precision 1)     */
+    double warm_scale;       /* scale of the previous delta_ER for guess 1   */
+} tt_solver_opts;
+
+typedef struct tt_sor_stats {
+    int32_t sor_iter;        /* SOR_ITER: full red+black iterations          */
+    int32_t outer_iter;      /* FP64 corrections (precision 1), else 0       */
+    int32_t fp32_sweeps;     /* FP32 red+black iterations (precision 1)      */
+    int32_t guess_used;      /* 0 cold, 1 warm, 2 learned actually used      */
+    double rhsnorm;          /* global_RHSNORM                               */
+    double rnorm;            /* global_RNORM of the last iteration           */
+    double r0_guess, r0_cold;/* FP64 L1 residual of the guess and of zero    */
+} tt_sor_stats;
+
+int tt_set_solver(const tt_solver_opts* o);
+int tt_get_solver(tt_solver_opts* o);
+
+/* SOLVE_WITH_SOR(ppsor.f:2) on the system already on the device:
+ * fill_matrix != 0 runs FILL_SOR_MATRIX + B_VECTOR_BOUNDARIES from EVAL1..8,
+ * EVAL_1_6, RHS_ER and upperbnd (ppsor.f:18-27) first; fill_matrix == 0 takes
+ * SOR_A..G and RHS_ER as uploaded.  Result in delta_ER. */
+int tt_solve_sor(int fill_matrix, tt_sor_stats* st);
+
+/* ------------------------------------------------------------------------
+ * Physics entry points, in the step order of pp.f:483-821.
+ * ------------------------------------------------------------------------ */
+int tt_bounds(void);          /* BOUNDS      ppbou.f:1      */
+int tt_pressg(void);          /* PRESSG      ppphy.f:1      */
+int tt_qdens(void);           /* QDENS       ppphy.f:47     */
+int tt_sound(void);           /* SOUND       ppphy.f:79     */
+int tt_advect3d(double delt); /* ADVECT3D    ppadv.f:525    */
+int tt_forgas3d(double delt); /* FORGAS3D    ppfor.f:142    */
+
+typedef struct tt_energy_stats {
+    int32_t energy_iter;     /* Newton iterations                             */
+    int32_t sor_iter_total;  /* sum of SOR_ITER over the Newton iterations    */
+    int32_t sor_iter_last;   /* SOR_ITER of the last solve                    */
+    int32_t sor_iter_first;  /* SOR_ITER of the first solve                   */
+    int32_t fp32_sweeps;     /* FP32 iterations (precision 1)                 */
+    int32_t fallbacks;       /* guesses rejected for the cold start           */
+    int32_t learned_used;    /* solves that used the learned guess            */
+    int32_t pad0;
+    double max_dq, max_de;   /* global_deltaQ, global_deltaER of last iter    */
+} tt_energy_stats;
+
+/* COUPLED_ENERGY_EQS (ppenergy.f:2-791).  iii is the step counter III (read
+ * by IBDRAD=9).  ADJUST_SOR_PARAM (ppenergy.f:372) is never called. */
+int tt_coupled_energy(double delt, int iii, tt_energy_stats* st);
+
+/* SCHRITT (ppsub.f:1-137): CFL time step into *delt (the positivity check
+ * returns 2 when rho, T or PG is non-positive). */
+int tt_schritt(double* delt);
+
+/* DIFFUSION (ppdif.f:1-167) applied to T with constant diffusivity
+ * xnue/pr_num (the IDIF=2 call of pp.f:765-770); returns SOR iterations in
+ * *iters.  ITMIN=5, SORITMAX=500 as in ppdif.f:14-15. */
+int tt_diffusion(double delt, double xnue, double pr_num, int* iters);
+
+typedef struct tt_step_stats {
+    tt_energy_stats energy;
+    double ms_advect, ms_force, ms_energy, ms_total;  /* device time, ms */
+} tt_step_stats;
+
+/* One step of the pp.f:483-821 loop for IADV=IFOR=1, IRAD=3: ADVECT3D,
+ * FORGAS3D, COUPLED_ENERGY_EQS, BOUNDS, BOUNDS, PRESSG (time step held fixed). */
+int tt_step(double delt, int iii, tt_step_stats* st);
+
+/* Block until all queued device work is done. */
+int tt_sync(void);
+
+/* Per-launch device timing of the SOR half-sweep kernels (HIP events on the
+ * library's stream).  tt_sweep_timing(1) resets and enables; the getter
+ * returns launches and summed milliseconds since the reset, split by FP64 and
+ * FP32 sweeps. */
+int tt_sweep_timing(int enable);
+int tt_sweep_timing_get(int64_t* n64, double* ms64, int64_t* n32, double* ms32);
+
+/* ------------------------------------------------------------------------
+ * Learned initial guess (new).  A 3-D CNN maps 5 channels
+ * (E^n, T, rho, kappa_R, S) to delta_E.  Weights are BF16 on the device,
+ * accumulation FP32 on MFMA.  Layout: layer l has cin x cout x 27 weights
+ * [cout][kz][ky][kx][cin] and cout FP32 biases; layers 5->16->16->16->16->1.
+ * ------------------------------------------------------------------------ */
+int tt_cnn_set_weights(const float* w, int64_t nw, const float* b, int64_t nb);
+int tt_cnn_get_weights(float* w, int64_t nw, float* b, int64_t nb);
+int64_t tt_cnn_num_weights(void);
+int64_t tt_cnn_num_biases(void);
+/* Forward on host-provided channels-last input x[nz][ny][nx][5] (FP32,
+ * already normalised) into y[nz][ny][nx] (FP32).  For tests. */
+int tt_cnn_forward_host(int nx, int ny, int nz, const float* x, float* y);
+/* One SGD/Adam step on (x, target) pairs held on the host; returns the loss
+ * and fills grads (same layout as weights/biases) when non-null. */
+int tt_cnn_train_step_host(int nx, int ny, int nz, const float* x, const float* target,
+                           float lr, float* loss, float* gw, float* gb);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TTGPU_H */

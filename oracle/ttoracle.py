@@ -1,0 +1,130 @@
+"""ctypes binding of oracle/libttoracle.so, the CPU FP64 restatement of the
+ttcode hot path.  TEST INFRASTRUCTURE ONLY: imported by tests/,
+__graft_entry__.smoke() and bench.py's CPU-baseline leg, never by the product
+path (ttcode_amd/)."""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+from ttcode_amd import params as P
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "libttoracle.so")
+
+
+def build(force: bool = False) -> str:
+    """Compile oracle/ttoracle.c (gcc, no FMA contraction, OpenMP)."""
+    if force or not os.path.exists(LIB):
+        subprocess.check_call(["make", "-C", HERE, "-f", "Makefile", "liboracle"])
+    return LIB
+
+
+class Oracle:
+    """The oracle holds one state (like the reference's module globals)."""
+
+    _lib = None
+
+    def __init__(self, p: P.TTParams, threads: int = 1):
+        if Oracle._lib is None:
+            build()
+            Oracle._lib = ctypes.CDLL(LIB)
+            L = Oracle._lib
+            L.tor_last_error.restype = ctypes.c_char_p
+            for name in ("tor_advect3d", "tor_forgas3d"):
+                getattr(L, name).argtypes = [ctypes.c_double]
+            L.tor_coupled_energy.argtypes = [ctypes.c_double, ctypes.c_int,
+                                             ctypes.POINTER(P.EnergyStats)]
+            L.tor_step.argtypes = [ctypes.c_double, ctypes.c_int, ctypes.POINTER(P.StepStats)]
+            L.tor_solve_sor.argtypes = [ctypes.c_int, ctypes.POINTER(P.SorStats)]
+            L.tor_schritt.argtypes = [ctypes.POINTER(ctypes.c_double)]
+            L.tor_diffusion.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                        ctypes.POINTER(ctypes.c_int)]
+        self.L = Oracle._lib
+        self.p = p
+        self.L.tor_set_threads(int(threads))
+        self._check(self.L.tor_init(ctypes.byref(p)), "tor_init")
+
+    def _check(self, rc, what):
+        if rc > 0:
+            raise RuntimeError(f"{what}: {self.L.tor_last_error().decode()} (rc={rc})")
+        return rc
+
+    def set_threads(self, n):
+        self.L.tor_set_threads(int(n))
+
+    # -- data movement (wire format) --
+    def upload(self, f, a):
+        if isinstance(f, str):
+            f = P.FIELD_IDS[f]
+        shape, _, dt = P.wire_shape(f, self.p.nx, self.p.ny, self.p.nz)
+        a = np.ascontiguousarray(a, dtype=dt)
+        assert a.shape == shape, (f, a.shape, shape)
+        self._check(self.L.tor_upload(f, a.ctypes.data_as(ctypes.c_void_p)), "upload")
+
+    def download(self, f):
+        if isinstance(f, str):
+            f = P.FIELD_IDS[f]
+        out = P.empty_field(f, self.p.nx, self.p.ny, self.p.nz)
+        self._check(self.L.tor_download(f, out.ctypes.data_as(ctypes.c_void_p)), "download")
+        return out
+
+    def metric(self, m):
+        lo, hi = ctypes.c_int32(), ctypes.c_int32()
+        self.L.tor_metric_info(m, ctypes.byref(lo), ctypes.byref(hi))
+        out = np.zeros(hi.value - lo.value + 1)
+        self.L.tor_download_metric(m, out.ctypes.data_as(ctypes.c_void_p))
+        return out
+
+    def load_state(self, st: dict):
+        for name, a in st.items():
+            if name in P.FIELD_IDS:
+                self.upload(P.FIELD_IDS[name], a)
+
+    # -- routines --
+    def bounds(self):
+        return self._check(self.L.tor_bounds(), "BOUNDS")
+
+    def pressg(self):
+        return self._check(self.L.tor_pressg(), "PRESSG")
+
+    def advect3d(self, delt):
+        return self._check(self.L.tor_advect3d(delt), "ADVECT3D")
+
+    def forgas3d(self, delt):
+        return self._check(self.L.tor_forgas3d(delt), "FORGAS3D")
+
+    def coupled_energy(self, delt, iii=1):
+        st = P.EnergyStats()
+        rc = self._check(self.L.tor_coupled_energy(delt, iii, ctypes.byref(st)), "COUPLED_ENERGY_EQS")
+        return st, rc
+
+    def solve_sor(self, fill=True):
+        st = P.SorStats()
+        rc = self._check(self.L.tor_solve_sor(1 if fill else 0, ctypes.byref(st)), "SOLVE_WITH_SOR")
+        return st, rc
+
+    def schritt(self):
+        d = ctypes.c_double()
+        self._check(self.L.tor_schritt(ctypes.byref(d)), "SCHRITT")
+        return d.value
+
+    def diffusion(self, delt, xnue, pr_num):
+        it = ctypes.c_int()
+        rc = self._check(self.L.tor_diffusion(delt, xnue, pr_num, ctypes.byref(it)), "DIFFUSION")
+        return it.value, rc
+
+    def step(self, delt, iii=1):
+        st = P.StepStats()
+        rc = self._check(self.L.tor_step(delt, iii, ctypes.byref(st)), "step")
+        return st, rc
+
+    def field_info(self, f):
+        lo = (ctypes.c_int32 * 3)()
+        hi = (ctypes.c_int32 * 3)()
+        rank, isint = ctypes.c_int32(), ctypes.c_int32()
+        rc = self.L.tor_field_info(f, lo, hi, ctypes.byref(rank), ctypes.byref(isint))
+        return rc, tuple(lo), tuple(hi), rank.value, isint.value
