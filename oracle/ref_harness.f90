@@ -146,6 +146,10 @@ program tt_ref_harness
   case ('advect')
      call ADVECT3D
   case ('forgas')
+     ! FORGAS3D reads the face densities ADVECT3D leaves behind (it calls
+     ! QDENS itself only when IADV=0, ppfor.f:167)
+     call BOUNDS
+     call QDENS
      call FORGAS3D
   case ('energy')
      call COUPLED_ENERGY_EQS

@@ -1175,7 +1175,9 @@ static void energy_z_bc(void) /* ppenergy.f:204-227 / :727-757 */
 
 int tor_coupled_energy(double delt, int iii, tt_energy_stats* st)
 {
-    const double delta_max = 0.005, conv_parm = 0.1, theta = 1.0;
+    /* conv_parm = 0.1 is a REAL literal (ppenergy.f:49): the damping in use
+     * is the nearest float, 0.100000001490116 */
+    const double delta_max = 0.005, conv_parm = (double)0.1f, theta = 1.0;
     int rc;
     if (O.p.bcxmin != 12) return seterr(1, "COUPLED_ENERGY_EQS: only BCXMIN=12 is supported");
     g_iii = iii;

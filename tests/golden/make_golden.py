@@ -157,7 +157,6 @@ def main():
     q.dkap0 = float(np.float32(0.00342))
     oq = Oracle(q)
     oq.load_state(inp8)
-    oq.bounds()
     oq.set_threads(1)
     # one FLIM-level KAPPA evaluation: run energy for 0 Newton work is not
     # exposed, so take the opacities the oracle computes in a full call

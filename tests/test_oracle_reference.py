@@ -1,0 +1,110 @@
+"""The CPU oracle against the reference Fortran's own outputs (golden
+fixtures written by tests/golden/make_golden.py from a flang build of the
+reference).  Compiled without FMA contraction the oracle follows the
+reference operation for operation, so agreement is bit for bit."""
+import numpy as np
+import pytest
+
+from oracle.ttoracle import Oracle
+from tests.golden import compare as C
+
+
+def _run(name, action):
+    g = C.load(name)
+    p = C.params_of(g)
+    o = Oracle(p, threads=1)
+    o.load_state(C.inputs_of(g))
+    extra = action(o, g)
+    return g, p, o, extra
+
+
+def _check_fields(g, p, o, skip=()):
+    for k, ref in C.outputs_of(g).items():
+        if k in skip:
+            continue
+        reg = C.compare_region(k, p.nx, p.ny, p.nz)
+        mine = o.download(k)[reg]
+        want = ref[reg]
+        same = (mine == want) | (np.isnan(mine) & np.isnan(want)) if mine.dtype.kind == "f" else mine == want
+        assert same.all(), f"{k}: {np.count_nonzero(~same)} cells differ, max rel {C.rel_err(mine, want):.3e}"
+
+
+def test_gitter_metrics():
+    g = C.load("bounds_12x16x12")
+    p = C.params_of(g)
+    o = Oracle(p)
+    names = {"xxa": 0, "xxb": 1, "dxa": 2, "dxb": 3, "xya": 4, "xyb": 5, "dya": 6, "dyb": 7,
+             "xza": 8, "xzb": 9, "dza": 10, "dzb": 11, "ddx0kep": 20, "ddx1kep": 21,
+             "surxa": 22, "surxb": 23, "volxa": 24, "volxb": 25, "surza": 34, "surzb": 35,
+             "volza": 36, "volzb": 37, "ccza": 38, "geoxga": 40, "geoxg": 42, "geozg": 44,
+             "geozga": 45}
+    for n, m in names.items():
+        np.testing.assert_array_equal(o.metric(m), g["out_m_" + n], err_msg=n)
+
+
+def test_bounds():
+    g, p, o, _ = _run("bounds_12x16x12", lambda o, g: o.bounds())
+    _check_fields(g, p, o)
+
+
+def test_advect3d():
+    g, p, o, _ = _run("advect_12x16x12", lambda o, g: o.advect3d(float(g["delt"])))
+    _check_fields(g, p, o)
+
+
+def test_forgas3d():
+    def act(o, g):
+        o.bounds()
+        o.L.tor_qdens()
+        o.forgas3d(float(g["delt"]))
+    g, p, o, _ = _run("forgas_12x16x12", act)
+    _check_fields(g, p, o)
+
+
+@pytest.mark.parametrize("name", ["energy_const_12x16x12", "energy_pow8_12x16x12"])
+def test_coupled_energy(name):
+    g, p, o, st = _run(name, lambda o, g: o.coupled_energy(float(g["delt"]), int(g["iii"]))[0])
+    assert st.energy_iter == int(g["scal_energy_iter"])
+    assert st.sor_iter_last == int(g["scal_SOR_ITER"])
+    _check_fields(g, p, o)
+
+
+def test_solve_with_sor():
+    def act(o, g):
+        for k in ("EVAL1", "EVAL2", "EVAL3", "EVAL4", "EVAL5", "EVAL6", "EVAL7", "EVAL8",
+                  "EVAL_1_6", "RHS_ER", "upperbnd"):
+            o.upload(k, g["in_" + k])
+        return o.solve_sor(True)[0]
+    g, p, o, st = _run("sor_12x16x12", act)
+    assert st.sor_iter == int(g["scal_SOR_ITER"])
+    assert st.rnorm == float(g["scal_global_RNORM"])
+    assert st.rhsnorm == float(g["scal_global_RHSNORM"])
+    _check_fields(g, p, o)
+
+
+def test_two_steps():
+    def act(o, g):
+        for s in range(int(g["nsteps"])):
+            st, _ = o.step(float(g["delt"]), int(g["iii"]) + s)
+        return st
+    g, p, o, st = _run("step2_12x16x12", act)
+    assert st.energy.energy_iter == int(g["scal_energy_iter"])
+    assert st.energy.sor_iter_last == int(g["scal_SOR_ITER"])
+    _check_fields(g, p, o)
+
+
+def test_diffusion():
+    def act(o, g):
+        o.bounds()
+        return o.diffusion(float(g["delt"]), float(g["xnue"]), float(g["pr_num"]))[0]
+    g, p, o, it = _run("diffusion_12x16x12", act)
+    assert it == int(g["scal_SOR_ITER"])
+    _check_fields(g, p, o)
+
+
+def test_reference_constants():
+    """The reference stores its physical constants as floats (ppmod.f:1081)."""
+    from ttcode_amd import params as P
+    g = C.load("bounds_12x16x12")
+    assert P.FD == float(g["scal_FD"])
+    assert P.ARAD == float(g["scal_ARAD"])
