@@ -1,0 +1,180 @@
+"""Benchmark of the ttcode hot path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Workload: the production-size plain method -- spherical-polar staggered grid
+n_r x n_theta x n_phi = 128 x 128 x 256, FP64 throughout, one time step =
+ADVECT3D + FORGAS3D + COUPLED_ENERGY_EQS (Newton loop with red-black SOR,
+omega = 1.7, to an L1 residual ratio of 1e-10) + BOUNDS + PRESSG, with the
+time step fixed at its CFL = 0.4 value from step 0.  The state starts from
+the seeded synthetic generator (ttcode_amd.params.generate_state) and stays
+in HBM; the timed region holds K consecutive steps.
+
+Prints one JSON line: ms/step of the plain FP64 method (`value`), SOR sweeps
+per step, cell-updates/s, the same for FP32 SOR inside FP64 defect
+correction (`mixed`), the HBM roofline of the dominant kernel (the FP64 SOR
+half sweep, timed per launch with HIP events on the library's stream), and
+the CPU oracle timed on the host cores as the reported CPU baseline.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from ttcode_amd import params as P  # noqa: E402
+
+METRIC = "ms/step and SOR sweeps to 1e-10 residual: learned+FP32 SOR vs plain FP64 SOR"
+HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "sor_sweep_fp64_sph128x128x256_traffic.txt")
+
+
+def read_traffic():
+    """HBM bytes per FP64 half-sweep launch measured by rocprofv3 --pmc
+    (FETCH_SIZE + WRITE_SIZE), from the committed profile summary."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
+def run_mode(dev, st0, dt, steps, warmup, precision, guess):
+    dev.load_state(st0)
+    dev.set_solver(precision=precision, guess=guess, batch=16, inner_tol=1e-5, inner_max=20000)
+    iii = 1
+    for _ in range(warmup):
+        dev.step(dt, iii)
+        iii += 1
+    dev.sync()
+    dev.sweep_timing(True)
+    newton = sor = f32 = 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s, _ = dev.step(dt, iii)
+        iii += 1
+        newton += s.energy.energy_iter
+        sor += s.energy.sor_iter_total
+        f32 += s.energy.fp32_sweeps
+    dev.sync()
+    t1 = time.perf_counter()
+    n64, ms64, n32, ms32 = dev.sweep_timing_get()
+    dev.sweep_timing(False)
+    return {
+        "ms_per_step": (t1 - t0) * 1e3 / steps,
+        "newton_per_step": newton / steps,
+        "sor_sweeps_per_step": sor / steps,
+        "fp32_sweeps_per_step": f32 / steps,
+        "sweep64_launches": n64, "sweep64_ms": ms64,
+        "sweep32_launches": n32, "sweep32_ms": ms32,
+    }
+
+
+def cpu_baseline(p, st0, dt, threads):
+    """One full step of the CPU oracle at the same size on the host cores."""
+    from oracle.ttoracle import Oracle
+    o = Oracle(p, threads=threads)
+    o.load_state(st0)
+    t0 = time.perf_counter()
+    s, _ = o.step(dt, 1)
+    t1 = time.perf_counter()
+    return (t1 - t0) * 1e3, s
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--nr", type=int, default=128)
+    ap.add_argument("--ntheta", type=int, default=128)
+    ap.add_argument("--nphi", type=int, default=256)
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
+              "(the multi-GPU decomposition is a design, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from ttcode_amd import ttgpu
+    nx, ny, nz = a.nr, a.nphi, a.ntheta
+    p = P.make_params(nx, ny, nz, epsmat=1e-10)
+    st0 = P.generate_state(p, seed=a.seed)
+    dev = ttgpu.Device(p)
+    dev.load_state(st0)
+    dev.bounds()
+    dev.pressg()
+    dt = dev.schritt()
+    print(f"# workload {nx}x{ny}x{nz} dt={dt:.4g}s", file=sys.stderr, flush=True)
+
+    plain = run_mode(dev, st0, dt, a.steps, a.warmup, 0, 0)
+    print(f"# fp64: {plain}", file=sys.stderr, flush=True)
+    mixed = None
+    if not a.no_mixed:
+        mixed = run_mode(dev, st0, dt, a.steps, a.warmup, 1, 1)
+        print(f"# mixed+warm: {mixed}", file=sys.stderr, flush=True)
+    dev.close()
+
+    ncell = nx * ny * nz
+    launch_ms = plain["sweep64_ms"] / max(plain["sweep64_launches"], 1)
+    alg_bytes = 44.0 * ncell  # 88 B/cell per red+black iteration, half per launch
+    achieved = alg_bytes / (launch_ms * 1e-3) / 1e9
+    traffic = read_traffic()
+    out = {
+        "metric": METRIC,
+        "value": round(plain["ms_per_step"], 4),
+        "unit": "ms/step",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(plain["ms_per_step"], 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded hydrostatic atmosphere, 1% c_s velocity noise)",
+        "config": {"workload": f"sph{nx}x{nz}x{ny}: n_r={nx} n_theta={nz} n_phi={ny}, FP64 SOR "
+                               f"omega=1.7 to L1 ratio 1e-10, fixed CFL=0.4 dt",
+                   "nr": nx, "ntheta": nz, "nphi": ny, "dt_s": dt, "epsmat": 1e-10},
+        "sor_sweeps_per_step": plain["sor_sweeps_per_step"],
+        "newton_per_step": plain["newton_per_step"],
+        "cell_updates_per_s": ncell / (plain["ms_per_step"] * 1e-3),
+        "roofline": {"bound": "hbm", "kernel": "k_sweep<double> (red or black half sweep)",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic, "avg_launch_us": round(launch_ms * 1e3, 3),
+                     "algorithmic_bytes_per_launch": alg_bytes},
+    }
+    if mixed:
+        out["mixed"] = {"method": "FP32 SOR in FP64 defect correction, warm start",
+                        "ms_per_step": round(mixed["ms_per_step"], 4),
+                        "fp32_sweeps_per_step": mixed["fp32_sweeps_per_step"],
+                        "speedup_vs_fp64": round(plain["ms_per_step"] / mixed["ms_per_step"], 3)}
+        if mixed["sweep32_launches"]:
+            l32 = mixed["sweep32_ms"] / mixed["sweep32_launches"]
+            out["mixed"]["fp32_sweep_GBs"] = round(22.0 * ncell / (l32 * 1e-3) / 1e9, 1)
+    if not a.no_cpu:
+        ms, s = cpu_baseline(p, st0, dt, a.cpu_threads)
+        out["cpu_baseline"] = {"value": round(ms, 1), "unit": "ms/step", "cores": a.cpu_threads,
+                               "kind": "port",
+                               "sample": f"one full step of the C FP64 oracle (OpenMP) at {nx}x{ny}x{nz}, "
+                                         f"same state and dt; {s.energy.energy_iter} Newton, "
+                                         f"{s.energy.sor_iter_total} SOR iterations"}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,201 @@
+"""Parity of the HIP path (through the C ABI of libttgpu.so) with the
+reference Fortran's outputs stored in the golden fixtures (same inputs), and
+with the CPU oracle at larger sizes.
+
+Bars: advection/force/boundaries/energy fields within
+1e-12 relative of the reference (transcendental functions -- pow, exp, cos --
+come from the GPU math library, so results are not bit-identical); SOR
+iteration counts identical; delta_ER within the solver tolerance.
+"""
+import numpy as np
+import pytest
+
+from tests.golden import compare as C
+from ttcode_amd import params as P
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-12
+
+
+@pytest.fixture(scope="module")
+def ttgpu():
+    from ttcode_amd import ttgpu as T
+    T.load_library()
+    return T
+
+
+def _dev(ttgpu, g):
+    p = C.params_of(g)
+    d = ttgpu.Device(p)
+    d.load_state(C.inputs_of(g))
+    return p, d
+
+
+def _cmp(g, p, d, names=None, rel=REL, skip=()):
+    worst = {}
+    for k, ref in C.outputs_of(g).items():
+        if (names and k not in names) or k in skip:
+            continue
+        reg = C.compare_region(k, p.nx, p.ny, p.nz)
+        mine = d.download(k)[reg]
+        want = ref[reg]
+        if mine.dtype.kind == "i":
+            assert np.array_equal(mine, want), k
+            continue
+        fin = np.isfinite(want)
+        assert np.array_equal(np.isfinite(mine), fin), f"{k}: non-finite pattern differs"
+        e = C.rel_err(mine[fin], want[fin])
+        worst[k] = e
+        assert e <= rel, f"{k}: max rel err {e:.3e} > {rel:.1e}"
+    return worst
+
+
+def test_metrics(ttgpu):
+    g = C.load("bounds_12x16x12")
+    p, d = _dev(ttgpu, g)
+    ids = {"xxa": 0, "xxb": 1, "dxa": 2, "dxb": 3, "xyb": 5, "dyb": 7, "xzb": 9, "dzb": 11,
+           "surxa": 22, "volxb": 25, "surzb": 35, "volzb": 37, "geozg": 44}
+    for n, m in ids.items():
+        np.testing.assert_array_equal(d.metric(m), g["out_m_" + n], err_msg=n)
+    d.close()
+
+
+def test_bounds(ttgpu):
+    g = C.load("bounds_12x16x12")
+    p, d = _dev(ttgpu, g)
+    d.bounds()
+    _cmp(g, p, d, rel=0.0)
+    d.close()
+
+
+def test_advect3d(ttgpu):
+    g = C.load("advect_12x16x12")
+    p, d = _dev(ttgpu, g)
+    d.advect3d(float(g["delt"]))
+    _cmp(g, p, d)
+    d.close()
+
+
+def test_forgas3d(ttgpu):
+    g = C.load("forgas_12x16x12")
+    p, d = _dev(ttgpu, g)
+    d.bounds()
+    d.qdens()
+    d.forgas3d(float(g["delt"]))
+    _cmp(g, p, d)
+    d.close()
+
+
+def test_solve_with_sor(ttgpu):
+    g = C.load("sor_12x16x12")
+    p, d = _dev(ttgpu, g)
+    for k in ("EVAL1", "EVAL2", "EVAL3", "EVAL4", "EVAL5", "EVAL6", "EVAL7", "EVAL8",
+              "EVAL_1_6", "RHS_ER", "upperbnd"):
+        d.upload(k, g["in_" + k])
+    st, _ = d.solve_sor(True)
+    assert st.sor_iter == int(g["scal_SOR_ITER"])
+    assert abs(st.rhsnorm / float(g["scal_global_RHSNORM"]) - 1) < 1e-13
+    assert abs(st.rnorm / float(g["scal_global_RNORM"]) - 1) < 1e-10
+    # red-black sweeps are order independent: same operations as the reference
+    _cmp(g, p, d, rel=1e-13)
+    d.close()
+
+
+@pytest.mark.parametrize("name", ["energy_const_12x16x12", "energy_pow8_12x16x12"])
+def test_coupled_energy(ttgpu, name):
+    g = C.load(name)
+    p, d = _dev(ttgpu, g)
+    st, _ = d.coupled_energy(float(g["delt"]), int(g["iii"]))
+    assert st.energy_iter == int(g["scal_energy_iter"])
+    assert st.sor_iter_last == int(g["scal_SOR_ITER"])
+    # delta_ER is the whole step's change; compare it where it is not tiny
+    _cmp(g, p, d, rel=1e-9, skip=("delta_ER", "flux_y", "flux_z", "FR", "FQ", "RHS_ER"))
+    d.close()
+
+
+def test_two_steps(ttgpu):
+    g = C.load("step2_12x16x12")
+    p, d = _dev(ttgpu, g)
+    for s in range(int(g["nsteps"])):
+        st, _ = d.step(float(g["delt"]), int(g["iii"]) + s)
+    assert st.energy.energy_iter == int(g["scal_energy_iter"])
+    assert st.energy.sor_iter_last == int(g["scal_SOR_ITER"])
+    _cmp(g, p, d, rel=1e-9, skip=("delta_ER",))
+    d.close()
+
+
+def test_diffusion(ttgpu):
+    g = C.load("diffusion_12x16x12")
+    p, d = _dev(ttgpu, g)
+    d.bounds()
+    it, _ = d.diffusion(float(g["delt"]), float(g["xnue"]), float(g["pr_num"]))
+    assert it == int(g["scal_SOR_ITER"])
+    _cmp(g, p, d, rel=1e-13)
+    d.close()
+
+
+def test_schritt_matches_oracle(ttgpu):
+    from oracle.ttoracle import Oracle
+    p = P.make_params(32, 64, 32)
+    st = P.generate_state(p, seed=3)
+    o = Oracle(p)
+    o.load_state(st)
+    o.bounds()
+    o.pressg()
+    d = ttgpu.Device(p)
+    d.load_state(st)
+    d.bounds()
+    d.pressg()
+    assert d.schritt() == o.schritt()
+    d.close()
+
+
+def test_step_32x64x32_vs_oracle(ttgpu):
+    """One full step of the CPU-runnable workload (nr=32, ntheta=32, nphi=64)."""
+    from oracle.ttoracle import Oracle
+    p = P.make_params(32, 64, 32, epsmat=1e-8)
+    st = P.generate_state(p, seed=5)
+    o = Oracle(p, threads=1)
+    o.load_state(st)
+    o.bounds()
+    o.pressg()
+    dt = o.schritt()
+    o.load_state(st)
+    so, _ = o.step(dt, 1)
+    d = ttgpu.Device(p)
+    d.load_state(st)
+    sg, _ = d.step(dt, 1)
+    assert sg.energy.energy_iter == so.energy.energy_iter
+    assert abs(sg.energy.sor_iter_total - so.energy.sor_iter_total) <= so.energy.energy_iter
+    for k in ("RH", "V", "G", "H"):
+        e = C.rel_err(d.download(k), o.download(k))
+        assert e <= 1e-10, (k, e)
+    for k in ("T", "ER"):
+        reg = C.compare_region(k, p.nx, p.ny, p.nz)
+        e = C.rel_err(d.download(k)[reg], o.download(k)[reg])
+        assert e <= 1e-7, (k, e)
+    d.close()
+
+
+def test_mixed_precision_solve(ttgpu):
+    """FP32 sweeps inside FP64 defect correction reach the FP64 residual target
+    and agree with the FP64 solve within it."""
+    g = C.load("sor_12x16x12")
+    p = C.params_of(g)
+    p.epsmat = 1e-10
+    d = ttgpu.Device(p)
+    d.load_state(C.inputs_of(g))
+    for k in ("EVAL1", "EVAL2", "EVAL3", "EVAL4", "EVAL5", "EVAL6", "EVAL7", "EVAL8",
+              "EVAL_1_6", "RHS_ER", "upperbnd"):
+        d.upload(k, g["in_" + k])
+    st64, _ = d.solve_sor(True)
+    x64 = d.download("delta_ER")
+    d.set_solver(precision=1, inner_tol=1e-4)
+    st32, _ = d.solve_sor(True)
+    x32 = d.download("delta_ER")
+    assert st32.rnorm / st32.rhsnorm <= 1e-10
+    assert st32.outer_iter >= 1 and st32.fp32_sweeps > 0
+    scale = np.max(np.abs(x64))
+    assert np.max(np.abs(x32 - x64)) <= 1e-6 * scale
+    d.close()

@@ -1,0 +1,161 @@
+// tt_common.h -- shared definitions of the gfx950 implementation of the ttcode
+// hot path (libttgpu.so).
+//
+// Device layout.  Every 3-D field lives in HBM over the full ghosted box
+// (-1:nx+2, -1:ny+2, -1:nz+2) in the reference's own column-major order (r
+// fastest), so the wire format of tt_upload/tt_download is a sub-box copy.
+// Fields whose declared bounds are smaller (ppmod.f) use a part of the box.
+// The SOR working set has a separate colour-split layout (tt_sor.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/ttgpu.h"
+#include "../../include/ttgpu_fields.h"
+
+namespace tt {
+
+// ppmod.f:1071-1093: the reference stores its constants from REAL literals.
+constexpr double FD = (double)2.99792458e10f;
+constexpr double ARAD = (double)7.56596e-15f;
+constexpr double RGAS = (double)8.315e7f;
+constexpr double SBCONST = (double)5.6705e-5f;
+constexpr double PI = 3.141592653589793;
+
+// Box geometry passed by value to kernels.
+struct Geo {
+    int nx, ny, nz;
+    int gx, gy, gz;
+    long long n3;
+    __host__ __device__ inline long long idx(int i, int j, int k) const {
+        return (long long)(i + 1) + (long long)gx * ((long long)(j + 1) + (long long)gy * (long long)(k + 1));
+    }
+    __host__ __device__ inline int idx2(int j, int k) const { return (j + 1) + gy * (k + 1); }
+};
+
+// 1-D metric arrays: one buffer, metric m at base[m*len + i + 2].
+struct Met {
+    const double* base;
+    int len;
+    __device__ inline double operator()(int m, int i) const { return base[(long long)m * len + i + 2]; }
+};
+
+// Device-side control block: reductions, convergence state, error flags.
+struct Ctrl {
+    // SOR
+    int done;          // 0 running, 1 converged, 2 itmax reached
+    int it;            // completed iterations
+    int itmax, itmin;
+    unsigned int counter;   // last-block-done counter
+    unsigned int counter2;
+    double rhsnorm, eps, rnorm;
+    // Newton
+    double maxdq, maxde;
+    int bad_q;         // delta_Q positivity failure
+    int bad_bound;     // BOUNDXMAX(2) upperbnd above the domain
+    int bad_state;     // SCHRITT positivity failure
+    int pad;
+    double dtmin;
+    double scratch[8];
+};
+
+// Library state (one device, one stream).
+struct Lib {
+    bool ready = false;
+    tt_params p{};
+    Geo g{};
+    hipStream_t s = nullptr;
+    double* f[TT_FIELD_MAX] = {};   // ghosted device arrays, by field id
+    int32_t* ub = nullptr;          // upperbnd (-1:ny+2,-1:nz+2)
+    double* mustar = nullptr;
+    double* met = nullptr;          // device metric buffer
+    double* hmet = nullptr;         // host copy
+    int metlen = 0;
+    // temporaries
+    double *ED = nullptr, *VD = nullptr, *GDN = nullptr, *HDN = nullptr;
+    double *FLMX = nullptr, *FLMY = nullptr, *FLMZ = nullptr;
+    double *TOLD = nullptr, *VH = nullptr, *GH = nullptr, *HH = nullptr;
+    double *DFQ = nullptr;
+    double* partial = nullptr;      // block partial sums (reductions)
+    int npartial = 0;
+    Ctrl* ctrl = nullptr;           // device
+    Ctrl* hctrl = nullptr;          // pinned host mirror
+    tt_solver_opts opts{};
+    std::string err;
+    // sweep timing
+    bool timing = false;
+    int64_t n64 = 0, n32 = 0;
+    double ms64 = 0.0, ms32 = 0.0;
+    int energy_iter = 0;
+    int iii = 0;
+    int have_prev_guess = 0;
+};
+
+extern Lib L;
+
+int fail(int code, const std::string& msg);
+
+#define TT_HIP(call)                                                                  \
+    do {                                                                              \
+        hipError_t e_ = (call);                                                       \
+        if (e_ != hipSuccess) return tt::fail(10, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define TT_LAUNCH()                                                                   \
+    do {                                                                              \
+        hipError_t e_ = hipGetLastError();                                            \
+        if (e_ != hipSuccess) return tt::fail(11, std::string("launch: ") + hipGetErrorString(e_)); \
+    } while (0)
+
+inline Met met() { return Met{L.met, L.metlen}; }
+inline unsigned nblk(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// Fortran order helpers for the full box
+__device__ inline void box_ijk(const Geo& g, long long t, int& i, int& j, int& k) {
+    i = (int)(t % g.gx) - 1;
+    long long r = t / g.gx;
+    j = (int)(r % g.gy) - 1;
+    k = (int)(r / g.gy) - 1;
+}
+
+// Sub-box decomposition: t over ni*nj*nk with origin (i0,j0,k0)
+__device__ inline void sub_ijk(long long t, int ni, int nj, int i0, int j0, int k0, int& i, int& j, int& k) {
+    i = (int)(t % ni) + i0;
+    long long r = t / ni;
+    j = (int)(r % nj) + j0;
+    k = (int)(r / nj) + k0;
+}
+
+// Wavefront + block reductions (64-wide wavefronts)
+__device__ inline double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ inline double wave_max(double v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ inline double wave_min(double v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Internal entry points shared between translation units
+int bounds_impl();
+int pressg_impl();
+int qdens_impl();
+int sound_impl();
+int planetshift_impl(double* a, int f);
+int sor_solve_impl(int fill_from_evals, tt_sor_stats* st, int fieldmode);
+int sor_pack_impl();
+int diffusion_impl(double delt, double xnue, double pr, int* iters);
+int energy_impl(double delt, int iii, tt_energy_stats* st);
+int advect_impl(double delt);
+int forgas_impl(double delt);
+int sync_check();
+
+}  // namespace tt

@@ -1,0 +1,603 @@
+// tt_energy.hip -- COUPLED_ENERGY_EQS (ppenergy.f:2-791) on gfx950: the
+// backward-Euler gas/radiation energy Newton loop with the flux-limited
+// diffusion build (KAPPA, FLIM, DIFF, CALCULATEFLUX, FLUX_BC,
+// CALCULATEDIVF_WITHF, FILL_EVALUES, FILL_SOR_MATRIX) and one red-black SOR
+// solve per Newton iteration.
+//
+// Per Newton iteration: the phi wrap of ER, one kernel per face direction
+// (limiter, diffusion coefficient, flux and its boundary values), one
+// per-cell kernel (divergence, EVAL1..8, the pointwise Newton terms, the SOR
+// matrix and RHS with their boundary rows), the SOR solve, one per-cell kernel
+// for delta_Q and the damped update with the two max reductions, then the
+// Q/ER boundary blocks.
+#include <cmath>
+
+#include "tt_common.h"
+
+namespace tt {
+
+int sor_scale_guess(double sc);
+int cnn_predict_guess(double delt);
+bool cnn_ready();
+
+#define IX(i, j, k) g.idx(i, j, k)
+
+__global__ void k_q_init(double* Q, const double* __restrict__ RH, const double* __restrict__ CV,
+                         const double* __restrict__ T, long long n) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) Q[t] = RH[t] * CV[t] * T[t];
+}
+
+// KAPPA (pprphy.f:1-360): IKAPTYP=1 constant; IKAPTYP=26 (new) power law
+// evaluated up to upperbnd, cells above take the upperbnd value; SUBSIG.
+__global__ void k_kappa(Geo g, tt_params p, const int32_t* __restrict__ ub, const double* __restrict__ RH,
+                        const double* __restrict__ T, double* XKR, double* XKP, double* XKA, double* DKP,
+                        double* DKA, double* SIG, double sig) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.n3) return;
+    SIG[t] = sig;
+    if (p.ikaptyp == 1) {
+        XKR[t] = p.zkapmult; XKP[t] = p.zkapmult; XKA[t] = p.zkapmult;
+        DKP[t] = 0.0; DKA[t] = 0.0;
+        return;
+    }
+    int i, j, k;
+    box_ijk(g, t, i, j, k);
+    int u = ub[g.idx2(j, k)];
+    int ii = i <= u ? i : u;
+    long long n = g.idx(ii, j, k);
+    double rr = RH[n] / p.kap_rho0, tt = T[n] / p.kap_t0;
+    double kap = p.kap0 * pow(rr, p.kap_a) * pow(tt, p.kap_b);
+    double dk = p.dkap0 * pow(rr, p.kap_a) * pow(tt, p.kap_b - 1.0);
+    XKR[t] = kap; XKP[t] = kap; XKA[t] = kap;
+    DKP[t] = dk; DKA[t] = dk;
+}
+
+// CALCSTELLARTAU + STELLAR_DTAU + STELLARINPUT_HEATING grey branch
+// (pprad.f:191-242, 408-516) for one (j,k) column: reverse scan along r.
+__global__ void k_stellar(Geo g, Met m, tt_params p, const int32_t* __restrict__ ub, const double* __restrict__ XKA,
+                          const double* __restrict__ RH, const double* __restrict__ MU, double* STAU, double* DTAUS,
+                          double* SIN) {
+    int nj = g.ny + 4, nk = g.nz + 4;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nj * nk) return;
+    int j = t % nj - 1, k = t / nj - 1;
+    int u = ub[g.idx2(j, k)];
+    const double dr = m(TT_M_XXB, 2) - m(TT_M_XXB, 1);
+    if (u < g.nx)
+        for (int i = u + 1; i <= g.nx + 1; ++i) STAU[IX(i, j, k)] = 0.0;
+    double tot = 0.0;
+    for (int i = u; i >= 1; --i) {
+        tot = tot + (XKA[IX(i, j, k)] * RH[IX(i, j, k)] * dr);
+        STAU[IX(i, j, k)] = tot;
+    }
+    for (int i = 2; i <= g.nx; ++i) DTAUS[IX(i, j, k)] = (STAU[IX(i - 1, j, k)] - STAU[IX(i + 1, j, k)]) / 2.0;
+    DTAUS[IX(1, j, k)] = DTAUS[IX(2, j, k)];
+    DTAUS[IX(g.nx + 1, j, k)] = DTAUS[IX(g.nx, j, k)];
+    double angle = cos(m(TT_M_XYB, j)) * cos(m(TT_M_XZB, k));
+    double tinc = 0.0;
+    if (angle >= 0.0) {
+        tinc = p.tirr * pow(angle, 0.25);
+        tinc = tinc > 0.0 ? tinc : 0.0;
+    }
+    double mu = MU[g.idx2(j, k)];
+    double t4 = pow(tinc, 4.0);
+    for (int i = 1; i <= g.nx; ++i)
+        SIN[IX(i, j, k)] = SBCONST * (t4 / mu) * (exp(-STAU[IX(i, j, k)] / mu)) * DTAUS[IX(i, j, k)] / m(TT_M_DXB, i);
+}
+
+__global__ void k_fill_box(double* a, double v, long long n) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) a[t] = v;
+}
+
+// Levermore-Pomraning limiter on the three face families (pprphy.f:410-522)
+__device__ inline double lp(double rr) { return (2.0 + rr) / (6.0 + 3.0 * rr + rr * rr); }
+
+__device__ double flimx_at(const Geo& g, const Met& m, const double* ER, const double* RHQ, const double* XKR,
+                           const double* SIG, int i, int j, int k) {
+    double gx = (ER[IX(i, j, k)] - ER[IX(i - 1, j, k)]) / m(TT_M_DXB, i);
+    double gy = (m(TT_M_DDX1, i) * (ER[IX(i, j + 1, k)] - ER[IX(i, j - 1, k)]) +
+                 m(TT_M_DDX0, i) * (ER[IX(i - 1, j + 1, k)] - ER[IX(i - 1, j - 1, k)])) /
+                (m(TT_M_GEOXGA, i) * m(TT_M_GEOZG, k) * (m(TT_M_XYB, j + 1) - m(TT_M_XYB, j - 1)));
+    double gz = (m(TT_M_DDX1, i) * (ER[IX(i, j, k + 1)] - ER[IX(i, j, k - 1)]) +
+                 m(TT_M_DDX0, i) * (ER[IX(i - 1, j, k + 1)] - ER[IX(i - 1, j, k - 1)])) /
+                (m(TT_M_GEOXHA, i) * (m(TT_M_XZB, k + 1) - m(TT_M_XZB, k - 1)));
+    double grad = sqrt(gx * gx + gy * gy + gz * gz) + 1.0e-50;
+    double s = SIG[IX(i, j, k)];
+    double rr = 2.0 * grad /
+                (RHQ[IX(i, j, k)] * (m(TT_M_DDX1, i) * (XKR[IX(i, j, k)] + s) + m(TT_M_DDX0, i) * (XKR[IX(i - 1, j, k)] + s)) *
+                 (ER[IX(i, j, k)] + ER[IX(i - 1, j, k)]));
+    return lp(rr);
+}
+
+__device__ double flimy_at(const Geo& g, const Met& m, const double* ER, const double* RHQ, const double* XKR,
+                           const double* SIG, int i, int j, int k) {
+    double gx = (m(TT_M_DDY1, j) * (ER[IX(i + 1, j, k)] - ER[IX(i - 1, j, k)]) +
+                 m(TT_M_DDY0, j) * (ER[IX(i + 1, j - 1, k)] - ER[IX(i - 1, j - 1, k)])) /
+                (m(TT_M_XXB, i + 1) - m(TT_M_XXB, i - 1));
+    double gy = (ER[IX(i, j, k)] - ER[IX(i, j - 1, k)]) / (m(TT_M_GEOXG, i) * m(TT_M_GEOZG, k) * m(TT_M_DYB, j));
+    double gz = (m(TT_M_DDY1, j) * (ER[IX(i, j, k + 1)] - ER[IX(i, j, k - 1)]) +
+                 m(TT_M_DDY0, j) * (ER[IX(i, j - 1, k + 1)] - ER[IX(i, j - 1, k - 1)])) /
+                ((m(TT_M_XZB, k + 1) - m(TT_M_XZB, k - 1)) * m(TT_M_GEOXH, i));
+    double grad = sqrt(gx * gx + gy * gy + gz * gz) + 1.0e-50;
+    double s = SIG[IX(i, j, k)];
+    double rr = 2.0 * grad /
+                (RHQ[IX(i, j, k)] * (m(TT_M_DDY1, j) * (XKR[IX(i, j, k)] + s) + m(TT_M_DDY0, j) * (XKR[IX(i, j - 1, k)] + s)) *
+                 (ER[IX(i, j, k)] + ER[IX(i, j - 1, k)]));
+    return lp(rr);
+}
+
+__device__ double flimz_at(const Geo& g, const Met& m, const double* ER, const double* RHQ, const double* XKR,
+                           const double* SIG, int i, int j, int k) {
+    double gx = (m(TT_M_DDZ1, k) * (ER[IX(i + 1, j, k)] - ER[IX(i - 1, j, k)]) +
+                 m(TT_M_DDZ0, k) * (ER[IX(i + 1, j, k - 1)] - ER[IX(i - 1, j, k - 1)])) /
+                (m(TT_M_XXB, i + 1) - m(TT_M_XXB, i - 1));
+    double gy = (m(TT_M_DDZ1, k) * (ER[IX(i, j + 1, k)] - ER[IX(i, j - 1, k)]) +
+                 m(TT_M_DDZ0, k) * (ER[IX(i, j + 1, k - 1)] - ER[IX(i, j - 1, k - 1)])) /
+                ((m(TT_M_XYB, j + 1) - m(TT_M_XYB, j - 1)) * m(TT_M_GEOXG, i) * m(TT_M_GEOZGA, k));
+    double gz = (ER[IX(i, j, k)] - ER[IX(i, j, k - 1)]) / (m(TT_M_GEOXH, i) * m(TT_M_DZB, k));
+    double grad = sqrt(gx * gx + gy * gy + gz * gz) + 1.0e-50;
+    double s = SIG[IX(i, j, k)];
+    double rr = 2.0 * grad /
+                (RHQ[IX(i, j, k)] * (m(TT_M_DDZ1, k) * (XKR[IX(i, j, k)] + s) + m(TT_M_DDZ0, k) * (XKR[IX(i, j, k - 1)] + s)) *
+                 (ER[IX(i, j, k)] + ER[IX(i, j, k - 1)]));
+    return lp(rr);
+}
+
+struct FaceArgs {
+    const double *ER, *RHQ, *XKR, *SIG;
+    double *FLIM, *DIFR, *FLUX;
+    const int32_t* ub;
+    int iflimtyp;
+    double fbottom;
+};
+
+// x faces i = 1..nx+1: FLIM x loop, DIFF x, CALCULATEFLUX x, FLUX_BC x
+__global__ void k_face_x(Geo g, Met m, FaceArgs a) {
+    long long n = (long long)(g.nx + 1) * g.ny * g.nz;
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    int i, j, k;
+    sub_ijk(t, g.nx + 1, g.ny, 1, 1, 1, i, j, k);
+    int u = a.ub[g.idx2(j, k)];
+    double fl;
+    if (a.iflimtyp == 1) fl = 1.0 / 3.0;
+    else fl = flimx_at(g, m, a.ER, a.RHQ, a.XKR, a.SIG, i <= u - 1 ? i : u - 1, j, k);
+    long long c = IX(i, j, k);
+    a.FLIM[c] = fl;
+    double d = FD * fl /
+               (m(TT_M_DXB, i) * a.RHQ[c] *
+                (m(TT_M_DDX1, i) * (a.XKR[c] + a.SIG[c]) + m(TT_M_DDX0, i) * (a.XKR[IX(i - 1, j, k)] + a.SIG[IX(i - 1, j, k)])));
+    a.DIFR[c] = d;
+    double fx = -d * (a.ER[c] - a.ER[IX(i - 1, j, k)]);
+    if (i >= u) fx = FD * a.ER[IX(u - 1, j, k)] / 2.0;
+    if (i == 1) fx = a.fbottom;
+    a.FLUX[c] = fx;
+}
+
+// y faces j = 1..ny+1 (FLIMY uses the x loop's last xmax_index, pprphy.f:464)
+__global__ void k_face_y(Geo g, Met m, FaceArgs a) {
+    long long n = (long long)g.nx * (g.ny + 1) * g.nz;
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    int i, j, k;
+    sub_ijk(t, g.nx, g.ny + 1, 1, 1, 1, i, j, k);
+    long long c = IX(i, j, k);
+    int u = a.ub[g.idx2(j, k)];
+    double fl;
+    if (a.iflimtyp == 1) {
+        fl = 1.0 / 3.0;
+    } else {
+        int xmaxs = a.ub[g.idx2(g.ny, g.nz)] - 1;
+        if (i >= u) {
+            int src = u - 1;
+            fl = src <= xmaxs ? flimy_at(g, m, a.ER, a.RHQ, a.XKR, a.SIG, src, j, k) : a.FLIM[IX(src, j, k)];
+        } else if (i <= xmaxs) {
+            fl = flimy_at(g, m, a.ER, a.RHQ, a.XKR, a.SIG, i, j, k);
+        } else {
+            fl = a.FLIM[c];
+        }
+    }
+    a.FLIM[c] = fl;
+    double d = FD * fl /
+               (m(TT_M_DYB, j) * m(TT_M_GEOXG, i) * m(TT_M_GEOZG, k) * a.RHQ[c] *
+                (m(TT_M_DDY1, j) * (a.XKR[c] + a.SIG[c]) + m(TT_M_DDY0, j) * (a.XKR[IX(i, j - 1, k)] + a.SIG[IX(i, j - 1, k)])));
+    a.DIFR[c] = d;
+    double fy = -d * (a.ER[c] - a.ER[IX(i, j - 1, k)]);
+    if (j <= g.ny && i >= u) fy = 0.0;
+    a.FLUX[c] = fy;
+}
+
+// z faces k = 1..nz+1
+__global__ void k_face_z(Geo g, Met m, FaceArgs a) {
+    long long n = (long long)g.nx * g.ny * (g.nz + 1);
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    int i, j, k;
+    sub_ijk(t, g.nx, g.ny, 1, 1, 1, i, j, k);
+    long long c = IX(i, j, k);
+    int u = a.ub[g.idx2(j, k)];
+    double fl;
+    if (a.iflimtyp == 1) {
+        fl = 1.0 / 3.0;
+    } else {
+        int xmaxs = a.ub[g.idx2(g.ny, g.nz)] - 1;
+        if (i >= u) {
+            int src = u - 1;
+            fl = src <= xmaxs ? flimz_at(g, m, a.ER, a.RHQ, a.XKR, a.SIG, src, j, k) : a.FLIM[IX(src, j, k)];
+        } else if (i <= xmaxs) {
+            fl = flimz_at(g, m, a.ER, a.RHQ, a.XKR, a.SIG, i, j, k);
+        } else {
+            fl = a.FLIM[c];
+        }
+    }
+    a.FLIM[c] = fl;
+    double d = FD * fl /
+               (m(TT_M_DZB, k) * m(TT_M_GEOXH, i) * a.RHQ[c] *
+                (m(TT_M_DDZ1, k) * (a.XKR[c] + a.SIG[c]) + m(TT_M_DDZ0, k) * (a.XKR[IX(i, j, k - 1)] + a.SIG[IX(i, j, k - 1)])));
+    a.DIFR[c] = d;
+    double fz = -d * (a.ER[c] - a.ER[IX(i, j, k - 1)]);
+    if (k <= g.nz && i >= u) fz = 0.0;
+    a.FLUX[c] = fz;
+}
+
+struct CellArgs {
+    const double *FX, *FY, *FZ, *DX, *DY, *DZ, *Q, *QOLD, *ER, *EROLD, *RH, *CV, *XKP, *XKA, *DKP, *DKA, *SIN;
+    double *E1, *E2, *E3, *E4, *E5, *E6, *E7, *E8, *E16, *ETA, *FQ, *FR, *RHS, *DFQ;
+    double *SA, *SB, *SC, *SD, *SE, *SF, *SG;
+    const int32_t* ub;
+    int f_incident;
+    double delt;
+};
+
+// CALCULATEDIVF_WITHF (:825-878) + FILL_EVALUES (:984-1030) + the Newton
+// pointwise loop (:286-345) + FILL_EVALUES + FILL_SOR_MATRIX and
+// B_VECTOR_BOUNDARIES (ppsor.f:189-329), per interior cell.
+__global__ void k_newton_cell(Geo g, Met m, CellArgs a) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)g.nx * g.ny * g.nz) return;
+    int i, j, k;
+    sub_ijk(t, g.nx, g.ny, 1, 1, 1, i, j, k);
+    long long c = IX(i, j, k);
+    const double theta = 1.0, delt = a.delt;
+    double sxb = m(TT_M_SURXB, i), dxa = m(TT_M_DXA, i), xxb = m(TT_M_XXB, i), szb = m(TT_M_SURZB, k);
+    double dya = m(TT_M_DYA, j), dza = m(TT_M_DZA, k), sxa0 = m(TT_M_SURXA, i), sxa1 = m(TT_M_SURXA, i + 1);
+    double sza0 = m(TT_M_SURZA, k), sza1 = m(TT_M_SURZA, k + 1);
+    double divf = ((sxa1 / (sxb * dxa)) * a.FX[IX(i + 1, j, k)] - (sxa0 / (sxb * dxa)) * a.FX[c]) +
+                  ((1.0 / (xxb * szb * dya)) * a.FY[IX(i, j + 1, k)] - (1.0 / (xxb * szb * dya)) * a.FY[c]) +
+                  ((sza1 / (xxb * szb * dza)) * a.FZ[IX(i, j, k + 1)] - (sza0 / (xxb * szb * dza)) * a.FZ[c]);
+    double e1 = -sxa1 * a.DX[IX(i + 1, j, k)] / (sxb * dxa);
+    double e2 = -sxa0 * a.DX[c] / (sxb * dxa);
+    double e3 = -a.DY[IX(i, j + 1, k)] / (xxb * szb * dya);
+    double e4 = -a.DY[c] / (xxb * szb * dya);
+    double e5 = -a.DZ[IX(i, j, k + 1)] * sza1 / (xxb * szb * dza);
+    double e6 = -a.DZ[c] * sza0 / (xxb * szb * dza);
+    double e16 = e1 + e2 + e3 + e4 + e5 + e6;
+    double rh = a.RH[c], cv = a.CV[c], q = a.Q[c], er = a.ER[c], kp = a.XKP[c];
+    double B = ARAD * pow(q / (rh * cv), 4.0);
+    double fq, heat = 0.0;
+    // (1-theta)*(divF_old - divF) vanishes for theta = 1
+    if (a.f_incident > 1) {
+        heat = a.SIN[c] / (rh * a.XKA[c]);
+        fq = q - a.QOLD[c] + delt * ((1.0 - theta) * 0.0 + (rh * kp * FD * (B - er)) - a.SIN[c]);
+    } else {
+        fq = q - a.QOLD[c] + delt * ((1.0 - theta) * 0.0 + rh * kp * FD * (B - er));
+    }
+    double fr = er - a.EROLD[c] + delt * (divf - rh * kp * FD * (B - er));
+    double t3 = pow(q / (rh * cv), 3.0);
+    double dfq;
+    if (a.f_incident > 1)
+        dfq = 1.0 + (delt / cv) * (kp * 4.0 * ARAD * FD * t3 + (FD * (B - er) * a.DKP[c]) - heat * a.DKA[c]);
+    else
+        dfq = 1.0 + (delt / cv) * (kp * 4.0 * ARAD * FD * t3 + FD * (B - er) * a.DKP[c]);
+    double dfr = -(delt / cv) * (kp * 4.0 * ARAD * FD * t3 + FD * (B - er) * a.DKP[c]);
+    double eta = dfq != 0.0 ? -dfr / dfq : 0.0;
+    double rhs = -fr - eta * fq;
+    double e7 = 1.0 + (1.0 - eta) * delt * rh * kp * FD;
+    double e8 = (eta * (1.0 - theta) - 1.0) * delt;
+    a.E1[c] = e1; a.E2[c] = e2; a.E3[c] = e3; a.E4[c] = e4; a.E5[c] = e5; a.E6[c] = e6;
+    a.E7[c] = e7; a.E8[c] = e8; a.E16[c] = e16; a.ETA[c] = eta; a.FQ[c] = fq; a.FR[c] = fr; a.DFQ[c] = dfq;
+    // FILL_SOR_MATRIX + B_VECTOR_BOUNDARIES
+    double sa = e7 + e8 * e16, sb = -e8 * e2, sc = -e8 * e1, sd = -e8 * e4, se = -e8 * e3, sf = -e8 * e6,
+           sg = -e8 * e5;
+    int u = a.ub[g.idx2(j, k)];
+    bool ident = (i >= u) || (i == g.nx) || (k == 1) || (k == g.nz);
+    if (ident) { sa = 1.0; sb = sc = sd = se = sf = sg = 0.0; rhs = 0.0; }
+    if (i == u - 1 || i == g.nx - 1) sc = 0.0;
+    if (k == 2) sf = 0.0;
+    if (k == g.nz - 1) sg = 0.0;
+    a.SA[c] = sa; a.SB[c] = sb; a.SC[c] = sc; a.SD[c] = sd; a.SE[c] = se; a.SF[c] = sf; a.SG[c] = sg;
+    a.RHS[c] = rhs;
+}
+
+struct UpdArgs {
+    const double *DER, *E1, *E2, *E3, *E4, *E5, *E6, *E16, *DFQ, *FQ, *RH, *XKP;
+    double *Q, *ER;
+    const int32_t* ub;
+    double delt, conv;
+};
+
+// delta_Q from the delta_ER stencil (ppenergy.f:379-408), the positivity
+// test, the two max reductions and the damped update (ppenergy.f:551-573).
+__global__ __launch_bounds__(256) void k_dq_update(Geo g, UpdArgs a, double* part, int nb, Ctrl* ctrl) {
+    __shared__ double sh[2][4];
+    __shared__ int last;
+    double mq = 0.0, me = 0.0;
+    int bad = 0;
+    long long n = (long long)g.nx * g.ny * g.nz;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (long long)gridDim.x * blockDim.x) {
+        int i, j, k;
+        sub_ijk(t, g.nx, g.ny, 1, 1, 1, i, j, k);
+        int u = a.ub[g.idx2(j, k)];
+        if (i > u - 1) continue;
+        long long c = IX(i, j, k);
+        const double theta = 1.0;
+        double Q1 = -a.delt * (1.0 - theta);
+        double Q2 = a.delt * a.RH[c] * a.XKP[c] * FD;
+        double d = a.DER[c];
+        double dq = (1.0 / a.DFQ[c]) *
+                    (Q1 * a.E16[c] * d - Q1 * a.E1[c] * a.DER[IX(i + 1, j, k)] - Q1 * a.E2[c] * a.DER[IX(i - 1, j, k)] -
+                     Q1 * a.E3[c] * a.DER[IX(i, j + 1, k)] - Q1 * a.E4[c] * a.DER[IX(i, j - 1, k)] -
+                     Q1 * a.E5[c] * a.DER[IX(i, j, k + 1)] - Q1 * a.E6[c] * a.DER[IX(i, j, k - 1)] + Q2 * d - a.FQ[c]);
+        double q = a.Q[c], er = a.ER[c];
+        if ((q + a.conv * dq) <= 0.0) bad = 1;
+        mq = fmax(mq, fabs(dq / q));
+        me = fmax(me, fabs(d / er));
+        a.ER[c] = er + a.conv * d;
+        a.Q[c] = q + a.conv * dq;
+    }
+    if (bad) ctrl->bad_q = 1;
+    double x = wave_max(mq), y = wave_max(me);
+    int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { sh[0][w] = x; sh[1][w] = y; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = fmax(fmax(sh[0][0], sh[0][1]), fmax(sh[0][2], sh[0][3]));
+        part[nb + blockIdx.x] = fmax(fmax(sh[1][0], sh[1][1]), fmax(sh[1][2], sh[1][3]));
+        __threadfence();
+        unsigned prev = atomicAdd(&ctrl->counter2, 1u);
+        last = prev == (unsigned)(nb - 1);
+    }
+    __syncthreads();
+    if (last && threadIdx.x == 0) {
+        __threadfence();
+        double X = 0.0, Y = 0.0;
+        for (int q = 0; q < nb; ++q) {
+            X = fmax(X, part[q]);
+            Y = fmax(Y, part[nb + q]);
+        }
+        ctrl->maxdq = X;
+        ctrl->maxde = Y;
+        ctrl->counter2 = 0;
+    }
+}
+
+// flux_x as the top-boundary blocks read it: declared (1:nx+1,1:ny,1:nz) but
+// indexed over ghost j,k and i = nx+2; the read lands on the element with the
+// same column-major offset (0 outside the array: those ghost values are
+// overwritten by the theta blocks and the phi wrap).
+__device__ inline double flux_x_alias(const Geo& g, const double* FX, int i, int j, int k) {
+    long long sx = g.nx + 1, sy = g.ny;
+    long long off = (long long)(i - 1) + sx * ((long long)(j - 1) + sy * (long long)(k - 1));
+    if (off < 0 || off >= sx * sy * g.nz) return 0.0;
+    int ii = (int)(off % sx) + 1;
+    long long r = off / sx;
+    int jj = (int)(r % sy) + 1, kk = (int)(r / sy) + 1;
+    return FX[IX(ii, jj, kk)];
+}
+
+// Q/ER r boundaries per (j,k) column: upper (ppenergy.f:80-104, :681-723) and
+// BCXMIN=12 bottom (:184-195, :640-651).  order 0: top first, 1: bottom first.
+__global__ void k_energy_bc_x(Geo g, Met m, const int32_t* __restrict__ ub, double* Q, double* ER,
+                              const double* __restrict__ RH, const double* __restrict__ CV,
+                              const double* __restrict__ T, const double* __restrict__ FX, double d_t, int order) {
+    int nj = g.ny + 4, nk = g.nz + 4;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nj * nk) return;
+    int j = t % nj - 1, k = t / nj - 1;
+    for (int pass = 0; pass < 2; ++pass) {
+        bool top = (pass == 0) == (order == 0);
+        if (top) {
+            int u = ub[g.idx2(j, k)];
+            double qup = RH[IX(u - 1, j, k)] * CV[IX(u - 1, j, k)] * T[IX(u, j, k)];
+            for (int i = u; i <= g.nx + 2; ++i) {
+                Q[IX(i, j, k)] = qup;
+                ER[IX(i, j, k)] = fabs(2.0 * flux_x_alias(g, FX, i, j, k) / FD);
+            }
+        } else {
+            double dq = CV[IX(1, j, k)] * RH[IX(1, j, k)] * d_t;
+            double q1 = Q[IX(1, j, k)];
+            double q0 = q1 + dq * (m(TT_M_XXB, 0) - m(TT_M_XXB, 1));
+            double qm = q1 + dq * (m(TT_M_XXB, -1) - m(TT_M_XXB, 1));
+            Q[IX(0, j, k)] = q0;
+            Q[IX(-1, j, k)] = qm;
+            double den = RH[IX(1, j, k)] * CV[IX(1, j, k)];
+            ER[IX(0, j, k)] = ARAD * pow(q0 / den, 4.0);
+            ER[IX(-1, j, k)] = ARAD * pow(qm / den, 4.0);
+        }
+    }
+}
+
+// theta edges: zero gradient (ppenergy.f:204-227, :727-757) per (i,j)
+__global__ void k_energy_bc_z(Geo g, double* Q, double* ER) {
+    int ni = g.nx + 4, nj = g.ny + 4;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ni * nj) return;
+    int i = t % ni - 1, j = t / ni - 1;
+    const int nz = g.nz;
+    double q = Q[IX(i, j, 2)], e = ER[IX(i, j, 2)];
+    Q[IX(i, j, 1)] = q; Q[IX(i, j, 0)] = q; Q[IX(i, j, -1)] = q;
+    ER[IX(i, j, 1)] = e; ER[IX(i, j, 0)] = e; ER[IX(i, j, -1)] = e;
+    q = Q[IX(i, j, nz - 1)];
+    e = ER[IX(i, j, nz - 1)];
+    Q[IX(i, j, nz)] = q; Q[IX(i, j, nz + 1)] = q; Q[IX(i, j, nz + 2)] = q;
+    ER[IX(i, j, nz)] = e; ER[IX(i, j, nz + 1)] = e; ER[IX(i, j, nz + 2)] = e;
+}
+
+__global__ void k_energy_final(double* DER, double* T, const double* __restrict__ ER, const double* __restrict__ EROLD,
+                               const double* __restrict__ Q, const double* __restrict__ RH,
+                               const double* __restrict__ CV, long long n) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    DER[t] = ER[t] - EROLD[t];
+    T[t] = Q[t] / (RH[t] * CV[t]);
+}
+
+static int kappa_impl() {
+    double sig;
+    if (L.p.isigtyp == 0) sig = 0.0;
+    else if (L.p.isigtyp == 1) sig = 0.4;
+    else return fail(1, "SUBSIG: ISIGTYP must be 0 or 1");
+    if (L.p.ikaptyp != 1 && L.p.ikaptyp != 26) return fail(1, "KAPPA: IKAPTYP must be 1 or 26");
+    double** f = L.f;
+    k_kappa<<<nblk(L.g.n3, 256), 256, 0, L.s>>>(L.g, L.p, L.ub, f[TT_RH], f[TT_T], f[TT_XKAPR], f[TT_XKAPP],
+                                                f[TT_XKAPA], f[TT_DXKAPP_DT], f[TT_DXKAPA_DT], f[TT_SIG], sig);
+    TT_LAUNCH();
+    return 0;
+}
+
+static int stellar_impl() {
+    double** f = L.f;
+    if (L.p.f_incident == 0) {
+        k_fill_box<<<nblk(L.g.n3, 256), 256, 0, L.s>>>(f[TT_STELLARINPUT], 0.0, L.g.n3);
+        TT_LAUNCH();
+        return 0;
+    }
+    if (L.p.f_incident != 2) return fail(1, "STELLARINPUT_HEATING: F_INCIDENT must be 0 or 2");
+    int ncol = (L.g.ny + 4) * (L.g.nz + 4);
+    k_stellar<<<nblk(ncol, 64), 64, 0, L.s>>>(L.g, met(), L.p, L.ub, f[TT_XKAPA], f[TT_RH], L.mustar,
+                                              f[TT_STELLARTAU], f[TT_DTAUS], f[TT_STELLARINPUT]);
+    TT_LAUNCH();
+    return 0;
+}
+
+static int flux_impl() {
+    if (L.p.iflimtyp != 1 && L.p.iflimtyp != 2) return fail(1, "FLIM: IFLIMTYP must be 1 or 2");
+    Geo& g = L.g;
+    double** f = L.f;
+    FaceArgs a{f[TT_ER], f[TT_RHQX], f[TT_XKAPR], f[TT_SIG], f[TT_FLIMX], f[TT_DIFRX], f[TT_FLUX_X], L.ub,
+               L.p.iflimtyp, L.p.fbottom};
+    long long nxf = (long long)(g.nx + 1) * g.ny * g.nz;
+    k_face_x<<<nblk(nxf, 256), 256, 0, L.s>>>(g, met(), a);
+    TT_LAUNCH();
+    a.RHQ = f[TT_RHQY]; a.FLIM = f[TT_FLIMY]; a.DIFR = f[TT_DIFRY]; a.FLUX = f[TT_FLUX_Y];
+    long long nyf = (long long)g.nx * (g.ny + 1) * g.nz;
+    k_face_y<<<nblk(nyf, 256), 256, 0, L.s>>>(g, met(), a);
+    TT_LAUNCH();
+    a.RHQ = f[TT_RHQZ]; a.FLIM = f[TT_FLIMZ]; a.DIFR = f[TT_DIFRZ]; a.FLUX = f[TT_FLUX_Z];
+    long long nzf = (long long)g.nx * g.ny * (g.nz + 1);
+    k_face_z<<<nblk(nzf, 256), 256, 0, L.s>>>(g, met(), a);
+    TT_LAUNCH();
+    return 0;
+}
+
+static int energy_bcs(int order) {
+    Geo& g = L.g;
+    double** f = L.f;
+    int ncol = (g.ny + 4) * (g.nz + 4), nline = (g.nx + 4) * (g.ny + 4);
+    k_energy_bc_x<<<nblk(ncol, 128), 128, 0, L.s>>>(g, met(), L.ub, f[TT_Q], f[TT_ER], f[TT_RH], f[TT_CV], f[TT_T],
+                                                    f[TT_FLUX_X], L.p.d_t, order);
+    TT_LAUNCH();
+    k_energy_bc_z<<<nblk(nline, 256), 256, 0, L.s>>>(g, f[TT_Q], f[TT_ER]);
+    TT_LAUNCH();
+    return 0;
+}
+
+int energy_impl(double delt, int iii, tt_energy_stats* st) {
+    Geo& g = L.g;
+    double** f = L.f;
+    const double delta_max = 0.005, conv_parm = (double)0.1f;  // ppenergy.f:42,49 (0.1 is a REAL literal)
+    tt_energy_stats es{};
+    if (L.p.bcxmin != 12) return fail(1, "COUPLED_ENERGY_EQS: only BCXMIN=12 is supported");
+    L.iii = iii;
+    int rc;
+    L.energy_iter = 0;
+    TT_HIP(hipMemsetAsync(f[TT_DELTA_ER], 0, (size_t)g.n3 * sizeof(double), L.s));
+    if ((rc = bounds_impl())) return rc;
+    k_q_init<<<nblk(g.n3, 256), 256, 0, L.s>>>(f[TT_Q], f[TT_RH], f[TT_CV], f[TT_T], g.n3);
+    TT_LAUNCH();
+    // KAPPA, SUBSIG, QDENS: T and RH stay fixed through the Newton loop, so the
+    // opacities FLIM recomputes while energy_iter <= 1 are these
+    if ((rc = kappa_impl())) return rc;
+    if ((rc = qdens_impl())) return rc;
+    if (L.p.f_incident > 1 && (rc = stellar_impl())) return rc;
+    if ((rc = flux_impl())) return rc;
+    if ((rc = energy_bcs(0))) return rc;
+    if ((rc = planetshift_impl(f[TT_ER], 6))) return rc;
+    TT_HIP(hipMemcpyAsync(f[TT_ER_OLD], f[TT_ER], (size_t)g.n3 * sizeof(double), hipMemcpyDeviceToDevice, L.s));
+    TT_HIP(hipMemcpyAsync(L.VD, f[TT_Q], (size_t)g.n3 * sizeof(double), hipMemcpyDeviceToDevice, L.s));  // Q_old
+    int srcode = 0;
+    for (;;) {
+        L.energy_iter++;
+        if ((rc = planetshift_impl(f[TT_ER], 6))) return rc;
+        if ((rc = flux_impl())) return rc;
+        CellArgs ca{f[TT_FLUX_X], f[TT_FLUX_Y], f[TT_FLUX_Z], f[TT_DIFRX], f[TT_DIFRY], f[TT_DIFRZ], f[TT_Q], L.VD,
+                    f[TT_ER], f[TT_ER_OLD], f[TT_RH], f[TT_CV], f[TT_XKAPP], f[TT_XKAPA], f[TT_DXKAPP_DT],
+                    f[TT_DXKAPA_DT], f[TT_STELLARINPUT], f[TT_EVAL1], f[TT_EVAL2], f[TT_EVAL3], f[TT_EVAL4],
+                    f[TT_EVAL5], f[TT_EVAL6], f[TT_EVAL7], f[TT_EVAL8], f[TT_EVAL_1_6], f[TT_JACOB_ETA], f[TT_FQ],
+                    f[TT_FR], f[TT_RHS_ER], L.DFQ, f[TT_SOR_A], f[TT_SOR_B], f[TT_SOR_C], f[TT_SOR_D],
+                    f[TT_SOR_E], f[TT_SOR_F], f[TT_SOR_G], L.ub, L.p.f_incident, delt};
+        long long nc = (long long)g.nx * g.ny * g.nz;
+        k_newton_cell<<<nblk(nc, 256), 256, 0, L.s>>>(g, met(), ca);
+        TT_LAUNCH();
+        // initial guess (new): warm start from the previous Newton correction,
+        // or the learned prediction on the first Newton iteration
+        L.have_prev_guess = 0;
+        if (L.opts.guess == 1 && L.energy_iter > 1) {
+            if ((rc = sor_scale_guess(L.opts.warm_scale))) return rc;
+            L.have_prev_guess = 1;
+        } else if (L.opts.guess == 2) {
+            if (L.energy_iter > 1) {
+                if ((rc = sor_scale_guess(L.opts.warm_scale))) return rc;
+                L.have_prev_guess = 1;
+            } else if (cnn_ready()) {
+                if ((rc = cnn_predict_guess(delt))) return rc;
+                L.have_prev_guess = 1;
+            }
+        }
+        tt_sor_stats ss{};
+        int src = sor_solve_impl(0, &ss, 0);
+        if (src > 0) return src;
+        if (src < 0) srcode = src;
+        es.sor_iter_total += ss.sor_iter;
+        es.sor_iter_last = ss.sor_iter;
+        es.fp32_sweeps += ss.fp32_sweeps;
+        if (L.energy_iter == 1) es.sor_iter_first = ss.sor_iter;
+        if (L.have_prev_guess && ss.guess_used == 0) es.fallbacks++;
+        if (ss.guess_used == 2) es.learned_used++;
+        int nb = 1024;
+        UpdArgs ua{f[TT_DELTA_ER], f[TT_EVAL1], f[TT_EVAL2], f[TT_EVAL3], f[TT_EVAL4], f[TT_EVAL5], f[TT_EVAL6],
+                   f[TT_EVAL_1_6], L.DFQ, f[TT_FQ], f[TT_RH], f[TT_XKAPP], f[TT_Q], f[TT_ER], L.ub, delt, conv_parm};
+        k_dq_update<<<nb, 256, 0, L.s>>>(g, ua, L.partial, nb, L.ctrl);
+        TT_LAUNCH();
+        if ((rc = energy_bcs(1))) return rc;
+        if ((rc = sync_check())) return rc;
+        es.max_dq = L.hctrl->maxdq;
+        es.max_de = L.hctrl->maxde;
+        es.energy_iter = L.energy_iter;
+        if (L.energy_iter == L.p.itmax) {
+            srcode = fail(-2, "Max iteration reached in coupled_energy_eqs");
+            break;
+        }
+        if (!(es.max_dq > delta_max)) break;
+    }
+    k_energy_final<<<nblk(g.n3, 256), 256, 0, L.s>>>(f[TT_DELTA_ER], f[TT_T], f[TT_ER], f[TT_ER_OLD], f[TT_Q],
+                                                     f[TT_RH], f[TT_CV], g.n3);
+    TT_LAUNCH();
+    if (st) *st = es;
+    return srcode;
+}
+
+}  // namespace tt
+
+using namespace tt;
+
+extern "C" int tt_coupled_energy(double delt, int iii, tt_energy_stats* st) {
+    if (!L.ready) return fail(1, "tt_init has not been called");
+    int rc = energy_impl(delt, iii, st);
+    if (rc > 0) return rc;
+    int rc2 = sync_check();
+    return rc2 ? rc2 : rc;
+}

@@ -1,0 +1,428 @@
+// tt_phys.hip -- BOUNDS (ppbou.f), PRESSG/QDENS/SOUND (ppphy.f), SCHRITT
+// (ppsub.f) and DIFFUSION (ppdif.f) on gfx950.
+//
+// BOUNDS for MODTYP=2 on one rank (ppbou.f:1-92) is: PLANETSHIFT of
+// T,RH,V,G,H; BOUNDPLANET (upperbnd search + PASSUPPERBND, BOUNDXMIN(bcxmin),
+// BOUNDXMAX(2), BOUNDZMIN(0), BOUNDZMAX(0)); PLANETSHIFT again.  The kernels
+// keep that order: the r boundaries of one (j,k) column touch only that
+// column, the theta boundaries of one (i,j) line only that line, so one thread
+// per column / line reproduces the reference's loops exactly.
+#include <cmath>
+
+#include "tt_common.h"
+
+namespace tt {
+
+int planetshift_impl(double* a, int f);
+
+__global__ void k_shift5(Geo g, double* T, double* RH, double* V, double* G, double* H) {
+    int ni = g.nx + 4;
+    long long n = (long long)ni * g.nz;
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    int i = (int)(t % ni) - 1, k = (int)(t / ni) + 1;
+    const int ny = g.ny;
+    double* f3[3] = {T, RH, H};
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        double* a = f3[q];
+        a[g.idx(i, 0, k)] = a[g.idx(i, ny, k)];
+        a[g.idx(i, -1, k)] = a[g.idx(i, ny - 1, k)];
+        a[g.idx(i, ny + 1, k)] = a[g.idx(i, 1, k)];
+        a[g.idx(i, ny + 2, k)] = a[g.idx(i, 2, k)];
+    }
+    if (i >= 0) {  // V(0:nx+2,...)
+        V[g.idx(i, 0, k)] = V[g.idx(i, ny, k)];
+        V[g.idx(i, -1, k)] = V[g.idx(i, ny - 1, k)];
+        V[g.idx(i, ny + 1, k)] = V[g.idx(i, 1, k)];
+        V[g.idx(i, ny + 2, k)] = V[g.idx(i, 2, k)];
+    }
+    // G(:,0:ny+2,:) has no j = -1 plane
+    G[g.idx(i, 0, k)] = G[g.idx(i, ny, k)];
+    G[g.idx(i, ny + 1, k)] = G[g.idx(i, 1, k)];
+    G[g.idx(i, ny + 2, k)] = G[g.idx(i, 2, k)];
+}
+
+// upperbnd (ppbou.f:124-138) for every column incl. ghosts, which PASSUPPERBND
+// (ppbou.f:522-682) fills from the phi-wrapped column and the nearest theta
+// row when the theta/corner partners are MPI_PROC_NULL.
+__global__ void k_upperbnd(Geo g, const double* __restrict__ RH, int32_t* ub) {
+    int nj = g.ny + 4, nk = g.nz + 4;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nj * nk) return;
+    int j = t % nj - 1, k = t / nj - 1;
+    int jw = j <= 0 ? j + g.ny : (j > g.ny ? j - g.ny : j);
+    int kc = k < 1 ? 1 : (k > g.nz ? g.nz : k);
+    const double thr = (double)1.0e-9f;
+    int u = g.nx - 1;
+    for (int i = 1; i <= g.nx; ++i)
+        if (RH[g.idx(i, jw, kc)] < thr) { u = i; break; }
+    ub[g.idx2(j, k)] = u;
+}
+
+// BOUNDXMIN(12) (ppbou.f:1118-1139) then BOUNDXMAX(2) (ppbou.f:1269-1395)
+__global__ void k_xbc(Geo g, Met m, const int32_t* __restrict__ ub, double* T, double* RH, double* V, double* G,
+                      double* H, const double* __restrict__ ER, const double* __restrict__ SIN,
+                      const double* __restrict__ XKP, double d_t, int ibdrad, int iii, double tirr, Ctrl* ctrl) {
+    int nj = g.ny + 4, nk = g.nz + 4;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nj * nk) return;
+    int j = t % nj - 1, k = t / nj - 1;
+    // BOUNDXMIN(12)
+    T[g.idx(0, j, k)] = T[g.idx(1, j, k)] + d_t * (m(TT_M_XXB, 0) - m(TT_M_XXB, 1));
+    T[g.idx(-1, j, k)] = T[g.idx(1, j, k)] + d_t * (m(TT_M_XXB, -1) - m(TT_M_XXB, 1));
+    V[g.idx(1, j, k)] = 0.0;
+    V[g.idx(0, j, k)] = 0.0;
+    if (j >= 0) {
+        G[g.idx(0, j, k)] = G[g.idx(1, j, k)];
+        G[g.idx(-1, j, k)] = G[g.idx(1, j, k)];
+    }
+    if (k >= 0) {
+        H[g.idx(0, j, k)] = H[g.idx(1, j, k)];
+        H[g.idx(-1, j, k)] = H[g.idx(1, j, k)];
+    }
+    // BOUNDXMAX(2): temperature, density and radial velocity above upperbnd
+    int u = ub[g.idx2(j, k)];
+    if (j >= 1 && j <= g.ny && k >= 1 && k <= g.nz) {
+        if (u >= g.nx + 1) {
+            ctrl->bad_bound = 1;
+            return;
+        }
+        double TJ;
+        if (ibdrad == 3) {
+            TJ = tirr * pow(cos(m(TT_M_XYB, j)) * cos(m(TT_M_XZB, k)), 0.25);
+            TJ = TJ > 100.0 ? TJ : 100.0;
+        } else if (ibdrad == 4) {
+            TJ = tirr;
+        } else if (ibdrad == 9 && iii > 1) {
+            long long q = g.idx(u - 1, j, k);
+            TJ = pow((ER[q] / ARAD) + (SIN[q] / (4.0 * SBCONST * RH[q] * XKP[q])), 0.25);
+        } else {  // 7, or 9 at III <= 1
+            TJ = T[g.idx(u - 1, j, k)];
+        }
+        T[g.idx(u, j, k)] = TJ;
+        const double rh_top = (double)1.0e-10f;
+        for (int i = u + 1; i <= g.nx + 2; ++i) {
+            T[g.idx(i, j, k)] = TJ;
+            RH[g.idx(i, j, k)] = rh_top;
+            V[g.idx(i, j, k)] = 0.0;
+        }
+        if (u == g.nx) {
+            V[g.idx(g.nx, j, k)] = 0.0;
+            RH[g.idx(g.nx, j, k)] = rh_top;
+            T[g.idx(g.nx, j, k)] = TJ;
+        }
+    }
+    // stress free above upperbnd
+    if (j >= 0) {
+        double gv = G[g.idx(u, j, k)];
+        for (int i = u + 1; i <= g.nx + 2; ++i) G[g.idx(i, j, k)] = gv;
+    }
+    if (k >= 0) {
+        double hv = H[g.idx(u, j, k)];
+        for (int i = u + 1; i <= g.nx + 2; ++i) H[g.idx(i, j, k)] = hv;
+    }
+}
+
+// BOUNDZMIN(0) (ppbou.f:1833-1878) then BOUNDZMAX(0) (ppbou.f:1946-1988)
+__global__ void k_zbc(Geo g, Met m, double* T, double* RH, double* V, double* G, double* H) {
+    int ni = g.nx + 4, nj = g.ny + 4;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ni * nj) return;
+    int i = t % ni - 1, j = t / ni - 1;
+    const int nz = g.nz;
+    RH[g.idx(i, j, 1)] = RH[g.idx(i, j, 2)];
+    RH[g.idx(i, j, 0)] = RH[g.idx(i, j, 1)];
+    RH[g.idx(i, j, -1)] = RH[g.idx(i, j, 1)];
+    T[g.idx(i, j, 1)] = T[g.idx(i, j, 2)];
+    T[g.idx(i, j, 0)] = T[g.idx(i, j, 1)];
+    T[g.idx(i, j, -1)] = T[g.idx(i, j, 1)];
+    H[g.idx(i, j, 2)] = 0.0;
+    H[g.idx(i, j, 1)] = 0.0;
+    H[g.idx(i, j, 0)] = 0.0;
+    if (i >= 0) {
+        V[g.idx(i, j, 1)] = 0.0;
+        V[g.idx(i, j, 0)] = V[g.idx(i, j, 1)];
+        V[g.idx(i, j, -1)] = V[g.idx(i, j, 1)];
+    }
+    if (j >= 0) {
+        G[g.idx(i, j, 1)] = G[g.idx(i, j, 2)] * m(TT_M_SURZB, 2) / m(TT_M_SURZB, 1);
+        G[g.idx(i, j, 0)] = G[g.idx(i, j, 1)] * m(TT_M_SURZB, 1) / m(TT_M_SURZB, 0);
+        G[g.idx(i, j, -1)] = G[g.idx(i, j, 1)] * m(TT_M_SURZB, 1) / cos(m(TT_M_XZB, -1));
+    }
+    RH[g.idx(i, j, nz)] = RH[g.idx(i, j, nz - 1)];
+    RH[g.idx(i, j, nz + 1)] = RH[g.idx(i, j, nz)];
+    RH[g.idx(i, j, nz + 2)] = RH[g.idx(i, j, nz)];
+    T[g.idx(i, j, nz)] = T[g.idx(i, j, nz - 1)];
+    T[g.idx(i, j, nz + 1)] = T[g.idx(i, j, nz)];
+    T[g.idx(i, j, nz + 2)] = T[g.idx(i, j, nz)];
+    H[g.idx(i, j, nz)] = 0.0;
+    H[g.idx(i, j, nz + 1)] = 0.0;
+    H[g.idx(i, j, nz + 2)] = 0.0;
+    if (i >= 0) {
+        V[g.idx(i, j, nz)] = 0.0;
+        V[g.idx(i, j, nz + 1)] = V[g.idx(i, j, nz)];
+        V[g.idx(i, j, nz + 2)] = V[g.idx(i, j, nz)];
+    }
+    if (j >= 0) {
+        G[g.idx(i, j, nz)] = G[g.idx(i, j, nz - 1)] * m(TT_M_SURZB, nz - 1) / m(TT_M_SURZB, nz);
+        G[g.idx(i, j, nz + 1)] = G[g.idx(i, j, nz)] * m(TT_M_SURZB, nz) / m(TT_M_SURZB, nz + 1);
+        G[g.idx(i, j, nz + 2)] = G[g.idx(i, j, nz)] * m(TT_M_SURZB, nz) / m(TT_M_SURZB, nz + 2);
+    }
+}
+
+int bounds_impl() {
+    Geo& g = L.g;
+    double** f = L.f;
+    if (L.p.bcxmin != 12) return fail(1, "BOUNDXMIN: only type 12 is supported");
+    if (L.p.ibdrad != 3 && L.p.ibdrad != 4 && L.p.ibdrad != 7 && L.p.ibdrad != 9)
+        return fail(1, "BOUNDXMAX(2): IBDRAD must be 3, 4, 7 or 9");
+    long long ns = (long long)(g.nx + 4) * g.nz;
+    int ncol = (g.ny + 4) * (g.nz + 4), nline = (g.nx + 4) * (g.ny + 4);
+    k_shift5<<<nblk(ns, 256), 256, 0, L.s>>>(g, f[TT_T], f[TT_RH], f[TT_V], f[TT_G], f[TT_H]);
+    TT_LAUNCH();
+    k_upperbnd<<<nblk(ncol, 128), 128, 0, L.s>>>(g, f[TT_RH], L.ub);
+    TT_LAUNCH();
+    k_xbc<<<nblk(ncol, 128), 128, 0, L.s>>>(g, met(), L.ub, f[TT_T], f[TT_RH], f[TT_V], f[TT_G], f[TT_H],
+                                            f[TT_ER], f[TT_STELLARINPUT], f[TT_XKAPP], L.p.d_t, L.p.ibdrad,
+                                            L.iii, L.p.tirr, L.ctrl);
+    TT_LAUNCH();
+    k_zbc<<<nblk(nline, 256), 256, 0, L.s>>>(g, met(), f[TT_T], f[TT_RH], f[TT_V], f[TT_G], f[TT_H]);
+    TT_LAUNCH();
+    k_shift5<<<nblk(ns, 256), 256, 0, L.s>>>(g, f[TT_T], f[TT_RH], f[TT_V], f[TT_G], f[TT_H]);
+    TT_LAUNCH();
+    return 0;
+}
+
+__global__ void k_pressg(double* PG, const double* __restrict__ T, const double* __restrict__ RH, double mu,
+                         long long n) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) PG[t] = RGAS * T[t] * RH[t] / mu;
+}
+
+__global__ void k_sound(double* CS, const double* __restrict__ PG, const double* __restrict__ RH, double gam,
+                        long long n) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) CS[t] = sqrt(gam * PG[t] / RH[t]);
+}
+
+// QDENS (ppphy.f:47-76): face densities over their declared ranges
+__global__ void k_qdens(Geo g, Met m, const double* __restrict__ RH, double* QX, double* QY, double* QZ) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.n3) return;
+    int i, j, k;
+    box_ijk(g, t, i, j, k);
+    double r = RH[t];
+    if (i >= 0) QX[t] = m(TT_M_DDX1, i) * r + m(TT_M_DDX0, i) * RH[g.idx(i - 1, j, k)];
+    if (j >= 0) QY[t] = m(TT_M_DDY1, j) * r + m(TT_M_DDY0, j) * RH[g.idx(i, j - 1, k)];
+    if (k >= 0) QZ[t] = m(TT_M_DDZ1, k) * r + m(TT_M_DDZ0, k) * RH[g.idx(i, j, k - 1)];
+}
+
+int pressg_impl() {
+    k_pressg<<<nblk(L.g.n3, 256), 256, 0, L.s>>>(L.f[TT_PG], L.f[TT_T], L.f[TT_RH], L.p.mu_gas, L.g.n3);
+    TT_LAUNCH();
+    return 0;
+}
+
+int qdens_impl() {
+    k_qdens<<<nblk(L.g.n3, 256), 256, 0, L.s>>>(L.g, met(), L.f[TT_RH], L.f[TT_RHQX], L.f[TT_RHQY], L.f[TT_RHQZ]);
+    TT_LAUNCH();
+    return 0;
+}
+
+int sound_impl() {
+    k_sound<<<nblk(L.g.n3, 256), 256, 0, L.s>>>(L.f[TT_CS], L.f[TT_PG], L.f[TT_RH], L.p.gamma, L.g.n3);
+    TT_LAUNCH();
+    return 0;
+}
+
+// ---------------------------------------------------------------- SCHRITT
+// ppsub.f:1-137: minima of RH and T over the whole box, then the Courant
+// limit over the interior (block minima, last block folds them).
+__global__ __launch_bounds__(256) void k_schritt(Geo g, Met m, const double* __restrict__ RH,
+                                                  const double* __restrict__ T, const double* __restrict__ PG,
+                                                  const double* __restrict__ V, const double* __restrict__ G,
+                                                  const double* __restrict__ H, const double* __restrict__ CS,
+                                                  double dtmax, double* part, int nb, Ctrl* ctrl) {
+    __shared__ double sh[3][4];
+    __shared__ int last;
+    double mdt = dtmax, mrh = 1e308, mt = 1e308;
+    int badint = 0;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < g.n3;
+         t += (long long)gridDim.x * blockDim.x) {
+        int i, j, k;
+        box_ijk(g, t, i, j, k);
+        mrh = fmin(mrh, RH[t]);
+        mt = fmin(mt, T[t]);
+        if (i >= 1 && i <= g.nx && j >= 1 && j <= g.ny && k >= 1 && k <= g.nz) {
+            if (RH[t] <= 0.0 || T[t] <= 0.0 || PG[t] <= 0.0) badint = 1;
+            double v1 = fabs(V[g.idx(i + 1, j, k)]), v2 = fabs(G[g.idx(i, j + 1, k)]), v3 = fabs(H[g.idx(i, j, k + 1)]);
+            double cs = CS[t];
+            double z1 = m(TT_M_DXA, i + 1) / (v1 + cs);
+            double z2 = m(TT_M_DYA, j + 1) / (v2 + cs / m(TT_M_GEOXG, i) / m(TT_M_GEOZG, k));
+            double z3 = m(TT_M_DZA, k + 1) / (v3 + cs / m(TT_M_GEOXH, i));
+            mdt = fmin(mdt, fmin(z1, fmin(z2, z3)));
+        }
+    }
+    if (badint) atomicOr(&ctrl->bad_state, 2);
+    double a = wave_min(mdt), b = wave_min(mrh), c = wave_min(mt);
+    int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { sh[0][w] = a; sh[1][w] = b; sh[2][w] = c; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 0; q < 3; ++q)
+            part[q * nb + blockIdx.x] = fmin(fmin(sh[q][0], sh[q][1]), fmin(sh[q][2], sh[q][3]));
+        __threadfence();
+        unsigned prev = atomicAdd(&ctrl->counter2, 1u);
+        last = prev == (unsigned)(nb - 1);
+    }
+    __syncthreads();
+    if (last && threadIdx.x == 0) {
+        __threadfence();
+        double x = 1e308, y = 1e308, z = 1e308;
+        for (int q = 0; q < nb; ++q) {
+            x = fmin(x, part[q]);
+            y = fmin(y, part[nb + q]);
+            z = fmin(z, part[2 * nb + q]);
+        }
+        ctrl->dtmin = x;
+        ctrl->scratch[1] = y;
+        ctrl->scratch[2] = z;
+        ctrl->counter2 = 0;
+    }
+}
+
+// -------------------------------------------------------------- DIFFUSION
+// DIFFUSION_COEF + FILL_DVALUES + DIFFUSION_MAT_ELEM (ppdif.f:169-308):
+// constant diffusivity c = xnue/Pr, SOR matrix with the zero-gradient top
+// row; RHS = the field.
+__global__ void k_dif_matrix(Geo g, Met m, double delt, double c, const double* __restrict__ fld, double* SA,
+                             double* SB, double* SC, double* SD, double* SE, double* SF, double* SG, double* RHS) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)g.nx * g.ny * g.nz) return;
+    int i, j, k;
+    sub_ijk(t, g.nx, g.ny, 1, 1, 1, i, j, k);
+    long long n = g.idx(i, j, k);
+    double rx0 = c / m(TT_M_DXB, i), rx1 = c / m(TT_M_DXB, i + 1);
+    double ry0 = c / (m(TT_M_DYB, j) * m(TT_M_GEOXG, i) * m(TT_M_GEOZG, k));
+    double ry1 = c / (m(TT_M_DYB, j + 1) * m(TT_M_GEOXG, i) * m(TT_M_GEOZG, k));
+    double rz0 = c / (m(TT_M_DZB, k) * m(TT_M_GEOXH, i)), rz1 = c / (m(TT_M_DZB, k + 1) * m(TT_M_GEOXH, i));
+    double sxb = m(TT_M_SURXB, i), dxa = m(TT_M_DXA, i);
+    double gx = m(TT_M_GEOXG, i), szb = m(TT_M_SURZB, k), dya = m(TT_M_DYA, j), dza = m(TT_M_DZA, k);
+    double d1 = delt * m(TT_M_SURXA, i + 1) * rx1 / (sxb * dxa);
+    double d2 = delt * m(TT_M_SURXA, i) * rx0 / (sxb * dxa);
+    double d3 = delt * ry1 / (gx * szb * dya);
+    double d4 = delt * ry0 / (gx * szb * dya);
+    double d5 = delt * rz1 * m(TT_M_SURZA, k + 1) / (gx * szb * dza);
+    double d6 = delt * rz0 * m(TT_M_SURZA, k) / (gx * szb * dza);
+    double d16 = d1 + d2 + d3 + d4 + d5 + d6;
+    if (i == g.nx) {
+        SA[n] = 1.0 + d2; SB[n] = -d2;
+        SC[n] = 0.0; SD[n] = 0.0; SE[n] = 0.0; SF[n] = 0.0; SG[n] = 0.0;
+    } else {
+        SA[n] = 1.0 + d16; SB[n] = -d2; SC[n] = -d1; SD[n] = -d4; SE[n] = -d3; SF[n] = -d6; SG[n] = -d5;
+    }
+    RHS[n] = fld[n];
+}
+
+int sync_check() {
+    TT_HIP(hipMemcpyAsync(L.hctrl, L.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, L.s));
+    TT_HIP(hipStreamSynchronize(L.s));
+    if (L.hctrl->bad_bound) return fail(3, "error in boundxmax: upperbnd above the domain");
+    if (L.hctrl->bad_q) return fail(2, "ERROR IN DELTA_Q: non-positive value of Q");
+    return 0;
+}
+
+int diffusion_impl(double delt, double xnue, double pr, int* iters) {
+    Geo& g = L.g;
+    double** f = L.f;
+    long long n = (long long)g.nx * g.ny * g.nz;
+    k_dif_matrix<<<nblk(n, 256), 256, 0, L.s>>>(g, met(), delt, xnue / pr, f[TT_T], f[TT_SOR_A], f[TT_SOR_B],
+                                                f[TT_SOR_C], f[TT_SOR_D], f[TT_SOR_E], f[TT_SOR_F], f[TT_SOR_G],
+                                                f[TT_RHS_ER]);
+    TT_LAUNCH();
+    tt_solver_opts keep = L.opts;
+    L.opts.precision = 0;
+    tt_sor_stats st{};
+    int rc = sor_solve_impl(0, &st, 1);
+    L.opts = keep;
+    if (iters) *iters = st.sor_iter;
+    return rc;
+}
+
+}  // namespace tt
+
+using namespace tt;
+
+extern "C" {
+
+int tt_bounds(void) {
+    if (!L.ready) return fail(1, "tt_init has not been called");
+    int rc = bounds_impl();
+    if (rc) return rc;
+    return sync_check();
+}
+
+int tt_pressg(void) {
+    if (!L.ready) return fail(1, "tt_init has not been called");
+    int rc = pressg_impl();
+    return rc ? rc : sync_check();
+}
+
+int tt_qdens(void) {
+    if (!L.ready) return fail(1, "tt_init has not been called");
+    int rc = qdens_impl();
+    return rc ? rc : sync_check();
+}
+
+int tt_sound(void) {
+    if (!L.ready) return fail(1, "tt_init has not been called");
+    int rc = sound_impl();
+    return rc ? rc : sync_check();
+}
+
+int tt_schritt(double* delt) {
+    if (!L.ready) return fail(1, "tt_init has not been called");
+    if (!delt) return fail(1, "null delt");
+    Geo& g = L.g;
+    if (L.p.fdelt == 0.0) {
+        *delt = L.p.dtmax;
+        return 0;
+    }
+    int rc = sound_impl();
+    if (rc) return rc;
+    int nb = 1024;
+    if (3 * nb > 2 * L.npartial) nb = 2 * L.npartial / 3;
+    k_schritt<<<nb, 256, 0, L.s>>>(g, met(), L.f[TT_RH], L.f[TT_T], L.f[TT_PG], L.f[TT_V], L.f[TT_G], L.f[TT_H],
+                                   L.f[TT_CS], L.p.dtmax, L.partial, nb, L.ctrl);
+    TT_LAUNCH();
+    rc = sync_check();
+    if (rc) return rc;
+    if (L.hctrl->scratch[1] <= 0.0 || L.hctrl->scratch[2] <= 0.0) {
+        if (L.hctrl->bad_state & 2) {
+            L.hctrl->bad_state = 0;
+            hipMemcpyAsync(L.ctrl, L.hctrl, sizeof(Ctrl), hipMemcpyHostToDevice, L.s);
+            hipStreamSynchronize(L.s);
+            return fail(2, "non-positive value of rh/pg/t");
+        }
+    }
+    if (L.hctrl->bad_state) {
+        L.hctrl->bad_state = 0;
+        hipMemcpyAsync(L.ctrl, L.hctrl, sizeof(Ctrl), hipMemcpyHostToDevice, L.s);
+        hipStreamSynchronize(L.s);
+    }
+    double ldt = L.hctrl->dtmin;
+    if (ldt <= 0.0) return fail(2, "local dt negative");
+    ldt = (ldt < L.p.dtmax ? ldt : L.p.dtmax) * L.p.fdelt;
+    *delt = ldt;
+    return 0;
+}
+
+int tt_diffusion(double delt, double xnue, double pr_num, int* iters) {
+    if (!L.ready) return fail(1, "tt_init has not been called");
+    int rc = diffusion_impl(delt, xnue, pr_num, iters);
+    int rc2 = sync_check();
+    return rc2 ? rc2 : rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,719 @@
+// tt_sor.hip -- red-black SOR of ttcode (ppsor.f, ppdif.f) on gfx950.
+//
+// SOLVE_WITH_SOR (ppsor.f:2-187) sweeps red cells ((i+j+k) even) then black
+// cells, refreshing the periodic phi ghosts after each half sweep, and stops
+// when the L1 sum of the residuals taken during the sweep, divided by
+// sum|RHS|, is <= EPSMAT (or at ITMAX).  Red cells depend only on black cells
+// and vice versa (ny even keeps the colouring consistent across the phi
+// wrap), so each half sweep is one fully parallel kernel whose result equals
+// the reference's sequential sweep; only the summation order of the L1 norm
+// differs.
+//
+// Colour-split layout.  For colour c and row (j,k) the cells of that colour
+// are i = 2h + 1 + s, h = 0..hx-1 (hx = nx/2), s = (j+k+c+1) & 1.  The seven
+// coefficients and the RHS of a colour are dense arrays [k][j][h] (one
+// coalesced stream each); the unknowns of a colour are padded by one slot at
+// h = -1 and h = hx and one row at k = 0 and k = nz+1, which hold the fixed
+// r and theta ghost values (zero for delta_ER).  The r neighbours of a cell
+// of colour c are slots h+s-1 and h+s of the other colour's row, its phi and
+// theta neighbours slot h of the neighbouring rows.
+//
+// Convergence.  Each half sweep writes one partial |resid| sum per block; the
+// last block of the black sweep sums them in a fixed order, advances the
+// iteration counter and sets ctrl->done, which every later sweep reads first
+// and returns on.  The host queues `batch` iterations between reads of the
+// control block, so the iteration at which the solve stops is exactly the
+// first one that meets the tolerance.
+#include <cmath>
+#include <vector>
+
+#include "tt_common.h"
+
+namespace tt {
+
+struct CS {
+    int hx, ny, nz, px;
+    long long ncell;  // cells per colour
+};
+
+static CS g_cs;
+// FP64 coefficients + RHS per colour: [c][0..7] = A,B,C,D,E,F,G,RHS
+static double* c64[2][8];
+static float* c32[2][8];
+static double* x64[2];  // padded unknowns
+static float* x32[2];
+static double* r64 = nullptr;  // scratch for the FP64 residual (both colours)
+static int g_nb = 0;           // sweep blocks
+static int g_chunk = 0;
+static double** g_dtab = nullptr;  // device table of the 16 FP64 coefficient arrays
+static std::vector<hipEvent_t> ev_pool, ev_pend;
+static std::vector<int> ev_kind;
+
+static long long padded_len() { return (long long)g_cs.px * g_cs.ny * (g_cs.nz + 2); }
+
+void sor_free() {
+    for (int c = 0; c < 2; ++c) {
+        for (int q = 0; q < 8; ++q) {
+            if (c64[c][q]) hipFree(c64[c][q]);
+            if (c32[c][q]) hipFree(c32[c][q]);
+            c64[c][q] = nullptr;
+            c32[c][q] = nullptr;
+        }
+        if (x64[c]) hipFree(x64[c]);
+        if (x32[c]) hipFree(x32[c]);
+        x64[c] = nullptr;
+        x32[c] = nullptr;
+    }
+    if (r64) hipFree(r64);
+    r64 = nullptr;
+    if (g_dtab) hipFree(g_dtab);
+    g_dtab = nullptr;
+    for (auto e : ev_pool) hipEventDestroy(e);
+    for (auto e : ev_pend) hipEventDestroy(e);
+    ev_pool.clear();
+    ev_pend.clear();
+    ev_kind.clear();
+}
+
+int sor_alloc() {
+    g_cs.hx = L.g.nx / 2;
+    g_cs.ny = L.g.ny;
+    g_cs.nz = L.g.nz;
+    g_cs.px = g_cs.hx + 2;
+    g_cs.ncell = (long long)g_cs.hx * g_cs.ny * g_cs.nz;
+    for (int c = 0; c < 2; ++c) {
+        for (int q = 0; q < 8; ++q) {
+            TT_HIP(hipMalloc(&c64[c][q], g_cs.ncell * sizeof(double)));
+            TT_HIP(hipMalloc(&c32[c][q], g_cs.ncell * sizeof(float)));
+        }
+        TT_HIP(hipMalloc(&x64[c], padded_len() * sizeof(double)));
+        TT_HIP(hipMalloc(&x32[c], padded_len() * sizeof(float)));
+        TT_HIP(hipMemsetAsync(x64[c], 0, padded_len() * sizeof(double), L.s));
+        TT_HIP(hipMemsetAsync(x32[c], 0, padded_len() * sizeof(float), L.s));
+    }
+    TT_HIP(hipMalloc(&r64, 2 * g_cs.ncell * sizeof(double)));
+    // blocks: enough to fill 256 CUs several times, a multiple of 8 (XCDs)
+    long long want = (g_cs.ncell + 255) / 256;
+    int nb = (int)(want < 2048 ? want : 2048);
+    nb = ((nb + 7) / 8) * 8;
+    if (nb > L.npartial) nb = L.npartial;
+    g_nb = nb;
+    g_chunk = (int)((g_cs.ncell + nb - 1) / nb);
+    g_chunk = ((g_chunk + 255) / 256) * 256;
+    return 0;
+}
+
+// logical block id: blocks b, b+8, b+16, ... run on the same XCD; give each
+// XCD one contiguous slab of the domain so phi/theta neighbour rows share its L2.
+__device__ inline int xcd_block(int nb) {
+    int b = blockIdx.x;
+    return (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
+}
+
+__device__ inline double block_sum256(double v, double* sh) {
+    v = wave_sum(v);
+    int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0) s = sh[0] + sh[1] + sh[2] + sh[3];
+    __syncthreads();
+    return s;
+}
+
+// Finish an iteration: called by the last block of a kernel that has written
+// nparts partials.  Sums them in a fixed order and updates the control block.
+__device__ void finish_iteration(Ctrl* ctrl, const double* part, int nparts, double* sh) {
+    double s = 0.0;
+    for (int q = threadIdx.x; q < nparts; q += 256) s += part[q];
+    s = block_sum256(s, sh);
+    if (threadIdx.x == 0) {
+        int it = ctrl->it + 1;
+        ctrl->it = it;
+        ctrl->rnorm = s;
+        double ratio = s / ctrl->rhsnorm;
+        if (it == ctrl->itmax) ctrl->done = 2;
+        else if (!(ratio > ctrl->eps || it < ctrl->itmin)) ctrl->done = 1;
+        ctrl->counter = 0;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_sweep(int c, const T* __restrict__ A, const T* __restrict__ B,
+                                               const T* __restrict__ C, const T* __restrict__ D,
+                                               const T* __restrict__ E, const T* __restrict__ F,
+                                               const T* __restrict__ G, const T* __restrict__ R,
+                                               T* __restrict__ xc, const T* __restrict__ xo, CS cs, T omega,
+                                               double* part, int nb, int chunk, Ctrl* ctrl, int is_last) {
+    __shared__ double sh[4];
+    __shared__ int last;
+    if (ctrl->done) return;
+    const int lb = xcd_block(nb);
+    const long long start = (long long)lb * chunk;
+    long long end = start + chunk;
+    if (end > cs.ncell) end = cs.ncell;
+    double acc = 0.0;
+    const int hx = cs.hx, ny = cs.ny, px = cs.px;
+    for (long long t = start + threadIdx.x; t < end; t += 256) {
+        const int h = (int)(t % hx);
+        const long long r = t / hx;
+        const int jj = (int)(r % ny);
+        const int kk = (int)(r / ny);
+        const int s = (jj + kk + c + 1) & 1;  // (j+k+c+1)&1 with j=jj+1, k=kk+1
+        const long long row = (long long)px * (jj + (long long)ny * (kk + 1));
+        const int jm = jj == 0 ? ny - 1 : jj - 1;
+        const int jp = jj == ny - 1 ? 0 : jj + 1;
+        const T xs = xc[row + h + 1];
+        const T xm = xo[row + h + s];
+        const T xp = xo[row + h + s + 1];
+        const T xjm = xo[(long long)px * (jm + (long long)ny * (kk + 1)) + h + 1];
+        const T xjp = xo[(long long)px * (jp + (long long)ny * (kk + 1)) + h + 1];
+        const T xkm = xo[(long long)px * (jj + (long long)ny * kk) + h + 1];
+        const T xkp = xo[(long long)px * (jj + (long long)ny * (kk + 2)) + h + 1];
+        const T a = A[t];
+        const T resid = a * xs + B[t] * xm + C[t] * xp + D[t] * xjm + E[t] * xjp + F[t] * xkm + G[t] * xkp - R[t];
+        acc += fabs((double)resid);
+        xc[row + h + 1] = xs - omega * resid / a;
+    }
+    double bs = block_sum256(acc, sh);
+    if (threadIdx.x == 0) {
+        part[c * nb + lb] = bs;
+        last = 0;
+        if (is_last) {
+            __threadfence();
+            unsigned prev = atomicAdd(&ctrl->counter, 1u);
+            last = (prev == (unsigned)(nb - 1));
+        }
+    }
+    __syncthreads();
+    if (last) {
+        __threadfence();
+        finish_iteration(ctrl, part, 2 * nb, sh);
+    }
+}
+
+// ---------------------------------------------------------- matrix fill
+// FILL_SOR_MATRIX (ppsor.f:189-277) + B_VECTOR_BOUNDARIES (ppsor.f:280-329)
+// for one rank without poles: identity rows at i >= upperbnd, i = nx, k = 1,
+// k = nz; couplings cut next to them.  Writes the ghosted SOR_A..G, RHS_ER.
+__global__ void k_fill_matrix(Geo g, const int32_t* __restrict__ ub, const double* __restrict__ e1,
+                              const double* __restrict__ e2, const double* __restrict__ e3,
+                              const double* __restrict__ e4, const double* __restrict__ e5,
+                              const double* __restrict__ e6, const double* __restrict__ e7,
+                              const double* __restrict__ e8, const double* __restrict__ e16, double* SA,
+                              double* SB, double* SC, double* SD, double* SE, double* SF, double* SG,
+                              double* RHS) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)g.nx * g.ny * g.nz) return;
+    int i, j, k;
+    sub_ijk(t, g.nx, g.ny, 1, 1, 1, i, j, k);
+    long long n = g.idx(i, j, k);
+    double E8 = e8[n];
+    double a = e7[n] + E8 * e16[n], b = -E8 * e2[n], c = -E8 * e1[n], d = -E8 * e4[n], e = -E8 * e3[n],
+           f = -E8 * e6[n], gg = -E8 * e5[n];
+    int u = ub[g.idx2(j, k)];
+    bool ident = (i >= u) || (i == g.nx) || (k == 1) || (k == g.nz);
+    if (ident) { a = 1.0; b = c = d = e = f = gg = 0.0; }
+    if (i == u - 1 || i == g.nx - 1) c = 0.0;
+    if (k == 2) f = 0.0;
+    if (k == g.nz - 1) gg = 0.0;
+    SA[n] = a; SB[n] = b; SC[n] = c; SD[n] = d; SE[n] = e; SF[n] = f; SG[n] = gg;
+    if (ident) RHS[n] = 0.0;
+}
+
+// ghosted SOR_A..G, RHS -> colour-split FP64; per-block sum |RHS|; the last
+// block sets ctrl->rhsnorm.  Cells are visited colour by colour.
+__global__ __launch_bounds__(256) void k_pack(Geo g, CS cs, const double* __restrict__ SA,
+                                              const double* __restrict__ SB, const double* __restrict__ SC,
+                                              const double* __restrict__ SD, const double* __restrict__ SE,
+                                              const double* __restrict__ SF, const double* __restrict__ SG,
+                                              const double* __restrict__ RHS, double* A0, double* B0, double* C0,
+                                              double* D0, double* E0, double* F0, double* G0, double* R0,
+                                              double* A1, double* B1, double* C1, double* D1, double* E1,
+                                              double* F1, double* G1, double* R1, double* part, int nb, int chunk,
+                                              Ctrl* ctrl) {
+    __shared__ double sh[4];
+    __shared__ int last;
+    const int lb = blockIdx.x;
+    long long start = (long long)lb * chunk, end = start + chunk;
+    long long tot = 2 * cs.ncell;
+    if (end > tot) end = tot;
+    double acc = 0.0;
+    for (long long t = start + threadIdx.x; t < end; t += 256) {
+        int c = t >= cs.ncell;
+        long long e = t - c * cs.ncell;
+        int h = (int)(e % cs.hx);
+        long long r = e / cs.hx;
+        int j = (int)(r % cs.ny) + 1, k = (int)(r / cs.ny) + 1;
+        int s = (j + k + c + 1) & 1;
+        int i = 2 * h + 1 + s;
+        long long n = g.idx(i, j, k);
+        double rhs = RHS[n];
+        acc += fabs(rhs);
+        if (c == 0) {
+            A0[e] = SA[n]; B0[e] = SB[n]; C0[e] = SC[n]; D0[e] = SD[n];
+            E0[e] = SE[n]; F0[e] = SF[n]; G0[e] = SG[n]; R0[e] = rhs;
+        } else {
+            A1[e] = SA[n]; B1[e] = SB[n]; C1[e] = SC[n]; D1[e] = SD[n];
+            E1[e] = SE[n]; F1[e] = SF[n]; G1[e] = SG[n]; R1[e] = rhs;
+        }
+    }
+    double bs = block_sum256(acc, sh);
+    if (threadIdx.x == 0) {
+        part[lb] = bs;
+        __threadfence();
+        unsigned prev = atomicAdd(&ctrl->counter2, 1u);
+        last = (prev == (unsigned)(nb - 1));
+    }
+    __syncthreads();
+    if (last) {
+        __threadfence();
+        double s = 0.0;
+        for (int q = threadIdx.x; q < nb; q += 256) s += part[q];
+        s = block_sum256(s, sh);
+        if (threadIdx.x == 0) {
+            ctrl->rhsnorm = s;
+            ctrl->counter2 = 0;
+        }
+    }
+}
+
+// padded unknowns of colour c from a ghosted field (every slot, pads included)
+__global__ void k_load_x(Geo g, CS cs, int c, const double* __restrict__ fld, double* x, int zero_ghosts) {
+    long long n = (long long)cs.px * cs.ny * (cs.nz + 2);
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    int hp = (int)(t % cs.px);
+    long long r = t / cs.px;
+    int j = (int)(r % cs.ny) + 1, k = (int)(r / cs.ny);
+    int h = hp - 1;
+    int s = (j + k + c + 1) & 1;
+    int i = 2 * h + 1 + s;
+    bool interior = (h >= 0 && h < cs.hx && k >= 1 && k <= cs.nz);
+    double v = 0.0;
+    if (interior || !zero_ghosts) {
+        if (i >= -1 && i <= g.nx + 2) v = fld[g.idx(i, j, k)];
+    }
+    x[t] = v;
+}
+
+// unknowns of both colours -> interior of a ghosted field
+__global__ void k_store_x(Geo g, CS cs, const double* __restrict__ x0, const double* __restrict__ x1, double* fld) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 2 * cs.ncell) return;
+    int c = t >= cs.ncell;
+    long long e = t - c * cs.ncell;
+    int h = (int)(e % cs.hx);
+    long long r = e / cs.hx;
+    int j = (int)(r % cs.ny) + 1, k = (int)(r / cs.ny) + 1;
+    int s = (j + k + c + 1) & 1;
+    int i = 2 * h + 1 + s;
+    const double* x = c ? x1 : x0;
+    fld[g.idx(i, j, k)] = x[(long long)cs.px * ((j - 1) + (long long)cs.ny * k) + h + 1];
+}
+
+// PLANETSHIFT for one rank (ppshift.f:1009-1214): periodic phi ghosts for
+// k = 1..nz, i = low..nx+2; skipm1 for arrays without a j = -1 plane.
+__global__ void k_planetshift(double* a, Geo g, int low, int skipm1) {
+    int ni = g.nx + 2 - low + 1;
+    long long n = (long long)ni * g.nz;
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    int i = (int)(t % ni) + low, k = (int)(t / ni) + 1;
+    a[g.idx(i, 0, k)] = a[g.idx(i, g.ny, k)];
+    if (!skipm1) a[g.idx(i, -1, k)] = a[g.idx(i, g.ny - 1, k)];
+    a[g.idx(i, g.ny + 1, k)] = a[g.idx(i, 1, k)];
+    a[g.idx(i, g.ny + 2, k)] = a[g.idx(i, 2, k)];
+}
+
+int planetshift_impl(double* a, int f) {
+    int low = (f == 3) ? 0 : -1;
+    int skip = (f == 4 || f == 11) ? 1 : 0;
+    long long n = (long long)(L.g.nx + 3 - low) * L.g.nz;
+    k_planetshift<<<nblk(n, 256), 256, 0, L.s>>>(a, L.g, low, skip);
+    TT_LAUNCH();
+    return 0;
+}
+
+// FP64 true residual r = RHS - A x for both colours (defect correction and
+// the guess check); per-block sum |r|, last block -> ctrl->scratch[0].
+__global__ __launch_bounds__(256) void k_resid64(CS cs, const double* const* __restrict__ cf, const double* x0,
+                                                  const double* x1, double* r, float* r32_0, float* r32_1,
+                                                  double* part, int nb, int chunk, Ctrl* ctrl) {
+    __shared__ double sh[4];
+    __shared__ int last;
+    const int lb = blockIdx.x;
+    long long start = (long long)lb * chunk, end = start + chunk;
+    if (end > 2 * cs.ncell) end = 2 * cs.ncell;
+    double acc = 0.0;
+    const int hx = cs.hx, ny = cs.ny, px = cs.px;
+    for (long long t = start + threadIdx.x; t < end; t += 256) {
+        int c = t >= cs.ncell;
+        long long e = t - c * cs.ncell;
+        int h = (int)(e % hx);
+        long long rr = e / hx;
+        int jj = (int)(rr % ny), kk = (int)(rr / ny);
+        int s = (jj + kk + c + 1) & 1;
+        const double* xc = c ? x1 : x0;
+        const double* xo = c ? x0 : x1;
+        const double* const* q = cf + 8 * c;
+        long long row = (long long)px * (jj + (long long)ny * (kk + 1));
+        int jm = jj == 0 ? ny - 1 : jj - 1, jp = jj == ny - 1 ? 0 : jj + 1;
+        double ax = q[0][e] * xc[row + h + 1] + q[1][e] * xo[row + h + s] + q[2][e] * xo[row + h + s + 1] +
+                    q[3][e] * xo[(long long)px * (jm + (long long)ny * (kk + 1)) + h + 1] +
+                    q[4][e] * xo[(long long)px * (jp + (long long)ny * (kk + 1)) + h + 1] +
+                    q[5][e] * xo[(long long)px * (jj + (long long)ny * kk) + h + 1] +
+                    q[6][e] * xo[(long long)px * (jj + (long long)ny * (kk + 2)) + h + 1];
+        double res = q[7][e] - ax;
+        acc += fabs(res);
+        if (r) r[t] = res;
+        if (r32_0) (c ? r32_1 : r32_0)[e] = (float)res;
+    }
+    double bs = block_sum256(acc, sh);
+    if (threadIdx.x == 0) {
+        part[lb] = bs;
+        __threadfence();
+        unsigned prev = atomicAdd(&ctrl->counter2, 1u);
+        last = (prev == (unsigned)(nb - 1));
+    }
+    __syncthreads();
+    if (last) {
+        __threadfence();
+        double s = 0.0;
+        for (int q = threadIdx.x; q < nb; q += 256) s += part[q];
+        s = block_sum256(s, sh);
+        if (threadIdx.x == 0) {
+            ctrl->scratch[0] = s;
+            ctrl->counter2 = 0;
+        }
+    }
+}
+
+__global__ void k_to_f32(const double* __restrict__ a, float* b, long long n) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) b[t] = (float)a[t];
+}
+
+// x64 (padded) += x32 (padded) over the interior slots
+__global__ void k_add_corr(CS cs, double* x, const float* __restrict__ c) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cs.ncell) return;
+    int h = (int)(t % cs.hx);
+    long long r = t / cs.hx;
+    int jj = (int)(r % cs.ny), kk = (int)(r / cs.ny);
+    long long p = (long long)cs.px * (jj + (long long)cs.ny * (kk + 1)) + h + 1;
+    x[p] = x[p] + (double)c[p];
+}
+
+__global__ void k_scale(double* a, double s, long long n) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) a[t] = a[t] * s;
+}
+
+// ------------------------------------------------------------- host side
+static hipEvent_t ev_get() {
+    hipEvent_t e;
+    if (!ev_pool.empty()) {
+        e = ev_pool.back();
+        ev_pool.pop_back();
+    } else {
+        hipEventCreate(&e);
+    }
+    return e;
+}
+
+static void ev_drain() {
+    for (size_t q = 0; q + 1 < ev_pend.size(); q += 2) {
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, ev_pend[q], ev_pend[q + 1]);
+        if (ev_kind[q / 2] == 64) { L.ms64 += ms; L.n64++; }
+        else { L.ms32 += ms; L.n32++; }
+        ev_pool.push_back(ev_pend[q]);
+        ev_pool.push_back(ev_pend[q + 1]);
+    }
+    ev_pend.clear();
+    ev_kind.clear();
+}
+
+template <typename T>
+static int launch_sweep(int c, T** coef0, T** coef1, T* xa, T* xb, double omega, int is_last) {
+    T** q = c ? coef1 : coef0;
+    T* xc = c ? xb : xa;
+    T* xo = c ? xa : xb;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (L.timing) {
+        e0 = ev_get();
+        e1 = ev_get();
+        hipEventRecord(e0, L.s);
+    }
+    k_sweep<T><<<g_nb, 256, 0, L.s>>>(c, q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7], xc, xo, g_cs, (T)omega,
+                                      L.partial, g_nb, g_chunk, L.ctrl, is_last);
+    TT_LAUNCH();
+    if (L.timing) {
+        hipEventRecord(e1, L.s);
+        ev_pend.push_back(e0);
+        ev_pend.push_back(e1);
+        ev_kind.push_back(sizeof(T) == 8 ? 64 : 32);
+    }
+    return 0;
+}
+
+static int read_ctrl() {
+    TT_HIP(hipMemcpyAsync(L.hctrl, L.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, L.s));
+    TT_HIP(hipStreamSynchronize(L.s));
+    if (L.timing) ev_drain();
+    return 0;
+}
+
+static int set_ctrl_solve(int itmax, int itmin, double eps) {
+    // host-side fields of the control block, written before the solve
+    Ctrl c{};
+    TT_HIP(hipMemcpyAsync(L.hctrl, L.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, L.s));
+    TT_HIP(hipStreamSynchronize(L.s));
+    c = *L.hctrl;
+    c.done = 0;
+    c.it = 0;
+    c.itmax = itmax;
+    c.itmin = itmin;
+    c.eps = eps;
+    c.counter = 0;
+    c.counter2 = 0;
+    *L.hctrl = c;
+    TT_HIP(hipMemcpyAsync(L.ctrl, L.hctrl, sizeof(Ctrl), hipMemcpyHostToDevice, L.s));
+    return 0;
+}
+
+// SOR iterations on (coef, x) until ctrl->done; returns completed iterations
+template <typename T>
+static int run_sweeps(T** coef0, T** coef1, T* xa, T* xb, double omega, int* iters, double* rnorm) {
+    int batch = L.opts.batch > 0 ? L.opts.batch : 16;
+    long long launched = 0;
+    for (;;) {
+        // the control block's iteration cap ends the loop; a solve that has
+        // run past it without the device reporting completion is an error
+        if (launched > (long long)L.hctrl->itmax + 2 * batch)
+            return fail(12, "SOR: device did not report completion");
+        launched += batch;
+        for (int b = 0; b < batch; ++b) {
+            int rc = launch_sweep<T>(0, coef0, coef1, xa, xb, omega, 0);
+            if (rc) return rc;
+            rc = launch_sweep<T>(1, coef0, coef1, xa, xb, omega, 1);
+            if (rc) return rc;
+        }
+        int rc = read_ctrl();
+        if (rc) return rc;
+        if (L.hctrl->done) break;
+    }
+    *iters = L.hctrl->it;
+    *rnorm = L.hctrl->rnorm;
+    return 0;
+}
+
+int sor_pack_impl() {
+    double** f = L.f;
+    int nb = L.npartial;
+    long long tot = 2 * g_cs.ncell;
+    int chunk = (int)((tot + nb - 1) / nb);
+    chunk = ((chunk + 255) / 256) * 256;
+    nb = (int)((tot + chunk - 1) / chunk);
+    k_pack<<<nb, 256, 0, L.s>>>(L.g, g_cs, f[TT_SOR_A], f[TT_SOR_B], f[TT_SOR_C], f[TT_SOR_D], f[TT_SOR_E],
+                                f[TT_SOR_F], f[TT_SOR_G], f[TT_RHS_ER], c64[0][0], c64[0][1], c64[0][2],
+                                c64[0][3], c64[0][4], c64[0][5], c64[0][6], c64[0][7], c64[1][0], c64[1][1],
+                                c64[1][2], c64[1][3], c64[1][4], c64[1][5], c64[1][6], c64[1][7],
+                                L.partial + 2 * L.npartial - nb, nb, chunk, L.ctrl);
+    TT_LAUNCH();
+    return 0;
+}
+
+static int resid64(double* out_norm, bool to32) {
+    // device pointer table of the 16 coefficient arrays
+    if (!g_dtab) {
+        TT_HIP(hipMalloc(&g_dtab, 16 * sizeof(double*)));
+        double* h[16];
+        for (int c = 0; c < 2; ++c)
+            for (int q = 0; q < 8; ++q) h[8 * c + q] = c64[c][q];
+        TT_HIP(hipMemcpy(g_dtab, h, sizeof h, hipMemcpyHostToDevice));
+    }
+    double** dtab = g_dtab;
+    int nb = L.npartial;
+    long long tot = 2 * g_cs.ncell;
+    int chunk = (int)((tot + nb - 1) / nb);
+    chunk = ((chunk + 255) / 256) * 256;
+    nb = (int)((tot + chunk - 1) / chunk);
+    k_resid64<<<nb, 256, 0, L.s>>>(g_cs, (const double* const*)dtab, x64[0], x64[1], nullptr,
+                                   to32 ? c32[0][7] : nullptr, to32 ? c32[1][7] : nullptr,
+                                   L.partial + 2 * L.npartial - nb, nb, chunk, L.ctrl);
+    TT_LAUNCH();
+    int rc = read_ctrl();
+    if (rc) return rc;
+    *out_norm = L.hctrl->scratch[0];
+    return 0;
+}
+
+// mode 0: delta_ER (x from 0 or a guess held in x64, zero ghosts, ITMIN=1,
+// ITMAX from the parameters); mode 1: DIFFUSION of T (x and ghosts from T,
+// ITMIN=5, SORITMAX=500, ppdif.f:14-15).
+int sor_solve_impl(int fill_from_evals, tt_sor_stats* st, int mode) {
+    Geo& g = L.g;
+    double** f = L.f;
+    tt_sor_stats s{};
+    if (fill_from_evals) {
+        long long n = (long long)g.nx * g.ny * g.nz;
+        k_fill_matrix<<<nblk(n, 256), 256, 0, L.s>>>(g, L.ub, f[TT_EVAL1], f[TT_EVAL2], f[TT_EVAL3], f[TT_EVAL4],
+                                                     f[TT_EVAL5], f[TT_EVAL6], f[TT_EVAL7], f[TT_EVAL8],
+                                                     f[TT_EVAL_1_6], f[TT_SOR_A], f[TT_SOR_B], f[TT_SOR_C],
+                                                     f[TT_SOR_D], f[TT_SOR_E], f[TT_SOR_F], f[TT_SOR_G],
+                                                     f[TT_RHS_ER]);
+        TT_LAUNCH();
+    }
+    int rc = set_ctrl_solve(mode == 1 ? 500 : L.p.itmax, mode == 1 ? 5 : 1, L.p.epsmat);
+    if (rc) return rc;
+    rc = sor_pack_impl();
+    if (rc) return rc;
+    long long pl = padded_len();
+    double* target = mode == 1 ? f[TT_T] : f[TT_DELTA_ER];
+    int prec = mode == 1 ? 0 : L.opts.precision;
+    // initial unknowns
+    if (mode == 1) {
+        for (int c = 0; c < 2; ++c) {
+            k_load_x<<<nblk(pl, 256), 256, 0, L.s>>>(g, g_cs, c, f[TT_T], x64[c], 0);
+            TT_LAUNCH();
+        }
+    } else {
+        // cold start: zero; a guess (warm/learned) is staged in DELTA_ER by
+        // the caller and accepted only if its residual is not larger
+        bool use_guess = L.opts.guess != 0 && L.have_prev_guess;
+        for (int c = 0; c < 2; ++c) {
+            k_load_x<<<nblk(pl, 256), 256, 0, L.s>>>(g, g_cs, c, f[TT_DELTA_ER], x64[c], 1);
+            TT_LAUNCH();
+        }
+        rc = read_ctrl();
+        if (rc) return rc;
+        s.r0_cold = L.hctrl->rhsnorm;
+        s.guess_used = 0;
+        if (use_guess) {
+            double r0;
+            rc = resid64(&r0, false);
+            if (rc) return rc;
+            s.r0_guess = r0;
+            if (r0 <= s.r0_cold) s.guess_used = L.opts.guess;
+        }
+        if (!s.guess_used) {
+            TT_HIP(hipMemsetAsync(x64[0], 0, pl * sizeof(double), L.s));
+            TT_HIP(hipMemsetAsync(x64[1], 0, pl * sizeof(double), L.s));
+        }
+    }
+    int iters = 0;
+    double rnorm = 0.0;
+    if (prec == 0) {
+        rc = run_sweeps<double>(c64[0], c64[1], x64[0], x64[1], L.p.sorparam, &iters, &rnorm);
+        if (rc) return rc;
+        s.sor_iter = iters;
+        s.rnorm = rnorm;
+    } else {
+        // FP32 SOR inside FP64 defect correction
+        for (int c = 0; c < 2; ++c)
+            for (int q = 0; q < 7; ++q) {
+                k_to_f32<<<nblk(g_cs.ncell, 256), 256, 0, L.s>>>(c64[c][q], c32[c][q], g_cs.ncell);
+                TT_LAUNCH();
+            }
+        double rhs_norm = L.hctrl->rhsnorm;
+        double rn;
+        rc = resid64(&rn, true);
+        if (rc) return rc;
+        int outer = 0, total32 = 0;
+        const int max_outer = 200;
+        while (!(rn / rhs_norm <= L.p.epsmat) && outer < max_outer && total32 < L.p.itmax) {
+            ++outer;
+            // inner: A c = r in FP32 from c = 0, L1 during-sweep ratio vs sum|r|
+            TT_HIP(hipMemsetAsync(x32[0], 0, pl * sizeof(float), L.s));
+            TT_HIP(hipMemsetAsync(x32[1], 0, pl * sizeof(float), L.s));
+            rc = read_ctrl();
+            if (rc) return rc;
+            Ctrl c = *L.hctrl;
+            c.done = 0; c.it = 0; c.counter = 0; c.counter2 = 0;
+            c.itmin = 1;
+            c.itmax = L.opts.inner_max;
+            c.eps = L.opts.inner_tol;
+            c.rhsnorm = rn;
+            *L.hctrl = c;
+            TT_HIP(hipMemcpyAsync(L.ctrl, L.hctrl, sizeof(Ctrl), hipMemcpyHostToDevice, L.s));
+            int it32 = 0;
+            double rn32 = 0;
+            rc = run_sweeps<float>(c32[0], c32[1], x32[0], x32[1], L.p.sorparam, &it32, &rn32);
+            if (rc) return rc;
+            total32 += it32;
+            for (int cc = 0; cc < 2; ++cc) {
+                k_add_corr<<<nblk(g_cs.ncell, 256), 256, 0, L.s>>>(g_cs, x64[cc], x32[cc]);
+                TT_LAUNCH();
+            }
+            rc = resid64(&rn, true);
+            if (rc) return rc;
+        }
+        s.sor_iter = total32;
+        s.fp32_sweeps = total32;
+        s.outer_iter = outer;
+        s.rnorm = rn;
+        // restore the FP64 control block's solve fields
+        L.hctrl->rhsnorm = rhs_norm;
+        L.hctrl->it = total32;
+        L.hctrl->done = (rn / rhs_norm <= L.p.epsmat) ? 1 : 2;
+        TT_HIP(hipMemcpyAsync(L.ctrl, L.hctrl, sizeof(Ctrl), hipMemcpyHostToDevice, L.s));
+    }
+    s.rhsnorm = L.hctrl->rhsnorm;
+    // unknowns back into the ghosted field, then the phi wrap
+    if (mode == 0) {
+        TT_HIP(hipMemsetAsync(target, 0, (size_t)g.n3 * sizeof(double), L.s));
+    }
+    k_store_x<<<nblk(2 * g_cs.ncell, 256), 256, 0, L.s>>>(g, g_cs, x64[0], x64[1], target);
+    TT_LAUNCH();
+    rc = planetshift_impl(target, mode == 1 ? 10 : 8);
+    if (rc) return rc;
+    if (st) *st = s;
+    if (L.hctrl->done == 2) {
+        fail(-1, mode == 1 ? "Max iteration reached: DIF SOR matrix solver" : "MAX COUPLED SOR ITERATION");
+        return -1;
+    }
+    return 0;
+}
+
+// scale the last delta_ER in place (warm start: delta_ER <- s * delta_ER)
+int sor_scale_guess(double sc) {
+    k_scale<<<nblk(L.g.n3, 256), 256, 0, L.s>>>(L.f[TT_DELTA_ER], sc, L.g.n3);
+    TT_LAUNCH();
+    return 0;
+}
+
+}  // namespace tt
+
+using namespace tt;
+
+extern "C" {
+
+int tt_solve_sor(int fill_matrix, tt_sor_stats* st) {
+    if (!L.ready) return fail(1, "tt_init has not been called");
+    int rc = sor_solve_impl(fill_matrix, st, 0);
+    int rc2 = sync_check();
+    return rc2 ? rc2 : rc;
+}
+
+int tt_sweep_timing(int enable) {
+    if (L.s) hipStreamSynchronize(L.s);
+    ev_drain();
+    L.timing = enable != 0;
+    L.n64 = L.n32 = 0;
+    L.ms64 = L.ms32 = 0.0;
+    return 0;
+}
+
+int tt_sweep_timing_get(int64_t* n64, double* ms64, int64_t* n32, double* ms32) {
+    if (L.s) hipStreamSynchronize(L.s);
+    ev_drain();
+    if (n64) *n64 = L.n64;
+    if (ms64) *ms64 = L.ms64;
+    if (n32) *n32 = L.n32;
+    if (ms32) *ms32 = L.ms32;
+    return 0;
+}
+
+}  // extern "C"

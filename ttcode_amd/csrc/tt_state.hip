@@ -1,0 +1,322 @@
+// tt_state.hip -- device state of libttgpu.so: allocation, the grid (GITTER),
+// host<->device transfer in the reference's wire format, errors.
+//
+// Replaces the device side of ppmod.f's allocate*var routines, GITTER
+// (ppinit.f:1028-1406), GASINIT (ppinit.f:1408-1417) and CALCMUSTAR
+// (ppsub.f:156-179).
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "tt_common.h"
+
+namespace tt {
+
+Lib L;
+
+int fail(int code, const std::string& msg) {
+    L.err = msg;
+    return code;
+}
+
+// ---------------------------------------------------------------- GITTER
+// Host restatement of GITTER for NCOSYS=2 and uniform grids, in the
+// reference's operation order (the metric is an input of every kernel, so it
+// is computed once on the host and uploaded).
+static void gitter_host(double* mb, int len, const tt_params& p) {
+    const int NX = p.nx, NY = p.ny, NZ = p.nz;
+    auto M = [&](int m, int i) -> double& { return mb[(size_t)m * len + i + 2]; };
+    double YMAX1 = p.ymax * 2.0 * PI, YMIN1 = p.ymin * 2.0 * PI;
+    double ZMAX1 = (p.zmax / 2.0) * PI, ZMIN1 = (p.zmin / 2.0) * PI;
+    auto axis = [&](int n, double lo, double hi, int A_, int B_, int DA, int DB) {
+        double del = (hi - lo) / (double)n;
+        for (int i = 1; i <= n + 2; ++i) M(DA, i) = del;
+        M(A_, 1) = lo + (0.0 * (hi - lo) / 1.0);
+        M(A_, 0) = M(A_, 1) - M(DA, 1);
+        for (int i = 2; i <= n + 2; ++i) M(A_, i) = M(A_, i - 1) + M(DA, i);
+        for (int i = 0; i <= n + 1; ++i) M(B_, i) = (M(A_, i) + M(A_, i + 1)) / 2.0;
+        double d3 = M(DA, n + 2) * 1.0;
+        double a3 = M(A_, n + 2) + d3;
+        M(B_, n + 2) = (M(A_, n + 2) + a3) / 2.0;
+        d3 = M(DA, 1) / 1.0;
+        a3 = M(A_, 0) - d3;
+        M(B_, -1) = (M(A_, 0) + a3) / 2.0;
+        for (int i = 0; i <= n + 2; ++i) M(DB, i) = M(B_, i) - M(B_, i - 1);
+    };
+    axis(NX, p.xmin, p.xmax, TT_M_XXA, TT_M_XXB, TT_M_DXA, TT_M_DXB);
+    axis(NY, YMIN1, YMAX1, TT_M_XYA, TT_M_XYB, TT_M_DYA, TT_M_DYB);
+    axis(NZ, ZMIN1, ZMAX1, TT_M_XZA, TT_M_XZB, TT_M_DZA, TT_M_DZB);
+    for (int i = 0; i <= NX + 2; ++i) { M(TT_M_DDX0, i) = 0.5; M(TT_M_DDX1, i) = 0.5; }
+    for (int i = 0; i <= NY + 2; ++i) { M(TT_M_DDY0, i) = 0.5; M(TT_M_DDY1, i) = 0.5; }
+    for (int i = 1; i <= NY + 2; ++i) { M(TT_M_DCY0, i) = 0.5; M(TT_M_DCY1, i) = 0.5; }
+    for (int i = 0; i <= NZ + 2; ++i) { M(TT_M_DDZ0, i) = 0.5; M(TT_M_DDZ1, i) = 0.5; }
+    for (int i = 1; i <= NX + 2; ++i) {
+        M(TT_M_DDX0KEP, i) = M(TT_M_DDX0, i) * std::pow(M(TT_M_XXB, i - 1), 1.5);
+        M(TT_M_DDX1KEP, i) = M(TT_M_DDX1, i) * std::pow(M(TT_M_XXB, i), 1.5);
+    }
+    double xxa1 = M(TT_M_XXA, 1);
+    if (xxa1 == 0.0) xxa1 = (double)1.e-20f;
+    for (int i = 0; i <= NX + 1; ++i) {
+        double a1 = M(TT_M_XXA, i + 1), a0 = M(TT_M_XXA, i);
+        M(TT_M_VOLXB, i) = (a1 * a1 * a1 - a0 * a0 * a0) / 3.0;
+    }
+    for (int i = 0; i <= NX + 2; ++i) {
+        double a = M(TT_M_XXA, i), b = M(TT_M_XXB, i), bm = M(TT_M_XXB, i - 1);
+        M(TT_M_SURXA, i) = a * a;
+        M(TT_M_CCXA, i) = a;
+        M(TT_M_VOLXA, i) = (b * b * b - bm * bm * bm) / 3.0;
+        M(TT_M_SURXB, i) = b * b;
+        M(TT_M_CCXB, i) = b;
+        M(TT_M_GEOXGA, i) = a;
+        M(TT_M_GEOXHA, i) = a;
+    }
+    M(TT_M_GEOXGA, 1) = xxa1;
+    M(TT_M_GEOXHA, 1) = xxa1;
+    for (int i = -1; i <= NX + 2; ++i) { M(TT_M_GEOXG, i) = M(TT_M_XXB, i); M(TT_M_GEOXH, i) = M(TT_M_XXB, i); }
+    for (int i = 0; i <= NY + 1; ++i) M(TT_M_VOLYB, i) = M(TT_M_XYA, i + 1) - M(TT_M_XYA, i);
+    for (int i = 0; i <= NY + 2; ++i) {
+        M(TT_M_SURYA, i) = 1.0;
+        M(TT_M_CCYA, i) = 0.0;
+        M(TT_M_VOLYA, i) = M(TT_M_XYB, i) - M(TT_M_XYB, i - 1);
+        M(TT_M_SURYB, i) = 1.0;
+        M(TT_M_CCYB, i) = 0.0;
+    }
+    for (int i = 0; i <= NZ + 1; ++i) M(TT_M_VOLZB, i) = std::sin(M(TT_M_XZA, i + 1)) - std::sin(M(TT_M_XZA, i));
+    for (int i = 0; i <= NZ + 2; ++i) {
+        M(TT_M_SURZA, i) = std::cos(M(TT_M_XZA, i));
+        M(TT_M_CCZA, i) = std::sin(M(TT_M_XZA, i)) / 2.0;
+        M(TT_M_VOLZA, i) = std::fabs(std::sin(M(TT_M_XZB, i)) - std::sin(M(TT_M_XZB, i - 1)));
+        M(TT_M_SURZB, i) = std::cos(M(TT_M_XZB, i));
+        M(TT_M_CCZB, i) = std::sin(M(TT_M_XZB, i)) / 2.0;
+        M(TT_M_GEOZGA, i) = std::cos(M(TT_M_XZA, i));
+    }
+    for (int i = -1; i <= NZ + 2; ++i) M(TT_M_GEOZG, i) = std::cos(M(TT_M_XZB, i));
+    if (M(TT_M_GEOZGA, NZ + 1) == 0.0) M(TT_M_GEOZGA, NZ + 1) = (double)1.e-10f;
+    if (M(TT_M_GEOZGA, 1) == 0.0) M(TT_M_GEOZGA, 1) = (double)1.e-10f;
+}
+
+// ------------------------------------------------------------ wire format
+__global__ void k_scatter(double* box, const double* wire, Geo g, int lo0, int lo1, int lo2, int ni, int nj,
+                          long long n) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    int i, j, k;
+    sub_ijk(t, ni, nj, lo0, lo1, lo2, i, j, k);
+    box[g.idx(i, j, k)] = wire[t];
+}
+
+__global__ void k_gather(const double* box, double* wire, Geo g, int lo0, int lo1, int lo2, int ni, int nj,
+                         long long n) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    int i, j, k;
+    sub_ijk(t, ni, nj, lo0, lo1, lo2, i, j, k);
+    wire[t] = box[g.idx(i, j, k)];
+}
+
+__global__ void k_fill(double* a, double v, long long n) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) a[t] = v;
+}
+
+static double* g_stage = nullptr;  // device staging buffer (one box)
+
+static int xfer(int f, void* host, bool to_host) {
+    if (!L.ready) return fail(1, "tt_init has not been called");
+    int32_t lo[3], hi[3], rank, isint;
+    if (tt_field_bounds(f, L.g.nx, L.g.ny, L.g.nz, lo, hi, &rank, &isint)) return fail(1, "unknown field id");
+    if (rank == 2) {
+        size_t n = (size_t)(hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1);
+        void* d = isint ? (void*)L.ub : (void*)L.mustar;
+        size_t bytes = n * (isint ? sizeof(int32_t) : sizeof(double));
+        if (to_host) TT_HIP(hipMemcpyAsync(host, d, bytes, hipMemcpyDeviceToHost, L.s));
+        else TT_HIP(hipMemcpyAsync(d, host, bytes, hipMemcpyHostToDevice, L.s));
+        TT_HIP(hipStreamSynchronize(L.s));
+        return 0;
+    }
+    double* a = L.f[f];
+    if (!a) return fail(1, "field not held on the device");
+    int ni = hi[0] - lo[0] + 1, nj = hi[1] - lo[1] + 1, nk = hi[2] - lo[2] + 1;
+    long long n = (long long)ni * nj * nk;
+    if (to_host) {
+        k_gather<<<nblk(n, 256), 256, 0, L.s>>>(a, g_stage, L.g, lo[0], lo[1], lo[2], ni, nj, n);
+        TT_LAUNCH();
+        TT_HIP(hipMemcpyAsync(host, g_stage, n * sizeof(double), hipMemcpyDeviceToHost, L.s));
+    } else {
+        TT_HIP(hipMemcpyAsync(g_stage, host, n * sizeof(double), hipMemcpyHostToDevice, L.s));
+        k_scatter<<<nblk(n, 256), 256, 0, L.s>>>(a, g_stage, L.g, lo[0], lo[1], lo[2], ni, nj, n);
+        TT_LAUNCH();
+    }
+    TT_HIP(hipStreamSynchronize(L.s));
+    return 0;
+}
+
+static void free_all() {
+    for (int f = 0; f < TT_FIELD_MAX; ++f) {
+        if (L.f[f]) hipFree(L.f[f]);
+        L.f[f] = nullptr;
+    }
+    double** tmp[] = {&L.ED, &L.VD, &L.GDN, &L.HDN, &L.FLMX, &L.FLMY, &L.FLMZ, &L.TOLD, &L.VH, &L.GH, &L.HH,
+                      &L.DFQ, &L.partial, &L.mustar, &L.met, &g_stage};
+    for (auto p : tmp) {
+        if (*p) hipFree(*p);
+        *p = nullptr;
+    }
+    if (L.ub) hipFree(L.ub);
+    L.ub = nullptr;
+    if (L.ctrl) hipFree(L.ctrl);
+    L.ctrl = nullptr;
+    if (L.hctrl) hipHostFree(L.hctrl);
+    L.hctrl = nullptr;
+    delete[] L.hmet;
+    L.hmet = nullptr;
+    if (L.s) hipStreamDestroy(L.s);
+    L.s = nullptr;
+    L.ready = false;
+}
+
+void sor_free();
+int sor_alloc();
+void cnn_free();
+
+}  // namespace tt
+
+using namespace tt;
+
+extern "C" {
+
+const char* tt_last_error(void) { return L.err.c_str(); }
+
+int tt_field_info(int field, int32_t lo[3], int32_t hi[3], int32_t* rank, int32_t* is_int) {
+    return tt_field_bounds(field, L.g.nx, L.g.ny, L.g.nz, lo, hi, rank, is_int);
+}
+
+int tt_metric_info(int metric, int32_t* lo, int32_t* hi) {
+    return tt_metric_bounds(metric, L.g.nx, L.g.ny, L.g.nz, lo, hi);
+}
+
+void tt_finalize(void) {
+    if (L.s) hipStreamSynchronize(L.s);
+    sor_free();
+    cnn_free();
+    free_all();
+}
+
+int tt_init(const tt_params* p) {
+    if (L.ready) tt_finalize();
+    L = Lib();
+    if (!p) return fail(1, "null params");
+    if (p->nx < 4 || p->ny < 4 || p->nz < 4) return fail(1, "nx, ny, nz must be >= 4");
+    if (p->nx % 2 || p->ny % 2) return fail(1, "nx and ny must be even (red-black colouring across the phi wrap)");
+    int ndev = 0;
+    TT_HIP(hipGetDeviceCount(&ndev));
+    if (ndev < 1) return fail(10, "no HIP device");
+    TT_HIP(hipSetDevice(0));
+    L.p = *p;
+    Geo& g = L.g;
+    g.nx = p->nx; g.ny = p->ny; g.nz = p->nz;
+    g.gx = g.nx + 4; g.gy = g.ny + 4; g.gz = g.nz + 4;
+    g.n3 = (long long)g.gx * g.gy * g.gz;
+    TT_HIP(hipStreamCreateWithFlags(&L.s, hipStreamNonBlocking));
+    const size_t bytes = (size_t)g.n3 * sizeof(double);
+    for (int f = 1; f < TT_FIELD_MAX; ++f) {
+        int32_t lo[3], hi[3], rank, isint;
+        if (tt_field_bounds(f, g.nx, g.ny, g.nz, lo, hi, &rank, &isint) || rank != 3) continue;
+        TT_HIP(hipMalloc(&L.f[f], bytes));
+        TT_HIP(hipMemsetAsync(L.f[f], 0, bytes, L.s));
+    }
+    double** tmp[] = {&L.ED, &L.VD, &L.GDN, &L.HDN, &L.FLMX, &L.FLMY, &L.FLMZ, &L.TOLD, &L.VH, &L.GH, &L.HH,
+                      &L.DFQ, &g_stage};
+    for (auto q : tmp) {
+        TT_HIP(hipMalloc(q, bytes));
+        TT_HIP(hipMemsetAsync(*q, 0, bytes, L.s));
+    }
+    L.npartial = 8192;
+    TT_HIP(hipMalloc(&L.partial, (size_t)L.npartial * 2 * sizeof(double)));
+    TT_HIP(hipMalloc(&L.ub, (size_t)g.gy * g.gz * sizeof(int32_t)));
+    TT_HIP(hipMemsetAsync(L.ub, 0, (size_t)g.gy * g.gz * sizeof(int32_t), L.s));
+    TT_HIP(hipMalloc(&L.mustar, (size_t)g.gy * g.gz * sizeof(double)));
+    TT_HIP(hipMalloc(&L.ctrl, sizeof(Ctrl)));
+    TT_HIP(hipMemsetAsync(L.ctrl, 0, sizeof(Ctrl), L.s));
+    TT_HIP(hipHostMalloc((void**)&L.hctrl, sizeof(Ctrl), hipHostMallocDefault));
+    int nmax = g.nx > g.ny ? g.nx : g.ny;
+    if (g.nz > nmax) nmax = g.nz;
+    L.metlen = nmax + 6;
+    L.hmet = new double[(size_t)TT_M_COUNT * L.metlen]();
+    gitter_host(L.hmet, L.metlen, *p);
+    TT_HIP(hipMalloc(&L.met, (size_t)TT_M_COUNT * L.metlen * sizeof(double)));
+    TT_HIP(hipMemcpyAsync(L.met, L.hmet, (size_t)TT_M_COUNT * L.metlen * sizeof(double), hipMemcpyHostToDevice, L.s));
+    // CALCMUSTAR (ppsub.f:156-179)
+    std::vector<double> mu((size_t)g.gy * g.gz);
+    for (int k = -1; k <= g.nz + 2; ++k)
+        for (int j = -1; j <= g.ny + 2; ++j) {
+            double v = 1.0;
+            if (p->f_incident == 2 || p->f_incident == 3 || p->f_incident == 8) {
+                double a = std::cos(L.hmet[(size_t)TT_M_XYB * L.metlen + j + 2]) *
+                           std::cos(L.hmet[(size_t)TT_M_XZB * L.metlen + k + 2]);
+                double lim = (double)1.e-7f;
+                v = a > lim ? a : lim;
+            }
+            mu[g.idx2(j, k)] = v;
+        }
+    TT_HIP(hipMemcpyAsync(L.mustar, mu.data(), mu.size() * sizeof(double), hipMemcpyHostToDevice, L.s));
+    // GASINIT: cv = Rgas/(mu (gamma-1)) (ppinit.f:1415)
+    double cv = RGAS / (p->mu_gas * (p->gamma - 1.0));
+    k_fill<<<nblk(g.n3, 256), 256, 0, L.s>>>(L.f[TT_CV], cv, g.n3);
+    TT_LAUNCH();
+    L.opts.precision = 0;
+    L.opts.guess = 0;
+    L.opts.batch = 16;
+    L.opts.inner_max = 4000;
+    L.opts.inner_tol = 1e-5;
+    L.opts.warm_scale = 0.9;
+    int rc = sor_alloc();
+    if (rc) return rc;
+    TT_HIP(hipStreamSynchronize(L.s));
+    L.ready = true;
+    return 0;
+}
+
+int tt_upload(int field, const void* host) { return xfer(field, const_cast<void*>(host), false); }
+int tt_download(int field, void* host) { return xfer(field, host, true); }
+
+int tt_upload_metric(int metric, const double* host) {
+    if (!L.ready) return fail(1, "tt_init has not been called");
+    int32_t lo, hi;
+    if (tt_metric_bounds(metric, L.g.nx, L.g.ny, L.g.nz, &lo, &hi)) return fail(1, "unknown metric");
+    for (int i = lo; i <= hi; ++i) L.hmet[(size_t)metric * L.metlen + i + 2] = host[i - lo];
+    TT_HIP(hipMemcpyAsync(L.met + (size_t)metric * L.metlen, L.hmet + (size_t)metric * L.metlen,
+                          L.metlen * sizeof(double), hipMemcpyHostToDevice, L.s));
+    TT_HIP(hipStreamSynchronize(L.s));
+    return 0;
+}
+
+int tt_download_metric(int metric, double* host) {
+    if (!L.ready) return fail(1, "tt_init has not been called");
+    int32_t lo, hi;
+    if (tt_metric_bounds(metric, L.g.nx, L.g.ny, L.g.nz, &lo, &hi)) return fail(1, "unknown metric");
+    for (int i = lo; i <= hi; ++i) host[i - lo] = L.hmet[(size_t)metric * L.metlen + i + 2];
+    return 0;
+}
+
+int tt_sync(void) {
+    if (!L.ready) return fail(1, "tt_init has not been called");
+    TT_HIP(hipStreamSynchronize(L.s));
+    return 0;
+}
+
+int tt_set_solver(const tt_solver_opts* o) {
+    if (!o) return fail(1, "null options");
+    if (o->precision < 0 || o->precision > 1 || o->guess < 0 || o->guess > 2 || o->batch < 1)
+        return fail(1, "invalid solver options");
+    L.opts = *o;
+    return 0;
+}
+
+int tt_get_solver(tt_solver_opts* o) {
+    if (!o) return fail(1, "null options");
+    *o = L.opts;
+    return 0;
+}
+
+}  // extern "C"

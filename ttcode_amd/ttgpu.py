@@ -1,0 +1,165 @@
+"""Host-side binding of libttgpu.so, the gfx950 implementation of ttcode's
+per-step hot path.
+
+``Device`` mirrors the reference's routines (ADVECT3D, FORGAS3D,
+COUPLED_ENERGY_EQS, SOLVE_WITH_SOR, BOUNDS, SCHRITT, DIFFUSION) on a
+device-resident copy of the module globals; arrays cross in the reference's
+wire format (see ttcode_amd.params).  There is no CPU fallback: constructing
+a Device without the built library or without a GPU raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from . import params as P
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libttgpu.so")
+
+_lib = None
+
+
+def load_library():
+    """Load the in-tree libttgpu.so (built by ttcode_amd.build)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: run `python -m ttcode_amd.build` first")
+    L = ctypes.CDLL(LIB_PATH)
+    L.tt_last_error.restype = ctypes.c_char_p
+    L.tt_init.argtypes = [ctypes.POINTER(P.TTParams)]
+    L.tt_upload.argtypes = [ctypes.c_int, ctypes.c_void_p]
+    L.tt_download.argtypes = [ctypes.c_int, ctypes.c_void_p]
+    L.tt_upload_metric.argtypes = [ctypes.c_int, ctypes.c_void_p]
+    L.tt_download_metric.argtypes = [ctypes.c_int, ctypes.c_void_p]
+    L.tt_advect3d.argtypes = [ctypes.c_double]
+    L.tt_forgas3d.argtypes = [ctypes.c_double]
+    L.tt_coupled_energy.argtypes = [ctypes.c_double, ctypes.c_int, ctypes.POINTER(P.EnergyStats)]
+    L.tt_step.argtypes = [ctypes.c_double, ctypes.c_int, ctypes.POINTER(P.StepStats)]
+    L.tt_solve_sor.argtypes = [ctypes.c_int, ctypes.POINTER(P.SorStats)]
+    L.tt_schritt.argtypes = [ctypes.POINTER(ctypes.c_double)]
+    L.tt_diffusion.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_int)]
+    L.tt_set_solver.argtypes = [ctypes.POINTER(P.SolverOpts)]
+    L.tt_get_solver.argtypes = [ctypes.POINTER(P.SolverOpts)]
+    L.tt_sweep_timing.argtypes = [ctypes.c_int]
+    L.tt_sweep_timing_get.argtypes = [ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double),
+                                      ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double)]
+    L.tt_cnn_num_weights.restype = ctypes.c_int64
+    L.tt_cnn_num_biases.restype = ctypes.c_int64
+    _lib = L
+    return L
+
+
+class TTError(RuntimeError):
+    pass
+
+
+class Device:
+    """One device-resident ttcode state (one rank's whole subdomain)."""
+
+    def __init__(self, p: P.TTParams):
+        self.L = load_library()
+        self.p = p
+        self._check(self.L.tt_init(ctypes.byref(p)), "tt_init")
+        self.last_rc = 0
+
+    def close(self):
+        self.L.tt_finalize()
+
+    def _check(self, rc, what):
+        self.last_rc = rc
+        if rc > 0:
+            raise TTError(f"{what}: {self.L.tt_last_error().decode()} (rc={rc})")
+        return rc
+
+    # ---- data movement
+    def upload(self, f, a):
+        if isinstance(f, str):
+            f = P.FIELD_IDS[f]
+        shape, _, dt = P.wire_shape(f, self.p.nx, self.p.ny, self.p.nz)
+        a = np.ascontiguousarray(a, dtype=dt)
+        if a.shape != shape:
+            raise ValueError(f"field {f}: shape {a.shape} != {shape}")
+        self._check(self.L.tt_upload(f, a.ctypes.data), "tt_upload")
+
+    def download(self, f):
+        if isinstance(f, str):
+            f = P.FIELD_IDS[f]
+        out = P.empty_field(f, self.p.nx, self.p.ny, self.p.nz)
+        self._check(self.L.tt_download(f, out.ctypes.data), "tt_download")
+        return out
+
+    def metric(self, m):
+        lo, hi = ctypes.c_int32(), ctypes.c_int32()
+        self.L.tt_metric_info(m, ctypes.byref(lo), ctypes.byref(hi))
+        out = np.zeros(hi.value - lo.value + 1)
+        self._check(self.L.tt_download_metric(m, out.ctypes.data), "tt_download_metric")
+        return out
+
+    def load_state(self, st: dict):
+        for name, a in st.items():
+            if name in P.FIELD_IDS:
+                self.upload(P.FIELD_IDS[name], a)
+
+    # ---- solver options
+    def set_solver(self, precision=0, guess=0, batch=16, inner_max=4000, inner_tol=1e-5, warm_scale=0.9):
+        o = P.SolverOpts(precision, guess, batch, inner_max, inner_tol, warm_scale)
+        self._check(self.L.tt_set_solver(ctypes.byref(o)), "tt_set_solver")
+
+    # ---- the reference's routines
+    def bounds(self):
+        return self._check(self.L.tt_bounds(), "BOUNDS")
+
+    def pressg(self):
+        return self._check(self.L.tt_pressg(), "PRESSG")
+
+    def qdens(self):
+        return self._check(self.L.tt_qdens(), "QDENS")
+
+    def advect3d(self, delt):
+        return self._check(self.L.tt_advect3d(delt), "ADVECT3D")
+
+    def forgas3d(self, delt):
+        return self._check(self.L.tt_forgas3d(delt), "FORGAS3D")
+
+    def coupled_energy(self, delt, iii=1):
+        st = P.EnergyStats()
+        rc = self._check(self.L.tt_coupled_energy(delt, iii, ctypes.byref(st)), "COUPLED_ENERGY_EQS")
+        return st, rc
+
+    def solve_sor(self, fill=True):
+        st = P.SorStats()
+        rc = self._check(self.L.tt_solve_sor(1 if fill else 0, ctypes.byref(st)), "SOLVE_WITH_SOR")
+        return st, rc
+
+    def schritt(self):
+        d = ctypes.c_double()
+        self._check(self.L.tt_schritt(ctypes.byref(d)), "SCHRITT")
+        return d.value
+
+    def diffusion(self, delt, xnue, pr_num):
+        it = ctypes.c_int()
+        rc = self._check(self.L.tt_diffusion(delt, xnue, pr_num, ctypes.byref(it)), "DIFFUSION")
+        return it.value, rc
+
+    def step(self, delt, iii=1):
+        st = P.StepStats()
+        rc = self._check(self.L.tt_step(delt, iii, ctypes.byref(st)), "step")
+        return st, rc
+
+    def sync(self):
+        return self._check(self.L.tt_sync(), "tt_sync")
+
+    # ---- per-launch timing of the SOR half sweeps
+    def sweep_timing(self, enable=True):
+        self.L.tt_sweep_timing(1 if enable else 0)
+
+    def sweep_timing_get(self):
+        n64, n32 = ctypes.c_int64(), ctypes.c_int64()
+        m64, m32 = ctypes.c_double(), ctypes.c_double()
+        self.L.tt_sweep_timing_get(ctypes.byref(n64), ctypes.byref(m64), ctypes.byref(n32), ctypes.byref(m32))
+        return n64.value, m64.value, n32.value, m32.value
