@@ -51,7 +51,30 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_fortran(verbose)
     return LIB
+
+
+FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
+FDIR = os.path.join(HERE, "fortran")
+DRIVER = os.path.join(FDIR, "tt_driver")
+
+
+def build_fortran(verbose: bool = False) -> str | None:
+    """The ISO_C_BINDING layer and the Fortran step driver (needs flang)."""
+    if not os.path.exists(FLANG):
+        return None
+    mod = os.path.join(OBJ, "fmod")
+    os.makedirs(mod, exist_ok=True)
+    cmd = [FLANG, "-O2", "-module-dir", mod, "-I", mod, os.path.join(FDIR, "ttgpu_mod.f90"),
+           os.path.join(FDIR, "tt_driver.f90"), "-o", DRIVER, "-L", HERE, "-lttgpu",
+           "-Wl,-rpath," + HERE]
+    r = subprocess.run(cmd, capture_output=True, text=True, cwd=OBJ)
+    if r.returncode != 0:
+        raise RuntimeError(f"flang failed:\n{r.stderr}")
+    if verbose:
+        print("built", DRIVER)
+    return DRIVER
 
 
 if __name__ == "__main__":
