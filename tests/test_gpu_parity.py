@@ -32,7 +32,16 @@ def _dev(ttgpu, g):
     return p, d
 
 
-def _cmp(g, p, d, names=None, rel=REL, skip=()):
+# fields that are differences of nearly equal numbers (fluxes, Newton
+# residuals, corrections) are compared against the array's own scale
+SCALED = {"flux_x", "flux_y", "flux_z", "FQ", "FR", "RHS_ER", "delta_ER", "Jacob_eta"}
+
+
+def scaled_err(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def _cmp(g, p, d, names=None, rel=REL, skip=(), scaled_tol=None):
     worst = {}
     for k, ref in C.outputs_of(g).items():
         if (names and k not in names) or k in skip:
@@ -45,9 +54,15 @@ def _cmp(g, p, d, names=None, rel=REL, skip=()):
             continue
         fin = np.isfinite(want)
         assert np.array_equal(np.isfinite(mine), fin), f"{k}: non-finite pattern differs"
-        e = C.rel_err(mine[fin], want[fin])
+        if k in SCALED:
+            e = scaled_err(mine[fin], want[fin])
+            tol = scaled_tol if scaled_tol is not None else rel
+        else:
+            e = C.rel_err(mine[fin], want[fin])
+            tol = rel
         worst[k] = e
-        assert e <= rel, f"{k}: max rel err {e:.3e} > {rel:.1e}"
+        print(f"  {g.fid.name if hasattr(g, 'fid') else ''} {k:12s} err {e:.3e} (tol {tol:.0e})")
+        assert e <= tol, f"{k}: max err {e:.3e} > {tol:.1e}"
     return worst
 
 
@@ -110,7 +125,7 @@ def test_coupled_energy(ttgpu, name):
     assert st.energy_iter == int(g["scal_energy_iter"])
     assert st.sor_iter_last == int(g["scal_SOR_ITER"])
     # delta_ER is the whole step's change; compare it where it is not tiny
-    _cmp(g, p, d, rel=1e-9, skip=("delta_ER", "flux_y", "flux_z", "FR", "FQ", "RHS_ER"))
+    _cmp(g, p, d, rel=1e-12, scaled_tol=1e-10)
     d.close()
 
 
@@ -121,7 +136,7 @@ def test_two_steps(ttgpu):
         st, _ = d.step(float(g["delt"]), int(g["iii"]) + s)
     assert st.energy.energy_iter == int(g["scal_energy_iter"])
     assert st.energy.sor_iter_last == int(g["scal_SOR_ITER"])
-    _cmp(g, p, d, rel=1e-9, skip=("delta_ER",))
+    _cmp(g, p, d, rel=1e-12, scaled_tol=1e-10)
     d.close()
 
 
