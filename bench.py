@@ -103,11 +103,23 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first copy 1 GiB device to device (reference for rocprofv3 byte counters)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
               "(the multi-GPU decomposition is a design, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+
+    if a.calibrate_copy:
+        import torch
+        x = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(f"# calibration copy: 3 x {x.numel() * 8} bytes", file=sys.stderr, flush=True)
+        del x, y
 
     from ttcode_amd import ttgpu
     nx, ny, nz = a.nr, a.nphi, a.ntheta

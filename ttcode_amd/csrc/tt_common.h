@@ -116,18 +116,22 @@ inline unsigned nblk(long long n, int bs) { return (unsigned)((n + bs - 1) / bs)
 
 // Fortran order helpers for the full box
 __device__ inline void box_ijk(const Geo& g, long long t, int& i, int& j, int& k) {
-    i = (int)(t % g.gx) - 1;
-    long long r = t / g.gx;
-    j = (int)(r % g.gy) - 1;
-    k = (int)(r / g.gy) - 1;
+    const int u = (int)t;  // boxes stay below 2^31 cells
+    const int r = u / g.gx;
+    i = u - r * g.gx - 1;
+    k = r / g.gy;
+    j = r - k * g.gy - 1;
+    k -= 1;
 }
 
 // Sub-box decomposition: t over ni*nj*nk with origin (i0,j0,k0)
 __device__ inline void sub_ijk(long long t, int ni, int nj, int i0, int j0, int k0, int& i, int& j, int& k) {
-    i = (int)(t % ni) + i0;
-    long long r = t / ni;
-    j = (int)(r % nj) + j0;
-    k = (int)(r / nj) + k0;
+    const int u = (int)t;
+    const int r = u / ni;
+    i = u - r * ni + i0;
+    const int q = r / nj;
+    j = r - q * nj + j0;
+    k = q + k0;
 }
 
 // Wavefront + block reductions (64-wide wavefronts)

@@ -92,14 +92,14 @@ int sor_alloc() {
         TT_HIP(hipMemsetAsync(x32[c], 0, padded_len() * sizeof(float), L.s));
     }
     TT_HIP(hipMalloc(&r64, 2 * g_cs.ncell * sizeof(double)));
-    // blocks: enough to fill 256 CUs several times, a multiple of 8 (XCDs)
-    long long want = (g_cs.ncell + 255) / 256;
-    int nb = (int)(want < 2048 ? want : 2048);
+    // sweep blocks: about 4 rows per wavefront, a multiple of 8 (XCDs)
+    int nrows = g_cs.ny * g_cs.nz;
+    int nb = (nrows + 15) / 16;
+    if (nb > 2048) nb = 2048;
     nb = ((nb + 7) / 8) * 8;
-    if (nb > L.npartial) nb = L.npartial;
+    if (nb < 8) nb = 8;
     g_nb = nb;
-    g_chunk = (int)((g_cs.ncell + nb - 1) / nb);
-    g_chunk = ((g_chunk + 255) / 256) * 256;
+    g_chunk = (nrows + nb - 1) / nb;  // rows per block
     return 0;
 }
 
@@ -138,42 +138,50 @@ __device__ void finish_iteration(Ctrl* ctrl, const double* part, int nparts, dou
     }
 }
 
+// One half sweep.  Rows (j,k) of the colour are split into contiguous
+// ranges, one per logical block; the four wavefronts of a block take every
+// fourth row of its range and the 64 lanes of a wavefront walk h.  Index
+// arithmetic is 32-bit and per row.
 template <typename T>
 __global__ __launch_bounds__(256) void k_sweep(int c, const T* __restrict__ A, const T* __restrict__ B,
                                                const T* __restrict__ C, const T* __restrict__ D,
                                                const T* __restrict__ E, const T* __restrict__ F,
                                                const T* __restrict__ G, const T* __restrict__ R,
                                                T* __restrict__ xc, const T* __restrict__ xo, CS cs, T omega,
-                                               double* part, int nb, int chunk, Ctrl* ctrl, int is_last) {
+                                               double* part, int nb, int rows_per_block, Ctrl* ctrl, int is_last) {
     __shared__ double sh[4];
     __shared__ int last;
     if (ctrl->done) return;
     const int lb = xcd_block(nb);
-    const long long start = (long long)lb * chunk;
-    long long end = start + chunk;
-    if (end > cs.ncell) end = cs.ncell;
-    double acc = 0.0;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
     const int hx = cs.hx, ny = cs.ny, px = cs.px;
-    for (long long t = start + threadIdx.x; t < end; t += 256) {
-        const int h = (int)(t % hx);
-        const long long r = t / hx;
-        const int jj = (int)(r % ny);
-        const int kk = (int)(r / ny);
+    const int nrows = ny * cs.nz;
+    const int r0 = lb * rows_per_block;
+    const int r1 = min(r0 + rows_per_block, nrows);
+    double acc = 0.0;
+    for (int r = r0 + ty; r < r1; r += 4) {
+        const int kk = r / ny;
+        const int jj = r - kk * ny;
         const int s = (jj + kk + c + 1) & 1;  // (j+k+c+1)&1 with j=jj+1, k=kk+1
-        const long long row = (long long)px * (jj + (long long)ny * (kk + 1));
         const int jm = jj == 0 ? ny - 1 : jj - 1;
         const int jp = jj == ny - 1 ? 0 : jj + 1;
-        const T xs = xc[row + h + 1];
-        const T xm = xo[row + h + s];
-        const T xp = xo[row + h + s + 1];
-        const T xjm = xo[(long long)px * (jm + (long long)ny * (kk + 1)) + h + 1];
-        const T xjp = xo[(long long)px * (jp + (long long)ny * (kk + 1)) + h + 1];
-        const T xkm = xo[(long long)px * (jj + (long long)ny * kk) + h + 1];
-        const T xkp = xo[(long long)px * (jj + (long long)ny * (kk + 2)) + h + 1];
-        const T a = A[t];
-        const T resid = a * xs + B[t] * xm + C[t] * xp + D[t] * xjm + E[t] * xjp + F[t] * xkm + G[t] * xkp - R[t];
-        acc += fabs((double)resid);
-        xc[row + h + 1] = xs - omega * resid / a;
+        const int row = px * (jj + ny * (kk + 1));
+        const T* __restrict__ xr = xo + row + s;
+        const T* __restrict__ xrjm = xo + px * (jm + ny * (kk + 1)) + 1;
+        const T* __restrict__ xrjp = xo + px * (jp + ny * (kk + 1)) + 1;
+        const T* __restrict__ xrkm = xo + px * (jj + ny * kk) + 1;
+        const T* __restrict__ xrkp = xo + px * (jj + ny * (kk + 2)) + 1;
+        T* __restrict__ xw = xc + row + 1;
+        const int e0 = hx * r;
+        for (int h = tx; h < hx; h += 64) {
+            const int e = e0 + h;
+            const T xs = xw[h];
+            const T a = A[e];
+            const T resid = a * xs + B[e] * xr[h] + C[e] * xr[h + 1] + D[e] * xrjm[h] + E[e] * xrjp[h] +
+                            F[e] * xrkm[h] + G[e] * xrkp[h] - R[e];
+            acc += fabs((double)resid);
+            xw[h] = xs - omega * resid / a;
+        }
     }
     double bs = block_sum256(acc, sh);
     if (threadIdx.x == 0) {
@@ -241,10 +249,11 @@ __global__ __launch_bounds__(256) void k_pack(Geo g, CS cs, const double* __rest
     double acc = 0.0;
     for (long long t = start + threadIdx.x; t < end; t += 256) {
         int c = t >= cs.ncell;
-        long long e = t - c * cs.ncell;
-        int h = (int)(e % cs.hx);
-        long long r = e / cs.hx;
-        int j = (int)(r % cs.ny) + 1, k = (int)(r / cs.ny) + 1;
+        int e = (int)(t - c * cs.ncell);
+        int r = e / cs.hx;
+        int h = e - r * cs.hx;
+        int kq = r / cs.ny;
+        int j = r - kq * cs.ny + 1, k = kq + 1;
         int s = (j + k + c + 1) & 1;
         int i = 2 * h + 1 + s;
         long long n = g.idx(i, j, k);
@@ -283,9 +292,11 @@ __global__ void k_load_x(Geo g, CS cs, int c, const double* __restrict__ fld, do
     long long n = (long long)cs.px * cs.ny * (cs.nz + 2);
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
-    int hp = (int)(t % cs.px);
-    long long r = t / cs.px;
-    int j = (int)(r % cs.ny) + 1, k = (int)(r / cs.ny);
+    int tt = (int)t;
+    int r = tt / cs.px;
+    int hp = tt - r * cs.px;
+    int kq = r / cs.ny;
+    int j = r - kq * cs.ny + 1, k = kq;
     int h = hp - 1;
     int s = (j + k + c + 1) & 1;
     int i = 2 * h + 1 + s;
@@ -302,10 +313,11 @@ __global__ void k_store_x(Geo g, CS cs, const double* __restrict__ x0, const dou
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= 2 * cs.ncell) return;
     int c = t >= cs.ncell;
-    long long e = t - c * cs.ncell;
-    int h = (int)(e % cs.hx);
-    long long r = e / cs.hx;
-    int j = (int)(r % cs.ny) + 1, k = (int)(r / cs.ny) + 1;
+    int e = (int)(t - c * cs.ncell);
+    int r = e / cs.hx;
+    int h = e - r * cs.hx;
+    int kq = r / cs.ny;
+    int j = r - kq * cs.ny + 1, k = kq + 1;
     int s = (j + k + c + 1) & 1;
     int i = 2 * h + 1 + s;
     const double* x = c ? x1 : x0;
@@ -349,10 +361,11 @@ __global__ __launch_bounds__(256) void k_resid64(CS cs, const double* const* __r
     const int hx = cs.hx, ny = cs.ny, px = cs.px;
     for (long long t = start + threadIdx.x; t < end; t += 256) {
         int c = t >= cs.ncell;
-        long long e = t - c * cs.ncell;
-        int h = (int)(e % hx);
-        long long rr = e / hx;
-        int jj = (int)(rr % ny), kk = (int)(rr / ny);
+        int e = (int)(t - c * cs.ncell);
+        int rr = e / hx;
+        int h = e - rr * hx;
+        int kk = rr / ny;
+        int jj = rr - kk * ny;
         int s = (jj + kk + c + 1) & 1;
         const double* xc = c ? x1 : x0;
         const double* xo = c ? x0 : x1;
@@ -398,9 +411,11 @@ __global__ void k_to_f32(const double* __restrict__ a, float* b, long long n) {
 __global__ void k_add_corr(CS cs, double* x, const float* __restrict__ c) {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= cs.ncell) return;
-    int h = (int)(t % cs.hx);
-    long long r = t / cs.hx;
-    int jj = (int)(r % cs.ny), kk = (int)(r / cs.ny);
+    int tt = (int)t;
+    int r = tt / cs.hx;
+    int h = tt - r * cs.hx;
+    int kk = r / cs.ny;
+    int jj = r - kk * cs.ny;
     long long p = (long long)cs.px * (jj + (long long)cs.ny * (kk + 1)) + h + 1;
     x[p] = x[p] + (double)c[p];
 }
