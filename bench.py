@@ -49,7 +49,7 @@ def read_traffic():
     return None
 
 
-def run_mode(dev, st0, dt, steps, warmup, precision, guess):
+def run_mode(dev, st0, dt, steps, warmup, precision, guess, events=True):
     dev.load_state(st0)
     dev.set_solver(precision=precision, guess=guess, batch=16, inner_tol=1e-5, inner_max=20000)
     iii = 1
@@ -57,7 +57,7 @@ def run_mode(dev, st0, dt, steps, warmup, precision, guess):
         dev.step(dt, iii)
         iii += 1
     dev.sync()
-    dev.sweep_timing(True)
+    dev.sweep_timing(events)
     newton = sor = f32 = 0
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -102,6 +102,8 @@ def main():
     ap.add_argument("--nphi", type=int, default=256)
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-sweep-events", action="store_true",
+                    help="time the steps without per-launch HIP events (no roofline figure)")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy 1 GiB device to device (reference for rocprofv3 byte counters)")
@@ -132,18 +134,18 @@ def main():
     dt = dev.schritt()
     print(f"# workload {nx}x{ny}x{nz} dt={dt:.4g}s", file=sys.stderr, flush=True)
 
-    plain = run_mode(dev, st0, dt, a.steps, a.warmup, 0, 0)
+    plain = run_mode(dev, st0, dt, a.steps, a.warmup, 0, 0, not a.no_sweep_events)
     print(f"# fp64: {plain}", file=sys.stderr, flush=True)
     mixed = None
     if not a.no_mixed:
-        mixed = run_mode(dev, st0, dt, a.steps, a.warmup, 1, 1)
+        mixed = run_mode(dev, st0, dt, a.steps, a.warmup, 1, 1, not a.no_sweep_events)
         print(f"# mixed+warm: {mixed}", file=sys.stderr, flush=True)
     dev.close()
 
     ncell = nx * ny * nz
     launch_ms = plain["sweep64_ms"] / max(plain["sweep64_launches"], 1)
     alg_bytes = 44.0 * ncell  # 88 B/cell per red+black iteration, half per launch
-    achieved = alg_bytes / (launch_ms * 1e-3) / 1e9
+    achieved = alg_bytes / (launch_ms * 1e-3) / 1e9 if launch_ms > 0 else 0.0
     traffic = read_traffic()
     out = {
         "metric": METRIC,

@@ -11,19 +11,23 @@
 //
 // Colour-split layout.  For colour c and row (j,k) the cells of that colour
 // are i = 2h + 1 + s, h = 0..hx-1 (hx = nx/2), s = (j+k+c+1) & 1.  The seven
-// coefficients and the RHS of a colour are dense arrays [k][j][h] (one
-// coalesced stream each); the unknowns of a colour are padded by one slot at
-// h = -1 and h = hx and one row at k = 0 and k = nz+1, which hold the fixed
-// r and theta ghost values (zero for delta_ER).  The r neighbours of a cell
-// of colour c are slots h+s-1 and h+s of the other colour's row, its phi and
-// theta neighbours slot h of the neighbouring rows.
+// coefficients and the RHS of a colour are stored row-blocked: row r = (j-1)
+// + ny (k-1) holds the eight vectors A..G, RHS of length hx one after the
+// other, so a wavefront reads one contiguous 8*hx block per row.  The unknowns
+// of a colour are padded by one slot at h = -1 and h = hx and one row at k = 0
+// and k = nz+1, which hold the fixed r and theta ghost values (zero for
+// delta_ER).  The r neighbours of a cell of colour c are slots h+s-1 and h+s
+// of the other colour's row, its phi and theta neighbours slot h of the
+// neighbouring rows.
 //
-// Convergence.  Each half sweep writes one partial |resid| sum per block; the
-// last block of the black sweep sums them in a fixed order, advances the
-// iteration counter and sets ctrl->done, which every later sweep reads first
-// and returns on.  The host queues `batch` iterations between reads of the
-// control block, so the iteration at which the solve stops is exactly the
-// first one that meets the tolerance.
+// Convergence.  Each half sweep writes one partial |resid| sum per block; a
+// one-block finish kernel after the black sweep sums them in a fixed order,
+// advances the iteration counter and sets ctrl->done, which every later
+// sweep reads first and returns on.  The kernel boundary orders the partial
+// sums before the finish kernel reads them (a per-block device-scope fence
+// would write the XCD's L2 back once per block).  The host queues `batch`
+// iterations between reads of the control block, so the iteration at which
+// the solve stops is exactly the first one that meets the tolerance.
 #include <cmath>
 #include <vector>
 
@@ -37,15 +41,14 @@ struct CS {
 };
 
 static CS g_cs;
-// FP64 coefficients + RHS per colour: [c][0..7] = A,B,C,D,E,F,G,RHS
-static double* c64[2][8];
-static float* c32[2][8];
+// Coefficients + RHS per colour, row-blocked: element (q, r, h) of the
+// vectors A,B,C,D,E,F,G,RHS (q = 0..7) at (8 r + q) hx + h.
+static double* c64[2];
+static float* c32[2];
 static double* x64[2];  // padded unknowns
 static float* x32[2];
-static double* r64 = nullptr;  // scratch for the FP64 residual (both colours)
 static int g_nb = 0;           // sweep blocks
 static int g_chunk = 0;
-static double** g_dtab = nullptr;  // device table of the 16 FP64 coefficient arrays
 static std::vector<hipEvent_t> ev_pool, ev_pend;
 static std::vector<int> ev_kind;
 
@@ -53,21 +56,15 @@ static long long padded_len() { return (long long)g_cs.px * g_cs.ny * (g_cs.nz +
 
 void sor_free() {
     for (int c = 0; c < 2; ++c) {
-        for (int q = 0; q < 8; ++q) {
-            if (c64[c][q]) hipFree(c64[c][q]);
-            if (c32[c][q]) hipFree(c32[c][q]);
-            c64[c][q] = nullptr;
-            c32[c][q] = nullptr;
-        }
+        if (c64[c]) hipFree(c64[c]);
+        if (c32[c]) hipFree(c32[c]);
+        c64[c] = nullptr;
+        c32[c] = nullptr;
         if (x64[c]) hipFree(x64[c]);
         if (x32[c]) hipFree(x32[c]);
         x64[c] = nullptr;
         x32[c] = nullptr;
     }
-    if (r64) hipFree(r64);
-    r64 = nullptr;
-    if (g_dtab) hipFree(g_dtab);
-    g_dtab = nullptr;
     for (auto e : ev_pool) hipEventDestroy(e);
     for (auto e : ev_pend) hipEventDestroy(e);
     ev_pool.clear();
@@ -82,16 +79,13 @@ int sor_alloc() {
     g_cs.px = g_cs.hx + 2;
     g_cs.ncell = (long long)g_cs.hx * g_cs.ny * g_cs.nz;
     for (int c = 0; c < 2; ++c) {
-        for (int q = 0; q < 8; ++q) {
-            TT_HIP(hipMalloc(&c64[c][q], g_cs.ncell * sizeof(double)));
-            TT_HIP(hipMalloc(&c32[c][q], g_cs.ncell * sizeof(float)));
-        }
+        TT_HIP(hipMalloc(&c64[c], 8 * g_cs.ncell * sizeof(double)));
+        TT_HIP(hipMalloc(&c32[c], 8 * g_cs.ncell * sizeof(float)));
         TT_HIP(hipMalloc(&x64[c], padded_len() * sizeof(double)));
         TT_HIP(hipMalloc(&x32[c], padded_len() * sizeof(float)));
         TT_HIP(hipMemsetAsync(x64[c], 0, padded_len() * sizeof(double), L.s));
         TT_HIP(hipMemsetAsync(x32[c], 0, padded_len() * sizeof(float), L.s));
     }
-    TT_HIP(hipMalloc(&r64, 2 * g_cs.ncell * sizeof(double)));
     // sweep blocks: about 4 rows per wavefront, a multiple of 8 (XCDs)
     int nrows = g_cs.ny * g_cs.nz;
     int nb = (nrows + 15) / 16;
@@ -121,9 +115,11 @@ __device__ inline double block_sum256(double v, double* sh) {
     return s;
 }
 
-// Finish an iteration: called by the last block of a kernel that has written
-// nparts partials.  Sums them in a fixed order and updates the control block.
-__device__ void finish_iteration(Ctrl* ctrl, const double* part, int nparts, double* sh) {
+// Finish an iteration (one block of 256 threads, after both half sweeps):
+// sums the 2*nb partials in a fixed order and updates the control block.
+__global__ __launch_bounds__(256) void k_finish(Ctrl* ctrl, const double* __restrict__ part, int nparts) {
+    __shared__ double sh[4];
+    if (ctrl->done) return;
     double s = 0.0;
     for (int q = threadIdx.x; q < nparts; q += 256) s += part[q];
     s = block_sum256(s, sh);
@@ -134,7 +130,6 @@ __device__ void finish_iteration(Ctrl* ctrl, const double* part, int nparts, dou
         double ratio = s / ctrl->rhsnorm;
         if (it == ctrl->itmax) ctrl->done = 2;
         else if (!(ratio > ctrl->eps || it < ctrl->itmin)) ctrl->done = 1;
-        ctrl->counter = 0;
     }
 }
 
@@ -143,14 +138,10 @@ __device__ void finish_iteration(Ctrl* ctrl, const double* part, int nparts, dou
 // fourth row of its range and the 64 lanes of a wavefront walk h.  Index
 // arithmetic is 32-bit and per row.
 template <typename T>
-__global__ __launch_bounds__(256) void k_sweep(int c, const T* __restrict__ A, const T* __restrict__ B,
-                                               const T* __restrict__ C, const T* __restrict__ D,
-                                               const T* __restrict__ E, const T* __restrict__ F,
-                                               const T* __restrict__ G, const T* __restrict__ R,
+__global__ __launch_bounds__(256) void k_sweep(int c, const T* __restrict__ coef,
                                                T* __restrict__ xc, const T* __restrict__ xo, CS cs, T omega,
-                                               double* part, int nb, int rows_per_block, Ctrl* ctrl, int is_last) {
+                                               double* part, int nb, int rows_per_block, const Ctrl* ctrl) {
     __shared__ double sh[4];
-    __shared__ int last;
     if (ctrl->done) return;
     const int lb = xcd_block(nb);
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
@@ -172,32 +163,19 @@ __global__ __launch_bounds__(256) void k_sweep(int c, const T* __restrict__ A, c
         const T* __restrict__ xrkm = xo + px * (jj + ny * kk) + 1;
         const T* __restrict__ xrkp = xo + px * (jj + ny * (kk + 2)) + 1;
         T* __restrict__ xw = xc + row + 1;
-        const int e0 = hx * r;
+        const T* __restrict__ cr = coef + (long long)r * 8 * hx;
         for (int h = tx; h < hx; h += 64) {
-            const int e = e0 + h;
             const T xs = xw[h];
-            const T a = A[e];
-            const T resid = a * xs + B[e] * xr[h] + C[e] * xr[h + 1] + D[e] * xrjm[h] + E[e] * xrjp[h] +
-                            F[e] * xrkm[h] + G[e] * xrkp[h] - R[e];
+            const T a = cr[h];
+            const T resid = a * xs + cr[hx + h] * xr[h] + cr[2 * hx + h] * xr[h + 1] + cr[3 * hx + h] * xrjm[h] +
+                            cr[4 * hx + h] * xrjp[h] + cr[5 * hx + h] * xrkm[h] + cr[6 * hx + h] * xrkp[h] -
+                            cr[7 * hx + h];
             acc += fabs((double)resid);
             xw[h] = xs - omega * resid / a;
         }
     }
     double bs = block_sum256(acc, sh);
-    if (threadIdx.x == 0) {
-        part[c * nb + lb] = bs;
-        last = 0;
-        if (is_last) {
-            __threadfence();
-            unsigned prev = atomicAdd(&ctrl->counter, 1u);
-            last = (prev == (unsigned)(nb - 1));
-        }
-    }
-    __syncthreads();
-    if (last) {
-        __threadfence();
-        finish_iteration(ctrl, part, 2 * nb, sh);
-    }
+    if (threadIdx.x == 0) part[c * nb + lb] = bs;
 }
 
 // ---------------------------------------------------------- matrix fill
@@ -229,17 +207,14 @@ __global__ void k_fill_matrix(Geo g, const int32_t* __restrict__ ub, const doubl
     if (ident) RHS[n] = 0.0;
 }
 
-// ghosted SOR_A..G, RHS -> colour-split FP64; per-block sum |RHS|; the last
-// block sets ctrl->rhsnorm.  Cells are visited colour by colour.
+// ghosted SOR_A..G, RHS -> colour-split row-blocked FP64; per-block sum
+// |RHS|; the last block sets ctrl->rhsnorm.  Cells are visited colour by colour.
 __global__ __launch_bounds__(256) void k_pack(Geo g, CS cs, const double* __restrict__ SA,
                                               const double* __restrict__ SB, const double* __restrict__ SC,
                                               const double* __restrict__ SD, const double* __restrict__ SE,
                                               const double* __restrict__ SF, const double* __restrict__ SG,
-                                              const double* __restrict__ RHS, double* A0, double* B0, double* C0,
-                                              double* D0, double* E0, double* F0, double* G0, double* R0,
-                                              double* A1, double* B1, double* C1, double* D1, double* E1,
-                                              double* F1, double* G1, double* R1, double* part, int nb, int chunk,
-                                              Ctrl* ctrl) {
+                                              const double* __restrict__ RHS, double* C0, double* C1, double* part,
+                                              int nb, int chunk, Ctrl* ctrl) {
     __shared__ double sh[4];
     __shared__ int last;
     const int lb = blockIdx.x;
@@ -259,13 +234,10 @@ __global__ __launch_bounds__(256) void k_pack(Geo g, CS cs, const double* __rest
         long long n = g.idx(i, j, k);
         double rhs = RHS[n];
         acc += fabs(rhs);
-        if (c == 0) {
-            A0[e] = SA[n]; B0[e] = SB[n]; C0[e] = SC[n]; D0[e] = SD[n];
-            E0[e] = SE[n]; F0[e] = SF[n]; G0[e] = SG[n]; R0[e] = rhs;
-        } else {
-            A1[e] = SA[n]; B1[e] = SB[n]; C1[e] = SC[n]; D1[e] = SD[n];
-            E1[e] = SE[n]; F1[e] = SF[n]; G1[e] = SG[n]; R1[e] = rhs;
-        }
+        double* cr = (c ? C1 : C0) + (long long)r * 8 * cs.hx + h;
+        const int hx = cs.hx;
+        cr[0] = SA[n]; cr[hx] = SB[n]; cr[2 * hx] = SC[n]; cr[3 * hx] = SD[n];
+        cr[4 * hx] = SE[n]; cr[5 * hx] = SF[n]; cr[6 * hx] = SG[n]; cr[7 * hx] = rhs;
     }
     double bs = block_sum256(acc, sh);
     if (threadIdx.x == 0) {
@@ -348,9 +320,10 @@ int planetshift_impl(double* a, int f) {
 }
 
 // FP64 true residual r = RHS - A x for both colours (defect correction and
-// the guess check); per-block sum |r|, last block -> ctrl->scratch[0].
-__global__ __launch_bounds__(256) void k_resid64(CS cs, const double* const* __restrict__ cf, const double* x0,
-                                                  const double* x1, double* r, float* r32_0, float* r32_1,
+// the guess check); per-block sum |r|, last block -> ctrl->scratch[0];
+// optionally the residual in FP32 into the RHS slot of the FP32 operator.
+__global__ __launch_bounds__(256) void k_resid64(CS cs, const double* __restrict__ C0, const double* __restrict__ C1,
+                                                  const double* x0, const double* x1, float* r32_0, float* r32_1,
                                                   double* part, int nb, int chunk, Ctrl* ctrl) {
     __shared__ double sh[4];
     __shared__ int last;
@@ -369,18 +342,17 @@ __global__ __launch_bounds__(256) void k_resid64(CS cs, const double* const* __r
         int s = (jj + kk + c + 1) & 1;
         const double* xc = c ? x1 : x0;
         const double* xo = c ? x0 : x1;
-        const double* const* q = cf + 8 * c;
+        const double* q = (c ? C1 : C0) + (long long)rr * 8 * hx + h;
         long long row = (long long)px * (jj + (long long)ny * (kk + 1));
         int jm = jj == 0 ? ny - 1 : jj - 1, jp = jj == ny - 1 ? 0 : jj + 1;
-        double ax = q[0][e] * xc[row + h + 1] + q[1][e] * xo[row + h + s] + q[2][e] * xo[row + h + s + 1] +
-                    q[3][e] * xo[(long long)px * (jm + (long long)ny * (kk + 1)) + h + 1] +
-                    q[4][e] * xo[(long long)px * (jp + (long long)ny * (kk + 1)) + h + 1] +
-                    q[5][e] * xo[(long long)px * (jj + (long long)ny * kk) + h + 1] +
-                    q[6][e] * xo[(long long)px * (jj + (long long)ny * (kk + 2)) + h + 1];
-        double res = q[7][e] - ax;
+        double ax = q[0] * xc[row + h + 1] + q[hx] * xo[row + h + s] + q[2 * hx] * xo[row + h + s + 1] +
+                    q[3 * hx] * xo[(long long)px * (jm + (long long)ny * (kk + 1)) + h + 1] +
+                    q[4 * hx] * xo[(long long)px * (jp + (long long)ny * (kk + 1)) + h + 1] +
+                    q[5 * hx] * xo[(long long)px * (jj + (long long)ny * kk) + h + 1] +
+                    q[6 * hx] * xo[(long long)px * (jj + (long long)ny * (kk + 2)) + h + 1];
+        double res = q[7 * hx] - ax;
         acc += fabs(res);
-        if (r) r[t] = res;
-        if (r32_0) (c ? r32_1 : r32_0)[e] = (float)res;
+        if (r32_0) (c ? r32_1 : r32_0)[(long long)rr * 8 * hx + 7 * hx + h] = (float)res;
     }
     double bs = block_sum256(acc, sh);
     if (threadIdx.x == 0) {
@@ -451,8 +423,8 @@ static void ev_drain() {
 }
 
 template <typename T>
-static int launch_sweep(int c, T** coef0, T** coef1, T* xa, T* xb, double omega, int is_last) {
-    T** q = c ? coef1 : coef0;
+static int launch_sweep(int c, T* coef0, T* coef1, T* xa, T* xb, double omega) {
+    T* q = c ? coef1 : coef0;
     T* xc = c ? xb : xa;
     T* xo = c ? xa : xb;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -461,8 +433,7 @@ static int launch_sweep(int c, T** coef0, T** coef1, T* xa, T* xb, double omega,
         e1 = ev_get();
         hipEventRecord(e0, L.s);
     }
-    k_sweep<T><<<g_nb, 256, 0, L.s>>>(c, q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7], xc, xo, g_cs, (T)omega,
-                                      L.partial, g_nb, g_chunk, L.ctrl, is_last);
+    k_sweep<T><<<g_nb, 256, 0, L.s>>>(c, q, xc, xo, g_cs, (T)omega, L.partial, g_nb, g_chunk, L.ctrl);
     TT_LAUNCH();
     if (L.timing) {
         hipEventRecord(e1, L.s);
@@ -500,7 +471,7 @@ static int set_ctrl_solve(int itmax, int itmin, double eps) {
 
 // SOR iterations on (coef, x) until ctrl->done; returns completed iterations
 template <typename T>
-static int run_sweeps(T** coef0, T** coef1, T* xa, T* xb, double omega, int* iters, double* rnorm) {
+static int run_sweeps(T* coef0, T* coef1, T* xa, T* xb, double omega, int* iters, double* rnorm) {
     int batch = L.opts.batch > 0 ? L.opts.batch : 16;
     long long launched = 0;
     for (;;) {
@@ -510,10 +481,12 @@ static int run_sweeps(T** coef0, T** coef1, T* xa, T* xb, double omega, int* ite
             return fail(12, "SOR: device did not report completion");
         launched += batch;
         for (int b = 0; b < batch; ++b) {
-            int rc = launch_sweep<T>(0, coef0, coef1, xa, xb, omega, 0);
+            int rc = launch_sweep<T>(0, coef0, coef1, xa, xb, omega);
             if (rc) return rc;
-            rc = launch_sweep<T>(1, coef0, coef1, xa, xb, omega, 1);
+            rc = launch_sweep<T>(1, coef0, coef1, xa, xb, omega);
             if (rc) return rc;
+            k_finish<<<1, 256, 0, L.s>>>(L.ctrl, L.partial, 2 * g_nb);
+            TT_LAUNCH();
         }
         int rc = read_ctrl();
         if (rc) return rc;
@@ -532,32 +505,20 @@ int sor_pack_impl() {
     chunk = ((chunk + 255) / 256) * 256;
     nb = (int)((tot + chunk - 1) / chunk);
     k_pack<<<nb, 256, 0, L.s>>>(L.g, g_cs, f[TT_SOR_A], f[TT_SOR_B], f[TT_SOR_C], f[TT_SOR_D], f[TT_SOR_E],
-                                f[TT_SOR_F], f[TT_SOR_G], f[TT_RHS_ER], c64[0][0], c64[0][1], c64[0][2],
-                                c64[0][3], c64[0][4], c64[0][5], c64[0][6], c64[0][7], c64[1][0], c64[1][1],
-                                c64[1][2], c64[1][3], c64[1][4], c64[1][5], c64[1][6], c64[1][7],
+                                f[TT_SOR_F], f[TT_SOR_G], f[TT_RHS_ER], c64[0], c64[1],
                                 L.partial + 2 * L.npartial - nb, nb, chunk, L.ctrl);
     TT_LAUNCH();
     return 0;
 }
 
 static int resid64(double* out_norm, bool to32) {
-    // device pointer table of the 16 coefficient arrays
-    if (!g_dtab) {
-        TT_HIP(hipMalloc(&g_dtab, 16 * sizeof(double*)));
-        double* h[16];
-        for (int c = 0; c < 2; ++c)
-            for (int q = 0; q < 8; ++q) h[8 * c + q] = c64[c][q];
-        TT_HIP(hipMemcpy(g_dtab, h, sizeof h, hipMemcpyHostToDevice));
-    }
-    double** dtab = g_dtab;
     int nb = L.npartial;
     long long tot = 2 * g_cs.ncell;
     int chunk = (int)((tot + nb - 1) / nb);
     chunk = ((chunk + 255) / 256) * 256;
     nb = (int)((tot + chunk - 1) / chunk);
-    k_resid64<<<nb, 256, 0, L.s>>>(g_cs, (const double* const*)dtab, x64[0], x64[1], nullptr,
-                                   to32 ? c32[0][7] : nullptr, to32 ? c32[1][7] : nullptr,
-                                   L.partial + 2 * L.npartial - nb, nb, chunk, L.ctrl);
+    k_resid64<<<nb, 256, 0, L.s>>>(g_cs, c64[0], c64[1], x64[0], x64[1], to32 ? c32[0] : nullptr,
+                                   to32 ? c32[1] : nullptr, L.partial + 2 * L.npartial - nb, nb, chunk, L.ctrl);
     TT_LAUNCH();
     int rc = read_ctrl();
     if (rc) return rc;
@@ -627,11 +588,10 @@ int sor_solve_impl(int fill_from_evals, tt_sor_stats* st, int mode) {
         s.rnorm = rnorm;
     } else {
         // FP32 SOR inside FP64 defect correction
-        for (int c = 0; c < 2; ++c)
-            for (int q = 0; q < 7; ++q) {
-                k_to_f32<<<nblk(g_cs.ncell, 256), 256, 0, L.s>>>(c64[c][q], c32[c][q], g_cs.ncell);
-                TT_LAUNCH();
-            }
+        for (int c = 0; c < 2; ++c) {
+            k_to_f32<<<nblk(8 * g_cs.ncell, 256), 256, 0, L.s>>>(c64[c], c32[c], 8 * g_cs.ncell);
+            TT_LAUNCH();
+        }
         double rhs_norm = L.hctrl->rhsnorm;
         double rn;
         rc = resid64(&rn, true);
