@@ -13,8 +13,10 @@ in HBM; the timed region holds K consecutive steps.
 Prints one JSON line: ms/step of the plain FP64 method (`value`), SOR sweeps
 per step, cell-updates/s, the same for FP32 SOR inside FP64 defect
 correction (`mixed`), the HBM roofline of the dominant kernel (the FP64 SOR
-half sweep, timed per launch with HIP events on the library's stream), and
-the CPU oracle timed on the host cores as the reported CPU baseline.
+half sweep; every 7th launch in the timed region is bracketed by HIP events
+on the library's stream, which alternates red and black sweeps and keeps the
+events' own cost out of ms/step), and the CPU oracle timed on the host cores
+as the reported CPU baseline.
 """
 from __future__ import annotations
 
@@ -49,7 +51,7 @@ def read_traffic():
     return None
 
 
-def run_mode(dev, st0, dt, steps, warmup, precision, guess, events=True):
+def run_mode(dev, st0, dt, steps, warmup, precision, guess, event_period=7):
     dev.load_state(st0)
     dev.set_solver(precision=precision, guess=guess, batch=16, inner_tol=1e-5, inner_max=20000)
     iii = 1
@@ -57,7 +59,7 @@ def run_mode(dev, st0, dt, steps, warmup, precision, guess, events=True):
         dev.step(dt, iii)
         iii += 1
     dev.sync()
-    dev.sweep_timing(events)
+    dev.sweep_timing(event_period)
     newton = sor = f32 = 0
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -102,8 +104,8 @@ def main():
     ap.add_argument("--nphi", type=int, default=256)
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-sweep-events", action="store_true",
-                    help="time the steps without per-launch HIP events (no roofline figure)")
+    ap.add_argument("--event-period", type=int, default=7,
+                    help="bracket every N-th SOR half-sweep launch with HIP events (0: none, no roofline)")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy 1 GiB device to device (reference for rocprofv3 byte counters)")
@@ -134,11 +136,11 @@ def main():
     dt = dev.schritt()
     print(f"# workload {nx}x{ny}x{nz} dt={dt:.4g}s", file=sys.stderr, flush=True)
 
-    plain = run_mode(dev, st0, dt, a.steps, a.warmup, 0, 0, not a.no_sweep_events)
+    plain = run_mode(dev, st0, dt, a.steps, a.warmup, 0, 0, a.event_period)
     print(f"# fp64: {plain}", file=sys.stderr, flush=True)
     mixed = None
     if not a.no_mixed:
-        mixed = run_mode(dev, st0, dt, a.steps, a.warmup, 1, 1, not a.no_sweep_events)
+        mixed = run_mode(dev, st0, dt, a.steps, a.warmup, 1, 1, a.event_period)
         print(f"# mixed+warm: {mixed}", file=sys.stderr, flush=True)
     dev.close()
 
@@ -170,6 +172,7 @@ def main():
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "avg_launch_us": round(launch_ms * 1e3, 3),
+                     "timed_launches": plain["sweep64_launches"],
                      "algorithmic_bytes_per_launch": alg_bytes},
     }
     if mixed:

@@ -230,10 +230,12 @@ int tt_step(double delt, int iii, tt_step_stats* st);
 int tt_sync(void);
 
 /* Per-launch device timing of the SOR half-sweep kernels (HIP events on the
- * library's stream).  tt_sweep_timing(1) resets and enables; the getter
- * returns launches and summed milliseconds since the reset, split by FP64 and
- * FP32 sweeps. */
-int tt_sweep_timing(int enable);
+ * library's stream).  tt_sweep_timing(period) resets the counters and, for
+ * period >= 1, brackets every period-th half-sweep launch with a pair of
+ * events (0 disables; sampling keeps the events' own cost out of the step
+ * time).  The getter returns the timed launches and their summed
+ * milliseconds since the reset, split by FP64 and FP32 sweeps. */
+int tt_sweep_timing(int period);
 int tt_sweep_timing_get(int64_t* n64, double* ms64, int64_t* n32, double* ms32);
 
 /* ------------------------------------------------------------------------

@@ -88,6 +88,8 @@ struct Lib {
     std::string err;
     // sweep timing
     bool timing = false;
+    int timing_period = 1;
+    int64_t timing_count = 0;
     int64_t n64 = 0, n32 = 0;
     double ms64 = 0.0, ms32 = 0.0;
     int energy_iter = 0;
@@ -147,6 +149,33 @@ __device__ inline double wave_min(double v) {
     for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
     return v;
 }
+
+// Reduction over the 256 threads of a block, in a fixed order (wavefront
+// butterflies, then the four wavefront results in order); op 0 sum, 1 max,
+// 2 min.  The result is valid in thread 0.
+__device__ inline double block_reduce256(double v, double* sh, int op) {
+    for (int o = 32; o > 0; o >>= 1) {
+        double u = __shfl_xor(v, o, 64);
+        v = op == 0 ? v + u : (op == 1 ? fmax(v, u) : fmin(v, u));
+    }
+    int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0) {
+        s = sh[0];
+        for (int q = 1; q < 4; ++q) s = op == 0 ? s + sh[q] : (op == 1 ? fmax(s, sh[q]) : fmin(s, sh[q]));
+    }
+    __syncthreads();
+    return s;
+}
+__device__ inline double block_sum256(double v, double* sh) { return block_reduce256(v, sh, 0); }
+
+// Fold nvec (<= 3) vectors of nb block partials, part[v*nb + b], into the
+// device scalars out[v] with one block launched after the producing kernel
+// (the kernel boundary orders the partials before the fold reads them).
+int fold_partials(const double* part, int nb, int nvec, int op, double* out0, double* out1 = nullptr,
+                  double* out2 = nullptr);
 
 // Internal entry points shared between translation units
 int bounds_impl();

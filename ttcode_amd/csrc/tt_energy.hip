@@ -322,7 +322,6 @@ struct UpdArgs {
 // test, the two max reductions and the damped update (ppenergy.f:551-573).
 __global__ __launch_bounds__(256) void k_dq_update(Geo g, UpdArgs a, double* part, int nb, Ctrl* ctrl) {
     __shared__ double sh[2][4];
-    __shared__ int last;
     double mq = 0.0, me = 0.0;
     int bad = 0;
     long long n = (long long)g.nx * g.ny * g.nz;
@@ -356,21 +355,6 @@ __global__ __launch_bounds__(256) void k_dq_update(Geo g, UpdArgs a, double* par
     if (threadIdx.x == 0) {
         part[blockIdx.x] = fmax(fmax(sh[0][0], sh[0][1]), fmax(sh[0][2], sh[0][3]));
         part[nb + blockIdx.x] = fmax(fmax(sh[1][0], sh[1][1]), fmax(sh[1][2], sh[1][3]));
-        __threadfence();
-        unsigned prev = atomicAdd(&ctrl->counter2, 1u);
-        last = prev == (unsigned)(nb - 1);
-    }
-    __syncthreads();
-    if (last && threadIdx.x == 0) {
-        __threadfence();
-        double X = 0.0, Y = 0.0;
-        for (int q = 0; q < nb; ++q) {
-            X = fmax(X, part[q]);
-            Y = fmax(Y, part[nb + q]);
-        }
-        ctrl->maxdq = X;
-        ctrl->maxde = Y;
-        ctrl->counter2 = 0;
     }
 }
 
@@ -572,6 +556,8 @@ int energy_impl(double delt, int iii, tt_energy_stats* st) {
                    f[TT_EVAL_1_6], L.DFQ, f[TT_FQ], f[TT_RH], f[TT_XKAPP], f[TT_Q], f[TT_ER], L.ub, delt, conv_parm};
         k_dq_update<<<nb, 256, 0, L.s>>>(g, ua, L.partial, nb, L.ctrl);
         TT_LAUNCH();
+        rc = fold_partials(L.partial, nb, 2, 1, &L.ctrl->maxdq, &L.ctrl->maxde);
+        if (rc) return rc;
         if ((rc = energy_bcs(1))) return rc;
         if ((rc = sync_check())) return rc;
         es.max_dq = L.hctrl->maxdq;

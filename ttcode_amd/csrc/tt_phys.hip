@@ -245,7 +245,6 @@ __global__ __launch_bounds__(256) void k_schritt(Geo g, Met m, const double* __r
                                                   const double* __restrict__ H, const double* __restrict__ CS,
                                                   double dtmax, double* part, int nb, Ctrl* ctrl) {
     __shared__ double sh[3][4];
-    __shared__ int last;
     double mdt = dtmax, mrh = 1e308, mt = 1e308;
     int badint = 0;
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < g.n3;
@@ -272,23 +271,6 @@ __global__ __launch_bounds__(256) void k_schritt(Geo g, Met m, const double* __r
     if (threadIdx.x == 0) {
         for (int q = 0; q < 3; ++q)
             part[q * nb + blockIdx.x] = fmin(fmin(sh[q][0], sh[q][1]), fmin(sh[q][2], sh[q][3]));
-        __threadfence();
-        unsigned prev = atomicAdd(&ctrl->counter2, 1u);
-        last = prev == (unsigned)(nb - 1);
-    }
-    __syncthreads();
-    if (last && threadIdx.x == 0) {
-        __threadfence();
-        double x = 1e308, y = 1e308, z = 1e308;
-        for (int q = 0; q < nb; ++q) {
-            x = fmin(x, part[q]);
-            y = fmin(y, part[nb + q]);
-            z = fmin(z, part[2 * nb + q]);
-        }
-        ctrl->dtmin = x;
-        ctrl->scratch[1] = y;
-        ctrl->scratch[2] = z;
-        ctrl->counter2 = 0;
     }
 }
 
@@ -396,6 +378,10 @@ int tt_schritt(double* delt) {
     k_schritt<<<nb, 256, 0, L.s>>>(g, met(), L.f[TT_RH], L.f[TT_T], L.f[TT_PG], L.f[TT_V], L.f[TT_G], L.f[TT_H],
                                    L.f[TT_CS], L.p.dtmax, L.partial, nb, L.ctrl);
     TT_LAUNCH();
+    rc = fold_partials(L.partial, nb, 1, 2, &L.ctrl->dtmin);
+    if (rc) return rc;
+    rc = fold_partials(L.partial + nb, nb, 2, 2, &L.ctrl->scratch[1], &L.ctrl->scratch[2]);
+    if (rc) return rc;
     rc = sync_check();
     if (rc) return rc;
     if (L.hctrl->scratch[1] <= 0.0 || L.hctrl->scratch[2] <= 0.0) {

@@ -51,6 +51,7 @@ static int g_nb = 0;           // sweep blocks
 static int g_chunk = 0;
 static std::vector<hipEvent_t> ev_pool, ev_pend;
 static std::vector<int> ev_kind;
+static std::vector<int> ev_iter;  // SOR iteration (0-based) of each timed launch
 
 static long long padded_len() { return (long long)g_cs.px * g_cs.ny * (g_cs.nz + 2); }
 
@@ -70,6 +71,7 @@ void sor_free() {
     ev_pool.clear();
     ev_pend.clear();
     ev_kind.clear();
+    ev_iter.clear();
 }
 
 int sor_alloc() {
@@ -104,17 +106,6 @@ __device__ inline int xcd_block(int nb) {
     return (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
 }
 
-__device__ inline double block_sum256(double v, double* sh) {
-    v = wave_sum(v);
-    int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    if (lane == 0) sh[w] = v;
-    __syncthreads();
-    double s = 0.0;
-    if (threadIdx.x == 0) s = sh[0] + sh[1] + sh[2] + sh[3];
-    __syncthreads();
-    return s;
-}
-
 // Finish an iteration (one block of 256 threads, after both half sweeps):
 // sums the 2*nb partials in a fixed order and updates the control block.
 __global__ __launch_bounds__(256) void k_finish(Ctrl* ctrl, const double* __restrict__ part, int nparts) {
@@ -136,14 +127,19 @@ __global__ __launch_bounds__(256) void k_finish(Ctrl* ctrl, const double* __rest
 // One half sweep.  Rows (j,k) of the colour are split into contiguous
 // ranges, one per logical block; the four wavefronts of a block take every
 // fourth row of its range and the 64 lanes of a wavefront walk h.  Index
-// arithmetic is 32-bit and per row.
+// arithmetic is 32-bit and per row.  With rev set the logical blocks run in
+// reverse order: alternating the direction from one iteration to the next
+// makes a sweep start on the rows the previous sweep touched last, which are
+// still in the memory-side (Infinity) cache.
 template <typename T>
 __global__ __launch_bounds__(256) void k_sweep(int c, const T* __restrict__ coef,
                                                T* __restrict__ xc, const T* __restrict__ xo, CS cs, T omega,
-                                               double* part, int nb, int rows_per_block, const Ctrl* ctrl) {
+                                               double* part, int nb, int rows_per_block, const Ctrl* ctrl,
+                                               int rev) {
     __shared__ double sh[4];
     if (ctrl->done) return;
-    const int lb = xcd_block(nb);
+    int lb = xcd_block(nb);
+    if (rev) lb = nb - 1 - lb;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
     const int hx = cs.hx, ny = cs.ny, px = cs.px;
     const int nrows = ny * cs.nz;
@@ -165,13 +161,15 @@ __global__ __launch_bounds__(256) void k_sweep(int c, const T* __restrict__ coef
         T* __restrict__ xw = xc + row + 1;
         const T* __restrict__ cr = coef + (long long)r * 8 * hx;
         for (int h = tx; h < hx; h += 64) {
+            // all fourteen loads first, so they are in flight together
+            T q[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) q[u] = cr[u * hx + h];
             const T xs = xw[h];
-            const T a = cr[h];
-            const T resid = a * xs + cr[hx + h] * xr[h] + cr[2 * hx + h] * xr[h + 1] + cr[3 * hx + h] * xrjm[h] +
-                            cr[4 * hx + h] * xrjp[h] + cr[5 * hx + h] * xrkm[h] + cr[6 * hx + h] * xrkp[h] -
-                            cr[7 * hx + h];
+            const T x1 = xr[h], x2 = xr[h + 1], x3 = xrjm[h], x4 = xrjp[h], x5 = xrkm[h], x6 = xrkp[h];
+            const T resid = q[0] * xs + q[1] * x1 + q[2] * x2 + q[3] * x3 + q[4] * x4 + q[5] * x5 + q[6] * x6 - q[7];
             acc += fabs((double)resid);
-            xw[h] = xs - omega * resid / a;
+            xw[h] = xs - omega * resid / q[0];
         }
     }
     double bs = block_sum256(acc, sh);
@@ -207,56 +205,40 @@ __global__ void k_fill_matrix(Geo g, const int32_t* __restrict__ ub, const doubl
     if (ident) RHS[n] = 0.0;
 }
 
-// ghosted SOR_A..G, RHS -> colour-split row-blocked FP64; per-block sum
-// |RHS|; the last block sets ctrl->rhsnorm.  Cells are visited colour by colour.
+// ghosted SOR_A..G, RHS -> colour-split row-blocked FP64, plus per-block
+// sums of |RHS|.  One thread per cell pair (2p+1, 2p+2) of a row: the two
+// cells are adjacent in the ghosted arrays and land on slot p of the two
+// colours (the odd cell has s = 0).
 __global__ __launch_bounds__(256) void k_pack(Geo g, CS cs, const double* __restrict__ SA,
                                               const double* __restrict__ SB, const double* __restrict__ SC,
                                               const double* __restrict__ SD, const double* __restrict__ SE,
                                               const double* __restrict__ SF, const double* __restrict__ SG,
-                                              const double* __restrict__ RHS, double* C0, double* C1, double* part,
-                                              int nb, int chunk, Ctrl* ctrl) {
+                                              const double* __restrict__ RHS, double* C0, double* C1, double* part) {
     __shared__ double sh[4];
-    __shared__ int last;
-    const int lb = blockIdx.x;
-    long long start = (long long)lb * chunk, end = start + chunk;
-    long long tot = 2 * cs.ncell;
-    if (end > tot) end = tot;
+    const int hx = cs.hx;
+    const long long tot = cs.ncell;  // pairs = cells per colour
     double acc = 0.0;
-    for (long long t = start + threadIdx.x; t < end; t += 256) {
-        int c = t >= cs.ncell;
-        int e = (int)(t - c * cs.ncell);
-        int r = e / cs.hx;
-        int h = e - r * cs.hx;
-        int kq = r / cs.ny;
-        int j = r - kq * cs.ny + 1, k = kq + 1;
-        int s = (j + k + c + 1) & 1;
-        int i = 2 * h + 1 + s;
-        long long n = g.idx(i, j, k);
-        double rhs = RHS[n];
-        acc += fabs(rhs);
-        double* cr = (c ? C1 : C0) + (long long)r * 8 * cs.hx + h;
-        const int hx = cs.hx;
-        cr[0] = SA[n]; cr[hx] = SB[n]; cr[2 * hx] = SC[n]; cr[3 * hx] = SD[n];
-        cr[4 * hx] = SE[n]; cr[5 * hx] = SF[n]; cr[6 * hx] = SG[n]; cr[7 * hx] = rhs;
-    }
-    double bs = block_sum256(acc, sh);
-    if (threadIdx.x == 0) {
-        part[lb] = bs;
-        __threadfence();
-        unsigned prev = atomicAdd(&ctrl->counter2, 1u);
-        last = (prev == (unsigned)(nb - 1));
-    }
-    __syncthreads();
-    if (last) {
-        __threadfence();
-        double s = 0.0;
-        for (int q = threadIdx.x; q < nb; q += 256) s += part[q];
-        s = block_sum256(s, sh);
-        if (threadIdx.x == 0) {
-            ctrl->rhsnorm = s;
-            ctrl->counter2 = 0;
+    for (long long t = (long long)blockIdx.x * 256 + threadIdx.x; t < tot; t += (long long)gridDim.x * 256) {
+        const int e = (int)t;
+        const int r = e / hx;
+        const int p = e - r * hx;
+        const int kk = r / cs.ny;
+        const int j = r - kk * cs.ny + 1, k = kk + 1;
+        const int codd = (j + k + 1) & 1;  // colour of cell i = 2p+1
+        const long long n = g.idx(2 * p + 1, j, k);
+        double* co = (codd ? C1 : C0) + (long long)r * 8 * hx + p;
+        double* ce = (codd ? C0 : C1) + (long long)r * 8 * hx + p;
+        const double* src[8] = {SA, SB, SC, SD, SE, SF, SG, RHS};
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const double vo = src[q][n], ve = src[q][n + 1];
+            co[q * hx] = vo;
+            ce[q * hx] = ve;
+            if (q == 7) acc += fabs(vo) + fabs(ve);
         }
     }
+    double bs = block_sum256(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = bs;
 }
 
 // padded unknowns of colour c from a ghosted field (every slot, pads included)
@@ -320,13 +302,12 @@ int planetshift_impl(double* a, int f) {
 }
 
 // FP64 true residual r = RHS - A x for both colours (defect correction and
-// the guess check); per-block sum |r|, last block -> ctrl->scratch[0];
+// the guess check); per-block sums of |r| (folded into ctrl->scratch[0]);
 // optionally the residual in FP32 into the RHS slot of the FP32 operator.
 __global__ __launch_bounds__(256) void k_resid64(CS cs, const double* __restrict__ C0, const double* __restrict__ C1,
                                                   const double* x0, const double* x1, float* r32_0, float* r32_1,
-                                                  double* part, int nb, int chunk, Ctrl* ctrl) {
+                                                  double* part, int chunk) {
     __shared__ double sh[4];
-    __shared__ int last;
     const int lb = blockIdx.x;
     long long start = (long long)lb * chunk, end = start + chunk;
     if (end > 2 * cs.ncell) end = 2 * cs.ncell;
@@ -355,23 +336,7 @@ __global__ __launch_bounds__(256) void k_resid64(CS cs, const double* __restrict
         if (r32_0) (c ? r32_1 : r32_0)[(long long)rr * 8 * hx + 7 * hx + h] = (float)res;
     }
     double bs = block_sum256(acc, sh);
-    if (threadIdx.x == 0) {
-        part[lb] = bs;
-        __threadfence();
-        unsigned prev = atomicAdd(&ctrl->counter2, 1u);
-        last = (prev == (unsigned)(nb - 1));
-    }
-    __syncthreads();
-    if (last) {
-        __threadfence();
-        double s = 0.0;
-        for (int q = threadIdx.x; q < nb; q += 256) s += part[q];
-        s = block_sum256(s, sh);
-        if (threadIdx.x == 0) {
-            ctrl->scratch[0] = s;
-            ctrl->counter2 = 0;
-        }
-    }
+    if (threadIdx.x == 0) part[lb] = bs;
 }
 
 __global__ void k_to_f32(const double* __restrict__ a, float* b, long long n) {
@@ -409,37 +374,48 @@ static hipEvent_t ev_get() {
     return e;
 }
 
+// Fold the pending event pairs into the totals.  Launches queued after the
+// solve had finished (iteration >= the completed count once done is set)
+// return at once and are left out.
 static void ev_drain() {
+    const bool fin = L.hctrl && L.hctrl->done != 0;
+    const int itc = L.hctrl ? L.hctrl->it : 0;
     for (size_t q = 0; q + 1 < ev_pend.size(); q += 2) {
-        float ms = 0.f;
-        hipEventElapsedTime(&ms, ev_pend[q], ev_pend[q + 1]);
-        if (ev_kind[q / 2] == 64) { L.ms64 += ms; L.n64++; }
-        else { L.ms32 += ms; L.n32++; }
+        const size_t u = q / 2;
+        if (!(fin && ev_iter[u] >= itc)) {
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, ev_pend[q], ev_pend[q + 1]);
+            if (ev_kind[u] == 64) { L.ms64 += ms; L.n64++; }
+            else { L.ms32 += ms; L.n32++; }
+        }
         ev_pool.push_back(ev_pend[q]);
         ev_pool.push_back(ev_pend[q + 1]);
     }
     ev_pend.clear();
     ev_kind.clear();
+    ev_iter.clear();
 }
 
 template <typename T>
-static int launch_sweep(int c, T* coef0, T* coef1, T* xa, T* xb, double omega) {
+static int launch_sweep(int c, T* coef0, T* coef1, T* xa, T* xb, double omega, int rev, int iter) {
     T* q = c ? coef1 : coef0;
     T* xc = c ? xb : xa;
     T* xo = c ? xa : xb;
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (L.timing) {
+    const bool timed = L.timing && (L.timing_count++ % L.timing_period) == 0;
+    if (timed) {
         e0 = ev_get();
         e1 = ev_get();
         hipEventRecord(e0, L.s);
     }
-    k_sweep<T><<<g_nb, 256, 0, L.s>>>(c, q, xc, xo, g_cs, (T)omega, L.partial, g_nb, g_chunk, L.ctrl);
+    k_sweep<T><<<g_nb, 256, 0, L.s>>>(c, q, xc, xo, g_cs, (T)omega, L.partial, g_nb, g_chunk, L.ctrl, rev);
     TT_LAUNCH();
-    if (L.timing) {
+    if (timed) {
         hipEventRecord(e1, L.s);
         ev_pend.push_back(e0);
         ev_pend.push_back(e1);
         ev_kind.push_back(sizeof(T) == 8 ? 64 : 32);
+        ev_iter.push_back(iter);
     }
     return 0;
 }
@@ -473,6 +449,7 @@ static int set_ctrl_solve(int itmax, int itmin, double eps) {
 template <typename T>
 static int run_sweeps(T* coef0, T* coef1, T* xa, T* xb, double omega, int* iters, double* rnorm) {
     int batch = L.opts.batch > 0 ? L.opts.batch : 16;
+    batch += batch & 1;
     long long launched = 0;
     for (;;) {
         // the control block's iteration cap ends the loop; a solve that has
@@ -481,9 +458,11 @@ static int run_sweeps(T* coef0, T* coef1, T* xa, T* xb, double omega, int* iters
             return fail(12, "SOR: device did not report completion");
         launched += batch;
         for (int b = 0; b < batch; ++b) {
-            int rc = launch_sweep<T>(0, coef0, coef1, xa, xb, omega);
+            const int rev = (b & 1);  // batch is even: the direction alternates across batches too
+            const int iter = (int)(launched - batch) + b;
+            int rc = launch_sweep<T>(0, coef0, coef1, xa, xb, omega, rev, iter);
             if (rc) return rc;
-            rc = launch_sweep<T>(1, coef0, coef1, xa, xb, omega);
+            rc = launch_sweep<T>(1, coef0, coef1, xa, xb, omega, rev, iter);
             if (rc) return rc;
             k_finish<<<1, 256, 0, L.s>>>(L.ctrl, L.partial, 2 * g_nb);
             TT_LAUNCH();
@@ -499,16 +478,13 @@ static int run_sweeps(T* coef0, T* coef1, T* xa, T* xb, double omega, int* iters
 
 int sor_pack_impl() {
     double** f = L.f;
-    int nb = L.npartial;
-    long long tot = 2 * g_cs.ncell;
-    int chunk = (int)((tot + nb - 1) / nb);
-    chunk = ((chunk + 255) / 256) * 256;
-    nb = (int)((tot + chunk - 1) / chunk);
+    int nb = (int)((g_cs.ncell + 255) / 256);
+    if (nb > L.npartial) nb = L.npartial;
+    double* part = L.partial + L.npartial;  // second half: the sweeps use the first
     k_pack<<<nb, 256, 0, L.s>>>(L.g, g_cs, f[TT_SOR_A], f[TT_SOR_B], f[TT_SOR_C], f[TT_SOR_D], f[TT_SOR_E],
-                                f[TT_SOR_F], f[TT_SOR_G], f[TT_RHS_ER], c64[0], c64[1],
-                                L.partial + 2 * L.npartial - nb, nb, chunk, L.ctrl);
+                                f[TT_SOR_F], f[TT_SOR_G], f[TT_RHS_ER], c64[0], c64[1], part);
     TT_LAUNCH();
-    return 0;
+    return fold_partials(part, nb, 1, 0, &L.ctrl->rhsnorm);
 }
 
 static int resid64(double* out_norm, bool to32) {
@@ -517,10 +493,13 @@ static int resid64(double* out_norm, bool to32) {
     int chunk = (int)((tot + nb - 1) / nb);
     chunk = ((chunk + 255) / 256) * 256;
     nb = (int)((tot + chunk - 1) / chunk);
+    double* part = L.partial + L.npartial;
     k_resid64<<<nb, 256, 0, L.s>>>(g_cs, c64[0], c64[1], x64[0], x64[1], to32 ? c32[0] : nullptr,
-                                   to32 ? c32[1] : nullptr, L.partial + 2 * L.npartial - nb, nb, chunk, L.ctrl);
+                                   to32 ? c32[1] : nullptr, part, chunk);
     TT_LAUNCH();
-    int rc = read_ctrl();
+    int rc = fold_partials(part, nb, 1, 0, &L.ctrl->scratch[0]);
+    if (rc) return rc;
+    rc = read_ctrl();
     if (rc) return rc;
     *out_norm = L.hctrl->scratch[0];
     return 0;
@@ -672,10 +651,12 @@ int tt_solve_sor(int fill_matrix, tt_sor_stats* st) {
     return rc2 ? rc2 : rc;
 }
 
-int tt_sweep_timing(int enable) {
+int tt_sweep_timing(int period) {
     if (L.s) hipStreamSynchronize(L.s);
     ev_drain();
-    L.timing = enable != 0;
+    L.timing = period > 0;
+    L.timing_period = period > 0 ? period : 1;
+    L.timing_count = 0;
     L.n64 = L.n32 = 0;
     L.ms64 = L.ms32 = 0.0;
     return 0;

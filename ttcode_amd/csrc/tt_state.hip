@@ -12,6 +12,28 @@
 
 namespace tt {
 
+__global__ __launch_bounds__(256) void k_fold(const double* __restrict__ part, int nb, int nvec, int op, double* o0,
+                                              double* o1, double* o2) {
+    __shared__ double sh[4];
+    for (int v = 0; v < nvec; ++v) {
+        const double id = op == 0 ? 0.0 : (op == 1 ? -1e308 : 1e308);
+        double s = id;
+        for (int q = threadIdx.x; q < nb; q += 256) {
+            double u = part[(long long)v * nb + q];
+            s = op == 0 ? s + u : (op == 1 ? fmax(s, u) : fmin(s, u));
+        }
+        s = block_reduce256(s, sh, op);
+        if (threadIdx.x == 0) *(v == 0 ? o0 : (v == 1 ? o1 : o2)) = s;
+    }
+}
+
+int fold_partials(const double* part, int nb, int nvec, int op, double* out0, double* out1, double* out2) {
+    k_fold<<<1, 256, 0, L.s>>>(part, nb, nvec, op, out0, out1, out2);
+    TT_LAUNCH();
+    return 0;
+}
+
+
 Lib L;
 
 int fail(int code, const std::string& msg) {
@@ -218,6 +240,9 @@ int tt_init(const tt_params* p) {
     g.gx = g.nx + 4; g.gy = g.ny + 4; g.gz = g.nz + 4;
     g.n3 = (long long)g.gx * g.gy * g.gz;
     TT_HIP(hipStreamCreateWithFlags(&L.s, hipStreamNonBlocking));
+    // the SOR working set first: the hot loop's buffers get the first pages
+    int rc = sor_alloc();
+    if (rc) return rc;
     const size_t bytes = (size_t)g.n3 * sizeof(double);
     for (int f = 1; f < TT_FIELD_MAX; ++f) {
         int32_t lo[3], hi[3], rank, isint;
@@ -270,8 +295,6 @@ int tt_init(const tt_params* p) {
     L.opts.inner_max = 4000;
     L.opts.inner_tol = 1e-5;
     L.opts.warm_scale = 0.9;
-    int rc = sor_alloc();
-    if (rc) return rc;
     TT_HIP(hipStreamSynchronize(L.s));
     L.ready = true;
     return 0;

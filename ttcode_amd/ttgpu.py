@@ -155,8 +155,10 @@ class Device:
         return self._check(self.L.tt_sync(), "tt_sync")
 
     # ---- per-launch timing of the SOR half sweeps
-    def sweep_timing(self, enable=True):
-        self.L.tt_sweep_timing(1 if enable else 0)
+    def sweep_timing(self, period=1):
+        """Time every period-th SOR half-sweep launch with HIP events (0 or
+        False: off)."""
+        self.L.tt_sweep_timing(int(period))
 
     def sweep_timing_get(self):
         n64, n32 = ctypes.c_int64(), ctypes.c_int64()
