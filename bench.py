@@ -13,10 +13,10 @@ in HBM; the timed region holds K consecutive steps.
 Prints one JSON line: ms/step of the plain FP64 method (`value`), SOR sweeps
 per step, cell-updates/s, the same for FP32 SOR inside FP64 defect
 correction (`mixed`), the HBM roofline of the dominant kernel (the FP64 SOR
-half sweep; every 7th launch in the timed region is bracketed by HIP events
-on the library's stream, which alternates red and black sweeps and keeps the
-events' own cost out of ms/step), and the CPU oracle timed on the host cores
-as the reported CPU baseline.
+half sweep; every 7th launch in the timed region carries HIP start/stop
+events recorded by its own dispatch packet on the library's stream, which
+alternates red and black sweeps and keeps the events' cost out of ms/step),
+and the CPU oracle timed on the host cores as the reported CPU baseline.
 """
 from __future__ import annotations
 

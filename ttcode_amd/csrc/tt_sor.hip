@@ -31,6 +31,8 @@
 #include <cmath>
 #include <vector>
 
+#include <hip/hip_ext.h>
+
 #include "tt_common.h"
 
 namespace tt {
@@ -401,22 +403,22 @@ static int launch_sweep(int c, T* coef0, T* coef1, T* xa, T* xb, double omega, i
     T* q = c ? coef1 : coef0;
     T* xc = c ? xb : xa;
     T* xo = c ? xa : xb;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
     const bool timed = L.timing && (L.timing_count++ % L.timing_period) == 0;
-    if (timed) {
-        e0 = ev_get();
-        e1 = ev_get();
-        hipEventRecord(e0, L.s);
+    if (!timed) {
+        k_sweep<T><<<g_nb, 256, 0, L.s>>>(c, q, xc, xo, g_cs, (T)omega, L.partial, g_nb, g_chunk, L.ctrl, rev);
+        TT_LAUNCH();
+        return 0;
     }
-    k_sweep<T><<<g_nb, 256, 0, L.s>>>(c, q, xc, xo, g_cs, (T)omega, L.partial, g_nb, g_chunk, L.ctrl, rev);
+    // timed launch: the dispatch packet itself records the start and stop
+    // events, so the interval is the kernel's own execution
+    hipEvent_t e0 = ev_get(), e1 = ev_get();
+    hipExtLaunchKernelGGL(k_sweep<T>, dim3(g_nb), dim3(256), 0, L.s, e0, e1, 0, c, (const T*)q, xc, (const T*)xo,
+                          g_cs, (T)omega, L.partial, g_nb, g_chunk, (const Ctrl*)L.ctrl, rev);
     TT_LAUNCH();
-    if (timed) {
-        hipEventRecord(e1, L.s);
-        ev_pend.push_back(e0);
-        ev_pend.push_back(e1);
-        ev_kind.push_back(sizeof(T) == 8 ? 64 : 32);
-        ev_iter.push_back(iter);
-    }
+    ev_pend.push_back(e0);
+    ev_pend.push_back(e1);
+    ev_kind.push_back(sizeof(T) == 8 ? 64 : 32);
+    ev_iter.push_back(iter);
     return 0;
 }
 
