@@ -136,6 +136,7 @@ def main():
     dt = dev.schritt()
     print(f"# workload {nx}x{ny}x{nz} dt={dt:.4g}s", file=sys.stderr, flush=True)
 
+    hbm_bytes = dev.device_bytes()
     plain = run_mode(dev, st0, dt, a.steps, a.warmup, 0, 0, a.event_period)
     print(f"# fp64: {plain}", file=sys.stderr, flush=True)
     mixed = None
@@ -168,6 +169,7 @@ def main():
         "sor_sweeps_per_step": plain["sor_sweeps_per_step"],
         "newton_per_step": plain["newton_per_step"],
         "cell_updates_per_s": ncell / (plain["ms_per_step"] * 1e-3),
+        "hbm_footprint_GB": round(hbm_bytes / 1e9, 3),
         "roofline": {"bound": "hbm", "kernel": "k_sweep<double> (red or black half sweep)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),

@@ -229,6 +229,10 @@ int tt_step(double delt, int iii, tt_step_stats* st);
 /* Block until all queued device work is done. */
 int tt_sync(void);
 
+/* Device memory (bytes) the library holds for the current tt_init: fields,
+ * temporaries and the SOR working set. */
+int64_t tt_device_bytes(void);
+
 /* Per-launch device timing of the SOR half-sweep kernels (HIP events on the
  * library's stream).  tt_sweep_timing(period) resets the counters and, for
  * period >= 1, brackets every period-th half-sweep launch with a pair of

@@ -194,5 +194,42 @@ def main():
          {"delt": dt, "xnue": 1e13, "pr_num": 1.0})
 
 
+
+def variants():
+    """Fixtures for the switch values the main set leaves out: BOUNDXMAX(2)
+    with IBDRAD 3, 4 and 9 (9 reads ER, STELLARINPUT and xkapP from the
+    energy solve of the step before), and COUPLED_ENERGY_EQS with the
+    constant limiter (IFLIMTYP=1), Thomson scattering (ISIGTYP=1) and no
+    irradiation (F_INCIDENT=0)."""
+    nx, ny, nz = 12, 16, 12
+    keep_state = ["T", "RH", "V", "G", "H", "ER", "upperbnd"]
+    for ib in (3, 4):
+        p = P.make_params(nx, ny, nz, ikaptyp=1, kap0=1e-2, ibdrad=ib, tirr=1800.0)
+        inp = base_inputs(p, seed=23 + ib)
+        outs, sc = run_ref("bounds", p, inp)
+        save(f"bounds_ibdrad{ib}_12x16x12", p, inp, outs, sc, keep_state)
+    ps = P.make_params(nx, ny, nz, ikaptyp=1, kap0=1e-1, epsmat=1e-8, tirr=2000.0, ibdrad=9)
+    inp_st = base_inputs(ps, seed=29)
+    o = Oracle(ps)
+    o.load_state(inp_st)
+    o.bounds()
+    o.pressg()
+    dt = o.schritt()
+    outs, sc = run_ref("step", ps, inp_st, delt=dt, iii=2, nsteps=2)
+    save("step2_ibdrad9_12x16x12", ps, inp_st, outs, sc, keep_state + ["delta_ER", "PG"],
+         {"delt": dt, "iii": 2, "nsteps": 2})
+    pe = P.make_params(nx, ny, nz, ikaptyp=1, kap0=1e-1, epsmat=1e-8, iflimtyp=1, isigtyp=1,
+                       f_incident=0)
+    inp_e = base_inputs(pe, seed=31)
+    outs, sc = run_ref("energy", pe, inp_e, delt=dt, iii=2)
+    save("energy_flim1_sig_noirr_12x16x12", pe, inp_e, outs, sc,
+         ["T", "ER", "ER_OLD", "delta_ER", "FQ", "FR", "RHS_ER", "EVAL1", "EVAL7",
+          "FLIMX", "FLIMY", "FLIMZ", "difrx", "difry", "difrz", "flux_x", "flux_y", "flux_z",
+          "upperbnd", "SOR_A", "SOR_C"], {"delt": dt, "iii": 2})
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["variants"]:
+        variants()
+    else:
+        main()

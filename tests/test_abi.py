@@ -68,3 +68,19 @@ def test_field_table_matches_header(f):
     plo, phi, dt = P.field_bounds(f, 12, 16, 10)
     assert tuple(lo[:rank]) == tuple(plo) and tuple(hi[:rank]) == tuple(phi)
     assert bool(isint) == (dt == __import__("numpy").int32)
+
+
+@pytest.mark.parametrize("field,value,msg", [("bcxmin", 9, "BCXMIN"), ("ibdrad", 5, "IBDRAD"),
+                                             ("ikaptyp", 8, "IKAPTYP"), ("iflimtyp", 3, "IFLIMTYP"),
+                                             ("f_incident", 1, "F_INCIDENT"), ("isigtyp", 2, "ISIGTYP"),
+                                             ("nx", 31, "even")])
+def test_init_rejects_unsupported_switches(field, value, msg):
+    """tt_init refuses switch values the kernels do not implement with a
+    positive (fatal) code and a message, before touching a device."""
+    lib = ctypes.CDLL(LIB)
+    lib.tt_last_error.restype = ctypes.c_char_p
+    p = P.make_params(32, 64, 32)
+    setattr(p, field, value)
+    rc = lib.tt_init(ctypes.byref(p))
+    assert rc == 1
+    assert msg in lib.tt_last_error().decode()

@@ -76,8 +76,9 @@ def test_metrics(ttgpu):
     d.close()
 
 
-def test_bounds(ttgpu):
-    g = C.load("bounds_12x16x12")
+@pytest.mark.parametrize("name", ["bounds_12x16x12", "bounds_ibdrad3_12x16x12", "bounds_ibdrad4_12x16x12"])
+def test_bounds(ttgpu, name):
+    g = C.load(name)
     p, d = _dev(ttgpu, g)
     d.bounds()
     _cmp(g, p, d, rel=0.0)
@@ -117,7 +118,8 @@ def test_solve_with_sor(ttgpu):
     d.close()
 
 
-@pytest.mark.parametrize("name", ["energy_const_12x16x12", "energy_pow8_12x16x12"])
+@pytest.mark.parametrize("name", ["energy_const_12x16x12", "energy_pow8_12x16x12",
+                                  "energy_flim1_sig_noirr_12x16x12"])
 def test_coupled_energy(ttgpu, name):
     g = C.load(name)
     p, d = _dev(ttgpu, g)
@@ -129,8 +131,9 @@ def test_coupled_energy(ttgpu, name):
     d.close()
 
 
-def test_two_steps(ttgpu):
-    g = C.load("step2_12x16x12")
+@pytest.mark.parametrize("name", ["step2_12x16x12", "step2_ibdrad9_12x16x12"])
+def test_two_steps(ttgpu, name):
+    g = C.load(name)
     p, d = _dev(ttgpu, g)
     for s in range(int(g["nsteps"])):
         st, _ = d.step(float(g["delt"]), int(g["iii"]) + s)

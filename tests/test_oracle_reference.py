@@ -42,8 +42,9 @@ def test_gitter_metrics():
         np.testing.assert_array_equal(o.metric(m), g["out_m_" + n], err_msg=n)
 
 
-def test_bounds():
-    g, p, o, _ = _run("bounds_12x16x12", lambda o, g: o.bounds())
+@pytest.mark.parametrize("name", ["bounds_12x16x12", "bounds_ibdrad3_12x16x12", "bounds_ibdrad4_12x16x12"])
+def test_bounds(name):
+    g, p, o, _ = _run(name, lambda o, g: o.bounds())
     _check_fields(g, p, o)
 
 
@@ -61,7 +62,8 @@ def test_forgas3d():
     _check_fields(g, p, o)
 
 
-@pytest.mark.parametrize("name", ["energy_const_12x16x12", "energy_pow8_12x16x12"])
+@pytest.mark.parametrize("name", ["energy_const_12x16x12", "energy_pow8_12x16x12",
+                                  "energy_flim1_sig_noirr_12x16x12"])
 def test_coupled_energy(name):
     g, p, o, st = _run(name, lambda o, g: o.coupled_energy(float(g["delt"]), int(g["iii"]))[0])
     assert st.energy_iter == int(g["scal_energy_iter"])
@@ -82,12 +84,13 @@ def test_solve_with_sor():
     _check_fields(g, p, o)
 
 
-def test_two_steps():
+@pytest.mark.parametrize("name", ["step2_12x16x12", "step2_ibdrad9_12x16x12"])
+def test_two_steps(name):
     def act(o, g):
         for s in range(int(g["nsteps"])):
             st, _ = o.step(float(g["delt"]), int(g["iii"]) + s)
         return st
-    g, p, o, st = _run("step2_12x16x12", act)
+    g, p, o, st = _run(name, act)
     assert st.energy.energy_iter == int(g["scal_energy_iter"])
     assert st.energy.sor_iter_last == int(g["scal_SOR_ITER"])
     _check_fields(g, p, o)

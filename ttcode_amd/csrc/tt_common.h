@@ -95,6 +95,7 @@ struct Lib {
     int energy_iter = 0;
     int iii = 0;
     int have_prev_guess = 0;
+    int64_t dev_bytes = 0;          // device memory held by the library
 };
 
 extern Lib L;
@@ -114,6 +115,14 @@ int fail(int code, const std::string& msg);
     } while (0)
 
 inline Met met() { return Met{L.met, L.metlen}; }
+
+// hipMalloc that counts what the library holds (freed all at once by tt_finalize)
+template <typename T>
+inline hipError_t dev_alloc(T** p, size_t bytes) {
+    hipError_t e = hipMalloc((void**)p, bytes);
+    if (e == hipSuccess) L.dev_bytes += (int64_t)bytes;
+    return e;
+}
 inline unsigned nblk(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 // Fortran order helpers for the full box

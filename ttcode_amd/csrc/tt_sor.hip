@@ -83,10 +83,10 @@ int sor_alloc() {
     g_cs.px = g_cs.hx + 2;
     g_cs.ncell = (long long)g_cs.hx * g_cs.ny * g_cs.nz;
     for (int c = 0; c < 2; ++c) {
-        TT_HIP(hipMalloc(&c64[c], 8 * g_cs.ncell * sizeof(double)));
-        TT_HIP(hipMalloc(&c32[c], 8 * g_cs.ncell * sizeof(float)));
-        TT_HIP(hipMalloc(&x64[c], padded_len() * sizeof(double)));
-        TT_HIP(hipMalloc(&x32[c], padded_len() * sizeof(float)));
+        TT_HIP(dev_alloc(&c64[c], 8 * g_cs.ncell * sizeof(double)));
+        TT_HIP(dev_alloc(&c32[c], 8 * g_cs.ncell * sizeof(float)));
+        TT_HIP(dev_alloc(&x64[c], padded_len() * sizeof(double)));
+        TT_HIP(dev_alloc(&x32[c], padded_len() * sizeof(float)));
         TT_HIP(hipMemsetAsync(x64[c], 0, padded_len() * sizeof(double), L.s));
         TT_HIP(hipMemsetAsync(x32[c], 0, padded_len() * sizeof(float), L.s));
     }

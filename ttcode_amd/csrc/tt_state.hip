@@ -209,6 +209,8 @@ extern "C" {
 
 const char* tt_last_error(void) { return L.err.c_str(); }
 
+int64_t tt_device_bytes(void) { return L.ready ? L.dev_bytes : 0; }
+
 int tt_field_info(int field, int32_t lo[3], int32_t hi[3], int32_t* rank, int32_t* is_int) {
     return tt_field_bounds(field, L.g.nx, L.g.ny, L.g.nz, lo, hi, rank, is_int);
 }
@@ -230,6 +232,16 @@ int tt_init(const tt_params* p) {
     if (!p) return fail(1, "null params");
     if (p->nx < 4 || p->ny < 4 || p->nz < 4) return fail(1, "nx, ny, nz must be >= 4");
     if (p->nx % 2 || p->ny % 2) return fail(1, "nx and ny must be even (red-black colouring across the phi wrap)");
+    // the switch values the kernels implement (everything else is rejected,
+    // not silently treated as one of these)
+    if (p->bcxmin != 12) return fail(1, "BCXMIN must be 12");
+    if (p->ibdrad != 3 && p->ibdrad != 4 && p->ibdrad != 7 && p->ibdrad != 9)
+        return fail(1, "IBDRAD must be 3, 4, 7 or 9");
+    if (p->ikaptyp != 1 && p->ikaptyp != 26) return fail(1, "IKAPTYP must be 1 or 26");
+    if (p->iflimtyp != 1 && p->iflimtyp != 2) return fail(1, "IFLIMTYP must be 1 or 2");
+    if (p->f_incident != 0 && p->f_incident != 2) return fail(1, "F_INCIDENT must be 0 or 2");
+    if (p->isigtyp != 0 && p->isigtyp != 1) return fail(1, "ISIGTYP must be 0 or 1");
+    if (p->isoth != 0) return fail(1, "ISOTH must be 0");
     int ndev = 0;
     TT_HIP(hipGetDeviceCount(&ndev));
     if (ndev < 1) return fail(10, "no HIP device");
@@ -247,21 +259,21 @@ int tt_init(const tt_params* p) {
     for (int f = 1; f < TT_FIELD_MAX; ++f) {
         int32_t lo[3], hi[3], rank, isint;
         if (tt_field_bounds(f, g.nx, g.ny, g.nz, lo, hi, &rank, &isint) || rank != 3) continue;
-        TT_HIP(hipMalloc(&L.f[f], bytes));
+        TT_HIP(dev_alloc(&L.f[f], bytes));
         TT_HIP(hipMemsetAsync(L.f[f], 0, bytes, L.s));
     }
     double** tmp[] = {&L.ED, &L.VD, &L.GDN, &L.HDN, &L.FLMX, &L.FLMY, &L.FLMZ, &L.TOLD, &L.VH, &L.GH, &L.HH,
                       &L.DFQ, &g_stage};
     for (auto q : tmp) {
-        TT_HIP(hipMalloc(q, bytes));
+        TT_HIP(dev_alloc(q, bytes));
         TT_HIP(hipMemsetAsync(*q, 0, bytes, L.s));
     }
     L.npartial = 8192;
-    TT_HIP(hipMalloc(&L.partial, (size_t)L.npartial * 2 * sizeof(double)));
-    TT_HIP(hipMalloc(&L.ub, (size_t)g.gy * g.gz * sizeof(int32_t)));
+    TT_HIP(dev_alloc(&L.partial, (size_t)L.npartial * 2 * sizeof(double)));
+    TT_HIP(dev_alloc(&L.ub, (size_t)g.gy * g.gz * sizeof(int32_t)));
     TT_HIP(hipMemsetAsync(L.ub, 0, (size_t)g.gy * g.gz * sizeof(int32_t), L.s));
-    TT_HIP(hipMalloc(&L.mustar, (size_t)g.gy * g.gz * sizeof(double)));
-    TT_HIP(hipMalloc(&L.ctrl, sizeof(Ctrl)));
+    TT_HIP(dev_alloc(&L.mustar, (size_t)g.gy * g.gz * sizeof(double)));
+    TT_HIP(dev_alloc(&L.ctrl, sizeof(Ctrl)));
     TT_HIP(hipMemsetAsync(L.ctrl, 0, sizeof(Ctrl), L.s));
     TT_HIP(hipHostMalloc((void**)&L.hctrl, sizeof(Ctrl), hipHostMallocDefault));
     int nmax = g.nx > g.ny ? g.nx : g.ny;
@@ -269,7 +281,7 @@ int tt_init(const tt_params* p) {
     L.metlen = nmax + 6;
     L.hmet = new double[(size_t)TT_M_COUNT * L.metlen]();
     gitter_host(L.hmet, L.metlen, *p);
-    TT_HIP(hipMalloc(&L.met, (size_t)TT_M_COUNT * L.metlen * sizeof(double)));
+    TT_HIP(dev_alloc(&L.met, (size_t)TT_M_COUNT * L.metlen * sizeof(double)));
     TT_HIP(hipMemcpyAsync(L.met, L.hmet, (size_t)TT_M_COUNT * L.metlen * sizeof(double), hipMemcpyHostToDevice, L.s));
     // CALCMUSTAR (ppsub.f:156-179)
     std::vector<double> mu((size_t)g.gy * g.gz);

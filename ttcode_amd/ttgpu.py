@@ -46,6 +46,7 @@ def load_library():
     L.tt_set_solver.argtypes = [ctypes.POINTER(P.SolverOpts)]
     L.tt_get_solver.argtypes = [ctypes.POINTER(P.SolverOpts)]
     L.tt_sweep_timing.argtypes = [ctypes.c_int]
+    L.tt_device_bytes.restype = ctypes.c_int64
     L.tt_sweep_timing_get.argtypes = [ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double),
                                       ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double)]
     L.tt_cnn_num_weights.restype = ctypes.c_int64
@@ -153,6 +154,10 @@ class Device:
 
     def sync(self):
         return self._check(self.L.tt_sync(), "tt_sync")
+
+    def device_bytes(self):
+        """Device memory held by the library for this grid."""
+        return int(self.L.tt_device_bytes())
 
     # ---- per-launch timing of the SOR half sweeps
     def sweep_timing(self, period=1):
