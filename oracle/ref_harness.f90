@@ -43,6 +43,7 @@ program tt_ref_harness
   double precision :: h_omrot, h_sorparam, h_epsmat, h_fdelt, h_dtmax, h_delt
   double precision :: h_xnue, h_pr_num
   integer :: h_prefill_kappa
+  logical :: have_sin
   character(len=32) :: cmd
   character(len=512) :: dir
   namelist /ttin/ h_nx, h_ny, h_nz, h_bcxmin, h_ibdrad, h_iflimtyp, &
@@ -137,6 +138,10 @@ program tt_ref_harness
         call load3(dxkapP_dT, 'dxkapP_dT')
         call load3(dxkapA_dT, 'dxkapA_dT')
      end if
+     ! BOUNDXMAX(2) version 9 at III > 1 reads STELLARINPUT from the energy
+     ! solve of the step before: take it as an input when one is given
+     inquire(file=trim(dir)//'/in_STELLARINPUT.bin', exist=have_sin)
+     if (have_sin) call load3(STELLARINPUT, 'STELLARINPUT')
   end if
 
   select case (trim(cmd))

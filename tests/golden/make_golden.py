@@ -215,8 +215,19 @@ def variants():
     o.bounds()
     o.pressg()
     dt = o.schritt()
-    outs, sc = run_ref("step", ps, inp_st, delt=dt, iii=2, nsteps=2)
-    save("step2_ibdrad9_12x16x12", ps, inp_st, outs, sc, keep_state + ["delta_ER", "PG"],
+    # Version 9 at III > 1 reads STELLARINPUT and xkapP left by the previous
+    # step's energy solve.  III = 1 is not usable: the reference then runs
+    # ADJUST_SOR_PARAM (ppenergy.f:372-374), which the configs switch off.
+    # So the fields come from one oracle energy solve at III = 2 on this
+    # state and are handed to both sides as inputs.
+    o = Oracle(ps, threads=1)
+    o.load_state(inp_st)
+    o.coupled_energy(dt, 2)
+    pre = {k: o.download(k) for k in ("STELLARINPUT", "xkapR", "xkapP", "xkapA", "dxkapP_dT",
+                                      "dxkapA_dT")}
+    inp9 = dict(inp_st, **pre)
+    outs, sc = run_ref("step", ps, inp9, delt=dt, iii=2, nsteps=2, prefill=True)
+    save("step2_ibdrad9_12x16x12", ps, inp9, outs, sc, keep_state + ["delta_ER", "PG"],
          {"delt": dt, "iii": 2, "nsteps": 2})
     pe = P.make_params(nx, ny, nz, ikaptyp=1, kap0=1e-1, epsmat=1e-8, iflimtyp=1, isigtyp=1,
                        f_incident=0)

@@ -41,6 +41,22 @@ def scaled_err(a, b):
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
 
 
+# A face flux is -difr * (E_i - E_i-1): where E is nearly uniform across the
+# face the difference cancels, and a last-bit difference of E becomes a large
+# fraction of the flux.  Its forward-error scale is |difr| * |E|.
+FLUX_DIFR = {"flux_x": "difrx", "flux_y": "difry", "flux_z": "difrz"}
+
+
+def flux_err(g, k, mine, want, reg):
+    outs = C.outputs_of(g)
+    d = outs.get(FLUX_DIFR[k])
+    e = outs.get("ER")
+    if d is None or e is None:
+        return None
+    scale = float(np.max(np.abs(d[reg]))) * float(np.max(np.abs(e)))
+    return float(np.max(np.abs(mine - want)) / max(scale, 1e-300))
+
+
 def _cmp(g, p, d, names=None, rel=REL, skip=(), scaled_tol=None):
     worst = {}
     for k, ref in C.outputs_of(g).items():
@@ -54,7 +70,10 @@ def _cmp(g, p, d, names=None, rel=REL, skip=(), scaled_tol=None):
             continue
         fin = np.isfinite(want)
         assert np.array_equal(np.isfinite(mine), fin), f"{k}: non-finite pattern differs"
-        if k in SCALED:
+        fe = flux_err(g, k, mine, want, reg) if k in FLUX_DIFR else None
+        if fe is not None:
+            e, tol = fe, rel
+        elif k in SCALED:
             e = scaled_err(mine[fin], want[fin])
             tol = scaled_tol if scaled_tol is not None else rel
         else:
@@ -76,12 +95,15 @@ def test_metrics(ttgpu):
     d.close()
 
 
-@pytest.mark.parametrize("name", ["bounds_12x16x12", "bounds_ibdrad3_12x16x12", "bounds_ibdrad4_12x16x12"])
-def test_bounds(ttgpu, name):
+# IBDRAD=3 takes pow(cos*cos, 1/4) from the device math library: the
+# transcendental bar; the other versions are pure arithmetic: bit-identical
+@pytest.mark.parametrize("name,rel", [("bounds_12x16x12", 0.0), ("bounds_ibdrad3_12x16x12", REL),
+                                      ("bounds_ibdrad4_12x16x12", 0.0)])
+def test_bounds(ttgpu, name, rel):
     g = C.load(name)
     p, d = _dev(ttgpu, g)
     d.bounds()
-    _cmp(g, p, d, rel=0.0)
+    _cmp(g, p, d, rel=rel)
     d.close()
 
 

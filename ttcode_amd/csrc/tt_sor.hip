@@ -14,11 +14,12 @@
 // coefficients and the RHS of a colour are stored row-blocked: row r = (j-1)
 // + ny (k-1) holds the eight vectors A..G, RHS of length hx one after the
 // other, so a wavefront reads one contiguous 8*hx block per row.  The unknowns
-// of a colour are padded by one slot at h = -1 and h = hx and one row at k = 0
-// and k = nz+1, which hold the fixed r and theta ghost values (zero for
-// delta_ER).  The r neighbours of a cell of colour c are slots h+s-1 and h+s
-// of the other colour's row, its phi and theta neighbours slot h of the
-// neighbouring rows.
+// of a colour are stored in rows of px = hx + 4 slots, slot h at row + 2 + h
+// (even, so FP32 pairs load as one 8-byte word when hx is even); slots h = -1
+// and h = hx hold the fixed r ghost values and one extra row at k = 0 and
+// k = nz+1 the theta ghosts (zero for delta_ER).  The r neighbours of a cell
+// of colour c are slots h+s-1 and h+s of the other colour's row, its phi and
+// theta neighbours slot h of the neighbouring rows.
 //
 // Convergence.  Each half sweep writes one partial |resid| sum per block; a
 // one-block finish kernel after the black sweep sums them in a fixed order,
@@ -38,7 +39,7 @@
 namespace tt {
 
 struct CS {
-    int hx, ny, nz, px;
+    int hx, ny, nz, px;  // px = hx + 4 slots per row of unknowns, slot h at 2 + h
     long long ncell;  // cells per colour
 };
 
@@ -80,7 +81,7 @@ int sor_alloc() {
     g_cs.hx = L.g.nx / 2;
     g_cs.ny = L.g.ny;
     g_cs.nz = L.g.nz;
-    g_cs.px = g_cs.hx + 2;
+    g_cs.px = g_cs.hx + 4;
     g_cs.ncell = (long long)g_cs.hx * g_cs.ny * g_cs.nz;
     for (int c = 0; c < 2; ++c) {
         TT_HIP(dev_alloc(&c64[c], 8 * g_cs.ncell * sizeof(double)));
@@ -110,18 +111,20 @@ __device__ inline int xcd_block(int nb) {
 
 // Finish an iteration (one block of 256 threads, after both half sweeps):
 // sums the 2*nb partials in a fixed order and updates the control block.
-__global__ __launch_bounds__(256) void k_finish(Ctrl* ctrl, const double* __restrict__ part, int nparts) {
+// step is the number of iterations since the previous check (1 on the
+// reference path, which tests every iteration).
+__global__ __launch_bounds__(256) void k_finish(Ctrl* ctrl, const double* __restrict__ part, int nparts, int step) {
     __shared__ double sh[4];
     if (ctrl->done) return;
     double s = 0.0;
     for (int q = threadIdx.x; q < nparts; q += 256) s += part[q];
     s = block_sum256(s, sh);
     if (threadIdx.x == 0) {
-        int it = ctrl->it + 1;
+        int it = ctrl->it + step;
         ctrl->it = it;
         ctrl->rnorm = s;
         double ratio = s / ctrl->rhsnorm;
-        if (it == ctrl->itmax) ctrl->done = 2;
+        if (it >= ctrl->itmax) ctrl->done = 2;
         else if (!(ratio > ctrl->eps || it < ctrl->itmin)) ctrl->done = 1;
     }
 }
@@ -155,12 +158,12 @@ __global__ __launch_bounds__(256) void k_sweep(int c, const T* __restrict__ coef
         const int jm = jj == 0 ? ny - 1 : jj - 1;
         const int jp = jj == ny - 1 ? 0 : jj + 1;
         const int row = px * (jj + ny * (kk + 1));
-        const T* __restrict__ xr = xo + row + s;
-        const T* __restrict__ xrjm = xo + px * (jm + ny * (kk + 1)) + 1;
-        const T* __restrict__ xrjp = xo + px * (jp + ny * (kk + 1)) + 1;
-        const T* __restrict__ xrkm = xo + px * (jj + ny * kk) + 1;
-        const T* __restrict__ xrkp = xo + px * (jj + ny * (kk + 2)) + 1;
-        T* __restrict__ xw = xc + row + 1;
+        const T* __restrict__ xr = xo + row + 1 + s;  // slot h+s-1
+        const T* __restrict__ xrjm = xo + px * (jm + ny * (kk + 1)) + 2;
+        const T* __restrict__ xrjp = xo + px * (jp + ny * (kk + 1)) + 2;
+        const T* __restrict__ xrkm = xo + px * (jj + ny * kk) + 2;
+        const T* __restrict__ xrkp = xo + px * (jj + ny * (kk + 2)) + 2;
+        T* __restrict__ xw = xc + row + 2;
         const T* __restrict__ cr = coef + (long long)r * 8 * hx;
         for (int h = tx; h < hx; h += 64) {
             // all fourteen loads first, so they are in flight together
@@ -172,6 +175,66 @@ __global__ __launch_bounds__(256) void k_sweep(int c, const T* __restrict__ coef
             const T resid = q[0] * xs + q[1] * x1 + q[2] * x2 + q[3] * x3 + q[4] * x4 + q[5] * x5 + q[6] * x6 - q[7];
             acc += fabs((double)resid);
             xw[h] = xs - omega * resid / q[0];
+        }
+    }
+    double bs = block_sum256(acc, sh);
+    if (threadIdx.x == 0) part[c * nb + lb] = bs;
+}
+
+// FP32 half sweep with two cells per lane (hx even): half a wavefront per
+// row, so the eight coefficient loads and the six unknown loads of a lane are
+// 8-byte words and a wavefront still moves 512 bytes per load.  Same
+// arithmetic per cell as k_sweep<float>.
+__global__ __launch_bounds__(256) void k_sweep_f32x2(int c, const float* __restrict__ coef, float* __restrict__ xc,
+                                                     const float* __restrict__ xo, CS cs, float omega,
+                                                     double* part, int nb, int rows_per_block, const Ctrl* ctrl,
+                                                     int rev) {
+    __shared__ double sh[4];
+    if (ctrl->done) return;
+    int lb = xcd_block(nb);
+    if (rev) lb = nb - 1 - lb;
+    const int lane = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const int half = lane >> 5, tp = lane & 31;
+    const int hx = cs.hx, hp = hx >> 1, ny = cs.ny, px = cs.px;
+    const int nrows = ny * cs.nz;
+    const int r0 = lb * rows_per_block;
+    const int r1 = min(r0 + rows_per_block, nrows);
+    double acc = 0.0;
+    for (int r = r0 + 2 * ty + half; r < r1; r += 8) {
+        const int kk = r / ny;
+        const int jj = r - kk * ny;
+        const int s = (jj + kk + c + 1) & 1;
+        const int jm = jj == 0 ? ny - 1 : jj - 1;
+        const int jp = jj == ny - 1 ? 0 : jj + 1;
+        const int row = px * (jj + ny * (kk + 1));
+        const float* __restrict__ xrjm = xo + px * (jm + ny * (kk + 1)) + 2;
+        const float* __restrict__ xrjp = xo + px * (jp + ny * (kk + 1)) + 2;
+        const float* __restrict__ xrkm = xo + px * (jj + ny * kk) + 2;
+        const float* __restrict__ xrkp = xo + px * (jj + ny * (kk + 2)) + 2;
+        const float* __restrict__ xrr = xo + row + 2 * s;  // aligned window over slots h+s-1 .. h+s+1
+        float* __restrict__ xw = xc + row + 2;
+        const float* __restrict__ cr = coef + (long long)r * 8 * hx;
+        for (int pp = tp; pp < hp; pp += 32) {
+            const int h = 2 * pp;
+            float2 q[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) q[u] = *(const float2*)(cr + u * hx + h);
+            const float2 xs = *(const float2*)(xw + h);
+            const float2 w0 = *(const float2*)(xrr + h), w1 = *(const float2*)(xrr + h + 2);
+            const float2 a3 = *(const float2*)(xrjm + h), a4 = *(const float2*)(xrjp + h);
+            const float2 a5 = *(const float2*)(xrkm + h), a6 = *(const float2*)(xrkp + h);
+            // slots h+s-1, h+s, h+s+1 of the other colour
+            const float v0 = s ? w0.x : w0.y, v1 = s ? w0.y : w1.x, v2 = s ? w1.x : w1.y;
+            const float ra = q[0].x * xs.x + q[1].x * v0 + q[2].x * v1 + q[3].x * a3.x + q[4].x * a4.x +
+                             q[5].x * a5.x + q[6].x * a6.x - q[7].x;
+            const float rb = q[0].y * xs.y + q[1].y * v1 + q[2].y * v2 + q[3].y * a3.y + q[4].y * a4.y +
+                             q[5].y * a5.y + q[6].y * a6.y - q[7].y;
+            acc += fabs((double)ra);
+            acc += fabs((double)rb);
+            float2 o;
+            o.x = xs.x - omega * ra / q[0].x;
+            o.y = xs.y - omega * rb / q[0].y;
+            *(float2*)(xw + h) = o;
         }
     }
     double bs = block_sum256(acc, sh);
@@ -253,12 +316,12 @@ __global__ void k_load_x(Geo g, CS cs, int c, const double* __restrict__ fld, do
     int hp = tt - r * cs.px;
     int kq = r / cs.ny;
     int j = r - kq * cs.ny + 1, k = kq;
-    int h = hp - 1;
+    int h = hp - 2;
     int s = (j + k + c + 1) & 1;
     int i = 2 * h + 1 + s;
     bool interior = (h >= 0 && h < cs.hx && k >= 1 && k <= cs.nz);
     double v = 0.0;
-    if (interior || !zero_ghosts) {
+    if (h >= -1 && h <= cs.hx && (interior || !zero_ghosts)) {
         if (i >= -1 && i <= g.nx + 2) v = fld[g.idx(i, j, k)];
     }
     x[t] = v;
@@ -277,7 +340,7 @@ __global__ void k_store_x(Geo g, CS cs, const double* __restrict__ x0, const dou
     int s = (j + k + c + 1) & 1;
     int i = 2 * h + 1 + s;
     const double* x = c ? x1 : x0;
-    fld[g.idx(i, j, k)] = x[(long long)cs.px * ((j - 1) + (long long)cs.ny * k) + h + 1];
+    fld[g.idx(i, j, k)] = x[(long long)cs.px * ((j - 1) + (long long)cs.ny * k) + h + 2];
 }
 
 // PLANETSHIFT for one rank (ppshift.f:1009-1214): periodic phi ghosts for
@@ -328,11 +391,11 @@ __global__ __launch_bounds__(256) void k_resid64(CS cs, const double* __restrict
         const double* q = (c ? C1 : C0) + (long long)rr * 8 * hx + h;
         long long row = (long long)px * (jj + (long long)ny * (kk + 1));
         int jm = jj == 0 ? ny - 1 : jj - 1, jp = jj == ny - 1 ? 0 : jj + 1;
-        double ax = q[0] * xc[row + h + 1] + q[hx] * xo[row + h + s] + q[2 * hx] * xo[row + h + s + 1] +
-                    q[3 * hx] * xo[(long long)px * (jm + (long long)ny * (kk + 1)) + h + 1] +
-                    q[4 * hx] * xo[(long long)px * (jp + (long long)ny * (kk + 1)) + h + 1] +
-                    q[5 * hx] * xo[(long long)px * (jj + (long long)ny * kk) + h + 1] +
-                    q[6 * hx] * xo[(long long)px * (jj + (long long)ny * (kk + 2)) + h + 1];
+        double ax = q[0] * xc[row + h + 2] + q[hx] * xo[row + h + s + 1] + q[2 * hx] * xo[row + h + s + 2] +
+                    q[3 * hx] * xo[(long long)px * (jm + (long long)ny * (kk + 1)) + h + 2] +
+                    q[4 * hx] * xo[(long long)px * (jp + (long long)ny * (kk + 1)) + h + 2] +
+                    q[5 * hx] * xo[(long long)px * (jj + (long long)ny * kk) + h + 2] +
+                    q[6 * hx] * xo[(long long)px * (jj + (long long)ny * (kk + 2)) + h + 2];
         double res = q[7 * hx] - ax;
         acc += fabs(res);
         if (r32_0) (c ? r32_1 : r32_0)[(long long)rr * 8 * hx + 7 * hx + h] = (float)res;
@@ -355,7 +418,7 @@ __global__ void k_add_corr(CS cs, double* x, const float* __restrict__ c) {
     int h = tt - r * cs.hx;
     int kk = r / cs.ny;
     int jj = r - kk * cs.ny;
-    long long p = (long long)cs.px * (jj + (long long)cs.ny * (kk + 1)) + h + 1;
+    long long p = (long long)cs.px * (jj + (long long)cs.ny * (kk + 1)) + h + 2;
     x[p] = x[p] + (double)c[p];
 }
 
@@ -404,17 +467,24 @@ static int launch_sweep(int c, T* coef0, T* coef1, T* xa, T* xb, double omega, i
     T* xc = c ? xb : xa;
     T* xo = c ? xa : xb;
     const bool timed = L.timing && (L.timing_count++ % L.timing_period) == 0;
-    if (!timed) {
-        k_sweep<T><<<g_nb, 256, 0, L.s>>>(c, q, xc, xo, g_cs, (T)omega, L.partial, g_nb, g_chunk, L.ctrl, rev);
-        TT_LAUNCH();
-        return 0;
-    }
-    // timed launch: the dispatch packet itself records the start and stop
+    // timed launches: the dispatch packet itself records the start and stop
     // events, so the interval is the kernel's own execution
-    hipEvent_t e0 = ev_get(), e1 = ev_get();
-    hipExtLaunchKernelGGL(k_sweep<T>, dim3(g_nb), dim3(256), 0, L.s, e0, e1, 0, c, (const T*)q, xc, (const T*)xo,
-                          g_cs, (T)omega, L.partial, g_nb, g_chunk, (const Ctrl*)L.ctrl, rev);
+    hipEvent_t e0 = timed ? ev_get() : nullptr, e1 = timed ? ev_get() : nullptr;
+    if constexpr (sizeof(T) == 4) {
+        if (g_cs.hx % 2 == 0) {
+            hipExtLaunchKernelGGL(k_sweep_f32x2, dim3(g_nb), dim3(256), 0, L.s, e0, e1, 0, c, (const float*)q,
+                                  (float*)xc, (const float*)xo, g_cs, (float)omega, L.partial, g_nb, g_chunk,
+                                  (const Ctrl*)L.ctrl, rev);
+        } else {
+            hipExtLaunchKernelGGL(k_sweep<T>, dim3(g_nb), dim3(256), 0, L.s, e0, e1, 0, c, (const T*)q, xc,
+                                  (const T*)xo, g_cs, (T)omega, L.partial, g_nb, g_chunk, (const Ctrl*)L.ctrl, rev);
+        }
+    } else {
+        hipExtLaunchKernelGGL(k_sweep<T>, dim3(g_nb), dim3(256), 0, L.s, e0, e1, 0, c, (const T*)q, xc,
+                              (const T*)xo, g_cs, (T)omega, L.partial, g_nb, g_chunk, (const Ctrl*)L.ctrl, rev);
+    }
     TT_LAUNCH();
+    if (!timed) return 0;
     ev_pend.push_back(e0);
     ev_pend.push_back(e1);
     ev_kind.push_back(sizeof(T) == 8 ? 64 : 32);
@@ -449,9 +519,10 @@ static int set_ctrl_solve(int itmax, int itmin, double eps) {
 
 // SOR iterations on (coef, x) until ctrl->done; returns completed iterations
 template <typename T>
-static int run_sweeps(T* coef0, T* coef1, T* xa, T* xb, double omega, int* iters, double* rnorm) {
+static int run_sweeps(T* coef0, T* coef1, T* xa, T* xb, double omega, int* iters, double* rnorm,
+                      int check_every) {
     int batch = L.opts.batch > 0 ? L.opts.batch : 16;
-    batch += batch & 1;
+    batch = ((batch + 3) / 4) * 4;  // a multiple of 4: directions alternate and checks stay aligned
     long long launched = 0;
     for (;;) {
         // the control block's iteration cap ends the loop; a solve that has
@@ -466,8 +537,10 @@ static int run_sweeps(T* coef0, T* coef1, T* xa, T* xb, double omega, int* iters
             if (rc) return rc;
             rc = launch_sweep<T>(1, coef0, coef1, xa, xb, omega, rev, iter);
             if (rc) return rc;
-            k_finish<<<1, 256, 0, L.s>>>(L.ctrl, L.partial, 2 * g_nb);
-            TT_LAUNCH();
+            if ((b + 1) % check_every == 0) {
+                k_finish<<<1, 256, 0, L.s>>>(L.ctrl, L.partial, 2 * g_nb, check_every);
+                TT_LAUNCH();
+            }
         }
         int rc = read_ctrl();
         if (rc) return rc;
@@ -563,7 +636,7 @@ int sor_solve_impl(int fill_from_evals, tt_sor_stats* st, int mode) {
     int iters = 0;
     double rnorm = 0.0;
     if (prec == 0) {
-        rc = run_sweeps<double>(c64[0], c64[1], x64[0], x64[1], L.p.sorparam, &iters, &rnorm);
+        rc = run_sweeps<double>(c64[0], c64[1], x64[0], x64[1], L.p.sorparam, &iters, &rnorm, 1);
         if (rc) return rc;
         s.sor_iter = iters;
         s.rnorm = rnorm;
@@ -590,13 +663,17 @@ int sor_solve_impl(int fill_from_evals, tt_sor_stats* st, int mode) {
             c.done = 0; c.it = 0; c.counter = 0; c.counter2 = 0;
             c.itmin = 1;
             c.itmax = L.opts.inner_max;
-            c.eps = L.opts.inner_tol;
+            // the inner solve only has to take the FP64 residual below the
+            // outer target (with a factor 2 margin); inner_tol is the floor,
+            // what FP32 sweeps can resolve
+            double need = 0.5 * L.p.epsmat * rhs_norm / rn;
+            c.eps = need > L.opts.inner_tol ? (need < 0.5 ? need : 0.5) : L.opts.inner_tol;
             c.rhsnorm = rn;
             *L.hctrl = c;
             TT_HIP(hipMemcpyAsync(L.ctrl, L.hctrl, sizeof(Ctrl), hipMemcpyHostToDevice, L.s));
             int it32 = 0;
             double rn32 = 0;
-            rc = run_sweeps<float>(c32[0], c32[1], x32[0], x32[1], L.p.sorparam, &it32, &rn32);
+            rc = run_sweeps<float>(c32[0], c32[1], x32[0], x32[1], L.p.sorparam, &it32, &rn32, 4);
             if (rc) return rc;
             total32 += it32;
             for (int cc = 0; cc < 2; ++cc) {
