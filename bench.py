@@ -11,8 +11,10 @@ the seeded synthetic generator (ttcode_amd.params.generate_state) and stays
 in HBM; the timed region holds K consecutive steps.
 
 Prints one JSON line: ms/step of the plain FP64 method (`value`), SOR sweeps
-per step, cell-updates/s, the same for FP32 SOR inside FP64 defect
-correction (`mixed`), the HBM roofline of the dominant kernel (the FP64 SOR
+per step, cell-updates/s; the same for FP32 SOR inside FP64 defect
+correction with a cold start (`fp32`), a warm start (`fp32_warm`) and the
+learned initial guess (`learned_fp32`, weights from ttcode_amd/data); the HBM
+roofline of the dominant kernel (the FP64 SOR
 half sweep; every 7th launch in the timed region carries HIP start/stop
 events recorded by its own dispatch packet on the library's stream, which
 alternates red and black sweeps and keeps the events' cost out of ms/step),
@@ -36,6 +38,7 @@ from ttcode_amd import params as P  # noqa: E402
 METRIC = "ms/step and SOR sweeps to 1e-10 residual: learned+FP32 SOR vs plain FP64 SOR"
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "sor_sweep_fp64_sph128x128x256_traffic.txt")
+CNN_WEIGHTS = os.path.join(ROOT, "ttcode_amd", "data", "cnn_guess.npz")
 
 
 def read_traffic():
@@ -54,29 +57,39 @@ def read_traffic():
 def run_mode(dev, st0, dt, steps, warmup, precision, guess, event_period=7):
     dev.load_state(st0)
     dev.set_solver(precision=precision, guess=guess, batch=16, inner_tol=1e-5, inner_max=20000)
-    iii = 1
+    iii = 2
     for _ in range(warmup):
         dev.step(dt, iii)
         iii += 1
     dev.sync()
     dev.sweep_timing(event_period)
-    newton = sor = f32 = 0
+    newton = sor = f32 = first = used = fall = 0
+    scales, r0s = [], []
     t0 = time.perf_counter()
     for _ in range(steps):
         s, _ = dev.step(dt, iii)
         iii += 1
-        newton += s.energy.energy_iter
-        sor += s.energy.sor_iter_total
-        f32 += s.energy.fp32_sweeps
+        e = s.energy
+        newton += e.energy_iter
+        sor += e.sor_iter_total
+        f32 += e.fp32_sweeps
+        first += e.sor_iter_first
+        used += e.learned_used
+        fall += e.fallbacks
+        scales.append(e.guess_scale_first)
+        r0s.append(e.guess_r0_first)
     dev.sync()
     t1 = time.perf_counter()
     n64, ms64, n32, ms32 = dev.sweep_timing_get()
-    dev.sweep_timing(False)
+    dev.sweep_timing(0)
     return {
         "ms_per_step": (t1 - t0) * 1e3 / steps,
         "newton_per_step": newton / steps,
         "sor_sweeps_per_step": sor / steps,
         "fp32_sweeps_per_step": f32 / steps,
+        "first_solve_sweeps": first / steps,
+        "learned_used": used, "fallbacks": fall,
+        "guess_scale_first": float(np.mean(scales)), "guess_r0_first": float(np.mean(r0s)),
         "sweep64_launches": n64, "sweep64_ms": ms64,
         "sweep32_launches": n32, "sweep32_ms": ms32,
     }
@@ -139,10 +152,17 @@ def main():
     hbm_bytes = dev.device_bytes()
     plain = run_mode(dev, st0, dt, a.steps, a.warmup, 0, 0, a.event_period)
     print(f"# fp64: {plain}", file=sys.stderr, flush=True)
-    mixed = None
+    modes = {}
     if not a.no_mixed:
-        mixed = run_mode(dev, st0, dt, a.steps, a.warmup, 1, 1, a.event_period)
-        print(f"# mixed+warm: {mixed}", file=sys.stderr, flush=True)
+        modes["fp32_cold"] = run_mode(dev, st0, dt, a.steps, a.warmup, 1, 0, a.event_period)
+        modes["fp32_warm"] = run_mode(dev, st0, dt, a.steps, a.warmup, 1, 1, a.event_period)
+        if os.path.exists(CNN_WEIGHTS):
+            from ttcode_amd import cnn_train
+            cnn_train.load(dev, CNN_WEIGHTS)
+            modes["fp32_learned"] = run_mode(dev, st0, dt, a.steps, a.warmup, 1, 2, a.event_period)
+            dev.cnn_release()
+        for k, v in modes.items():
+            print(f"# {k}: {v}", file=sys.stderr, flush=True)
     dev.close()
 
     ncell = nx * ny * nz
@@ -177,14 +197,31 @@ def main():
                      "timed_launches": plain["sweep64_launches"],
                      "algorithmic_bytes_per_launch": alg_bytes},
     }
-    if mixed:
-        out["mixed"] = {"method": "FP32 SOR in FP64 defect correction, warm start",
-                        "ms_per_step": round(mixed["ms_per_step"], 4),
-                        "fp32_sweeps_per_step": mixed["fp32_sweeps_per_step"],
-                        "speedup_vs_fp64": round(plain["ms_per_step"] / mixed["ms_per_step"], 3)}
-        if mixed["sweep32_launches"]:
-            l32 = mixed["sweep32_ms"] / mixed["sweep32_launches"]
-            out["mixed"]["fp32_sweep_GBs"] = round(22.0 * ncell / (l32 * 1e-3) / 1e9, 1)
+    if modes:
+        def summary(m):
+            d = {"ms_per_step": round(m["ms_per_step"], 4),
+                 "speedup_vs_fp64": round(plain["ms_per_step"] / m["ms_per_step"], 3),
+                 "sor_sweeps_per_step": m["sor_sweeps_per_step"],
+                 "first_solve_sweeps": m["first_solve_sweeps"]}
+            if m["sweep32_launches"]:
+                l32 = m["sweep32_ms"] / m["sweep32_launches"]
+                d["fp32_sweep_us"] = round(l32 * 1e3, 3)
+                d["fp32_sweep_GBs"] = round(22.0 * ncell / (l32 * 1e-3) / 1e9, 1)
+            return d
+        out["fp32"] = {"method": "FP32 red-black SOR inside FP64 defect correction to the same 1e-10 FP64 "
+                                 "residual; cold start", **summary(modes["fp32_cold"])}
+        w = modes["fp32_warm"]
+        out["fp32_warm"] = {"method": "as fp32, initial guess = previous step's first Newton correction, "
+                                      "scaled to its least-residual multiple", **summary(w),
+                            "guess_r0_over_cold": round(w["guess_r0_first"], 4)}
+        if "fp32_learned" in modes:
+            m = modes["fp32_learned"]
+            out["learned_fp32"] = {"method": "as fp32, initial guess from the BF16 CNN trained on 2000 "
+                                             "32x64x32 FP64 solves, scaled to its least-residual multiple; "
+                                             "cold start if its residual is larger",
+                                   **summary(m), "learned_used": m["learned_used"], "fallbacks": m["fallbacks"],
+                                   "guess_scale": round(m["guess_scale_first"], 4),
+                                   "guess_r0_over_cold": round(m["guess_r0_first"], 4)}
     if not a.no_cpu:
         ms, s = cpu_baseline(p, st0, dt, a.cpu_threads)
         out["cpu_baseline"] = {"value": round(ms, 1), "unit": "ms/step", "cores": a.cpu_threads,

@@ -171,6 +171,7 @@ typedef struct tt_sor_stats {
     double rhsnorm;          /* global_RHSNORM                               */
     double rnorm;            /* global_RNORM of the last iteration           */
     double r0_guess, r0_cold;/* FP64 L1 residual of the guess and of zero    */
+    double guess_scale;      /* residual-minimising factor applied to a guess */
 } tt_sor_stats;
 
 int tt_set_solver(const tt_solver_opts* o);
@@ -202,6 +203,8 @@ typedef struct tt_energy_stats {
     int32_t learned_used;    /* solves that used the learned guess            */
     int32_t pad0;
     double max_dq, max_de;   /* global_deltaQ, global_deltaER of last iter    */
+    double guess_scale_first;/* least-residual factor of the first solve's guess */
+    double guess_r0_first;   /* its L1 residual / the cold start's (0: no guess) */
 } tt_energy_stats;
 
 /* COUPLED_ENERGY_EQS (ppenergy.f:2-791).  iii is the step counter III (read
@@ -243,22 +246,44 @@ int tt_sweep_timing(int period);
 int tt_sweep_timing_get(int64_t* n64, double* ms64, int64_t* n32, double* ms32);
 
 /* ------------------------------------------------------------------------
- * Learned initial guess (new).  A 3-D CNN maps 5 channels
- * (E^n, T, rho, kappa_R, S) to delta_E.  Weights are BF16 on the device,
- * accumulation FP32 on MFMA.  Layout: layer l has cin x cout x 27 weights
- * [cout][kz][ky][kx][cin] and cout FP32 biases; layers 5->16->16->16->16->1.
+ * Learned initial guess (new: no counterpart in the reference).  A 3-D CNN
+ * maps five per-cell features -- log E, log T, log rho, log kappa_R,
+ * asinh(STELLARINPUT), each normalised by a mean and a scale -- to the first
+ * Newton correction of COUPLED_ENERGY_EQS as a rate, delta_ER / (E dt).  Layers
+ * 5 -> 16 -> 16 -> 16 -> 16 -> 1, 3x3x3 convolutions (zero padding in r and
+ * theta, periodic in phi), GELU (tanh form) after the hidden layers; bf16
+ * weights and activations, fp32 accumulation on the matrix cores.
+ * Parameters are fp32: weights W[tap][ci][co] per layer, tap = (dk+1)*9 +
+ * (dj+1)*3 + (di+1) over (theta, phi, r) offsets, layers concatenated; biases
+ * per layer.  n_r must be a multiple of 16.  The model and the training set
+ * live across tt_init/tt_finalize until tt_cnn_release.
  * ------------------------------------------------------------------------ */
+int64_t tt_cnn_num_weights(void);   /* 23,328 */
+int64_t tt_cnn_num_biases(void);    /* 65 */
 int tt_cnn_set_weights(const float* w, int64_t nw, const float* b, int64_t nb);
 int tt_cnn_get_weights(float* w, int64_t nw, float* b, int64_t nb);
-int64_t tt_cnn_num_weights(void);
-int64_t tt_cnn_num_biases(void);
-/* Forward on host-provided channels-last input x[nz][ny][nx][5] (FP32,
- * already normalised) into y[nz][ny][nx] (FP32).  For tests. */
-int tt_cnn_forward_host(int nx, int ny, int nz, const float* x, float* y);
-/* One SGD/Adam step on (x, target) pairs held on the host; returns the loss
- * and fills grads (same layout as weights/biases) when non-null. */
-int tt_cnn_train_step_host(int nx, int ny, int nz, const float* x, const float* target,
-                           float lr, float* loss, float* gw, float* gb);
+/* norm12 = mean[6], scale[6] of the five features and of the target */
+int tt_cnn_set_norm(const float* norm12);
+int tt_cnn_get_norm(float* norm12);
+/* Forward (and, with target non-null, the mean squared error and its
+ * gradients) on host data: x0[cells][5] normalised features, cells ordered
+ * r fastest, then phi, then theta; y[cells]. */
+int tt_cnn_eval_host(int nx, int ny, int nz, const float* x0, const float* target, float* y, float* loss,
+                     float* gw, float* gb);
+/* Training set on the device: reserve room for max_samples samples of one
+ * grid; with collection on, the first Newton iteration of every
+ * tt_coupled_energy / tt_step call appends its features and its
+ * delta_ER / (E dt) (from the FP64 SOR solve).  Samples beyond the reserve are dropped. */
+int tt_cnn_reserve(int max_samples, int nx, int ny, int nz);
+int tt_cnn_collect(int enable);
+int tt_cnn_dataset_size(void);
+/* Normalisation from the training set's means and standard deviations. */
+int tt_cnn_fit_norm(void);
+/* Adam over the training set (minibatches of `batch` samples, shuffled with
+ * seed); epoch_loss[epochs] receives the mean normalised loss per epoch. */
+int tt_cnn_train(int epochs, int batch, float lr, uint64_t seed, float* epoch_loss);
+/* Free the model, its workspace and the training set. */
+int tt_cnn_release(void);
 
 #ifdef __cplusplus
 }

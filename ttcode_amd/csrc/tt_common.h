@@ -80,6 +80,8 @@ struct Lib {
     double *FLMX = nullptr, *FLMY = nullptr, *FLMZ = nullptr;
     double *TOLD = nullptr, *VH = nullptr, *GH = nullptr, *HH = nullptr;
     double *DFQ = nullptr;
+    double *DE1 = nullptr;          // first Newton correction of the last step (warm start)
+    int have_de1 = 0;
     double* partial = nullptr;      // block partial sums (reductions)
     int npartial = 0;
     Ctrl* ctrl = nullptr;           // device

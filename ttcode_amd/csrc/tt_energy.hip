@@ -19,6 +19,7 @@ namespace tt {
 int sor_scale_guess(double sc);
 int cnn_predict_guess(double delt);
 bool cnn_ready();
+int cnn_collect_sample(double delt);
 
 #define IX(i, j, k) g.idx(i, j, k)
 
@@ -526,18 +527,23 @@ int energy_impl(double delt, int iii, tt_energy_stats* st) {
         long long nc = (long long)g.nx * g.ny * g.nz;
         k_newton_cell<<<nblk(nc, 256), 256, 0, L.s>>>(g, met(), ca);
         TT_LAUNCH();
-        // initial guess (new): warm start from the previous Newton correction,
-        // or the learned prediction on the first Newton iteration
+        // initial guess (new).  Later Newton iterations: the previous
+        // correction scaled by warm_scale.  First Newton iteration: the
+        // learned prediction (guess 2, when a network is loaded), else the
+        // first correction of the previous step (guess 1 and 2).  The SOR
+        // solve keeps a guess only if its residual is not above the cold
+        // start's.
         L.have_prev_guess = 0;
-        if (L.opts.guess == 1 && L.energy_iter > 1) {
-            if ((rc = sor_scale_guess(L.opts.warm_scale))) return rc;
-            L.have_prev_guess = 1;
-        } else if (L.opts.guess == 2) {
+        if (L.opts.guess != 0) {
             if (L.energy_iter > 1) {
                 if ((rc = sor_scale_guess(L.opts.warm_scale))) return rc;
                 L.have_prev_guess = 1;
-            } else if (cnn_ready()) {
+            } else if (L.opts.guess == 2 && cnn_ready()) {
                 if ((rc = cnn_predict_guess(delt))) return rc;
+                L.have_prev_guess = 2;
+            } else if (L.have_de1) {
+                TT_HIP(hipMemcpyAsync(f[TT_DELTA_ER], L.DE1, (size_t)g.n3 * sizeof(double),
+                                      hipMemcpyDeviceToDevice, L.s));
                 L.have_prev_guess = 1;
             }
         }
@@ -545,10 +551,20 @@ int energy_impl(double delt, int iii, tt_energy_stats* st) {
         int src = sor_solve_impl(0, &ss, 0);
         if (src > 0) return src;
         if (src < 0) srcode = src;
+        if (L.energy_iter == 1) {
+            TT_HIP(hipMemcpyAsync(L.DE1, f[TT_DELTA_ER], (size_t)g.n3 * sizeof(double), hipMemcpyDeviceToDevice,
+                                  L.s));
+            L.have_de1 = 1;
+            if ((rc = cnn_collect_sample(delt))) return rc;  // training set of the learned guess, when collecting
+        }
         es.sor_iter_total += ss.sor_iter;
         es.sor_iter_last = ss.sor_iter;
         es.fp32_sweeps += ss.fp32_sweeps;
-        if (L.energy_iter == 1) es.sor_iter_first = ss.sor_iter;
+        if (L.energy_iter == 1) {
+            es.sor_iter_first = ss.sor_iter;
+            es.guess_scale_first = ss.guess_scale;
+            es.guess_r0_first = (L.have_prev_guess && ss.r0_cold > 0.0) ? ss.r0_guess / ss.r0_cold : 0.0;
+        }
         if (L.have_prev_guess && ss.guess_used == 0) es.fallbacks++;
         if (ss.guess_used == 2) es.learned_used++;
         int nb = 1024;

@@ -404,6 +404,46 @@ __global__ __launch_bounds__(256) void k_resid64(CS cs, const double* __restrict
     if (threadIdx.x == 0) part[lb] = bs;
 }
 
+// dot products of the guess check: per block sum RHS * (A x) and (A x)^2
+// over both colours (A x = RHS - r)
+__global__ __launch_bounds__(256) void k_guess_dots(CS cs, const double* __restrict__ C0,
+                                                     const double* __restrict__ C1, const double* x0,
+                                                     const double* x1, double* part, int chunk, int nb) {
+    __shared__ double sh[4];
+    const int lb = blockIdx.x;
+    long long start = (long long)lb * chunk, end = start + chunk;
+    if (end > 2 * cs.ncell) end = 2 * cs.ncell;
+    double s1 = 0.0, s2 = 0.0;
+    const int hx = cs.hx, ny = cs.ny, px = cs.px;
+    for (long long t = start + threadIdx.x; t < end; t += 256) {
+        int c = t >= cs.ncell;
+        int e = (int)(t - c * cs.ncell);
+        int rr = e / hx;
+        int h = e - rr * hx;
+        int kk = rr / ny;
+        int jj = rr - kk * ny;
+        int s = (jj + kk + c + 1) & 1;
+        const double* xc = c ? x1 : x0;
+        const double* xo = c ? x0 : x1;
+        const double* q = (c ? C1 : C0) + (long long)rr * 8 * hx + h;
+        long long row = (long long)px * (jj + (long long)ny * (kk + 1));
+        int jm = jj == 0 ? ny - 1 : jj - 1, jp = jj == ny - 1 ? 0 : jj + 1;
+        double ax = q[0] * xc[row + h + 2] + q[hx] * xo[row + h + s + 1] + q[2 * hx] * xo[row + h + s + 2] +
+                    q[3 * hx] * xo[(long long)px * (jm + (long long)ny * (kk + 1)) + h + 2] +
+                    q[4 * hx] * xo[(long long)px * (jp + (long long)ny * (kk + 1)) + h + 2] +
+                    q[5 * hx] * xo[(long long)px * (jj + (long long)ny * kk) + h + 2] +
+                    q[6 * hx] * xo[(long long)px * (jj + (long long)ny * (kk + 2)) + h + 2];
+        s1 += q[7 * hx] * ax;
+        s2 += ax * ax;
+    }
+    double b1 = block_sum256(s1, sh);
+    double b2 = block_sum256(s2, sh);
+    if (threadIdx.x == 0) {
+        part[lb] = b1;
+        part[nb + lb] = b2;
+    }
+}
+
 __global__ void k_to_f32(const double* __restrict__ a, float* b, long long n) {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) b[t] = (float)a[t];
@@ -612,7 +652,7 @@ int sor_solve_impl(int fill_from_evals, tt_sor_stats* st, int mode) {
     } else {
         // cold start: zero; a guess (warm/learned) is staged in DELTA_ER by
         // the caller and accepted only if its residual is not larger
-        bool use_guess = L.opts.guess != 0 && L.have_prev_guess;
+        bool use_guess = L.opts.guess != 0 && L.have_prev_guess != 0;
         for (int c = 0; c < 2; ++c) {
             k_load_x<<<nblk(pl, 256), 256, 0, L.s>>>(g, g_cs, c, f[TT_DELTA_ER], x64[c], 1);
             TT_LAUNCH();
@@ -622,11 +662,37 @@ int sor_solve_impl(int fill_from_evals, tt_sor_stats* st, int mode) {
         s.r0_cold = L.hctrl->rhsnorm;
         s.guess_used = 0;
         if (use_guess) {
-            double r0;
-            rc = resid64(&r0, false);
+            // scale the guess by the factor that minimises its residual's
+            // L2 norm, alpha = <b, A x> / <A x, A x>: a guess with the right
+            // shape but the wrong amplitude, or rough error that A amplifies,
+            // is pulled towards the cold start instead of being lost
+            // two partial vectors (nb each) in the second half of the buffer
+            int nb = L.npartial / 2;
+            long long tot = 2 * g_cs.ncell;
+            int chunk = (int)((tot + nb - 1) / nb);
+            chunk = ((chunk + 255) / 256) * 256;
+            nb = (int)((tot + chunk - 1) / chunk);
+            double* part = L.partial + L.npartial;
+            k_guess_dots<<<nb, 256, 0, L.s>>>(g_cs, c64[0], c64[1], x64[0], x64[1], part, chunk, nb);
+            TT_LAUNCH();
+            rc = fold_partials(part, nb, 2, 0, &L.ctrl->scratch[3], &L.ctrl->scratch[4]);
             if (rc) return rc;
-            s.r0_guess = r0;
-            if (r0 <= s.r0_cold) s.guess_used = L.opts.guess;
+            rc = read_ctrl();
+            if (rc) return rc;
+            double s1 = L.hctrl->scratch[3], s2 = L.hctrl->scratch[4];
+            double alpha = (s2 > 0.0 && s1 > 0.0) ? s1 / s2 : 0.0;
+            s.guess_scale = alpha;
+            if (alpha > 0.0) {
+                for (int c = 0; c < 2; ++c) {
+                    k_scale<<<nblk(pl, 256), 256, 0, L.s>>>(x64[c], alpha, pl);
+                    TT_LAUNCH();
+                }
+                double r0;
+                rc = resid64(&r0, false);
+                if (rc) return rc;
+                s.r0_guess = r0;
+                if (r0 <= s.r0_cold) s.guess_used = L.have_prev_guess;  // 1 warm, 2 learned
+            }
         }
         if (!s.guess_used) {
             TT_HIP(hipMemsetAsync(x64[0], 0, pl * sizeof(double), L.s));

@@ -263,7 +263,7 @@ int tt_init(const tt_params* p) {
         TT_HIP(hipMemsetAsync(L.f[f], 0, bytes, L.s));
     }
     double** tmp[] = {&L.ED, &L.VD, &L.GDN, &L.HDN, &L.FLMX, &L.FLMY, &L.FLMZ, &L.TOLD, &L.VH, &L.GH, &L.HH,
-                      &L.DFQ, &g_stage};
+                      &L.DFQ, &L.DE1, &g_stage};
     for (auto q : tmp) {
         TT_HIP(dev_alloc(q, bytes));
         TT_HIP(hipMemsetAsync(*q, 0, bytes, L.s));

@@ -24,12 +24,12 @@ module ttgpu_c
   type, bind(C) :: tt_energy_stats
      integer(c_int32_t) :: energy_iter, sor_iter_total, sor_iter_last, sor_iter_first
      integer(c_int32_t) :: fp32_sweeps, fallbacks, learned_used, pad0
-     real(c_double) :: max_dq, max_de
+     real(c_double) :: max_dq, max_de, guess_scale_first, guess_r0_first
   end type tt_energy_stats
 
   type, bind(C) :: tt_sor_stats
      integer(c_int32_t) :: sor_iter, outer_iter, fp32_sweeps, guess_used
-     real(c_double) :: rhsnorm, rnorm, r0_guess, r0_cold
+     real(c_double) :: rhsnorm, rnorm, r0_guess, r0_cold, guess_scale
   end type tt_sor_stats
 
   ! field ids (include/ttgpu.h)

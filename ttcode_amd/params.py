@@ -71,7 +71,8 @@ class SorStats(ctypes.Structure):
     _fields_ = [("sor_iter", ctypes.c_int32), ("outer_iter", ctypes.c_int32),
                 ("fp32_sweeps", ctypes.c_int32), ("guess_used", ctypes.c_int32),
                 ("rhsnorm", ctypes.c_double), ("rnorm", ctypes.c_double),
-                ("r0_guess", ctypes.c_double), ("r0_cold", ctypes.c_double)]
+                ("r0_guess", ctypes.c_double), ("r0_cold", ctypes.c_double),
+                ("guess_scale", ctypes.c_double)]
 
 
 class EnergyStats(ctypes.Structure):
@@ -81,7 +82,8 @@ class EnergyStats(ctypes.Structure):
                 ("sor_iter_last", ctypes.c_int32), ("sor_iter_first", ctypes.c_int32),
                 ("fp32_sweeps", ctypes.c_int32), ("fallbacks", ctypes.c_int32),
                 ("learned_used", ctypes.c_int32), ("pad0", ctypes.c_int32),
-                ("max_dq", ctypes.c_double), ("max_de", ctypes.c_double)]
+                ("max_dq", ctypes.c_double), ("max_de", ctypes.c_double),
+                ("guess_scale_first", ctypes.c_double), ("guess_r0_first", ctypes.c_double)]
 
 
 class StepStats(ctypes.Structure):

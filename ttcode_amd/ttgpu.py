@@ -51,6 +51,15 @@ def load_library():
                                       ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double)]
     L.tt_cnn_num_weights.restype = ctypes.c_int64
     L.tt_cnn_num_biases.restype = ctypes.c_int64
+    fp = ctypes.POINTER(ctypes.c_float)
+    L.tt_cnn_set_weights.argtypes = [fp, ctypes.c_int64, fp, ctypes.c_int64]
+    L.tt_cnn_get_weights.argtypes = [fp, ctypes.c_int64, fp, ctypes.c_int64]
+    L.tt_cnn_set_norm.argtypes = [fp]
+    L.tt_cnn_get_norm.argtypes = [fp]
+    L.tt_cnn_eval_host.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, fp, fp, fp, fp, fp, fp]
+    L.tt_cnn_reserve.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.tt_cnn_collect.argtypes = [ctypes.c_int]
+    L.tt_cnn_train.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_uint64, fp]
     _lib = L
     return L
 
@@ -158,6 +167,69 @@ class Device:
     def device_bytes(self):
         """Device memory held by the library for this grid."""
         return int(self.L.tt_device_bytes())
+
+    # ---- learned initial guess
+    @staticmethod
+    def _fptr(a):
+        return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float)) if a is not None else None
+
+    def cnn_set_weights(self, w, b):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        b = np.ascontiguousarray(b, dtype=np.float32)
+        self._check(self.L.tt_cnn_set_weights(self._fptr(w), w.size, self._fptr(b), b.size), "tt_cnn_set_weights")
+
+    def cnn_get_weights(self):
+        w = np.zeros(self.L.tt_cnn_num_weights(), np.float32)
+        b = np.zeros(self.L.tt_cnn_num_biases(), np.float32)
+        self._check(self.L.tt_cnn_get_weights(self._fptr(w), w.size, self._fptr(b), b.size), "tt_cnn_get_weights")
+        return w, b
+
+    def cnn_set_norm(self, norm12):
+        a = np.ascontiguousarray(norm12, dtype=np.float32)
+        self._check(self.L.tt_cnn_set_norm(self._fptr(a)), "tt_cnn_set_norm")
+
+    def cnn_get_norm(self):
+        a = np.zeros(12, np.float32)
+        self._check(self.L.tt_cnn_get_norm(self._fptr(a)), "tt_cnn_get_norm")
+        return a
+
+    def cnn_eval(self, x0, target=None):
+        """x0: (nz, ny, nx, 5) normalised features.  Returns y (nz, ny, nx)
+        and, with a target, (loss, grad_w, grad_b)."""
+        nz, ny, nx, _ = x0.shape
+        x = np.ascontiguousarray(x0, dtype=np.float32)
+        y = np.zeros((nz, ny, nx), np.float32)
+        if target is None:
+            self._check(self.L.tt_cnn_eval_host(nx, ny, nz, self._fptr(x), None, self._fptr(y), None, None, None),
+                        "tt_cnn_eval_host")
+            return y
+        t = np.ascontiguousarray(target, dtype=np.float32)
+        loss = np.zeros(1, np.float32)
+        gw = np.zeros(self.L.tt_cnn_num_weights(), np.float32)
+        gb = np.zeros(self.L.tt_cnn_num_biases(), np.float32)
+        self._check(self.L.tt_cnn_eval_host(nx, ny, nz, self._fptr(x), self._fptr(t), self._fptr(y), self._fptr(loss),
+                                            self._fptr(gw), self._fptr(gb)), "tt_cnn_eval_host")
+        return y, float(loss[0]), gw, gb
+
+    def cnn_reserve(self, max_samples, nx, ny, nz):
+        self._check(self.L.tt_cnn_reserve(max_samples, nx, ny, nz), "tt_cnn_reserve")
+
+    def cnn_collect(self, enable=True):
+        self._check(self.L.tt_cnn_collect(1 if enable else 0), "tt_cnn_collect")
+
+    def cnn_dataset_size(self):
+        return int(self.L.tt_cnn_dataset_size())
+
+    def cnn_fit_norm(self):
+        self._check(self.L.tt_cnn_fit_norm(), "tt_cnn_fit_norm")
+
+    def cnn_train(self, epochs, batch=8, lr=1e-3, seed=0):
+        losses = np.zeros(epochs, np.float32)
+        self._check(self.L.tt_cnn_train(epochs, batch, lr, seed, self._fptr(losses)), "tt_cnn_train")
+        return losses
+
+    def cnn_release(self):
+        self.L.tt_cnn_release()
 
     # ---- per-launch timing of the SOR half sweeps
     def sweep_timing(self, period=1):
