@@ -82,7 +82,9 @@ def run_mode(dev, st0, dt, steps, warmup, precision, guess, event_period=7):
     t1 = time.perf_counter()
     n64, ms64, n32, ms32 = dev.sweep_timing_get()
     dev.sweep_timing(0)
+    final = {k: dev.download(k) for k in ("ER", "T")}
     return {
+        "final": final,
         "ms_per_step": (t1 - t0) * 1e3 / steps,
         "newton_per_step": newton / steps,
         "sor_sweeps_per_step": sor / steps,
@@ -151,7 +153,7 @@ def main():
 
     hbm_bytes = dev.device_bytes()
     plain = run_mode(dev, st0, dt, a.steps, a.warmup, 0, 0, a.event_period)
-    print(f"# fp64: {plain}", file=sys.stderr, flush=True)
+    print(f"# fp64: { {k: v for k, v in plain.items() if k != 'final'} }", file=sys.stderr, flush=True)
     modes = {}
     if not a.no_mixed:
         modes["fp32_cold"] = run_mode(dev, st0, dt, a.steps, a.warmup, 1, 0, a.event_period)
@@ -162,7 +164,7 @@ def main():
             modes["fp32_learned"] = run_mode(dev, st0, dt, a.steps, a.warmup, 1, 2, a.event_period)
             dev.cnn_release()
         for k, v in modes.items():
-            print(f"# {k}: {v}", file=sys.stderr, flush=True)
+            print(f"# {k}: { {q: w for q, w in v.items() if q != 'final'} }", file=sys.stderr, flush=True)
     dev.close()
 
     ncell = nx * ny * nz
@@ -203,6 +205,11 @@ def main():
                  "speedup_vs_fp64": round(plain["ms_per_step"] / m["ms_per_step"], 3),
                  "sor_sweeps_per_step": m["sor_sweeps_per_step"],
                  "first_solve_sweeps": m["first_solve_sweeps"]}
+            errs = {}
+            for k in ("ER", "T"):
+                a_, b_ = m["final"][k][2:-2, 2:-2, 2:-2], plain["final"][k][2:-2, 2:-2, 2:-2]
+                errs[k] = float(np.max(np.abs(a_ - b_) / np.abs(b_)))
+            d["max_rel_err_vs_fp64_after_steps"] = {k: float(f"{v:.3e}") for k, v in errs.items()}
             if m["sweep32_launches"]:
                 l32 = m["sweep32_ms"] / m["sweep32_launches"]
                 d["fp32_sweep_us"] = round(l32 * 1e3, 3)
