@@ -233,8 +233,14 @@ int tt_step(double delt, int iii, tt_step_stats* st);
 int tt_sync(void);
 
 /* Device memory (bytes) the library holds for the current tt_init: fields,
- * temporaries and the SOR working set. */
+ * temporaries and the SOR working set (the learned guess's model and
+ * training set are not included). */
 int64_t tt_device_bytes(void);
+
+/* Diagnostics: volume integrals over the interior cells (dV = volxb volyb
+ * volzb) of the thermal energy rho c_v T, the radiation energy E and the
+ * irradiation heating rate STELLARINPUT, into out3[0..2]. */
+int tt_energy_totals(double* out3);
 
 /* Per-launch device timing of the SOR half-sweep kernels (HIP events on the
  * library's stream).  tt_sweep_timing(period) resets the counters and, for

@@ -239,3 +239,28 @@ def test_mixed_precision_solve(ttgpu):
     scale = np.max(np.abs(x64))
     assert np.max(np.abs(x32 - x64)) <= 1e-6 * scale
     d.close()
+
+
+def test_energy_totals_match_host_integral(ttgpu):
+    """tt_energy_totals against the same volume integrals in numpy from the
+    downloaded fields and the GITTER volume metrics."""
+    p = P.make_params(32, 64, 32, epsmat=1e-8)
+    st = P.generate_state(p, seed=9)
+    d = ttgpu.Device(p)
+    d.load_state(st)
+    d.bounds()
+    d.pressg()
+    dt = d.schritt()
+    d.step(dt, 2)
+    tot = d.energy_totals()
+    nx, ny, nz = p.nx, p.ny, p.nz
+    inner = (slice(2, nz + 2), slice(2, ny + 2), slice(2, nx + 2))
+    # volxb, volyb, volzb have lower bound 0: entries 1..n are the interior
+    vx, vy, vz = (d.metric(m)[1:n + 1] for m, n in ((25, nx), (31, ny), (37, nz)))
+    dv = vz[:, None, None] * vy[None, :, None] * vx[None, None, :]
+    rh, t, er, cv = (d.download(k)[inner] for k in ("RH", "T", "ER", "CV"))
+    sin = d.download("STELLARINPUT")[2:nz + 2, 2:ny + 2, :]   # declared (1:nx, -1:ny+2, -1:nz+2)
+    d.close()
+    for got, want in ((tot[0], np.sum(rh * cv * t * dv)), (tot[1], np.sum(er * dv)), (tot[2], np.sum(sin * dv))):
+        assert abs(got / want - 1) < 1e-12, (got, want)
+    assert tot[2] > 0.0

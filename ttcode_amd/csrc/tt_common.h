@@ -61,6 +61,10 @@ struct Ctrl {
     int pad;
     double dtmin;
     double scratch[8];
+    // stagnation watch of inner (FP32) solves
+    double best;       // best L1 ratio seen
+    int since;         // iterations since it last improved by 1%
+    int stag;          // window (0: off)
 };
 
 // Library state (one device, one stream).

@@ -54,11 +54,11 @@ __global__ void k_kappa(Geo g, tt_params p, const int32_t* __restrict__ ub, cons
     DKP[t] = dk; DKA[t] = dk;
 }
 
-// CALCSTELLARTAU + STELLAR_DTAU + STELLARINPUT_HEATING grey branch
-// (pprad.f:191-242, 408-516) for one (j,k) column: reverse scan along r.
-__global__ void k_stellar(Geo g, Met m, tt_params p, const int32_t* __restrict__ ub, const double* __restrict__ XKA,
-                          const double* __restrict__ RH, const double* __restrict__ MU, double* STAU, double* DTAUS,
-                          double* SIN) {
+// CALCSTELLARTAU (pprad.f:191-221): tau* by a reverse scan along r, one
+// thread per (j,k) column, summed in the reference's order; the products
+// are loaded sixteen cells at a time so the loads overlap.
+__global__ void k_stau(Geo g, Met m, const int32_t* __restrict__ ub, const double* __restrict__ XKA,
+                       const double* __restrict__ RH, double* STAU) {
     int nj = g.ny + 4, nk = g.nz + 4;
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nj * nk) return;
@@ -68,13 +68,41 @@ __global__ void k_stellar(Geo g, Met m, tt_params p, const int32_t* __restrict__
     if (u < g.nx)
         for (int i = u + 1; i <= g.nx + 1; ++i) STAU[IX(i, j, k)] = 0.0;
     double tot = 0.0;
-    for (int i = u; i >= 1; --i) {
-        tot = tot + (XKA[IX(i, j, k)] * RH[IX(i, j, k)] * dr);
-        STAU[IX(i, j, k)] = tot;
+    for (int i0 = u; i0 >= 1; i0 -= 16) {
+        double ka[16], rh[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            int i = i0 - q;
+            ka[q] = i >= 1 ? XKA[IX(i, j, k)] : 0.0;
+            rh[q] = i >= 1 ? RH[IX(i, j, k)] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            int i = i0 - q;
+            if (i >= 1) {
+                tot = tot + (ka[q] * rh[q] * dr);
+                STAU[IX(i, j, k)] = tot;
+            }
+        }
     }
-    for (int i = 2; i <= g.nx; ++i) DTAUS[IX(i, j, k)] = (STAU[IX(i - 1, j, k)] - STAU[IX(i + 1, j, k)]) / 2.0;
-    DTAUS[IX(1, j, k)] = DTAUS[IX(2, j, k)];
-    DTAUS[IX(g.nx + 1, j, k)] = DTAUS[IX(g.nx, j, k)];
+}
+
+// STELLAR_DTAU + STELLARINPUT_HEATING grey branch (pprad.f:223-242,
+// 408-516), per cell i = 1..nx+1 of every column incl. ghosts
+__global__ void k_sin(Geo g, Met m, tt_params p, const double* __restrict__ STAU, const double* __restrict__ MU,
+                      double* DTAUS, double* SIN) {
+    int nj = g.ny + 4, nk = g.nz + 4;
+    long long n = (long long)(g.nx + 1) * nj * nk;
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    int i = (int)(t % (g.nx + 1)) + 1;
+    int col = (int)(t / (g.nx + 1));
+    int j = col % nj - 1, k = col / nj - 1;
+    // DTAUS(1) = DTAUS(2) and DTAUS(nx+1) = DTAUS(nx)
+    int ic = i == 1 ? 2 : (i == g.nx + 1 ? g.nx : i);
+    double dt = (STAU[IX(ic - 1, j, k)] - STAU[IX(ic + 1, j, k)]) / 2.0;
+    DTAUS[IX(i, j, k)] = dt;
+    if (i > g.nx) return;
     double angle = cos(m(TT_M_XYB, j)) * cos(m(TT_M_XZB, k));
     double tinc = 0.0;
     if (angle >= 0.0) {
@@ -83,8 +111,7 @@ __global__ void k_stellar(Geo g, Met m, tt_params p, const int32_t* __restrict__
     }
     double mu = MU[g.idx2(j, k)];
     double t4 = pow(tinc, 4.0);
-    for (int i = 1; i <= g.nx; ++i)
-        SIN[IX(i, j, k)] = SBCONST * (t4 / mu) * (exp(-STAU[IX(i, j, k)] / mu)) * DTAUS[IX(i, j, k)] / m(TT_M_DXB, i);
+    SIN[IX(i, j, k)] = SBCONST * (t4 / mu) * (exp(-STAU[IX(i, j, k)] / mu)) * dt / m(TT_M_DXB, i);
 }
 
 __global__ void k_fill_box(double* a, double v, long long n) {
@@ -452,8 +479,11 @@ static int stellar_impl() {
     }
     if (L.p.f_incident != 2) return fail(1, "STELLARINPUT_HEATING: F_INCIDENT must be 0 or 2");
     int ncol = (L.g.ny + 4) * (L.g.nz + 4);
-    k_stellar<<<nblk(ncol, 64), 64, 0, L.s>>>(L.g, met(), L.p, L.ub, f[TT_XKAPA], f[TT_RH], L.mustar,
-                                              f[TT_STELLARTAU], f[TT_DTAUS], f[TT_STELLARINPUT]);
+    k_stau<<<nblk(ncol, 64), 64, 0, L.s>>>(L.g, met(), L.ub, f[TT_XKAPA], f[TT_RH], f[TT_STELLARTAU]);
+    TT_LAUNCH();
+    long long ncell = (long long)(L.g.nx + 1) * ncol;
+    k_sin<<<nblk(ncell, 256), 256, 0, L.s>>>(L.g, met(), L.p, f[TT_STELLARTAU], L.mustar, f[TT_DTAUS],
+                                             f[TT_STELLARINPUT]);
     TT_LAUNCH();
     return 0;
 }

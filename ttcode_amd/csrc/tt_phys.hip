@@ -45,19 +45,29 @@ __global__ void k_shift5(Geo g, double* T, double* RH, double* V, double* G, dou
 
 // upperbnd (ppbou.f:124-138) for every column incl. ghosts, which PASSUPPERBND
 // (ppbou.f:522-682) fills from the phi-wrapped column and the nearest theta
-// row when the theta/corner partners are MPI_PROC_NULL.
-__global__ void k_upperbnd(Geo g, const double* __restrict__ RH, int32_t* ub) {
+// row when the theta/corner partners are MPI_PROC_NULL.  One wavefront per
+// column: the lanes read 64 consecutive r cells (one coalesced load) and a
+// ballot finds the first with rho below the threshold.
+__global__ __launch_bounds__(256) void k_upperbnd(Geo g, const double* __restrict__ RH, int32_t* ub) {
     int nj = g.ny + 4, nk = g.nz + 4;
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nj * nk) return;
-    int j = t % nj - 1, k = t / nj - 1;
+    int col = blockIdx.x * 4 + (threadIdx.x >> 6);
+    int lane = threadIdx.x & 63;
+    if (col >= nj * nk) return;
+    int j = col % nj - 1, k = col / nj - 1;
     int jw = j <= 0 ? j + g.ny : (j > g.ny ? j - g.ny : j);
     int kc = k < 1 ? 1 : (k > g.nz ? g.nz : k);
     const double thr = (double)1.0e-9f;
     int u = g.nx - 1;
-    for (int i = 1; i <= g.nx; ++i)
-        if (RH[g.idx(i, jw, kc)] < thr) { u = i; break; }
-    ub[g.idx2(j, k)] = u;
+    for (int base = 1; base <= g.nx; base += 64) {
+        int i = base + lane;
+        bool low = i <= g.nx && RH[g.idx(i, jw, kc)] < thr;
+        unsigned long long m = __ballot(low);
+        if (m) {
+            u = base + __ffsll((long long)m) - 1;
+            break;
+        }
+    }
+    if (lane == 0) ub[g.idx2(j, k)] = u;
 }
 
 // BOUNDXMIN(12) (ppbou.f:1118-1139) then BOUNDXMAX(2) (ppbou.f:1269-1395)
@@ -181,7 +191,7 @@ int bounds_impl() {
     int ncol = (g.ny + 4) * (g.nz + 4), nline = (g.nx + 4) * (g.ny + 4);
     k_shift5<<<nblk(ns, 256), 256, 0, L.s>>>(g, f[TT_T], f[TT_RH], f[TT_V], f[TT_G], f[TT_H]);
     TT_LAUNCH();
-    k_upperbnd<<<nblk(ncol, 128), 128, 0, L.s>>>(g, f[TT_RH], L.ub);
+    k_upperbnd<<<nblk(ncol, 4), 256, 0, L.s>>>(g, f[TT_RH], L.ub);
     TT_LAUNCH();
     k_xbc<<<nblk(ncol, 128), 128, 0, L.s>>>(g, met(), L.ub, f[TT_T], f[TT_RH], f[TT_V], f[TT_G], f[TT_H],
                                             f[TT_ER], f[TT_STELLARINPUT], f[TT_XKAPP], L.p.d_t, L.p.ibdrad,
@@ -271,6 +281,35 @@ __global__ __launch_bounds__(256) void k_schritt(Geo g, Met m, const double* __r
     if (threadIdx.x == 0) {
         for (int q = 0; q < 3; ++q)
             part[q * nb + blockIdx.x] = fmin(fmin(sh[q][0], sh[q][1]), fmin(sh[q][2], sh[q][3]));
+    }
+}
+
+// Volume integrals over the interior cells, dV = volxb(i) volyb(j) volzb(k):
+// thermal energy rho c_v T, radiation energy E and the irradiation heating
+// STELLARINPUT (per-block partials, folded by k_fold)
+__global__ __launch_bounds__(256) void k_energy_totals(Geo g, Met m, const double* __restrict__ RH,
+                                                        const double* __restrict__ CV, const double* __restrict__ T,
+                                                        const double* __restrict__ ER,
+                                                        const double* __restrict__ SIN, double* part, int nb) {
+    __shared__ double sh[4];
+    double a = 0.0, b = 0.0, c = 0.0;
+    long long n = (long long)g.nx * g.ny * g.nz;
+    for (long long t = (long long)blockIdx.x * 256 + threadIdx.x; t < n; t += (long long)gridDim.x * 256) {
+        int i, j, k;
+        sub_ijk(t, g.nx, g.ny, 1, 1, 1, i, j, k);
+        long long q = g.idx(i, j, k);
+        double dv = m(TT_M_VOLXB, i) * m(TT_M_VOLYB, j) * m(TT_M_VOLZB, k);
+        a += RH[q] * CV[q] * T[q] * dv;
+        b += ER[q] * dv;
+        c += SIN[q] * dv;
+    }
+    a = block_sum256(a, sh);
+    b = block_sum256(b, sh);
+    c = block_sum256(c, sh);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = a;
+        part[nb + blockIdx.x] = b;
+        part[2 * nb + blockIdx.x] = c;
     }
 }
 
@@ -412,3 +451,21 @@ int tt_diffusion(double delt, double xnue, double pr_num, int* iters) {
 }
 
 }  // extern "C"
+
+extern "C" int tt_energy_totals(double* out3) {
+    if (!L.ready) return fail(1, "tt_init has not been called");
+    if (!out3) return fail(1, "null output");
+    const int nb = 1024;
+    if (3 * nb > 2 * L.npartial) return fail(1, "energy totals: partial buffer too small");
+    k_energy_totals<<<nb, 256, 0, L.s>>>(L.g, met(), L.f[TT_RH], L.f[TT_CV], L.f[TT_T], L.f[TT_ER],
+                                         L.f[TT_STELLARINPUT], L.partial, nb);
+    TT_LAUNCH();
+    int rc = fold_partials(L.partial, nb, 3, 0, &L.ctrl->scratch[5], &L.ctrl->scratch[6], &L.ctrl->scratch[7]);
+    if (rc) return rc;
+    TT_HIP(hipMemcpyAsync(L.hctrl, L.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, L.s));
+    TT_HIP(hipStreamSynchronize(L.s));
+    out3[0] = L.hctrl->scratch[5];
+    out3[1] = L.hctrl->scratch[6];
+    out3[2] = L.hctrl->scratch[7];
+    return 0;
+}

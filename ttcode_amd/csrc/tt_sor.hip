@@ -115,17 +115,28 @@ __device__ inline int xcd_block(int nb) {
 // reference path, which tests every iteration).
 __global__ __launch_bounds__(256) void k_finish(Ctrl* ctrl, const double* __restrict__ part, int nparts, int step) {
     __shared__ double sh[4];
-    if (ctrl->done) return;
+    // the partial loads and the flag load are issued together; a solve that
+    // has already stopped only skips the update
     double s = 0.0;
     for (int q = threadIdx.x; q < nparts; q += 256) s += part[q];
+    const int done = ctrl->done;
     s = block_sum256(s, sh);
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0 && !done) {
         int it = ctrl->it + step;
         ctrl->it = it;
         ctrl->rnorm = s;
         double ratio = s / ctrl->rhsnorm;
         if (it >= ctrl->itmax) ctrl->done = 2;
         else if (!(ratio > ctrl->eps || it < ctrl->itmin)) ctrl->done = 1;
+        else if (ctrl->stag > 0) {  // inner solves: stop when the ratio no longer falls
+            if (ratio < 0.99 * ctrl->best) {
+                ctrl->best = ratio;
+                ctrl->since = 0;
+            } else {
+                ctrl->since += step;
+                if (ctrl->since >= ctrl->stag) ctrl->done = 3;
+            }
+        }
     }
 }
 
@@ -552,6 +563,7 @@ static int set_ctrl_solve(int itmax, int itmin, double eps) {
     c.eps = eps;
     c.counter = 0;
     c.counter2 = 0;
+    c.stag = 0;
     *L.hctrl = c;
     TT_HIP(hipMemcpyAsync(L.ctrl, L.hctrl, sizeof(Ctrl), hipMemcpyHostToDevice, L.s));
     return 0;
@@ -729,6 +741,12 @@ int sor_solve_impl(int fill_from_evals, tt_sor_stats* st, int mode) {
             c.done = 0; c.it = 0; c.counter = 0; c.counter2 = 0;
             c.itmin = 1;
             c.itmax = L.opts.inner_max;
+            // FP32 sweeps stop reducing the residual at their rounding floor:
+            // hand over to the FP64 correction when the ratio has not fallen
+            // by 1% over 256 iterations
+            c.best = 1e300;
+            c.since = 0;
+            c.stag = 256;
             // the inner solve only has to take the FP64 residual below the
             // outer target (with a factor 2 margin); inner_tol is the floor,
             // what FP32 sweeps can resolve
@@ -746,8 +764,10 @@ int sor_solve_impl(int fill_from_evals, tt_sor_stats* st, int mode) {
                 k_add_corr<<<nblk(g_cs.ncell, 256), 256, 0, L.s>>>(g_cs, x64[cc], x32[cc]);
                 TT_LAUNCH();
             }
+            double rn_prev = rn;
             rc = resid64(&rn, true);
             if (rc) return rc;
+            if (!(rn < rn_prev)) break;  // no progress: report the cap below
         }
         s.sor_iter = total32;
         s.fp32_sweeps = total32;

@@ -168,6 +168,12 @@ class Device:
         """Device memory held by the library for this grid."""
         return int(self.L.tt_device_bytes())
 
+    def energy_totals(self):
+        """(integral rho c_v T dV, integral E dV, integral STELLARINPUT dV)."""
+        out = (ctypes.c_double * 3)()
+        self._check(self.L.tt_energy_totals(out), "tt_energy_totals")
+        return float(out[0]), float(out[1]), float(out[2])
+
     # ---- learned initial guess
     @staticmethod
     def _fptr(a):
