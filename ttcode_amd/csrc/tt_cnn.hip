@@ -166,48 +166,72 @@ __global__ void k_pack_frags(const float* __restrict__ w, bf16_t* frag, float* w
 __device__ inline bf16x8 ld8(const bf16_t* p) { return __builtin_bit_cast(bf16x8, *(const uint4*)p); }
 
 // Hidden-width convolution on the matrix cores.  CIN 8 (layer 0 input) or 16.
+// A wavefront computes TPW consecutive tiles of 16 cells: each weight
+// fragment is loaded once for TPW independent accumulation chains.
 // mode 0: out = bf16(gelu(acc + bias)), pre = acc + bias (when pre != null)
 // mode 1: out = bf16(acc * gelu'(gpre))  (input gradient of the layer, times
 //         the previous layer's gelu derivative)
+constexpr int TPW = 4;
 template <int CIN>
 __global__ __launch_bounds__(256) void k_conv_mfma(const bf16_t* __restrict__ x, const bf16_t* __restrict__ frag,
                                                    const float* __restrict__ bias, Grid3 g, int ntiles, int mode,
                                                    bf16_t* __restrict__ out, float* __restrict__ pre,
                                                    const float* __restrict__ gpre) {
     const int lane = threadIdx.x & 63;
-    const int tile = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (tile >= ntiles) return;
+    const int tile0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * TPW;
+    if (tile0 >= ntiles) return;
     const int tpr = g.nx >> 4;  // tiles per r row
-    const int row = tile / tpr, i0 = (tile - row * tpr) * 16;
-    const int k = row / g.ny, j = row - k * g.ny;
     const int m = lane & 15, kg = lane >> 4;
+    int ti[TPW], tj[TPW], tk[TPW];
+    bool tv[TPW];
+#pragma unroll
+    for (int u = 0; u < TPW; ++u) {
+        int tile = tile0 + u;
+        tv[u] = tile < ntiles;
+        if (!tv[u]) tile = tile0;
+        const int row = tile / tpr;
+        ti[u] = (tile - row * tpr) * 16;
+        tk[u] = row / g.ny;
+        tj[u] = row - tk[u] * g.ny;
+    }
     constexpr int NQ = CIN == 8 ? FQ0 : FQH;
-    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    f32x4 acc[TPW];
+#pragma unroll
+    for (int u = 0; u < TPW; ++u) acc[u] = f32x4{0.f, 0.f, 0.f, 0.f};
     const uint4 zero = {0u, 0u, 0u, 0u};
-#pragma unroll 2
     for (int q = 0; q < NQ; ++q) {
         const int tap = CIN == 8 ? 4 * q + kg : 2 * q + (kg >> 1);
         const int c0 = CIN == 8 ? 0 : 8 * (kg & 1);
-        uint4 av = zero;
-        if (tap < 27) {
-            int nb = g.nbr(i0 + m, j, k, tap);
-            if (nb >= 0) av = *(const uint4*)(x + (long long)nb * CIN + c0);
-        }
-        const bf16x8 a = __builtin_bit_cast(bf16x8, av);
         const bf16x8 b = ld8(frag + ((long long)q * 64 + lane) * 8);
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+        uint4 av[TPW];
+#pragma unroll
+        for (int u = 0; u < TPW; ++u) {
+            av[u] = zero;
+            if (tap < 27) {
+                int nb = g.nbr(ti[u] + m, tj[u], tk[u], tap);
+                if (nb >= 0) av[u] = *(const uint4*)(x + (long long)nb * CIN + c0);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < TPW; ++u)
+            acc[u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, av[u]), b, acc[u], 0, 0, 0);
     }
     const int n = lane & 15;
-    const int base = g.cell(i0, j, k);
+    const float bn = mode == 0 ? bias[n] : 0.f;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const long long c = (long long)(base + 4 * kg + r) * 16 + n;
-        if (mode == 0) {
-            float v = acc[r] + bias[n];
-            if (pre) pre[c] = v;
-            out[c] = f2bf(gelu(v));
-        } else {
-            out[c] = f2bf(acc[r] * gelu_d(gpre[c]));
+    for (int u = 0; u < TPW; ++u) {
+        if (!tv[u]) continue;
+        const int base = g.cell(ti[u], tj[u], tk[u]);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const long long c = (long long)(base + 4 * kg + r) * 16 + n;
+            if (mode == 0) {
+                float v = acc[u][r] + bn;
+                if (pre) pre[c] = v;
+                out[c] = f2bf(gelu(v));
+            } else {
+                out[c] = f2bf(acc[u][r] * gelu_d(gpre[c]));
+            }
         }
     }
 }
@@ -538,7 +562,7 @@ static int repack() {
 static int forward(Grid3 g, bool train) {
     long long n = (long long)g.nx * g.ny * g.nz;
     int ntiles = (int)(n / 16);
-    unsigned nb = (unsigned)((ntiles + 3) / 4);
+    unsigned nb = (unsigned)((ntiles + 4 * TPW - 1) / (4 * TPW));
     k_conv_mfma<8><<<nb, 256, 0, L.s>>>(M.act[0], M.frag + FRAG_L0, M.b + boff(0), g, ntiles, 0, M.act[1],
                                         train ? M.pre[0] : nullptr, nullptr);
     TT_LAUNCH();
@@ -575,7 +599,7 @@ static int backward(Grid3 g, double* loss_part, int* loss_nb) {
     k_bwd_out<<<nblk(n, 256), 256, 0, L.s>>>(M.gy, M.w4, M.pre[3], g, n, M.gu);
     TT_LAUNCH();
     int ntiles = (int)(n / 16);
-    unsigned tnb = (unsigned)((ntiles + 3) / 4);
+    unsigned tnb = (unsigned)((ntiles + 4 * TPW - 1) / (4 * TPW));
     bf16_t* gu = M.gu;
     bf16_t* gn = M.gu2;
     for (int l = NL - 2; l >= 0; --l) {  // hidden layers 3..0: gu = dL/d(pre-activation of layer l)

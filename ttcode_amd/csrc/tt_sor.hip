@@ -109,19 +109,31 @@ __device__ inline int xcd_block(int nb) {
     return (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
 }
 
-// Finish an iteration (one block of 256 threads, after both half sweeps):
+// Finish an iteration (one block of 1024 threads, after both half sweeps):
 // sums the 2*nb partials in a fixed order and updates the control block.
 // step is the number of iterations since the previous check (1 on the
-// reference path, which tests every iteration).
-__global__ __launch_bounds__(256) void k_finish(Ctrl* ctrl, const double* __restrict__ part, int nparts, int step) {
-    __shared__ double sh[4];
-    // the partial loads and the flag load are issued together; a solve that
-    // has already stopped only skips the update
-    double s = 0.0;
-    for (int q = threadIdx.x; q < nparts; q += 256) s += part[q];
+// reference path, which tests every iteration).  Each thread loads its
+// (at most 8) partials at once, so the block waits for one memory round
+// trip, not for a chain of them.
+__global__ __launch_bounds__(1024) void k_finish(Ctrl* ctrl, const double* __restrict__ part, int nparts, int step) {
+    __shared__ double sh[16];
+    const int tid = threadIdx.x;
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        const int q = tid + 1024 * u;
+        v[u] = q < nparts ? part[q] : 0.0;
+    }
     const int done = ctrl->done;
-    s = block_sum256(s, sh);
-    if (threadIdx.x == 0 && !done) {
+    double s = 0.0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s += v[u];
+    s = wave_sum(s);
+    if ((tid & 63) == 0) sh[tid >> 6] = s;
+    __syncthreads();
+    if (tid == 0 && !done) {
+        s = 0.0;
+        for (int w = 0; w < 16; ++w) s += sh[w];
         int it = ctrl->it + step;
         ctrl->it = it;
         ctrl->rnorm = s;
@@ -590,7 +602,7 @@ static int run_sweeps(T* coef0, T* coef1, T* xa, T* xb, double omega, int* iters
             rc = launch_sweep<T>(1, coef0, coef1, xa, xb, omega, rev, iter);
             if (rc) return rc;
             if ((b + 1) % check_every == 0) {
-                k_finish<<<1, 256, 0, L.s>>>(L.ctrl, L.partial, 2 * g_nb, check_every);
+                k_finish<<<1, 1024, 0, L.s>>>(L.ctrl, L.partial, 2 * g_nb, check_every);
                 TT_LAUNCH();
             }
         }
