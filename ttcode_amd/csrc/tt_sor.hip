@@ -55,6 +55,13 @@ static int g_chunk = 0;
 static std::vector<hipEvent_t> ev_pool, ev_pend;
 static std::vector<int> ev_kind;
 static std::vector<int> ev_iter;  // SOR iteration (0-based) of each timed launch
+static std::vector<long long> ev_bid;  // batch of each timed launch
+// Two pinned copies of the control block, so that the host reads batch k's
+// convergence state while batch k+1 already runs on the device.
+static Ctrl* g_hc[2] = {nullptr, nullptr};
+static hipEvent_t g_cev[2] = {nullptr, nullptr};
+static long long g_cbid[2] = {0, 0};
+static long long g_bid = 0;  // batches enqueued so far
 
 static long long padded_len() { return (long long)g_cs.px * g_cs.ny * (g_cs.nz + 2); }
 
@@ -75,6 +82,13 @@ void sor_free() {
     ev_pend.clear();
     ev_kind.clear();
     ev_iter.clear();
+    ev_bid.clear();
+    for (int q = 0; q < 2; ++q) {
+        if (g_hc[q]) hipHostFree(g_hc[q]);
+        if (g_cev[q]) hipEventDestroy(g_cev[q]);
+        g_hc[q] = nullptr;
+        g_cev[q] = nullptr;
+    }
 }
 
 int sor_alloc() {
@@ -90,6 +104,10 @@ int sor_alloc() {
         TT_HIP(dev_alloc(&x32[c], padded_len() * sizeof(float)));
         TT_HIP(hipMemsetAsync(x64[c], 0, padded_len() * sizeof(double), L.s));
         TT_HIP(hipMemsetAsync(x32[c], 0, padded_len() * sizeof(float), L.s));
+    }
+    for (int q = 0; q < 2; ++q) {
+        TT_HIP(hipHostMalloc((void**)&g_hc[q], sizeof(Ctrl), hipHostMallocDefault));
+        TT_HIP(hipEventCreateWithFlags(&g_cev[q], hipEventDisableTiming));
     }
     // sweep blocks: about 4 rows per wavefront, a multiple of 8 (XCDs)
     int nrows = g_cs.ny * g_cs.nz;
@@ -502,26 +520,38 @@ static hipEvent_t ev_get() {
     return e;
 }
 
-// Fold the pending event pairs into the totals.  Launches queued after the
-// solve had finished (iteration >= the completed count once done is set)
-// return at once and are left out.
-static void ev_drain() {
-    const bool fin = L.hctrl && L.hctrl->done != 0;
-    const int itc = L.hctrl ? L.hctrl->it : 0;
-    for (size_t q = 0; q + 1 < ev_pend.size(); q += 2) {
-        const size_t u = q / 2;
+// Fold the pending event pairs of batches <= upto into the totals; st is
+// the control block after those batches.  Launches queued after the solve
+// had finished (iteration >= the completed count once done is set) return
+// at once and are left out.
+static void ev_drain(const Ctrl* st, long long upto) {
+    const bool fin = st && st->done != 0;
+    const int itc = st ? st->it : 0;
+    size_t keep = 0;
+    for (size_t u = 0; u < ev_kind.size(); ++u) {
+        hipEvent_t a = ev_pend[2 * u], b = ev_pend[2 * u + 1];
+        if (ev_bid[u] > upto) {
+            ev_pend[2 * keep] = a;
+            ev_pend[2 * keep + 1] = b;
+            ev_kind[keep] = ev_kind[u];
+            ev_iter[keep] = ev_iter[u];
+            ev_bid[keep] = ev_bid[u];
+            ++keep;
+            continue;
+        }
         if (!(fin && ev_iter[u] >= itc)) {
             float ms = 0.f;
-            hipEventElapsedTime(&ms, ev_pend[q], ev_pend[q + 1]);
+            hipEventElapsedTime(&ms, a, b);
             if (ev_kind[u] == 64) { L.ms64 += ms; L.n64++; }
             else { L.ms32 += ms; L.n32++; }
         }
-        ev_pool.push_back(ev_pend[q]);
-        ev_pool.push_back(ev_pend[q + 1]);
+        ev_pool.push_back(a);
+        ev_pool.push_back(b);
     }
-    ev_pend.clear();
-    ev_kind.clear();
-    ev_iter.clear();
+    ev_pend.resize(2 * keep);
+    ev_kind.resize(keep);
+    ev_iter.resize(keep);
+    ev_bid.resize(keep);
 }
 
 template <typename T>
@@ -552,13 +582,14 @@ static int launch_sweep(int c, T* coef0, T* coef1, T* xa, T* xb, double omega, i
     ev_pend.push_back(e1);
     ev_kind.push_back(sizeof(T) == 8 ? 64 : 32);
     ev_iter.push_back(iter);
+    ev_bid.push_back(g_bid);
     return 0;
 }
 
 static int read_ctrl() {
     TT_HIP(hipMemcpyAsync(L.hctrl, L.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, L.s));
     TT_HIP(hipStreamSynchronize(L.s));
-    if (L.timing) ev_drain();
+    if (L.timing) ev_drain(L.hctrl, g_bid);
     return 0;
 }
 
@@ -588,12 +619,14 @@ static int run_sweeps(T* coef0, T* coef1, T* xa, T* xb, double omega, int* iters
     int batch = L.opts.batch > 0 ? L.opts.batch : 16;
     batch = ((batch + 3) / 4) * 4;  // a multiple of 4: directions alternate and checks stay aligned
     long long launched = 0;
+    int slot = 0, pend = -1, last = 0;
     for (;;) {
         // the control block's iteration cap ends the loop; a solve that has
         // run past it without the device reporting completion is an error
-        if (launched > (long long)L.hctrl->itmax + 2 * batch)
+        if (launched > (long long)L.hctrl->itmax + 3 * batch)
             return fail(12, "SOR: device did not report completion");
         launched += batch;
+        ++g_bid;
         for (int b = 0; b < batch; ++b) {
             const int rev = (b & 1);  // batch is even: the direction alternates across batches too
             const int iter = (int)(launched - batch) + b;
@@ -606,9 +639,26 @@ static int run_sweeps(T* coef0, T* coef1, T* xa, T* xb, double omega, int* iters
                 TT_LAUNCH();
             }
         }
-        int rc = read_ctrl();
-        if (rc) return rc;
-        if (L.hctrl->done) break;
+        TT_HIP(hipMemcpyAsync(g_hc[slot], L.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, L.s));
+        TT_HIP(hipEventRecord(g_cev[slot], L.s));
+        g_cbid[slot] = g_bid;
+        last = slot;
+        // the previous batch's state: this batch is already queued behind it,
+        // and returns at once if the previous one finished the solve
+        if (pend >= 0) {
+            TT_HIP(hipEventSynchronize(g_cev[pend]));
+            if (L.timing) ev_drain(g_hc[pend], g_cbid[pend]);
+            if (g_hc[pend]->done) {
+                *L.hctrl = *g_hc[pend];
+                break;
+            }
+        }
+        pend = slot;
+        slot ^= 1;
+    }
+    if (L.timing) {
+        TT_HIP(hipEventSynchronize(g_cev[last]));
+        ev_drain(L.hctrl, g_cbid[last]);
     }
     *iters = L.hctrl->it;
     *rnorm = L.hctrl->rnorm;
@@ -830,7 +880,7 @@ int tt_solve_sor(int fill_matrix, tt_sor_stats* st) {
 
 int tt_sweep_timing(int period) {
     if (L.s) hipStreamSynchronize(L.s);
-    ev_drain();
+    ev_drain(L.hctrl, g_bid);
     L.timing = period > 0;
     L.timing_period = period > 0 ? period : 1;
     L.timing_count = 0;
@@ -841,7 +891,7 @@ int tt_sweep_timing(int period) {
 
 int tt_sweep_timing_get(int64_t* n64, double* ms64, int64_t* n32, double* ms32) {
     if (L.s) hipStreamSynchronize(L.s);
-    ev_drain();
+    ev_drain(L.hctrl, g_bid);
     if (n64) *n64 = L.n64;
     if (ms64) *ms64 = L.ms64;
     if (n32) *n32 = L.n32;
