@@ -264,3 +264,33 @@ def test_energy_totals_match_host_integral(ttgpu):
     for got, want in ((tot[0], np.sum(rh * cv * t * dv)), (tot[1], np.sum(er * dv)), (tot[2], np.sum(sin * dv))):
         assert abs(got / want - 1) < 1e-12, (got, want)
     assert tot[2] > 0.0
+
+
+@pytest.mark.parametrize("epsmat", [3e-2, 1e-3, 3e-5, 1e-6, 1e-9])
+def test_sor_stops_at_the_reference_iteration(ttgpu, epsmat):
+    """The convergence test runs once per group of iterations; a stop inside
+    a group is replayed from a snapshot (or from the solve's start) so that
+    the unknowns equal those of a test after every iteration.  Tolerances
+    that stop early, mid-group and late, against the CPU oracle."""
+    from oracle.ttoracle import Oracle
+    g = C.load("sor_12x16x12")
+    p = C.params_of(g)
+    p.epsmat = epsmat
+    keys = ("EVAL1", "EVAL2", "EVAL3", "EVAL4", "EVAL5", "EVAL6", "EVAL7", "EVAL8", "EVAL_1_6", "RHS_ER", "upperbnd")
+    o = Oracle(p, threads=1)
+    o.load_state(C.inputs_of(g))
+    for k in keys:
+        o.upload(k, g["in_" + k])
+    so, _ = o.solve_sor(True)
+    d = ttgpu.Device(p)
+    d.load_state(C.inputs_of(g))
+    for k in keys:
+        d.upload(k, g["in_" + k])
+    sg, _ = d.solve_sor(True)
+    x = d.download("delta_ER")
+    d.close()
+    print(f"epsmat {epsmat}: iterations {sg.sor_iter} (oracle {so.sor_iter})")
+    assert sg.sor_iter == so.sor_iter
+    want = o.download("delta_ER")
+    reg = C.compare_region("delta_ER", p.nx, p.ny, p.nz)
+    assert C.rel_err(x[reg], want[reg]) <= 1e-13
