@@ -31,8 +31,9 @@ struct Geo {
     int nx, ny, nz;
     int gx, gy, gz;
     long long n3;
+    // 32-bit arithmetic: tt_init rejects boxes of 2^31 cells or more
     __host__ __device__ inline long long idx(int i, int j, int k) const {
-        return (long long)(i + 1) + (long long)gx * ((long long)(j + 1) + (long long)gy * (long long)(k + 1));
+        return (long long)((i + 1) + gx * ((j + 1) + gy * (k + 1)));
     }
     __host__ __device__ inline int idx2(int j, int k) const { return (j + 1) + gy * (k + 1); }
 };

@@ -76,22 +76,24 @@ __global__ void k_adv_mass(Geo g, Met m, double dt, const double* __restrict__ R
     }
 }
 
-// ---- thermal energy ED = T*RH*CV (ppadv.f:741-891)
-__device__ inline double edv(const Geo& g, const double* T, const double* RH, const double* CV, int i, int j, int k) {
-    long long c = g.idx(i, j, k);
-    return T[c] * RH[c] * CV[c];
+// ---- thermal energy ED = T*RH*CV (ppadv.f:741-891).  The product is formed
+// once per cell of the box (k_ed0, the reference's expression), so the
+// sweep reads one value per stencil point instead of three.
+__global__ void k_ed0(const double* __restrict__ T, const double* __restrict__ RH, const double* __restrict__ CV,
+                      double* E0, long long n) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) E0[t] = T[t] * RH[t] * CV[t];
 }
 
-__global__ void k_adv_energy(Geo g, Met m, double dt, const double* __restrict__ T, const double* __restrict__ RH,
-                             const double* __restrict__ CV, const double* __restrict__ V, const double* __restrict__ G,
-                             const double* __restrict__ H, double* ED) {
+__global__ void k_adv_energy(Geo g, Met m, double dt, const double* __restrict__ E0, const double* __restrict__ V,
+                             const double* __restrict__ G, const double* __restrict__ H, double* ED) {
     long long n = (long long)g.nx * g.ny * g.nz;
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     int i, j, k;
     sub_ijk(t, g.nx, g.ny, 1, 1, 1, i, j, k);
     const double half = 0.5;
-    auto E = [&](int a, int b, int c) { return edv(g, T, RH, CV, a, b, c); };
+    auto E = [&](int a, int b, int c) { return E0[g.idx(a, b, c)]; };
     double fx[2], fy[2], fz[2];
     for (int q = 0; q < 2; ++q) {
         int I = i + q;
@@ -291,7 +293,9 @@ int advect_impl(double dt) {
     TT_LAUNCH();
     if ((rc = qdens_impl())) return rc;
     long long nc = (long long)g.nx * g.ny * g.nz;
-    k_adv_energy<<<nblk(nc, 256), 256, 0, L.s>>>(g, met(), dt, f[TT_T], f[TT_RH], f[TT_CV], f[TT_V], f[TT_G], f[TT_H], L.ED);
+    k_ed0<<<nblk(g.n3, 256), 256, 0, L.s>>>(f[TT_T], f[TT_RH], f[TT_CV], L.TOLD, g.n3);
+    TT_LAUNCH();
+    k_adv_energy<<<nblk(nc, 256), 256, 0, L.s>>>(g, met(), dt, L.TOLD, f[TT_V], f[TT_G], f[TT_H], L.ED);
     TT_LAUNCH();
     long long nv = (long long)(g.nx - 1) * g.ny * g.nz;
     k_adv_v<<<nblk(nv, 256), 256, 0, L.s>>>(g, met(), dt, f[TT_V], f[TT_G], f[TT_H], L.FLMX, L.FLMY, L.FLMZ, f[TT_RHQX], L.VD);

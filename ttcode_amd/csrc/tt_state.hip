@@ -251,6 +251,7 @@ int tt_init(const tt_params* p) {
     g.nx = p->nx; g.ny = p->ny; g.nz = p->nz;
     g.gx = g.nx + 4; g.gy = g.ny + 4; g.gz = g.nz + 4;
     g.n3 = (long long)g.gx * g.gy * g.gz;
+    if (g.n3 >= (1LL << 31)) return fail(1, "grid too large: the ghosted box must hold fewer than 2^31 cells");
     TT_HIP(hipStreamCreateWithFlags(&L.s, hipStreamNonBlocking));
     // the SOR working set first: the hot loop's buffers get the first pages
     int rc = sor_alloc();
