@@ -159,6 +159,10 @@ typedef struct tt_solver_opts {
     int32_t guess;           /* 0 cold, 1 warm, 2 learned                    */
     int32_t batch;           /* sweeps queued between host convergence checks */
     int32_t inner_max;       /* FP32 sweeps per correction (precision 1)     */
+    int32_t adjust_omega;    /* 1: ADJUST_SOR_PARAM at the first Newton
+                                iteration of steps III = 1 and III % 10000 = 0
+                                (ppenergy.f:372-374); 0 (default): omega fixed */
+    int32_t pad0;
     double inner_tol;        /* FP32 inner L1 ratio target (precision 1)     */
     double warm_scale;       /* scale of the previous delta_ER for guess 1   */
 } tt_solver_opts;
@@ -182,6 +186,22 @@ int tt_get_solver(tt_solver_opts* o);
  * EVAL_1_6, RHS_ER and upperbnd (ppsor.f:18-27) first; fill_matrix == 0 takes
  * SOR_A..G and RHS_ER as uploaded.  Result in delta_ER. */
 int tt_solve_sor(int fill_matrix, tt_sor_stats* st);
+
+/* ADJUST_SOR_PARAM (ppsor.f:332-354) on the system already on the device:
+ * 50 cold-start solves with SORPARAM = 1.01, 1.03, ..., 1.99 (REAL literals:
+ * the start is the float nearest 1.01, the step the float nearest 0.02,
+ * added in double), then SORPARAM = the first omega with the fewest
+ * iterations.  fill_matrix as for tt_solve_sor (the solves refill from the
+ * same EVALs, so once is the same).  Each solve runs in the current solver
+ * precision (FP64: SOR_ITER; precision 1: FP32 iterations).  omega and
+ * iters[50] (may be null) receive the chosen omega and each solve's count;
+ * delta_ER holds the last solve's result, as in the reference. */
+int tt_adjust_sor_param(int fill_matrix, double* omega, int32_t* iters);
+
+/* SORPARAM of the running library (set by tt_init from tt_params.sorparam,
+ * changed by ADJUST_SOR_PARAM). */
+int tt_set_sorparam(double omega);
+double tt_get_sorparam(void);
 
 /* ------------------------------------------------------------------------
  * Physics entry points, in the step order of pp.f:483-821.
@@ -208,7 +228,8 @@ typedef struct tt_energy_stats {
 } tt_energy_stats;
 
 /* COUPLED_ENERGY_EQS (ppenergy.f:2-791).  iii is the step counter III (read
- * by IBDRAD=9).  ADJUST_SOR_PARAM (ppenergy.f:372) is never called. */
+ * by IBDRAD=9, and by the ADJUST_SOR_PARAM rule of ppenergy.f:372-374 when
+ * tt_solver_opts.adjust_omega is set; off by default). */
 int tt_coupled_energy(double delt, int iii, tt_energy_stats* st);
 
 /* SCHRITT (ppsub.f:1-137): CFL time step into *delt (the positivity check

@@ -7,7 +7,7 @@
 ! SETUP3DMPIGRID, GITTER) but takes the module switches from a namelist
 ! instead of ipp, loads the state from raw Fortran-order binaries
 ! (in_<name>.bin) and runs one command:
-!   gitter | bounds | advect | forgas | energy | sor | step | diffusion
+!   gitter | bounds | advect | forgas | energy | sor | adjust | step | diffusion
 ! then writes every array of interest to out_<name>.bin and the scalars to
 ! out_scalars.txt.  CALCMUSTAR lives in ppsub.f, which needs the absent hypre
 ! include; mu_star is therefore an input (in_mu_star.bin).
@@ -52,6 +52,7 @@ program tt_ref_harness
        h_zkapmult, h_tirr, h_fbottom, h_d_t, h_omrot, h_sorparam, h_epsmat, &
        h_fdelt, h_dtmax, h_delt, h_xnue, h_pr_num, h_prefill_kappa, cmd
   integer :: u, n, s
+  integer :: adj_iter(50)
 
   call get_command_argument(1, dir)
   h_nsteps = 1
@@ -166,6 +167,23 @@ program tt_ref_harness
      call load3(EVAL_1_6, 'EVAL_1_6'); call load3(RHS_ER, 'RHS_ER')
      call load2i(upperbnd, 'upperbnd')
      call SOLVE_WITH_SOR(.false.)
+  case ('adjust')
+     call load3(EVAL1, 'EVAL1'); call load3(EVAL2, 'EVAL2')
+     call load3(EVAL3, 'EVAL3'); call load3(EVAL4, 'EVAL4')
+     call load3(EVAL5, 'EVAL5'); call load3(EVAL6, 'EVAL6')
+     call load3(EVAL7, 'EVAL7'); call load3(EVAL8, 'EVAL8')
+     call load3(EVAL_1_6, 'EVAL_1_6'); call load3(RHS_ER, 'RHS_ER')
+     call load2i(upperbnd, 'upperbnd')
+     ! SOR_ITER at each omega ADJUST_SOR_PARAM tries (its own counts are
+     ! local to it), then the routine itself picks SORPARAM
+     SORPARAM = 1.01
+     do s = 1, 50
+        call SOLVE_WITH_SOR(.false.)
+        adj_iter(s) = SOR_ITER
+        SORPARAM = SORPARAM + 0.02
+     end do
+     SORPARAM = h_sorparam
+     call ADJUST_SOR_PARAM
   case ('step')
      do s = 1, h_nsteps
         III = h_iii + s - 1
@@ -223,6 +241,12 @@ program tt_ref_harness
   write(u, '(A,1X,ES25.17)') 'global_RHSNORM', global_RHSNORM
   write(u, '(A,1X,ES25.17)') 'FD', FD_c
   write(u, '(A,1X,ES25.17)') 'ARAD', ARAD_c
+  write(u, '(A,1X,ES25.17)') 'SORPARAM', SORPARAM
+  if (trim(cmd) == 'adjust') then
+     do s = 1, 50
+        write(u, '(A,I2.2,1X,I0)') 'ADJ_ITER_', s, adj_iter(s)
+     end do
+  end if
   close(u)
   call MPI_FINALIZE(ierr)
 

@@ -394,9 +394,12 @@ void tor_free(void)
     for (int m = 0; m < TT_M_COUNT; ++m) { free(O.m[m]); O.m[m] = NULL; }
 }
 
+static int g_adjust; /* ADJUST_SOR_PARAM inside COUPLED_ENERGY_EQS (off by default) */
+
 int tor_init(const tt_params* p)
 {
     tor_free();
+    g_adjust = 0;
     int nt = O.nthreads ? O.nthreads : 1;
     memset(&O, 0, sizeof O);
     O.nthreads = nt;
@@ -1124,6 +1127,34 @@ static int solve_with_sor(int fill, tt_sor_stats* st)
 
 int tor_solve_sor(int fill, tt_sor_stats* st) { return solve_with_sor(fill, st); }
 
+/* ADJUST_SOR_PARAM, ppsor.f:332-354: SORPARAM = 1.01 then += 0.02, fifty
+ * times (REAL literals: float values widened to double); every trial is a
+ * full SOLVE_WITH_SOR, which refills the matrix from the same EVALs; keep the
+ * first omega of minimal SOR_ITER (minloc). */
+int tor_adjust_sor_param(double* omega, int32_t* iters)
+{
+    int32_t cnt[50];
+    double om[50];
+    O.p.sorparam = (double)1.01f;
+    for (int t = 0; t < 50; ++t) {
+        tt_sor_stats ss;
+        solve_with_sor(1, &ss);
+        cnt[t] = ss.sor_iter;
+        om[t] = O.p.sorparam;
+        O.p.sorparam = O.p.sorparam + (double)0.02f;
+    }
+    int best = 0;
+    for (int t = 1; t < 50; ++t)
+        if (cnt[t] < cnt[best]) best = t;
+    O.p.sorparam = om[best];
+    if (omega) *omega = om[best];
+    if (iters) memcpy(iters, cnt, sizeof cnt);
+    return 0;
+}
+
+void tor_set_adjust(int on) { g_adjust = on; }
+double tor_get_sorparam(void) { return O.p.sorparam; }
+
 /* COUPLED_ENERGY_EQS, ppenergy.f:2-791 (IRAD=3, theta=1, conv_parm=0.1,
  * delta_max=0.005, NCOSYS=2, BCXMIN=12, no poles). */
 static void energy_bottom_bc(void) /* ppenergy.f:184-195 / :640-651 (BCXMIN=12) */
@@ -1246,6 +1277,8 @@ int tor_coupled_energy(double delt, int iii, tt_energy_stats* st)
                     O.RHS[n] = -O.FR[n] - O.JETA[n] * O.FQ[n];
                 }
         fill_evalues(theta, delt);
+        /* ppenergy.f:372-374 */
+        if (g_adjust && O.energy_iter == 1 && (iii % 10000 == 0 || iii == 1)) tor_adjust_sor_param(NULL, NULL);
         tt_sor_stats ss;
         int src = solve_with_sor(1, &ss);
         if (st) {

@@ -40,6 +40,8 @@ class Oracle:
                                              ctypes.POINTER(P.EnergyStats)]
             L.tor_step.argtypes = [ctypes.c_double, ctypes.c_int, ctypes.POINTER(P.StepStats)]
             L.tor_solve_sor.argtypes = [ctypes.c_int, ctypes.POINTER(P.SorStats)]
+            L.tor_adjust_sor_param.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.c_void_p]
+            L.tor_get_sorparam.restype = ctypes.c_double
             L.tor_schritt.argtypes = [ctypes.POINTER(ctypes.c_double)]
             L.tor_diffusion.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                         ctypes.POINTER(ctypes.c_int)]
@@ -106,6 +108,21 @@ class Oracle:
         st = P.SorStats()
         rc = self._check(self.L.tor_solve_sor(1 if fill else 0, ctypes.byref(st)), "SOLVE_WITH_SOR")
         return st, rc
+
+    def adjust_sor_param(self):
+        """ADJUST_SOR_PARAM on the current EVALs; returns (omega, counts)."""
+        om = ctypes.c_double()
+        it = np.zeros(50, np.int32)
+        self._check(self.L.tor_adjust_sor_param(ctypes.byref(om), it.ctypes.data_as(ctypes.c_void_p)),
+                    "ADJUST_SOR_PARAM")
+        return om.value, it
+
+    def set_adjust(self, on):
+        """Run ADJUST_SOR_PARAM inside COUPLED_ENERGY_EQS (ppenergy.f:372-374)."""
+        self.L.tor_set_adjust(1 if on else 0)
+
+    def get_sorparam(self):
+        return self.L.tor_get_sorparam()
 
     def schritt(self):
         d = ctypes.c_double()

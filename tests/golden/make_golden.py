@@ -239,8 +239,37 @@ def variants():
           "upperbnd", "SOR_A", "SOR_C"], {"delt": dt, "iii": 2})
 
 
+def adjust():
+    """ADJUST_SOR_PARAM (ppsor.f:332-354): the trial counts and the chosen
+    SORPARAM on the first Newton system of the constant-opacity energy case,
+    and COUPLED_ENERGY_EQS at III = 1, where the reference runs it before
+    the first solve (ppenergy.f:372-374)."""
+    nx, ny, nz = 12, 16, 12
+    p = P.make_params(nx, ny, nz, ikaptyp=1, kap0=1e-2)
+    o = Oracle(p)
+    o.load_state(base_inputs(p))
+    o.bounds()
+    o.pressg()
+    dt = o.schritt()
+    pe = P.make_params(nx, ny, nz, ikaptyp=1, kap0=1e-1, epsmat=1e-8, tirr=2000.0)
+    inp_e = base_inputs(pe, seed=11)
+    oe = Oracle(pe)
+    oe.load_state(inp_e)
+    oe.coupled_energy(dt, 2)
+    sor_in = {k: oe.download(k) for k in ("EVAL1", "EVAL2", "EVAL3", "EVAL4", "EVAL5", "EVAL6",
+                                          "EVAL7", "EVAL8", "EVAL_1_6", "RHS_ER", "upperbnd")}
+    inp_s = dict(inp_e, **sor_in)
+    outs, sc = run_ref("adjust", pe, inp_s, delt=dt)
+    save("adjust_12x16x12", pe, inp_s, outs, sc, ["delta_ER"])
+    outs, sc = run_ref("energy", pe, inp_e, delt=dt, iii=1)
+    save("energy_adjust_iii1_12x16x12", pe, inp_e, outs, sc, ["T", "ER", "delta_ER", "upperbnd"],
+         {"delt": dt, "iii": 1})
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["variants"]:
         variants()
+    elif sys.argv[1:] == ["adjust"]:
+        adjust()
     else:
         main()

@@ -140,6 +140,38 @@ def test_solve_with_sor(ttgpu):
     d.close()
 
 
+def test_adjust_sor_param(ttgpu):
+    """ADJUST_SOR_PARAM (ppsor.f:332-354): the 50 trial counts, the chosen
+    SORPARAM and the last trial's delta_ER against the reference's."""
+    g = C.load("adjust_12x16x12")
+    p, d = _dev(ttgpu, g)
+    for k in ("EVAL1", "EVAL2", "EVAL3", "EVAL4", "EVAL5", "EVAL6", "EVAL7", "EVAL8",
+              "EVAL_1_6", "RHS_ER", "upperbnd"):
+        d.upload(k, g["in_" + k])
+    om, it = d.adjust_sor_param(fill_matrix=True)
+    want = [int(g[f"scal_ADJ_ITER_{i:02d}"]) for i in range(1, 51)]
+    print("counts", list(it))
+    assert list(it) == want
+    assert om == float(g["scal_SORPARAM"])
+    assert d.get_sorparam() == om
+    _cmp(g, p, d, rel=1e-13)
+    d.close()
+
+
+def test_coupled_energy_adjusts_omega_at_step_one(ttgpu):
+    """With adjust_omega on, COUPLED_ENERGY_EQS at III = 1 retunes omega
+    before its first solve, as the reference does (ppenergy.f:372-374)."""
+    g = C.load("energy_adjust_iii1_12x16x12")
+    p, d = _dev(ttgpu, g)
+    d.set_solver(adjust_omega=1)
+    st, _ = d.coupled_energy(float(g["delt"]), int(g["iii"]))
+    assert st.energy_iter == int(g["scal_energy_iter"])
+    assert st.sor_iter_last == int(g["scal_SOR_ITER"])
+    assert d.get_sorparam() == float(g["scal_SORPARAM"])
+    _cmp(g, p, d, rel=1e-12, scaled_tol=1e-10)
+    d.close()
+
+
 @pytest.mark.parametrize("name", ["energy_const_12x16x12", "energy_pow8_12x16x12",
                                   "energy_flim1_sig_noirr_12x16x12"])
 def test_coupled_energy(ttgpu, name):

@@ -84,6 +84,49 @@ def test_solve_with_sor():
     _check_fields(g, p, o)
 
 
+def _load_system(o, g):
+    for k in ("EVAL1", "EVAL2", "EVAL3", "EVAL4", "EVAL5", "EVAL6", "EVAL7", "EVAL8",
+              "EVAL_1_6", "RHS_ER", "upperbnd"):
+        o.upload(k, g["in_" + k])
+
+
+def test_adjust_sor_param():
+    """ADJUST_SOR_PARAM (ppsor.f:332-354): every trial's SOR_ITER, the chosen
+    SORPARAM (the float-widened 1.01 + k 0.02 sequence) and delta_ER of the
+    last trial equal the reference's."""
+    def act(o, g):
+        _load_system(o, g)
+        return o.adjust_sor_param()
+    g, p, o, (om, it) = _run("adjust_12x16x12", act)
+    want = [int(g[f"scal_ADJ_ITER_{i:02d}"]) for i in range(1, 51)]
+    assert list(it) == want
+    assert om == float(g["scal_SORPARAM"])
+    assert o.get_sorparam() == om
+    # minloc picks the first minimum, and the tuned omega is not the input
+    assert om == (np.float64(np.float32(1.01)) + 3 * np.float64(np.float32(0.02)))
+    assert p.sorparam != om
+    _check_fields(g, p, o)
+
+
+def test_coupled_energy_adjusts_omega_at_step_one():
+    """COUPLED_ENERGY_EQS at III = 1 runs ADJUST_SOR_PARAM before its first
+    solve (ppenergy.f:372-374); the oracle does so when the switch is on."""
+    def act(o, g):
+        o.set_adjust(True)
+        return o.coupled_energy(float(g["delt"]), int(g["iii"]))[0]
+    g, p, o, st = _run("energy_adjust_iii1_12x16x12", act)
+    assert st.energy_iter == int(g["scal_energy_iter"])
+    assert st.sor_iter_last == int(g["scal_SOR_ITER"])
+    assert o.get_sorparam() == float(g["scal_SORPARAM"])
+    _check_fields(g, p, o)
+    # without the switch the same call keeps omega = 1.7 and takes another path
+    o2 = Oracle(p, threads=1)
+    o2.load_state(C.inputs_of(g))
+    st2, _ = o2.coupled_energy(float(g["delt"]), 1)
+    assert o2.get_sorparam() == p.sorparam
+    assert not np.array_equal(o2.download("T"), g["out_T"])
+
+
 @pytest.mark.parametrize("name", ["step2_12x16x12", "step2_ibdrad9_12x16x12"])
 def test_two_steps(name):
     def act(o, g):

@@ -200,6 +200,7 @@ int qdens_impl();
 int sound_impl();
 int planetshift_impl(double* a, int f);
 int sor_solve_impl(int fill_from_evals, tt_sor_stats* st, int fieldmode);
+int sor_adjust_impl(int fill_from_evals, double* omega_out, int32_t* iters);
 int sor_pack_impl();
 int diffusion_impl(double delt, double xnue, double pr, int* iters);
 int energy_impl(double delt, int iii, tt_energy_stats* st);

@@ -557,6 +557,11 @@ int energy_impl(double delt, int iii, tt_energy_stats* st) {
         long long nc = (long long)g.nx * g.ny * g.nz;
         k_newton_cell<<<nblk(nc, 256), 256, 0, L.s>>>(g, met(), ca);
         TT_LAUNCH();
+        // ADJUST_SOR_PARAM at the first Newton iteration of step 1 and of
+        // every 10000th step (ppenergy.f:372-374), when switched on
+        if (L.opts.adjust_omega && L.energy_iter == 1 && (iii % 10000 == 0 || iii == 1)) {
+            if ((rc = sor_adjust_impl(0, nullptr, nullptr))) return rc;
+        }
         // initial guess (new).  Later Newton iterations: the previous
         // correction scaled by warm_scale.  First Newton iteration: the
         // learned prediction (guess 2, when a network is loaded), else the

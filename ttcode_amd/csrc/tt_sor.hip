@@ -30,6 +30,7 @@
 // iterations between reads of the control block, so the iteration at which
 // the solve stops is exactly the first one that meets the tolerance.
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 #include <hip/hip_ext.h>
@@ -865,6 +866,45 @@ int sor_scale_guess(double sc) {
     return 0;
 }
 
+// ADJUST_SOR_PARAM, ppsor.f:332-354: fifty cold-start solves of the same
+// system with SORPARAM from 1.01 in steps of 0.02 (both REAL literals, so
+// the float values widened, the sum formed in double as SORPARAM is
+// double), keeping the first omega with the fewest iterations (minloc).
+int sor_adjust_impl(int fill_from_evals, double* omega_out, int32_t* iters) {
+    const int ntry = 50;
+    int32_t cnt[ntry];
+    double om[ntry];
+    const double omega0 = L.p.sorparam;
+    const int guess = L.opts.guess, have_guess = L.have_prev_guess;
+    L.opts.guess = 0;  // SOLVE_WITH_SOR starts from delta_ER = 0 (ppsor.f:29)
+    L.have_prev_guess = 0;
+    double w = (double)1.01f;
+    int rc = 0;
+    for (int t = 0; t < ntry; ++t) {
+        L.p.sorparam = w;
+        tt_sor_stats s{};
+        rc = sor_solve_impl(t == 0 ? fill_from_evals : 0, &s, 0);
+        if (rc > 0) break;
+        rc = 0;  // an iteration cap only counts ITMAX iterations
+        cnt[t] = s.sor_iter;
+        om[t] = w;
+        w = w + (double)0.02f;
+    }
+    L.opts.guess = guess;
+    L.have_prev_guess = have_guess;
+    if (rc > 0) {
+        L.p.sorparam = omega0;
+        return rc;
+    }
+    int best = 0;
+    for (int t = 1; t < ntry; ++t)
+        if (cnt[t] < cnt[best]) best = t;
+    L.p.sorparam = om[best];
+    if (omega_out) *omega_out = om[best];
+    if (iters) memcpy(iters, cnt, sizeof cnt);
+    return 0;
+}
+
 }  // namespace tt
 
 using namespace tt;
@@ -877,6 +917,22 @@ int tt_solve_sor(int fill_matrix, tt_sor_stats* st) {
     int rc2 = sync_check();
     return rc2 ? rc2 : rc;
 }
+
+int tt_adjust_sor_param(int fill_matrix, double* omega, int32_t* iters) {
+    if (!L.ready) return fail(1, "tt_init has not been called");
+    int rc = sor_adjust_impl(fill_matrix, omega, iters);
+    int rc2 = sync_check();
+    return rc2 ? rc2 : rc;
+}
+
+int tt_set_sorparam(double omega) {
+    if (!L.ready) return fail(1, "tt_init has not been called");
+    if (!(omega > 0.0 && omega < 2.0)) return fail(1, "SORPARAM must lie in (0, 2)");
+    L.p.sorparam = omega;
+    return 0;
+}
+
+double tt_get_sorparam(void) { return L.ready ? L.p.sorparam : 0.0; }
 
 int tt_sweep_timing(int period) {
     if (L.s) hipStreamSynchronize(L.s);

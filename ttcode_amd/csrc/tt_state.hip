@@ -306,6 +306,7 @@ int tt_init(const tt_params* p) {
     L.opts.guess = 0;
     L.opts.batch = 16;
     L.opts.inner_max = 4000;
+    L.opts.adjust_omega = 0;
     L.opts.inner_tol = 1e-5;
     L.opts.warm_scale = 0.9;
     TT_HIP(hipStreamSynchronize(L.s));
@@ -343,7 +344,8 @@ int tt_sync(void) {
 
 int tt_set_solver(const tt_solver_opts* o) {
     if (!o) return fail(1, "null options");
-    if (o->precision < 0 || o->precision > 1 || o->guess < 0 || o->guess > 2 || o->batch < 1)
+    if (o->precision < 0 || o->precision > 1 || o->guess < 0 || o->guess > 2 || o->batch < 1 ||
+        o->adjust_omega < 0 || o->adjust_omega > 1)
         return fail(1, "invalid solver options");
     L.opts = *o;
     return 0;

@@ -46,6 +46,9 @@ def load_library():
     L.tt_set_solver.argtypes = [ctypes.POINTER(P.SolverOpts)]
     L.tt_get_solver.argtypes = [ctypes.POINTER(P.SolverOpts)]
     L.tt_sweep_timing.argtypes = [ctypes.c_int]
+    L.tt_adjust_sor_param.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_void_p]
+    L.tt_set_sorparam.argtypes = [ctypes.c_double]
+    L.tt_get_sorparam.restype = ctypes.c_double
     L.tt_device_bytes.restype = ctypes.c_int64
     L.tt_sweep_timing_get.argtypes = [ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double),
                                       ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double)]
@@ -116,9 +119,25 @@ class Device:
                 self.upload(P.FIELD_IDS[name], a)
 
     # ---- solver options
-    def set_solver(self, precision=0, guess=0, batch=16, inner_max=4000, inner_tol=1e-5, warm_scale=0.9):
-        o = P.SolverOpts(precision, guess, batch, inner_max, inner_tol, warm_scale)
+    def set_solver(self, precision=0, guess=0, batch=16, inner_max=4000, inner_tol=1e-5, warm_scale=0.9,
+                   adjust_omega=0):
+        o = P.SolverOpts(precision, guess, batch, inner_max, adjust_omega, 0, inner_tol, warm_scale)
         self._check(self.L.tt_set_solver(ctypes.byref(o)), "tt_set_solver")
+
+    def adjust_sor_param(self, fill_matrix=False):
+        """ADJUST_SOR_PARAM (ppsor.f:332-354); returns (omega, iterations
+        of the 50 trial solves)."""
+        om = ctypes.c_double()
+        it = np.zeros(50, np.int32)
+        self._check(self.L.tt_adjust_sor_param(1 if fill_matrix else 0, ctypes.byref(om), it.ctypes.data),
+                    "ADJUST_SOR_PARAM")
+        return om.value, it
+
+    def set_sorparam(self, omega):
+        self._check(self.L.tt_set_sorparam(float(omega)), "tt_set_sorparam")
+
+    def get_sorparam(self):
+        return self.L.tt_get_sorparam()
 
     # ---- the reference's routines
     def bounds(self):
