@@ -241,13 +241,47 @@ int tt_schritt(double* delt);
  * *iters.  ITMIN=5, SORITMAX=500 as in ppdif.f:14-15. */
 int tt_diffusion(double delt, double xnue, double pr_num, int* iters);
 
+/* ------------------------------------------------------------------------
+ * Clamps of the step loop (pp.f:543-562, 673-678, 708-710): VARLIMIT
+ * (ppsub.f:140-154) and, for NCOSYS=2, SPHERICALVELLIMIT (ppsub.f:316-391),
+ * switched by IRHO, ITEMP and IVEL.  All off by default (tt_init).
+ * ------------------------------------------------------------------------ */
+typedef struct tt_limits {
+    int32_t irho;            /* != 0: RH into [rhmin, 1e10] after ADVECT3D     */
+    int32_t itemp;           /* != 0: T into [tempmin, tempmax] before
+                                ADVECT3D, after ADVECT3D, after FORGAS3D       */
+    int32_t ivel;            /* 0 off; 1-4: VARLIMIT of V to +-velmax before
+                                ADVECT3D, then for 1 and 2 SPHERICALVELLIMIT
+                                (2 ramps the ceiling over 25 days)             */
+    int32_t pad0;
+    double rhmin, tempmax, tempmin, velmax;
+    double hydrotime;        /* model time seen by the next tt_step's clamps
+                                (ivel 2); each tt_step then adds delt, as
+                                pp.f:637-639 does for IADV > 0                 */
+} tt_limits;
+
+int tt_set_limits(const tt_limits* l);
+int tt_get_limits(tt_limits* l);
+
+/* VARLIMIT(ilimit, feld, varmin, varmax), ppsub.f:140-154: every element of
+ * the field's declared array (TT_T, TT_RH, TT_V, TT_G or TT_H) set to
+ * max(min(x, varmax), varmin). */
+int tt_varlimit(int field, double varmin, double varmax);
+
+/* SPHERICALVELLIMIT, ppsub.f:316-391 (NCOSYS=2, no poles) for ivel 1-4:
+ * for 1 and 2, G into +-ceiling/(r cos(lat)) and H into +-ceiling/r; then V
+ * into +-ceiling.  The ceiling is velmax, or for ivel 2 and 4 the ramp
+ * velmax sin^4(pi/2 t/(25 days)) while 0 <= t <= 25 days (t = hydrotime). */
+int tt_sphericalvellimit(int ivel, double velmax, double hydrotime);
+
 typedef struct tt_step_stats {
     tt_energy_stats energy;
     double ms_advect, ms_force, ms_energy, ms_total;  /* device time, ms */
 } tt_step_stats;
 
 /* One step of the pp.f:483-821 loop for IADV=IFOR=1, IRAD=3: ADVECT3D,
- * FORGAS3D, COUPLED_ENERGY_EQS, BOUNDS, BOUNDS, PRESSG (time step held fixed). */
+ * FORGAS3D, COUPLED_ENERGY_EQS, BOUNDS, BOUNDS, PRESSG (time step held fixed),
+ * with the clamps tt_set_limits switched on at their places in that loop. */
 int tt_step(double delt, int iii, tt_step_stats* st);
 
 /* Block until all queued device work is done. */

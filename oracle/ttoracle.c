@@ -395,11 +395,13 @@ void tor_free(void)
 }
 
 static int g_adjust; /* ADJUST_SOR_PARAM inside COUPLED_ENERGY_EQS (off by default) */
+static tt_limits g_lim; /* clamps of the step loop (off by default) */
 
 int tor_init(const tt_params* p)
 {
     tor_free();
     g_adjust = 0;
+    memset(&g_lim, 0, sizeof g_lim);
     int nt = O.nthreads ? O.nthreads : 1;
     memset(&O, 0, sizeof O);
     O.nthreads = nt;
@@ -1924,13 +1926,106 @@ int tor_diffusion(double delt, double xnue, double pr_num, int* iters) /* DIFFUS
 }
 
 /* One step of the pp.f:483-821 loop for the configs' switches. */
+/* VARLIMIT, ppsub.f:140-154: feld = min(feld, varmax); feld = max(feld,
+ * varmin) over every element of the actual array (its declared bounds). */
+int tor_varlimit(int f, double vmin, double vmax)
+{
+    double* a;
+    switch (f) {
+    case TT_T: a = O.T; break;
+    case TT_RH: a = O.RH; break;
+    case TT_V: a = O.V; break;
+    case TT_G: a = O.G; break;
+    case TT_H: a = O.H; break;
+    default: return seterr(1, "VARLIMIT: field must be T, RH, V, G or H");
+    }
+    int32_t lo[3], hi[3], rank, isint;
+    tt_field_bounds(f, NX, NY, NZ, lo, hi, &rank, &isint);
+    for (int k = lo[2]; k <= hi[2]; ++k)
+        for (int j = lo[1]; j <= hi[1]; ++j)
+            for (int i = lo[0]; i <= hi[0]; ++i) {
+                double x = A(a, i, j, k);
+                if (!(x < vmax)) x = vmax;
+                if (!(x > vmin)) x = vmin;
+                A(a, i, j, k) = x;
+            }
+    return 0;
+}
+
+/* SPHERICALVELLIMIT, ppsub.f:316-391 (NCOSYS=2, no poles). */
+int tor_sphericalvellimit(int ivel, double velmax, double hydrotime)
+{
+    const double day = (double)8.64e4f; /* DAY = 8.64 E 4, a REAL constant (ppmod.f:1083) */
+    const double pi = 3.141592653589793;
+    double varmin, varmax;
+    if (ivel == 1 || ivel == 3) {
+        varmin = -velmax;
+        varmax = velmax;
+    } else if (ivel == 2 || ivel == 4) {
+        double time_start = 0.0 * day, damp_time = 25.0 * day, veldamp_start = 0.0;
+        if (hydrotime >= time_start && hydrotime <= time_start + damp_time) {
+            varmax = (velmax - veldamp_start) *
+                         pow(sin((pi / 2.0) * (hydrotime - time_start) / damp_time), (double)4.0f) +
+                     veldamp_start;
+            varmin = -varmax;
+        } else if (hydrotime > time_start + damp_time) {
+            varmax = velmax;
+            varmin = -velmax;
+        } else {
+            varmax = veldamp_start;
+            varmin = -veldamp_start;
+        }
+    } else {
+        return seterr(1, "SPHERICALVELLIMIT: ivel must be 1, 2, 3 or 4");
+    }
+    if (ivel == 1 || ivel == 2) {
+        for (int k = -1; k <= NZ + 2; ++k)
+            for (int j = -1; j <= NY + 2; ++j)
+                for (int i = -1; i <= NX + 2; ++i) {
+                    if (j != -1) {
+                        double g = A(O.G, i, j, k);
+                        double c = varmax / (XXB(i) * GEOZG(k));
+                        if (!(g < c)) g = c;
+                        c = varmin / (XXB(i) * GEOZG(k));
+                        if (!(g > c)) g = c;
+                        A(O.G, i, j, k) = g;
+                    }
+                    if (k != -1) {
+                        double h = A(O.H, i, j, k);
+                        double c = varmax / XXB(i);
+                        if (!(h < c)) h = c;
+                        c = varmin / XXB(i);
+                        if (!(h > c)) h = c;
+                        A(O.H, i, j, k) = h;
+                    }
+                }
+    }
+    return tor_varlimit(TT_V, varmin, varmax);
+}
+
+void tor_set_limits(const tt_limits* l) { g_lim = *l; }
+void tor_get_limits(tt_limits* l) { *l = g_lim; }
+
 int tor_step(double delt, int iii, tt_step_stats* st)
 {
     int rc;
     if (st) memset(st, 0, sizeof *st);
     g_iii = iii;
+    /* pp.f:543-562: velocity and temperature limits, then the time advance
+     * (:637-639) */
+    if (g_lim.ivel != 0) {
+        if ((rc = tor_varlimit(TT_V, -g_lim.velmax, g_lim.velmax))) return rc;
+        if (g_lim.ivel < 3 && (rc = tor_sphericalvellimit(g_lim.ivel, g_lim.velmax, g_lim.hydrotime))) return rc;
+    }
+    if (g_lim.itemp != 0 && (rc = tor_varlimit(TT_T, g_lim.tempmin, g_lim.tempmax))) return rc;
+    g_lim.hydrotime = g_lim.hydrotime + delt;
     if ((rc = tor_advect3d(delt))) return rc;
+    /* pp.f:673-678 */
+    if (g_lim.irho != 0 && (rc = tor_varlimit(TT_RH, g_lim.rhmin, 1.0e10))) return rc;
+    if (g_lim.itemp != 0 && (rc = tor_varlimit(TT_T, g_lim.tempmin, g_lim.tempmax))) return rc;
     if ((rc = tor_forgas3d(delt))) return rc;
+    /* pp.f:708-710 */
+    if (g_lim.itemp != 0 && (rc = tor_varlimit(TT_T, g_lim.tempmin, g_lim.tempmax))) return rc;
     rc = tor_coupled_energy(delt, iii, st ? &st->energy : NULL);
     if (rc > 0) return rc;
     int rc2 = tor_bounds();

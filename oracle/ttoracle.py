@@ -42,6 +42,10 @@ class Oracle:
             L.tor_solve_sor.argtypes = [ctypes.c_int, ctypes.POINTER(P.SorStats)]
             L.tor_adjust_sor_param.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.c_void_p]
             L.tor_get_sorparam.restype = ctypes.c_double
+            L.tor_varlimit.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double]
+            L.tor_sphericalvellimit.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double]
+            L.tor_set_limits.argtypes = [ctypes.POINTER(P.Limits)]
+            L.tor_get_limits.argtypes = [ctypes.POINTER(P.Limits)]
             L.tor_schritt.argtypes = [ctypes.POINTER(ctypes.c_double)]
             L.tor_diffusion.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                         ctypes.POINTER(ctypes.c_int)]
@@ -123,6 +127,24 @@ class Oracle:
 
     def get_sorparam(self):
         return self.L.tor_get_sorparam()
+
+    def set_limits(self, irho=0, rhmin=0.0, itemp=0, tempmin=0.0, tempmax=0.0, ivel=0, velmax=0.0,
+                   hydrotime=0.0):
+        self.L.tor_set_limits(ctypes.byref(P.Limits(irho, itemp, ivel, 0, rhmin, tempmax, tempmin, velmax,
+                                                    hydrotime)))
+
+    def get_limits(self):
+        lim = P.Limits()
+        self.L.tor_get_limits(ctypes.byref(lim))
+        return lim
+
+    def varlimit(self, field, vmin, vmax):
+        if isinstance(field, str):
+            field = P.FIELD_IDS[field]
+        return self._check(self.L.tor_varlimit(field, vmin, vmax), "VARLIMIT")
+
+    def sphericalvellimit(self, ivel, velmax, hydrotime=0.0):
+        return self._check(self.L.tor_sphericalvellimit(ivel, velmax, hydrotime), "SPHERICALVELLIMIT")
 
     def schritt(self):
         d = ctypes.c_double()

@@ -1,0 +1,142 @@
+"""VARLIMIT (ppsub.f:140-154) and SPHERICALVELLIMIT (ppsub.f:316-391) in the
+CPU oracle.  ppsub.f does not build here (it includes hypre's absent
+bHYPRE_SStructVariable.inc), so there is no reference fixture; the clamps are
+pinned by what they must satisfy: every element of the declared array lands
+in [varmin, varmax], elements inside are untouched, the ceiling scales as
+1/(r cos(lat)) for G and 1/r for H, and the ivel = 2 ramp equals
+velmax sin^4(pi/2 t/25 d), i.e. velmax/4 at 12.5 days."""
+import numpy as np
+import pytest
+
+from oracle.ttoracle import Oracle
+from ttcode_amd import params as P
+
+DAY = 86400.0
+
+
+@pytest.fixture()
+def o():
+    p = P.make_params(12, 16, 12)
+    orc = Oracle(p)
+    orc.load_state(P.generate_state(p, seed=5))
+    return orc
+
+
+def test_varlimit_clamps_every_declared_element(o):
+    t0 = o.download("T")
+    lo, hi = np.quantile(t0, 0.2), np.quantile(t0, 0.8)
+    o.varlimit("T", lo, hi)
+    t1 = o.download("T")
+    assert t1.min() == lo and t1.max() == hi
+    inside = (t0 >= lo) & (t0 <= hi)
+    assert np.array_equal(t1[inside], t0[inside])
+    assert np.all(t1[t0 < lo] == lo) and np.all(t1[t0 > hi] == hi)
+
+
+def test_varlimit_min_then_max(o):
+    """feld = min(feld, varmax) then max(feld, varmin): with varmin > varmax
+    everything ends at varmin."""
+    o.varlimit("RH", 2.0, 1.0)
+    assert np.all(o.download("RH") == 2.0)
+
+
+def test_varlimit_staggered_field(o):
+    v0 = o.download("V")
+    c = 0.5 * np.abs(v0).max()
+    o.varlimit("V", -c, c)
+    v1 = o.download("V")
+    assert np.abs(v1).max() == c
+    assert np.array_equal(np.clip(v0, -c, c), v1)
+
+
+def _ceilings(o):
+    nx, ny, nz = o.p.nx, o.p.ny, o.p.nz
+    xxb = o.metric(1)      # (-1:nx+2)
+    xzb = o.metric(9)      # (-1:nz+2)
+    rcos = xxb[None, None, :] * np.cos(xzb)[:, None, None]   # (k, j, i) over the box
+    return xxb, rcos
+
+
+@pytest.mark.parametrize("ivel", [1, 2])
+def test_sphericalvellimit_scales_with_geometry(o, ivel):
+    velmax = 0.3 * np.abs(o.download("V")).max()
+    g0, h0, v0 = o.download("G"), o.download("H"), o.download("V")
+    o.sphericalvellimit(ivel, velmax, hydrotime=30 * DAY)   # ramp finished: ceiling velmax
+    g1, h1, v1 = o.download("G"), o.download("H"), o.download("V")
+    xxb, rcos = _ceilings(o)
+    gc = velmax / rcos                      # broadcast over j (G declared from j = 0)
+    hc = velmax / xxb[None, None, :]
+    assert np.all(np.abs(g1) <= gc * (1 + 1e-15))
+    assert np.all(np.abs(h1[...]) <= hc * (1 + 1e-15))
+    assert np.abs(v1).max() <= velmax
+    # untouched where already inside; clipped values sit on the ceiling
+    gc = np.broadcast_to(gc, g0.shape)
+    hc = np.broadcast_to(hc, h0.shape)
+    gi = np.abs(g0) < gc * (1 - 1e-12)
+    assert np.array_equal(g1[gi], g0[gi])
+    go = g0 > gc * (1 + 1e-12)
+    assert go.any() and np.allclose(g1[go], gc[go], rtol=1e-15, atol=0)
+    hi = np.abs(h0) < hc * (1 - 1e-12)
+    assert np.array_equal(h1[hi], h0[hi])
+    assert np.array_equal(v1, np.clip(v0, -velmax, velmax))
+
+
+def test_sphericalvellimit_v_only(o):
+    g0, h0 = o.download("G"), o.download("H")
+    o.sphericalvellimit(3, 1.0)
+    assert np.array_equal(o.download("G"), g0) and np.array_equal(o.download("H"), h0)
+    assert np.abs(o.download("V")).max() <= 1.0
+
+
+@pytest.mark.parametrize("t,frac", [(12.5 * DAY, 0.25), (25 * DAY, 1.0), (40 * DAY, 1.0), (-1.0, 0.0)])
+def test_ramp_closed_form(o, t, frac):
+    """ivel = 4 ramps V's ceiling: velmax sin^4(pi/2 t/25d) while 0 <= t <= 25 d
+    (sin(pi/4)^4 = 1/4), velmax after, 0 before."""
+    velmax = 0.5 * np.abs(o.download("V")).max()
+    o.sphericalvellimit(4, velmax, hydrotime=t)
+    got = np.abs(o.download("V")).max()
+    assert got == pytest.approx(frac * velmax, rel=1e-14, abs=0.0)
+
+
+def test_step_applies_limits_in_loop_order():
+    """tt_step's clamps sit where pp.f's loop has them: V and T before
+    ADVECT3D, RH and T after it, T after FORGAS3D; hydrotime advances by dt."""
+    p = P.make_params(12, 16, 12, ikaptyp=1, kap0=1e-1, epsmat=1e-8, tirr=2000.0)
+    st = P.generate_state(p, seed=17)
+    o = Oracle(p)
+    o.load_state(st)
+    o.bounds()
+    o.pressg()
+    dt = o.schritt()
+    t = st["T"]
+    lim = dict(irho=1, rhmin=float(np.quantile(st["RH"], 0.3)), itemp=1,
+               tempmin=float(np.quantile(t, 0.1)), tempmax=float(np.quantile(t, 0.9)),
+               ivel=1, velmax=0.5 * float(np.abs(st["V"]).max()), hydrotime=3.0)
+    keys = ("T", "RH", "V", "G", "H", "ER")
+    # the oracle holds one state: run the three cases one after another
+    a = Oracle(p)
+    a.load_state(st)
+    a.set_limits(**lim)
+    a.step(dt, 2)
+    assert a.get_limits().hydrotime == 3.0 + dt
+    got = {k: a.download(k) for k in keys}
+    b = Oracle(p)
+    b.load_state(st)
+    b.varlimit("V", -lim["velmax"], lim["velmax"])
+    b.sphericalvellimit(1, lim["velmax"], 3.0)
+    b.varlimit("T", lim["tempmin"], lim["tempmax"])
+    b.advect3d(dt)
+    b.varlimit("RH", lim["rhmin"], 1e10)
+    b.varlimit("T", lim["tempmin"], lim["tempmax"])
+    b.forgas3d(dt)
+    b.varlimit("T", lim["tempmin"], lim["tempmax"])
+    b.coupled_energy(dt, 2)
+    b.bounds()
+    b.bounds()
+    b.pressg()
+    for k in keys:
+        assert np.array_equal(got[k], b.download(k)), k
+    c = Oracle(p)
+    c.load_state(st)
+    c.step(dt, 2)
+    assert not np.array_equal(c.download("T"), got["T"])

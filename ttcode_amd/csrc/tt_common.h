@@ -92,6 +92,7 @@ struct Lib {
     Ctrl* ctrl = nullptr;           // device
     Ctrl* hctrl = nullptr;          // pinned host mirror
     tt_solver_opts opts{};
+    tt_limits lim{};                // clamps applied by tt_step
     std::string err;
     // sweep timing
     bool timing = false;
@@ -201,6 +202,7 @@ int sound_impl();
 int planetshift_impl(double* a, int f);
 int sor_solve_impl(int fill_from_evals, tt_sor_stats* st, int fieldmode);
 int sor_adjust_impl(int fill_from_evals, double* omega_out, int32_t* iters);
+int limits_impl(int stage, double delt);
 int sor_pack_impl();
 int diffusion_impl(double delt, double xnue, double pr, int* iters);
 int energy_impl(double delt, int iii, tt_energy_stats* st);

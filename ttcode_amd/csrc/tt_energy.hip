@@ -204,6 +204,25 @@ __global__ void k_face_x(Geo g, Met m, FaceArgs a) {
     a.FLUX[c] = fx;
 }
 
+// The CALCULATEFLUX + FLUX_BC pair before the Newton loop (ppenergy.f:78-79)
+// feeds only the top-boundary blocks, which read flux_x at i >= upperbnd and,
+// through the i = nx+2 alias, at i = 1: faces FLUX_BC fixes to c E(upperbnd-1)/2
+// and Fbottom (ppenergy.f:947,966-967,976).  Those values are written here,
+// per interior column, with the expressions k_face_x uses.  Every other
+// flux, limiter and diffusion coefficient that call forms is overwritten by
+// the Newton loop's first call before it is read; the ghost columns the alias
+// reaches are rewritten by the theta blocks and the phi wrap.
+__global__ void k_flux_x_top(Geo g, const double* __restrict__ ER, const int32_t* __restrict__ ub, double* FX,
+                             double fbottom) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.ny * g.nz) return;
+    int j = t % g.ny + 1, k = t / g.ny + 1;
+    int u = ub[g.idx2(j, k)];
+    double top = FD * ER[IX(u - 1, j, k)] / 2.0;
+    for (int i = u > 1 ? u : 2; i <= g.nx + 1; ++i) FX[IX(i, j, k)] = top;
+    FX[IX(1, j, k)] = fbottom;
+}
+
 // y faces j = 1..ny+1 (FLIMY uses the x loop's last xmax_index, pprphy.f:464)
 __global__ void k_face_y(Geo g, Met m, FaceArgs a) {
     long long n = (long long)g.nx * (g.ny + 1) * g.nz;
@@ -538,7 +557,9 @@ int energy_impl(double delt, int iii, tt_energy_stats* st) {
     if ((rc = kappa_impl())) return rc;
     if ((rc = qdens_impl())) return rc;
     if (L.p.f_incident > 1 && (rc = stellar_impl())) return rc;
-    if ((rc = flux_impl())) return rc;
+    if (L.p.iflimtyp != 1 && L.p.iflimtyp != 2) return fail(1, "FLIM: IFLIMTYP must be 1 or 2");
+    k_flux_x_top<<<nblk((long long)g.ny * g.nz, 256), 256, 0, L.s>>>(g, f[TT_ER], L.ub, f[TT_FLUX_X], L.p.fbottom);
+    TT_LAUNCH();
     if ((rc = energy_bcs(0))) return rc;
     if ((rc = planetshift_impl(f[TT_ER], 6))) return rc;
     TT_HIP(hipMemcpyAsync(f[TT_ER_OLD], f[TT_ER], (size_t)g.n3 * sizeof(double), hipMemcpyDeviceToDevice, L.s));

@@ -471,11 +471,15 @@ int tt_step(double delt, int iii, tt_step_stats* st) {
     for (auto& x : e) TT_HIP(hipEventCreate(&x));
     L.iii = iii;
     TT_HIP(hipEventRecord(e[0], L.s));
-    int rc = advect_impl(delt);
+    int rc = limits_impl(0, delt);
     if (rc) return rc;
+    rc = advect_impl(delt);
+    if (rc) return rc;
+    if ((rc = limits_impl(1, delt))) return rc;
     TT_HIP(hipEventRecord(e[1], L.s));
     rc = forgas_impl(delt);
     if (rc) return rc;
+    if ((rc = limits_impl(2, delt))) return rc;
     TT_HIP(hipEventRecord(e[2], L.s));
     int erc = energy_impl(delt, iii, &s.energy);
     if (erc > 0) return erc;

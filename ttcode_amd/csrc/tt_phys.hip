@@ -371,11 +371,148 @@ int diffusion_impl(double delt, double xnue, double pr, int* iters) {
     return rc;
 }
 
+// VARLIMIT (ppsub.f:140-154) over a declared sub-box of a ghosted field
+__global__ void k_clamp(Geo g, double* a, int lo0, int lo1, int lo2, int n0, int n1, long long n, double vmin,
+                        double vmax) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    int i = (int)(t % n0) + lo0;
+    long long r = t / n0;
+    int j = (int)(r % n1) + lo1, k = (int)(r / n1) + lo2;
+    long long c = g.idx(i, j, k);
+    double x = a[c];
+    x = x < vmax ? x : vmax;
+    x = x > vmin ? x : vmin;
+    a[c] = x;
+}
+
+// the G and H loops of SPHERICALVELLIMIT (ppsub.f:364-384), no poles: over
+// the whole box, G for j /= -1 and H for k /= -1 (their declared ranges)
+__global__ void k_vel_sph(Geo g, Met m, double* G, double* H, double vmin, double vmax) {
+    long long n = (long long)(g.nx + 4) * (g.ny + 4) * (g.nz + 4);
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    int i, j, k;
+    sub_ijk(t, g.nx + 4, g.ny + 4, -1, -1, -1, i, j, k);
+    long long c = g.idx(i, j, k);
+    double xxb = m(TT_M_XXB, i);
+    if (j != -1) {
+        double x = G[c];
+        double top = vmax / (xxb * m(TT_M_GEOZG, k));  // geozg = cos(xzb), ppinit.f:1396
+        x = x < top ? x : top;
+        double bot = vmin / (xxb * m(TT_M_GEOZG, k));
+        x = x > bot ? x : bot;
+        G[c] = x;
+    }
+    if (k != -1) {
+        double x = H[c];
+        double top = vmax / xxb;
+        x = x < top ? x : top;
+        double bot = vmin / xxb;
+        x = x > bot ? x : bot;
+        H[c] = x;
+    }
+}
+
+static int varlimit_impl(int f, double vmin, double vmax) {
+    if (f != TT_T && f != TT_RH && f != TT_V && f != TT_G && f != TT_H)
+        return fail(1, "VARLIMIT: field must be T, RH, V, G or H");
+    Geo& g = L.g;
+    int32_t lo[3], hi[3], rank = 0, isint = 0;
+    tt_field_bounds(f, g.nx, g.ny, g.nz, lo, hi, &rank, &isint);
+    int n0 = hi[0] - lo[0] + 1, n1 = hi[1] - lo[1] + 1;
+    long long n = (long long)n0 * n1 * (hi[2] - lo[2] + 1);
+    k_clamp<<<nblk(n, 256), 256, 0, L.s>>>(g, L.f[f], lo[0], lo[1], lo[2], n0, n1, n, vmin, vmax);
+    TT_LAUNCH();
+    return 0;
+}
+
+// SPHERICALVELLIMIT's ceiling (ppsub.f:338-363): DAY = 8.64E4 is a REAL
+// constant (ppmod.f:1083, exact in float); TIME_START = VELDAMP_START = 0
+static int sphericalvellimit_impl(int ivel, double velmax, double hydrotime) {
+    double vmin, vmax;
+    if (ivel == 1 || ivel == 3) {
+        vmin = -velmax;
+        vmax = velmax;
+    } else if (ivel == 2 || ivel == 4) {
+        const double day = (double)8.64e4f, time_start = 0.0 * day, damp_time = 25.0 * day, start = 0.0;
+        const double pi = 3.141592653589793;
+        if (hydrotime >= time_start && hydrotime <= time_start + damp_time) {
+            vmax = (velmax - start) * pow(sin((pi / 2.0) * (hydrotime - time_start) / damp_time), (double)4.0f) +
+                   start;
+            vmin = -vmax;
+        } else if (hydrotime > time_start + damp_time) {
+            vmax = velmax;
+            vmin = -velmax;
+        } else {
+            vmax = start;
+            vmin = -start;
+        }
+    } else {
+        return fail(1, "SPHERICALVELLIMIT: ivel must be 1, 2, 3 or 4");
+    }
+    if (ivel == 1 || ivel == 2) {
+        Geo& g = L.g;
+        long long n = (long long)(g.nx + 4) * (g.ny + 4) * (g.nz + 4);
+        k_vel_sph<<<nblk(n, 256), 256, 0, L.s>>>(g, met(), L.f[TT_G], L.f[TT_H], vmin, vmax);
+        TT_LAUNCH();
+    }
+    return varlimit_impl(TT_V, vmin, vmax);
+}
+
+// the clamps of pp.f's step loop: stage 0 before ADVECT3D (velocity,
+// :543-557; temperature, :559-562; then the time advance, :637-639),
+// stage 1 after ADVECT3D (:673-678), stage 2 after FORGAS3D (:708-710)
+int limits_impl(int stage, double delt) {
+    const tt_limits& l = L.lim;
+    int rc = 0;
+    if (stage == 0) {
+        if (l.ivel != 0) {
+            if ((rc = varlimit_impl(TT_V, -l.velmax, l.velmax))) return rc;
+            if (l.ivel < 3 && (rc = sphericalvellimit_impl(l.ivel, l.velmax, l.hydrotime))) return rc;
+        }
+        if (l.itemp != 0 && (rc = varlimit_impl(TT_T, l.tempmin, l.tempmax))) return rc;
+        L.lim.hydrotime = l.hydrotime + delt;
+    } else if (stage == 1) {
+        if (l.irho != 0 && (rc = varlimit_impl(TT_RH, l.rhmin, 1.0e10))) return rc;
+        if (l.itemp != 0 && (rc = varlimit_impl(TT_T, l.tempmin, l.tempmax))) return rc;
+    } else {
+        if (l.itemp != 0 && (rc = varlimit_impl(TT_T, l.tempmin, l.tempmax))) return rc;
+    }
+    return 0;
+}
+
 }  // namespace tt
 
 using namespace tt;
 
 extern "C" {
+
+int tt_set_limits(const tt_limits* l) {
+    if (!L.ready) return fail(1, "tt_init has not been called");
+    if (!l) return fail(1, "null limits");
+    if (l->ivel < 0 || l->ivel > 4) return fail(1, "IVEL must be 0, 1, 2, 3 or 4");
+    L.lim = *l;
+    return 0;
+}
+
+int tt_get_limits(tt_limits* l) {
+    if (!l) return fail(1, "null limits");
+    *l = L.lim;
+    return 0;
+}
+
+int tt_varlimit(int field, double varmin, double varmax) {
+    if (!L.ready) return fail(1, "tt_init has not been called");
+    int rc = varlimit_impl(field, varmin, varmax);
+    return rc ? rc : sync_check();
+}
+
+int tt_sphericalvellimit(int ivel, double velmax, double hydrotime) {
+    if (!L.ready) return fail(1, "tt_init has not been called");
+    int rc = sphericalvellimit_impl(ivel, velmax, hydrotime);
+    return rc ? rc : sync_check();
+}
 
 int tt_bounds(void) {
     if (!L.ready) return fail(1, "tt_init has not been called");

@@ -66,6 +66,14 @@ class SolverOpts(ctypes.Structure):
                 ("inner_tol", ctypes.c_double), ("warm_scale", ctypes.c_double)]
 
 
+class Limits(ctypes.Structure):
+    """``tt_limits``."""
+
+    _fields_ = [("irho", ctypes.c_int32), ("itemp", ctypes.c_int32), ("ivel", ctypes.c_int32),
+                ("pad0", ctypes.c_int32), ("rhmin", ctypes.c_double), ("tempmax", ctypes.c_double),
+                ("tempmin", ctypes.c_double), ("velmax", ctypes.c_double), ("hydrotime", ctypes.c_double)]
+
+
 class SorStats(ctypes.Structure):
     """``tt_sor_stats``."""
 

@@ -48,6 +48,10 @@ def load_library():
     L.tt_sweep_timing.argtypes = [ctypes.c_int]
     L.tt_adjust_sor_param.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_void_p]
     L.tt_set_sorparam.argtypes = [ctypes.c_double]
+    L.tt_set_limits.argtypes = [ctypes.POINTER(P.Limits)]
+    L.tt_get_limits.argtypes = [ctypes.POINTER(P.Limits)]
+    L.tt_varlimit.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double]
+    L.tt_sphericalvellimit.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double]
     L.tt_get_sorparam.restype = ctypes.c_double
     L.tt_device_bytes.restype = ctypes.c_int64
     L.tt_sweep_timing_get.argtypes = [ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double),
@@ -132,6 +136,26 @@ class Device:
         self._check(self.L.tt_adjust_sor_param(1 if fill_matrix else 0, ctypes.byref(om), it.ctypes.data),
                     "ADJUST_SOR_PARAM")
         return om.value, it
+
+    def set_limits(self, irho=0, rhmin=0.0, itemp=0, tempmin=0.0, tempmax=0.0, ivel=0, velmax=0.0,
+                   hydrotime=0.0):
+        """Clamps tt_step applies (VARLIMIT / SPHERICALVELLIMIT, pp.f:543-562,
+        673-678, 708-710)."""
+        lim = P.Limits(irho, itemp, ivel, 0, rhmin, tempmax, tempmin, velmax, hydrotime)
+        self._check(self.L.tt_set_limits(ctypes.byref(lim)), "tt_set_limits")
+
+    def get_limits(self):
+        lim = P.Limits()
+        self._check(self.L.tt_get_limits(ctypes.byref(lim)), "tt_get_limits")
+        return lim
+
+    def varlimit(self, field, vmin, vmax):
+        if isinstance(field, str):
+            field = P.FIELD_IDS[field]
+        self._check(self.L.tt_varlimit(field, vmin, vmax), "VARLIMIT")
+
+    def sphericalvellimit(self, ivel, velmax, hydrotime=0.0):
+        self._check(self.L.tt_sphericalvellimit(ivel, velmax, hydrotime), "SPHERICALVELLIMIT")
 
     def set_sorparam(self, omega):
         self._check(self.L.tt_set_sorparam(float(omega)), "tt_set_sorparam")
