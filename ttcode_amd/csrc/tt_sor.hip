@@ -116,6 +116,9 @@ int sor_alloc() {
     if (nb > 2048) nb = 2048;
     nb = ((nb + 7) / 8) * 8;
     if (nb < 8) nb = 8;
+    // partials: two iterations (by parity) x two colours x nb, in the first
+    // half of the partial buffer; the fold reads at most 16 per thread of 256
+    if (4 * nb > L.npartial || 2 * nb > 16 * 256) return fail(1, "SOR: partial buffer too small");
     g_nb = nb;
     g_chunk = (nrows + nb - 1) / nb;  // rows per block
     return 0;
@@ -128,31 +131,26 @@ __device__ inline int xcd_block(int nb) {
     return (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
 }
 
-// Finish an iteration (one block of 1024 threads, after both half sweeps):
-// sums the 2*nb partials in a fixed order and updates the control block.
-// step is the number of iterations since the previous check (1 on the
-// reference path, which tests every iteration).  Each thread loads its
-// (at most 8) partials at once, so the block waits for one memory round
-// trip, not for a chain of them.
-__global__ __launch_bounds__(1024) void k_finish(Ctrl* ctrl, const double* __restrict__ part, int nparts, int step) {
-    __shared__ double sh[16];
+// Finish an iteration with one block of 256 threads: sum the 2*nb partials
+// of the iteration in a fixed order (thread t adds part[t + 256 u], u
+// ascending, then the wavefront and block sums) and update the control
+// block.  step is the number of iterations since the previous test (1 on
+// the reference path, which tests every iteration).  Every load is issued
+// before the first add, so the block waits for one memory round trip.
+__device__ inline void finish_block(Ctrl* ctrl, const double* __restrict__ part, int nparts, int step, double* sh) {
     const int tid = threadIdx.x;
-    double v[8];
+    double v[16];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-        const int q = tid + 1024 * u;
+    for (int u = 0; u < 16; ++u) {
+        const int q = tid + 256 * u;
         v[u] = q < nparts ? part[q] : 0.0;
     }
     const int done = ctrl->done;
     double s = 0.0;
 #pragma unroll
-    for (int u = 0; u < 8; ++u) s += v[u];
-    s = wave_sum(s);
-    if ((tid & 63) == 0) sh[tid >> 6] = s;
-    __syncthreads();
+    for (int u = 0; u < 16; ++u) s += v[u];
+    s = block_sum256(s, sh);
     if (tid == 0 && !done) {
-        s = 0.0;
-        for (int w = 0; w < 16; ++w) s += sh[w];
         int it = ctrl->it + step;
         ctrl->it = it;
         ctrl->rnorm = s;
@@ -171,6 +169,19 @@ __global__ __launch_bounds__(1024) void k_finish(Ctrl* ctrl, const double* __res
     }
 }
 
+__global__ __launch_bounds__(256) void k_finish(Ctrl* ctrl, const double* __restrict__ part, int nparts, int step) {
+    __shared__ double sh[4];
+    finish_block(ctrl, part, nparts, step, sh);
+}
+
+// Arguments of the convergence test a red sweep carries for the iteration
+// before it (fin != 0): one extra block, launched after the nb sweep blocks,
+// runs finish_block on that iteration's partials while the sweep proceeds.
+struct FinArgs {
+    const double* part;
+    int nparts, step, fin;
+};
+
 // One half sweep.  Rows (j,k) of the colour are split into contiguous
 // ranges, one per logical block; the four wavefronts of a block take every
 // fourth row of its range and the 64 lanes of a wavefront walk h.  Index
@@ -178,13 +189,25 @@ __global__ __launch_bounds__(1024) void k_finish(Ctrl* ctrl, const double* __res
 // reverse order: alternating the direction from one iteration to the next
 // makes a sweep start on the rows the previous sweep touched last, which are
 // still in the memory-side (Infinity) cache.
+//
+// The colour's unknowns are updated in place in xc; xo holds the other
+// colour's.  Block nb (present when fa.fin is set) runs the previous
+// iteration's convergence test instead of sweeping rows; a solve that test
+// ends has then had one more red half sweep, which only inner solves of the
+// defect correction allow.
 template <typename T>
-__global__ __launch_bounds__(256) void k_sweep(int c, const T* __restrict__ coef,
-                                               T* __restrict__ xc, const T* __restrict__ xo, CS cs, T omega,
-                                               double* part, int nb, int rows_per_block, const Ctrl* ctrl,
-                                               int rev) {
+__global__ __launch_bounds__(256) void k_sweep(int c, const T* __restrict__ coef, T* __restrict__ xc,
+                                               const T* __restrict__ xo, CS cs, T omega, double* part, int nb,
+                                               int rows_per_block, Ctrl* ctrl, int rev, FinArgs fa) {
     __shared__ double sh[4];
-    if (ctrl->done) return;
+    if ((int)blockIdx.x == nb) {
+        finish_block(ctrl, fa.part, fa.nparts, fa.step, sh);
+        return;
+    }
+    // The test block may set done while this sweep runs, so wavefronts of a
+    // block can read different values: none returns early (all reach the
+    // block sum's barriers); a sweep the test stops is discarded by the host.
+    const int done = ctrl->done;
     int lb = xcd_block(nb);
     if (rev) lb = nb - 1 - lb;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
@@ -193,7 +216,7 @@ __global__ __launch_bounds__(256) void k_sweep(int c, const T* __restrict__ coef
     const int r0 = lb * rows_per_block;
     const int r1 = min(r0 + rows_per_block, nrows);
     double acc = 0.0;
-    for (int r = r0 + ty; r < r1; r += 4) {
+    for (int r = done ? r1 : r0 + ty; r < r1; r += 4) {
         const int kk = r / ny;
         const int jj = r - kk * ny;
         const int s = (jj + kk + c + 1) & 1;  // (j+k+c+1)&1 with j=jj+1, k=kk+1
@@ -220,7 +243,7 @@ __global__ __launch_bounds__(256) void k_sweep(int c, const T* __restrict__ coef
         }
     }
     double bs = block_sum256(acc, sh);
-    if (threadIdx.x == 0) part[c * nb + lb] = bs;
+    if (threadIdx.x == 0 && !done) part[c * nb + lb] = bs;
 }
 
 // FP32 half sweep with two cells per lane (hx even): half a wavefront per
@@ -229,10 +252,17 @@ __global__ __launch_bounds__(256) void k_sweep(int c, const T* __restrict__ coef
 // arithmetic per cell as k_sweep<float>.
 __global__ __launch_bounds__(256) void k_sweep_f32x2(int c, const float* __restrict__ coef, float* __restrict__ xc,
                                                      const float* __restrict__ xo, CS cs, float omega,
-                                                     double* part, int nb, int rows_per_block, const Ctrl* ctrl,
-                                                     int rev) {
+                                                     double* part, int nb, int rows_per_block, Ctrl* ctrl,
+                                                     int rev, FinArgs fa) {
     __shared__ double sh[4];
-    if (ctrl->done) return;
+    if ((int)blockIdx.x == nb) {
+        finish_block(ctrl, fa.part, fa.nparts, fa.step, sh);
+        return;
+    }
+    // The test block may set done while this sweep runs, so wavefronts of a
+    // block can read different values: none returns early (all reach the
+    // block sum's barriers); a sweep the test stops is discarded by the host.
+    const int done = ctrl->done;
     int lb = xcd_block(nb);
     if (rev) lb = nb - 1 - lb;
     const int lane = threadIdx.x & 63, ty = threadIdx.x >> 6;
@@ -242,7 +272,7 @@ __global__ __launch_bounds__(256) void k_sweep_f32x2(int c, const float* __restr
     const int r0 = lb * rows_per_block;
     const int r1 = min(r0 + rows_per_block, nrows);
     double acc = 0.0;
-    for (int r = r0 + 2 * ty + half; r < r1; r += 8) {
+    for (int r = done ? r1 : r0 + 2 * ty + half; r < r1; r += 8) {
         const int kk = r / ny;
         const int jj = r - kk * ny;
         const int s = (jj + kk + c + 1) & 1;
@@ -280,7 +310,7 @@ __global__ __launch_bounds__(256) void k_sweep_f32x2(int c, const float* __restr
         }
     }
     double bs = block_sum256(acc, sh);
-    if (threadIdx.x == 0) part[c * nb + lb] = bs;
+    if (threadIdx.x == 0 && !done) part[c * nb + lb] = bs;
 }
 
 // ---------------------------------------------------------- matrix fill
@@ -555,27 +585,34 @@ static void ev_drain(const Ctrl* st, long long upto) {
     ev_bid.resize(keep);
 }
 
+// Half sweep c of iteration iter (0-based within the solve), in place.  The
+// partials of iteration iter go to slot iter % 2 (two colours x nb each).
+// fin: the red sweep also runs the convergence test of iteration iter - 1.
 template <typename T>
-static int launch_sweep(int c, T* coef0, T* coef1, T* xa, T* xb, double omega, int rev, int iter) {
+static int launch_sweep(int c, T* coef0, T* coef1, T* xa, T* xb, double omega, int rev, int iter, int fin,
+                        int step) {
     T* q = c ? coef1 : coef0;
     T* xc = c ? xb : xa;
-    T* xo = c ? xa : xb;
+    const T* xo = c ? xa : xb;
+    double* part = L.partial + (iter & 1) * 2 * g_nb;
+    FinArgs fa{L.partial + ((iter + 1) & 1) * 2 * g_nb, 2 * g_nb, step, fin};
+    const int grid = g_nb + (fin ? 1 : 0);
     const bool timed = L.timing && (L.timing_count++ % L.timing_period) == 0;
     // timed launches: the dispatch packet itself records the start and stop
     // events, so the interval is the kernel's own execution
     hipEvent_t e0 = timed ? ev_get() : nullptr, e1 = timed ? ev_get() : nullptr;
     if constexpr (sizeof(T) == 4) {
         if (g_cs.hx % 2 == 0) {
-            hipExtLaunchKernelGGL(k_sweep_f32x2, dim3(g_nb), dim3(256), 0, L.s, e0, e1, 0, c, (const float*)q,
-                                  (float*)xc, (const float*)xo, g_cs, (float)omega, L.partial, g_nb, g_chunk,
-                                  (const Ctrl*)L.ctrl, rev);
+            hipExtLaunchKernelGGL(k_sweep_f32x2, dim3(grid), dim3(256), 0, L.s, e0, e1, 0, c, (const float*)q,
+                                  (float*)xc, (const float*)xo, g_cs, (float)omega, part, g_nb, g_chunk, L.ctrl,
+                                  rev, fa);
         } else {
-            hipExtLaunchKernelGGL(k_sweep<T>, dim3(g_nb), dim3(256), 0, L.s, e0, e1, 0, c, (const T*)q, xc,
-                                  (const T*)xo, g_cs, (T)omega, L.partial, g_nb, g_chunk, (const Ctrl*)L.ctrl, rev);
+            hipExtLaunchKernelGGL(k_sweep<T>, dim3(grid), dim3(256), 0, L.s, e0, e1, 0, c, (const T*)q, xc, xo,
+                                  g_cs, (T)omega, part, g_nb, g_chunk, L.ctrl, rev, fa);
         }
     } else {
-        hipExtLaunchKernelGGL(k_sweep<T>, dim3(g_nb), dim3(256), 0, L.s, e0, e1, 0, c, (const T*)q, xc,
-                              (const T*)xo, g_cs, (T)omega, L.partial, g_nb, g_chunk, (const Ctrl*)L.ctrl, rev);
+        hipExtLaunchKernelGGL(k_sweep<T>, dim3(grid), dim3(256), 0, L.s, e0, e1, 0, c, (const T*)q, xc, xo, g_cs,
+                              (T)omega, part, g_nb, g_chunk, L.ctrl, rev, fa);
     }
     TT_LAUNCH();
     if (!timed) return 0;
@@ -613,10 +650,15 @@ static int set_ctrl_solve(int itmax, int itmin, double eps) {
     return 0;
 }
 
-// SOR iterations on (coef, x) until ctrl->done; returns completed iterations
+// SOR iterations on (coef, x) until ctrl->done; returns completed
+// iterations.  The convergence test runs every check_every-th iteration:
+// with deferred unset (the reference's solve, which must stop exactly where
+// the test is met) in a k_finish of its own; with deferred set (inner
+// solves) inside the next red sweep, except for the last iteration of a
+// batch, whose k_finish keeps the control block the host copies current.
 template <typename T>
 static int run_sweeps(T* coef0, T* coef1, T* xa, T* xb, double omega, int* iters, double* rnorm,
-                      int check_every) {
+                      int check_every, bool deferred) {
     int batch = L.opts.batch > 0 ? L.opts.batch : 16;
     batch = ((batch + 3) / 4) * 4;  // a multiple of 4: directions alternate and checks stay aligned
     long long launched = 0;
@@ -631,12 +673,14 @@ static int run_sweeps(T* coef0, T* coef1, T* xa, T* xb, double omega, int* iters
         for (int b = 0; b < batch; ++b) {
             const int rev = (b & 1);  // batch is even: the direction alternates across batches too
             const int iter = (int)(launched - batch) + b;
-            int rc = launch_sweep<T>(0, coef0, coef1, xa, xb, omega, rev, iter);
+            // the previous iteration was a test point inside this batch
+            const int fin = (deferred && b > 0 && b % check_every == 0) ? 1 : 0;
+            int rc = launch_sweep<T>(0, coef0, coef1, xa, xb, omega, rev, iter, fin, check_every);
             if (rc) return rc;
-            rc = launch_sweep<T>(1, coef0, coef1, xa, xb, omega, rev, iter);
+            rc = launch_sweep<T>(1, coef0, coef1, xa, xb, omega, rev, iter, 0, check_every);
             if (rc) return rc;
-            if ((b + 1) % check_every == 0) {
-                k_finish<<<1, 1024, 0, L.s>>>(L.ctrl, L.partial, 2 * g_nb, check_every);
+            if (deferred ? b == batch - 1 : (b + 1) % check_every == 0) {
+                k_finish<<<1, 256, 0, L.s>>>(L.ctrl, L.partial + (iter & 1) * 2 * g_nb, 2 * g_nb, check_every);
                 TT_LAUNCH();
             }
         }
@@ -777,7 +821,7 @@ int sor_solve_impl(int fill_from_evals, tt_sor_stats* st, int mode) {
     int iters = 0;
     double rnorm = 0.0;
     if (prec == 0) {
-        rc = run_sweeps<double>(c64[0], c64[1], x64[0], x64[1], L.p.sorparam, &iters, &rnorm, 1);
+        rc = run_sweeps<double>(c64[0], c64[1], x64[0], x64[1], L.p.sorparam, &iters, &rnorm, 1, false);
         if (rc) return rc;
         s.sor_iter = iters;
         s.rnorm = rnorm;
@@ -820,7 +864,7 @@ int sor_solve_impl(int fill_from_evals, tt_sor_stats* st, int mode) {
             TT_HIP(hipMemcpyAsync(L.ctrl, L.hctrl, sizeof(Ctrl), hipMemcpyHostToDevice, L.s));
             int it32 = 0;
             double rn32 = 0;
-            rc = run_sweeps<float>(c32[0], c32[1], x32[0], x32[1], L.p.sorparam, &it32, &rn32, 4);
+            rc = run_sweeps<float>(c32[0], c32[1], x32[0], x32[1], L.p.sorparam, &it32, &rn32, 4, true);
             if (rc) return rc;
             total32 += it32;
             for (int cc = 0; cc < 2; ++cc) {

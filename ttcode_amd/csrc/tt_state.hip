@@ -253,6 +253,7 @@ int tt_init(const tt_params* p) {
     g.n3 = (long long)g.gx * g.gy * g.gz;
     if (g.n3 >= (1LL << 31)) return fail(1, "grid too large: the ghosted box must hold fewer than 2^31 cells");
     TT_HIP(hipStreamCreateWithFlags(&L.s, hipStreamNonBlocking));
+    L.npartial = 8192;  // per-block partials: 2 x npartial doubles, allocated below
     // the SOR working set first: the hot loop's buffers get the first pages
     int rc = sor_alloc();
     if (rc) return rc;
@@ -269,7 +270,6 @@ int tt_init(const tt_params* p) {
         TT_HIP(dev_alloc(q, bytes));
         TT_HIP(hipMemsetAsync(*q, 0, bytes, L.s));
     }
-    L.npartial = 8192;
     TT_HIP(dev_alloc(&L.partial, (size_t)L.npartial * 2 * sizeof(double)));
     TT_HIP(dev_alloc(&L.ub, (size_t)g.gy * g.gz * sizeof(int32_t)));
     TT_HIP(hipMemsetAsync(L.ub, 0, (size_t)g.gy * g.gz * sizeof(int32_t), L.s));
