@@ -306,6 +306,20 @@ int tt_energy_totals(double* out3);
 int tt_sweep_timing(int period);
 int tt_sweep_timing_get(int64_t* n64, double* ms64, int64_t* n32, double* ms32);
 
+/* Speculative convergence tests of the FP64 SOR solve.  The test of an
+ * iteration normally runs in a kernel of its own, so the solve stops exactly
+ * where the reference's does.  Where the L1 ratios of the two latest batches,
+ * extrapolated margin times faster than observed, cannot reach EPSMAT
+ * before the end of the batch being queued, that batch's tests run inside
+ * the next red sweep instead; should one of those tests end the solve after
+ * all, the solve is rerun from its start values with tests of their own, so
+ * results never depend on the speculation.  margin > 0 sets the factor
+ * (default 2.5), 0 speculates on every batch but the first two (testing),
+ * < 0 switches speculation off.  The getter returns the speculative
+ * batches and the reruns since the last set call. */
+int tt_sor_speculation(double margin);
+int tt_sor_speculation_get(int64_t* spec_batches, int64_t* reruns);
+
 /* ------------------------------------------------------------------------
  * Learned initial guess (new: no counterpart in the reference).  A 3-D CNN
  * maps five per-cell features -- log E, log T, log rho, log kappa_R,

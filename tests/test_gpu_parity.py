@@ -250,6 +250,37 @@ def test_step_32x64x32_vs_oracle(ttgpu):
     d.close()
 
 
+def test_speculative_tests_never_change_results(ttgpu):
+    """Speculating on every batch (margin 0) makes deferred tests end solves
+    one red half sweep late; each such solve is rerun with tests of its own,
+    so two steps agree bit for bit with speculation switched off."""
+    p = P.make_params(32, 64, 32, epsmat=1e-8)
+    st = P.generate_state(p, seed=5)
+    out, stats = {}, {}
+    for margin in (-1.0, 0.0, 2.5):
+        d = ttgpu.Device(p)
+        d.load_state(st)
+        d.bounds()
+        d.pressg()
+        dt = d.schritt()
+        d.load_state(st)
+        d.sor_speculation(margin)
+        try:
+            its = [d.step(dt, 2 + s)[0].energy.sor_iter_total for s in range(2)]
+            stats[margin] = d.sor_speculation_get()
+        finally:
+            d.sor_speculation(2.5)
+        out[margin] = (its, {k: d.download(k) for k in ("T", "ER", "delta_ER")})
+        d.close()
+    print("iterations", {m: v[0] for m, v in out.items()}, "speculative batches, reruns", stats)
+    assert stats[0.0][0] > 0 and stats[0.0][1] > 0
+    assert stats[-1.0] == (0, 0)
+    for m in (0.0, 2.5):
+        assert out[m][0] == out[-1.0][0]
+        for k in ("T", "ER", "delta_ER"):
+            assert np.array_equal(out[m][1][k], out[-1.0][1][k]), (m, k)
+
+
 def test_mixed_precision_solve(ttgpu):
     """FP32 sweeps inside FP64 defect correction reach the FP64 residual target
     and agree with the FP64 solve within it."""

@@ -59,7 +59,7 @@ struct Ctrl {
     int bad_q;         // delta_Q positivity failure
     int bad_bound;     // BOUNDXMAX(2) upperbnd above the domain
     int bad_state;     // SCHRITT positivity failure
-    int pad;
+    int spec_hit;      // a deferred (speculative) test ended the solve
     double dtmin;
     double scratch[8];
     // stagnation watch of inner (FP32) solves
@@ -100,6 +100,9 @@ struct Lib {
     int64_t timing_count = 0;
     int64_t n64 = 0, n32 = 0;
     double ms64 = 0.0, ms32 = 0.0;
+    // speculative convergence tests of the FP64 SOR solve
+    double spec_margin = 2.5;
+    int64_t spec_batches = 0, spec_reruns = 0;
     int energy_iter = 0;
     int iii = 0;
     int have_prev_guess = 0;

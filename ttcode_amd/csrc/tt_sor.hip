@@ -137,7 +137,8 @@ __device__ inline int xcd_block(int nb) {
 // block.  step is the number of iterations since the previous test (1 on
 // the reference path, which tests every iteration).  Every load is issued
 // before the first add, so the block waits for one memory round trip.
-__device__ inline void finish_block(Ctrl* ctrl, const double* __restrict__ part, int nparts, int step, double* sh) {
+__device__ inline void finish_block(Ctrl* ctrl, const double* __restrict__ part, int nparts, int step, double* sh,
+                                    int deferred) {
     const int tid = threadIdx.x;
     double v[16];
 #pragma unroll
@@ -155,23 +156,28 @@ __device__ inline void finish_block(Ctrl* ctrl, const double* __restrict__ part,
         ctrl->it = it;
         ctrl->rnorm = s;
         double ratio = s / ctrl->rhsnorm;
-        if (it >= ctrl->itmax) ctrl->done = 2;
-        else if (!(ratio > ctrl->eps || it < ctrl->itmin)) ctrl->done = 1;
+        int d = 0;
+        if (it >= ctrl->itmax) d = 2;
+        else if (!(ratio > ctrl->eps || it < ctrl->itmin)) d = 1;
         else if (ctrl->stag > 0) {  // inner solves: stop when the ratio no longer falls
             if (ratio < 0.99 * ctrl->best) {
                 ctrl->best = ratio;
                 ctrl->since = 0;
             } else {
                 ctrl->since += step;
-                if (ctrl->since >= ctrl->stag) ctrl->done = 3;
+                if (ctrl->since >= ctrl->stag) d = 3;
             }
+        }
+        if (d) {
+            ctrl->done = d;
+            if (deferred) ctrl->spec_hit = 1;
         }
     }
 }
 
 __global__ __launch_bounds__(256) void k_finish(Ctrl* ctrl, const double* __restrict__ part, int nparts, int step) {
     __shared__ double sh[4];
-    finish_block(ctrl, part, nparts, step, sh);
+    finish_block(ctrl, part, nparts, step, sh, 0);
 }
 
 // Arguments of the convergence test a red sweep carries for the iteration
@@ -193,15 +199,15 @@ struct FinArgs {
 // The colour's unknowns are updated in place in xc; xo holds the other
 // colour's.  Block nb (present when fa.fin is set) runs the previous
 // iteration's convergence test instead of sweeping rows; a solve that test
-// ends has then had one more red half sweep, which only inner solves of the
-// defect correction allow.
+// ends has then had one more red half sweep, which inner solves of the
+// defect correction allow and the FP64 solve undoes by running again.
 template <typename T>
 __global__ __launch_bounds__(256) void k_sweep(int c, const T* __restrict__ coef, T* __restrict__ xc,
                                                const T* __restrict__ xo, CS cs, T omega, double* part, int nb,
                                                int rows_per_block, Ctrl* ctrl, int rev, FinArgs fa) {
     __shared__ double sh[4];
     if ((int)blockIdx.x == nb) {
-        finish_block(ctrl, fa.part, fa.nparts, fa.step, sh);
+        finish_block(ctrl, fa.part, fa.nparts, fa.step, sh, 1);
         return;
     }
     // The test block may set done while this sweep runs, so wavefronts of a
@@ -256,7 +262,7 @@ __global__ __launch_bounds__(256) void k_sweep_f32x2(int c, const float* __restr
                                                      int rev, FinArgs fa) {
     __shared__ double sh[4];
     if ((int)blockIdx.x == nb) {
-        finish_block(ctrl, fa.part, fa.nparts, fa.step, sh);
+        finish_block(ctrl, fa.part, fa.nparts, fa.step, sh, 1);
         return;
     }
     // The test block may set done while this sweep runs, so wavefronts of a
@@ -645,6 +651,7 @@ static int set_ctrl_solve(int itmax, int itmin, double eps) {
     c.counter = 0;
     c.counter2 = 0;
     c.stag = 0;
+    c.spec_hit = 0;
     *L.hctrl = c;
     TT_HIP(hipMemcpyAsync(L.ctrl, L.hctrl, sizeof(Ctrl), hipMemcpyHostToDevice, L.s));
     return 0;
@@ -652,17 +659,27 @@ static int set_ctrl_solve(int itmax, int itmin, double eps) {
 
 // SOR iterations on (coef, x) until ctrl->done; returns completed
 // iterations.  The convergence test runs every check_every-th iteration:
-// with deferred unset (the reference's solve, which must stop exactly where
-// the test is met) in a k_finish of its own; with deferred set (inner
-// solves) inside the next red sweep, except for the last iteration of a
-// batch, whose k_finish keeps the control block the host copies current.
+// with deferred set (inner solves) inside the next red sweep, except for the
+// last iteration of a batch, whose k_finish keeps the control block the host
+// copies current.  With deferred unset (the reference's solve, which must
+// stop exactly where the test is met) in a k_finish of its own, or, when
+// speculate is set and the latest ratios say the solve cannot converge
+// within the batch being queued, deferred as for inner solves; if such a
+// test ends the solve after all, SOR_SPEC_MISS is returned and the caller
+// reruns the solve without speculation.
+static const int SOR_SPEC_MISS = -100;
+
 template <typename T>
 static int run_sweeps(T* coef0, T* coef1, T* xa, T* xb, double omega, int* iters, double* rnorm,
-                      int check_every, bool deferred) {
+                      int check_every, bool deferred, bool speculate = false) {
     int batch = L.opts.batch > 0 ? L.opts.batch : 16;
     batch = ((batch + 3) / 4) * 4;  // a multiple of 4: directions alternate and checks stay aligned
     long long launched = 0;
     int slot = 0, pend = -1, last = 0;
+    // the two latest (iteration, log L1 ratio) states the host has read
+    int nknown = 0, it_a = 0, it_b = 0;
+    double lr_a = 0.0, lr_b = 0.0;
+    const double leps = log(L.hctrl->eps);
     for (;;) {
         // the control block's iteration cap ends the loop; a solve that has
         // run past it without the device reporting completion is an error
@@ -670,16 +687,26 @@ static int run_sweeps(T* coef0, T* coef1, T* xa, T* xb, double omega, int* iters
             return fail(12, "SOR: device did not report completion");
         launched += batch;
         ++g_bid;
+        bool defer = deferred;
+        if (!deferred && speculate && nknown >= 2 && it_b > it_a) {
+            // log-ratio fall per iteration over the last two reads, and the
+            // iterations from the newer read to the end of this batch
+            const double rate = (lr_a - lr_b) / (double)(it_b - it_a);
+            const double ahead = (double)(launched - it_b);
+            const double m = L.spec_margin;
+            defer = m == 0.0 || (m > 0.0 && rate > 0.0 && lr_b - leps > m * rate * ahead);
+            if (defer) ++L.spec_batches;
+        }
         for (int b = 0; b < batch; ++b) {
             const int rev = (b & 1);  // batch is even: the direction alternates across batches too
             const int iter = (int)(launched - batch) + b;
             // the previous iteration was a test point inside this batch
-            const int fin = (deferred && b > 0 && b % check_every == 0) ? 1 : 0;
+            const int fin = (defer && b > 0 && b % check_every == 0) ? 1 : 0;
             int rc = launch_sweep<T>(0, coef0, coef1, xa, xb, omega, rev, iter, fin, check_every);
             if (rc) return rc;
             rc = launch_sweep<T>(1, coef0, coef1, xa, xb, omega, rev, iter, 0, check_every);
             if (rc) return rc;
-            if (deferred ? b == batch - 1 : (b + 1) % check_every == 0) {
+            if (defer ? b == batch - 1 : (b + 1) % check_every == 0) {
                 k_finish<<<1, 256, 0, L.s>>>(L.ctrl, L.partial + (iter & 1) * 2 * g_nb, 2 * g_nb, check_every);
                 TT_LAUNCH();
             }
@@ -693,9 +720,17 @@ static int run_sweeps(T* coef0, T* coef1, T* xa, T* xb, double omega, int* iters
         if (pend >= 0) {
             TT_HIP(hipEventSynchronize(g_cev[pend]));
             if (L.timing) ev_drain(g_hc[pend], g_cbid[pend]);
-            if (g_hc[pend]->done) {
-                *L.hctrl = *g_hc[pend];
+            const Ctrl* h = g_hc[pend];
+            if (h->done) {
+                *L.hctrl = *h;
                 break;
+            }
+            if (h->it > it_b && h->rnorm > 0.0 && h->rhsnorm > 0.0) {
+                it_a = it_b;
+                lr_a = lr_b;
+                it_b = h->it;
+                lr_b = log(h->rnorm / h->rhsnorm);
+                ++nknown;
             }
         }
         pend = slot;
@@ -707,6 +742,7 @@ static int run_sweeps(T* coef0, T* coef1, T* xa, T* xb, double omega, int* iters
     }
     *iters = L.hctrl->it;
     *rnorm = L.hctrl->rnorm;
+    if (!deferred && L.hctrl->spec_hit) return SOR_SPEC_MISS;
     return 0;
 }
 
@@ -763,6 +799,24 @@ int sor_solve_impl(int fill_from_evals, tt_sor_stats* st, int mode) {
     double* target = mode == 1 ? f[TT_T] : f[TT_DELTA_ER];
     int prec = mode == 1 ? 0 : L.opts.precision;
     // initial unknowns
+    double alpha = 0.0;
+    // the start values again, for a rerun (their sources are unchanged)
+    auto reload = [&]() -> int {
+        for (int c = 0; c < 2; ++c) {
+            if (mode == 1 || s.guess_used) {
+                k_load_x<<<nblk(pl, 256), 256, 0, L.s>>>(g, g_cs, c, mode == 1 ? f[TT_T] : f[TT_DELTA_ER], x64[c],
+                                                         mode == 1 ? 0 : 1);
+                TT_LAUNCH();
+                if (mode == 0) {
+                    k_scale<<<nblk(pl, 256), 256, 0, L.s>>>(x64[c], alpha, pl);
+                    TT_LAUNCH();
+                }
+            } else {
+                TT_HIP(hipMemsetAsync(x64[c], 0, pl * sizeof(double), L.s));
+            }
+        }
+        return 0;
+    };
     if (mode == 1) {
         for (int c = 0; c < 2; ++c) {
             k_load_x<<<nblk(pl, 256), 256, 0, L.s>>>(g, g_cs, c, f[TT_T], x64[c], 0);
@@ -799,7 +853,7 @@ int sor_solve_impl(int fill_from_evals, tt_sor_stats* st, int mode) {
             rc = read_ctrl();
             if (rc) return rc;
             double s1 = L.hctrl->scratch[3], s2 = L.hctrl->scratch[4];
-            double alpha = (s2 > 0.0 && s1 > 0.0) ? s1 / s2 : 0.0;
+            alpha = (s2 > 0.0 && s1 > 0.0) ? s1 / s2 : 0.0;
             s.guess_scale = alpha;
             if (alpha > 0.0) {
                 for (int c = 0; c < 2; ++c) {
@@ -821,7 +875,22 @@ int sor_solve_impl(int fill_from_evals, tt_sor_stats* st, int mode) {
     int iters = 0;
     double rnorm = 0.0;
     if (prec == 0) {
-        rc = run_sweeps<double>(c64[0], c64[1], x64[0], x64[1], L.p.sorparam, &iters, &rnorm, 1, false);
+        rc = run_sweeps<double>(c64[0], c64[1], x64[0], x64[1], L.p.sorparam, &iters, &rnorm, 1, false,
+                                L.spec_margin >= 0.0);
+        if (rc == SOR_SPEC_MISS) {
+            // a speculative test ended the solve one red half sweep late:
+            // run it again from the same start with every test in a kernel
+            // of its own
+            ++L.spec_reruns;
+            if ((rc = reload())) return rc;
+            if ((rc = read_ctrl())) return rc;
+            L.hctrl->done = 0;
+            L.hctrl->it = 0;
+            L.hctrl->spec_hit = 0;
+            L.hctrl->rnorm = 0.0;
+            TT_HIP(hipMemcpyAsync(L.ctrl, L.hctrl, sizeof(Ctrl), hipMemcpyHostToDevice, L.s));
+            rc = run_sweeps<double>(c64[0], c64[1], x64[0], x64[1], L.p.sorparam, &iters, &rnorm, 1, false, false);
+        }
         if (rc) return rc;
         s.sor_iter = iters;
         s.rnorm = rnorm;
@@ -986,6 +1055,19 @@ int tt_sweep_timing(int period) {
     L.timing_count = 0;
     L.n64 = L.n32 = 0;
     L.ms64 = L.ms32 = 0.0;
+    return 0;
+}
+
+int tt_sor_speculation(double margin) {
+    L.spec_margin = margin;
+    L.spec_batches = 0;
+    L.spec_reruns = 0;
+    return 0;
+}
+
+int tt_sor_speculation_get(int64_t* spec_batches, int64_t* reruns) {
+    if (spec_batches) *spec_batches = L.spec_batches;
+    if (reruns) *reruns = L.spec_reruns;
     return 0;
 }
 

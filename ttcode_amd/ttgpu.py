@@ -46,6 +46,8 @@ def load_library():
     L.tt_set_solver.argtypes = [ctypes.POINTER(P.SolverOpts)]
     L.tt_get_solver.argtypes = [ctypes.POINTER(P.SolverOpts)]
     L.tt_sweep_timing.argtypes = [ctypes.c_int]
+    L.tt_sor_speculation.argtypes = [ctypes.c_double]
+    L.tt_sor_speculation_get.argtypes = [ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
     L.tt_adjust_sor_param.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_void_p]
     L.tt_set_sorparam.argtypes = [ctypes.c_double]
     L.tt_set_limits.argtypes = [ctypes.POINTER(P.Limits)]
@@ -285,6 +287,16 @@ class Device:
         """Time every period-th SOR half-sweep launch with HIP events (0 or
         False: off)."""
         self.L.tt_sweep_timing(int(period))
+
+    def sor_speculation(self, margin):
+        """Speculative convergence tests of the FP64 solve (margin > 0: the
+        factor, 0: always, < 0: off); resets the counters."""
+        self.L.tt_sor_speculation(float(margin))
+
+    def sor_speculation_get(self):
+        a, b = ctypes.c_int64(), ctypes.c_int64()
+        self.L.tt_sor_speculation_get(ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
 
     def sweep_timing_get(self):
         n64, n32 = ctypes.c_int64(), ctypes.c_int64()
