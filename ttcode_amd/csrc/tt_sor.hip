@@ -637,12 +637,26 @@ static int read_ctrl() {
     return 0;
 }
 
+// The solve fields of the control block, set in stream order by a kernel
+// (the other fields -- flags of earlier kernels not yet read -- stay as the
+// device has them); the host copy gets the same values.
+__global__ void k_ctrl_solve(Ctrl* c, int itmax, int itmin, double eps) {
+    if (threadIdx.x != 0) return;
+    c->done = 0;
+    c->it = 0;
+    c->itmax = itmax;
+    c->itmin = itmin;
+    c->eps = eps;
+    c->counter = 0;
+    c->counter2 = 0;
+    c->stag = 0;
+    c->spec_hit = 0;
+}
+
 static int set_ctrl_solve(int itmax, int itmin, double eps) {
-    // host-side fields of the control block, written before the solve
-    Ctrl c{};
-    TT_HIP(hipMemcpyAsync(L.hctrl, L.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, L.s));
-    TT_HIP(hipStreamSynchronize(L.s));
-    c = *L.hctrl;
+    k_ctrl_solve<<<1, 64, 0, L.s>>>(L.ctrl, itmax, itmin, eps);
+    TT_LAUNCH();
+    Ctrl& c = *L.hctrl;
     c.done = 0;
     c.it = 0;
     c.itmax = itmax;
@@ -652,8 +666,19 @@ static int set_ctrl_solve(int itmax, int itmin, double eps) {
     c.counter2 = 0;
     c.stag = 0;
     c.spec_hit = 0;
-    *L.hctrl = c;
-    TT_HIP(hipMemcpyAsync(L.ctrl, L.hctrl, sizeof(Ctrl), hipMemcpyHostToDevice, L.s));
+    return 0;
+}
+
+// The whole control block from a host value, in stream order (the value is
+// a kernel argument, captured at launch).  Only for a host copy read after
+// the last kernel that sets a device flag.
+__global__ void k_ctrl_put(Ctrl* dst, Ctrl v) {
+    if (threadIdx.x == 0) *dst = v;
+}
+
+static int ctrl_put() {
+    k_ctrl_put<<<1, 64, 0, L.s>>>(L.ctrl, *L.hctrl);
+    TT_LAUNCH();
     return 0;
 }
 
@@ -830,11 +855,11 @@ int sor_solve_impl(int fill_from_evals, tt_sor_stats* st, int mode) {
             k_load_x<<<nblk(pl, 256), 256, 0, L.s>>>(g, g_cs, c, f[TT_DELTA_ER], x64[c], 1);
             TT_LAUNCH();
         }
-        rc = read_ctrl();
-        if (rc) return rc;
-        s.r0_cold = L.hctrl->rhsnorm;
         s.guess_used = 0;
         if (use_guess) {
+            rc = read_ctrl();
+            if (rc) return rc;
+            s.r0_cold = L.hctrl->rhsnorm;
             // scale the guess by the factor that minimises its residual's
             // L2 norm, alpha = <b, A x> / <A x, A x>: a guess with the right
             // shape but the wrong amplitude, or rough error that A amplifies,
@@ -888,7 +913,7 @@ int sor_solve_impl(int fill_from_evals, tt_sor_stats* st, int mode) {
             L.hctrl->it = 0;
             L.hctrl->spec_hit = 0;
             L.hctrl->rnorm = 0.0;
-            TT_HIP(hipMemcpyAsync(L.ctrl, L.hctrl, sizeof(Ctrl), hipMemcpyHostToDevice, L.s));
+            if ((rc = ctrl_put())) return rc;
             rc = run_sweeps<double>(c64[0], c64[1], x64[0], x64[1], L.p.sorparam, &iters, &rnorm, 1, false, false);
         }
         if (rc) return rc;
@@ -900,10 +925,10 @@ int sor_solve_impl(int fill_from_evals, tt_sor_stats* st, int mode) {
             k_to_f32<<<nblk(8 * g_cs.ncell, 256), 256, 0, L.s>>>(c64[c], c32[c], 8 * g_cs.ncell);
             TT_LAUNCH();
         }
-        double rhs_norm = L.hctrl->rhsnorm;
         double rn;
-        rc = resid64(&rn, true);
+        rc = resid64(&rn, true);  // reads the control block: rhsnorm is current
         if (rc) return rc;
+        const double rhs_norm = L.hctrl->rhsnorm;
         int outer = 0, total32 = 0;
         const int max_outer = 200;
         while (!(rn / rhs_norm <= L.p.epsmat) && outer < max_outer && total32 < L.p.itmax) {
@@ -911,9 +936,7 @@ int sor_solve_impl(int fill_from_evals, tt_sor_stats* st, int mode) {
             // inner: A c = r in FP32 from c = 0, L1 during-sweep ratio vs sum|r|
             TT_HIP(hipMemsetAsync(x32[0], 0, pl * sizeof(float), L.s));
             TT_HIP(hipMemsetAsync(x32[1], 0, pl * sizeof(float), L.s));
-            rc = read_ctrl();
-            if (rc) return rc;
-            Ctrl c = *L.hctrl;
+            Ctrl c = *L.hctrl;  // current: resid64 has just read it
             c.done = 0; c.it = 0; c.counter = 0; c.counter2 = 0;
             c.itmin = 1;
             c.itmax = L.opts.inner_max;
@@ -930,7 +953,7 @@ int sor_solve_impl(int fill_from_evals, tt_sor_stats* st, int mode) {
             c.eps = need > L.opts.inner_tol ? (need < 0.5 ? need : 0.5) : L.opts.inner_tol;
             c.rhsnorm = rn;
             *L.hctrl = c;
-            TT_HIP(hipMemcpyAsync(L.ctrl, L.hctrl, sizeof(Ctrl), hipMemcpyHostToDevice, L.s));
+            if ((rc = ctrl_put())) return rc;
             int it32 = 0;
             double rn32 = 0;
             rc = run_sweeps<float>(c32[0], c32[1], x32[0], x32[1], L.p.sorparam, &it32, &rn32, 4, true);
@@ -953,9 +976,10 @@ int sor_solve_impl(int fill_from_evals, tt_sor_stats* st, int mode) {
         L.hctrl->rhsnorm = rhs_norm;
         L.hctrl->it = total32;
         L.hctrl->done = (rn / rhs_norm <= L.p.epsmat) ? 1 : 2;
-        TT_HIP(hipMemcpyAsync(L.ctrl, L.hctrl, sizeof(Ctrl), hipMemcpyHostToDevice, L.s));
+        if ((rc = ctrl_put())) return rc;
     }
     s.rhsnorm = L.hctrl->rhsnorm;
+    if (mode == 0 && !s.guess_used && s.r0_cold == 0.0) s.r0_cold = s.rhsnorm;  // zero start: residual = RHS
     // unknowns back into the ghosted field, then the phi wrap
     if (mode == 0) {
         TT_HIP(hipMemsetAsync(target, 0, (size_t)g.n3 * sizeof(double), L.s));
