@@ -604,24 +604,30 @@ static int launch_sweep(int c, T* coef0, T* coef1, T* xa, T* xb, double omega, i
     FinArgs fa{L.partial + ((iter + 1) & 1) * 2 * g_nb, 2 * g_nb, step, fin};
     const int grid = g_nb + (fin ? 1 : 0);
     const bool timed = L.timing && (L.timing_count++ % L.timing_period) == 0;
+    const bool pairs = sizeof(T) == 4 && g_cs.hx % 2 == 0;
+    if (!timed) {
+        // the plain launch path: the host's cost per launch bounds how fast
+        // queued sweeps that return at once (after convergence) drain
+        if (pairs)
+            k_sweep_f32x2<<<grid, 256, 0, L.s>>>(c, (const float*)q, (float*)xc, (const float*)xo, g_cs, (float)omega,
+                                                 part, g_nb, g_chunk, L.ctrl, rev, fa);
+        else
+            k_sweep<T><<<grid, 256, 0, L.s>>>(c, (const T*)q, xc, xo, g_cs, (T)omega, part, g_nb, g_chunk, L.ctrl, rev,
+                                              fa);
+        TT_LAUNCH();
+        return 0;
+    }
     // timed launches: the dispatch packet itself records the start and stop
     // events, so the interval is the kernel's own execution
-    hipEvent_t e0 = timed ? ev_get() : nullptr, e1 = timed ? ev_get() : nullptr;
-    if constexpr (sizeof(T) == 4) {
-        if (g_cs.hx % 2 == 0) {
-            hipExtLaunchKernelGGL(k_sweep_f32x2, dim3(grid), dim3(256), 0, L.s, e0, e1, 0, c, (const float*)q,
-                                  (float*)xc, (const float*)xo, g_cs, (float)omega, part, g_nb, g_chunk, L.ctrl,
-                                  rev, fa);
-        } else {
-            hipExtLaunchKernelGGL(k_sweep<T>, dim3(grid), dim3(256), 0, L.s, e0, e1, 0, c, (const T*)q, xc, xo,
-                                  g_cs, (T)omega, part, g_nb, g_chunk, L.ctrl, rev, fa);
-        }
+    hipEvent_t e0 = ev_get(), e1 = ev_get();
+    if (pairs) {
+        hipExtLaunchKernelGGL(k_sweep_f32x2, dim3(grid), dim3(256), 0, L.s, e0, e1, 0, c, (const float*)q,
+                              (float*)xc, (const float*)xo, g_cs, (float)omega, part, g_nb, g_chunk, L.ctrl, rev, fa);
     } else {
         hipExtLaunchKernelGGL(k_sweep<T>, dim3(grid), dim3(256), 0, L.s, e0, e1, 0, c, (const T*)q, xc, xo, g_cs,
                               (T)omega, part, g_nb, g_chunk, L.ctrl, rev, fa);
     }
     TT_LAUNCH();
-    if (!timed) return 0;
     ev_pend.push_back(e0);
     ev_pend.push_back(e1);
     ev_kind.push_back(sizeof(T) == 8 ? 64 : 32);
@@ -694,6 +700,36 @@ static int ctrl_put() {
 // reruns the solve without speculation.
 static const int SOR_SPEC_MISS = -100;
 
+// The host's view of a solve in progress: the two latest (iteration, log L1
+// ratio) states it has read.
+struct SolveView {
+    int nknown = 0, it_a = 0, it_b = 0;
+    double lr_a = 0.0, lr_b = 0.0;
+    // log-ratio fall per iteration over the two reads (0: unknown)
+    double rate() const { return nknown >= 2 && it_b > it_a ? (lr_a - lr_b) / (double)(it_b - it_a) : 0.0; }
+};
+
+// Wait for the control-block copy of batch q; *done is set when the solve
+// has finished (L.hctrl then holds the final state).
+static int take_batch(int q, SolveView& v, bool* done) {
+    TT_HIP(hipEventSynchronize(g_cev[q]));
+    if (L.timing) ev_drain(g_hc[q], g_cbid[q]);
+    const Ctrl* h = g_hc[q];
+    *done = h->done != 0;
+    if (*done) {
+        *L.hctrl = *h;
+        return 0;
+    }
+    if (h->it > v.it_b && h->rnorm > 0.0 && h->rhsnorm > 0.0) {
+        v.it_a = v.it_b;
+        v.lr_a = v.lr_b;
+        v.it_b = h->it;
+        v.lr_b = log(h->rnorm / h->rhsnorm);
+        ++v.nknown;
+    }
+    return 0;
+}
+
 template <typename T>
 static int run_sweeps(T* coef0, T* coef1, T* xa, T* xb, double omega, int* iters, double* rnorm,
                       int check_every, bool deferred, bool speculate = false) {
@@ -701,37 +737,57 @@ static int run_sweeps(T* coef0, T* coef1, T* xa, T* xb, double omega, int* iters
     batch = ((batch + 3) / 4) * 4;  // a multiple of 4: directions alternate and checks stay aligned
     long long launched = 0;
     int slot = 0, pend = -1, last = 0;
-    // the two latest (iteration, log L1 ratio) states the host has read
-    int nknown = 0, it_a = 0, it_b = 0;
-    double lr_a = 0.0, lr_b = 0.0;
+    SolveView view;
     const double leps = log(L.hctrl->eps);
+    bool done = false;
     for (;;) {
         // the control block's iteration cap ends the loop; a solve that has
         // run past it without the device reporting completion is an error
         if (launched > (long long)L.hctrl->itmax + 3 * batch)
             return fail(12, "SOR: device did not report completion");
-        launched += batch;
+        // Near the end, sweeps queued after the converging iteration return
+        // at once but still cost a launch each.  From the observed rate: if
+        // the solve should end within the queued work, read the batch in
+        // flight before queuing more; if it should end within less than a
+        // batch, queue a shorter one (a multiple of 4, with 25% + 2 spare).
+        const double rate = view.rate();
+        int bsize = batch;
+        if (rate > 0.0) {
+            const double remain = (double)view.it_b + (view.lr_b - leps) / rate - (double)launched;
+            if (remain <= 0.0 && pend >= 0) {
+                int rc = take_batch(pend, view, &done);
+                if (rc) return rc;
+                if (done) break;
+                pend = -1;
+                continue;
+            }
+            if (remain < batch) {
+                const int want = (int)ceil(1.25 * remain) + 2;
+                bsize = ((want + 3) / 4) * 4;
+                if (bsize < 4) bsize = 4;
+                if (bsize > batch) bsize = batch;
+            }
+        }
+        launched += bsize;
         ++g_bid;
         bool defer = deferred;
-        if (!deferred && speculate && nknown >= 2 && it_b > it_a) {
-            // log-ratio fall per iteration over the last two reads, and the
+        if (!deferred && speculate && rate != 0.0) {
             // iterations from the newer read to the end of this batch
-            const double rate = (lr_a - lr_b) / (double)(it_b - it_a);
-            const double ahead = (double)(launched - it_b);
+            const double ahead = (double)(launched - view.it_b);
             const double m = L.spec_margin;
-            defer = m == 0.0 || (m > 0.0 && rate > 0.0 && lr_b - leps > m * rate * ahead);
+            defer = m == 0.0 || (m > 0.0 && rate > 0.0 && view.lr_b - leps > m * rate * ahead);
             if (defer) ++L.spec_batches;
         }
-        for (int b = 0; b < batch; ++b) {
-            const int rev = (b & 1);  // batch is even: the direction alternates across batches too
-            const int iter = (int)(launched - batch) + b;
+        for (int b = 0; b < bsize; ++b) {
+            const int rev = (b & 1);  // bsize is even: the direction alternates across batches too
+            const int iter = (int)(launched - bsize) + b;
             // the previous iteration was a test point inside this batch
             const int fin = (defer && b > 0 && b % check_every == 0) ? 1 : 0;
             int rc = launch_sweep<T>(0, coef0, coef1, xa, xb, omega, rev, iter, fin, check_every);
             if (rc) return rc;
             rc = launch_sweep<T>(1, coef0, coef1, xa, xb, omega, rev, iter, 0, check_every);
             if (rc) return rc;
-            if (defer ? b == batch - 1 : (b + 1) % check_every == 0) {
+            if (defer ? b == bsize - 1 : (b + 1) % check_every == 0) {
                 k_finish<<<1, 256, 0, L.s>>>(L.ctrl, L.partial + (iter & 1) * 2 * g_nb, 2 * g_nb, check_every);
                 TT_LAUNCH();
             }
@@ -743,20 +799,9 @@ static int run_sweeps(T* coef0, T* coef1, T* xa, T* xb, double omega, int* iters
         // the previous batch's state: this batch is already queued behind it,
         // and returns at once if the previous one finished the solve
         if (pend >= 0) {
-            TT_HIP(hipEventSynchronize(g_cev[pend]));
-            if (L.timing) ev_drain(g_hc[pend], g_cbid[pend]);
-            const Ctrl* h = g_hc[pend];
-            if (h->done) {
-                *L.hctrl = *h;
-                break;
-            }
-            if (h->it > it_b && h->rnorm > 0.0 && h->rhsnorm > 0.0) {
-                it_a = it_b;
-                lr_a = lr_b;
-                it_b = h->it;
-                lr_b = log(h->rnorm / h->rhsnorm);
-                ++nknown;
-            }
+            int rc = take_batch(pend, view, &done);
+            if (rc) return rc;
+            if (done) break;
         }
         pend = slot;
         slot ^= 1;
