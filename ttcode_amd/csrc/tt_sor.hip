@@ -60,6 +60,7 @@ static std::vector<long long> ev_bid;  // batch of each timed launch
 // Two pinned copies of the control block, so that the host reads batch k's
 // convergence state while batch k+1 already runs on the device.
 static Ctrl* g_hc[2] = {nullptr, nullptr};
+static Ctrl* g_hcd[2] = {nullptr, nullptr};  // their device-side addresses
 static hipEvent_t g_cev[2] = {nullptr, nullptr};
 static long long g_cbid[2] = {0, 0};
 static long long g_bid = 0;  // batches enqueued so far
@@ -88,6 +89,7 @@ void sor_free() {
         if (g_hc[q]) hipHostFree(g_hc[q]);
         if (g_cev[q]) hipEventDestroy(g_cev[q]);
         g_hc[q] = nullptr;
+        g_hcd[q] = nullptr;
         g_cev[q] = nullptr;
     }
 }
@@ -108,6 +110,7 @@ int sor_alloc() {
     }
     for (int q = 0; q < 2; ++q) {
         TT_HIP(hipHostMalloc((void**)&g_hc[q], sizeof(Ctrl), hipHostMallocDefault));
+        TT_HIP(hipHostGetDevicePointer((void**)&g_hcd[q], g_hc[q], 0));
         TT_HIP(hipEventCreateWithFlags(&g_cev[q], hipEventDisableTiming));
     }
     // sweep blocks: about 4 rows per wavefront, a multiple of 8 (XCDs)
@@ -138,7 +141,7 @@ __device__ inline int xcd_block(int nb) {
 // the reference path, which tests every iteration).  Every load is issued
 // before the first add, so the block waits for one memory round trip.
 __device__ inline void finish_block(Ctrl* ctrl, const double* __restrict__ part, int nparts, int step, double* sh,
-                                    int deferred) {
+                                    int deferred, Ctrl* snap = nullptr) {
     const int tid = threadIdx.x;
     double v[16];
 #pragma unroll
@@ -173,11 +176,14 @@ __device__ inline void finish_block(Ctrl* ctrl, const double* __restrict__ part,
             if (deferred) ctrl->spec_hit = 1;
         }
     }
+    // the block as the host reads it after the batch (pinned host memory)
+    if (tid == 0 && snap) *snap = *ctrl;
 }
 
-__global__ __launch_bounds__(256) void k_finish(Ctrl* ctrl, const double* __restrict__ part, int nparts, int step) {
+__global__ __launch_bounds__(256) void k_finish(Ctrl* ctrl, const double* __restrict__ part, int nparts, int step,
+                                                Ctrl* snap) {
     __shared__ double sh[4];
-    finish_block(ctrl, part, nparts, step, sh, 0);
+    finish_block(ctrl, part, nparts, step, sh, 0, snap);
 }
 
 // Arguments of the convergence test a red sweep carries for the iteration
@@ -788,11 +794,13 @@ static int run_sweeps(T* coef0, T* coef1, T* xa, T* xb, double omega, int* iters
             rc = launch_sweep<T>(1, coef0, coef1, xa, xb, omega, rev, iter, 0, check_every);
             if (rc) return rc;
             if (defer ? b == bsize - 1 : (b + 1) % check_every == 0) {
-                k_finish<<<1, 256, 0, L.s>>>(L.ctrl, L.partial + (iter & 1) * 2 * g_nb, 2 * g_nb, check_every);
+                // the batch's last test also leaves the control block in the
+                // host's pinned copy for this batch (bsize is a multiple of 4)
+                k_finish<<<1, 256, 0, L.s>>>(L.ctrl, L.partial + (iter & 1) * 2 * g_nb, 2 * g_nb, check_every,
+                                             b == bsize - 1 ? g_hcd[slot] : nullptr);
                 TT_LAUNCH();
             }
         }
-        TT_HIP(hipMemcpyAsync(g_hc[slot], L.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, L.s));
         TT_HIP(hipEventRecord(g_cev[slot], L.s));
         g_cbid[slot] = g_bid;
         last = slot;
