@@ -3,11 +3,12 @@
 //
 // The reference advects with 1-D line sweeps that fill per-line scratch
 // (S, IND, GRAD) and then the face fluxes.  Every face flux depends only on
-// the cells within two of it along its line, so here each thread evaluates
-// the fluxes through its own cell's faces directly from the state (the two
-// faces shared with a neighbour are evaluated by both; the arithmetic is the
-// reference's, so both copies are equal).  Mass fluxes, which the momentum
-// sweeps reuse through face averages, are stored once.
+// the cells within two of it along its line, so here the fluxes are formed
+// per face directly from the state, each face once: the mass fluxes by
+// k_adv_mass (stored, the momentum sweeps reuse them through face
+// averages), the transported quantities by k_adv_tiled, which shares each
+// face between its two cells through LDS.  The arithmetic is the
+// reference's, face by face.
 #include <cmath>
 
 #include "tt_common.h"
@@ -24,6 +25,23 @@ __device__ inline double vl_slope(double qm, double q0, double qp, double wlo, d
     if (!(g1 > 0.0)) return 0.0;
     double den = order == 0 ? ((q0 - qm) * whi + (qp - q0) * wlo) : ((qp - q0) * wlo + (q0 - qm) * whi);
     return 2.0 * g1 / den;
+}
+
+// The upwind cell of a face depends on the sign of the face velocity, and
+// the van Leer slope reads the cells on either side of it.  Loading only
+// after the sign is known chains two memory round trips; each face function
+// below loads the whole window (the four values straddling the face and the
+// spacings and coordinates of both candidate cells) at once and then
+// selects, with the reference's arithmetic on the selected values.
+// up != 0 selects the low-side candidate.
+__device__ inline double vl_face(bool up, double q_2, double q_1, double q0, double q1, double w_1, double w0,
+                                 double w1, int order, double* qup) {
+    // candidates: low side (q_2, q_1, q0; spacings w_1, w0), high side
+    // (q_1, q0, q1; spacings w0, w1)
+    const double qm = up ? q_2 : q_1, qc = up ? q_1 : q0, qp = up ? q0 : q1;
+    const double wlo = up ? w_1 : w0, whi = up ? w0 : w1;
+    *qup = qc;
+    return vl_slope(qm, qc, qp, wlo, whi, order);
 }
 
 struct Box3 {
@@ -85,73 +103,6 @@ __global__ void k_ed0(const double* __restrict__ T, const double* __restrict__ R
     if (t < n) E0[t] = T[t] * RH[t] * CV[t];
 }
 
-__global__ void k_adv_energy(Geo g, Met m, double dt, const double* __restrict__ E0, const double* __restrict__ V,
-                             const double* __restrict__ G, const double* __restrict__ H, double* ED) {
-    long long n = (long long)g.nx * g.ny * g.nz;
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    int i, j, k;
-    sub_ijk(t, g.nx, g.ny, 1, 1, 1, i, j, k);
-    const double half = 0.5;
-    auto E = [&](int a, int b, int c) { return E0[g.idx(a, b, c)]; };
-    double fx[2], fy[2], fz[2];
-    for (int q = 0; q < 2; ++q) {
-        int I = i + q;
-        double S = V[IX(I, j, k)] * dt;
-        int ind = V[IX(I, j, k)] > 0.0 ? I - 1 : I;
-        double gr = vl_slope(E(ind - 1, j, k), E(ind, j, k), E(ind + 1, j, k), m(TT_M_DXB, ind), m(TT_M_DXB, ind + 1), 0);
-        fx[q] = (m(TT_M_SURXA, I) - m(TT_M_CCXA, I) * S) * S * (E(ind, j, k) + (m(TT_M_XXA, I) - m(TT_M_XXB, ind) - half * S) * gr);
-        int J = j + q;
-        S = G[IX(i, J, k)] * dt;
-        ind = G[IX(i, J, k)] > 0.0 ? J - 1 : J;
-        gr = vl_slope(E(i, ind - 1, k), E(i, ind, k), E(i, ind + 1, k), m(TT_M_DYB, ind), m(TT_M_DYB, ind + 1), 0);
-        fy[q] = (m(TT_M_SURYA, J) - m(TT_M_CCYA, J) * S) * S * (E(i, ind, k) + (m(TT_M_XYA, J) - m(TT_M_XYB, ind) - half * S) * gr);
-        int K = k + q;
-        S = H[IX(i, j, K)] * dt;
-        ind = H[IX(i, j, K)] > 0.0 ? K - 1 : K;
-        gr = vl_slope(E(i, j, ind - 1), E(i, j, ind), E(i, j, ind + 1), m(TT_M_DZB, ind), m(TT_M_DZB, ind + 1), 0);
-        fz[q] = (m(TT_M_SURZA, K) - m(TT_M_CCZA, K) * S) * S * (E(i, j, ind) + (m(TT_M_XZA, K) - m(TT_M_XZB, ind) - half * S) * gr);
-    }
-    ED[IX(i, j, k)] = E(i, j, k) - (fx[1] - fx[0]) / m(TT_M_VOLXB, i) - (fy[1] - fy[0]) / m(TT_M_VOLYB, j) -
-                      (fz[1] - fz[0]) / m(TT_M_VOLZB, k);
-}
-
-// ---- radial momentum (ppadv.f:896-1073); cells i = 2..nx
-__global__ void k_adv_v(Geo g, Met m, double dt, const double* __restrict__ V, const double* __restrict__ G,
-                        const double* __restrict__ H, const double* __restrict__ FLMX, const double* __restrict__ FLMY,
-                        const double* __restrict__ FLMZ, const double* __restrict__ RHQX, double* VD) {
-    long long n = (long long)(g.nx - 1) * g.ny * g.nz;
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    int i, j, k;
-    sub_ijk(t, g.nx - 1, g.ny, 2, 1, 1, i, j, k);
-    const double half = 0.5;
-    Box3 Vb{V, g};
-    double fx[2], fy[2], fz[2];
-    for (int q = 0; q < 2; ++q) {
-        int I = i - 1 + q;  // FLX(I) for I = i-1, i
-        double S = (Vb(I, j, k) + Vb(I + 1, j, k)) * dt / 2.0;
-        int ind = S > 0.0 ? I : I + 1;
-        double gr = vl_slope(Vb(ind - 1, j, k), Vb(ind, j, k), Vb(ind + 1, j, k), m(TT_M_DXA, ind), m(TT_M_DXA, ind + 1), 1);
-        fx[q] = half * (FLMX[IX(I, j, k)] + FLMX[IX(I + 1, j, k)]) *
-                (Vb(ind, j, k) + (m(TT_M_XXB, I) - m(TT_M_XXA, ind) - half * S) * gr);
-        int J = j + q;
-        S = (m(TT_M_DDX1, i) * G[IX(i, J, k)] + m(TT_M_DDX0, i) * G[IX(i - 1, J, k)]) * dt;
-        ind = S > 0.0 ? J - 1 : J;
-        gr = vl_slope(Vb(i, ind - 1, k), Vb(i, ind, k), Vb(i, ind + 1, k), m(TT_M_DYB, ind), m(TT_M_DYB, ind + 1), 1);
-        fy[q] = (m(TT_M_DDX1, i) * FLMY[IX(i, J, k)] + m(TT_M_DDX0, i) * FLMY[IX(i - 1, J, k)]) *
-                (Vb(i, ind, k) + (m(TT_M_XYA, J) - m(TT_M_XYB, ind) - half * S) * gr);
-        int K = k + q;
-        S = (m(TT_M_DDX1, i) * H[IX(i, j, K)] + m(TT_M_DDX0, i) * H[IX(i - 1, j, K)]) * dt;
-        ind = S > 0.0 ? K - 1 : K;
-        gr = vl_slope(Vb(i, j, ind - 1), Vb(i, j, ind), Vb(i, j, ind + 1), m(TT_M_DZB, ind), m(TT_M_DZB, ind + 1), 1);
-        fz[q] = (m(TT_M_DDX1, i) * FLMZ[IX(i, j, K)] + m(TT_M_DDX0, i) * FLMZ[IX(i - 1, j, K)]) *
-                (Vb(i, j, ind) + (m(TT_M_XZA, K) - m(TT_M_XZB, ind) - half * S) * gr);
-    }
-    VD[IX(i, j, k)] = Vb(i, j, k) * RHQX[IX(i, j, k)] - (fx[1] - fx[0]) / m(TT_M_VOLXA, i) -
-                      (fy[1] - fy[0]) / m(TT_M_VOLYB, j) - (fz[1] - fz[0]) / m(TT_M_VOLZB, k);
-}
-
 // GD = (G + OMROT) r^2 cos^2 (ppadv.f:1078-1084); the phi ghosts equal the
 // formula at the wrapped column because G's ghosts were wrapped by BOUNDS.
 __global__ void k_gd_init(Geo g, Met m, double om, const double* __restrict__ G, double* GD) {
@@ -164,45 +115,6 @@ __global__ void k_gd_init(Geo g, Met m, double om, const double* __restrict__ G,
     GD[t] = (G[t] + om) * (gx * gx) * (gz * gz);
 }
 
-// ---- azimuthal angular momentum (ppadv.f:1096-1266)
-__global__ void k_adv_g(Geo g, Met m, double dt, const double* __restrict__ V, const double* __restrict__ G,
-                        const double* __restrict__ H, const double* __restrict__ GD, const double* __restrict__ FLMX,
-                        const double* __restrict__ FLMY, const double* __restrict__ FLMZ,
-                        const double* __restrict__ RHQY, double* GDN) {
-    long long n = (long long)g.nx * g.ny * g.nz;
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    int i, j, k;
-    sub_ijk(t, g.nx, g.ny, 1, 1, 1, i, j, k);
-    const double half = 0.5;
-    Box3 D{GD, g};
-    double fx[2], fy[2], fz[2];
-    for (int q = 0; q < 2; ++q) {
-        int I = i + q;
-        double S = (m(TT_M_DDY1, j) * V[IX(I, j, k)] + m(TT_M_DDY0, j) * V[IX(I, j - 1, k)]) * dt;
-        int ind = S > 0.0 ? I - 1 : I;
-        double gr = vl_slope(D(ind - 1, j, k), D(ind, j, k), D(ind + 1, j, k), m(TT_M_DXB, ind), m(TT_M_DXB, ind + 1), 1);
-        fx[q] = (m(TT_M_DDY1, j) * FLMX[IX(I, j, k)] + m(TT_M_DDY0, j) * FLMX[IX(I, j - 1, k)]) *
-                (D(ind, j, k) + (m(TT_M_XXA, I) - m(TT_M_XXB, ind) - half * S) * gr);
-        // FLY(J) for J = j-1, j; FLY(0) is the periodic copy of FLY(ny)
-        int J = j - 1 + q;
-        if (J == 0) J = g.ny;
-        S = (m(TT_M_DCY0, J + 1) * G[IX(i, J, k)] + m(TT_M_DCY1, J + 1) * G[IX(i, J + 1, k)]) * dt;
-        ind = S > 0.0 ? J : J + 1;
-        gr = vl_slope(D(i, ind - 1, k), D(i, ind, k), D(i, ind + 1, k), m(TT_M_DYA, ind), m(TT_M_DYA, ind + 1), 1);
-        fy[q] = (m(TT_M_DCY0, J + 1) * FLMY[IX(i, J, k)] + m(TT_M_DCY1, J + 1) * FLMY[IX(i, J + 1, k)]) *
-                (D(i, ind, k) + (m(TT_M_XYB, J) - m(TT_M_XYA, ind) - half * S) * gr);
-        int K = k + q;
-        S = (m(TT_M_DDY1, j) * H[IX(i, j, K)] + m(TT_M_DDY0, j) * H[IX(i, j - 1, K)]) * dt;
-        ind = S > 0.0 ? K - 1 : K;
-        gr = vl_slope(D(i, j, ind - 1), D(i, j, ind), D(i, j, ind + 1), m(TT_M_DZB, ind), m(TT_M_DZB, ind + 1), 1);
-        fz[q] = (m(TT_M_DDY1, j) * FLMZ[IX(i, j, K)] + m(TT_M_DDY0, j) * FLMZ[IX(i, j - 1, K)]) *
-                (D(i, j, ind) + (m(TT_M_XZA, K) - m(TT_M_XZB, ind) - half * S) * gr);
-    }
-    GDN[IX(i, j, k)] = D(i, j, k) * RHQY[IX(i, j, k)] - (fx[1] - fx[0]) / m(TT_M_VOLXB, i) -
-                       (fy[1] - fy[0]) / m(TT_M_VOLYA, j) - (fz[1] - fz[0]) / m(TT_M_VOLZB, k);
-}
-
 __global__ void k_hd_init(Geo g, Met m, const double* __restrict__ H, double* HD) {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= g.n3) return;
@@ -213,41 +125,334 @@ __global__ void k_hd_init(Geo g, Met m, const double* __restrict__ H, double* HD
     HD[t] = H[t] * (gh * gh);
 }
 
-// ---- meridional angular momentum (ppadv.f:1278-1461); cells k = 2..nz
-__global__ void k_adv_h(Geo g, Met m, double dt, const double* __restrict__ V, const double* __restrict__ G,
-                        const double* __restrict__ H, const double* __restrict__ HD, const double* __restrict__ FLMX,
-                        const double* __restrict__ FLMY, const double* __restrict__ FLMZ,
-                        const double* __restrict__ RHQZ, double* HDN) {
-    long long n = (long long)g.nx * g.ny * (g.nz - 1);
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    int i, j, k;
-    sub_ijk(t, g.nx, g.ny, 1, 1, 2, i, j, k);
-    const double half = 0.5;
-    Box3 D{HD, g};
-    double fx[2], fy[2], fz[2];
-    for (int q = 0; q < 2; ++q) {
-        int I = i + q;
-        double S = (m(TT_M_DDZ1, k) * V[IX(I, j, k)] + m(TT_M_DDZ0, k) * V[IX(I, j, k - 1)]) * dt;
-        int ind = S > 0.0 ? I - 1 : I;
-        double gr = vl_slope(D(ind - 1, j, k), D(ind, j, k), D(ind + 1, j, k), m(TT_M_DXB, ind), m(TT_M_DXB, ind + 1), 1);
-        fx[q] = (m(TT_M_DDZ1, k) * FLMX[IX(I, j, k)] + m(TT_M_DDZ0, k) * FLMX[IX(I, j, k - 1)]) *
-                (D(ind, j, k) + (m(TT_M_XXA, I) - m(TT_M_XXB, ind) - half * S) * gr);
-        int J = j + q;
-        S = (m(TT_M_DDZ1, k) * G[IX(i, J, k)] + m(TT_M_DDZ0, k) * G[IX(i, J, k - 1)]) * dt;
-        ind = S > 0.0 ? J - 1 : J;
-        gr = vl_slope(D(i, ind - 1, k), D(i, ind, k), D(i, ind + 1, k), m(TT_M_DYB, ind), m(TT_M_DYB, ind + 1), 1);
-        fy[q] = (m(TT_M_DDZ1, k) * FLMY[IX(i, J, k)] + m(TT_M_DDZ0, k) * FLMY[IX(i, J, k - 1)]) *
-                (D(i, ind, k) + (m(TT_M_XYA, J) - m(TT_M_XYB, ind) - half * S) * gr);
-        int K = k - 1 + q;  // FLZ(K) for K = k-1, k
-        S = half * (H[IX(i, j, K)] + H[IX(i, j, K + 1)]) * dt;
-        ind = S > 0.0 ? K : K + 1;
-        gr = vl_slope(D(i, j, ind - 1), D(i, j, ind), D(i, j, ind + 1), m(TT_M_DZA, ind), m(TT_M_DZA, ind + 1), 1);
-        fz[q] = half * (FLMZ[IX(i, j, K)] + FLMZ[IX(i, j, K + 1)]) *
-                (D(i, j, ind) + (m(TT_M_XZB, K) - m(TT_M_XZA, ind) - half * S) * gr);
+// ---- face-shared sweeps.  The four transport updates above evaluate, per
+// cell, the fluxes through all six of its faces, so every face flux is
+// formed twice.  k_adv_tiled forms each face flux once: a block takes a tile
+// of 32 (i) x 8 (j) cells and marches through 2 planes of k (longer marches
+// share more z faces but leave too few blocks to hide the loads' latency:
+// 2 planes measured best at 128x128x256).  Per
+// plane each cell lane forms the x face on its low side and the y face on its
+// low side into LDS (double-buffered, one barrier per plane) and the z face
+// on its high side in a register, which is the low z face of the next plane;
+// a fifth wavefront forms the tile's extra high x faces (8) and high y faces
+// (32).  The flux expressions are the reference's, face by face, so the
+// results equal the per-cell form bit for bit.
+struct AdvArgs {
+    const double *A;                 // advected quantity: rho c_v T, V, GD or HD
+    const double *V, *G, *H;         // velocities
+    const double *FX, *FY, *FZ;      // mass fluxes FLMX, FLMY, FLMZ
+    const double *RHQ;               // face density of the momentum (unused for energy)
+    double* out;
+    double dt;
+};
+
+constexpr int ADV_TI = 32, ADV_TJ = 8, ADV_KC = 2;
+
+// KIND 0: thermal energy (ppadv.f:741-891), cell faces I = i, i+1 etc.; the
+// upwind cell is I-1 for a positive velocity, else I
+struct AdvE {
+    static constexpr int ox = 0, oy = 0, oz = 0;
+    __device__ static int ilo(const Geo&) { return 1; }
+    __device__ static int ihi(const Geo& g) { return g.nx; }
+    __device__ static int jlo(const Geo&) { return 1; }
+    __device__ static int jhi(const Geo& g) { return g.ny; }
+    __device__ static int klo(const Geo&) { return 1; }
+    __device__ static int khi(const Geo& g) { return g.nz; }
+    __device__ static double fx(const Geo& g, const Met& m, const AdvArgs& a, int I, int j, int k) {
+        const double half = 0.5;
+        Box3 E{a.A, g};
+        const double v = a.V[IX(I, j, k)];
+        const double e_2 = E(I - 2, j, k), e_1 = E(I - 1, j, k), e0 = E(I, j, k), e1 = E(I + 1, j, k);
+        const double d_1 = m(TT_M_DXB, I - 1), d0 = m(TT_M_DXB, I), d1 = m(TT_M_DXB, I + 1);
+        const double x_1 = m(TT_M_XXB, I - 1), x0 = m(TT_M_XXB, I);
+        const double sa = m(TT_M_SURXA, I), ca = m(TT_M_CCXA, I), xa = m(TT_M_XXA, I);
+        double S = v * a.dt;
+        const bool up = v > 0.0;
+        double eu;
+        double gr = vl_face(up, e_2, e_1, e0, e1, d_1, d0, d1, 0, &eu);
+        return (sa - ca * S) * S * (eu + (xa - (up ? x_1 : x0) - half * S) * gr);
     }
-    HDN[IX(i, j, k)] = D(i, j, k) * RHQZ[IX(i, j, k)] - (fx[1] - fx[0]) / m(TT_M_VOLXB, i) -
-                       (fy[1] - fy[0]) / m(TT_M_VOLYB, j) - (fz[1] - fz[0]) / m(TT_M_VOLZA, k);
+    __device__ static double fy(const Geo& g, const Met& m, const AdvArgs& a, int i, int J, int k) {
+        const double half = 0.5;
+        Box3 E{a.A, g};
+        const double v = a.G[IX(i, J, k)];
+        const double e_2 = E(i, J - 2, k), e_1 = E(i, J - 1, k), e0 = E(i, J, k), e1 = E(i, J + 1, k);
+        const double d_1 = m(TT_M_DYB, J - 1), d0 = m(TT_M_DYB, J), d1 = m(TT_M_DYB, J + 1);
+        const double x_1 = m(TT_M_XYB, J - 1), x0 = m(TT_M_XYB, J);
+        const double sa = m(TT_M_SURYA, J), ca = m(TT_M_CCYA, J), xa = m(TT_M_XYA, J);
+        double S = v * a.dt;
+        const bool up = v > 0.0;
+        double eu;
+        double gr = vl_face(up, e_2, e_1, e0, e1, d_1, d0, d1, 0, &eu);
+        return (sa - ca * S) * S * (eu + (xa - (up ? x_1 : x0) - half * S) * gr);
+    }
+    __device__ static double fz(const Geo& g, const Met& m, const AdvArgs& a, int i, int j, int K) {
+        const double half = 0.5;
+        Box3 E{a.A, g};
+        const double v = a.H[IX(i, j, K)];
+        const double e_2 = E(i, j, K - 2), e_1 = E(i, j, K - 1), e0 = E(i, j, K), e1 = E(i, j, K + 1);
+        const double d_1 = m(TT_M_DZB, K - 1), d0 = m(TT_M_DZB, K), d1 = m(TT_M_DZB, K + 1);
+        const double x_1 = m(TT_M_XZB, K - 1), x0 = m(TT_M_XZB, K);
+        const double sa = m(TT_M_SURZA, K), ca = m(TT_M_CCZA, K), xa = m(TT_M_XZA, K);
+        double S = v * a.dt;
+        const bool up = v > 0.0;
+        double eu;
+        double gr = vl_face(up, e_2, e_1, e0, e1, d_1, d0, d1, 0, &eu);
+        return (sa - ca * S) * S * (eu + (xa - (up ? x_1 : x0) - half * S) * gr);
+    }
+    __device__ static void finish(const Geo& g, const Met& m, const AdvArgs& a, int i, int j, int k, double fx0,
+                                  double fx1, double fy0, double fy1, double fz0, double fz1) {
+        a.out[IX(i, j, k)] = a.A[IX(i, j, k)] - (fx1 - fx0) / m(TT_M_VOLXB, i) - (fy1 - fy0) / m(TT_M_VOLYB, j) -
+                             (fz1 - fz0) / m(TT_M_VOLZB, k);
+    }
+};
+
+// KIND 1: radial momentum (ppadv.f:896-1073), cells i = 2..nx, x faces
+// FLX(I) at I = i-1, i (upwind cell I for a positive S, else I+1)
+struct AdvV {
+    static constexpr int ox = -1, oy = 0, oz = 0;
+    __device__ static int ilo(const Geo&) { return 2; }
+    __device__ static int ihi(const Geo& g) { return g.nx; }
+    __device__ static int jlo(const Geo&) { return 1; }
+    __device__ static int jhi(const Geo& g) { return g.ny; }
+    __device__ static int klo(const Geo&) { return 1; }
+    __device__ static int khi(const Geo& g) { return g.nz; }
+    __device__ static double fx(const Geo& g, const Met& m, const AdvArgs& a, int I, int j, int k) {
+        const double half = 0.5;
+        Box3 Vb{a.V, g};
+        const double v_1 = Vb(I - 1, j, k), v0 = Vb(I, j, k), v1 = Vb(I + 1, j, k), v2 = Vb(I + 2, j, k);
+        const double f0 = a.FX[IX(I, j, k)], f1 = a.FX[IX(I + 1, j, k)];
+        const double d0 = m(TT_M_DXA, I), d1 = m(TT_M_DXA, I + 1), d2 = m(TT_M_DXA, I + 2);
+        const double x0 = m(TT_M_XXA, I), x1 = m(TT_M_XXA, I + 1), xb = m(TT_M_XXB, I);
+        double S = (v0 + v1) * a.dt / 2.0;
+        const bool up = S > 0.0;
+        double vu;
+        double gr = vl_face(up, v_1, v0, v1, v2, d0, d1, d2, 1, &vu);
+        return half * (f0 + f1) * (vu + (xb - (up ? x0 : x1) - half * S) * gr);
+    }
+    __device__ static double fy(const Geo& g, const Met& m, const AdvArgs& a, int i, int J, int k) {
+        const double half = 0.5;
+        Box3 Vb{a.V, g};
+        const double p1 = m(TT_M_DDX1, i), p0 = m(TT_M_DDX0, i);
+        const double g1 = a.G[IX(i, J, k)], g0 = a.G[IX(i - 1, J, k)];
+        const double f1 = a.FY[IX(i, J, k)], f0 = a.FY[IX(i - 1, J, k)];
+        const double v_2 = Vb(i, J - 2, k), v_1 = Vb(i, J - 1, k), v0 = Vb(i, J, k), v1 = Vb(i, J + 1, k);
+        const double d_1 = m(TT_M_DYB, J - 1), d0 = m(TT_M_DYB, J), d1 = m(TT_M_DYB, J + 1);
+        const double x_1 = m(TT_M_XYB, J - 1), x0 = m(TT_M_XYB, J), xa = m(TT_M_XYA, J);
+        double S = (p1 * g1 + p0 * g0) * a.dt;
+        const bool up = S > 0.0;
+        double vu;
+        double gr = vl_face(up, v_2, v_1, v0, v1, d_1, d0, d1, 1, &vu);
+        return (p1 * f1 + p0 * f0) * (vu + (xa - (up ? x_1 : x0) - half * S) * gr);
+    }
+    __device__ static double fz(const Geo& g, const Met& m, const AdvArgs& a, int i, int j, int K) {
+        const double half = 0.5;
+        Box3 Vb{a.V, g};
+        const double p1 = m(TT_M_DDX1, i), p0 = m(TT_M_DDX0, i);
+        const double h1 = a.H[IX(i, j, K)], h0 = a.H[IX(i - 1, j, K)];
+        const double f1 = a.FZ[IX(i, j, K)], f0 = a.FZ[IX(i - 1, j, K)];
+        const double v_2 = Vb(i, j, K - 2), v_1 = Vb(i, j, K - 1), v0 = Vb(i, j, K), v1 = Vb(i, j, K + 1);
+        const double d_1 = m(TT_M_DZB, K - 1), d0 = m(TT_M_DZB, K), d1 = m(TT_M_DZB, K + 1);
+        const double x_1 = m(TT_M_XZB, K - 1), x0 = m(TT_M_XZB, K), xa = m(TT_M_XZA, K);
+        double S = (p1 * h1 + p0 * h0) * a.dt;
+        const bool up = S > 0.0;
+        double vu;
+        double gr = vl_face(up, v_2, v_1, v0, v1, d_1, d0, d1, 1, &vu);
+        return (p1 * f1 + p0 * f0) * (vu + (xa - (up ? x_1 : x0) - half * S) * gr);
+    }
+    __device__ static void finish(const Geo& g, const Met& m, const AdvArgs& a, int i, int j, int k, double fx0,
+                                  double fx1, double fy0, double fy1, double fz0, double fz1) {
+        a.out[IX(i, j, k)] = a.V[IX(i, j, k)] * a.RHQ[IX(i, j, k)] - (fx1 - fx0) / m(TT_M_VOLXA, i) -
+                             (fy1 - fy0) / m(TT_M_VOLYB, j) - (fz1 - fz0) / m(TT_M_VOLZB, k);
+    }
+};
+
+// KIND 2: azimuthal angular momentum (ppadv.f:1096-1266), y faces FLY(J) at
+// J = j-1, j (upwind J for a positive S, else J+1); FLY(0) is the periodic
+// copy of FLY(ny)
+struct AdvG {
+    static constexpr int ox = 0, oy = -1, oz = 0;
+    __device__ static int ilo(const Geo&) { return 1; }
+    __device__ static int ihi(const Geo& g) { return g.nx; }
+    __device__ static int jlo(const Geo&) { return 1; }
+    __device__ static int jhi(const Geo& g) { return g.ny; }
+    __device__ static int klo(const Geo&) { return 1; }
+    __device__ static int khi(const Geo& g) { return g.nz; }
+    __device__ static double fx(const Geo& g, const Met& m, const AdvArgs& a, int I, int j, int k) {
+        const double half = 0.5;
+        Box3 D{a.A, g};
+        const double p1 = m(TT_M_DDY1, j), p0 = m(TT_M_DDY0, j);
+        const double u1 = a.V[IX(I, j, k)], u0 = a.V[IX(I, j - 1, k)];
+        const double f1 = a.FX[IX(I, j, k)], f0 = a.FX[IX(I, j - 1, k)];
+        const double q_2 = D(I - 2, j, k), q_1 = D(I - 1, j, k), q0 = D(I, j, k), q1 = D(I + 1, j, k);
+        const double d_1 = m(TT_M_DXB, I - 1), d0 = m(TT_M_DXB, I), d1 = m(TT_M_DXB, I + 1);
+        const double x_1 = m(TT_M_XXB, I - 1), x0 = m(TT_M_XXB, I), xa = m(TT_M_XXA, I);
+        double S = (p1 * u1 + p0 * u0) * a.dt;
+        const bool up = S > 0.0;
+        double qu;
+        double gr = vl_face(up, q_2, q_1, q0, q1, d_1, d0, d1, 1, &qu);
+        return (p1 * f1 + p0 * f0) * (qu + (xa - (up ? x_1 : x0) - half * S) * gr);
+    }
+    __device__ static double fy(const Geo& g, const Met& m, const AdvArgs& a, int i, int J, int k) {
+        const double half = 0.5;
+        Box3 D{a.A, g};
+        if (J == 0) J = g.ny;
+        const double c0 = m(TT_M_DCY0, J + 1), c1 = m(TT_M_DCY1, J + 1);
+        const double g0 = a.G[IX(i, J, k)], g1 = a.G[IX(i, J + 1, k)];
+        const double f0 = a.FY[IX(i, J, k)], f1 = a.FY[IX(i, J + 1, k)];
+        const double q_1 = D(i, J - 1, k), q0 = D(i, J, k), q1 = D(i, J + 1, k), q2 = D(i, J + 2, k);
+        const double d0 = m(TT_M_DYA, J), d1 = m(TT_M_DYA, J + 1), d2 = m(TT_M_DYA, J + 2);
+        const double y0 = m(TT_M_XYA, J), y1 = m(TT_M_XYA, J + 1), yb = m(TT_M_XYB, J);
+        double S = (c0 * g0 + c1 * g1) * a.dt;
+        const bool up = S > 0.0;
+        double qu;
+        double gr = vl_face(up, q_1, q0, q1, q2, d0, d1, d2, 1, &qu);
+        return (c0 * f0 + c1 * f1) * (qu + (yb - (up ? y0 : y1) - half * S) * gr);
+    }
+    __device__ static double fz(const Geo& g, const Met& m, const AdvArgs& a, int i, int j, int K) {
+        const double half = 0.5;
+        Box3 D{a.A, g};
+        const double p1 = m(TT_M_DDY1, j), p0 = m(TT_M_DDY0, j);
+        const double h1 = a.H[IX(i, j, K)], h0 = a.H[IX(i, j - 1, K)];
+        const double f1 = a.FZ[IX(i, j, K)], f0 = a.FZ[IX(i, j - 1, K)];
+        const double q_2 = D(i, j, K - 2), q_1 = D(i, j, K - 1), q0 = D(i, j, K), q1 = D(i, j, K + 1);
+        const double d_1 = m(TT_M_DZB, K - 1), d0 = m(TT_M_DZB, K), d1 = m(TT_M_DZB, K + 1);
+        const double x_1 = m(TT_M_XZB, K - 1), x0 = m(TT_M_XZB, K), xa = m(TT_M_XZA, K);
+        double S = (p1 * h1 + p0 * h0) * a.dt;
+        const bool up = S > 0.0;
+        double qu;
+        double gr = vl_face(up, q_2, q_1, q0, q1, d_1, d0, d1, 1, &qu);
+        return (p1 * f1 + p0 * f0) * (qu + (xa - (up ? x_1 : x0) - half * S) * gr);
+    }
+    __device__ static void finish(const Geo& g, const Met& m, const AdvArgs& a, int i, int j, int k, double fx0,
+                                  double fx1, double fy0, double fy1, double fz0, double fz1) {
+        a.out[IX(i, j, k)] = a.A[IX(i, j, k)] * a.RHQ[IX(i, j, k)] - (fx1 - fx0) / m(TT_M_VOLXB, i) -
+                             (fy1 - fy0) / m(TT_M_VOLYA, j) - (fz1 - fz0) / m(TT_M_VOLZB, k);
+    }
+};
+
+// KIND 3: meridional angular momentum (ppadv.f:1278-1461), cells k = 2..nz,
+// z faces FLZ(K) at K = k-1, k (upwind K for a positive S, else K+1)
+struct AdvH {
+    static constexpr int ox = 0, oy = 0, oz = -1;
+    __device__ static int ilo(const Geo&) { return 1; }
+    __device__ static int ihi(const Geo& g) { return g.nx; }
+    __device__ static int jlo(const Geo&) { return 1; }
+    __device__ static int jhi(const Geo& g) { return g.ny; }
+    __device__ static int klo(const Geo&) { return 2; }
+    __device__ static int khi(const Geo& g) { return g.nz; }
+    __device__ static double fx(const Geo& g, const Met& m, const AdvArgs& a, int I, int j, int k) {
+        const double half = 0.5;
+        Box3 D{a.A, g};
+        const double p1 = m(TT_M_DDZ1, k), p0 = m(TT_M_DDZ0, k);
+        const double u1 = a.V[IX(I, j, k)], u0 = a.V[IX(I, j, k - 1)];
+        const double f1 = a.FX[IX(I, j, k)], f0 = a.FX[IX(I, j, k - 1)];
+        const double q_2 = D(I - 2, j, k), q_1 = D(I - 1, j, k), q0 = D(I, j, k), q1 = D(I + 1, j, k);
+        const double d_1 = m(TT_M_DXB, I - 1), d0 = m(TT_M_DXB, I), d1 = m(TT_M_DXB, I + 1);
+        const double x_1 = m(TT_M_XXB, I - 1), x0 = m(TT_M_XXB, I), xa = m(TT_M_XXA, I);
+        double S = (p1 * u1 + p0 * u0) * a.dt;
+        const bool up = S > 0.0;
+        double qu;
+        double gr = vl_face(up, q_2, q_1, q0, q1, d_1, d0, d1, 1, &qu);
+        return (p1 * f1 + p0 * f0) * (qu + (xa - (up ? x_1 : x0) - half * S) * gr);
+    }
+    __device__ static double fy(const Geo& g, const Met& m, const AdvArgs& a, int i, int J, int k) {
+        const double half = 0.5;
+        Box3 D{a.A, g};
+        const double p1 = m(TT_M_DDZ1, k), p0 = m(TT_M_DDZ0, k);
+        const double g1 = a.G[IX(i, J, k)], g0 = a.G[IX(i, J, k - 1)];
+        const double f1 = a.FY[IX(i, J, k)], f0 = a.FY[IX(i, J, k - 1)];
+        const double q_2 = D(i, J - 2, k), q_1 = D(i, J - 1, k), q0 = D(i, J, k), q1 = D(i, J + 1, k);
+        const double d_1 = m(TT_M_DYB, J - 1), d0 = m(TT_M_DYB, J), d1 = m(TT_M_DYB, J + 1);
+        const double x_1 = m(TT_M_XYB, J - 1), x0 = m(TT_M_XYB, J), xa = m(TT_M_XYA, J);
+        double S = (p1 * g1 + p0 * g0) * a.dt;
+        const bool up = S > 0.0;
+        double qu;
+        double gr = vl_face(up, q_2, q_1, q0, q1, d_1, d0, d1, 1, &qu);
+        return (p1 * f1 + p0 * f0) * (qu + (xa - (up ? x_1 : x0) - half * S) * gr);
+    }
+    __device__ static double fz(const Geo& g, const Met& m, const AdvArgs& a, int i, int j, int K) {
+        const double half = 0.5;
+        Box3 D{a.A, g};
+        const double h0 = a.H[IX(i, j, K)], h1 = a.H[IX(i, j, K + 1)];
+        const double f0 = a.FZ[IX(i, j, K)], f1 = a.FZ[IX(i, j, K + 1)];
+        const double q_1 = D(i, j, K - 1), q0 = D(i, j, K), q1 = D(i, j, K + 1), q2 = D(i, j, K + 2);
+        const double d0 = m(TT_M_DZA, K), d1 = m(TT_M_DZA, K + 1), d2 = m(TT_M_DZA, K + 2);
+        const double z0 = m(TT_M_XZA, K), z1 = m(TT_M_XZA, K + 1), zb = m(TT_M_XZB, K);
+        double S = half * (h0 + h1) * a.dt;
+        const bool up = S > 0.0;
+        double qu;
+        double gr = vl_face(up, q_1, q0, q1, q2, d0, d1, d2, 1, &qu);
+        return half * (f0 + f1) * (qu + (zb - (up ? z0 : z1) - half * S) * gr);
+    }
+    __device__ static void finish(const Geo& g, const Met& m, const AdvArgs& a, int i, int j, int k, double fx0,
+                                  double fx1, double fy0, double fy1, double fz0, double fz1) {
+        a.out[IX(i, j, k)] = a.A[IX(i, j, k)] * a.RHQ[IX(i, j, k)] - (fx1 - fx0) / m(TT_M_VOLXB, i) -
+                             (fy1 - fy0) / m(TT_M_VOLYB, j) - (fz1 - fz0) / m(TT_M_VOLZA, k);
+    }
+};
+
+template <class K>
+__global__ __launch_bounds__(320) void k_adv_tiled(Geo g, Met m, AdvArgs a) {
+    __shared__ double sx[2][ADV_TJ][ADV_TI + 1];  // x faces of the tile rows, plus the high one
+    __shared__ double sy[2][ADV_TJ + 1][ADV_TI];  // y faces of the tile columns, plus the high row
+    const int ilo = K::ilo(g), ihi = K::ihi(g), jlo = K::jlo(g), jhi = K::jhi(g);
+    const int klo = K::klo(g), khi = K::khi(g);
+    const int nti = (ihi - ilo + ADV_TI) / ADV_TI, ntj = (jhi - jlo + ADV_TJ) / ADV_TJ;
+    int b = blockIdx.x;
+    const int ti = b % nti;
+    b /= nti;
+    const int tj = b % ntj, kc = b / ntj;
+    const int ci0 = ilo + ti * ADV_TI, cj0 = jlo + tj * ADV_TJ;
+    const int k0 = klo + kc * ADV_KC;
+    const int k1 = min(khi, k0 + ADV_KC - 1);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const bool helper = w == 4;
+    const int r = 2 * w + (l >> 5), c = l & 31;
+    const int i = ci0 + c, j = cj0 + r;
+    const bool vi = i <= ihi, vj = j <= jhi;
+    const bool cell = !helper && vi && vj;
+    // the highest face of each family
+    const int Ihi = ihi + K::ox + 1, Jhi = jhi + K::oy + 1;
+    double zlo = 0.0;
+    int p = 0;
+    for (int k = k0; k <= k1; ++k, p ^= 1) {
+        double zhi = 0.0;
+        if (!helper) {
+            // the low x face of the lane's cell is also the high face of the
+            // cell before it, and of the last valid cell when this one is
+            // past the end of the row
+            if (vj && i + K::ox <= Ihi) sx[p][r][c] = K::fx(g, m, a, i + K::ox, j, k);
+            if (vi && j + K::oy <= Jhi) sy[p][r][c] = K::fy(g, m, a, i, j + K::oy, k);
+            if (cell) {
+                zhi = K::fz(g, m, a, i, j, k + K::oz + 1);
+                if (k == k0) zlo = K::fz(g, m, a, i, j, k + K::oz);
+            }
+        } else if (l < ADV_TJ) {
+            const int jr = cj0 + l, I = ci0 + ADV_TI + K::ox;
+            if (jr <= jhi && I <= Ihi) sx[p][l][ADV_TI] = K::fx(g, m, a, I, jr, k);
+        } else if (l < ADV_TJ + ADV_TI) {
+            const int ic = ci0 + (l - ADV_TJ), J = cj0 + ADV_TJ + K::oy;
+            if (ic <= ihi && J <= Jhi) sy[p][ADV_TJ][l - ADV_TJ] = K::fy(g, m, a, ic, J, k);
+        }
+        __syncthreads();
+        if (cell) {
+            K::finish(g, m, a, i, j, k, sx[p][r][c], sx[p][r][c + 1], sy[p][r][c], sy[p][r + 1][c], zlo, zhi);
+            zlo = zhi;
+        }
+    }
+}
+
+template <class K>
+static int launch_adv_tiled(const AdvArgs& a) {
+    Geo& g = L.g;
+    // host copies of the cell ranges (the device functions above)
+    const int ni = K::ox == -1 ? g.nx - 1 : g.nx, nj = g.ny, nk = K::oz == -1 ? g.nz - 1 : g.nz;
+    const long long nb = (long long)((ni + ADV_TI - 1) / ADV_TI) * ((nj + ADV_TJ - 1) / ADV_TJ) *
+                         ((nk + ADV_KC - 1) / ADV_KC);
+    k_adv_tiled<K><<<(unsigned)nb, 320, 0, L.s>>>(g, met(), a);
+    TT_LAUNCH();
+    return 0;
 }
 
 __global__ void k_adv_rho(Geo g, Met m, double* RH, const double* __restrict__ FLMX, const double* __restrict__ FLMY,
@@ -295,22 +500,18 @@ int advect_impl(double dt) {
     long long nc = (long long)g.nx * g.ny * g.nz;
     k_ed0<<<nblk(g.n3, 256), 256, 0, L.s>>>(f[TT_T], f[TT_RH], f[TT_CV], L.TOLD, g.n3);
     TT_LAUNCH();
-    k_adv_energy<<<nblk(nc, 256), 256, 0, L.s>>>(g, met(), dt, L.TOLD, f[TT_V], f[TT_G], f[TT_H], L.ED);
-    TT_LAUNCH();
-    long long nv = (long long)(g.nx - 1) * g.ny * g.nz;
-    k_adv_v<<<nblk(nv, 256), 256, 0, L.s>>>(g, met(), dt, f[TT_V], f[TT_G], f[TT_H], L.FLMX, L.FLMY, L.FLMZ, f[TT_RHQX], L.VD);
-    TT_LAUNCH();
+    AdvArgs aa{L.TOLD, f[TT_V], f[TT_G], f[TT_H], L.FLMX, L.FLMY, L.FLMZ, nullptr, L.ED, dt};
+    if ((rc = launch_adv_tiled<AdvE>(aa))) return rc;
+    aa.A = f[TT_V]; aa.RHQ = f[TT_RHQX]; aa.out = L.VD;
+    if ((rc = launch_adv_tiled<AdvV>(aa))) return rc;
     k_gd_init<<<nblk(g.n3, 256), 256, 0, L.s>>>(g, met(), L.p.omrot, f[TT_G], f[TT_GD]);
     TT_LAUNCH();
-    k_adv_g<<<nblk(nc, 256), 256, 0, L.s>>>(g, met(), dt, f[TT_V], f[TT_G], f[TT_H], f[TT_GD], L.FLMX, L.FLMY, L.FLMZ,
-                                            f[TT_RHQY], L.GDN);
-    TT_LAUNCH();
+    aa.A = f[TT_GD]; aa.RHQ = f[TT_RHQY]; aa.out = L.GDN;
+    if ((rc = launch_adv_tiled<AdvG>(aa))) return rc;
     k_hd_init<<<nblk(g.n3, 256), 256, 0, L.s>>>(g, met(), f[TT_H], L.HH);
     TT_LAUNCH();
-    long long nh = (long long)g.nx * g.ny * (g.nz - 1);
-    k_adv_h<<<nblk(nh, 256), 256, 0, L.s>>>(g, met(), dt, f[TT_V], f[TT_G], f[TT_H], L.HH, L.FLMX, L.FLMY, L.FLMZ,
-                                            f[TT_RHQZ], L.HDN);
-    TT_LAUNCH();
+    aa.A = L.HH; aa.RHQ = f[TT_RHQZ]; aa.out = L.HDN;
+    if ((rc = launch_adv_tiled<AdvH>(aa))) return rc;
     k_adv_rho<<<nblk(nc, 256), 256, 0, L.s>>>(g, met(), f[TT_RH], L.FLMX, L.FLMY, L.FLMZ);
     TT_LAUNCH();
     if ((rc = bounds_impl())) return rc;
