@@ -76,7 +76,7 @@ def run_mode(dev, st0, dt, steps, warmup, precision, guess, event_period=7):
         iii += 1
     dev.sync()
     dev.sweep_timing(event_period)
-    newton = sor = f32 = first = used = fall = 0
+    newton = sor = f32 = first = used = fall = skip = 0
     scales, r0s = [], []
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -89,6 +89,7 @@ def run_mode(dev, st0, dt, steps, warmup, precision, guess, event_period=7):
         first += e.sor_iter_first
         used += e.learned_used
         fall += e.fallbacks
+        skip += e.learned_skipped
         scales.append(e.guess_scale_first)
         r0s.append(e.guess_r0_first)
     dev.sync()
@@ -103,7 +104,7 @@ def run_mode(dev, st0, dt, steps, warmup, precision, guess, event_period=7):
         "sor_sweeps_per_step": sor / steps,
         "fp32_sweeps_per_step": f32 / steps,
         "first_solve_sweeps": first / steps,
-        "learned_used": used, "fallbacks": fall,
+        "learned_used": used, "fallbacks": fall, "learned_skipped": skip,
         "guess_scale_first": float(np.mean(scales)), "guess_r0_first": float(np.mean(r0s)),
         "sweep64_launches": n64, "sweep64_ms": ms64,
         "sweep32_launches": n32, "sweep32_ms": ms32,
@@ -147,7 +148,7 @@ def run_large(a):
     dev.bounds()
     e0 = dev.energy_totals()
     dev.sync()
-    sweeps = learned = fall = 0
+    sweeps = learned = fall = skipped = 0
     heat = 0.0
     t0 = time.perf_counter()
     for s_ in range(a.steps):
@@ -156,6 +157,7 @@ def run_large(a):
         sweeps += st.energy.fp32_sweeps
         learned += st.energy.learned_used
         fall += st.energy.fallbacks
+        skipped += st.energy.learned_skipped
         if (s_ + 1) % 20 == 0:
             print(f"# step {s_ + 1}/{a.steps}: {(time.perf_counter() - t0) * 1e3 / (s_ + 1):.1f} ms/step",
                   file=sys.stderr, flush=True)
@@ -177,7 +179,7 @@ def run_large(a):
                    "nr": nx, "ntheta": nz, "nphi": ny, "dt_s": dt},
         "cells": nx * ny * nz, "cell_updates_per_s": nx * ny * nz / (ms * 1e-3),
         "fp32_sweeps_per_step": sweeps / a.steps,
-        "learned_guess_used": learned, "fallbacks_to_cold": fall,
+        "learned_guess_used": learned, "fallbacks_to_cold": fall, "learned_skipped": skipped,
         "fallback_rate": round(fall / max(learned + fall, 1), 4),
         "energy": {"thermal_plus_radiation_start": tot0, "end": tot1,
                    "relative_change": (tot1 - tot0) / tot0,
@@ -343,6 +345,7 @@ def main():
                                              "32x64x32 FP64 solves, scaled to its least-residual multiple; "
                                              "cold start if its residual is larger",
                                    **summary(m), "learned_used": m["learned_used"], "fallbacks": m["fallbacks"],
+                                   "learned_skipped": m["learned_skipped"],
                                    "guess_scale": round(m["guess_scale_first"], 4),
                                    "guess_r0_over_cold": round(m["guess_r0_first"], 4)}
     if adj:
@@ -360,7 +363,8 @@ def main():
                 o[name] = {"omega": round(m["omega"], 6), "search_s": round(m["search_s"], 3), **summary(m),
                            "speedup_vs_fp64_same_search": round(f64["ms_per_step"] / m["ms_per_step"], 3)}
                 if k == "fp32_learned":
-                    o[name].update(learned_used=m["learned_used"], fallbacks=m["fallbacks"])
+                    o[name].update(learned_used=m["learned_used"], fallbacks=m["fallbacks"],
+                                   learned_skipped=m["learned_skipped"])
         out["omega_adjusted"] = o
     if not a.no_cpu:
         ms, s = cpu_baseline(p, st0, dt, a.cpu_threads)

@@ -221,7 +221,9 @@ typedef struct tt_energy_stats {
     int32_t fp32_sweeps;     /* FP32 iterations (precision 1)                 */
     int32_t fallbacks;       /* guesses rejected for the cold start           */
     int32_t learned_used;    /* solves that used the learned guess            */
-    int32_t pad0;
+    int32_t learned_skipped; /* first solves that did not run the network:
+                                after 3 rejections in a row it runs on every
+                                16th first solve only, until it is kept again */
     double max_dq, max_de;   /* global_deltaQ, global_deltaER of last iter    */
     double guess_scale_first;/* least-residual factor of the first solve's guess */
     double guess_r0_first;   /* its L1 residual / the cold start's (0: no guess) */

@@ -117,6 +117,41 @@ def test_adversarial_guess_falls_back_to_cold_start():
         np.testing.assert_array_equal(lrn[k], cold[k], err_msg=k)
 
 
+def test_rejected_network_is_skipped_then_retried():
+    """After 3 first solves in a row reject the learned guess, the network
+    runs only on every 16th first solve (each skipped one falls back to the
+    warm start); a rejected guess never changes the solve."""
+    p = P.make_params(16, 32, 16, epsmat=1e-8)
+    st = P.generate_state(p, seed=9)
+    w, b = init_params(3)
+    w = (w * 50.0).astype(np.float32)
+    b = (b + 5.0).astype(np.float32)
+    norm = np.array([0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 1, 1e-2], np.float32)
+    from ttcode_amd import ttgpu
+    d = ttgpu.Device(p)
+    d.load_state(st)
+    d.bounds()
+    d.pressg()
+    dt = d.schritt()
+    d.load_state(st)
+    d.cnn_set_weights(w, b)
+    d.cnn_set_norm(norm)
+    d.set_solver(guess=2)
+    ran, skipped = [], []
+    try:
+        for s in range(22):
+            es, _ = d.coupled_energy(dt, 2 + s)
+            skipped.append(es.learned_skipped)
+            ran.append(1 - es.learned_skipped)
+            assert es.learned_used == 0
+    finally:
+        d.close()
+        ttgpu.load_library().tt_cnn_release()
+    # runs on calls 0, 1, 2, then is skipped 15 times, runs on call 18, skipped after
+    print("network ran on calls", [i for i, r in enumerate(ran) if r])
+    assert ran[:3] == [1, 1, 1] and sum(ran[3:18]) == 0 and ran[18] == 1 and sum(ran[19:]) == 0
+
+
 def test_trained_guess_effect_on_sweeps():
     """Trains on 200 of the library's own solves at 32x64x32 and reports, on
     held-out configurations, the SOR iterations of the first Newton solve

@@ -702,6 +702,8 @@ int tt_cnn_set_weights(const float* w, int64_t nw, const float* b, int64_t nb) {
     TT_HIP(hipMemsetAsync(M.mb, 0, NB * 4, L.s));
     TT_HIP(hipMemsetAsync(M.vb, 0, NB * 4, L.s));
     M.adam_t = 0;
+    L.learned_rejects = 0;  // a new model gets its chances again
+    L.learned_idle = 0;
     if ((rc = repack())) return rc;
     TT_HIP(hipStreamSynchronize(L.s));
     M.have_w = true;

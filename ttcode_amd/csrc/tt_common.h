@@ -93,6 +93,9 @@ struct Lib {
     Ctrl* hctrl = nullptr;          // pinned host mirror
     tt_solver_opts opts{};
     tt_limits lim{};                // clamps applied by tt_step
+    // learned guess: consecutive rejections, and first solves since the
+    // network last ran
+    int learned_rejects = 0, learned_idle = 0;
     std::string err;
     // sweep timing
     bool timing = false;

@@ -594,13 +594,26 @@ int energy_impl(double delt, int iii, tt_energy_stats* st) {
             if (L.energy_iter > 1) {
                 if ((rc = sor_scale_guess(L.opts.warm_scale))) return rc;
                 L.have_prev_guess = 1;
-            } else if (L.opts.guess == 2 && cnn_ready()) {
-                if ((rc = cnn_predict_guess(delt))) return rc;
-                L.have_prev_guess = 2;
-            } else if (L.have_de1) {
-                TT_HIP(hipMemcpyAsync(f[TT_DELTA_ER], L.DE1, (size_t)g.n3 * sizeof(double),
-                                      hipMemcpyDeviceToDevice, L.s));
-                L.have_prev_guess = 1;
+            } else {
+                // the network's prediction, unless it was rejected on the
+                // last 3 first solves: then it runs again on every 16th only
+                // (inference costs about as much as 40 FP32 sweeps)
+                const bool learned = L.opts.guess == 2 && cnn_ready();
+                if (learned && (L.learned_rejects < 3 || L.learned_idle >= 15)) {
+                    if ((rc = cnn_predict_guess(delt))) return rc;
+                    L.have_prev_guess = 2;
+                    L.learned_idle = 0;
+                } else {
+                    if (learned) {
+                        es.learned_skipped++;
+                        L.learned_idle++;
+                    }
+                    if (L.have_de1) {
+                        TT_HIP(hipMemcpyAsync(f[TT_DELTA_ER], L.DE1, (size_t)g.n3 * sizeof(double),
+                                              hipMemcpyDeviceToDevice, L.s));
+                        L.have_prev_guess = 1;
+                    }
+                }
             }
         }
         tt_sor_stats ss{};
@@ -623,6 +636,7 @@ int energy_impl(double delt, int iii, tt_energy_stats* st) {
         }
         if (L.have_prev_guess && ss.guess_used == 0) es.fallbacks++;
         if (ss.guess_used == 2) es.learned_used++;
+        if (L.have_prev_guess == 2) L.learned_rejects = ss.guess_used == 2 ? 0 : L.learned_rejects + 1;
         int nb = 1024;
         UpdArgs ua{f[TT_DELTA_ER], f[TT_EVAL1], f[TT_EVAL2], f[TT_EVAL3], f[TT_EVAL4], f[TT_EVAL5], f[TT_EVAL6],
                    f[TT_EVAL_1_6], L.DFQ, f[TT_FQ], f[TT_RH], f[TT_XKAPP], f[TT_Q], f[TT_ER], L.ub, delt, conv_parm};

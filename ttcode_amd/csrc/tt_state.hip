@@ -308,6 +308,8 @@ int tt_init(const tt_params* p) {
     L.opts.inner_max = 4000;
     L.opts.adjust_omega = 0;
     L.lim = tt_limits{};
+    L.learned_rejects = 0;
+    L.learned_idle = 0;
     L.opts.inner_tol = 1e-5;
     L.opts.warm_scale = 0.9;
     TT_HIP(hipStreamSynchronize(L.s));

@@ -23,7 +23,7 @@ module ttgpu_c
 
   type, bind(C) :: tt_energy_stats
      integer(c_int32_t) :: energy_iter, sor_iter_total, sor_iter_last, sor_iter_first
-     integer(c_int32_t) :: fp32_sweeps, fallbacks, learned_used, pad0
+     integer(c_int32_t) :: fp32_sweeps, fallbacks, learned_used, learned_skipped
      real(c_double) :: max_dq, max_de, guess_scale_first, guess_r0_first
   end type tt_energy_stats
 

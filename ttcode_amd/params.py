@@ -90,7 +90,7 @@ class EnergyStats(ctypes.Structure):
     _fields_ = [("energy_iter", ctypes.c_int32), ("sor_iter_total", ctypes.c_int32),
                 ("sor_iter_last", ctypes.c_int32), ("sor_iter_first", ctypes.c_int32),
                 ("fp32_sweeps", ctypes.c_int32), ("fallbacks", ctypes.c_int32),
-                ("learned_used", ctypes.c_int32), ("pad0", ctypes.c_int32),
+                ("learned_used", ctypes.c_int32), ("learned_skipped", ctypes.c_int32),
                 ("max_dq", ctypes.c_double), ("max_de", ctypes.c_double),
                 ("guess_scale_first", ctypes.c_double), ("guess_r0_first", ctypes.c_double)]
 
