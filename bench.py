@@ -179,8 +179,12 @@ def run_large(a):
                    "nr": nx, "ntheta": nz, "nphi": ny, "dt_s": dt},
         "cells": nx * ny * nz, "cell_updates_per_s": nx * ny * nz / (ms * 1e-3),
         "fp32_sweeps_per_step": sweeps / a.steps,
-        "learned_guess_used": learned, "fallbacks_to_cold": fall, "learned_skipped": skipped,
-        "fallback_rate": round(fall / max(learned + fall, 1), 4),
+        # the network runs on first Newton solves only; each run's guess is
+        # kept or falls back to the plain (cold/warm) solve
+        "learned_network_ran": a.steps - skipped, "learned_guess_used": learned,
+        "learned_fallbacks": (a.steps - skipped) - learned, "learned_skipped": skipped,
+        "fallback_rate": round(((a.steps - skipped) - learned) / max(a.steps - skipped, 1), 4),
+        "guesses_rejected_any_kind": fall,
         "energy": {"thermal_plus_radiation_start": tot0, "end": tot1,
                    "relative_change": (tot1 - tot0) / tot0,
                    "irradiation_input_estimate": heat,

@@ -210,6 +210,19 @@ int sor_solve_impl(int fill_from_evals, tt_sor_stats* st, int fieldmode);
 int sor_adjust_impl(int fill_from_evals, double* omega_out, int32_t* iters);
 int limits_impl(int stage, double delt);
 int sor_pack_impl();
+// The SOR system lives in two forms: the module arrays SOR_A..G, RHS_ER
+// (wire layout) and the colour-split copy the sweeps read.  A kernel that
+// writes the colour-split copy directly reports it; the module arrays are
+// then rebuilt from it only when the host downloads one of them.
+struct PackedSys {
+    double* c64[2];
+    float* c32[2];
+    int hx;
+};
+PackedSys sor_packed();
+void sor_sys_packed_written(bool with32);
+void sor_sys_module_written();
+int sor_sys_before_download(int field);
 int diffusion_impl(double delt, double xnue, double pr, int* iters);
 int energy_impl(double delt, int iii, tt_energy_stats* st);
 int advect_impl(double delt);

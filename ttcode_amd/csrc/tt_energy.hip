@@ -173,42 +173,117 @@ __device__ double flimz_at(const Geo& g, const Met& m, const double* ER, const d
     return lp(rr);
 }
 
+// The three face families: FLIM (x loop, y and z with FLIM's stale
+// xmax_index, pprphy.f:410-522), DIFF (pprad.f:74-156), CALCULATEFLUX and
+// FLUX_BC (ppenergy.f:880-982).
 struct FaceArgs {
-    const double *ER, *RHQ, *XKR, *SIG;
-    double *FLIM, *DIFR, *FLUX;
+    const double *ER, *XKR, *SIG;
+    const double *RHQ[3];
+    double *FLIM[3], *DIFR[3], *FLUX[3];
     const int32_t* ub;
     int iflimtyp;
     double fbottom;
 };
 
-// x faces i = 1..nx+1: FLIM x loop, DIFF x, CALCULATEFLUX x, FLUX_BC x
-__global__ void k_face_x(Geo g, Met m, FaceArgs a) {
-    long long n = (long long)(g.nx + 1) * g.ny * g.nz;
+// x face i = 1..nx+1 of column (j, k)
+__device__ inline void face_x_eval(const Geo& g, const Met& m, const FaceArgs& a, int i, int j, int k, double& fl,
+                                   double& d, double& fx) {
+    const double* RHQ = a.RHQ[0];
+    int u = a.ub[g.idx2(j, k)];
+    if (a.iflimtyp == 1) fl = 1.0 / 3.0;
+    else fl = flimx_at(g, m, a.ER, RHQ, a.XKR, a.SIG, i <= u - 1 ? i : u - 1, j, k);
+    long long c = IX(i, j, k);
+    d = FD * fl /
+        (m(TT_M_DXB, i) * RHQ[c] *
+         (m(TT_M_DDX1, i) * (a.XKR[c] + a.SIG[c]) + m(TT_M_DDX0, i) * (a.XKR[IX(i - 1, j, k)] + a.SIG[IX(i - 1, j, k)])));
+    fx = -d * (a.ER[c] - a.ER[IX(i - 1, j, k)]);
+    if (i >= u) fx = FD * a.ER[IX(u - 1, j, k)] / 2.0;
+    if (i == 1) fx = a.fbottom;
+}
+
+// y face j = 1..ny+1 (FLIMY uses the x loop's last xmax_index, pprphy.f:464:
+// cells between it and upperbnd keep the limiter of an earlier call)
+__device__ inline void face_y_eval(const Geo& g, const Met& m, const FaceArgs& a, int i, int j, int k, double& fl,
+                                   double& d, double& fy) {
+    const double* RHQ = a.RHQ[1];
+    const double* FLIM = a.FLIM[1];
+    long long c = IX(i, j, k);
+    int u = a.ub[g.idx2(j, k)];
+    if (a.iflimtyp == 1) {
+        fl = 1.0 / 3.0;
+    } else {
+        int xmaxs = a.ub[g.idx2(g.ny, g.nz)] - 1;
+        if (i >= u) {
+            int src = u - 1;
+            fl = src <= xmaxs ? flimy_at(g, m, a.ER, RHQ, a.XKR, a.SIG, src, j, k) : FLIM[IX(src, j, k)];
+        } else if (i <= xmaxs) {
+            fl = flimy_at(g, m, a.ER, RHQ, a.XKR, a.SIG, i, j, k);
+        } else {
+            fl = FLIM[c];
+        }
+    }
+    d = FD * fl /
+        (m(TT_M_DYB, j) * m(TT_M_GEOXG, i) * m(TT_M_GEOZG, k) * RHQ[c] *
+         (m(TT_M_DDY1, j) * (a.XKR[c] + a.SIG[c]) + m(TT_M_DDY0, j) * (a.XKR[IX(i, j - 1, k)] + a.SIG[IX(i, j - 1, k)])));
+    fy = -d * (a.ER[c] - a.ER[IX(i, j - 1, k)]);
+    if (j <= g.ny && i >= u) fy = 0.0;
+}
+
+// z face k = 1..nz+1
+__device__ inline void face_z_eval(const Geo& g, const Met& m, const FaceArgs& a, int i, int j, int k, double& fl,
+                                   double& d, double& fz) {
+    const double* RHQ = a.RHQ[2];
+    const double* FLIM = a.FLIM[2];
+    long long c = IX(i, j, k);
+    int u = a.ub[g.idx2(j, k)];
+    if (a.iflimtyp == 1) {
+        fl = 1.0 / 3.0;
+    } else {
+        int xmaxs = a.ub[g.idx2(g.ny, g.nz)] - 1;
+        if (i >= u) {
+            int src = u - 1;
+            fl = src <= xmaxs ? flimz_at(g, m, a.ER, RHQ, a.XKR, a.SIG, src, j, k) : FLIM[IX(src, j, k)];
+        } else if (i <= xmaxs) {
+            fl = flimz_at(g, m, a.ER, RHQ, a.XKR, a.SIG, i, j, k);
+        } else {
+            fl = FLIM[c];
+        }
+    }
+    d = FD * fl /
+        (m(TT_M_DZB, k) * m(TT_M_GEOXH, i) * RHQ[c] *
+         (m(TT_M_DDZ1, k) * (a.XKR[c] + a.SIG[c]) + m(TT_M_DDZ0, k) * (a.XKR[IX(i, j, k - 1)] + a.SIG[IX(i, j, k - 1)])));
+    fz = -d * (a.ER[c] - a.ER[IX(i, j, k - 1)]);
+    if (k <= g.nz && i >= u) fz = 0.0;
+}
+
+// All three families in one pass over (i, j, k) in [1, nx+1] x [1, ny+1] x
+// [1, nz+1]: each thread forms the faces on the low side of its cell that
+// exist there, so the E, kappa and sigma neighbourhood is read once for
+// three faces.  Every value is formed before any is stored; a stale limiter
+// read (cells between xmax_index and upperbnd) is of a cell no thread
+// changes in this call.
+__global__ __launch_bounds__(256) void k_faces(Geo g, Met m, FaceArgs a) {
+    long long n = (long long)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     int i, j, k;
-    sub_ijk(t, g.nx + 1, g.ny, 1, 1, 1, i, j, k);
-    int u = a.ub[g.idx2(j, k)];
-    double fl;
-    if (a.iflimtyp == 1) fl = 1.0 / 3.0;
-    else fl = flimx_at(g, m, a.ER, a.RHQ, a.XKR, a.SIG, i <= u - 1 ? i : u - 1, j, k);
-    long long c = IX(i, j, k);
-    a.FLIM[c] = fl;
-    double d = FD * fl /
-               (m(TT_M_DXB, i) * a.RHQ[c] *
-                (m(TT_M_DDX1, i) * (a.XKR[c] + a.SIG[c]) + m(TT_M_DDX0, i) * (a.XKR[IX(i - 1, j, k)] + a.SIG[IX(i - 1, j, k)])));
-    a.DIFR[c] = d;
-    double fx = -d * (a.ER[c] - a.ER[IX(i - 1, j, k)]);
-    if (i >= u) fx = FD * a.ER[IX(u - 1, j, k)] / 2.0;
-    if (i == 1) fx = a.fbottom;
-    a.FLUX[c] = fx;
+    sub_ijk(t, g.nx + 1, g.ny + 1, 1, 1, 1, i, j, k);
+    const bool hx = j <= g.ny && k <= g.nz, hy = i <= g.nx && k <= g.nz, hz = i <= g.nx && j <= g.ny;
+    double flx = 0.0, dx = 0.0, fx = 0.0, fly = 0.0, dy = 0.0, fy = 0.0, flz = 0.0, dz = 0.0, fz = 0.0;
+    if (hx) face_x_eval(g, m, a, i, j, k, flx, dx, fx);
+    if (hy) face_y_eval(g, m, a, i, j, k, fly, dy, fy);
+    if (hz) face_z_eval(g, m, a, i, j, k, flz, dz, fz);
+    const long long c = IX(i, j, k);
+    if (hx) { a.FLIM[0][c] = flx; a.DIFR[0][c] = dx; a.FLUX[0][c] = fx; }
+    if (hy) { a.FLIM[1][c] = fly; a.DIFR[1][c] = dy; a.FLUX[1][c] = fy; }
+    if (hz) { a.FLIM[2][c] = flz; a.DIFR[2][c] = dz; a.FLUX[2][c] = fz; }
 }
 
 // The CALCULATEFLUX + FLUX_BC pair before the Newton loop (ppenergy.f:78-79)
 // feeds only the top-boundary blocks, which read flux_x at i >= upperbnd and,
 // through the i = nx+2 alias, at i = 1: faces FLUX_BC fixes to c E(upperbnd-1)/2
 // and Fbottom (ppenergy.f:947,966-967,976).  Those values are written here,
-// per interior column, with the expressions k_face_x uses.  Every other
+// per interior column, with the expressions face_x_eval uses.  Every other
 // flux, limiter and diffusion coefficient that call forms is overwritten by
 // the Newton loop's first call before it is read; the ghost columns the alias
 // reaches are rewritten by the theta blocks and the phi wrap.
@@ -223,76 +298,12 @@ __global__ void k_flux_x_top(Geo g, const double* __restrict__ ER, const int32_t
     FX[IX(1, j, k)] = fbottom;
 }
 
-// y faces j = 1..ny+1 (FLIMY uses the x loop's last xmax_index, pprphy.f:464)
-__global__ void k_face_y(Geo g, Met m, FaceArgs a) {
-    long long n = (long long)g.nx * (g.ny + 1) * g.nz;
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    int i, j, k;
-    sub_ijk(t, g.nx, g.ny + 1, 1, 1, 1, i, j, k);
-    long long c = IX(i, j, k);
-    int u = a.ub[g.idx2(j, k)];
-    double fl;
-    if (a.iflimtyp == 1) {
-        fl = 1.0 / 3.0;
-    } else {
-        int xmaxs = a.ub[g.idx2(g.ny, g.nz)] - 1;
-        if (i >= u) {
-            int src = u - 1;
-            fl = src <= xmaxs ? flimy_at(g, m, a.ER, a.RHQ, a.XKR, a.SIG, src, j, k) : a.FLIM[IX(src, j, k)];
-        } else if (i <= xmaxs) {
-            fl = flimy_at(g, m, a.ER, a.RHQ, a.XKR, a.SIG, i, j, k);
-        } else {
-            fl = a.FLIM[c];
-        }
-    }
-    a.FLIM[c] = fl;
-    double d = FD * fl /
-               (m(TT_M_DYB, j) * m(TT_M_GEOXG, i) * m(TT_M_GEOZG, k) * a.RHQ[c] *
-                (m(TT_M_DDY1, j) * (a.XKR[c] + a.SIG[c]) + m(TT_M_DDY0, j) * (a.XKR[IX(i, j - 1, k)] + a.SIG[IX(i, j - 1, k)])));
-    a.DIFR[c] = d;
-    double fy = -d * (a.ER[c] - a.ER[IX(i, j - 1, k)]);
-    if (j <= g.ny && i >= u) fy = 0.0;
-    a.FLUX[c] = fy;
-}
-
-// z faces k = 1..nz+1
-__global__ void k_face_z(Geo g, Met m, FaceArgs a) {
-    long long n = (long long)g.nx * g.ny * (g.nz + 1);
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    int i, j, k;
-    sub_ijk(t, g.nx, g.ny, 1, 1, 1, i, j, k);
-    long long c = IX(i, j, k);
-    int u = a.ub[g.idx2(j, k)];
-    double fl;
-    if (a.iflimtyp == 1) {
-        fl = 1.0 / 3.0;
-    } else {
-        int xmaxs = a.ub[g.idx2(g.ny, g.nz)] - 1;
-        if (i >= u) {
-            int src = u - 1;
-            fl = src <= xmaxs ? flimz_at(g, m, a.ER, a.RHQ, a.XKR, a.SIG, src, j, k) : a.FLIM[IX(src, j, k)];
-        } else if (i <= xmaxs) {
-            fl = flimz_at(g, m, a.ER, a.RHQ, a.XKR, a.SIG, i, j, k);
-        } else {
-            fl = a.FLIM[c];
-        }
-    }
-    a.FLIM[c] = fl;
-    double d = FD * fl /
-               (m(TT_M_DZB, k) * m(TT_M_GEOXH, i) * a.RHQ[c] *
-                (m(TT_M_DDZ1, k) * (a.XKR[c] + a.SIG[c]) + m(TT_M_DDZ0, k) * (a.XKR[IX(i, j, k - 1)] + a.SIG[IX(i, j, k - 1)])));
-    a.DIFR[c] = d;
-    double fz = -d * (a.ER[c] - a.ER[IX(i, j, k - 1)]);
-    if (k <= g.nz && i >= u) fz = 0.0;
-    a.FLUX[c] = fz;
-}
-
 struct CellArgs {
     const double *FX, *FY, *FZ, *DX, *DY, *DZ, *Q, *QOLD, *ER, *EROLD, *RH, *CV, *XKP, *XKA, *DKP, *DKA, *SIN;
     double *E1, *E2, *E3, *E4, *E5, *E6, *E7, *E8, *E16, *ETA, *FQ, *FR, *RHS, *DFQ;
-    double *SA, *SB, *SC, *SD, *SE, *SF, *SG;
+    double *P0, *P1;  // the packed FP64 system, colours 0 and 1
+    float *Q0, *Q1;   // its FP32 copy (null: not written)
+    int hx;
     const int32_t* ub;
     int f_incident;
     double delt;
@@ -345,6 +356,7 @@ __global__ void k_newton_cell(Geo g, Met m, CellArgs a) {
     double e8 = (eta * (1.0 - theta) - 1.0) * delt;
     a.E1[c] = e1; a.E2[c] = e2; a.E3[c] = e3; a.E4[c] = e4; a.E5[c] = e5; a.E6[c] = e6;
     a.E7[c] = e7; a.E8[c] = e8; a.E16[c] = e16; a.ETA[c] = eta; a.FQ[c] = fq; a.FR[c] = fr; a.DFQ[c] = dfq;
+    (void)a.RHS;
     // FILL_SOR_MATRIX + B_VECTOR_BOUNDARIES
     double sa = e7 + e8 * e16, sb = -e8 * e2, sc = -e8 * e1, sd = -e8 * e4, se = -e8 * e3, sf = -e8 * e6,
            sg = -e8 * e5;
@@ -354,8 +366,19 @@ __global__ void k_newton_cell(Geo g, Met m, CellArgs a) {
     if (i == u - 1 || i == g.nx - 1) sc = 0.0;
     if (k == 2) sf = 0.0;
     if (k == g.nz - 1) sg = 0.0;
-    a.SA[c] = sa; a.SB[c] = sb; a.SC[c] = sc; a.SD[c] = sd; a.SE[c] = se; a.SF[c] = sf; a.SG[c] = sg;
-    a.RHS[c] = rhs;
+    // straight into the colour-split row-blocked system the sweeps read:
+    // colour (i+j+k)&1, row (j-1) + ny (k-1), slot (i-1)/2, vectors A..G, RHS
+    const int col = (i + j + k) & 1;
+    const long long o = (long long)((j - 1) + g.ny * (k - 1)) * 8 * a.hx + (i - 1) / 2;
+    const double v[8] = {sa, sb, sc, sd, se, sf, sg, rhs};
+    double* P = col ? a.P1 : a.P0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) P[o + (long long)q * a.hx] = v[q];
+    if (a.Q0) {
+        float* Q = col ? a.Q1 : a.Q0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) Q[o + (long long)q * a.hx] = (float)v[q];
+    }
 }
 
 struct UpdArgs {
@@ -511,18 +534,11 @@ static int flux_impl() {
     if (L.p.iflimtyp != 1 && L.p.iflimtyp != 2) return fail(1, "FLIM: IFLIMTYP must be 1 or 2");
     Geo& g = L.g;
     double** f = L.f;
-    FaceArgs a{f[TT_ER], f[TT_RHQX], f[TT_XKAPR], f[TT_SIG], f[TT_FLIMX], f[TT_DIFRX], f[TT_FLUX_X], L.ub,
-               L.p.iflimtyp, L.p.fbottom};
-    long long nxf = (long long)(g.nx + 1) * g.ny * g.nz;
-    k_face_x<<<nblk(nxf, 256), 256, 0, L.s>>>(g, met(), a);
-    TT_LAUNCH();
-    a.RHQ = f[TT_RHQY]; a.FLIM = f[TT_FLIMY]; a.DIFR = f[TT_DIFRY]; a.FLUX = f[TT_FLUX_Y];
-    long long nyf = (long long)g.nx * (g.ny + 1) * g.nz;
-    k_face_y<<<nblk(nyf, 256), 256, 0, L.s>>>(g, met(), a);
-    TT_LAUNCH();
-    a.RHQ = f[TT_RHQZ]; a.FLIM = f[TT_FLIMZ]; a.DIFR = f[TT_DIFRZ]; a.FLUX = f[TT_FLUX_Z];
-    long long nzf = (long long)g.nx * g.ny * (g.nz + 1);
-    k_face_z<<<nblk(nzf, 256), 256, 0, L.s>>>(g, met(), a);
+    FaceArgs a{f[TT_ER], f[TT_XKAPR], f[TT_SIG], {f[TT_RHQX], f[TT_RHQY], f[TT_RHQZ]},
+               {f[TT_FLIMX], f[TT_FLIMY], f[TT_FLIMZ]}, {f[TT_DIFRX], f[TT_DIFRY], f[TT_DIFRZ]},
+               {f[TT_FLUX_X], f[TT_FLUX_Y], f[TT_FLUX_Z]}, L.ub, L.p.iflimtyp, L.p.fbottom};
+    long long n = (long long)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
+    k_faces<<<nblk(n, 256), 256, 0, L.s>>>(g, met(), a);
     TT_LAUNCH();
     return 0;
 }
@@ -573,11 +589,16 @@ int energy_impl(double delt, int iii, tt_energy_stats* st) {
                     f[TT_ER], f[TT_ER_OLD], f[TT_RH], f[TT_CV], f[TT_XKAPP], f[TT_XKAPA], f[TT_DXKAPP_DT],
                     f[TT_DXKAPA_DT], f[TT_STELLARINPUT], f[TT_EVAL1], f[TT_EVAL2], f[TT_EVAL3], f[TT_EVAL4],
                     f[TT_EVAL5], f[TT_EVAL6], f[TT_EVAL7], f[TT_EVAL8], f[TT_EVAL_1_6], f[TT_JACOB_ETA], f[TT_FQ],
-                    f[TT_FR], f[TT_RHS_ER], L.DFQ, f[TT_SOR_A], f[TT_SOR_B], f[TT_SOR_C], f[TT_SOR_D],
-                    f[TT_SOR_E], f[TT_SOR_F], f[TT_SOR_G], L.ub, L.p.f_incident, delt};
+                    f[TT_FR], f[TT_RHS_ER], L.DFQ, nullptr, nullptr, nullptr, nullptr, 0, L.ub, L.p.f_incident,
+                    delt};
+        const PackedSys ps = sor_packed();
+        const bool with32 = L.opts.precision == 1;
+        ca.P0 = ps.c64[0]; ca.P1 = ps.c64[1]; ca.hx = ps.hx;
+        if (with32) { ca.Q0 = ps.c32[0]; ca.Q1 = ps.c32[1]; }
         long long nc = (long long)g.nx * g.ny * g.nz;
         k_newton_cell<<<nblk(nc, 256), 256, 0, L.s>>>(g, met(), ca);
         TT_LAUNCH();
+        sor_sys_packed_written(with32);
         // ADJUST_SOR_PARAM at the first Newton iteration of step 1 and of
         // every 10000th step (ppenergy.f:372-374), when switched on
         if (L.opts.adjust_omega && L.energy_iter == 1 && (iii % 10000 == 0 || iii == 1)) {

@@ -15,10 +15,12 @@ namespace tt {
 
 int planetshift_impl(double* a, int f);
 
-__global__ void k_shift5(Geo g, double* T, double* RH, double* V, double* G, double* H) {
+// PLANETSHIFT of T, RH, V, G, H (phi wrap of the two ghost planes), one
+// thread per (i, k) line; b is the block's index within this work
+__device__ inline void shift5_body(const Geo& g, double* T, double* RH, double* V, double* G, double* H, int b) {
     int ni = g.nx + 4;
     long long n = (long long)ni * g.nz;
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long t = (long long)b * 256 + threadIdx.x;
     if (t >= n) return;
     int i = (int)(t % ni) - 1, k = (int)(t / ni) + 1;
     const int ny = g.ny;
@@ -43,14 +45,18 @@ __global__ void k_shift5(Geo g, double* T, double* RH, double* V, double* G, dou
     G[g.idx(i, ny + 2, k)] = G[g.idx(i, 2, k)];
 }
 
+__global__ __launch_bounds__(256) void k_shift5(Geo g, double* T, double* RH, double* V, double* G, double* H) {
+    shift5_body(g, T, RH, V, G, H, blockIdx.x);
+}
+
 // upperbnd (ppbou.f:124-138) for every column incl. ghosts, which PASSUPPERBND
 // (ppbou.f:522-682) fills from the phi-wrapped column and the nearest theta
 // row when the theta/corner partners are MPI_PROC_NULL.  One wavefront per
 // column: the lanes read 64 consecutive r cells (one coalesced load) and a
 // ballot finds the first with rho below the threshold.
-__global__ __launch_bounds__(256) void k_upperbnd(Geo g, const double* __restrict__ RH, int32_t* ub) {
+__device__ inline void upperbnd_body(const Geo& g, const double* __restrict__ RH, int32_t* ub, int b) {
     int nj = g.ny + 4, nk = g.nz + 4;
-    int col = blockIdx.x * 4 + (threadIdx.x >> 6);
+    int col = b * 4 + (threadIdx.x >> 6);
     int lane = threadIdx.x & 63;
     if (col >= nj * nk) return;
     int j = col % nj - 1, k = col / nj - 1;
@@ -68,6 +74,15 @@ __global__ __launch_bounds__(256) void k_upperbnd(Geo g, const double* __restric
         }
     }
     if (lane == 0) ub[g.idx2(j, k)] = u;
+}
+
+// BOUNDS' first PLANETSHIFT and upperbnd in one launch: upperbnd reads only
+// the interior columns it wraps to, which the shift does not write, and the
+// shift does not read upperbnd.  Blocks [0, nshift) shift, the rest search.
+__global__ __launch_bounds__(256) void k_shift5_upperbnd(Geo g, double* T, double* RH, double* V, double* G,
+                                                         double* H, int32_t* ub, int nshift) {
+    if ((int)blockIdx.x < nshift) shift5_body(g, T, RH, V, G, H, blockIdx.x);
+    else upperbnd_body(g, RH, ub, blockIdx.x - nshift);
 }
 
 // BOUNDXMIN(12) (ppbou.f:1118-1139) then BOUNDXMAX(2) (ppbou.f:1269-1395)
@@ -189,9 +204,9 @@ int bounds_impl() {
         return fail(1, "BOUNDXMAX(2): IBDRAD must be 3, 4, 7 or 9");
     long long ns = (long long)(g.nx + 4) * g.nz;
     int ncol = (g.ny + 4) * (g.nz + 4), nline = (g.nx + 4) * (g.ny + 4);
-    k_shift5<<<nblk(ns, 256), 256, 0, L.s>>>(g, f[TT_T], f[TT_RH], f[TT_V], f[TT_G], f[TT_H]);
-    TT_LAUNCH();
-    k_upperbnd<<<nblk(ncol, 4), 256, 0, L.s>>>(g, f[TT_RH], L.ub);
+    const unsigned nshift = nblk(ns, 256);
+    k_shift5_upperbnd<<<nshift + nblk(ncol, 4), 256, 0, L.s>>>(g, f[TT_T], f[TT_RH], f[TT_V], f[TT_G], f[TT_H],
+                                                               L.ub, (int)nshift);
     TT_LAUNCH();
     k_xbc<<<nblk(ncol, 128), 128, 0, L.s>>>(g, met(), L.ub, f[TT_T], f[TT_RH], f[TT_V], f[TT_G], f[TT_H],
                                             f[TT_ER], f[TT_STELLARINPUT], f[TT_XKAPP], L.p.d_t, L.p.ibdrad,
@@ -362,6 +377,7 @@ int diffusion_impl(double delt, double xnue, double pr, int* iters) {
                                                 f[TT_SOR_C], f[TT_SOR_D], f[TT_SOR_E], f[TT_SOR_F], f[TT_SOR_G],
                                                 f[TT_RHS_ER]);
     TT_LAUNCH();
+    sor_sys_module_written();
     tt_solver_opts keep = L.opts;
     L.opts.precision = 0;
     tt_sor_stats st{};

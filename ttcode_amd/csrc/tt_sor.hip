@@ -824,13 +824,89 @@ static int run_sweeps(T* coef0, T* coef1, T* xa, T* xb, double omega, int* iters
     return 0;
 }
 
+// Which form of the system is current: 0 the module arrays (the packed copy
+// must be rebuilt), 1 the packed copy (the module arrays are stale), 2 both.
+// g_c32_ok: the FP32 copy's A..G match the FP64 one (its RHS slot is
+// rewritten with the residual before every inner solve).
+static int g_sys = 0;
+static bool g_c32_ok = false;
+
+PackedSys sor_packed() { return PackedSys{{c64[0], c64[1]}, {c32[0], c32[1]}, g_cs.hx}; }
+void sor_sys_packed_written(bool with32) {
+    g_sys = 1;
+    g_c32_ok = with32;
+}
+void sor_sys_module_written() {
+    g_sys = 0;
+    g_c32_ok = false;
+}
+
+// sum |RHS| over the packed copy, pair by pair in k_pack's order (same
+// blocks, same per-thread sums), so the norm is the same to the bit
+__global__ __launch_bounds__(256) void k_rhsnorm(CS cs, const double* __restrict__ C0, const double* __restrict__ C1,
+                                                 double* part) {
+    __shared__ double sh[4];
+    const int hx = cs.hx;
+    const long long tot = cs.ncell;
+    double acc = 0.0;
+    for (long long t = (long long)blockIdx.x * 256 + threadIdx.x; t < tot; t += (long long)gridDim.x * 256) {
+        const int e = (int)t;
+        const int r = e / hx;
+        const int p = e - r * hx;
+        const int kk = r / cs.ny;
+        const int j = r - kk * cs.ny + 1, k = kk + 1;
+        const int codd = (j + k + 1) & 1;
+        const long long o = ((long long)r * 8 + 7) * hx + p;
+        const double vo = (codd ? C1 : C0)[o], ve = (codd ? C0 : C1)[o];
+        acc += fabs(vo) + fabs(ve);
+    }
+    double bs = block_sum256(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = bs;
+}
+
+// the packed copy back into the module arrays (interior cells)
+__global__ void k_unpack(Geo g, CS cs, const double* __restrict__ C0, const double* __restrict__ C1, double* SA,
+                         double* SB, double* SC, double* SD, double* SE, double* SF, double* SG, double* RHS) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)g.nx * g.ny * g.nz) return;
+    int i, j, k;
+    sub_ijk(t, g.nx, g.ny, 1, 1, 1, i, j, k);
+    const int r = (j - 1) + cs.ny * (k - 1);
+    const double* C = ((i + j + k) & 1) ? C1 : C0;
+    const long long o = (long long)r * 8 * cs.hx + (i - 1) / 2;
+    const long long n = g.idx(i, j, k);
+    double* dst[8] = {SA, SB, SC, SD, SE, SF, SG, RHS};
+#pragma unroll
+    for (int q = 0; q < 8; ++q) dst[q][n] = C[o + (long long)q * cs.hx];
+}
+
+int sor_sys_before_download(int field) {
+    const bool sys = (field >= TT_SOR_A && field <= TT_SOR_G) || field == TT_RHS_ER;
+    if (!sys || g_sys != 1) return 0;
+    double** f = L.f;
+    long long n = (long long)L.g.nx * L.g.ny * L.g.nz;
+    k_unpack<<<nblk(n, 256), 256, 0, L.s>>>(L.g, g_cs, c64[0], c64[1], f[TT_SOR_A], f[TT_SOR_B], f[TT_SOR_C],
+                                            f[TT_SOR_D], f[TT_SOR_E], f[TT_SOR_F], f[TT_SOR_G], f[TT_RHS_ER]);
+    TT_LAUNCH();
+    g_sys = 2;
+    return 0;
+}
+
+// The packed system and sum |RHS| for a solve: packed from the module
+// arrays when those are newer, else the norm of the packed copy.
 int sor_pack_impl() {
     double** f = L.f;
     int nb = (int)((g_cs.ncell + 255) / 256);
     if (nb > L.npartial) nb = L.npartial;
     double* part = L.partial + L.npartial;  // second half: the sweeps use the first
-    k_pack<<<nb, 256, 0, L.s>>>(L.g, g_cs, f[TT_SOR_A], f[TT_SOR_B], f[TT_SOR_C], f[TT_SOR_D], f[TT_SOR_E],
-                                f[TT_SOR_F], f[TT_SOR_G], f[TT_RHS_ER], c64[0], c64[1], part);
+    if (g_sys == 0) {
+        k_pack<<<nb, 256, 0, L.s>>>(L.g, g_cs, f[TT_SOR_A], f[TT_SOR_B], f[TT_SOR_C], f[TT_SOR_D], f[TT_SOR_E],
+                                    f[TT_SOR_F], f[TT_SOR_G], f[TT_RHS_ER], c64[0], c64[1], part);
+        g_sys = 2;
+        g_c32_ok = false;
+    } else {
+        k_rhsnorm<<<nb, 256, 0, L.s>>>(g_cs, c64[0], c64[1], part);
+    }
     TT_LAUNCH();
     return fold_partials(part, nb, 1, 0, &L.ctrl->rhsnorm);
 }
@@ -868,6 +944,7 @@ int sor_solve_impl(int fill_from_evals, tt_sor_stats* st, int mode) {
                                                      f[TT_SOR_D], f[TT_SOR_E], f[TT_SOR_F], f[TT_SOR_G],
                                                      f[TT_RHS_ER]);
         TT_LAUNCH();
+        sor_sys_module_written();
     }
     int rc = set_ctrl_solve(mode == 1 ? 500 : L.p.itmax, mode == 1 ? 5 : 1, L.p.epsmat);
     if (rc) return rc;
@@ -974,9 +1051,12 @@ int sor_solve_impl(int fill_from_evals, tt_sor_stats* st, int mode) {
         s.rnorm = rnorm;
     } else {
         // FP32 SOR inside FP64 defect correction
-        for (int c = 0; c < 2; ++c) {
-            k_to_f32<<<nblk(8 * g_cs.ncell, 256), 256, 0, L.s>>>(c64[c], c32[c], 8 * g_cs.ncell);
-            TT_LAUNCH();
+        if (!g_c32_ok) {
+            for (int c = 0; c < 2; ++c) {
+                k_to_f32<<<nblk(8 * g_cs.ncell, 256), 256, 0, L.s>>>(c64[c], c32[c], 8 * g_cs.ncell);
+                TT_LAUNCH();
+            }
+            g_c32_ok = true;
         }
         double rn;
         rc = resid64(&rn, true);  // reads the control block: rhsnorm is current

@@ -158,6 +158,12 @@ static int xfer(int f, void* host, bool to_host) {
     }
     double* a = L.f[f];
     if (!a) return fail(1, "field not held on the device");
+    if (to_host) {
+        int rc = sor_sys_before_download(f);
+        if (rc) return rc;
+    } else if ((f >= TT_SOR_A && f <= TT_SOR_G) || f == TT_RHS_ER) {
+        sor_sys_module_written();
+    }
     int ni = hi[0] - lo[0] + 1, nj = hi[1] - lo[1] + 1, nk = hi[2] - lo[2] + 1;
     long long n = (long long)ni * nj * nk;
     if (to_host) {
