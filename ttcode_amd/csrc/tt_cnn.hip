@@ -166,62 +166,84 @@ __global__ void k_pack_frags(const float* __restrict__ w, bf16_t* frag, float* w
 __device__ inline bf16x8 ld8(const bf16_t* p) { return __builtin_bit_cast(bf16x8, *(const uint4*)p); }
 
 // Hidden-width convolution on the matrix cores.  CIN 8 (layer 0 input) or 16.
-// A wavefront computes TPW consecutive tiles of 16 cells: each weight
-// fragment is loaded once for TPW independent accumulation chains.
+// A block computes a tile of 16 (r) x 4 (phi) x 4 (theta) cells: its input
+// with a one-cell halo (18 x 6 x 6 cells, zero outside r and theta, phi
+// wrapped) is staged in LDS once, and every A fragment (one tap, 8 channels
+// of one cell per lane) is a 16-byte LDS read instead of a gather from
+// global memory.  Each wavefront owns 4 rows of 16 cells (4 accumulation
+// chains); each weight fragment is loaded once for the 4.  K runs over the
+// taps in the same order as the weight fragments.
 // mode 0: out = bf16(gelu(acc + bias)), pre = acc + bias (when pre != null)
 // mode 1: out = bf16(acc * gelu'(gpre))  (input gradient of the layer, times
 //         the previous layer's gelu derivative)
 constexpr int TPW = 4;
+constexpr int CT_I = 16, CT_J = 4, CT_K = 4;
+constexpr int CH_I = CT_I + 2, CH_J = CT_J + 2, CH_K = CT_K + 2, CH_N = CH_I * CH_J * CH_K;
+__host__ __device__ inline unsigned conv_blocks(const Grid3& g) {
+    return (unsigned)((g.nx / CT_I) * ((g.ny + CT_J - 1) / CT_J) * ((g.nz + CT_K - 1) / CT_K));
+}
+
 template <int CIN>
 __global__ __launch_bounds__(256) void k_conv_mfma(const bf16_t* __restrict__ x, const bf16_t* __restrict__ frag,
-                                                   const float* __restrict__ bias, Grid3 g, int ntiles, int mode,
+                                                   const float* __restrict__ bias, Grid3 g, int mode,
                                                    bf16_t* __restrict__ out, float* __restrict__ pre,
                                                    const float* __restrict__ gpre) {
-    const int lane = threadIdx.x & 63;
-    const int tile0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * TPW;
-    if (tile0 >= ntiles) return;
-    const int tpr = g.nx >> 4;  // tiles per r row
+    constexpr int V = CIN / 8;  // 16-byte words per cell
+    constexpr int NQ = CIN == 8 ? FQ0 : FQH;
+    __shared__ uint4 tile[CH_N * V];
+    const int nti = g.nx / CT_I, ntj = (g.ny + CT_J - 1) / CT_J;
+    int b = blockIdx.x;
+    const int bi = b % nti;
+    b /= nti;
+    const int bj = b % ntj, bk = b / ntj;
+    const int i0 = bi * CT_I, j0 = bj * CT_J, k0 = bk * CT_K;
+    const uint4 zero = {0u, 0u, 0u, 0u};
+    for (int e = threadIdx.x; e < CH_N * V; e += 256) {
+        const int cell = e / V, v = e - cell * V;
+        const int ii = cell % CH_I, rr = cell / CH_I, jj = rr % CH_J, kk = rr / CH_J;
+        const int i = i0 - 1 + ii, k = k0 - 1 + kk;
+        int j = (j0 - 1 + jj) % g.ny;
+        if (j < 0) j += g.ny;
+        uint4 val = zero;
+        if (i >= 0 && i < g.nx && k >= 0 && k < g.nz) val = *(const uint4*)(x + (long long)g.cell(i, j, k) * CIN + 8 * v);
+        tile[e] = val;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int m = lane & 15, kg = lane >> 4;
-    int ti[TPW], tj[TPW], tk[TPW];
+    int jr[TPW], kr[TPW];
     bool tv[TPW];
 #pragma unroll
     for (int u = 0; u < TPW; ++u) {
-        int tile = tile0 + u;
-        tv[u] = tile < ntiles;
-        if (!tv[u]) tile = tile0;
-        const int row = tile / tpr;
-        ti[u] = (tile - row * tpr) * 16;
-        tk[u] = row / g.ny;
-        tj[u] = row - tk[u] * g.ny;
+        const int row = w * TPW + u;  // 16 rows: (phi, theta) = (row % 4, row / 4)
+        jr[u] = row % CT_J;
+        kr[u] = row / CT_J;
+        tv[u] = j0 + jr[u] < g.ny && k0 + kr[u] < g.nz;
     }
-    constexpr int NQ = CIN == 8 ? FQ0 : FQH;
     f32x4 acc[TPW];
 #pragma unroll
     for (int u = 0; u < TPW; ++u) acc[u] = f32x4{0.f, 0.f, 0.f, 0.f};
-    const uint4 zero = {0u, 0u, 0u, 0u};
     for (int q = 0; q < NQ; ++q) {
         const int tap = CIN == 8 ? 4 * q + kg : 2 * q + (kg >> 1);
-        const int c0 = CIN == 8 ? 0 : 8 * (kg & 1);
-        const bf16x8 b = ld8(frag + ((long long)q * 64 + lane) * 8);
+        const int c0 = CIN == 8 ? 0 : (kg & 1);  // 16-byte word within the cell
+        const bf16x8 bw = ld8(frag + ((long long)q * 64 + lane) * 8);
+        const int dk = tap / 9, dj = (tap / 3) % 3, di = tap % 3;  // offsets + 1
         uint4 av[TPW];
 #pragma unroll
         for (int u = 0; u < TPW; ++u) {
-            av[u] = zero;
-            if (tap < 27) {
-                int nb = g.nbr(ti[u] + m, tj[u], tk[u], tap);
-                if (nb >= 0) av[u] = *(const uint4*)(x + (long long)nb * CIN + c0);
-            }
+            const int lc = ((kr[u] + dk) * CH_J + (jr[u] + dj)) * CH_I + (m + di);
+            av[u] = tap < 27 ? tile[lc * V + c0] : zero;
         }
 #pragma unroll
         for (int u = 0; u < TPW; ++u)
-            acc[u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, av[u]), b, acc[u], 0, 0, 0);
+            acc[u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, av[u]), bw, acc[u], 0, 0, 0);
     }
     const int n = lane & 15;
     const float bn = mode == 0 ? bias[n] : 0.f;
 #pragma unroll
     for (int u = 0; u < TPW; ++u) {
         if (!tv[u]) continue;
-        const int base = g.cell(ti[u], tj[u], tk[u]);
+        const int base = g.cell(i0, j0 + jr[u], k0 + kr[u]);
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const long long c = (long long)(base + 4 * kg + r) * 16 + n;
@@ -561,14 +583,13 @@ static int repack() {
 // the pre-activations
 static int forward(Grid3 g, bool train) {
     long long n = (long long)g.nx * g.ny * g.nz;
-    int ntiles = (int)(n / 16);
-    unsigned nb = (unsigned)((ntiles + 4 * TPW - 1) / (4 * TPW));
-    k_conv_mfma<8><<<nb, 256, 0, L.s>>>(M.act[0], M.frag + FRAG_L0, M.b + boff(0), g, ntiles, 0, M.act[1],
+    const unsigned nb = conv_blocks(g);
+    k_conv_mfma<8><<<nb, 256, 0, L.s>>>(M.act[0], M.frag + FRAG_L0, M.b + boff(0), g, 0, M.act[1],
                                         train ? M.pre[0] : nullptr, nullptr);
     TT_LAUNCH();
     for (int l = 1; l < NL - 1; ++l) {
-        k_conv_mfma<16><<<nb, 256, 0, L.s>>>(M.act[l], M.frag + frag_fwd(l), M.b + boff(l), g, ntiles, 0,
-                                             M.act[l + 1], train ? M.pre[l] : nullptr, nullptr);
+        k_conv_mfma<16><<<nb, 256, 0, L.s>>>(M.act[l], M.frag + frag_fwd(l), M.b + boff(l), g, 0, M.act[l + 1],
+                                             train ? M.pre[l] : nullptr, nullptr);
         TT_LAUNCH();
     }
     k_conv_out<<<nblk(n, 256), 256, 0, L.s>>>(M.act[NL - 1], M.w4, M.b, g, n, M.y);
@@ -598,8 +619,7 @@ static int backward(Grid3 g, double* loss_part, int* loss_nb) {
     TT_LAUNCH();
     k_bwd_out<<<nblk(n, 256), 256, 0, L.s>>>(M.gy, M.w4, M.pre[3], g, n, M.gu);
     TT_LAUNCH();
-    int ntiles = (int)(n / 16);
-    unsigned tnb = (unsigned)((ntiles + 4 * TPW - 1) / (4 * TPW));
+    const unsigned tnb = conv_blocks(g);
     bf16_t* gu = M.gu;
     bf16_t* gn = M.gu2;
     for (int l = NL - 2; l >= 0; --l) {  // hidden layers 3..0: gu = dL/d(pre-activation of layer l)
@@ -615,7 +635,7 @@ static int backward(Grid3 g, double* loss_part, int* loss_nb) {
         k_fold_vec<<<1, 64, 0, L.s>>>(M.part, cnb, 16, M.gb + boff(l));
         TT_LAUNCH();
         if (l > 0) {
-            k_conv_mfma<16><<<tnb, 256, 0, L.s>>>(gu, M.frag + frag_bwd(l), nullptr, g, ntiles, 1, gn, nullptr,
+            k_conv_mfma<16><<<tnb, 256, 0, L.s>>>(gu, M.frag + frag_bwd(l), nullptr, g, 1, gn, nullptr,
                                                   M.pre[l - 1]);
             TT_LAUNCH();
             std::swap(gu, gn);
