@@ -119,59 +119,8 @@ __global__ void k_fill_box(double* a, double v, long long n) {
     if (t < n) a[t] = v;
 }
 
-// Levermore-Pomraning limiter on the three face families (pprphy.f:410-522)
+// Levermore-Pomraning limiter (pprphy.f:410-522)
 __device__ inline double lp(double rr) { return (2.0 + rr) / (6.0 + 3.0 * rr + rr * rr); }
-
-__device__ double flimx_at(const Geo& g, const Met& m, const double* ER, const double* RHQ, const double* XKR,
-                           const double* SIG, int i, int j, int k) {
-    double gx = (ER[IX(i, j, k)] - ER[IX(i - 1, j, k)]) / m(TT_M_DXB, i);
-    double gy = (m(TT_M_DDX1, i) * (ER[IX(i, j + 1, k)] - ER[IX(i, j - 1, k)]) +
-                 m(TT_M_DDX0, i) * (ER[IX(i - 1, j + 1, k)] - ER[IX(i - 1, j - 1, k)])) /
-                (m(TT_M_GEOXGA, i) * m(TT_M_GEOZG, k) * (m(TT_M_XYB, j + 1) - m(TT_M_XYB, j - 1)));
-    double gz = (m(TT_M_DDX1, i) * (ER[IX(i, j, k + 1)] - ER[IX(i, j, k - 1)]) +
-                 m(TT_M_DDX0, i) * (ER[IX(i - 1, j, k + 1)] - ER[IX(i - 1, j, k - 1)])) /
-                (m(TT_M_GEOXHA, i) * (m(TT_M_XZB, k + 1) - m(TT_M_XZB, k - 1)));
-    double grad = sqrt(gx * gx + gy * gy + gz * gz) + 1.0e-50;
-    double s = SIG[IX(i, j, k)];
-    double rr = 2.0 * grad /
-                (RHQ[IX(i, j, k)] * (m(TT_M_DDX1, i) * (XKR[IX(i, j, k)] + s) + m(TT_M_DDX0, i) * (XKR[IX(i - 1, j, k)] + s)) *
-                 (ER[IX(i, j, k)] + ER[IX(i - 1, j, k)]));
-    return lp(rr);
-}
-
-__device__ double flimy_at(const Geo& g, const Met& m, const double* ER, const double* RHQ, const double* XKR,
-                           const double* SIG, int i, int j, int k) {
-    double gx = (m(TT_M_DDY1, j) * (ER[IX(i + 1, j, k)] - ER[IX(i - 1, j, k)]) +
-                 m(TT_M_DDY0, j) * (ER[IX(i + 1, j - 1, k)] - ER[IX(i - 1, j - 1, k)])) /
-                (m(TT_M_XXB, i + 1) - m(TT_M_XXB, i - 1));
-    double gy = (ER[IX(i, j, k)] - ER[IX(i, j - 1, k)]) / (m(TT_M_GEOXG, i) * m(TT_M_GEOZG, k) * m(TT_M_DYB, j));
-    double gz = (m(TT_M_DDY1, j) * (ER[IX(i, j, k + 1)] - ER[IX(i, j, k - 1)]) +
-                 m(TT_M_DDY0, j) * (ER[IX(i, j - 1, k + 1)] - ER[IX(i, j - 1, k - 1)])) /
-                ((m(TT_M_XZB, k + 1) - m(TT_M_XZB, k - 1)) * m(TT_M_GEOXH, i));
-    double grad = sqrt(gx * gx + gy * gy + gz * gz) + 1.0e-50;
-    double s = SIG[IX(i, j, k)];
-    double rr = 2.0 * grad /
-                (RHQ[IX(i, j, k)] * (m(TT_M_DDY1, j) * (XKR[IX(i, j, k)] + s) + m(TT_M_DDY0, j) * (XKR[IX(i, j - 1, k)] + s)) *
-                 (ER[IX(i, j, k)] + ER[IX(i, j - 1, k)]));
-    return lp(rr);
-}
-
-__device__ double flimz_at(const Geo& g, const Met& m, const double* ER, const double* RHQ, const double* XKR,
-                           const double* SIG, int i, int j, int k) {
-    double gx = (m(TT_M_DDZ1, k) * (ER[IX(i + 1, j, k)] - ER[IX(i - 1, j, k)]) +
-                 m(TT_M_DDZ0, k) * (ER[IX(i + 1, j, k - 1)] - ER[IX(i - 1, j, k - 1)])) /
-                (m(TT_M_XXB, i + 1) - m(TT_M_XXB, i - 1));
-    double gy = (m(TT_M_DDZ1, k) * (ER[IX(i, j + 1, k)] - ER[IX(i, j - 1, k)]) +
-                 m(TT_M_DDZ0, k) * (ER[IX(i, j + 1, k - 1)] - ER[IX(i, j - 1, k - 1)])) /
-                ((m(TT_M_XYB, j + 1) - m(TT_M_XYB, j - 1)) * m(TT_M_GEOXG, i) * m(TT_M_GEOZGA, k));
-    double gz = (ER[IX(i, j, k)] - ER[IX(i, j, k - 1)]) / (m(TT_M_GEOXH, i) * m(TT_M_DZB, k));
-    double grad = sqrt(gx * gx + gy * gy + gz * gz) + 1.0e-50;
-    double s = SIG[IX(i, j, k)];
-    double rr = 2.0 * grad /
-                (RHQ[IX(i, j, k)] * (m(TT_M_DDZ1, k) * (XKR[IX(i, j, k)] + s) + m(TT_M_DDZ0, k) * (XKR[IX(i, j, k - 1)] + s)) *
-                 (ER[IX(i, j, k)] + ER[IX(i, j, k - 1)]));
-    return lp(rr);
-}
 
 // The three face families: FLIM (x loop, y and z with FLIM's stale
 // xmax_index, pprphy.f:410-522), DIFF (pprad.f:74-156), CALCULATEFLUX and
@@ -185,83 +134,20 @@ struct FaceArgs {
     double fbottom;
 };
 
-// x face i = 1..nx+1 of column (j, k)
-__device__ inline void face_x_eval(const Geo& g, const Met& m, const FaceArgs& a, int i, int j, int k, double& fl,
-                                   double& d, double& fx) {
-    const double* RHQ = a.RHQ[0];
-    int u = a.ub[g.idx2(j, k)];
-    if (a.iflimtyp == 1) fl = 1.0 / 3.0;
-    else fl = flimx_at(g, m, a.ER, RHQ, a.XKR, a.SIG, i <= u - 1 ? i : u - 1, j, k);
-    long long c = IX(i, j, k);
-    d = FD * fl /
-        (m(TT_M_DXB, i) * RHQ[c] *
-         (m(TT_M_DDX1, i) * (a.XKR[c] + a.SIG[c]) + m(TT_M_DDX0, i) * (a.XKR[IX(i - 1, j, k)] + a.SIG[IX(i - 1, j, k)])));
-    fx = -d * (a.ER[c] - a.ER[IX(i - 1, j, k)]);
-    if (i >= u) fx = FD * a.ER[IX(u - 1, j, k)] / 2.0;
-    if (i == 1) fx = a.fbottom;
-}
-
-// y face j = 1..ny+1 (FLIMY uses the x loop's last xmax_index, pprphy.f:464:
-// cells between it and upperbnd keep the limiter of an earlier call)
-__device__ inline void face_y_eval(const Geo& g, const Met& m, const FaceArgs& a, int i, int j, int k, double& fl,
-                                   double& d, double& fy) {
-    const double* RHQ = a.RHQ[1];
-    const double* FLIM = a.FLIM[1];
-    long long c = IX(i, j, k);
-    int u = a.ub[g.idx2(j, k)];
-    if (a.iflimtyp == 1) {
-        fl = 1.0 / 3.0;
-    } else {
-        int xmaxs = a.ub[g.idx2(g.ny, g.nz)] - 1;
-        if (i >= u) {
-            int src = u - 1;
-            fl = src <= xmaxs ? flimy_at(g, m, a.ER, RHQ, a.XKR, a.SIG, src, j, k) : FLIM[IX(src, j, k)];
-        } else if (i <= xmaxs) {
-            fl = flimy_at(g, m, a.ER, RHQ, a.XKR, a.SIG, i, j, k);
-        } else {
-            fl = FLIM[c];
-        }
-    }
-    d = FD * fl /
-        (m(TT_M_DYB, j) * m(TT_M_GEOXG, i) * m(TT_M_GEOZG, k) * RHQ[c] *
-         (m(TT_M_DDY1, j) * (a.XKR[c] + a.SIG[c]) + m(TT_M_DDY0, j) * (a.XKR[IX(i, j - 1, k)] + a.SIG[IX(i, j - 1, k)])));
-    fy = -d * (a.ER[c] - a.ER[IX(i, j - 1, k)]);
-    if (j <= g.ny && i >= u) fy = 0.0;
-}
-
-// z face k = 1..nz+1
-__device__ inline void face_z_eval(const Geo& g, const Met& m, const FaceArgs& a, int i, int j, int k, double& fl,
-                                   double& d, double& fz) {
-    const double* RHQ = a.RHQ[2];
-    const double* FLIM = a.FLIM[2];
-    long long c = IX(i, j, k);
-    int u = a.ub[g.idx2(j, k)];
-    if (a.iflimtyp == 1) {
-        fl = 1.0 / 3.0;
-    } else {
-        int xmaxs = a.ub[g.idx2(g.ny, g.nz)] - 1;
-        if (i >= u) {
-            int src = u - 1;
-            fl = src <= xmaxs ? flimz_at(g, m, a.ER, RHQ, a.XKR, a.SIG, src, j, k) : FLIM[IX(src, j, k)];
-        } else if (i <= xmaxs) {
-            fl = flimz_at(g, m, a.ER, RHQ, a.XKR, a.SIG, i, j, k);
-        } else {
-            fl = FLIM[c];
-        }
-    }
-    d = FD * fl /
-        (m(TT_M_DZB, k) * m(TT_M_GEOXH, i) * RHQ[c] *
-         (m(TT_M_DDZ1, k) * (a.XKR[c] + a.SIG[c]) + m(TT_M_DDZ0, k) * (a.XKR[IX(i, j, k - 1)] + a.SIG[IX(i, j, k - 1)])));
-    fz = -d * (a.ER[c] - a.ER[IX(i, j, k - 1)]);
-    if (k <= g.nz && i >= u) fz = 0.0;
-}
-
 // All three families in one pass over (i, j, k) in [1, nx+1] x [1, ny+1] x
 // [1, nz+1]: each thread forms the faces on the low side of its cell that
-// exist there, so the E, kappa and sigma neighbourhood is read once for
-// three faces.  Every value is formed before any is stored; a stale limiter
-// read (cells between xmax_index and upperbnd) is of a cell no thread
-// changes in this call.
+// exist there.
+//
+// Every limiter of the thread is evaluated at one cell, s = min(i, upperbnd-1):
+// FLIMX takes the limiter of face min(i, upperbnd-1); FLIMY/FLIMZ copy the one
+// of upperbnd-1 above the upper boundary, form their own below xmax_index and
+// keep an earlier call's value (FLIM read back) for s beyond xmax_index
+// (pprphy.f:464).  So the thread loads the union of the three limiters' E
+// stencils around s (16 cells), and the cells at i that DIFF and the fluxes
+// read, unconditionally and before any arithmetic; the expressions are those
+// of the reference, operation for operation.  A stale limiter read is of a
+// cell no thread changes in this call.  Indices stay inside the two ghost
+// layers: s >= 0 since upperbnd >= 1, and i, j, k reach nx+1, ny+1, nz+1.
 __global__ __launch_bounds__(256) void k_faces(Geo g, Met m, FaceArgs a) {
     long long n = (long long)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -269,11 +155,85 @@ __global__ __launch_bounds__(256) void k_faces(Geo g, Met m, FaceArgs a) {
     int i, j, k;
     sub_ijk(t, g.nx + 1, g.ny + 1, 1, 1, 1, i, j, k);
     const bool hx = j <= g.ny && k <= g.nz, hy = i <= g.nx && k <= g.nz, hz = i <= g.nx && j <= g.ny;
-    double flx = 0.0, dx = 0.0, fx = 0.0, fly = 0.0, dy = 0.0, fy = 0.0, flz = 0.0, dz = 0.0, fz = 0.0;
-    if (hx) face_x_eval(g, m, a, i, j, k, flx, dx, fx);
-    if (hy) face_y_eval(g, m, a, i, j, k, fly, dy, fy);
-    if (hz) face_z_eval(g, m, a, i, j, k, flz, dz, fz);
-    const long long c = IX(i, j, k);
+    const int u = a.ub[g.idx2(j, k)];
+    const int s = i <= u - 1 ? i : u - 1;
+    const long long sy = g.gx, sz = (long long)g.gx * g.gy;
+    const long long cs = IX(s, j, k), c = IX(i, j, k);
+    const double* __restrict__ ER = a.ER;
+    const double* __restrict__ XKR = a.XKR;
+    const double* __restrict__ SIG = a.SIG;
+    // E around s: eABC = E(s+A, j+B, k+C), m = -1, p = +1
+    const double e000 = ER[cs], em00 = ER[cs - 1], ep00 = ER[cs + 1];
+    const double e0p0 = ER[cs + sy], e0m0 = ER[cs - sy], emp0 = ER[cs - 1 + sy], emm0 = ER[cs - 1 - sy],
+                 epm0 = ER[cs + 1 - sy];
+    const double e00p = ER[cs + sz], e00m = ER[cs - sz], em0p = ER[cs - 1 + sz], em0m = ER[cs - 1 - sz],
+                 ep0m = ER[cs + 1 - sz];
+    const double e0mp = ER[cs - sy + sz], e0mm = ER[cs - sy - sz], e0pm = ER[cs + sy - sz];
+    const double ks = XKR[cs], ksx = XKR[cs - 1], ksy = XKR[cs - sy], ksz = XKR[cs - sz], ss = SIG[cs];
+    const double qsx = a.RHQ[0][cs], qsy = a.RHQ[1][cs], qsz = a.RHQ[2][cs];
+    // the cells at i
+    const double ei = ER[c], eix = ER[c - 1], eiy = ER[c - sy], eiz = ER[c - sz];
+    const double ki = XKR[c], kix = XKR[c - 1], kiy = XKR[c - sy], kiz = XKR[c - sz];
+    const double si = SIG[c], six = SIG[c - 1], siy = SIG[c - sy], siz = SIG[c - sz];
+    const double qix = a.RHQ[0][c], qiy = a.RHQ[1][c], qiz = a.RHQ[2][c];
+    const int xmaxs = a.ub[g.idx2(g.ny, g.nz)] - 1;
+
+    double flx = 1.0 / 3.0, fly = 1.0 / 3.0, flz = 1.0 / 3.0;
+    if (a.iflimtyp != 1) {
+        {  // FLIMX at (s, j, k)
+            double gx = (e000 - em00) / m(TT_M_DXB, s);
+            double gy = (m(TT_M_DDX1, s) * (e0p0 - e0m0) + m(TT_M_DDX0, s) * (emp0 - emm0)) /
+                        (m(TT_M_GEOXGA, s) * m(TT_M_GEOZG, k) * (m(TT_M_XYB, j + 1) - m(TT_M_XYB, j - 1)));
+            double gz = (m(TT_M_DDX1, s) * (e00p - e00m) + m(TT_M_DDX0, s) * (em0p - em0m)) /
+                        (m(TT_M_GEOXHA, s) * (m(TT_M_XZB, k + 1) - m(TT_M_XZB, k - 1)));
+            double grad = sqrt(gx * gx + gy * gy + gz * gz) + 1.0e-50;
+            double rr = 2.0 * grad /
+                        (qsx * (m(TT_M_DDX1, s) * (ks + ss) + m(TT_M_DDX0, s) * (ksx + ss)) * (e000 + em00));
+            flx = lp(rr);
+        }
+        if (s <= xmaxs) {
+            {  // FLIMY at (s, j, k)
+                double gx = (m(TT_M_DDY1, j) * (ep00 - em00) + m(TT_M_DDY0, j) * (epm0 - emm0)) /
+                            (m(TT_M_XXB, s + 1) - m(TT_M_XXB, s - 1));
+                double gy = (e000 - e0m0) / (m(TT_M_GEOXG, s) * m(TT_M_GEOZG, k) * m(TT_M_DYB, j));
+                double gz = (m(TT_M_DDY1, j) * (e00p - e00m) + m(TT_M_DDY0, j) * (e0mp - e0mm)) /
+                            ((m(TT_M_XZB, k + 1) - m(TT_M_XZB, k - 1)) * m(TT_M_GEOXH, s));
+                double grad = sqrt(gx * gx + gy * gy + gz * gz) + 1.0e-50;
+                double rr = 2.0 * grad /
+                            (qsy * (m(TT_M_DDY1, j) * (ks + ss) + m(TT_M_DDY0, j) * (ksy + ss)) * (e000 + e0m0));
+                fly = lp(rr);
+            }
+            {  // FLIMZ at (s, j, k)
+                double gx = (m(TT_M_DDZ1, k) * (ep00 - em00) + m(TT_M_DDZ0, k) * (ep0m - em0m)) /
+                            (m(TT_M_XXB, s + 1) - m(TT_M_XXB, s - 1));
+                double gy = (m(TT_M_DDZ1, k) * (e0p0 - e0m0) + m(TT_M_DDZ0, k) * (e0pm - e0mm)) /
+                            ((m(TT_M_XYB, j + 1) - m(TT_M_XYB, j - 1)) * m(TT_M_GEOXG, s) * m(TT_M_GEOZGA, k));
+                double gz = (e000 - e00m) / (m(TT_M_GEOXH, s) * m(TT_M_DZB, k));
+                double grad = sqrt(gx * gx + gy * gy + gz * gz) + 1.0e-50;
+                double rr = 2.0 * grad /
+                            (qsz * (m(TT_M_DDZ1, k) * (ks + ss) + m(TT_M_DDZ0, k) * (ksz + ss)) * (e000 + e00m));
+                flz = lp(rr);
+            }
+        } else {
+            fly = a.FLIM[1][cs];
+            flz = a.FLIM[2][cs];
+        }
+    }
+    // DIFF and CALCULATEFLUX + FLUX_BC at the faces of (i, j, k)
+    const double dx = FD * flx / (m(TT_M_DXB, i) * qix * (m(TT_M_DDX1, i) * (ki + si) + m(TT_M_DDX0, i) * (kix + six)));
+    double fx = -dx * (ei - eix);
+    if (i >= u) fx = FD * e000 / 2.0;  // s = upperbnd-1 here
+    if (i == 1) fx = a.fbottom;
+    const double dy = FD * fly /
+                      (m(TT_M_DYB, j) * m(TT_M_GEOXG, i) * m(TT_M_GEOZG, k) * qiy *
+                       (m(TT_M_DDY1, j) * (ki + si) + m(TT_M_DDY0, j) * (kiy + siy)));
+    double fy = -dy * (ei - eiy);
+    if (j <= g.ny && i >= u) fy = 0.0;
+    const double dz = FD * flz /
+                      (m(TT_M_DZB, k) * m(TT_M_GEOXH, i) * qiz *
+                       (m(TT_M_DDZ1, k) * (ki + si) + m(TT_M_DDZ0, k) * (kiz + siz)));
+    double fz = -dz * (ei - eiz);
+    if (k <= g.nz && i >= u) fz = 0.0;
     if (hx) { a.FLIM[0][c] = flx; a.DIFR[0][c] = dx; a.FLUX[0][c] = fx; }
     if (hy) { a.FLIM[1][c] = fly; a.DIFR[1][c] = dy; a.FLUX[1][c] = fy; }
     if (hz) { a.FLIM[2][c] = flz; a.DIFR[2][c] = dz; a.FLUX[2][c] = fz; }
