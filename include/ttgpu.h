@@ -151,8 +151,10 @@ int tt_metric_info(int metric, int32_t* lo, int32_t* hi);
  * SOR sweeps inside an FP64 defect-correction loop whose FP64 L1 residual
  * ratio sum|RHS - A d| / sum|RHS| must reach epsmat.
  * guess 0 = cold (delta_ER = 0), 1 = warm (previous solve's delta_ER times
- * warm_scale), 2 = learned (CNN prediction); a warm or learned guess is kept
- * only if its FP64 residual does not exceed the cold start's.
+ * warm_scale), 2 = learned (CNN prediction on the first Newton iteration; after
+ * 3 rejections in a row the network runs on every 16th first solve only and
+ * the others start cold); a warm or learned guess is kept only if its FP64
+ * residual does not exceed the cold start's.
  * ------------------------------------------------------------------------ */
 typedef struct tt_solver_opts {
     int32_t precision;       /* 0 = FP64 SOR, 1 = FP32 SOR + FP64 correction */

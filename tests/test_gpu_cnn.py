@@ -119,8 +119,8 @@ def test_adversarial_guess_falls_back_to_cold_start():
 
 def test_rejected_network_is_skipped_then_retried():
     """After 3 first solves in a row reject the learned guess, the network
-    runs only on every 16th first solve (each skipped one falls back to the
-    warm start); a rejected guess never changes the solve."""
+    runs only on every 16th first solve (each skipped one is a cold start:
+    no guess is staged); a rejected guess never changes the solve."""
     p = P.make_params(16, 32, 16, epsmat=1e-8)
     st = P.generate_state(p, seed=9)
     w, b = init_params(3)
@@ -144,6 +144,9 @@ def test_rejected_network_is_skipped_then_retried():
             skipped.append(es.learned_skipped)
             ran.append(1 - es.learned_skipped)
             assert es.learned_used == 0
+            if es.learned_skipped:
+                # no first-iteration guess staged: neither scaled nor tested
+                assert es.guess_scale_first == 0.0 and es.guess_r0_first == 0.0
     finally:
         d.close()
         ttgpu.load_library().tt_cnn_release()

@@ -566,10 +566,10 @@ int energy_impl(double delt, int iii, tt_energy_stats* st) {
         }
         // initial guess (new).  Later Newton iterations: the previous
         // correction scaled by warm_scale.  First Newton iteration: the
-        // learned prediction (guess 2, when a network is loaded), else the
-        // first correction of the previous step (guess 1 and 2).  The SOR
-        // solve keeps a guess only if its residual is not above the cold
-        // start's.
+        // first correction of the previous step (guess 1), or the learned
+        // prediction (guess 2, when a network is loaded; the cold start when
+        // it is skipped).  The SOR solve keeps a guess only if its residual
+        // is not above the cold start's.
         L.have_prev_guess = 0;
         if (L.opts.guess != 0) {
             if (L.energy_iter > 1) {
@@ -589,7 +589,7 @@ int energy_impl(double delt, int iii, tt_energy_stats* st) {
                         es.learned_skipped++;
                         L.learned_idle++;
                     }
-                    if (L.have_de1) {
+                    if (L.opts.guess == 1 && L.have_de1) {
                         TT_HIP(hipMemcpyAsync(f[TT_DELTA_ER], L.DE1, (size_t)g.n3 * sizeof(double),
                                               hipMemcpyDeviceToDevice, L.s));
                         L.have_prev_guess = 1;
