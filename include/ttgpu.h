@@ -152,9 +152,11 @@ int tt_metric_info(int metric, int32_t* lo, int32_t* hi);
  * ratio sum|RHS - A d| / sum|RHS| must reach epsmat.
  * guess 0 = cold (delta_ER = 0), 1 = warm (previous solve's delta_ER times
  * warm_scale), 2 = learned (CNN prediction on the first Newton iteration; after
- * 3 rejections in a row the network runs on every 16th first solve only and
- * the others start cold); a warm or learned guess is kept only if its FP64
- * residual does not exceed the cold start's.
+ * 3 rejections in a row the network runs on every 16th first solve only, and
+ * the others take the warm or the cold start, whichever has taken fewer SOR
+ * iterations on recent first solves -- a moving average with weight 1/4 --
+ * trying the other on every 16th); a warm or learned guess is kept only if its
+ * FP64 residual does not exceed the cold start's.
  * ------------------------------------------------------------------------ */
 typedef struct tt_solver_opts {
     int32_t precision;       /* 0 = FP64 SOR, 1 = FP32 SOR + FP64 correction */
@@ -229,6 +231,9 @@ typedef struct tt_energy_stats {
     double max_dq, max_de;   /* global_deltaQ, global_deltaER of last iter    */
     double guess_scale_first;/* least-residual factor of the first solve's guess */
     double guess_r0_first;   /* its L1 residual / the cold start's (0: no guess) */
+    int32_t first_start;     /* start the first solve used: 0 cold, 1 warm,
+                                2 learned                                     */
+    int32_t pad0;
 } tt_energy_stats;
 
 /* COUPLED_ENERGY_EQS (ppenergy.f:2-791).  iii is the step counter III (read

@@ -119,8 +119,11 @@ def test_adversarial_guess_falls_back_to_cold_start():
 
 def test_rejected_network_is_skipped_then_retried():
     """After 3 first solves in a row reject the learned guess, the network
-    runs only on every 16th first solve (each skipped one is a cold start:
-    no guess is staged); a rejected guess never changes the solve."""
+    runs only on every 16th first solve.  The others start warm or cold,
+    whichever has taken fewer SOR iterations on recent first solves (moving
+    averages with weight 1/4; an untried warm start is tried first; every
+    16th such solve takes the other start).  The policy is replayed here from
+    the per-call statistics.  A rejected guess never changes the solve."""
     p = P.make_params(16, 32, 16, epsmat=1e-8)
     st = P.generate_state(p, seed=9)
     w, b = init_params(3)
@@ -137,22 +140,33 @@ def test_rejected_network_is_skipped_then_retried():
     d.cnn_set_weights(w, b)
     d.cnn_set_norm(norm)
     d.set_solver(guess=2)
-    ran, skipped = [], []
+    ran, starts = [], []
+    avg, picks = [0.0, 0.0], 0
     try:
         for s in range(22):
             es, _ = d.coupled_energy(dt, 2 + s)
-            skipped.append(es.learned_skipped)
             ran.append(1 - es.learned_skipped)
-            assert es.learned_used == 0
+            assert es.learned_used == 0 and es.first_start in (0, 1)
             if es.learned_skipped:
-                # no first-iteration guess staged: neither scaled nor tested
-                assert es.guess_scale_first == 0.0 and es.guess_r0_first == 0.0
+                prefer_warm = avg[1] == 0.0 or (avg[0] > 0.0 and avg[1] <= avg[0])
+                picks += 1
+                want_warm = (not prefer_warm) if picks % 16 == 0 else prefer_warm
+                staged = es.guess_scale_first != 0.0
+                assert staged == want_warm, (s, avg, picks)
+                if not want_warm:
+                    assert es.first_start == 0
+            else:
+                assert es.first_start == 0  # the wild network's guess is rejected
+            starts.append(es.first_start)
+            k = es.first_start
+            avg[k] = float(es.sor_iter_first) if avg[k] == 0.0 else 0.75 * avg[k] + 0.25 * es.sor_iter_first
     finally:
         d.close()
         ttgpu.load_library().tt_cnn_release()
     # runs on calls 0, 1, 2, then is skipped 15 times, runs on call 18, skipped after
-    print("network ran on calls", [i for i, r in enumerate(ran) if r])
+    print("network ran on calls", [i for i, r in enumerate(ran) if r], "starts", starts, "averages", avg)
     assert ran[:3] == [1, 1, 1] and sum(ran[3:18]) == 0 and ran[18] == 1 and sum(ran[19:]) == 0
+    assert 1 in starts  # the warm start was tried
 
 
 def test_trained_guess_effect_on_sweeps():

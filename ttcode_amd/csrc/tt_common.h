@@ -96,6 +96,11 @@ struct Lib {
     // learned guess: consecutive rejections, and first solves since the
     // network last ran
     int learned_rejects = 0, learned_idle = 0;
+    // guess 2 without the network: moving averages of the first solve's SOR
+    // iterations from the cold [0] and the warm [1] start (0: none yet), and
+    // the count of such solves (every 16th tries the start not preferred)
+    double start_avg[2] = {0.0, 0.0};
+    int start_picks = 0;
     std::string err;
     // sweep timing
     bool timing = false;

@@ -567,9 +567,12 @@ int energy_impl(double delt, int iii, tt_energy_stats* st) {
         // initial guess (new).  Later Newton iterations: the previous
         // correction scaled by warm_scale.  First Newton iteration: the
         // first correction of the previous step (guess 1), or the learned
-        // prediction (guess 2, when a network is loaded; the cold start when
-        // it is skipped).  The SOR solve keeps a guess only if its residual
-        // is not above the cold start's.
+        // prediction (guess 2, when a network is loaded).  When guess 2 does
+        // not run the network, the first solve starts warm or cold, whichever
+        // has taken fewer SOR iterations on recent first solves; every 16th
+        // such solve takes the other, so its average stays current.  The SOR
+        // solve keeps a guess only if its residual is not above the cold
+        // start's.
         L.have_prev_guess = 0;
         if (L.opts.guess != 0) {
             if (L.energy_iter > 1) {
@@ -589,7 +592,13 @@ int energy_impl(double delt, int iii, tt_energy_stats* st) {
                         es.learned_skipped++;
                         L.learned_idle++;
                     }
-                    if (L.opts.guess == 1 && L.have_de1) {
+                    bool warm = true;
+                    if (L.opts.guess == 2) {
+                        const double ac = L.start_avg[0], aw = L.start_avg[1];
+                        const bool prefer_warm = aw == 0.0 || (ac > 0.0 && aw <= ac);  // untried: try it
+                        warm = (++L.start_picks % 16 == 0) ? !prefer_warm : prefer_warm;
+                    }
+                    if (warm && L.have_de1) {
                         TT_HIP(hipMemcpyAsync(f[TT_DELTA_ER], L.DE1, (size_t)g.n3 * sizeof(double),
                                               hipMemcpyDeviceToDevice, L.s));
                         L.have_prev_guess = 1;
@@ -612,6 +621,11 @@ int energy_impl(double delt, int iii, tt_energy_stats* st) {
         es.fp32_sweeps += ss.fp32_sweeps;
         if (L.energy_iter == 1) {
             es.sor_iter_first = ss.sor_iter;
+            es.first_start = ss.guess_used;
+            if (L.opts.guess == 2 && ss.guess_used < 2) {
+                double& a = L.start_avg[ss.guess_used];
+                a = a == 0.0 ? (double)ss.sor_iter : 0.75 * a + 0.25 * (double)ss.sor_iter;
+            }
             es.guess_scale_first = ss.guess_scale;
             es.guess_r0_first = (L.have_prev_guess && ss.r0_cold > 0.0) ? ss.r0_guess / ss.r0_cold : 0.0;
         }

@@ -316,6 +316,8 @@ int tt_init(const tt_params* p) {
     L.lim = tt_limits{};
     L.learned_rejects = 0;
     L.learned_idle = 0;
+    L.start_avg[0] = L.start_avg[1] = 0.0;
+    L.start_picks = 0;
     L.opts.inner_tol = 1e-5;
     L.opts.warm_scale = 0.9;
     TT_HIP(hipStreamSynchronize(L.s));
@@ -356,6 +358,11 @@ int tt_set_solver(const tt_solver_opts* o) {
     if (o->precision < 0 || o->precision > 1 || o->guess < 0 || o->guess > 2 || o->batch < 1 ||
         o->adjust_omega < 0 || o->adjust_omega > 1)
         return fail(1, "invalid solver options");
+    if (o->guess != L.opts.guess || o->precision != L.opts.precision) {
+        // sweep counts of another mode say nothing about this one
+        L.start_avg[0] = L.start_avg[1] = 0.0;
+        L.start_picks = 0;
+    }
     L.opts = *o;
     return 0;
 }

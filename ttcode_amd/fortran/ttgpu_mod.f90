@@ -25,6 +25,7 @@ module ttgpu_c
      integer(c_int32_t) :: energy_iter, sor_iter_total, sor_iter_last, sor_iter_first
      integer(c_int32_t) :: fp32_sweeps, fallbacks, learned_used, learned_skipped
      real(c_double) :: max_dq, max_de, guess_scale_first, guess_r0_first
+     integer(c_int32_t) :: first_start, pad0
   end type tt_energy_stats
 
   type, bind(C) :: tt_sor_stats

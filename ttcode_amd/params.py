@@ -92,7 +92,8 @@ class EnergyStats(ctypes.Structure):
                 ("fp32_sweeps", ctypes.c_int32), ("fallbacks", ctypes.c_int32),
                 ("learned_used", ctypes.c_int32), ("learned_skipped", ctypes.c_int32),
                 ("max_dq", ctypes.c_double), ("max_de", ctypes.c_double),
-                ("guess_scale_first", ctypes.c_double), ("guess_r0_first", ctypes.c_double)]
+                ("guess_scale_first", ctypes.c_double), ("guess_r0_first", ctypes.c_double),
+                ("first_start", ctypes.c_int32), ("pad0", ctypes.c_int32)]
 
 
 class StepStats(ctypes.Structure):
