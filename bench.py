@@ -220,7 +220,7 @@ def main():
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
-              "(the multi-GPU decomposition is a design, see DESIGN.md)", file=sys.stderr)
+              "(the multi-GPU decomposition is designed but not built)", file=sys.stderr)
         sys.exit(2)
 
     if a.calibrate_copy:
