@@ -330,6 +330,27 @@ int tt_sor_speculation(double margin);
 int tt_sor_speculation_get(int64_t* spec_batches, int64_t* reruns);
 
 /* ------------------------------------------------------------------------
+ * Table opacities, IKAPTYP 4 and 5 (KAPPA, pprphy.f:84-120; FreedmanOPC,
+ * pprphy.f:800-942).  The tables hold log10 of the Planck, Rosseland and
+ * absorption mean opacities on the reference's fixed grid of 42 temperatures
+ * (75 ... 4000 K) x 18 pressures (0.001 ... 300000 mbar) (ppmod.f:690-699),
+ * temperature fastest: kap[it + 42 ip], as the reference's
+ * FreedmanXKAP_*(42,18) lie in memory.  Entries its table file leaves out
+ * hold -7 (READFREEDMANOPCTABLE, pprphy.f:944-1002, whose text format
+ * ttcode_amd/opacity.py reads).  IKAPTYP 4 and 5 differ only in the file.
+ * ------------------------------------------------------------------------ */
+#define TT_FREEDMAN_NT 42
+#define TT_FREEDMAN_NP 18
+/* Replaces READFREEDMANOPCTABLE's fill of FreedmanXKAP_plnk/_ross/_abs
+ * (pprphy.f:944-1002).  The library keeps a copy, across tt_init and
+ * tt_finalize, until the next call; it may be called before tt_init. */
+int tt_set_opacity_table(const double* kap_planck, const double* kap_ross, const double* kap_abs);
+/* KAPPA + SUBSIG (pprphy.f:1-360,567-583) on the device state, as FLIM calls
+ * them (pprphy.f:362-380).  IKAPTYP 4 and 5 call PRESSG first and stop with
+ * a positive code on T <= 0 (pprphy.f:825-828). */
+int tt_kappa(void);
+
+/* ------------------------------------------------------------------------
  * Learned initial guess (new: no counterpart in the reference).  A 3-D CNN
  * maps five per-cell features -- log E, log T, log rho, log kappa_R,
  * asinh(STELLARINPUT), each normalised by a mean and a scale -- to the first

@@ -7,7 +7,10 @@
 ! SETUP3DMPIGRID, GITTER) but takes the module switches from a namelist
 ! instead of ipp, loads the state from raw Fortran-order binaries
 ! (in_<name>.bin) and runs one command:
-!   gitter | bounds | advect | forgas | energy | sor | adjust | step | diffusion
+!   gitter | bounds | kappa | advect | forgas | energy | sor | adjust | step |
+!   diffusion
+! For IKAPTYP 4 and 5 the reference's own READFREEDMANOPCTABLE reads its table
+! file (Freedman6117.dat or Freedman6117_noTIVO.dat) from the working directory.
 ! then writes every array of interest to out_<name>.bin and the scalars to
 ! out_scalars.txt.  CALCMUSTAR lives in ppsub.f, which needs the absent hypre
 ! include; mu_star is therefore an input (in_mu_star.bin).
@@ -145,10 +148,17 @@ program tt_ref_harness
      if (have_sin) call load3(STELLARINPUT, 'STELLARINPUT')
   end if
 
+  if (IKAPTYP == 4 .or. IKAPTYP == 5) call READFREEDMANOPCTABLE
+
   select case (trim(cmd))
   case ('gitter')
   case ('bounds')
      call BOUNDS
+  case ('kappa')
+     ! KAPPA reads upperbnd, which BOUNDS sets (PASSUPPERBND)
+     call BOUNDS
+     call KAPPA
+     call SUBSIG
   case ('advect')
      call ADVECT3D
   case ('forgas')
@@ -219,6 +229,8 @@ program tt_ref_harness
      call dump3(delta_ER, 'delta_ER'); call dump3(GD, 'GD'); call dump3(PG, 'PG')
      call dump3(RHQX, 'RHQX'); call dump3(RHQY, 'RHQY'); call dump3(RHQZ, 'RHQZ')
      call dump3(xkapR, 'xkapR'); call dump3(dxkapP_dT, 'dxkapP_dT')
+     call dump3(xkapP, 'xkapP'); call dump3(xkapA, 'xkapA'); call dump3(dxkapA_dT, 'dxkapA_dT')
+     call dump3(sig, 'sig')
      call dump3(STELLARINPUT, 'STELLARINPUT'); call dump3(STELLARTAU, 'STELLARTAU')
      call dump3(DTAUS, 'DTAUS')
      call dump3(FLIMX, 'FLIMX'); call dump3(FLIMY, 'FLIMY'); call dump3(FLIMZ, 'FLIMZ')

@@ -688,9 +688,101 @@ int tor_bounds(void) /* BOUNDS, ppbou.f:1-92, MODTYP=2 */
 }
 
 /* -------------------------------------------------------------- pprphy.f */
+/* Axes of the IKAPTYP 4/5 tables: the reference's DATA statements
+ * (ppmod.f:690-699); temperatures in K, pressures in mbar. */
+static const double FREEDMAN_T[42] = {
+    75.0, 100.0, 110.0, 120.0, 130.0, 140.0, 150.0, 160.0, 170.0, 180.0, 190.0, 200.0, 210.0, 220.0,
+    230.0, 240.0, 250.0, 275.0, 300.0, 400.0, 500.0, 575.0, 650.0, 725.0, 800.0, 900.0, 1000.0, 1100.0,
+    1200.0, 1300.0, 1400.0, 1500.0, 1600.0, 1700.0, 1800.0, 1900.0, 2000.0, 2300.0, 2600.0, 3000.0,
+    3500.0, 4000.0};
+static const double FREEDMAN_P[18] = {0.001, 0.003, 0.01, 0.03, 0.1, 0.3, 1.0, 3.0, 10.0,
+                                      30.0, 100.0, 300.0, 1000.0, 3000.0, 10000.0, 30000.0,
+                                      100000.0, 300000.0};
+/* log10 Planck, Rosseland and absorption tables, temperature fastest */
+static double g_ftab[3][42 * 18];
+static int g_ftab_set;
+
+int tor_set_opacity_table(const double* kp, const double* kr, const double* ka)
+{
+    memcpy(g_ftab[0], kp, sizeof g_ftab[0]);
+    memcpy(g_ftab[1], kr, sizeof g_ftab[1]);
+    memcpy(g_ftab[2], ka, sizeof g_ftab[2]);
+    g_ftab_set = 1;
+    return 0;
+}
+
+/* FreedmanOPC, pprphy.f:800-942: bilinear interpolation of the log10 tables
+ * in (T, P), inside the table's clamps, then 10^x; the temperature
+ * derivatives are the reference's one-sided difference ("CLUDGE FOR NOW").
+ * T <= 0 is the caller's error. */
+static void freedman_opc(double t_in, double pg_in, double* bkp, double* bkr, double* bka, double* dkp,
+                         double* dka)
+{
+    double pg1 = pg_in * (0.001);
+    double tm1 = t_in;
+    if (tm1 < 75.0) tm1 = 75.001;
+    if (tm1 > 4000.0) tm1 = 3999.9;
+    if (pg1 < 0.001) pg1 = 0.001001;
+    if (pg1 > 300000.0) pg1 = 299999.9;
+    int it = 0; /* first node above T; the clamps keep it in 1..41 */
+    while (!(FREEDMAN_T[it] > tm1)) ++it;
+    const int it1 = it - 1;
+    double ws1 = FREEDMAN_T[it1] - tm1;
+    double ws = FREEDMAN_T[it1] - FREEDMAN_T[it];
+    ws1 = ws1 / ws;
+    ws = 1.0 - ws1;
+    int iw = 0;
+    while (!(FREEDMAN_P[iw] > pg1)) ++iw;
+    const int iw1 = iw - 1;
+    double ws2 = pg1 - FREEDMAN_P[iw1];
+    double ws3 = FREEDMAN_P[iw] - FREEDMAN_P[iw1];
+    ws2 = ws2 / ws3;
+    ws3 = 1.0 - ws2;
+    double out[3], k00[3], k01[3];
+    for (int q = 0; q < 3; ++q) {
+        const double* t = g_ftab[q];
+        const double a00 = t[it + 42 * iw], a10 = t[it + 42 * iw1], a11 = t[it1 + 42 * iw1],
+                     a01 = t[it1 + 42 * iw];
+        const double w4 = ws2 * (ws1 * a00 + ws * a01);
+        const double w5 = ws3 * (ws1 * a10 + ws * a11);
+        out[q] = pow(10.0, w4 + w5);
+        k00[q] = a00;
+        k01[q] = a01;
+    }
+    *bkp = out[0];
+    *bkr = out[1];
+    *bka = out[2];
+    *dkp = (pow(10.0, k00[0]) - pow(10.0, k01[0])) / (FREEDMAN_T[it] - FREEDMAN_T[it1]);
+    *dka = (pow(10.0, k00[2]) - pow(10.0, k01[2])) / (FREEDMAN_T[it] - FREEDMAN_T[it1]);
+}
+
 static int kappa(void) /* KAPPA, pprphy.f:1-360; case 26 is new */
 {
     const tt_params* p = &O.p;
+    if (p->ikaptyp == 4 || p->ikaptyp == 5) { /* pprphy.f:84-120, MODTYP=2 */
+        if (!g_ftab_set) return seterr(1, "KAPPA: IKAPTYP 4/5 need an opacity table");
+        tor_pressg(); /* pprphy.f:88 */
+        int bad = 0;
+#pragma omp parallel for reduction(| : bad)
+        for (int k = -1; k <= NZ + 2; ++k)
+            for (int j = -1; j <= NY + 2; ++j) {
+                int ub = UBND(j, k);
+                for (int i = -1; i <= ub; ++i) {
+                    if (A(O.T, i, j, k) <= 0.0) { bad = 1; continue; }
+                    freedman_opc(A(O.T, i, j, k), A(O.PG, i, j, k), &A(O.XKP, i, j, k), &A(O.XKR, i, j, k),
+                                 &A(O.XKA, i, j, k), &A(O.DKP, i, j, k), &A(O.DKA, i, j, k));
+                }
+                for (int i = ub + 1; i <= NX + 2; ++i) {
+                    A(O.XKR, i, j, k) = A(O.XKR, ub, j, k);
+                    A(O.XKP, i, j, k) = A(O.XKP, ub, j, k);
+                    A(O.XKA, i, j, k) = A(O.XKA, ub, j, k);
+                    A(O.DKP, i, j, k) = A(O.DKP, ub, j, k);
+                    A(O.DKA, i, j, k) = A(O.DKA, ub, j, k);
+                }
+            }
+        if (bad) return seterr(2, "NEG. T. IN FREEDMAN OPC");
+        return 0;
+    }
     if (p->ikaptyp == 1) { /* pprphy.f:41-47 */
         for (size_t n = 0; n < O.n3; ++n) {
             O.XKR[n] = p->zkapmult; O.XKP[n] = p->zkapmult; O.XKA[n] = p->zkapmult;
@@ -698,7 +790,7 @@ static int kappa(void) /* KAPPA, pprphy.f:1-360; case 26 is new */
         }
         return 0;
     }
-    if (p->ikaptyp != 26) return seterr(1, "KAPPA: IKAPTYP must be 1 or 26");
+    if (p->ikaptyp != 26) return seterr(1, "KAPPA: IKAPTYP must be 1, 4, 5 or 26");
     /* Power law kappa_R = kappa_P = kappa_A = kap0 (rho/rho0)^a (T/T0)^b in the
      * loop structure of the IKAPTYP=8 power law (pprphy.f:176-198): cells up to
      * upperbnd are evaluated, cells above copy the upperbnd value.  Unlike
@@ -732,6 +824,12 @@ static int subsig(void) /* SUBSIG, pprphy.f:567-583 */
     else return seterr(1, "SUBSIG: ISIGTYP must be 0 or 1");
     for (size_t n = 0; n < O.n3; ++n) O.SIG[n] = s;
     return 0;
+}
+
+int tor_kappa(void) /* KAPPA + SUBSIG as FLIM calls them (pprphy.f:362-380) */
+{
+    int rc = kappa();
+    return rc ? rc : subsig();
 }
 
 static int flim(void) /* FLIM, pprphy.f:362-565; IRAD=3 */

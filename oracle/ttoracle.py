@@ -49,6 +49,7 @@ class Oracle:
             L.tor_schritt.argtypes = [ctypes.POINTER(ctypes.c_double)]
             L.tor_diffusion.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                         ctypes.POINTER(ctypes.c_int)]
+            L.tor_set_opacity_table.argtypes = [ctypes.c_void_p] * 3
         self.L = Oracle._lib
         self.p = p
         self.L.tor_set_threads(int(threads))
@@ -93,6 +94,19 @@ class Oracle:
     # -- routines --
     def bounds(self):
         return self._check(self.L.tor_bounds(), "BOUNDS")
+
+    def set_opacity_table(self, kap_planck, kap_ross, kap_abs):
+        """The log10 tables of IKAPTYP 4/5 on the 42 x 18 (T, P) grid,
+        temperature fastest (held across tor_init, like the reference's
+        opc_init module)."""
+        t = [np.ascontiguousarray(np.asarray(a, np.float64).reshape(-1)) for a in (kap_planck, kap_ross, kap_abs)]
+        assert all(a.size == 42 * 18 for a in t)
+        self._check(self.L.tor_set_opacity_table(*(a.ctypes.data_as(ctypes.c_void_p) for a in t)),
+                    "set_opacity_table")
+
+    def kappa(self):
+        """KAPPA + SUBSIG (pprphy.f:1-360,567-583) on the current state."""
+        return self._check(self.L.tor_kappa(), "KAPPA")
 
     def pressg(self):
         return self._check(self.L.tor_pressg(), "PRESSG")

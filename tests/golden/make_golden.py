@@ -64,8 +64,11 @@ def _nml(p: P.TTParams, cmd, delt, iii, nsteps, prefill, xnue=1e9, pr_num=1.0):
     return "\n".join(lines) + "\n"
 
 
-def run_ref(cmd, p, inputs: dict, delt=1.0, iii=1, nsteps=1, prefill=False, **kw):
-    """Run the reference harness; returns (outputs dict, scalars dict)."""
+def run_ref(cmd, p, inputs: dict, delt=1.0, iii=1, nsteps=1, prefill=False, files=None, **kw):
+    """Run the reference harness; returns (outputs dict, scalars dict).
+    files: {name: text} written into the run directory, which is then the
+    harness's working directory (the reference opens its opacity tables by
+    relative name)."""
     if not os.path.exists(TTREF):
         subprocess.check_call(["make", "-C", os.path.join(ROOT, "oracle"), "-f", "Makefile.ref"])
     with tempfile.TemporaryDirectory(dir=os.path.join(ROOT, "oracle", "_ref")) as d:
@@ -73,8 +76,12 @@ def run_ref(cmd, p, inputs: dict, delt=1.0, iii=1, nsteps=1, prefill=False, **kw
             f.write(_nml(p, cmd, delt, iii, nsteps, prefill, **kw))
         for name, a in inputs.items():
             np.ascontiguousarray(a).tofile(os.path.join(d, f"in_{name}.bin"))
+        for name, text in (files or {}).items():
+            with open(os.path.join(d, name), "w") as f:
+                f.write(text)
         env = dict(os.environ, MALLOC_CHECK_="0")
-        subprocess.run([TTREF, d], check=True, env=env, stdout=subprocess.DEVNULL)
+        subprocess.run([TTREF, d], check=True, env=env, stdout=subprocess.DEVNULL,
+                       cwd=d if files else None)
         outs = {}
         for fn in os.listdir(d):
             if not fn.startswith("out_") or not fn.endswith(".bin"):
@@ -266,10 +273,79 @@ def adjust():
          {"delt": dt, "iii": 1})
 
 
+def freedman_table_text(seed=3):
+    """A synthetic IKAPTYP 4 table in the reference's text format
+    (READFREEDMANOPCTABLE, pprphy.f:944-1002): ten header lines, T_star, then
+    736 rows T P log_kP log_kR log_kABS in shuffled order.  The 20 grid
+    points at T >= 2300 K and 10 <= P <= 300 mbar are left out, so they keep
+    the reader's -7.  Smooth laws in log T and log P with a seeded ripple."""
+    from ttcode_amd import opacity as O
+    rng = np.random.default_rng(seed)
+    rows = []
+    for ip, pv in enumerate(O.FREEDMAN_P):
+        for it, tv in enumerate(O.FREEDMAN_T):
+            if it >= 37 and 8 <= ip <= 11:
+                continue
+            lt, lp = np.log10(tv / 1500.0), np.log10(pv)
+            kr = -2.0 + 0.45 * lp + 1.2 * lt + 0.05 * np.sin(3.0 * lp) + 0.02 * rng.standard_normal()
+            kp = kr + 0.3 - 0.1 * lt + 0.02 * rng.standard_normal()
+            ka = kr - 0.5 + 0.2 * np.cos(2.0 * np.log10(tv / 100.0)) + 0.02 * rng.standard_normal()
+            rows.append(f"{tv!r} {pv!r} {float(kp)!r} {float(kr)!r} {float(ka)!r}")
+    assert len(rows) == O.N_ENTRIES
+    order = rng.permutation(len(rows))
+    head = [f"synthetic opacity table line {i + 1}" for i in range(10)]
+    return "\n".join(head + ["6000.0"] + [rows[i] for i in order]) + "\n"
+
+
+def opacities():
+    """IKAPTYP 4 (table opacities, pprphy.f:84-120,800-1002) on the
+    synthetic table: KAPPA alone on a scrambled state that reaches both
+    clamps of the table, the -7 entries and the copy above upperbnd, and
+    COUPLED_ENERGY_EQS with a 5% day-night contrast."""
+    import tempfile as _tf
+    from ttcode_amd import opacity as O
+    nx, ny, nz = 12, 16, 12
+    text = freedman_table_text()
+    with _tf.TemporaryDirectory() as d:
+        fn = os.path.join(d, O.TABLE_FILES[4])
+        with open(fn, "w") as f:
+            f.write(text)
+        kp, kr, ka, tstar = O.read_freedman_table(fn)
+    tab = {"tab_planck": kp, "tab_ross": kr, "tab_abs": ka, "tab_text": np.array(text)}
+    files = {O.TABLE_FILES[4]: text}
+    # 1. KAPPA + SUBSIG on a scrambled state
+    p = P.make_params(nx, ny, nz, ikaptyp=4, isigtyp=1, epsmat=1e-8, tirr=2000.0)
+    inp = base_inputs(p, seed=21)
+    rng = np.random.default_rng(21)
+    T = 10.0 ** rng.uniform(1.3, 3.85, size=inp["T"].shape)
+    f = np.linspace(0.0, 1.0, inp["RH"].shape[-1])  # r is the fastest (last numpy) axis
+    RH = 10.0 ** (-1.0 - 8.6 * f + rng.uniform(-0.4, 0.4, size=inp["RH"].shape))
+    inp_k = dict(inp, T=T, RH=RH)
+    outs, sc = run_ref("kappa", p, inp_k, files=files)
+    save("kappa_freedman_12x16x12", p, inp_k, outs, sc,
+         ["T", "RH", "PG", "xkapR", "xkapP", "xkapA", "dxkapP_dT", "dxkapA_dT", "sig", "upperbnd"],
+         tab)
+    # 2. COUPLED_ENERGY_EQS
+    o = Oracle(P.make_params(nx, ny, nz, ikaptyp=1, kap0=1e-2))
+    o.load_state(base_inputs(p))
+    o.bounds()
+    o.pressg()
+    dt = o.schritt()
+    pe = P.make_params(nx, ny, nz, ikaptyp=4, epsmat=1e-8, tirr=2000.0)
+    inp_e = base_inputs(pe, seed=23, contrast=0.05)
+    outs, sc = run_ref("energy", pe, inp_e, delt=dt, iii=2, files=files)
+    save("energy_freedman_12x16x12", pe, inp_e, outs, sc,
+         ["T", "ER", "delta_ER", "PG", "xkapR", "xkapP", "xkapA", "dxkapP_dT", "dxkapA_dT",
+          "STELLARINPUT", "FLIMX", "FLIMY", "FLIMZ", "FQ", "FR", "RHS_ER", "upperbnd"],
+         dict(tab, delt=dt, iii=2))
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["variants"]:
         variants()
     elif sys.argv[1:] == ["adjust"]:
         adjust()
+    elif sys.argv[1:] == ["opacities"]:
+        opacities()
     else:
         main()

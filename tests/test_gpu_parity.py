@@ -185,6 +185,65 @@ def test_coupled_energy(ttgpu, name):
     d.close()
 
 
+def _table(d, g):
+    d.set_opacity_table(g["tab_planck"], g["tab_ross"], g["tab_abs"])
+
+
+def test_kappa_freedman_table(ttgpu):
+    """KAPPA with IKAPTYP 4 (table opacities, pprphy.f:84-120,800-942) and
+    SUBSIG against the reference, on a state that reaches both table clamps,
+    the -7 entries and the copy above upperbnd (10^x comes from the device
+    math library: within 1e-12)."""
+    g = C.load("kappa_freedman_12x16x12")
+    p, d = _dev(ttgpu, g)
+    try:
+        _table(d, g)
+        d.bounds()
+        d.kappa()
+        _cmp(g, p, d, rel=1e-12)
+    finally:
+        d.close()
+
+
+def test_coupled_energy_freedman_table(ttgpu):
+    g = C.load("energy_freedman_12x16x12")
+    p, d = _dev(ttgpu, g)
+    try:
+        _table(d, g)
+        st, _ = d.coupled_energy(float(g["delt"]), int(g["iii"]))
+        assert st.energy_iter == int(g["scal_energy_iter"])
+        assert st.sor_iter_last == int(g["scal_SOR_ITER"])
+        _cmp(g, p, d, rel=1e-12, scaled_tol=1e-10)
+    finally:
+        d.close()
+
+
+def test_kappa_freedman_table_stops_on_nonpositive_temperature(ttgpu):
+    """FreedmanOPC stops on T <= 0 (pprphy.f:825-828): a zero temperature
+    planted from the host makes KAPPA return a positive code; every kernel
+    still exits normally, and the next call on a good state succeeds."""
+    g = C.load("kappa_freedman_12x16x12")
+    p, d = _dev(ttgpu, g)
+    try:
+        _table(d, g)
+        d.bounds()
+        T = d.download("T")
+        T0 = T.copy()
+        T[3, 4, 5] = 0.0
+        d.upload("T", T)
+        with pytest.raises(ttgpu.TTError, match="FREEDMAN"):
+            d.kappa()
+    finally:
+        d.close()
+    p, d = _dev(ttgpu, g)
+    try:
+        _table(d, g)
+        d.bounds()
+        d.kappa()
+    finally:
+        d.close()
+
+
 @pytest.mark.parametrize("name", ["step2_12x16x12", "step2_ibdrad9_12x16x12"])
 def test_two_steps(ttgpu, name):
     g = C.load(name)

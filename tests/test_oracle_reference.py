@@ -154,3 +154,71 @@ def test_reference_constants():
     g = C.load("bounds_12x16x12")
     assert P.FD == float(g["scal_FD"])
     assert P.ARAD == float(g["scal_ARAD"])
+
+
+def _table(o, g):
+    o.set_opacity_table(g["tab_planck"], g["tab_ross"], g["tab_abs"])
+
+
+def test_freedman_table_reader(tmp_path):
+    """READFREEDMANOPCTABLE restated (ttcode_amd/opacity.py): the fixture's
+    table text, which the reference read for the fixtures below, parses to
+    the arrays the fixtures were computed from; the 20 grid points the text
+    leaves out stay -7."""
+    from ttcode_amd import opacity as O
+    g = C.load("kappa_freedman_12x16x12")
+    fn = tmp_path / O.TABLE_FILES[4]
+    fn.write_text(str(g["tab_text"]))
+    kp, kr, ka, t_star = O.read_freedman_table(str(fn))
+    for mine, k in ((kp, "tab_planck"), (kr, "tab_ross"), (ka, "tab_abs")):
+        np.testing.assert_array_equal(mine, g[k], err_msg=k)
+        assert np.count_nonzero(mine == O.MISSING) == O.NT * O.NP - O.N_ENTRIES
+    assert t_star == 6000.0
+    # the format's list-directed reads: blank records are skipped, a row may
+    # span records, values may be comma-separated or use a d exponent
+    lines = str(g["tab_text"]).splitlines()
+    t, p, a, b, c = lines[11].split()
+    lines[11] = f"{t}, {p}\n\n{a}d0 {b}\n{c}"
+    lines.insert(3, "")
+    fn.write_text("\n".join(lines) + "\n")
+    kp2, kr2, ka2, _ = O.read_freedman_table(str(fn))
+    np.testing.assert_array_equal(kr2, kr)
+    # a row off the grid is an error
+    lines = str(g["tab_text"]).splitlines()
+    lines[12] = "1234.5 " + " ".join(lines[12].split()[1:])
+    fn.write_text("\n".join(lines) + "\n")
+    with pytest.raises(ValueError, match="not on the table grid"):
+        O.read_freedman_table(str(fn))
+
+
+def test_kappa_freedman_table():
+    """KAPPA with IKAPTYP 4 (PRESSG, then FreedmanOPC up to upperbnd and the
+    upperbnd values above it, pprphy.f:84-120,800-942) and SUBSIG, on a
+    state that reaches both table clamps and the -7 entries: bit for bit."""
+    def act(o, g):
+        _table(o, g)
+        o.bounds()
+        o.kappa()
+    g, p, o, _ = _run("kappa_freedman_12x16x12", act)
+    T = g["in_T"]
+    assert T.min() < 75.0 and T.max() > 4000.0
+    pg = g["out_PG"] * 1e-3
+    assert pg.min() < 0.001 and pg.max() > 300000.0
+    # cells evaluated (i <= upperbnd) whose interpolation square has a -7
+    # corner: T above 2000 K and P between 3 and 1000 mbar
+    i = np.arange(T.shape[-1]) - 1
+    ev = i[None, None, :] <= g["out_upperbnd"][:, :, None]
+    Tb = g["out_T"]
+    hit = ev & (Tb > 2000.0) & (pg > 3.0) & (pg < 1000.0)
+    assert np.count_nonzero(hit) > 0
+    _check_fields(g, p, o)
+
+
+def test_coupled_energy_freedman_table():
+    def act(o, g):
+        _table(o, g)
+        return o.coupled_energy(float(g["delt"]), int(g["iii"]))[0]
+    g, p, o, st = _run("energy_freedman_12x16x12", act)
+    assert st.energy_iter == int(g["scal_energy_iter"])
+    assert st.sor_iter_last == int(g["scal_SOR_ITER"])
+    _check_fields(g, p, o)

@@ -59,6 +59,7 @@ struct Ctrl {
     int bad_q;         // delta_Q positivity failure
     int bad_bound;     // BOUNDXMAX(2) upperbnd above the domain
     int bad_state;     // SCHRITT positivity failure
+    int bad_kappa;     // table opacities: T <= 0 or an undefined pressure
     int spec_hit;      // a deferred (speculative) test ended the solve
     double dtmin;
     double scratch[8];
@@ -86,6 +87,8 @@ struct Lib {
     double *TOLD = nullptr, *VH = nullptr, *GH = nullptr, *HH = nullptr;
     double *DFQ = nullptr;
     double *DE1 = nullptr;          // first Newton correction of the last step (warm start)
+    double* ftab = nullptr;         // IKAPTYP 4/5 tables on the device (3 x 42 x 18)
+    int ftab_version = -1;          // version of the host tables ftab holds
     int have_de1 = 0;
     double* partial = nullptr;      // block partial sums (reductions)
     int npartial = 0;

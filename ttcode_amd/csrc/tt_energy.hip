@@ -11,6 +11,7 @@
 // for delta_Q and the damped update with the two max reductions, then the
 // Q/ER boundary blocks.
 #include <cmath>
+#include <cstring>
 
 #include "tt_common.h"
 
@@ -52,6 +53,82 @@ __global__ void k_kappa(Geo g, tt_params p, const int32_t* __restrict__ ub, cons
     double dk = p.dkap0 * pow(rr, p.kap_a) * pow(tt, p.kap_b - 1.0);
     XKR[t] = kap; XKP[t] = kap; XKA[t] = kap;
     DKP[t] = dk; DKA[t] = dk;
+}
+
+// ---- IKAPTYP 4/5: table opacities
+// Axes of the tables: the reference's DATA statements (ppmod.f:690-699),
+// temperatures in K, pressures in mbar.
+__constant__ double c_ft[TT_FREEDMAN_NT] = {
+    75.0, 100.0, 110.0, 120.0, 130.0, 140.0, 150.0, 160.0, 170.0, 180.0, 190.0, 200.0, 210.0, 220.0,
+    230.0, 240.0, 250.0, 275.0, 300.0, 400.0, 500.0, 575.0, 650.0, 725.0, 800.0, 900.0, 1000.0, 1100.0,
+    1200.0, 1300.0, 1400.0, 1500.0, 1600.0, 1700.0, 1800.0, 1900.0, 2000.0, 2300.0, 2600.0, 3000.0,
+    3500.0, 4000.0};
+__constant__ double c_fp[TT_FREEDMAN_NP] = {0.001, 0.003, 0.01, 0.03, 0.1, 0.3, 1.0, 3.0, 10.0,
+                                            30.0, 100.0, 300.0, 1000.0, 3000.0, 10000.0, 30000.0,
+                                            100000.0, 300000.0};
+
+// KAPPA, IKAPTYP 4/5 (pprphy.f:84-120, MODTYP=2): FreedmanOPC (pprphy.f:800-
+// 942) for cells up to upperbnd, whose values the cells above take, over the
+// whole ghosted box; SUBSIG.  One thread per cell evaluates at
+// (min(i, upperbnd), j, k).  FreedmanOPC: (T, P) clamped into the table, the
+// first node above each, bilinear weights as the reference forms them, the
+// log10 interpolants raised to 10^x, and dk/dT as the one-sided difference
+// of the upper T node's column.  tab: Planck, Rosseland, absorption.
+__global__ void k_kappa_tab(Geo g, const int32_t* __restrict__ ub, const double* __restrict__ T,
+                            const double* __restrict__ PG, const double* __restrict__ tab, double* XKR,
+                            double* XKP, double* XKA, double* DKP, double* DKA, double* SIG, double sig,
+                            Ctrl* ctrl) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.n3) return;
+    SIG[t] = sig;
+    int i, j, k;
+    box_ijk(g, t, i, j, k);
+    const int u = ub[g.idx2(j, k)];
+    const long long n = g.idx(i <= u ? i : u, j, k);
+    const double t_in = T[n];
+    double pg1 = PG[n] * (0.001);
+    if (!(t_in > 0.0) || pg1 != pg1) {  // the reference stops (pprphy.f:825-828,841-862)
+        ctrl->bad_kappa = 1;
+        return;
+    }
+    double tm1 = t_in;
+    if (tm1 < 75.0) tm1 = 75.001;
+    if (tm1 > 4000.0) tm1 = 3999.9;
+    if (pg1 < 0.001) pg1 = 0.001001;
+    if (pg1 > 300000.0) pg1 = 299999.9;
+    int it = 1;  // the clamps put the first node above T in 1..41
+    while (it < TT_FREEDMAN_NT - 1 && !(c_ft[it] > tm1)) ++it;
+    const int it1 = it - 1;
+    double ws1 = c_ft[it1] - tm1;
+    double ws = c_ft[it1] - c_ft[it];
+    ws1 = ws1 / ws;
+    ws = 1.0 - ws1;
+    int iw = 1;
+    while (iw < TT_FREEDMAN_NP - 1 && !(c_fp[iw] > pg1)) ++iw;
+    const int iw1 = iw - 1;
+    double ws2 = pg1 - c_fp[iw1];
+    double ws3 = c_fp[iw] - c_fp[iw1];
+    ws2 = ws2 / ws3;
+    ws3 = 1.0 - ws2;
+    const int TN = TT_FREEDMAN_NT * TT_FREEDMAN_NP;
+    double out[3], a00s[3], a01s[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const double* tq = tab + q * TN;
+        const double a00 = tq[it + TT_FREEDMAN_NT * iw], a10 = tq[it + TT_FREEDMAN_NT * iw1];
+        const double a11 = tq[it1 + TT_FREEDMAN_NT * iw1], a01 = tq[it1 + TT_FREEDMAN_NT * iw];
+        const double w4 = ws2 * (ws1 * a00 + ws * a01);
+        const double w5 = ws3 * (ws1 * a10 + ws * a11);
+        out[q] = pow(10.0, w4 + w5);
+        a00s[q] = a00;
+        a01s[q] = a01;
+    }
+    XKP[t] = out[0];
+    XKR[t] = out[1];
+    XKA[t] = out[2];
+    const double dt = c_ft[it] - c_ft[it1];
+    DKP[t] = (pow(10.0, a00s[0]) - pow(10.0, a01s[0])) / dt;
+    DKA[t] = (pow(10.0, a00s[2]) - pow(10.0, a01s[2])) / dt;
 }
 
 // CALCSTELLARTAU (pprad.f:191-221): tau* by a reverse scan along r, one
@@ -459,13 +536,34 @@ __global__ void k_energy_final(double* DER, double* T, const double* __restrict_
     T[t] = Q[t] / (RH[t] * CV[t]);
 }
 
+// IKAPTYP 4/5 tables on the host (log10 Planck, Rosseland, absorption;
+// temperature fastest), kept across tt_init/tt_finalize like the reference's
+// opc_init module; KAPPA copies them to the device when they have changed.
+static double g_ftab[3 * TT_FREEDMAN_NT * TT_FREEDMAN_NP];
+static int g_ftab_version = 0;  // 0: never set
+
 static int kappa_impl() {
     double sig;
     if (L.p.isigtyp == 0) sig = 0.0;
     else if (L.p.isigtyp == 1) sig = 0.4;
     else return fail(1, "SUBSIG: ISIGTYP must be 0 or 1");
-    if (L.p.ikaptyp != 1 && L.p.ikaptyp != 26) return fail(1, "KAPPA: IKAPTYP must be 1 or 26");
     double** f = L.f;
+    if (L.p.ikaptyp == 4 || L.p.ikaptyp == 5) {
+        if (g_ftab_version == 0) return fail(1, "KAPPA: IKAPTYP 4 and 5 need tt_set_opacity_table first");
+        if (L.ftab_version != g_ftab_version) {
+            TT_HIP(hipMemcpyAsync(L.ftab, g_ftab, sizeof g_ftab, hipMemcpyHostToDevice, L.s));
+            TT_HIP(hipStreamSynchronize(L.s));  // g_ftab may change after this call
+            L.ftab_version = g_ftab_version;
+        }
+        int rc = pressg_impl();  // pprphy.f:88
+        if (rc) return rc;
+        k_kappa_tab<<<nblk(L.g.n3, 256), 256, 0, L.s>>>(L.g, L.ub, f[TT_T], f[TT_PG], L.ftab, f[TT_XKAPR],
+                                                       f[TT_XKAPP], f[TT_XKAPA], f[TT_DXKAPP_DT],
+                                                       f[TT_DXKAPA_DT], f[TT_SIG], sig, L.ctrl);
+        TT_LAUNCH();
+        return 0;
+    }
+    if (L.p.ikaptyp != 1 && L.p.ikaptyp != 26) return fail(1, "KAPPA: IKAPTYP must be 1, 4, 5 or 26");
     k_kappa<<<nblk(L.g.n3, 256), 256, 0, L.s>>>(L.g, L.p, L.ub, f[TT_RH], f[TT_T], f[TT_XKAPR], f[TT_XKAPP],
                                                 f[TT_XKAPA], f[TT_DXKAPP_DT], f[TT_DXKAPA_DT], f[TT_SIG], sig);
     TT_LAUNCH();
@@ -667,4 +765,21 @@ extern "C" int tt_coupled_energy(double delt, int iii, tt_energy_stats* st) {
     if (rc > 0) return rc;
     int rc2 = sync_check();
     return rc2 ? rc2 : rc;
+}
+
+extern "C" int tt_set_opacity_table(const double* kap_planck, const double* kap_ross, const double* kap_abs) {
+    if (!kap_planck || !kap_ross || !kap_abs) return fail(1, "tt_set_opacity_table: null table");
+    const int n = TT_FREEDMAN_NT * TT_FREEDMAN_NP;
+    memcpy(g_ftab, kap_planck, n * sizeof(double));
+    memcpy(g_ftab + n, kap_ross, n * sizeof(double));
+    memcpy(g_ftab + 2 * n, kap_abs, n * sizeof(double));
+    ++g_ftab_version;
+    return 0;
+}
+
+extern "C" int tt_kappa(void) {
+    if (!L.ready) return fail(1, "tt_init has not been called");
+    int rc = kappa_impl();
+    if (rc) return rc;
+    return sync_check();
 }

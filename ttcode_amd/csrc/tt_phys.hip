@@ -366,6 +366,7 @@ int sync_check() {
     TT_HIP(hipStreamSynchronize(L.s));
     if (L.hctrl->bad_bound) return fail(3, "error in boundxmax: upperbnd above the domain");
     if (L.hctrl->bad_q) return fail(2, "ERROR IN DELTA_Q: non-positive value of Q");
+    if (L.hctrl->bad_kappa) return fail(2, "NEG. T. IN FREEDMAN OPC (or an undefined pressure)");
     return 0;
 }
 

@@ -63,6 +63,15 @@ module ttgpu_c
      integer(c_int) function tt_pressg() bind(C, name='tt_pressg')
        import :: c_int
      end function tt_pressg
+     ! the reference's FreedmanXKAP_plnk/_ross/_abs(42,18) after
+     ! READFREEDMANOPCTABLE pass straight through (same memory order)
+     integer(c_int) function tt_set_opacity_table(kp, kr, ka) bind(C, name='tt_set_opacity_table')
+       import :: c_int, c_double
+       real(c_double), intent(in) :: kp(42, 18), kr(42, 18), ka(42, 18)
+     end function tt_set_opacity_table
+     integer(c_int) function tt_kappa() bind(C, name='tt_kappa')
+       import :: c_int
+     end function tt_kappa
      integer(c_int) function tt_advect3d(delt) bind(C, name='tt_advect3d')
        import :: c_int, c_double
        real(c_double), value :: delt

@@ -54,6 +54,7 @@ def load_library():
     L.tt_get_limits.argtypes = [ctypes.POINTER(P.Limits)]
     L.tt_varlimit.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double]
     L.tt_sphericalvellimit.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double]
+    L.tt_set_opacity_table.argtypes = [ctypes.c_void_p] * 3
     L.tt_get_sorparam.restype = ctypes.c_double
     L.tt_device_bytes.restype = ctypes.c_int64
     L.tt_sweep_timing_get.argtypes = [ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double),
@@ -168,6 +169,20 @@ class Device:
     # ---- the reference's routines
     def bounds(self):
         return self._check(self.L.tt_bounds(), "BOUNDS")
+
+    def set_opacity_table(self, kap_planck, kap_ross, kap_abs):
+        """The log10 tables of IKAPTYP 4/5 on the 42 x 18 (T, P) grid,
+        temperature fastest (``ttcode_amd.opacity.read_freedman_table``
+        reads the reference's table files)."""
+        t = [np.ascontiguousarray(np.asarray(a, np.float64).reshape(-1)) for a in (kap_planck, kap_ross, kap_abs)]
+        if any(a.size != 42 * 18 for a in t):
+            raise ValueError("opacity tables must hold 42 x 18 values")
+        self._check(self.L.tt_set_opacity_table(*(a.ctypes.data_as(ctypes.c_void_p) for a in t)),
+                    "tt_set_opacity_table")
+
+    def kappa(self):
+        """KAPPA + SUBSIG (pprphy.f:1-360,567-583)."""
+        return self._check(self.L.tt_kappa(), "KAPPA")
 
     def pressg(self):
         return self._check(self.L.tt_pressg(), "PRESSG")
