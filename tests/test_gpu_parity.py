@@ -244,6 +244,40 @@ def test_kappa_freedman_table_stops_on_nonpositive_temperature(ttgpu):
         d.close()
 
 
+def test_step_freedman_table_32x64x32_vs_oracle(ttgpu):
+    """One full step with table opacities (IKAPTYP 4, the synthetic table)
+    at 32x64x32 with a 5% day-night contrast, against the oracle."""
+    from oracle.ttoracle import Oracle
+    from ttcode_amd import opacity
+    p = P.make_params(32, 64, 32, epsmat=1e-8, ikaptyp=4)
+    st = P.generate_state(p, seed=29, contrast=0.05)
+    tab = opacity.synthetic_table(1)
+    o = Oracle(p, threads=1)
+    o.set_opacity_table(*tab)
+    o.load_state(st)
+    o.bounds()
+    o.pressg()
+    dt = o.schritt()
+    o.load_state(st)
+    so, _ = o.step(dt, 1)
+    d = ttgpu.Device(p)
+    try:
+        d.set_opacity_table(*tab)
+        d.load_state(st)
+        sg, _ = d.step(dt, 1)
+        assert sg.energy.energy_iter == so.energy.energy_iter
+        same = sg.energy.sor_iter_total == so.energy.sor_iter_total
+        assert same or abs(sg.energy.sor_iter_total - so.energy.sor_iter_total) <= so.energy.energy_iter
+        for k in ("RH", "V", "G", "H", "T", "ER", "xkapR", "dxkapP_dT"):
+            reg = C.compare_region(k, p.nx, p.ny, p.nz)
+            e = C.rel_err(d.download(k)[reg], o.download(k)[reg])
+            tol = 1e-12 if same or k in ("RH", "V", "G", "H") else 1e-7
+            print(f"  {k}: {e:.2e}")
+            assert e <= tol, (k, e)
+    finally:
+        d.close()
+
+
 @pytest.mark.parametrize("name", ["step2_12x16x12", "step2_ibdrad9_12x16x12"])
 def test_two_steps(ttgpu, name):
     g = C.load(name)

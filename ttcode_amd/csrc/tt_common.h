@@ -87,7 +87,7 @@ struct Lib {
     double *TOLD = nullptr, *VH = nullptr, *GH = nullptr, *HH = nullptr;
     double *DFQ = nullptr;
     double *DE1 = nullptr;          // first Newton correction of the last step (warm start)
-    double* ftab = nullptr;         // IKAPTYP 4/5 tables on the device (3 x 42 x 18)
+    double* ftab = nullptr;         // IKAPTYP 4/5 tables on the device and 10^ of them (6 x 42 x 18)
     int ftab_version = -1;          // version of the host tables ftab holds
     int have_de1 = 0;
     double* partial = nullptr;      // block partial sums (reductions)

@@ -73,7 +73,9 @@ __constant__ double c_fp[TT_FREEDMAN_NP] = {0.001, 0.003, 0.01, 0.03, 0.1, 0.3, 
 // (min(i, upperbnd), j, k).  FreedmanOPC: (T, P) clamped into the table, the
 // first node above each, bilinear weights as the reference forms them, the
 // log10 interpolants raised to 10^x, and dk/dT as the one-sided difference
-// of the upper T node's column.  tab: Planck, Rosseland, absorption.
+// of the upper T node's column.  tab: Planck, Rosseland, absorption, then
+// 10^ of each entry (k_pow10), which the difference reads.  Compute-bound:
+// three 10^x per cell (the four of the derivative come from the 10^ table).
 __global__ void k_kappa_tab(Geo g, const int32_t* __restrict__ ub, const double* __restrict__ T,
                             const double* __restrict__ PG, const double* __restrict__ tab, double* XKR,
                             double* XKP, double* XKA, double* DKP, double* DKA, double* SIG, double sig,
@@ -111,7 +113,7 @@ __global__ void k_kappa_tab(Geo g, const int32_t* __restrict__ ub, const double*
     ws2 = ws2 / ws3;
     ws3 = 1.0 - ws2;
     const int TN = TT_FREEDMAN_NT * TT_FREEDMAN_NP;
-    double out[3], a00s[3], a01s[3];
+    double out[3];
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
         const double* tq = tab + q * TN;
@@ -119,16 +121,22 @@ __global__ void k_kappa_tab(Geo g, const int32_t* __restrict__ ub, const double*
         const double a11 = tq[it1 + TT_FREEDMAN_NT * iw1], a01 = tq[it1 + TT_FREEDMAN_NT * iw];
         const double w4 = ws2 * (ws1 * a00 + ws * a01);
         const double w5 = ws3 * (ws1 * a10 + ws * a11);
-        out[q] = pow(10.0, w4 + w5);
-        a00s[q] = a00;
-        a01s[q] = a01;
+        out[q] = exp10(w4 + w5);  // 10.d0**x; within a few ulp of pow(10, x), at half its cost
     }
     XKP[t] = out[0];
     XKR[t] = out[1];
     XKA[t] = out[2];
     const double dt = c_ft[it] - c_ft[it1];
-    DKP[t] = (pow(10.0, a00s[0]) - pow(10.0, a01s[0])) / dt;
-    DKA[t] = (pow(10.0, a00s[2]) - pow(10.0, a01s[2])) / dt;
+    const double* p10 = tab + 3 * TN;  // 10^ of the Planck table
+    const double* a10 = tab + 5 * TN;  // and of the absorption table
+    DKP[t] = (p10[it + TT_FREEDMAN_NT * iw] - p10[it1 + TT_FREEDMAN_NT * iw]) / dt;
+    DKA[t] = (a10[it + TT_FREEDMAN_NT * iw] - a10[it1 + TT_FREEDMAN_NT * iw]) / dt;
+}
+
+// 10^ of every table entry, once per table
+__global__ void k_pow10(const double* __restrict__ a, double* b, int n) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) b[t] = pow(10.0, a[t]);
 }
 
 // CALCSTELLARTAU (pprad.f:191-221): tau* by a reverse scan along r, one
@@ -551,7 +559,10 @@ static int kappa_impl() {
     if (L.p.ikaptyp == 4 || L.p.ikaptyp == 5) {
         if (g_ftab_version == 0) return fail(1, "KAPPA: IKAPTYP 4 and 5 need tt_set_opacity_table first");
         if (L.ftab_version != g_ftab_version) {
+            const int n = 3 * TT_FREEDMAN_NT * TT_FREEDMAN_NP;
             TT_HIP(hipMemcpyAsync(L.ftab, g_ftab, sizeof g_ftab, hipMemcpyHostToDevice, L.s));
+            k_pow10<<<nblk(n, 256), 256, 0, L.s>>>(L.ftab, L.ftab + n, n);
+            TT_LAUNCH();
             TT_HIP(hipStreamSynchronize(L.s));  // g_ftab may change after this call
             L.ftab_version = g_ftab_version;
         }

@@ -281,7 +281,7 @@ int tt_init(const tt_params* p) {
     TT_HIP(dev_alloc(&L.ub, (size_t)g.gy * g.gz * sizeof(int32_t)));
     TT_HIP(hipMemsetAsync(L.ub, 0, (size_t)g.gy * g.gz * sizeof(int32_t), L.s));
     TT_HIP(dev_alloc(&L.mustar, (size_t)g.gy * g.gz * sizeof(double)));
-    TT_HIP(dev_alloc(&L.ftab, (size_t)3 * TT_FREEDMAN_NT * TT_FREEDMAN_NP * sizeof(double)));
+    TT_HIP(dev_alloc(&L.ftab, (size_t)6 * TT_FREEDMAN_NT * TT_FREEDMAN_NP * sizeof(double)));
     TT_HIP(dev_alloc(&L.ctrl, sizeof(Ctrl)));
     TT_HIP(hipMemsetAsync(L.ctrl, 0, sizeof(Ctrl), L.s));
     TT_HIP(hipHostMalloc((void**)&L.hctrl, sizeof(Ctrl), hipHostMallocDefault));

@@ -73,3 +73,21 @@ def read_freedman_table(path):
         tabs[:, pidx[p], tidx[t]] = (kp, kr, ka)
     kp, kr, ka = (tabs[q].reshape(-1) for q in range(3))
     return kp, kr, ka, t_star
+
+
+def synthetic_table(seed: int = 0):
+    """A smooth synthetic table on the reference's grid, for measurements
+    where no table file is at hand: log10 kappa_R = -2 + 0.45 log10 P +
+    1.2 log10(T / 1500 K) + 0.05 sin(3 log10 P), Planck 0.3 dex above it
+    (less 0.1 log10(T / 1500 K)), absorption 0.5 dex below it with a ripple
+    in log T, each with seeded 0.02 dex noise; P in mbar.  Returns
+    ``(kap_planck, kap_ross, kap_abs)``, temperature fastest, no entry
+    missing."""
+    rng = np.random.default_rng(seed)
+    lt = np.log10(np.asarray(FREEDMAN_T) / 1500.0)[None, :]
+    lp = np.log10(np.asarray(FREEDMAN_P))[:, None]
+    kr = -2.0 + 0.45 * lp + 1.2 * lt + 0.05 * np.sin(3.0 * lp) + 0.02 * rng.standard_normal((NP, NT))
+    kp = kr + 0.3 - 0.1 * lt + 0.02 * rng.standard_normal((NP, NT))
+    ka = kr - 0.5 + 0.2 * np.cos(2.0 * np.log10(np.asarray(FREEDMAN_T) / 100.0))[None, :] \
+        + 0.02 * rng.standard_normal((NP, NT))
+    return kp.reshape(-1), kr.reshape(-1), ka.reshape(-1)
