@@ -8,7 +8,7 @@
 ! instead of ipp, loads the state from raw Fortran-order binaries
 ! (in_<name>.bin) and runs one command:
 !   gitter | bounds | kappa | advect | forgas | energy | sor | adjust | step |
-!   diffusion
+!   diffusion | printout | restart
 ! For IKAPTYP 4 and 5 the reference's own READFREEDMANOPCTABLE reads its table
 ! file (Freedman6117.dat or Freedman6117_noTIVO.dat) from the working directory.
 ! then writes every array of interest to out_<name>.bin and the scalars to
@@ -35,6 +35,8 @@ program tt_ref_harness
   use scalar_var_alloc
   use dif_var_init
   use dif_var_alloc
+  use global_var_init
+  use global_var_alloc
   use global_constants, only: FD_c => FD, ARAD_c => ARAD
   implicit none
   include "mpif.h"
@@ -45,7 +47,9 @@ program tt_ref_harness
   double precision :: h_gamma, h_mu_gas, h_zkapmult, h_tirr, h_fbottom, h_d_t
   double precision :: h_omrot, h_sorparam, h_epsmat, h_fdelt, h_dtmax, h_delt
   double precision :: h_xnue, h_pr_num
-  integer :: h_prefill_kappa
+  integer :: h_prefill_kappa, h_num_iter
+  double precision :: h_zeit, h_tinitbottom, h_tinittop, h_rhinittop, h_rhinitbottom
+  double precision :: h_hydrotime, h_maccreated
   logical :: have_sin
   character(len=32) :: cmd
   character(len=512) :: dir
@@ -53,12 +57,16 @@ program tt_ref_harness
        h_f_incident, h_ikaptyp, h_isigtyp, h_itmax, h_iii, h_nsteps, &
        h_xmin, h_xmax, h_ymin, h_ymax, h_zmin, h_zmax, h_gamma, h_mu_gas, &
        h_zkapmult, h_tirr, h_fbottom, h_d_t, h_omrot, h_sorparam, h_epsmat, &
-       h_fdelt, h_dtmax, h_delt, h_xnue, h_pr_num, h_prefill_kappa, cmd
+       h_fdelt, h_dtmax, h_delt, h_xnue, h_pr_num, h_prefill_kappa, cmd, &
+       h_zeit, h_num_iter, h_tinitbottom, h_tinittop, h_rhinittop, h_rhinitbottom, &
+       h_hydrotime, h_maccreated
   integer :: u, n, s
   integer :: adj_iter(50)
 
   call get_command_argument(1, dir)
   h_nsteps = 1
+  h_zeit = 0.d0; h_num_iter = 0; h_tinitbottom = 0.d0; h_tinittop = 0.d0
+  h_rhinittop = 0.d0; h_rhinitbottom = 0.d0; h_hydrotime = 0.d0; h_maccreated = 0.d0
   h_prefill_kappa = 0
   h_xnue = 1.d9
   h_pr_num = 1.d0
@@ -209,6 +217,28 @@ program tt_ref_harness
      pass_sc(:,:,:) = T(:,:,:)
      call DIFFUSION(1)
      T(:,:,:) = pass_sc(:,:,:)
+  case ('printout')
+     ! PRINTOUT(1) (ppprt.f:13-124) after one energy solve, with the
+     ! namelist's time, counters and initial-profile scalars.  The arrays it
+     ! writes that this path does not compute (viscous dissipation, the
+     ! convective flux diagnostic) are zero, as pp.f:454 starts DISFN.
+     DISFN = 0.d0
+     conv_flux = 0.d0
+     call COUPLED_ENERGY_EQS
+     call PRESSG
+     call SOUND
+     ZEIT = h_zeit; NUM_ITER = h_num_iter
+     TinitBottom = h_tinitbottom; Tinittop = h_tinittop
+     rhinittop = h_rhinittop; rhinitbottom = h_rhinitbottom
+     hydrotime = h_hydrotime; Maccreated = h_maccreated
+     open(12, file=trim(dir)//'/SNAP_000.0001', form='unformatted', status='replace')
+     call PRINTOUT(1)
+     close(12)
+  case ('restart')
+     ! READRESTART (ppinit.f:1543-1600) of the file AKTTD in the working
+     ! directory; the global arrays it fills are written out as read
+     call allocateglobalvar
+     call READRESTART
   case default
      print *, 'unknown cmd ', trim(cmd)
      stop 2
@@ -254,6 +284,20 @@ program tt_ref_harness
   write(u, '(A,1X,ES25.17)') 'FD', FD_c
   write(u, '(A,1X,ES25.17)') 'ARAD', ARAD_c
   write(u, '(A,1X,ES25.17)') 'SORPARAM', SORPARAM
+  if (trim(cmd) == 'restart') then
+     write(u, '(A,1X,ES25.17)') 'ZEIT', ZEIT
+     write(u, '(A,1X,ES25.17)') 'DELT', DELT
+     write(u, '(A,1X,I0)') 'NUM_ITER', NUM_ITER
+     write(u, '(A,1X,ES25.17)') 'TinitBottom', TinitBottom
+     write(u, '(A,1X,ES25.17)') 'Tinittop', Tinittop
+     write(u, '(A,1X,ES25.17)') 'rhinittop', rhinittop
+     write(u, '(A,1X,ES25.17)') 'rhinitbottom', rhinitbottom
+     write(u, '(A,1X,ES25.17)') 'hydrotime', hydrotime
+     write(u, '(A,1X,ES25.17)') 'Maccreated', Maccreated
+     call dump3(global_T, 'rs_T'); call dump3(global_ER, 'rs_ER'); call dump3(global_RH, 'rs_RH')
+     call dump3(global_V, 'rs_V'); call dump3(global_G, 'rs_G'); call dump3(global_H, 'rs_H')
+     call dump2i(global_upperbnd, 'rs_upperbnd')
+  end if
   if (trim(cmd) == 'adjust') then
      do s = 1, 50
         write(u, '(A,I2.2,1X,I0)') 'ADJ_ITER_', s, adj_iter(s)

@@ -111,6 +111,9 @@ class Oracle:
     def pressg(self):
         return self._check(self.L.tor_pressg(), "PRESSG")
 
+    def sound(self):
+        return self._check(self.L.tor_sound(), "SOUND")
+
     def advect3d(self, delt):
         return self._check(self.L.tor_advect3d(delt), "ADVECT3D")
 

@@ -40,7 +40,7 @@ METRIC_BOUNDS = {  # declared bounds, ppmod.f:605-642
 }
 
 
-def _nml(p: P.TTParams, cmd, delt, iii, nsteps, prefill, xnue=1e9, pr_num=1.0):
+def _nml(p: P.TTParams, cmd, delt, iii, nsteps, prefill, xnue=1e9, pr_num=1.0, extra=None):
     v = p.as_dict()
     ent = dict(h_nx=v["nx"], h_ny=v["ny"], h_nz=v["nz"], h_bcxmin=v["bcxmin"],
                h_ibdrad=v["ibdrad"], h_iflimtyp=v["iflimtyp"], h_f_incident=v["f_incident"],
@@ -52,6 +52,7 @@ def _nml(p: P.TTParams, cmd, delt, iii, nsteps, prefill, xnue=1e9, pr_num=1.0):
                h_sorparam=v["sorparam"], h_epsmat=v["epsmat"], h_fdelt=v["fdelt"],
                h_dtmax=v["dtmax"], h_delt=delt, h_xnue=xnue, h_pr_num=pr_num,
                h_prefill_kappa=1 if prefill else 0)
+    ent.update(extra or {})
     lines = ["&ttin"]
     for k, val in ent.items():
         if isinstance(val, float):
@@ -64,11 +65,12 @@ def _nml(p: P.TTParams, cmd, delt, iii, nsteps, prefill, xnue=1e9, pr_num=1.0):
     return "\n".join(lines) + "\n"
 
 
-def run_ref(cmd, p, inputs: dict, delt=1.0, iii=1, nsteps=1, prefill=False, files=None, **kw):
+def run_ref(cmd, p, inputs: dict, delt=1.0, iii=1, nsteps=1, prefill=False, files=None, collect=(), **kw):
     """Run the reference harness; returns (outputs dict, scalars dict).
-    files: {name: text} written into the run directory, which is then the
-    harness's working directory (the reference opens its opacity tables by
-    relative name)."""
+    files: {name: text or bytes} written into the run directory, which is
+    then the harness's working directory (the reference opens its opacity
+    tables and restart file by relative name).  collect: names of files the
+    run writes there, returned as bytes under "file:<name>"."""
     if not os.path.exists(TTREF):
         subprocess.check_call(["make", "-C", os.path.join(ROOT, "oracle"), "-f", "Makefile.ref"])
     with tempfile.TemporaryDirectory(dir=os.path.join(ROOT, "oracle", "_ref")) as d:
@@ -77,7 +79,7 @@ def run_ref(cmd, p, inputs: dict, delt=1.0, iii=1, nsteps=1, prefill=False, file
         for name, a in inputs.items():
             np.ascontiguousarray(a).tofile(os.path.join(d, f"in_{name}.bin"))
         for name, text in (files or {}).items():
-            with open(os.path.join(d, name), "w") as f:
+            with open(os.path.join(d, name), "wb" if isinstance(text, bytes) else "w") as f:
                 f.write(text)
         env = dict(os.environ, MALLOC_CHECK_="0")
         subprocess.run([TTREF, d], check=True, env=env, stdout=subprocess.DEVNULL,
@@ -87,12 +89,17 @@ def run_ref(cmd, p, inputs: dict, delt=1.0, iii=1, nsteps=1, prefill=False, file
             if not fn.startswith("out_") or not fn.endswith(".bin"):
                 continue
             name = fn[4:-4]
-            raw = np.fromfile(os.path.join(d, fn), dtype=np.int32 if name == "upperbnd" else np.float64)
-            if name in METRIC_BOUNDS:
+            raw = np.fromfile(os.path.join(d, fn), dtype=np.int32 if name.endswith("upperbnd") else np.float64)
+            if name.startswith("rs_"):  # arrays READRESTART filled, as written (Fortran order)
+                outs[name] = raw
+            elif name in METRIC_BOUNDS:
                 outs["m_" + name] = raw
             else:
                 shape, _, dt = P.wire_shape(P.FIELD_IDS[name], p.nx, p.ny, p.nz)
                 outs[name] = raw.reshape(shape)
+        for name in collect:
+            with open(os.path.join(d, name), "rb") as f:
+                outs["file:" + name] = f.read()
         scal = {}
         with open(os.path.join(d, "out_scalars.txt")) as f:
             for line in f:
@@ -340,8 +347,57 @@ def opacities():
          dict(tab, delt=dt, iii=2))
 
 
+CKPT_SCALARS = dict(h_zeit=8.64e5 + 0.375, h_num_iter=1234, h_tinitbottom=1512.5, h_tinittop=1488.25,
+                    h_rhinittop=2.5e-9, h_rhinitbottom=1.25e-4, h_hydrotime=3.5e5, h_maccreated=0.0)
+
+
+def checkpoints():
+    """PRINTOUT(1) (ppprt.f:13-124) of the state after one energy solve, as
+    the reference writes it; and READRESTART (ppinit.f:1543-1600) of a global
+    restart file this library wrote (ttcode_amd.checkpoint.write_restart)
+    from the oracle's state after the same solve."""
+    from ttcode_amd import checkpoint as K
+    nx, ny, nz = 12, 16, 12
+    o = Oracle(P.make_params(nx, ny, nz, ikaptyp=1, kap0=1e-2))
+    o.load_state(base_inputs(o.p))
+    o.bounds()
+    o.pressg()
+    dt = o.schritt()
+    pe = P.make_params(nx, ny, nz, ikaptyp=1, kap0=1e-1, epsmat=1e-8, tirr=2000.0)
+    inp = base_inputs(pe, seed=31, contrast=0.05)
+    sc = dict(CKPT_SCALARS)
+    outs, scal = run_ref("printout", pe, inp, delt=dt, iii=2, files={"keep": ""},
+                         collect=("SNAP_000.0001",), extra=sc)
+    snap = np.frombuffer(outs["file:SNAP_000.0001"], dtype=np.uint8)
+    # the library's global restart file of the oracle's state after the same
+    # solve, and what the reference reads from it
+    oe = Oracle(pe)
+    oe.load_state(inp)
+    oe.coupled_energy(dt, 2)
+    oe.pressg()
+    oe.sound()
+    init = (sc["h_tinitbottom"], sc["h_tinittop"], sc["h_rhinittop"], sc["h_rhinitbottom"],
+            sc["h_hydrotime"], sc["h_maccreated"])
+    with tempfile.TemporaryDirectory() as d:
+        fn = os.path.join(d, "AKTTD")
+        K.write_restart(fn, oe, pe, sc["h_zeit"], dt, sc["h_num_iter"], init)
+        akttd = open(fn, "rb").read()
+    routs, rscal = run_ref("restart", pe, inp, delt=dt, iii=2, files={"AKTTD": akttd})
+    extra = {"delt": dt, "iii": 2, "snap_bytes": snap, "akttd_bytes": np.frombuffer(akttd, dtype=np.uint8)}
+    for k, v in sc.items():
+        extra["ckpt_" + k] = v
+    for k, v in routs.items():
+        if k.startswith("rs_"):
+            extra["ref_" + k] = v
+    for k, v in rscal.items():
+        extra["ref_scal_" + k] = v
+    save("checkpoint_12x16x12", pe, inp, outs, scal, ["T", "ER"], extra)
+
+
 if __name__ == "__main__":
-    if sys.argv[1:] == ["variants"]:
+    if sys.argv[1:] == ["checkpoints"]:
+        checkpoints()
+    elif sys.argv[1:] == ["variants"]:
         variants()
     elif sys.argv[1:] == ["adjust"]:
         adjust()

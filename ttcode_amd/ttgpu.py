@@ -187,6 +187,9 @@ class Device:
     def pressg(self):
         return self._check(self.L.tt_pressg(), "PRESSG")
 
+    def sound(self):
+        return self._check(self.L.tt_sound(), "SOUND")
+
     def qdens(self):
         return self._check(self.L.tt_qdens(), "QDENS")
 
