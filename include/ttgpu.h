@@ -108,6 +108,11 @@ enum {
     TT_UPPERBND = 80,     /* int32 (-1:ny+2,-1:nz+2)  ppmod.f:321 */
     TT_MU_STAR = 81,      /* (-1:ny+2,-1:nz+2)        ppmod.f:549 */
     TT_Q = 82,            /* Q of COUPLED_ENERGY_EQS, ghosted (local there) */
+    TT_ETA = 83,          /* eta, dynamic viscosity, ghosted  ppmod.f:443 */
+    TT_DIVTX = 84,        /* divtx (1:nx+1,0:ny+1,0:nz+1)     ppmod.f:439 */
+    TT_DIVTY = 85,        /* divty (0:nx+1,1:ny+1,0:nz+1)     ppmod.f:440 */
+    TT_DIVTZ = 86,        /* divtz (0:nx+1,0:ny+1,1:nz+1)     ppmod.f:441 */
+    TT_DISFN = 87,        /* DISFN (0:nx+1,0:ny+1,0:nz+1)     ppmod.f:442 */
     TT_FIELD_MAX = 90
 };
 
@@ -282,6 +287,25 @@ int tt_varlimit(int field, double varmin, double varmax);
  * into +-ceiling.  The ceiling is velmax, or for ivel 2 and 4 the ramp
  * velmax sin^4(pi/2 t/(25 days)) while 0 <= t <= 25 days (t = hydrotime). */
 int tt_sphericalvellimit(int ivel, double velmax, double hydrotime);
+
+/* ------------------------------------------------------------------------
+ * Viscosity in FORGAS3D (ppfor.f:170-174): ETACALC (ppvisc.f:375-484) then
+ * VISC (ppvisc.f:2-373, IVISTYP 1 in spherical coordinates), whose stress
+ * divergence divtx/y/z enters the momentum updates (ppfor.f:222-262) and
+ * whose dissipation DISFN, with IDISS, the two temperature updates
+ * (ppfor.f:202-210,335-343).  Off by default (IVIS = 0), as in the configs.
+ * ------------------------------------------------------------------------ */
+typedef struct tt_viscosity {
+    int32_t ivis;            /* > 0: viscosity on                             */
+    int32_t ivistyp;         /* 0: divtx/y/z = 0; 1: the full stress tensor   */
+    int32_t ieta;            /* ETACALC: 0 and 5 eta = RH XNUE; 4 molecular,
+                                eta = RH min(CS mu_gas 1.67e-8 / RH, XNUE)    */
+    int32_t idiss;           /* != 0: dissipative heating DISFN               */
+    double xnue;             /* XNUE, kinematic viscosity, cm^2/s             */
+} tt_viscosity;
+
+int tt_set_viscosity(const tt_viscosity* v);
+int tt_get_viscosity(tt_viscosity* v);
 
 typedef struct tt_step_stats {
     tt_energy_stats energy;

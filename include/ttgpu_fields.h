@@ -29,13 +29,17 @@ static inline int tt_field_bounds(int f, int nx, int ny, int nz,
     case TT_T: case TT_RH: case TT_ER: case TT_DELTA_ER: case TT_GD:
     case TT_PG: case TT_CS: case TT_CV: case TT_XKAPR: case TT_XKAPP:
     case TT_XKAPA: case TT_DXKAPP_DT: case TT_DXKAPA_DT: case TT_GRAVX:
-    case TT_GRAVY: case TT_GRAVZ: case TT_ER_OLD: case TT_Q:
+    case TT_GRAVY: case TT_GRAVZ: case TT_ER_OLD: case TT_Q: case TT_ETA:
         return 0;
     case TT_V: case TT_RHQX: lo[0] = 0; return 0;
     case TT_G: case TT_RHQY: lo[1] = 0; return 0;
     case TT_H: case TT_RHQZ: lo[2] = 0; return 0;
-    case TT_SIG:
+    case TT_SIG: case TT_DISFN:
         for (d = 0; d < 3; ++d) { lo[d] = 0; hi[d] = n[d] + 1; }
+        return 0;
+    case TT_DIVTX: case TT_DIVTY: case TT_DIVTZ:
+        for (d = 0; d < 3; ++d) { lo[d] = 0; hi[d] = n[d] + 1; }
+        lo[f - TT_DIVTX] = 1;
         return 0;
     case TT_STELLARINPUT: lo[0] = 1; hi[0] = nx; return 0;
     case TT_STELLARTAU: case TT_DTAUS: lo[0] = 1; hi[0] = nx + 1; return 0;

@@ -47,7 +47,7 @@ program tt_ref_harness
   double precision :: h_gamma, h_mu_gas, h_zkapmult, h_tirr, h_fbottom, h_d_t
   double precision :: h_omrot, h_sorparam, h_epsmat, h_fdelt, h_dtmax, h_delt
   double precision :: h_xnue, h_pr_num
-  integer :: h_prefill_kappa, h_num_iter
+  integer :: h_prefill_kappa, h_num_iter, h_ivis, h_ivistyp, h_ieta, h_idiss
   double precision :: h_zeit, h_tinitbottom, h_tinittop, h_rhinittop, h_rhinitbottom
   double precision :: h_hydrotime, h_maccreated
   logical :: have_sin
@@ -59,7 +59,7 @@ program tt_ref_harness
        h_zkapmult, h_tirr, h_fbottom, h_d_t, h_omrot, h_sorparam, h_epsmat, &
        h_fdelt, h_dtmax, h_delt, h_xnue, h_pr_num, h_prefill_kappa, cmd, &
        h_zeit, h_num_iter, h_tinitbottom, h_tinittop, h_rhinittop, h_rhinitbottom, &
-       h_hydrotime, h_maccreated
+       h_hydrotime, h_maccreated, h_ivis, h_ivistyp, h_ieta, h_idiss
   integer :: u, n, s
   integer :: adj_iter(50)
 
@@ -67,6 +67,7 @@ program tt_ref_harness
   h_nsteps = 1
   h_zeit = 0.d0; h_num_iter = 0; h_tinitbottom = 0.d0; h_tinittop = 0.d0
   h_rhinittop = 0.d0; h_rhinitbottom = 0.d0; h_hydrotime = 0.d0; h_maccreated = 0.d0
+  h_ivis = 0; h_ivistyp = 1; h_ieta = 4; h_idiss = 0
   h_prefill_kappa = 0
   h_xnue = 1.d9
   h_pr_num = 1.d0
@@ -82,11 +83,11 @@ program tt_ref_harness
   ISTEPMAX = 1; IREA = 10000; FDELT = h_fdelt; DTMAX = h_dtmax
   iprstep = 0; FILESTART = 0
   ietot = 0; isoth = 0; isoden = 0
-  IADV = 1; IFOR = 1; IGRAV = 1; IVIS = 0; IRAD = 3; INUC = 0; ISCAL = 1; IDIF = 2
+  IADV = 1; IFOR = 1; IGRAV = 1; IVIS = h_ivis; IRAD = 3; INUC = 0; ISCAL = 1; IDIF = 2
   IRHO = 0; RHMIN = 1.d-11
   ITEMP = 0; TEMPMAX = 1.d4; TEMPMIN = 1.d1
   IVEL = 0; VELMAX = 6.d5; VELDAMP = 0.d0; DAMP_STEPS = 100000
-  IVISTYP = 1; XNUE = h_xnue; IDISS = 0; ALPHA = 1.d-4; IETA = 4; IARTVIS = 0; PR_NUM = h_pr_num
+  IVISTYP = h_ivistyp; XNUE = h_xnue; IDISS = h_idiss; ALPHA = 1.d-4; IETA = h_ieta; IARTVIS = 0; PR_NUM = h_pr_num
   IGASTYP = 1; GAMMA = h_gamma; mu_gas = h_mu_gas; ICSTYP = 0; aspect = 0.04d0
   IKAPTYP = h_ikaptyp; ISIGTYP = h_isigtyp; ZKAPMULT = h_zkapmult; gamma_OPC = 0.1d0
   IROTATE = 0; PROT = 3.52d0; ramp_start = 0.d0; ramp_full = 0.d0; windAMP = 1.d0
@@ -261,6 +262,8 @@ program tt_ref_harness
      call dump3(xkapR, 'xkapR'); call dump3(dxkapP_dT, 'dxkapP_dT')
      call dump3(xkapP, 'xkapP'); call dump3(xkapA, 'xkapA'); call dump3(dxkapA_dT, 'dxkapA_dT')
      call dump3(sig, 'sig')
+     call dump3(eta, 'eta'); call dump3(divtx, 'divtx'); call dump3(divty, 'divty')
+     call dump3(divtz, 'divtz'); call dump3(DISFN, 'DISFN')
      call dump3(STELLARINPUT, 'STELLARINPUT'); call dump3(STELLARTAU, 'STELLARTAU')
      call dump3(DTAUS, 'DTAUS')
      call dump3(FLIMX, 'FLIMX'); call dump3(FLIMY, 'FLIMY'); call dump3(FLIMZ, 'FLIMZ')
@@ -314,6 +317,7 @@ contains
     RHQX = 0.d0; RHQY = 0.d0; RHQZ = 0.d0; STELLARINPUT = 0.d0; upperbnd = 0
     centcorx = 0.d0; centcory = 0.d0; centcorz = 0.d0
     gravx = 0.d0; gravy = 0.d0; gravz = 0.d0; divtx = 0.d0; divty = 0.d0; divtz = 0.d0
+    eta = 0.d0; DISFN = 0.d0
     FLIMX = 0.d0; FLIMY = 0.d0; FLIMZ = 0.d0; difrx = 0.d0; difry = 0.d0; difrz = 0.d0
     flux_x = 0.d0; flux_y = 0.d0; flux_z = 0.d0
     xkapR = 0.d0; xkapP = 0.d0; xkapA = 0.d0; gam_kap = 0.d0

@@ -59,6 +59,7 @@ typedef struct {
     double *EV1, *EV2, *EV3, *EV4, *EV5, *EV6, *EV7, *EV8, *EV16, *JETA, *FQ, *FR;
     double *FLIMX, *FLIMY, *FLIMZ, *DIFRX, *DIFRY, *DIFRZ, *FLUXX, *FLUXY, *FLUXZ;
     double *CCX, *CCY, *CCZ, *Q;
+    double *ETA, *DIVTX, *DIVTY, *DIVTZ, *DISFN;
     /* 2-D */
     int32_t* UB;
     double* MUSTAR;
@@ -104,6 +105,7 @@ static oracle_t O;
 #define VOLXB(i) M(TT_M_VOLXB, i)
 #define CCXA(i) M(TT_M_CCXA, i)
 #define SURYA(i) M(TT_M_SURYA, i)
+#define SURYB(i) M(TT_M_SURYB, i)
 #define VOLYA(i) M(TT_M_VOLYA, i)
 #define VOLYB(i) M(TT_M_VOLYB, i)
 #define CCYA(i) M(TT_M_CCYA, i)
@@ -199,6 +201,11 @@ static double* field_ptr(int f)
     case TT_FLUX_Y: return O.FLUXY;
     case TT_FLUX_Z: return O.FLUXZ;
     case TT_CENTCORX: return O.CCX;
+    case TT_ETA: return O.ETA;
+    case TT_DIVTX: return O.DIVTX;
+    case TT_DIVTY: return O.DIVTY;
+    case TT_DIVTZ: return O.DIVTZ;
+    case TT_DISFN: return O.DISFN;
     case TT_CENTCORY: return O.CCY;
     case TT_CENTCORZ: return O.CCZ;
     case TT_Q: return O.Q;
@@ -387,7 +394,8 @@ void tor_free(void)
                      &O.SA, &O.SB, &O.SC, &O.SD, &O.SE, &O.SF, &O.SG, &O.EV1, &O.EV2, &O.EV3,
                      &O.EV4, &O.EV5, &O.EV6, &O.EV7, &O.EV8, &O.EV16, &O.JETA, &O.FQ, &O.FR,
                      &O.FLIMX, &O.FLIMY, &O.FLIMZ, &O.DIFRX, &O.DIFRY, &O.DIFRZ, &O.FLUXX,
-                     &O.FLUXY, &O.FLUXZ, &O.CCX, &O.CCY, &O.CCZ, &O.Q};
+                     &O.FLUXY, &O.FLUXZ, &O.CCX, &O.CCY, &O.CCZ, &O.Q, &O.ETA, &O.DIVTX,
+                     &O.DIVTY, &O.DIVTZ, &O.DISFN};
     for (size_t i = 0; i < sizeof f3 / sizeof f3[0]; ++i) { free(*f3[i]); *f3[i] = NULL; }
     free(O.UB); O.UB = NULL;
     free(O.MUSTAR); O.MUSTAR = NULL;
@@ -396,12 +404,30 @@ void tor_free(void)
 
 static int g_adjust; /* ADJUST_SOR_PARAM inside COUPLED_ENERGY_EQS (off by default) */
 static tt_limits g_lim; /* clamps of the step loop (off by default) */
+static tt_viscosity g_visc; /* viscosity in FORGAS3D (off by default) */
+
+int tor_set_viscosity(const tt_viscosity* v)
+{
+    if (v->ivis > 0) {
+        if (v->ivistyp != 0 && v->ivistyp != 1) return seterr(1, "VISC: IVISTYP must be 0 or 1");
+        if (v->ieta != 0 && v->ieta != 4 && v->ieta != 5) return seterr(1, "ETACALC: IETA must be 0, 4 or 5");
+    }
+    g_visc = *v;
+    return 0;
+}
+
+int tor_get_viscosity(tt_viscosity* v)
+{
+    *v = g_visc;
+    return 0;
+}
 
 int tor_init(const tt_params* p)
 {
     tor_free();
     g_adjust = 0;
     memset(&g_lim, 0, sizeof g_lim);
+    memset(&g_visc, 0, sizeof g_visc);
     int nt = O.nthreads ? O.nthreads : 1;
     memset(&O, 0, sizeof O);
     O.nthreads = nt;
@@ -417,7 +443,8 @@ int tor_init(const tt_params* p)
                      &O.SA, &O.SB, &O.SC, &O.SD, &O.SE, &O.SF, &O.SG, &O.EV1, &O.EV2, &O.EV3,
                      &O.EV4, &O.EV5, &O.EV6, &O.EV7, &O.EV8, &O.EV16, &O.JETA, &O.FQ, &O.FR,
                      &O.FLIMX, &O.FLIMY, &O.FLIMZ, &O.DIFRX, &O.DIFRY, &O.DIFRZ, &O.FLUXX,
-                     &O.FLUXY, &O.FLUXZ, &O.CCX, &O.CCY, &O.CCZ, &O.Q};
+                     &O.FLUXY, &O.FLUXZ, &O.CCX, &O.CCY, &O.CCZ, &O.Q, &O.ETA, &O.DIVTX,
+                     &O.DIVTY, &O.DIVTZ, &O.DISFN};
     for (size_t i = 0; i < sizeof f3 / sizeof f3[0]; ++i) {
         *f3[i] = alloc3();
         if (!*f3[i]) return seterr(1, "out of memory");
@@ -1858,11 +1885,156 @@ static void centrifcoriolis(void) /* CENTRIFCORIOLIS, ppfor.f:400-492, NCOSYS=2 
             }
 }
 
-int tor_forgas3d(double delt) /* FORGAS3D, ppfor.f:142-397 (IVIS=0, IVEL=0, IARTVIS=0, IGRAV=1) */
+/* ETACALC, ppvisc.f:375-484: the dynamic viscosity for IETA 0 and 5
+ * (constant kinematic viscosity) and 4 (molecular, capped at XNUE) */
+static int etacalc(void)
+{
+    const double xnue = g_visc.xnue;
+    if (g_visc.ieta == 0 || g_visc.ieta == 5) {
+        for (size_t n = 0; n < O.n3; ++n) O.ETA[n] = O.RH[n] * xnue;
+        return 0;
+    }
+    if (g_visc.ieta != 4) return seterr(1, "ETACALC: IETA must be 0, 4 or 5");
+    tor_sound();
+    const double c = (double)1.67f * pow(10.0, -8.0); /* 1.67*10.d0**(-8.d0), 1.67 a REAL */
+    for (size_t n = 0; n < O.n3; ++n) {
+        double e = O.CS[n] * O.p.mu_gas * c / O.RH[n];
+        e = e < xnue ? e : xnue;
+        O.ETA[n] = e * O.RH[n];
+    }
+    return 0;
+}
+
+/* VISC, ppvisc.f:2-373, IVISTYP=1 in spherical coordinates (NCOSYS=2, no
+ * poles, alprat = 1): the viscous stress tensor from the velocities, its
+ * divergence divtx/y/z on the faces and the dissipation function DISFN. */
+static int visc(void)
+{
+    if (g_visc.ivistyp == 0) {
+        memset(O.DIVTX, 0, O.n3 * sizeof(double));
+        memset(O.DIVTY, 0, O.n3 * sizeof(double));
+        memset(O.DIVTZ, 0, O.n3 * sizeof(double));
+        return 0;
+    }
+    const double alprat = 1.0;
+    double *DIVV = alloc3(), *TXX = alloc3(), *TYY = alloc3(), *TZZ = alloc3(), *TXY = alloc3(),
+           *TXZ = alloc3(), *TYZ = alloc3(), *TXYAV = alloc3(), *TYZAV = alloc3(), *TXZAV = alloc3();
+    double *V = O.V, *G = O.G, *H = O.H, *ETA = O.ETA;
+    for (int k = 0; k <= NZ + 1; ++k) /* :30-42 */
+        for (int j = 0; j <= NY + 1; ++j)
+            for (int i = 0; i <= NX + 1; ++i) {
+                A(TXX, i, j, k) = 2.0 * A(ETA, i, j, k) * (A(V, i + 1, j, k) - A(V, i, j, k)) / (XXA(i + 1) - XXA(i));
+                A(DIVV, i, j, k) = 2.0 / 3.0 * A(ETA, i, j, k) *
+                                   ((SURXA(i + 1) * A(V, i + 1, j, k) - SURXA(i) * A(V, i, j, k)) / VOLXB(i) +
+                                    (SURYA(j + 1) * A(G, i, j + 1, k) - SURYA(j) * A(G, i, j, k)) / VOLYB(j) +
+                                    (SURZA(k + 1) * A(H, i, j, k + 1) - SURZA(k) * A(H, i, j, k)) / VOLZB(k));
+            }
+    for (int k = 0; k <= NZ + 1; ++k) /* :75-88 */
+        for (int j = 0; j <= NY + 1; ++j)
+            for (int i = 0; i <= NX + 1; ++i) {
+                A(TYY, i, j, k) = 2.0 * A(ETA, i, j, k) *
+                                  ((A(G, i, j + 1, k) - A(G, i, j, k)) / (XYA(j + 1) - XYA(j)) +
+                                   (A(V, i + 1, j, k) + A(V, i, j, k)) / 2.0 / XXB(i) +
+                                   (A(H, i, j, k + 1) + A(H, i, j, k)) / 2.0 * (-tan(XZB(k))));
+                A(TZZ, i, j, k) = 2.0 * A(ETA, i, j, k) *
+                                  ((A(H, i, j, k + 1) - A(H, i, j, k)) / (XZA(k + 1) - XZA(k)) +
+                                   (A(V, i + 1, j, k) + A(V, i, j, k)) / 2.0 / XXB(i));
+            }
+    for (int k = 0; k <= NZ + 2; ++k) /* :95-118 */
+        for (int j = 0; j <= NY + 2; ++j)
+            for (int i = 0; i <= NX + 2; ++i) {
+                A(TXY, i, j, k) = 0.5 * ((A(V, i, j, k) - A(V, i, j - 1, k)) / (XYB(j) - XYB(j - 1)) / GEOXGA(i) / GEOZGA(k) +
+                                         (A(G, i, j, k) - A(G, i - 1, j, k)) / (XXB(i) - XXB(i - 1)) * GEOXGA(i) *
+                                             GEOZGA(k) / alprat);
+                A(TXZ, i, j, k) = 0.5 * ((A(V, i, j, k) - A(V, i, j, k - 1)) / (XZB(k) - XZB(k - 1)) / GEOXHA(i) +
+                                         (A(H, i, j, k) - A(H, i - 1, j, k)) / (XXB(i) - XXB(i - 1)) * GEOXHA(i));
+                if (GEOXHA(i) < (double)1.e-10f) A(TXZ, i, j, k) = 0.0;
+                A(TYZ, i, j, k) =
+                    0.5 * ((A(H, i, j, k) - A(H, i, j - 1, k)) / (XYB(j) - XYB(j - 1)) / GEOXG(i) / GEOZGA(k) * GEOXH(i) +
+                           (A(G, i, j, k) - A(G, i, j, k - 1)) / (XZB(k) - XZB(k - 1)) * GEOXG(i) * GEOZGA(k) / GEOXH(i) /
+                               alprat);
+            }
+    for (int k = 0; k <= NZ + 1; ++k) /* :127-139 */
+        for (int j = 0; j <= NY + 1; ++j)
+            for (int i = 0; i <= NX + 1; ++i) {
+                A(TXYAV, i, j, k) = A(ETA, i, j, k) *
+                                    (A(TXY, i, j, k) + A(TXY, i + 1, j, k) + A(TXY, i, j + 1, k) + A(TXY, i + 1, j + 1, k)) / 2.0;
+                A(TYZAV, i, j, k) = A(ETA, i, j, k) *
+                                    (A(TYZ, i, j, k) + A(TYZ, i, j + 1, k) + A(TYZ, i, j, k + 1) + A(TYZ, i, j + 1, k + 1)) / 2.0;
+                A(TXZAV, i, j, k) = A(ETA, i, j, k) *
+                                    (A(TXZ, i, j, k) + A(TXZ, i + 1, j, k) + A(TXZ, i, j, k + 1) + A(TXZ, i + 1, j, k + 1)) / 2.0;
+            }
+    for (int k = 0; k <= NZ + 2; ++k) /* :141-152 */
+        for (int j = 0; j <= NY + 2; ++j)
+            for (int i = 0; i <= NX + 2; ++i) {
+                A(TXY, i, j, k) = A(TXY, i, j, k) / 2.0 *
+                                  (A(ETA, i, j, k) + A(ETA, i - 1, j, k) + A(ETA, i, j - 1, k) + A(ETA, i - 1, j - 1, k));
+                A(TXZ, i, j, k) = A(TXZ, i, j, k) / 2.0 *
+                                  (A(ETA, i, j, k) + A(ETA, i - 1, j, k) + A(ETA, i, j, k - 1) + A(ETA, i - 1, j, k - 1));
+                A(TYZ, i, j, k) = A(TYZ, i, j, k) / 2.0 *
+                                  (A(ETA, i, j, k) + A(ETA, i, j, k - 1) + A(ETA, i, j - 1, k) + A(ETA, i, j - 1, k - 1));
+            }
+    if (g_visc.idiss != 0) /* :155-167 */
+        for (int k = 0; k <= NZ + 1; ++k)
+            for (int j = 0; j <= NY + 1; ++j)
+                for (int i = 0; i <= NX + 1; ++i)
+                    A(O.DISFN, i, j, k) =
+                        (3.0 * A(DIVV, i, j, k) * A(DIVV, i, j, k) + A(TXX, i, j, k) * A(TXX, i, j, k) +
+                         2.0 * A(TXYAV, i, j, k) * A(TXYAV, i, j, k) * alprat + A(TYY, i, j, k) * A(TYY, i, j, k) +
+                         2.0 * A(TYZAV, i, j, k) * A(TYZAV, i, j, k) * alprat + A(TZZ, i, j, k) * A(TZZ, i, j, k) +
+                         2.0 * A(TXZAV, i, j, k) * A(TXZAV, i, j, k)) /
+                        2.0 / A(ETA, i, j, k);
+    for (int k = 0; k <= NZ + 1; ++k) /* :272-286 */
+        for (int j = 0; j <= NY + 1; ++j)
+            for (int i = 1; i <= NX + 1; ++i)
+                A(O.DIVTX, i, j, k) =
+                    (SURXB(i - 1) * A(DIVV, i - 1, j, k) - SURXB(i) * A(DIVV, i, j, k)) / VOLXA(i) +
+                    (SURXB(i) * A(TXX, i, j, k) - SURXB(i - 1) * A(TXX, i - 1, j, k)) / VOLXA(i) +
+                    (SURYA(j + 1) * A(TXY, i, j + 1, k) - SURYA(j) * A(TXY, i, j, k)) / VOLYB(j) / GEOXGA(i) / GEOZG(k) +
+                    (SURZA(k + 1) * A(TXZ, i, j, k + 1) - SURZA(k) * A(TXZ, i, j, k)) / VOLZB(k) / GEOXHA(i) -
+                    (A(TYY, i, j, k) + A(TYY, i - 1, j, k) + A(TZZ, i, j, k) + A(TZZ, i - 1, j, k) -
+                     (A(DIVV, i, j, k) + A(DIVV, i - 1, j, k)) * 2.0) /
+                        2.0 / XXA(i);
+    for (int k = 0; k <= NZ + 1; ++k) /* :288-303 */
+        for (int j = 1; j <= NY + 1; ++j)
+            for (int i = 0; i <= NX + 1; ++i)
+                A(O.DIVTY, i, j, k) =
+                    (SURYB(j - 1) * A(DIVV, i, j - 1, k) - SURYB(j) * A(DIVV, i, j, k)) / VOLYA(j) / GEOXG(i) / GEOZG(k) +
+                    (SURXA(i + 1) * A(TXY, i + 1, j, k) - SURXA(i) * A(TXY, i, j, k)) / VOLXB(i) +
+                    (SURYB(j) * A(TYY, i, j, k) - SURYB(j - 1) * A(TYY, i, j - 1, k)) / VOLYA(j) / GEOXG(i) / GEOZG(k) +
+                    (SURZA(k + 1) * A(TYZ, i, j, k + 1) - SURZA(k) * A(TYZ, i, j, k)) / VOLZB(k) / GEOXH(i) +
+                    ((A(TYZAV, i, j, k) + A(TYZAV, i, j - 1, k)) * (-tan(XZB(k))) +
+                     (A(TXYAV, i, j, k) + A(TXYAV, i, j - 1, k))) /
+                        2.0 / XXB(i);
+    for (int k = 1; k <= NZ + 1; ++k) /* :305-321, no poles */
+        for (int j = 0; j <= NY + 1; ++j)
+            for (int i = 0; i <= NX + 1; ++i)
+                A(O.DIVTZ, i, j, k) =
+                    (SURZB(k - 1) * A(DIVV, i, j, k - 1) - SURZB(k) * A(DIVV, i, j, k)) / VOLZA(k) / GEOXH(i) +
+                    (SURXA(i + 1) * A(TXZ, i + 1, j, k) - SURXA(i) * A(TXZ, i, j, k)) / VOLXB(i) +
+                    (SURYA(j + 1) * A(TYZ, i, j + 1, k) - SURYA(j) * A(TYZ, i, j, k)) / VOLYB(j) / GEOXG(i) / GEOZGA(k) +
+                    (SURZB(k) * A(TZZ, i, j, k) - SURZB(k - 1) * A(TZZ, i, j, k - 1)) / VOLZA(k) / GEOXH(i) +
+                    ((A(TYY, i, j, k) + A(TYY, i, j, k - 1) - (A(DIVV, i, j, k) + A(DIVV, i, j, k - 1))) * tan(XZA(k)) +
+                     A(TXZAV, i, j, k) + A(TXZAV, i, j, k - 1)) /
+                        2.0 / XXB(i);
+    free(DIVV); free(TXX); free(TYY); free(TZZ); free(TXY); free(TXZ); free(TYZ);
+    free(TXYAV); free(TYZAV); free(TXZAV);
+    return 0;
+}
+
+int tor_forgas3d(double delt) /* FORGAS3D, ppfor.f:142-397 (IVEL=0, IARTVIS=0, IGRAV=1) */
 {
     int rc = tor_bounds();
     if (rc) return rc;
     tor_pressg();
+    const int vis = g_visc.ivis > 0;
+    if (vis) { /* ppfor.f:170-174 */
+        rc = etacalc();
+        if (rc) return rc;
+        rc = visc();
+        if (rc) return rc;
+    }
+    const int diss = vis && g_visc.idiss != 0;
     centrifcoriolis();
     double* TOLD = alloc3();
     double* VH = alloc3(); double* GH = alloc3(); double* HH = alloc3();
@@ -1877,25 +2049,30 @@ int tor_forgas3d(double delt) /* FORGAS3D, ppfor.f:142-397 (IVIS=0, IVEL=0, IART
                                               ((SURXA(i + 1) * A(O.V, i + 1, j, k) - SURXA(i) * A(O.V, i, j, k)) / VOLXB(i) +
                                                (SURYA(j + 1) * A(O.G, i, j + 1, k) - SURYA(j) * A(O.G, i, j, k)) / VOLYB(j) +
                                                (SURZA(k + 1) * A(O.H, i, j, k + 1) - SURZA(k) * A(O.H, i, j, k)) / VOLZB(k));
+        if (diss) /* ppfor.f:202-210 */
+            for (int k = 1; k <= NZ; ++k)
+                for (int j = 1; j <= NY; ++j)
+                    for (int i = 1; i <= NX; ++i)
+                        A(O.T, i, j, k) = A(O.T, i, j, k) + delt / 2.0 * A(O.DISFN, i, j, k) / A(O.CV, i, j, k) / A(O.RH, i, j, k);
         rc = tor_bounds();
         if (rc) goto out;
         tor_pressg();
     }
-    /* DIVTX/Y/Z are zero with IVIS=0 */
+    /* divtx/y/z: the viscous stress divergence (zero without viscosity) */
 #pragma omp parallel for
     for (int k = 1; k <= NZ; ++k)
         for (int j = 1; j <= NY; ++j)
             for (int i = 1; i <= NX; ++i) {
                 A(VH, i, j, k) = A(O.V, i, j, k) +
-                                 delt * ((-(A(O.PG, i, j, k) - A(O.PG, i - 1, j, k)) / DXB(i) + 0.0) / A(O.RHQX, i, j, k) +
+                                 delt * ((-(A(O.PG, i, j, k) - A(O.PG, i - 1, j, k)) / DXB(i) + (vis ? A(O.DIVTX, i, j, k) : 0.0)) / A(O.RHQX, i, j, k) +
                                          A(O.CCX, i, j, k) + A(O.GRAVX, i, j, k));
                 A(GH, i, j, k) = A(O.G, i, j, k) +
-                                 delt * (((-(A(O.PG, i, j, k) - A(O.PG, i, j - 1, k)) / DYB(j) / GEOXG(i) / GEOZG(k) + 0.0) /
+                                 delt * (((-(A(O.PG, i, j, k) - A(O.PG, i, j - 1, k)) / DYB(j) / GEOXG(i) / GEOZG(k) + (vis ? A(O.DIVTY, i, j, k) : 0.0)) /
                                               A(O.RHQY, i, j, k) +
                                           A(O.CCY, i, j, k) + A(O.GRAVY, i, j, k)) /
                                          GEOXG(i) / GEOZG(k));
                 A(HH, i, j, k) = A(O.H, i, j, k) +
-                                 delt * (((-(A(O.PG, i, j, k) - A(O.PG, i, j, k - 1)) / DZB(k) / GEOXH(i) + 0.0) /
+                                 delt * (((-(A(O.PG, i, j, k) - A(O.PG, i, j, k - 1)) / DZB(k) / GEOXH(i) + (vis ? A(O.DIVTZ, i, j, k) : 0.0)) /
                                               A(O.RHQZ, i, j, k) +
                                           A(O.CCZ, i, j, k) + A(O.GRAVZ, i, j, k)) /
                                          GEOXH(i));
@@ -1921,6 +2098,11 @@ int tor_forgas3d(double delt) /* FORGAS3D, ppfor.f:142-397 (IVIS=0, IVEL=0, IART
                                               ((SURXA(i + 1) * A(O.V, i + 1, j, k) - SURXA(i) * A(O.V, i, j, k)) / VOLXB(i) +
                                                (SURYA(j + 1) * A(O.G, i, j + 1, k) - SURYA(j) * A(O.G, i, j, k)) / VOLYB(j) +
                                                (SURZA(k + 1) * A(O.H, i, j, k + 1) - SURZA(k) * A(O.H, i, j, k)) / VOLZB(k));
+        if (diss) /* ppfor.f:335-343 */
+            for (int k = 1; k <= NZ; ++k)
+                for (int j = 1; j <= NY; ++j)
+                    for (int i = 1; i <= NX; ++i)
+                        A(O.T, i, j, k) = A(O.T, i, j, k) + delt * A(O.DISFN, i, j, k) / A(O.CV, i, j, k) / A(O.RH, i, j, k);
     }
 #pragma omp parallel for
     for (int k = 1; k <= NZ; ++k)

@@ -50,6 +50,8 @@ class Oracle:
             L.tor_diffusion.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                         ctypes.POINTER(ctypes.c_int)]
             L.tor_set_opacity_table.argtypes = [ctypes.c_void_p] * 3
+            L.tor_set_viscosity.argtypes = [ctypes.POINTER(P.Viscosity)]
+            L.tor_get_viscosity.argtypes = [ctypes.POINTER(P.Viscosity)]
         self.L = Oracle._lib
         self.p = p
         self.L.tor_set_threads(int(threads))
@@ -103,6 +105,11 @@ class Oracle:
         assert all(a.size == 42 * 18 for a in t)
         self._check(self.L.tor_set_opacity_table(*(a.ctypes.data_as(ctypes.c_void_p) for a in t)),
                     "set_opacity_table")
+
+    def set_viscosity(self, ivis=1, ivistyp=1, ieta=4, idiss=1, xnue=1e9):
+        """ETACALC + VISC inside FORGAS3D (ppfor.f:170-174), off by default."""
+        self._check(self.L.tor_set_viscosity(ctypes.byref(P.Viscosity(ivis, ivistyp, ieta, idiss, xnue))),
+                    "set_viscosity")
 
     def kappa(self):
         """KAPPA + SUBSIG (pprphy.f:1-360,567-583) on the current state."""

@@ -347,6 +347,27 @@ def opacities():
          dict(tab, delt=dt, iii=2))
 
 
+def viscosity():
+    """FORGAS3D with viscosity (ppfor.f:170-174: ETACALC, ppvisc.f:375-484;
+    VISC, ppvisc.f:2-373 with IVISTYP=1 and dissipative heating) after the
+    advection fixture's BOUNDS + QDENS: the shipped ipp's molecular
+    viscosity (IETA=4, XNUE=1e9) and a constant kinematic one (IETA=0)."""
+    nx, ny, nz = 12, 16, 12
+    p = P.make_params(nx, ny, nz, ikaptyp=1, kap0=1e-2)
+    o = Oracle(p)
+    o.load_state(base_inputs(p))
+    o.bounds()
+    o.pressg()
+    dt = o.schritt()
+    keep = ["T", "RH", "V", "G", "H", "PG", "eta", "divtx", "divty", "divtz", "DISFN"]
+    for ieta, xnue, seed in ((4, 1e9, 41), (0, 3e14, 43)):
+        inp = base_inputs(p, seed=seed, contrast=0.05)
+        vis = dict(h_ivis=1, h_ivistyp=1, h_ieta=ieta, h_idiss=1)
+        outs, sc = run_ref("forgas", p, inp, delt=dt, xnue=xnue, extra=vis)
+        save(f"forgas_visc_ieta{ieta}_12x16x12", p, inp, outs, sc, keep,
+             {"delt": dt, "vis_ivis": 1, "vis_ivistyp": 1, "vis_ieta": ieta, "vis_idiss": 1, "vis_xnue": xnue})
+
+
 CKPT_SCALARS = dict(h_zeit=8.64e5 + 0.375, h_num_iter=1234, h_tinitbottom=1512.5, h_tinittop=1488.25,
                     h_rhinittop=2.5e-9, h_rhinitbottom=1.25e-4, h_hydrotime=3.5e5, h_maccreated=0.0)
 
@@ -397,6 +418,8 @@ def checkpoints():
 if __name__ == "__main__":
     if sys.argv[1:] == ["checkpoints"]:
         checkpoints()
+    elif sys.argv[1:] == ["viscosity"]:
+        viscosity()
     elif sys.argv[1:] == ["variants"]:
         variants()
     elif sys.argv[1:] == ["adjust"]:

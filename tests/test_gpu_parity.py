@@ -278,6 +278,56 @@ def test_step_freedman_table_32x64x32_vs_oracle(ttgpu):
         d.close()
 
 
+@pytest.mark.parametrize("name", ["forgas_visc_ieta4_12x16x12", "forgas_visc_ieta0_12x16x12"])
+def test_forgas3d_viscosity(ttgpu, name):
+    """FORGAS3D with ETACALC + VISC (IVISTYP=1, spherical) and dissipative
+    heating against the reference (tan from the device math library)."""
+    g = C.load(name)
+    p, d = _dev(ttgpu, g)
+    try:
+        d.set_viscosity(int(g["vis_ivis"]), int(g["vis_ivistyp"]), int(g["vis_ieta"]), int(g["vis_idiss"]),
+                        float(g["vis_xnue"]))
+        d.bounds()
+        d.qdens()
+        d.forgas3d(float(g["delt"]))
+        _cmp(g, p, d, rel=1e-12)
+    finally:
+        d.close()
+
+
+def test_step_viscosity_32x64x32_vs_oracle(ttgpu):
+    """One full step with viscosity (IETA=0, strong enough to move the
+    flow) at 32x64x32 against the oracle."""
+    from oracle.ttoracle import Oracle
+    p = P.make_params(32, 64, 32, epsmat=1e-8)
+    st = P.generate_state(p, seed=37, contrast=0.05)
+    vis = (1, 1, 0, 1, 3e14)
+    o = Oracle(p, threads=1)
+    o.load_state(st)
+    o.bounds()
+    o.pressg()
+    dt = o.schritt()
+    o.load_state(st)
+    o.set_viscosity(*vis)
+    so, _ = o.step(dt, 1)
+    d = ttgpu.Device(p)
+    try:
+        d.load_state(st)
+        d.set_viscosity(*vis)
+        sg, _ = d.step(dt, 1)
+        assert sg.energy.energy_iter == so.energy.energy_iter
+        same = sg.energy.sor_iter_total == so.energy.sor_iter_total
+        assert same or abs(sg.energy.sor_iter_total - so.energy.sor_iter_total) <= so.energy.energy_iter
+        for k in ("RH", "V", "G", "H", "T", "ER", "divtx", "DISFN"):
+            reg = C.compare_region(k, p.nx, p.ny, p.nz)
+            e = C.rel_err(d.download(k)[reg], o.download(k)[reg])
+            tol = 1e-12 if same or k not in ("T", "ER") else 1e-7
+            print(f"  {k}: {e:.2e}")
+            assert e <= tol, (k, e)
+    finally:
+        d.close()
+
+
 @pytest.mark.parametrize("name", ["step2_12x16x12", "step2_ibdrad9_12x16x12"])
 def test_two_steps(ttgpu, name):
     g = C.load(name)

@@ -222,3 +222,31 @@ def test_coupled_energy_freedman_table():
     assert st.energy_iter == int(g["scal_energy_iter"])
     assert st.sor_iter_last == int(g["scal_SOR_ITER"])
     _check_fields(g, p, o)
+
+
+def _viscosity(o, g):
+    o.set_viscosity(int(g["vis_ivis"]), int(g["vis_ivistyp"]), int(g["vis_ieta"]), int(g["vis_idiss"]),
+                    float(g["vis_xnue"]))
+
+
+@pytest.mark.parametrize("name", ["forgas_visc_ieta4_12x16x12", "forgas_visc_ieta0_12x16x12"])
+def test_forgas3d_viscosity(name):
+    """FORGAS3D with ETACALC + VISC (IVISTYP=1, spherical) and dissipative
+    heating: eta, divtx/y/z, DISFN and the updated fields, bit for bit."""
+    def act(o, g):
+        _viscosity(o, g)
+        o.bounds()
+        o.L.tor_qdens()
+        o.forgas3d(float(g["delt"]))
+    g, p, o, _ = _run(name, act)
+    _check_fields(g, p, o)
+    if int(g["vis_ieta"]) == 0:
+        # the stresses move the velocities: without viscosity V differs (the
+        # oracle holds one state, so keep this run's V first)
+        v_visc = o.download("V")
+        o2 = Oracle(p, threads=1)
+        o2.load_state(C.inputs_of(g))
+        o2.bounds()
+        o2.L.tor_qdens()
+        o2.forgas3d(float(g["delt"]))
+        assert not np.array_equal(o2.download("V"), v_visc)

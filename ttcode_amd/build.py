@@ -27,7 +27,7 @@ FLAGS = [
     "-I", os.path.join(HERE, "..", "include"),
 ]
 
-SOURCES = ["tt_state.hip", "tt_sor.hip", "tt_phys.hip", "tt_energy.hip", "tt_hydro.hip", "tt_cnn.hip"]
+SOURCES = ["tt_state.hip", "tt_sor.hip", "tt_phys.hip", "tt_energy.hip", "tt_hydro.hip", "tt_cnn.hip", "tt_visc.hip"]
 
 
 def _compile(src: str) -> str:

@@ -45,6 +45,15 @@ struct Met {
     __device__ inline double operator()(int m, int i) const { return base[(long long)m * len + i + 2]; }
 };
 
+// div(v) of a cell in the reference's operation order (the p dV work,
+// ppfor.f:190-196, and VISC's compression term, ppvisc.f:36-39)
+__device__ inline double divv(const Geo& g, const Met& m, const double* V, const double* G, const double* H, int i,
+                              int j, int k) {
+    return (m(TT_M_SURXA, i + 1) * V[g.idx(i + 1, j, k)] - m(TT_M_SURXA, i) * V[g.idx(i, j, k)]) / m(TT_M_VOLXB, i) +
+           (m(TT_M_SURYA, j + 1) * G[g.idx(i, j + 1, k)] - m(TT_M_SURYA, j) * G[g.idx(i, j, k)]) / m(TT_M_VOLYB, j) +
+           (m(TT_M_SURZA, k + 1) * H[g.idx(i, j, k + 1)] - m(TT_M_SURZA, k) * H[g.idx(i, j, k)]) / m(TT_M_VOLZB, k);
+}
+
 // Device-side control block: reductions, convergence state, error flags.
 struct Ctrl {
     // SOR
@@ -99,6 +108,7 @@ struct Lib {
     // learned guess: consecutive rejections, and first solves since the
     // network last ran
     int learned_rejects = 0, learned_idle = 0;
+    tt_viscosity visc{};            // ETACALC/VISC in FORGAS3D (off: ivis 0)
     // guess 2 without the network: moving averages of the first solve's SOR
     // iterations from the cold [0] and the warm [1] start (0: none yet), and
     // the count of such solves (every 16th tries the start not preferred)
@@ -235,6 +245,8 @@ int diffusion_impl(double delt, double xnue, double pr, int* iters);
 int energy_impl(double delt, int iii, tt_energy_stats* st);
 int advect_impl(double delt);
 int forgas_impl(double delt);
+int visc_impl();
+void visc_free();
 int sync_check();
 
 }  // namespace tt

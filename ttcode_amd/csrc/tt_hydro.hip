@@ -550,17 +550,13 @@ __global__ void k_centcor(Geo g, Met m, double om, const double* __restrict__ G,
     CCZ[c] = -m(TT_M_XXB, i) * GQ * GQ * sin(za) * cos(za);
 }
 
-__device__ inline double divv(const Geo& g, const Met& m, const double* V, const double* G, const double* H, int i,
-                              int j, int k) {
-    return (m(TT_M_SURXA, i + 1) * V[g.idx(i + 1, j, k)] - m(TT_M_SURXA, i) * V[g.idx(i, j, k)]) / m(TT_M_VOLXB, i) +
-           (m(TT_M_SURYA, j + 1) * G[g.idx(i, j + 1, k)] - m(TT_M_SURYA, j) * G[g.idx(i, j, k)]) / m(TT_M_VOLYB, j) +
-           (m(TT_M_SURZA, k + 1) * H[g.idx(i, j, k + 1)] - m(TT_M_SURZA, k) * H[g.idx(i, j, k)]) / m(TT_M_VOLZB, k);
-}
-
-// p dV work: T -= fac*dt PG/RH/CV div(v) over cells lo..hi in each axis
+// p dV work: T -= fac*dt PG/RH/CV div(v) over cells lo..hi in each axis;
+// with DISFN (dissipative heating, IDISS) then T += fac*dt DISFN/CV/RH on the
+// interior, as the reference's following loop does cell by cell
+// (ppfor.f:202-210,335-343)
 __global__ void k_pdv(Geo g, Met m, double dtf, int lo, int hiadd, double* T, const double* __restrict__ PG,
                       const double* __restrict__ RH, const double* __restrict__ CV, const double* __restrict__ V,
-                      const double* __restrict__ G, const double* __restrict__ H) {
+                      const double* __restrict__ G, const double* __restrict__ H, const double* __restrict__ DISFN) {
     int ni = g.nx + hiadd - lo + 1, nj = g.ny + hiadd - lo + 1, nk = g.nz + hiadd - lo + 1;
     long long n = (long long)ni * nj * nk;
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -568,15 +564,20 @@ __global__ void k_pdv(Geo g, Met m, double dtf, int lo, int hiadd, double* T, co
     int i, j, k;
     sub_ijk(t, ni, nj, lo, lo, lo, i, j, k);
     long long c = IX(i, j, k);
-    T[c] = T[c] - dtf * PG[c] / RH[c] / CV[c] * divv(g, m, V, G, H, i, j, k);
+    double tn = T[c] - dtf * PG[c] / RH[c] / CV[c] * divv(g, m, V, G, H, i, j, k);
+    if (DISFN && i >= 1 && i <= g.nx && j >= 1 && j <= g.ny && k >= 1 && k <= g.nz)
+        tn = tn + dtf * DISFN[c] / CV[c] / RH[c];
+    T[c] = tn;
 }
 
-// VH, GH, HH (ppfor.f:217-278); DIVT* are zero with IVIS=0
+// VH, GH, HH (ppfor.f:217-278); DX/DY/DZ: divtx/y/z, null without viscosity
+// (the reference's zero arrays)
 __global__ void k_force_vh(Geo g, Met m, double dt, const double* __restrict__ V, const double* __restrict__ G,
                            const double* __restrict__ H, const double* __restrict__ PG, const double* __restrict__ QX,
                            const double* __restrict__ QY, const double* __restrict__ QZ, const double* __restrict__ CCX,
                            const double* __restrict__ CCY, const double* __restrict__ CCZ,
                            const double* __restrict__ GX, const double* __restrict__ GY, const double* __restrict__ GZ,
+                           const double* __restrict__ DX, const double* __restrict__ DY, const double* __restrict__ DZ,
                            double* VH, double* GH, double* HH) {
     long long n = (long long)g.nx * g.ny * g.nz;
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -585,9 +586,10 @@ __global__ void k_force_vh(Geo g, Met m, double dt, const double* __restrict__ V
     sub_ijk(t, g.nx, g.ny, 1, 1, 1, i, j, k);
     long long c = IX(i, j, k);
     double gx = m(TT_M_GEOXG, i), gz = m(TT_M_GEOZG, k), gh = m(TT_M_GEOXH, i);
-    VH[c] = V[c] + dt * ((-(PG[c] - PG[IX(i - 1, j, k)]) / m(TT_M_DXB, i) + 0.0) / QX[c] + CCX[c] + GX[c]);
-    GH[c] = G[c] + dt * (((-(PG[c] - PG[IX(i, j - 1, k)]) / m(TT_M_DYB, j) / gx / gz + 0.0) / QY[c] + CCY[c] + GY[c]) / gx / gz);
-    HH[c] = H[c] + dt * (((-(PG[c] - PG[IX(i, j, k - 1)]) / m(TT_M_DZB, k) / gh + 0.0) / QZ[c] + CCZ[c] + GZ[c]) / gh);
+    const double dx = DX ? DX[c] : 0.0, dy = DY ? DY[c] : 0.0, dz = DZ ? DZ[c] : 0.0;
+    VH[c] = V[c] + dt * ((-(PG[c] - PG[IX(i - 1, j, k)]) / m(TT_M_DXB, i) + dx) / QX[c] + CCX[c] + GX[c]);
+    GH[c] = G[c] + dt * (((-(PG[c] - PG[IX(i, j - 1, k)]) / m(TT_M_DYB, j) / gx / gz + dy) / QY[c] + CCY[c] + GY[c]) / gx / gz);
+    HH[c] = H[c] + dt * (((-(PG[c] - PG[IX(i, j, k - 1)]) / m(TT_M_DZB, k) / gh + dz) / QZ[c] + CCZ[c] + GZ[c]) / gh);
 }
 
 // mode 0: V = (V+VH)/2 etc.; mode 1: V = VH etc.
@@ -616,6 +618,9 @@ int forgas_impl(double dt) {
     int rc;
     if ((rc = bounds_impl())) return rc;
     if ((rc = pressg_impl())) return rc;
+    const bool vis = L.visc.ivis > 0;
+    if (vis && (rc = visc_impl())) return rc;  // ETACALC + VISC, ppfor.f:170-174
+    const double* disfn = vis && L.visc.idiss != 0 ? f[TT_DISFN] : nullptr;
     long long nc = (long long)g.nx * g.ny * g.nz;
     k_centcor<<<nblk(nc, 256), 256, 0, L.s>>>(g, met(), L.p.omrot, f[TT_G], f[TT_H], f[TT_CENTCORX], f[TT_CENTCORY],
                                               f[TT_CENTCORZ]);
@@ -624,14 +629,16 @@ int forgas_impl(double dt) {
     if (L.p.isoth == 0) {
         long long nh = (long long)(g.nx + 2) * (g.ny + 2) * (g.nz + 2);
         k_pdv<<<nblk(nh, 256), 256, 0, L.s>>>(g, met(), dt / 2.0, 0, 1, f[TT_T], f[TT_PG], f[TT_RH], f[TT_CV], f[TT_V],
-                                              f[TT_G], f[TT_H]);
+                                              f[TT_G], f[TT_H], disfn);
         TT_LAUNCH();
         if ((rc = bounds_impl())) return rc;
         if ((rc = pressg_impl())) return rc;
     }
     k_force_vh<<<nblk(nc, 256), 256, 0, L.s>>>(g, met(), dt, f[TT_V], f[TT_G], f[TT_H], f[TT_PG], f[TT_RHQX], f[TT_RHQY],
                                                f[TT_RHQZ], f[TT_CENTCORX], f[TT_CENTCORY], f[TT_CENTCORZ], f[TT_GRAVX],
-                                               f[TT_GRAVY], f[TT_GRAVZ], L.VH, L.GH, L.HH);
+                                               f[TT_GRAVY], f[TT_GRAVZ], vis ? f[TT_DIVTX] : nullptr,
+                                               vis ? f[TT_DIVTY] : nullptr, vis ? f[TT_DIVTZ] : nullptr, L.VH, L.GH,
+                                               L.HH);
     TT_LAUNCH();
     TT_HIP(hipMemcpyAsync(f[TT_T], L.TOLD, (size_t)g.n3 * sizeof(double), hipMemcpyDeviceToDevice, L.s));
     if (L.p.isoth == 0) {
@@ -639,7 +646,7 @@ int forgas_impl(double dt) {
         TT_LAUNCH();
         if ((rc = bounds_impl())) return rc;
         k_pdv<<<nblk(nc, 256), 256, 0, L.s>>>(g, met(), dt, 1, 0, f[TT_T], f[TT_PG], f[TT_RH], f[TT_CV], f[TT_V], f[TT_G],
-                                              f[TT_H]);
+                                              f[TT_H], disfn);
         TT_LAUNCH();
     }
     k_force_set<<<nblk(nc, 256), 256, 0, L.s>>>(g, 1, f[TT_V], f[TT_G], f[TT_H], L.VH, L.GH, L.HH);

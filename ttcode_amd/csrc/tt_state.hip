@@ -229,6 +229,7 @@ void tt_finalize(void) {
     if (L.s) hipStreamSynchronize(L.s);
     sor_free();
     cnn_free();
+    visc_free();
     free_all();
 }
 

@@ -28,6 +28,11 @@ module ttgpu_c
      integer(c_int32_t) :: first_start, pad0
   end type tt_energy_stats
 
+  type, bind(C) :: tt_viscosity
+     integer(c_int32_t) :: ivis, ivistyp, ieta, idiss
+     real(c_double) :: xnue
+  end type tt_viscosity
+
   type, bind(C) :: tt_sor_stats
      integer(c_int32_t) :: sor_iter, outer_iter, fp32_sweeps, guess_used
      real(c_double) :: rhsnorm, rnorm, r0_guess, r0_cold, guess_scale
@@ -72,6 +77,11 @@ module ttgpu_c
      integer(c_int) function tt_kappa() bind(C, name='tt_kappa')
        import :: c_int
      end function tt_kappa
+     ! IVIS, IVISTYP, IETA, IDISS and XNUE of input_init (ppmod.f:42-71)
+     integer(c_int) function tt_set_viscosity(v) bind(C, name='tt_set_viscosity')
+       import :: c_int, tt_viscosity
+       type(tt_viscosity), intent(in) :: v
+     end function tt_set_viscosity
      integer(c_int) function tt_advect3d(delt) bind(C, name='tt_advect3d')
        import :: c_int, c_double
        real(c_double), value :: delt

@@ -66,6 +66,13 @@ class SolverOpts(ctypes.Structure):
                 ("inner_tol", ctypes.c_double), ("warm_scale", ctypes.c_double)]
 
 
+class Viscosity(ctypes.Structure):
+    """``tt_viscosity``."""
+
+    _fields_ = [("ivis", ctypes.c_int32), ("ivistyp", ctypes.c_int32), ("ieta", ctypes.c_int32),
+                ("idiss", ctypes.c_int32), ("xnue", ctypes.c_double)]
+
+
 class Limits(ctypes.Structure):
     """``tt_limits``."""
 
@@ -116,6 +123,7 @@ EVAL7, EVAL8, EVAL_1_6, JACOB_ETA, FQ, FR = 56, 57, 58, 59, 60, 61
 FLIMX, FLIMY, FLIMZ, DIFRX, DIFRY, DIFRZ = 62, 63, 64, 65, 66, 67
 FLUX_X, FLUX_Y, FLUX_Z, CENTCORX, CENTCORY, CENTCORZ = 68, 69, 70, 71, 72, 73
 UPPERBND, MU_STAR, Q = 80, 81, 82
+ETA, DIVTX, DIVTY, DIVTZ, DISFN = 83, 84, 85, 86, 87
 
 FIELD_NAMES = {
     T: "T", RH: "RH", V: "V", G: "G", H: "H", ER: "ER", DELTA_ER: "delta_ER", GD: "GD",
@@ -131,6 +139,7 @@ FIELD_NAMES = {
     FLIMZ: "FLIMZ", DIFRX: "difrx", DIFRY: "difry", DIFRZ: "difrz", FLUX_X: "flux_x",
     FLUX_Y: "flux_y", FLUX_Z: "flux_z", CENTCORX: "centcorx", CENTCORY: "centcory",
     CENTCORZ: "centcorz", UPPERBND: "upperbnd", MU_STAR: "mu_star", Q: "Q",
+    ETA: "eta", DIVTX: "divtx", DIVTY: "divty", DIVTZ: "divtz", DISFN: "DISFN",
 }
 FIELD_IDS = {v: k for k, v in FIELD_NAMES.items()}
 
@@ -147,8 +156,11 @@ def field_bounds(f: int, nx: int, ny: int, nz: int):
         lo[1] = 0
     elif f in (H, RHQZ):
         lo[2] = 0
-    elif f == SIG:
+    elif f in (SIG, DISFN):
         lo, hi = [0, 0, 0], [nx + 1, ny + 1, nz + 1]
+    elif f in (DIVTX, DIVTY, DIVTZ):
+        lo, hi = [0, 0, 0], [nx + 1, ny + 1, nz + 1]
+        lo[f - DIVTX] = 1
     elif f == STELLARINPUT:
         lo[0], hi[0] = 1, nx
     elif f in (STELLARTAU, DTAUS):

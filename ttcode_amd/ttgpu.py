@@ -55,6 +55,8 @@ def load_library():
     L.tt_varlimit.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double]
     L.tt_sphericalvellimit.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double]
     L.tt_set_opacity_table.argtypes = [ctypes.c_void_p] * 3
+    L.tt_set_viscosity.argtypes = [ctypes.POINTER(P.Viscosity)]
+    L.tt_get_viscosity.argtypes = [ctypes.POINTER(P.Viscosity)]
     L.tt_get_sorparam.restype = ctypes.c_double
     L.tt_device_bytes.restype = ctypes.c_int64
     L.tt_sweep_timing_get.argtypes = [ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double),
@@ -179,6 +181,16 @@ class Device:
             raise ValueError("opacity tables must hold 42 x 18 values")
         self._check(self.L.tt_set_opacity_table(*(a.ctypes.data_as(ctypes.c_void_p) for a in t)),
                     "tt_set_opacity_table")
+
+    def set_viscosity(self, ivis=1, ivistyp=1, ieta=4, idiss=1, xnue=1e9):
+        """ETACALC + VISC inside FORGAS3D (ppfor.f:170-174); ivis 0 is off."""
+        self._check(self.L.tt_set_viscosity(ctypes.byref(P.Viscosity(ivis, ivistyp, ieta, idiss, xnue))),
+                    "tt_set_viscosity")
+
+    def get_viscosity(self):
+        v = P.Viscosity()
+        self._check(self.L.tt_get_viscosity(ctypes.byref(v)), "tt_get_viscosity")
+        return v
 
     def kappa(self):
         """KAPPA + SUBSIG (pprphy.f:1-360,567-583)."""
