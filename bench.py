@@ -230,6 +230,45 @@ def run_kappa_table(a):
     print(json.dumps(out), flush=True)
 
 
+def run_forgas_visc(a):
+    """FORGAS3D with viscosity at the headline grid: the shipped ipp's
+    IVIS=1, IVISTYP=1, IETA=4 (molecular, XNUE = 1e9) and IDISS=1, on the
+    generator's state with a 5% day-night contrast.  K timed calls of
+    tt_forgas3d after W untimed ones (each ends with a host sync); the same
+    without viscosity for the difference."""
+    from ttcode_amd import ttgpu
+    nx, ny, nz = a.nr, a.nphi, a.ntheta
+    p = P.make_params(nx, ny, nz, epsmat=1e-10)
+    st0 = P.generate_state(p, seed=a.seed, contrast=0.05)
+    dev = ttgpu.Device(p)
+    dev.load_state(st0)
+    dev.bounds()
+    dev.pressg()
+    dt = dev.schritt()
+    out_us = {}
+    for name, ivis in (("viscous", 1), ("inviscid", 0)):
+        dev.load_state(st0)
+        dev.set_viscosity(ivis, 1, 4, 1, 1e9)
+        dev.bounds()
+        dev.qdens()
+        for _ in range(a.warmup):
+            dev.forgas3d(dt)
+        dev.sync()
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            dev.forgas3d(dt)
+        dev.sync()
+        out_us[name] = (time.perf_counter() - t0) * 1e6 / a.steps
+    dev.close()
+    out = {"metric": "FORGAS3D with viscosity (ETACALC + VISC) time per call", "value": round(out_us["viscous"], 1),
+           "unit": "us", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "higher_is_better": False,
+           "dtype": "f64", "data": "synthetic (generator + 5% day-night contrast)",
+           "config": {"workload": f"forgas_visc sph{nx}x{nz}x{ny}: IVIS=1 IVISTYP=1 IETA=4 IDISS=1 XNUE=1e9"},
+           "inviscid_us": round(out_us["inviscid"], 1),
+           "viscosity_us": round(out_us["viscous"] - out_us["inviscid"], 1)}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -246,10 +285,11 @@ def main():
     ap.add_argument("--event-period", type=int, default=7,
                     help="bracket every N-th SOR half-sweep launch with HIP events (0: none, no roofline)")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
-    ap.add_argument("--config", choices=("c2", "large", "kappa_table"), default="c2",
+    ap.add_argument("--config", choices=("c2", "large", "kappa_table", "forgas_visc"), default="c2",
                     help="c2: the 128x128x256 headline workload; large: the 192x256x512 coupled run "
                          "(learned guess + FP32 SOR, 5%% day-night contrast); kappa_table: KAPPA with "
-                         "table opacities (IKAPTYP 4) at 128x128x256")
+                         "table opacities (IKAPTYP 4) at 128x128x256; forgas_visc: FORGAS3D with viscosity "
+                         "at 128x128x256")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy 1 GiB device to device (reference for rocprofv3 byte counters)")
     a = ap.parse_args()
@@ -273,6 +313,9 @@ def main():
         return
     if a.config == "kappa_table":
         run_kappa_table(a)
+        return
+    if a.config == "forgas_visc":
+        run_forgas_visc(a)
         return
 
     from ttcode_amd import ttgpu
