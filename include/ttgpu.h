@@ -36,7 +36,8 @@ extern "C" {
  * the switches the hot path reads, plus the power-law opacity that ttcode's
  * KAPPA dispatcher (pprphy.f:1-360) does not have.
  * Only NCOSYS=2 (spherical polar), MODTYP=2 (planet), IGASTYP=1 (ideal gas),
- * ICSTYP=0, IRAD=3 (SOR), IVIS=0, IARTVIS=0 and inc_poles=0 are supported.
+ * ICSTYP=0, IRAD=3 (SOR) and inc_poles=0 are supported; viscosity (IVIS,
+ * IARTVIS) is switched on by tt_set_viscosity.
  * ------------------------------------------------------------------------ */
 typedef struct tt_params {
     /* integers (keep the order: a Fortran bind(C) type mirrors it) */
@@ -294,6 +295,15 @@ int tt_sphericalvellimit(int ivel, double velmax, double hydrotime);
  * divergence divtx/y/z enters the momentum updates (ppfor.f:222-262) and
  * whose dissipation DISFN, with IDISS, the two temperature updates
  * (ppfor.f:202-210,335-343).  Off by default (IVIS = 0), as in the configs.
+ * With IVIS > 0 SCHRITT also limits the time step by the viscous condition
+ * (ppsub.f:97-119).
+ * IARTVIS = 1 adds the Neumann-Richtmyer artificial viscosity at the end of
+ * FORGAS3D (ppfor.f:374-395, NR_ARTIFICIAL_VISC ppfor.f:1032-1230), whose
+ * time-step bound dt_artvis SCHRITT takes (ppsub.f:127-128).  dt_artvis is
+ * DTMAX until the first such FORGAS3D (pp.f:436-440).  The compression
+ * fluxes QviscY/QviscZ stay on the device between calls, zero at first use:
+ * where upperbnd varies between neighbouring columns the reference reads
+ * entries the current call did not write, and so does the library.
  * ------------------------------------------------------------------------ */
 typedef struct tt_viscosity {
     int32_t ivis;            /* > 0: viscosity on                             */
@@ -302,10 +312,16 @@ typedef struct tt_viscosity {
                                 eta = RH min(CS mu_gas 1.67e-8 / RH, XNUE)    */
     int32_t idiss;           /* != 0: dissipative heating DISFN               */
     double xnue;             /* XNUE, kinematic viscosity, cm^2/s             */
+    int32_t iartvis;         /* 0 off; 1 NR_ARTIFICIAL_VISC                   */
+    int32_t pad0;
 } tt_viscosity;
 
 int tt_set_viscosity(const tt_viscosity* v);
 int tt_get_viscosity(tt_viscosity* v);
+/* dt_artvis (artvis_var_init, ppmod.f:388-395): min(1/|8 artvisc_coeff
+ * (m - TINY)|, DTMAX) with m the most negative compression rate dv/dx of the
+ * last NR_ARTIFICIAL_VISC (ppfor.f:1211-1212); DTMAX before the first. */
+int tt_get_dt_artvis(double* dt);
 
 typedef struct tt_step_stats {
     tt_energy_stats energy;

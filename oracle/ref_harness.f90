@@ -37,6 +37,8 @@ program tt_ref_harness
   use dif_var_alloc
   use global_var_init
   use global_var_alloc
+  use artvis_var_init
+  use artvis_var_alloc
   use global_constants, only: FD_c => FD, ARAD_c => ARAD
   implicit none
   include "mpif.h"
@@ -47,7 +49,7 @@ program tt_ref_harness
   double precision :: h_gamma, h_mu_gas, h_zkapmult, h_tirr, h_fbottom, h_d_t
   double precision :: h_omrot, h_sorparam, h_epsmat, h_fdelt, h_dtmax, h_delt
   double precision :: h_xnue, h_pr_num
-  integer :: h_prefill_kappa, h_num_iter, h_ivis, h_ivistyp, h_ieta, h_idiss
+  integer :: h_prefill_kappa, h_num_iter, h_ivis, h_ivistyp, h_ieta, h_idiss, h_iartvis
   double precision :: h_zeit, h_tinitbottom, h_tinittop, h_rhinittop, h_rhinitbottom
   double precision :: h_hydrotime, h_maccreated
   logical :: have_sin
@@ -59,7 +61,7 @@ program tt_ref_harness
        h_zkapmult, h_tirr, h_fbottom, h_d_t, h_omrot, h_sorparam, h_epsmat, &
        h_fdelt, h_dtmax, h_delt, h_xnue, h_pr_num, h_prefill_kappa, cmd, &
        h_zeit, h_num_iter, h_tinitbottom, h_tinittop, h_rhinittop, h_rhinitbottom, &
-       h_hydrotime, h_maccreated, h_ivis, h_ivistyp, h_ieta, h_idiss
+       h_hydrotime, h_maccreated, h_ivis, h_ivistyp, h_ieta, h_idiss, h_iartvis
   integer :: u, n, s
   integer :: adj_iter(50)
 
@@ -67,7 +69,7 @@ program tt_ref_harness
   h_nsteps = 1
   h_zeit = 0.d0; h_num_iter = 0; h_tinitbottom = 0.d0; h_tinittop = 0.d0
   h_rhinittop = 0.d0; h_rhinitbottom = 0.d0; h_hydrotime = 0.d0; h_maccreated = 0.d0
-  h_ivis = 0; h_ivistyp = 1; h_ieta = 4; h_idiss = 0
+  h_ivis = 0; h_ivistyp = 1; h_ieta = 4; h_idiss = 0; h_iartvis = 0
   h_prefill_kappa = 0
   h_xnue = 1.d9
   h_pr_num = 1.d0
@@ -87,7 +89,7 @@ program tt_ref_harness
   IRHO = 0; RHMIN = 1.d-11
   ITEMP = 0; TEMPMAX = 1.d4; TEMPMIN = 1.d1
   IVEL = 0; VELMAX = 6.d5; VELDAMP = 0.d0; DAMP_STEPS = 100000
-  IVISTYP = h_ivistyp; XNUE = h_xnue; IDISS = h_idiss; ALPHA = 1.d-4; IETA = h_ieta; IARTVIS = 0; PR_NUM = h_pr_num
+  IVISTYP = h_ivistyp; XNUE = h_xnue; IDISS = h_idiss; ALPHA = 1.d-4; IETA = h_ieta; IARTVIS = h_iartvis; PR_NUM = h_pr_num
   IGASTYP = 1; GAMMA = h_gamma; mu_gas = h_mu_gas; ICSTYP = 0; aspect = 0.04d0
   IKAPTYP = h_ikaptyp; ISIGTYP = h_isigtyp; ZKAPMULT = h_zkapmult; gamma_OPC = 0.1d0
   IROTATE = 0; PROT = 3.52d0; ramp_start = 0.d0; ramp_full = 0.d0; windAMP = 1.d0
@@ -115,6 +117,14 @@ program tt_ref_harness
   call allocatesorvar
   call allocatescalarvar
   call allocatedifvar
+  if (IARTVIS > 0) then
+     ! pp.f:145 allocates the fluxes undefined; zero them so that the entries
+     ! NR_ARTIFICIAL_VISC reads without writing (upperbnd varying between
+     ! columns) are defined; dt_artvis = DTMAX as pp.f:439 sets it
+     call allocateartvisvar
+     QviscX = 0.d0; QviscY = 0.d0; QviscZ = 0.d0
+     dt_artvis = DTMAX
+  end if
   ISCAL = 0
   call zero_modules()
 
@@ -287,6 +297,7 @@ program tt_ref_harness
   write(u, '(A,1X,ES25.17)') 'FD', FD_c
   write(u, '(A,1X,ES25.17)') 'ARAD', ARAD_c
   write(u, '(A,1X,ES25.17)') 'SORPARAM', SORPARAM
+  if (IARTVIS > 0) write(u, '(A,1X,ES25.17)') 'dt_artvis', dt_artvis
   if (trim(cmd) == 'restart') then
      write(u, '(A,1X,ES25.17)') 'ZEIT', ZEIT
      write(u, '(A,1X,ES25.17)') 'DELT', DELT

@@ -406,13 +406,28 @@ static int g_adjust; /* ADJUST_SOR_PARAM inside COUPLED_ENERGY_EQS (off by defau
 static tt_limits g_lim; /* clamps of the step loop (off by default) */
 static tt_viscosity g_visc; /* viscosity in FORGAS3D (off by default) */
 
+/* NR_ARTIFICIAL_VISC's module state (artvis_var_init, ppmod.f:388-395): the
+ * compression fluxes, allocated at first use and zeroed (the reference leaves
+ * them undefined; entries one call does not write are read by the next where
+ * upperbnd varies), and dt_artvis, DTMAX until the first call (pp.f:439) */
+static double *g_qvx, *g_qvy, *g_qvz;
+static double g_dt_artvis;
+
 int tor_set_viscosity(const tt_viscosity* v)
 {
     if (v->ivis > 0) {
         if (v->ivistyp != 0 && v->ivistyp != 1) return seterr(1, "VISC: IVISTYP must be 0 or 1");
         if (v->ieta != 0 && v->ieta != 4 && v->ieta != 5) return seterr(1, "ETACALC: IETA must be 0, 4 or 5");
     }
+    if (v->iartvis > 1) return seterr(1, "unknown IARTVIS"); /* ppfor.f:388-390 */
+    if (v->iartvis > 0 && g_visc.iartvis <= 0) g_dt_artvis = O.p.dtmax;
     g_visc = *v;
+    return 0;
+}
+
+int tor_get_dt_artvis(double* dt)
+{
+    *dt = g_visc.iartvis > 0 ? g_dt_artvis : O.p.dtmax;
     return 0;
 }
 
@@ -428,6 +443,8 @@ int tor_init(const tt_params* p)
     g_adjust = 0;
     memset(&g_lim, 0, sizeof g_lim);
     memset(&g_visc, 0, sizeof g_visc);
+    free(g_qvx); free(g_qvy); free(g_qvz);
+    g_qvx = g_qvy = g_qvz = NULL;
     int nt = O.nthreads ? O.nthreads : 1;
     memset(&O, 0, sizeof O);
     O.nthreads = nt;
@@ -2022,7 +2039,79 @@ static int visc(void)
     return 0;
 }
 
-int tor_forgas3d(double delt) /* FORGAS3D, ppfor.f:142-397 (IVEL=0, IARTVIS=0, IGRAV=1) */
+/* NR_ARTIFICIAL_VISC, ppfor.f:1032-1230, one rank without poles: the
+ * reference's three sweeps in its loop order, each cell's velocity and T
+ * updated in place as the sweep passes it */
+int tor_sphericalvellimit(int ivel, double velmax, double hydrotime);
+
+static void nr_artificial_visc(double delt)
+{
+    const double artvisc_coeff = 3.0, tiny = (double)1.0e-30f; /* ppmod.f:392,1092 */
+    if (!g_qvx) {
+        g_qvx = (double*)calloc(O.n3, sizeof(double));
+        g_qvy = (double*)calloc(O.n3, sizeof(double));
+        g_qvz = (double*)calloc(O.n3, sizeof(double));
+    }
+    double dt = 0.0, d;
+#define DMIN(x) do { double x_ = (x); if (x_ < dt) dt = x_; } while (0)
+    for (int k = 1; k <= NZ; ++k) /* X, :1066-1099 */
+        for (int j = 1; j <= NY; ++j) {
+            d = A(O.V, 1, j, k) - A(O.V, 0, j, k);
+            if (d > 0.0) d = 0.0;
+            A(g_qvx, 0, j, k) = artvisc_coeff * delt * A(O.RH, 0, j, k) * (d * d);
+            DMIN(d / DXB(0));
+            const int ub = UBND(j, k);
+            for (int i = 1; i <= ub - 2; ++i) {
+                d = A(O.V, i + 1, j, k) - A(O.V, i, j, k);
+                if (d > 0.0) d = 0.0;
+                A(g_qvx, i, j, k) = artvisc_coeff * delt * A(O.RH, i, j, k) * (d * d);
+                A(O.V, i, j, k) = A(O.V, i, j, k) - (A(g_qvx, i, j, k) - A(g_qvx, i - 1, j, k)) / (DXB(i) * A(O.RHQX, i, j, k));
+                A(O.T, i, j, k) = A(O.T, i, j, k) - A(g_qvx, i, j, k) * d / (DXA(i) * A(O.RH, i, j, k) * A(O.CV, i, j, k));
+                DMIN(d / DXB(i));
+            }
+        }
+    for (int k = 1; k <= NZ; ++k) /* Y, :1100-1142 */
+        for (int j = 1; j <= NY; ++j) {
+            const int ub = UBND(j, k);
+            for (int i = 1; i <= ub - 2; ++i) {
+                if (j == 1) {
+                    d = A(O.G, i, 1, k) - A(O.G, i, 0, k);
+                    if (d > 0.0) d = 0.0;
+                    A(g_qvy, i, 0, k) = artvisc_coeff * delt * A(O.RH, i, 0, k) * (d * d);
+                    DMIN(d / DYB(0));
+                }
+                d = A(O.G, i, j + 1, k) - A(O.G, i, j, k);
+                if (d > 0.0) d = 0.0;
+                A(g_qvy, i, j, k) = artvisc_coeff * delt * A(O.RH, i, j, k) * (d * d);
+                A(O.G, i, j, k) = A(O.G, i, j, k) - (A(g_qvy, i, j, k) - A(g_qvy, i, j - 1, k)) / (DYB(j) * A(O.RHQY, i, j, k));
+                A(O.T, i, j, k) = A(O.T, i, j, k) - A(g_qvy, i, j, k) * d / (DYA(j) * A(O.RH, i, j, k) * A(O.CV, i, j, k));
+                DMIN(d / DYB(j));
+            }
+        }
+    for (int k = 1; k <= NZ; ++k) /* Z, :1154-1197 */
+        for (int j = 1; j <= NY; ++j) {
+            const int ub = UBND(j, k);
+            for (int i = 1; i <= ub - 2; ++i) {
+                if (k == 1) {
+                    d = A(O.H, i, j, 1) - A(O.H, i, j, 0);
+                    if (d > 0.0) d = 0.0;
+                    A(g_qvz, i, j, 0) = artvisc_coeff * delt * A(O.RH, i, j, 0) * (d * d);
+                    DMIN(d / DZB(0));
+                }
+                d = A(O.H, i, j, k + 1) - A(O.H, i, j, k);
+                if (d > 0.0) d = 0.0;
+                A(g_qvz, i, j, k) = artvisc_coeff * delt * A(O.RH, i, j, k) * (d * d);
+                A(O.H, i, j, k) = A(O.H, i, j, k) - (A(g_qvz, i, j, k) - A(g_qvz, i, j, k - 1)) / (DZB(k) * A(O.RHQZ, i, j, k));
+                A(O.T, i, j, k) = A(O.T, i, j, k) - A(g_qvz, i, j, k) * d / (DZA(k) * A(O.RH, i, j, k) * A(O.CV, i, j, k));
+                DMIN(d / DZB(k));
+            }
+        }
+#undef DMIN
+    dt = 1.0 / fabs(8.0 * artvisc_coeff * (dt - tiny)); /* :1211-1212 */
+    g_dt_artvis = dt < O.p.dtmax ? dt : O.p.dtmax;
+}
+
+int tor_forgas3d(double delt) /* FORGAS3D, ppfor.f:142-397 (IGRAV=1) */
 {
     int rc = tor_bounds();
     if (rc) return rc;
@@ -2112,6 +2201,13 @@ int tor_forgas3d(double delt) /* FORGAS3D, ppfor.f:142-397 (IVEL=0, IARTVIS=0, I
                 A(O.G, i, j, k) = A(GH, i, j, k);
                 A(O.H, i, j, k) = A(HH, i, j, k);
             }
+    /* the velocity ceiling, ppfor.f:361-371 (SPHERICALVELLIMIT for NCOSYS=2) */
+    if (g_lim.ivel != 0 && (rc = tor_sphericalvellimit(g_lim.ivel, g_lim.velmax, g_lim.hydrotime))) goto out;
+    if (g_visc.iartvis > 0) { /* ppfor.f:374-395 */
+        rc = tor_bounds();
+        if (rc) goto out;
+        nr_artificial_visc(delt);
+    }
     rc = tor_bounds();
 out:
     free(TOLD); free(VH); free(GH); free(HH);
@@ -2119,7 +2215,7 @@ out:
 }
 
 /* ---------------------------------------------------------------- ppsub.f */
-int tor_schritt(double* delt) /* SCHRITT, ppsub.f:1-137 (IGASTYP=1, IVIS=0, IARTVIS=0) */
+int tor_schritt(double* delt) /* SCHRITT, ppsub.f:1-137 (IGASTYP=1) */
 {
     double mrh = O.RH[0], mt = O.T[0];
     for (size_t n = 0; n < O.n3; ++n) {
@@ -2150,6 +2246,17 @@ int tor_schritt(double* delt) /* SCHRITT, ppsub.f:1-137 (IGASTYP=1, IVIS=0, IART
                 ldt = m;
                 if (ldt <= 0.0) return seterr(2, "local dt negative");
             }
+    if (g_visc.ivis > 0) /* the viscous condition, ppsub.f:97-119 */
+        for (int k = 1; k <= NZ; ++k)
+            for (int j = 1; j <= NY; ++j)
+                for (int i = 1; i <= NX; ++i) {
+                    const double xnu = A(O.ETA, i, j, k) / A(O.RH, i, j, k);
+                    const double zcf = (1.0 / 8.0) / xnu /
+                                       (pow(DXA(i + 1), -2.0) + pow(DYA(j + 1) * GEOXG(i) * GEOZG(k), -2.0) +
+                                        pow(DZA(k + 1) * GEOXH(i), -2.0));
+                    if (zcf < ldt) ldt = zcf;
+                }
+    if (g_visc.iartvis > 0 && g_dt_artvis < ldt) ldt = g_dt_artvis; /* ppsub.f:127-128 */
     ldt = (ldt < O.p.dtmax ? ldt : O.p.dtmax) * O.p.fdelt;
     *delt = ldt;
     return 0;

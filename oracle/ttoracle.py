@@ -52,6 +52,7 @@ class Oracle:
             L.tor_set_opacity_table.argtypes = [ctypes.c_void_p] * 3
             L.tor_set_viscosity.argtypes = [ctypes.POINTER(P.Viscosity)]
             L.tor_get_viscosity.argtypes = [ctypes.POINTER(P.Viscosity)]
+            L.tor_get_dt_artvis.argtypes = [ctypes.POINTER(ctypes.c_double)]
         self.L = Oracle._lib
         self.p = p
         self.L.tor_set_threads(int(threads))
@@ -106,10 +107,16 @@ class Oracle:
         self._check(self.L.tor_set_opacity_table(*(a.ctypes.data_as(ctypes.c_void_p) for a in t)),
                     "set_opacity_table")
 
-    def set_viscosity(self, ivis=1, ivistyp=1, ieta=4, idiss=1, xnue=1e9):
-        """ETACALC + VISC inside FORGAS3D (ppfor.f:170-174), off by default."""
-        self._check(self.L.tor_set_viscosity(ctypes.byref(P.Viscosity(ivis, ivistyp, ieta, idiss, xnue))),
-                    "set_viscosity")
+    def set_viscosity(self, ivis=1, ivistyp=1, ieta=4, idiss=1, xnue=1e9, iartvis=0):
+        """ETACALC + VISC inside FORGAS3D (ppfor.f:170-174), off by default;
+        iartvis 1 adds NR_ARTIFICIAL_VISC (ppfor.f:374-395)."""
+        v = P.Viscosity(ivis, ivistyp, ieta, idiss, xnue, iartvis)
+        self._check(self.L.tor_set_viscosity(ctypes.byref(v)), "set_viscosity")
+
+    def dt_artvis(self):
+        d = ctypes.c_double()
+        self._check(self.L.tor_get_dt_artvis(ctypes.byref(d)), "dt_artvis")
+        return d.value
 
     def kappa(self):
         """KAPPA + SUBSIG (pprphy.f:1-360,567-583) on the current state."""

@@ -368,6 +368,42 @@ def viscosity():
              {"delt": dt, "vis_ivis": 1, "vis_ivistyp": 1, "vis_ieta": ieta, "vis_idiss": 1, "vis_xnue": xnue})
 
 
+def artvis():
+    """FORGAS3D with NR_ARTIFICIAL_VISC (IARTVIS=1, ppfor.f:374-395 and
+    1032-1230): alone, on a state whose upperbnd varies between columns
+    (9, 10 or higher: density cut by 500 above it), so that the Y and Z sweeps
+    read fluxes of lower neighbour columns the call did not write; and with
+    the shipped ipp's viscosity (IVIS=1, IETA=4, IDISS=1) on the uniform one.
+    dt_artvis is a scalar output."""
+    nx, ny, nz = 12, 16, 12
+    p = P.make_params(nx, ny, nz, ikaptyp=1, kap0=1e-2)
+    o = Oracle(p)
+    o.load_state(base_inputs(p))
+    o.bounds()
+    o.pressg()
+    dt = o.schritt()
+    keep = ["T", "RH", "V", "G", "H", "PG", "upperbnd"]
+    inp = base_inputs(p, seed=47, contrast=0.05)
+    rh = inp["RH"].copy()
+    for kk in range(nz + 4):
+        for jj in range(ny + 4):
+            r = (jj + kk) % 3
+            if r < 2:
+                rh[kk, jj, 1 + 9 + r:] *= 0.002  # first cell below 1e-9: i = 9 or 10
+    inp["RH"] = rh
+    av = dict(h_iartvis=1)
+    outs, sc = run_ref("forgas", p, inp, delt=dt, extra=av)
+    ub = outs["upperbnd"][2:-2, 2:-2]
+    assert {9, 10} < set(np.unique(ub).tolist()), np.unique(ub)
+    save("forgas_artvis_12x16x12", p, inp, outs, sc, keep, {"delt": dt, "vis_iartvis": 1})
+    inp = base_inputs(p, seed=41, contrast=0.05)
+    vis = dict(h_ivis=1, h_ivistyp=1, h_ieta=4, h_idiss=1, h_iartvis=1)
+    outs, sc = run_ref("forgas", p, inp, delt=dt, xnue=1e9, extra=vis)
+    save("forgas_visc_artvis_12x16x12", p, inp, outs, sc, keep + ["eta", "DISFN"],
+         {"delt": dt, "vis_ivis": 1, "vis_ivistyp": 1, "vis_ieta": 4, "vis_idiss": 1, "vis_xnue": 1e9,
+          "vis_iartvis": 1})
+
+
 CKPT_SCALARS = dict(h_zeit=8.64e5 + 0.375, h_num_iter=1234, h_tinitbottom=1512.5, h_tinittop=1488.25,
                     h_rhinittop=2.5e-9, h_rhinitbottom=1.25e-4, h_hydrotime=3.5e5, h_maccreated=0.0)
 
@@ -420,6 +456,8 @@ if __name__ == "__main__":
         checkpoints()
     elif sys.argv[1:] == ["viscosity"]:
         viscosity()
+    elif sys.argv[1:] == ["artvis"]:
+        artvis()
     elif sys.argv[1:] == ["variants"]:
         variants()
     elif sys.argv[1:] == ["adjust"]:

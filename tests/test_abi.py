@@ -45,8 +45,9 @@ def test_struct_sizes_match_header():
     probe = r'''
 #include <stdio.h>
 #include "ttgpu.h"
-int main(void){printf("%zu %zu %zu %zu %zu %zu\n", sizeof(tt_params), sizeof(tt_solver_opts),
- sizeof(tt_sor_stats), sizeof(tt_energy_stats), sizeof(tt_step_stats), sizeof(tt_limits)); return 0;}
+int main(void){printf("%zu %zu %zu %zu %zu %zu %zu\n", sizeof(tt_params), sizeof(tt_solver_opts),
+ sizeof(tt_sor_stats), sizeof(tt_energy_stats), sizeof(tt_step_stats), sizeof(tt_limits),
+ sizeof(tt_viscosity)); return 0;}
 '''
     import tempfile
     with tempfile.TemporaryDirectory() as d:
@@ -56,7 +57,8 @@ int main(void){printf("%zu %zu %zu %zu %zu %zu\n", sizeof(tt_params), sizeof(tt_
         subprocess.check_call(["gcc", "-I", os.path.join(ROOT, "include"), c, "-o", exe])
         sizes = list(map(int, subprocess.check_output([exe]).split()))
     assert sizes == [ctypes.sizeof(P.TTParams), ctypes.sizeof(P.SolverOpts), ctypes.sizeof(P.SorStats),
-                     ctypes.sizeof(P.EnergyStats), ctypes.sizeof(P.StepStats), ctypes.sizeof(P.Limits)]
+                     ctypes.sizeof(P.EnergyStats), ctypes.sizeof(P.StepStats), ctypes.sizeof(P.Limits),
+                     ctypes.sizeof(P.Viscosity)]
 
 
 @pytest.mark.parametrize("f", sorted(P.FIELD_NAMES))

@@ -295,6 +295,104 @@ def test_forgas3d_viscosity(ttgpu, name):
         d.close()
 
 
+@pytest.mark.parametrize("name", ["forgas_artvis_12x16x12", "forgas_visc_artvis_12x16x12"])
+def test_forgas3d_artificial_viscosity(ttgpu, name):
+    """FORGAS3D with NR_ARTIFICIAL_VISC against the reference: alone, with
+    upperbnd varying between columns, and with the shipped ipp's viscosity;
+    dt_artvis exactly (a minimum of quotients), then SCHRITT takes it."""
+    g = C.load(name)
+    p, d = _dev(ttgpu, g)
+    try:
+        get = lambda k, v: g[k].item() if k in g.files else v  # noqa: E731
+        d.set_viscosity(int(get("vis_ivis", 0)), int(get("vis_ivistyp", 1)), int(get("vis_ieta", 4)),
+                        int(get("vis_idiss", 0)), float(get("vis_xnue", 1e9)), 1)
+        assert d.dt_artvis() == p.dtmax
+        d.bounds()
+        d.qdens()
+        d.forgas3d(float(g["delt"]))
+        _cmp(g, p, d, rel=1e-12)
+        dav = d.dt_artvis()
+        assert dav == g["scal_dt_artvis"].item()
+        d.sound()
+        d_all = d.schritt()
+        v = d.get_viscosity()
+        d.set_viscosity(v.ivis, v.ivistyp, v.ieta, v.idiss, v.xnue, 0)
+        d_rest = d.schritt()
+        assert abs(d_all - min(d_rest / p.fdelt, dav) * p.fdelt) <= 1e-15 * d_all
+    finally:
+        d.close()
+
+
+def test_schritt_viscous_limit_vs_oracle(ttgpu):
+    """SCHRITT with IVIS = 1 (the viscous condition, ppsub.f:97-119) and
+    IARTVIS = 1 after one viscous FORGAS3D, against the oracle; pow(x, -2)
+    from the device math library, so within 1e-14."""
+    from oracle.ttoracle import Oracle
+    g = C.load("forgas_visc_ieta0_12x16x12")
+    p = C.params_of(g)
+    dt = float(g["delt"])
+    vis = (1, 1, 0, 1, float(g["vis_xnue"]), 1)
+    o = Oracle(p, threads=1)
+    o.load_state(C.inputs_of(g))
+    o.set_viscosity(*vis)
+    o.bounds()
+    o.L.tor_qdens()
+    o.forgas3d(dt)
+    want = o.schritt()
+    d = ttgpu.Device(p)
+    try:
+        d.load_state(C.inputs_of(g))
+        d.set_viscosity(*vis)
+        d.bounds()
+        d.qdens()
+        d.forgas3d(dt)
+        assert d.dt_artvis() == o.dt_artvis()
+        got = d.schritt()
+        print(f"  delt {got:.17e} vs {want:.17e}")
+        assert abs(got - want) <= 1e-14 * want
+    finally:
+        d.close()
+
+
+def test_step_artificial_viscosity_32x64x32_vs_oracle(ttgpu):
+    """Three steps with the shipped ipp's viscosity and artificial viscosity
+    (IVIS=1, IETA=4, IDISS=1, IARTVIS=1), SCHRITT between them taking the
+    viscous and artificial bounds, at 32x64x32 against the oracle."""
+    from oracle.ttoracle import Oracle
+    p = P.make_params(32, 64, 32, epsmat=1e-8)
+    st = P.generate_state(p, seed=53, contrast=0.05)
+    vis = (1, 1, 4, 1, 1e9, 1)
+    o = Oracle(p, threads=1)
+    d = ttgpu.Device(p)
+    try:
+        o.load_state(st)
+        o.set_viscosity(*vis)
+        d.load_state(st)
+        d.set_viscosity(*vis)
+        same = True
+        for it in range(3):
+            o.bounds()
+            o.pressg()
+            d.bounds()
+            d.pressg()
+            dto, dtg = o.schritt(), d.schritt()
+            assert abs(dtg - dto) <= 1e-13 * dto, (it, dtg, dto)
+            so, _ = o.step(dto, it + 1)
+            sg, _ = d.step(dto, it + 1)
+            assert sg.energy.energy_iter == so.energy.energy_iter
+            same = same and sg.energy.sor_iter_total == so.energy.sor_iter_total
+            assert abs(d.dt_artvis() - o.dt_artvis()) <= 1e-12 * o.dt_artvis()
+        # bit-near while the SOR counts agree; else within the solver tolerance
+        tol = 1e-11 if same else 1e-6
+        for k in ("RH", "V", "G", "H", "T", "ER"):
+            reg = C.compare_region(k, p.nx, p.ny, p.nz)
+            e = C.rel_err(d.download(k)[reg], o.download(k)[reg])
+            print(f"  {k}: {e:.2e}")
+            assert e <= tol, (k, e)
+    finally:
+        d.close()
+
+
 def test_step_viscosity_32x64x32_vs_oracle(ttgpu):
     """One full step with viscosity (IETA=0, strong enough to move the
     flow) at 32x64x32 against the oracle."""

@@ -100,7 +100,8 @@ def test_ramp_closed_form(o, t, frac):
 
 def test_step_applies_limits_in_loop_order():
     """tt_step's clamps sit where pp.f's loop has them: V and T before
-    ADVECT3D, RH and T after it, T after FORGAS3D; hydrotime advances by dt."""
+    ADVECT3D, RH and T after it, the velocity ceiling inside FORGAS3D
+    (ppfor.f:361-371), T after it; hydrotime advances by dt."""
     p = P.make_params(12, 16, 12, ikaptyp=1, kap0=1e-1, epsmat=1e-8, tirr=2000.0)
     st = P.generate_state(p, seed=17)
     o = Oracle(p)
@@ -128,6 +129,7 @@ def test_step_applies_limits_in_loop_order():
     b.advect3d(dt)
     b.varlimit("RH", lim["rhmin"], 1e10)
     b.varlimit("T", lim["tempmin"], lim["tempmax"])
+    b.set_limits(ivel=1, velmax=lim["velmax"], hydrotime=3.0 + dt)  # FORGAS3D's own SPHERICALVELLIMIT
     b.forgas3d(dt)
     b.varlimit("T", lim["tempmin"], lim["tempmax"])
     b.coupled_energy(dt, 2)

@@ -225,8 +225,10 @@ def test_coupled_energy_freedman_table():
 
 
 def _viscosity(o, g):
-    o.set_viscosity(int(g["vis_ivis"]), int(g["vis_ivistyp"]), int(g["vis_ieta"]), int(g["vis_idiss"]),
-                    float(g["vis_xnue"]))
+    def get(k, d):
+        return g[k].item() if k in g.files else d
+    o.set_viscosity(int(get("vis_ivis", 0)), int(get("vis_ivistyp", 1)), int(get("vis_ieta", 4)),
+                    int(get("vis_idiss", 0)), float(get("vis_xnue", 1e9)), int(get("vis_iartvis", 0)))
 
 
 @pytest.mark.parametrize("name", ["forgas_visc_ieta4_12x16x12", "forgas_visc_ieta0_12x16x12"])
@@ -250,3 +252,85 @@ def test_forgas3d_viscosity(name):
         o2.L.tor_qdens()
         o2.forgas3d(float(g["delt"]))
         assert not np.array_equal(o2.download("V"), v_visc)
+
+
+@pytest.mark.parametrize("name", ["forgas_artvis_12x16x12", "forgas_visc_artvis_12x16x12"])
+def test_forgas3d_artificial_viscosity(name):
+    """FORGAS3D with NR_ARTIFICIAL_VISC (ppfor.f:1032-1230), bit for bit,
+    and dt_artvis exactly: alone on a state whose upperbnd varies between
+    columns (the Y and Z sweeps read fluxes of lower neighbour columns that
+    the call does not write), and with the shipped ipp's viscosity."""
+    def act(o, g):
+        _viscosity(o, g)
+        o.bounds()
+        o.L.tor_qdens()
+        o.forgas3d(float(g["delt"]))
+    g, p, o, _ = _run(name, act)
+    _check_fields(g, p, o)
+    assert o.dt_artvis() == g["scal_dt_artvis"].item()
+    ub = o.download("upperbnd")[2:-1, 2:-1]
+    if len(np.unique(ub)) > 1:
+        # without the artificial viscosity the velocities differ
+        v_av = o.download("V")
+        o2 = Oracle(p, threads=1)
+        o2.load_state(C.inputs_of(g))
+        o2.bounds()
+        o2.L.tor_qdens()
+        o2.forgas3d(float(g["delt"]))
+        assert not np.array_equal(o2.download("V"), v_av)
+
+
+def _metric_ids():
+    """Metric ids of include/ttgpu.h's enum, in order."""
+    import os
+    import re
+    txt = open(os.path.join(os.path.dirname(__file__), "..", "include", "ttgpu.h")).read()
+    body = txt[txt.index("TT_M_XXA = 0"):txt.index("TT_M_COUNT")]
+    return {n: i for i, n in enumerate(re.findall(r"TT_M_(\w+)", body))}
+
+
+def test_schritt_viscous_and_artificial_limits():
+    """SCHRITT's viscous condition (ppsub.f:97-119) and the dt_artvis bound
+    (:127-128); ppsub.f does not build here, so the oracle is pinned by the
+    closed forms: delt = FDELT min(CFL bound, min over cells of
+    (1/8) / (eta/rho) / (dxa^-2 + (dya r cos)^-2 + (dza r)^-2), DTMAX), and
+    with IARTVIS, FDELT min(that, dt_artvis)."""
+    g = C.load("forgas_visc_ieta0_12x16x12")
+    p = C.params_of(g)
+    dt = float(g["delt"])
+    o = Oracle(p, threads=1)
+    o.load_state(C.inputs_of(g))
+    o.set_viscosity(1, 1, 0, 1, float(g["vis_xnue"]), 1)
+    assert o.dt_artvis() == p.dtmax  # before the first FORGAS3D (pp.f:439)
+    o.bounds()
+    o.L.tor_qdens()
+    o.forgas3d(dt)  # ETA, and dt_artvis
+    dav = o.dt_artvis()
+    assert 0 < dav < p.dtmax
+    o.set_viscosity(0, 1, 0, 1, float(g["vis_xnue"]), 0)
+    d_cfl = o.schritt()
+    o.set_viscosity(1, 1, 0, 1, float(g["vis_xnue"]), 0)
+    d_vis = o.schritt()
+    o.set_viscosity(1, 1, 0, 1, float(g["vis_xnue"]), 1)
+    d_all = o.schritt()
+    ids = _metric_ids()
+    nx, ny, nz = p.nx, p.ny, p.nz
+
+    def met(name, lo_i, hi_i):
+        a = o.metric(ids[name])
+        lo = {"DXA": 1, "DYA": 1, "DZA": 1}.get(name, -1)
+        return a[lo_i - lo:hi_i - lo + 1]
+    dxa = met("DXA", 2, nx + 1)[None, None, :]
+    dya = met("DYA", 2, ny + 1)[None, :, None]
+    dza = met("DZA", 2, nz + 1)[:, None, None]
+    gxg = met("GEOXG", 1, nx)[None, None, :]
+    gxh = met("GEOXH", 1, nx)[None, None, :]
+    gzg = met("GEOZG", 1, nz)[:, None, None]
+    eta = o.download("eta")[2:-2, 2:-2, 2:-2]
+    rh = o.download("RH")[2:-2, 2:-2, 2:-2]
+    s = dxa ** -2.0 + (dya * gxg * gzg) ** -2.0 + (dza * gxh) ** -2.0
+    zcf = (0.125 / (eta / rh) / s).min()
+    want = min(d_cfl / p.fdelt, zcf, p.dtmax) * p.fdelt
+    assert zcf * p.fdelt < d_cfl  # the viscous bound is the one that acts
+    np.testing.assert_allclose(d_vis, want, rtol=1e-14)
+    np.testing.assert_allclose(d_all, min(d_vis / p.fdelt, dav) * p.fdelt, rtol=1e-15)

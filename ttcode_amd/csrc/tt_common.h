@@ -72,6 +72,7 @@ struct Ctrl {
     int spec_hit;      // a deferred (speculative) test ended the solve
     double dtmin;
     double scratch[8];
+    double av_min[3];  // NR_ARTIFICIAL_VISC: min dv/dx of the x, y and z passes
     // stagnation watch of inner (FP32) solves
     double best;       // best L1 ratio seen
     int since;         // iterations since it last improved by 1%
@@ -109,6 +110,7 @@ struct Lib {
     // network last ran
     int learned_rejects = 0, learned_idle = 0;
     tt_viscosity visc{};            // ETACALC/VISC in FORGAS3D (off: ivis 0)
+    int av_valid = 0;               // ctrl->av_min holds a NR_ARTIFICIAL_VISC result
     // guess 2 without the network: moving averages of the first solve's SOR
     // iterations from the cold [0] and the warm [1] start (0: none yet), and
     // the count of such solves (every 16th tries the start not preferred)
@@ -247,6 +249,9 @@ int advect_impl(double delt);
 int forgas_impl(double delt);
 int visc_impl();
 void visc_free();
+int artvis_impl(double dt);       // NR_ARTIFICIAL_VISC, ppfor.f:1032-1230
+double dt_artvis();               // dt_artvis after the last artvis_impl (host; syncs)
+int sphericalvellimit_impl(int ivel, double velmax, double hydrotime);
 int sync_check();
 
 }  // namespace tt

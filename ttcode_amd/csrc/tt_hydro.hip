@@ -651,6 +651,12 @@ int forgas_impl(double dt) {
     }
     k_force_set<<<nblk(nc, 256), 256, 0, L.s>>>(g, 1, f[TT_V], f[TT_G], f[TT_H], L.VH, L.GH, L.HH);
     TT_LAUNCH();
+    // the velocity ceiling (ppfor.f:361-371: SPHERICALVELLIMIT for NCOSYS=2)
+    if (L.lim.ivel != 0 && (rc = sphericalvellimit_impl(L.lim.ivel, L.lim.velmax, L.lim.hydrotime))) return rc;
+    if (L.visc.iartvis > 0) {  // ppfor.f:374-395
+        if ((rc = bounds_impl())) return rc;
+        if ((rc = artvis_impl(dt))) return rc;
+    }
     return bounds_impl();
 }
 

@@ -268,7 +268,8 @@ __global__ __launch_bounds__(256) void k_schritt(Geo g, Met m, const double* __r
                                                   const double* __restrict__ T, const double* __restrict__ PG,
                                                   const double* __restrict__ V, const double* __restrict__ G,
                                                   const double* __restrict__ H, const double* __restrict__ CS,
-                                                  double dtmax, double* part, int nb, Ctrl* ctrl) {
+                                                  const double* __restrict__ ETA, double dtmax, double* part, int nb,
+                                                  Ctrl* ctrl) {
     __shared__ double sh[3][4];
     double mdt = dtmax, mrh = 1e308, mt = 1e308;
     int badint = 0;
@@ -286,6 +287,14 @@ __global__ __launch_bounds__(256) void k_schritt(Geo g, Met m, const double* __r
             double z2 = m(TT_M_DYA, j + 1) / (v2 + cs / m(TT_M_GEOXG, i) / m(TT_M_GEOZG, k));
             double z3 = m(TT_M_DZA, k + 1) / (v3 + cs / m(TT_M_GEOXH, i));
             mdt = fmin(mdt, fmin(z1, fmin(z2, z3)));
+            if (ETA) {  // the viscous condition, IVIS > 0 (ppsub.f:97-119)
+                const double xnu = ETA[t] / RH[t];
+                const double zcf = (1.0 / 8.0) / xnu /
+                                   (pow(m(TT_M_DXA, i + 1), -2.0) +
+                                    pow(m(TT_M_DYA, j + 1) * m(TT_M_GEOXG, i) * m(TT_M_GEOZG, k), -2.0) +
+                                    pow(m(TT_M_DZA, k + 1) * m(TT_M_GEOXH, i), -2.0));
+                mdt = fmin(mdt, zcf);
+            }
         }
     }
     if (badint) atomicOr(&ctrl->bad_state, 2);
@@ -446,7 +455,7 @@ static int varlimit_impl(int f, double vmin, double vmax) {
 
 // SPHERICALVELLIMIT's ceiling (ppsub.f:338-363): DAY = 8.64E4 is a REAL
 // constant (ppmod.f:1083, exact in float); TIME_START = VELDAMP_START = 0
-static int sphericalvellimit_impl(int ivel, double velmax, double hydrotime) {
+int sphericalvellimit_impl(int ivel, double velmax, double hydrotime) {
     double vmin, vmax;
     if (ivel == 1 || ivel == 3) {
         vmin = -velmax;
@@ -569,7 +578,8 @@ int tt_schritt(double* delt) {
     int nb = 1024;
     if (3 * nb > 2 * L.npartial) nb = 2 * L.npartial / 3;
     k_schritt<<<nb, 256, 0, L.s>>>(g, met(), L.f[TT_RH], L.f[TT_T], L.f[TT_PG], L.f[TT_V], L.f[TT_G], L.f[TT_H],
-                                   L.f[TT_CS], L.p.dtmax, L.partial, nb, L.ctrl);
+                                   L.f[TT_CS], L.visc.ivis > 0 ? L.f[TT_ETA] : nullptr, L.p.dtmax, L.partial, nb,
+                                   L.ctrl);
     TT_LAUNCH();
     rc = fold_partials(L.partial, nb, 1, 2, &L.ctrl->dtmin);
     if (rc) return rc;
@@ -592,6 +602,10 @@ int tt_schritt(double* delt) {
     }
     double ldt = L.hctrl->dtmin;
     if (ldt <= 0.0) return fail(2, "local dt negative");
+    if (L.visc.iartvis > 0) {  // ppsub.f:127-128
+        const double da = dt_artvis();
+        ldt = da < ldt ? da : ldt;
+    }
     ldt = (ldt < L.p.dtmax ? ldt : L.p.dtmax) * L.p.fdelt;
     *delt = ldt;
     return 0;

@@ -169,6 +169,142 @@ __global__ void k_visc_d(Geo g, Met m, VArgs a) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// NR_ARTIFICIAL_VISC (ppfor.f:1032-1230), von Neumann-Richtmyer, one rank
+// without poles (z range 1..nz).  Per direction the reference sweeps the
+// cells i = 1..upperbnd-2 of each column in order, forming the compression
+// flux Q = 3 delt rho min(dv, 0)^2 on the cell and updating the velocity by
+// the flux difference and T by the flux's work.  Each Q depends on the
+// velocities before that direction's update only, so a direction is two
+// parallel passes: the fluxes, then the updates; the passes are fused
+// across directions (X update + Y fluxes, Y update + Z fluxes) as they touch
+// disjoint arrays.  The lower neighbour's flux a Y or Z update reads is the
+// one this call wrote, or, where upperbnd of that neighbour column is lower,
+// what an earlier call left: QviscY and QviscZ persist, as the reference's
+// module arrays do.  Each flux pass also takes the minimum of dv/dx over the
+// fluxes it forms, the time-step bound (block partials, folded to
+// ctrl->av_min).
+enum { A_QX, A_DX, A_DY, A_DZ, A_QY, A_QZ, A_N };  // A_QY, A_QZ persist
+double* g_avq[A_N] = {};
+
+struct AArgs {
+    double *V, *G, *H, *T;
+    const double *RH, *CV, *RHQX, *RHQY, *RHQZ;
+    const int32_t* ub;
+    double* q[A_N];
+    double coef;  // artvisc_coeff delt
+    double* part;
+    int nb;
+};
+
+__device__ inline void block_min_out(double mn, double* out) {
+    __shared__ double sh[4];
+    double a = wave_min(mn);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = a;
+    __syncthreads();
+    if (threadIdx.x == 0) *out = fmin(fmin(sh[0], sh[1]), fmin(sh[2], sh[3]));
+}
+
+// fluxes of one direction at cell c from the velocity difference d (the
+// expression order of ppfor.f:1074-1075)
+__device__ inline double nr_flux(double coef, double rh, double& d) {
+    if (d > 0.0) d = 0.0;
+    return coef * rh * (d * d);
+}
+
+// pass 1: X fluxes (ppfor.f:1066-1099), Q(0) for every column
+__global__ __launch_bounds__(256) void k_av_x(Geo g, Met m, AArgs a) {
+    double mn = 0.0;
+    const long long n = (long long)g.nx * g.ny * g.nz;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+        int i, j, k;
+        sub_ijk(t, g.nx, g.ny, 1, 1, 1, i, j, k);
+        if (i == 1) {
+            const long long c0 = IX(0, j, k);
+            double d = a.V[IX(1, j, k)] - a.V[c0];
+            a.q[A_QX][c0] = nr_flux(a.coef, a.RH[c0], d);
+            mn = fmin(mn, d / m(TT_M_DXB, 0));
+        }
+        if (i > a.ub[g.idx2(j, k)] - 2) continue;
+        const long long c = IX(i, j, k);
+        double d = a.V[IX(i + 1, j, k)] - a.V[c];
+        a.q[A_QX][c] = nr_flux(a.coef, a.RH[c], d);
+        a.q[A_DX][c] = d;
+        mn = fmin(mn, d / m(TT_M_DXB, i));
+    }
+    block_min_out(mn, a.part + blockIdx.x);
+}
+
+// pass 2: X updates (ppfor.f:1076-1079), then Y fluxes (:1100-1142), Q(j=0)
+// from the j = 1 cells
+__global__ __launch_bounds__(256) void k_av_xy(Geo g, Met m, AArgs a) {
+    double mn = 0.0;
+    const long long n = (long long)g.nx * g.ny * g.nz;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+        int i, j, k;
+        sub_ijk(t, g.nx, g.ny, 1, 1, 1, i, j, k);
+        if (i > a.ub[g.idx2(j, k)] - 2) continue;
+        const long long c = IX(i, j, k);
+        const double rh = a.RH[c], cv = a.CV[c];
+        const double qx = a.q[A_QX][c];
+        a.V[c] = a.V[c] - (qx - a.q[A_QX][IX(i - 1, j, k)]) / (m(TT_M_DXB, i) * a.RHQX[c]);
+        a.T[c] = a.T[c] - qx * a.q[A_DX][c] / (m(TT_M_DXA, i) * rh * cv);
+        if (j == 1) {
+            const long long c0 = IX(i, 0, k);
+            double d = a.G[c] - a.G[c0];
+            a.q[A_QY][c0] = nr_flux(a.coef, a.RH[c0], d);
+            mn = fmin(mn, d / m(TT_M_DYB, 0));
+        }
+        double d = a.G[IX(i, j + 1, k)] - a.G[c];
+        a.q[A_QY][c] = nr_flux(a.coef, rh, d);
+        a.q[A_DY][c] = d;
+        mn = fmin(mn, d / m(TT_M_DYB, j));
+    }
+    block_min_out(mn, a.part + a.nb + blockIdx.x);
+}
+
+// pass 3: Y updates (ppfor.f:1124-1127), then Z fluxes (:1154-1197), Q(k=0)
+// from the k = 1 cells
+__global__ __launch_bounds__(256) void k_av_yz(Geo g, Met m, AArgs a) {
+    double mn = 0.0;
+    const long long n = (long long)g.nx * g.ny * g.nz;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+        int i, j, k;
+        sub_ijk(t, g.nx, g.ny, 1, 1, 1, i, j, k);
+        if (i > a.ub[g.idx2(j, k)] - 2) continue;
+        const long long c = IX(i, j, k);
+        const double rh = a.RH[c], cv = a.CV[c];
+        const double qy = a.q[A_QY][c];
+        a.G[c] = a.G[c] - (qy - a.q[A_QY][IX(i, j - 1, k)]) / (m(TT_M_DYB, j) * a.RHQY[c]);
+        a.T[c] = a.T[c] - qy * a.q[A_DY][c] / (m(TT_M_DYA, j) * rh * cv);
+        if (k == 1) {
+            const long long c0 = IX(i, j, 0);
+            double d = a.H[c] - a.H[c0];
+            a.q[A_QZ][c0] = nr_flux(a.coef, a.RH[c0], d);
+            mn = fmin(mn, d / m(TT_M_DZB, 0));
+        }
+        double d = a.H[IX(i, j, k + 1)] - a.H[c];
+        a.q[A_QZ][c] = nr_flux(a.coef, rh, d);
+        a.q[A_DZ][c] = d;
+        mn = fmin(mn, d / m(TT_M_DZB, k));
+    }
+    block_min_out(mn, a.part + 2 * a.nb + blockIdx.x);
+}
+
+// pass 4: Z updates (ppfor.f:1176-1179)
+__global__ __launch_bounds__(256) void k_av_z(Geo g, Met m, AArgs a) {
+    const long long n = (long long)g.nx * g.ny * g.nz;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+        int i, j, k;
+        sub_ijk(t, g.nx, g.ny, 1, 1, 1, i, j, k);
+        if (i > a.ub[g.idx2(j, k)] - 2) continue;
+        const long long c = IX(i, j, k);
+        const double qz = a.q[A_QZ][c];
+        a.H[c] = a.H[c] - (qz - a.q[A_QZ][IX(i, j, k - 1)]) / (m(TT_M_DZB, k) * a.RHQZ[c]);
+        a.T[c] = a.T[c] - qz * a.q[A_DZ][c] / (m(TT_M_DZA, k) * a.RH[c] * a.CV[c]);
+    }
+}
+
 }  // namespace
 
 void visc_free() {
@@ -176,6 +312,11 @@ void visc_free() {
         if (p) hipFree(p);
         p = nullptr;
     }
+    for (auto& p : g_avq) {
+        if (p) hipFree(p);
+        p = nullptr;
+    }
+    L.av_valid = 0;
 }
 
 // ETACALC + VISC on the device state (after PRESSG, as FORGAS3D calls them)
@@ -212,6 +353,54 @@ int visc_impl() {
     return 0;
 }
 
+// NR_ARTIFICIAL_VISC on the device state (ppfor.f:1032-1230); FORGAS3D
+// calls BOUNDS before and after it
+int artvis_impl(double dt) {
+    Geo& g = L.g;
+    double** f = L.f;
+    const size_t bytes = (size_t)g.n3 * sizeof(double);
+    if (!g_avq[0]) {
+        for (auto& p : g_avq) TT_HIP(hipMalloc(&p, bytes));
+        // QviscY/QviscZ start at zero: entries a call does not write are read
+        // by the next one where upperbnd varies
+        TT_HIP(hipMemsetAsync(g_avq[A_QY], 0, bytes, L.s));
+        TT_HIP(hipMemsetAsync(g_avq[A_QZ], 0, bytes, L.s));
+    }
+    int nb = 1024;
+    if (3 * nb > 2 * L.npartial) nb = 2 * L.npartial / 3;
+    const double artvisc_coeff = 3.0;  // artvis_var_init, ppmod.f:392
+    AArgs a{f[TT_V], f[TT_G], f[TT_H], f[TT_T], f[TT_RH], f[TT_CV], f[TT_RHQX], f[TT_RHQY], f[TT_RHQZ], L.ub, {},
+            artvisc_coeff * dt, L.partial, nb};
+    for (int q = 0; q < A_N; ++q) a.q[q] = g_avq[q];
+    const Met mt = met();
+    k_av_x<<<nb, 256, 0, L.s>>>(g, mt, a);
+    TT_LAUNCH();
+    k_av_xy<<<nb, 256, 0, L.s>>>(g, mt, a);
+    TT_LAUNCH();
+    k_av_yz<<<nb, 256, 0, L.s>>>(g, mt, a);
+    TT_LAUNCH();
+    k_av_z<<<nb, 256, 0, L.s>>>(g, mt, a);
+    TT_LAUNCH();
+    int rc = fold_partials(L.partial, nb, 3, 2, &L.ctrl->av_min[0], &L.ctrl->av_min[1], &L.ctrl->av_min[2]);
+    if (rc) return rc;
+    L.av_valid = 1;
+    return 0;
+}
+
+// dt_artvis (ppfor.f:1211-1212) from the minima of the last artvis_impl,
+// read from the host mirror of the control block after a sync_check; DTMAX
+// before the first call (pp.f:439)
+double dt_artvis() {
+    if (!L.av_valid) return L.p.dtmax;
+    const double* m = L.hctrl->av_min;
+    double mn = m[0];
+    if (m[1] < mn) mn = m[1];
+    if (m[2] < mn) mn = m[2];
+    const double tiny = (double)1.0e-30f;  // TINY, a REAL literal (ppmod.f:1092)
+    const double d = 1.0 / fabs(8.0 * 3.0 * (mn - tiny));
+    return d < L.p.dtmax ? d : L.p.dtmax;
+}
+
 }  // namespace tt
 
 using namespace tt;
@@ -224,7 +413,18 @@ extern "C" int tt_set_viscosity(const tt_viscosity* v) {
         if (v->ieta != 0 && v->ieta != 4 && v->ieta != 5) return fail(1, "ETACALC: IETA must be 0, 4 or 5");
         if (!(v->xnue > 0.0)) return fail(1, "ETACALC: XNUE must be positive");
     }
+    if (v->iartvis > 1) return fail(1, "unknown IARTVIS: only 1 (NR_ARTIFICIAL_VISC) is defined");
+    if (v->iartvis > 0 && L.visc.iartvis == 0) L.av_valid = 0;  // dt_artvis = DTMAX (pp.f:439)
     L.visc = *v;
+    return 0;
+}
+
+extern "C" int tt_get_dt_artvis(double* dt) {
+    if (!L.ready) return fail(1, "tt_init has not been called");
+    if (!dt) return fail(1, "null dt");
+    int rc = sync_check();
+    if (rc) return rc;
+    *dt = dt_artvis();
     return 0;
 }
 

@@ -31,6 +31,7 @@ module ttgpu_c
   type, bind(C) :: tt_viscosity
      integer(c_int32_t) :: ivis, ivistyp, ieta, idiss
      real(c_double) :: xnue
+     integer(c_int32_t) :: iartvis, pad0
   end type tt_viscosity
 
   type, bind(C) :: tt_sor_stats
@@ -82,6 +83,10 @@ module ttgpu_c
        import :: c_int, tt_viscosity
        type(tt_viscosity), intent(in) :: v
      end function tt_set_viscosity
+     integer(c_int) function tt_get_dt_artvis(dt) bind(C, name='tt_get_dt_artvis')
+       import :: c_int, c_double
+       real(c_double), intent(out) :: dt
+     end function tt_get_dt_artvis
      integer(c_int) function tt_advect3d(delt) bind(C, name='tt_advect3d')
        import :: c_int, c_double
        real(c_double), value :: delt

@@ -70,7 +70,8 @@ class Viscosity(ctypes.Structure):
     """``tt_viscosity``."""
 
     _fields_ = [("ivis", ctypes.c_int32), ("ivistyp", ctypes.c_int32), ("ieta", ctypes.c_int32),
-                ("idiss", ctypes.c_int32), ("xnue", ctypes.c_double)]
+                ("idiss", ctypes.c_int32), ("xnue", ctypes.c_double), ("iartvis", ctypes.c_int32),
+                ("pad0", ctypes.c_int32)]
 
 
 class Limits(ctypes.Structure):

@@ -57,6 +57,7 @@ def load_library():
     L.tt_set_opacity_table.argtypes = [ctypes.c_void_p] * 3
     L.tt_set_viscosity.argtypes = [ctypes.POINTER(P.Viscosity)]
     L.tt_get_viscosity.argtypes = [ctypes.POINTER(P.Viscosity)]
+    L.tt_get_dt_artvis.argtypes = [ctypes.POINTER(ctypes.c_double)]
     L.tt_get_sorparam.restype = ctypes.c_double
     L.tt_device_bytes.restype = ctypes.c_int64
     L.tt_sweep_timing_get.argtypes = [ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double),
@@ -182,10 +183,17 @@ class Device:
         self._check(self.L.tt_set_opacity_table(*(a.ctypes.data_as(ctypes.c_void_p) for a in t)),
                     "tt_set_opacity_table")
 
-    def set_viscosity(self, ivis=1, ivistyp=1, ieta=4, idiss=1, xnue=1e9):
-        """ETACALC + VISC inside FORGAS3D (ppfor.f:170-174); ivis 0 is off."""
-        self._check(self.L.tt_set_viscosity(ctypes.byref(P.Viscosity(ivis, ivistyp, ieta, idiss, xnue))),
-                    "tt_set_viscosity")
+    def set_viscosity(self, ivis=1, ivistyp=1, ieta=4, idiss=1, xnue=1e9, iartvis=0):
+        """ETACALC + VISC inside FORGAS3D (ppfor.f:170-174), ivis 0 is off;
+        iartvis 1 adds NR_ARTIFICIAL_VISC at its end (ppfor.f:374-395)."""
+        v = P.Viscosity(ivis, ivistyp, ieta, idiss, xnue, iartvis)
+        self._check(self.L.tt_set_viscosity(ctypes.byref(v)), "tt_set_viscosity")
+
+    def dt_artvis(self):
+        """dt_artvis of the last NR_ARTIFICIAL_VISC (DTMAX before the first)."""
+        d = ctypes.c_double()
+        self._check(self.L.tt_get_dt_artvis(ctypes.byref(d)), "tt_get_dt_artvis")
+        return d.value
 
     def get_viscosity(self):
         v = P.Viscosity()
