@@ -235,7 +235,8 @@ def run_forgas_visc(a):
     IVIS=1, IVISTYP=1, IETA=4 (molecular, XNUE = 1e9) and IDISS=1, on the
     generator's state with a 5% day-night contrast.  K timed calls of
     tt_forgas3d after W untimed ones (each ends with a host sync); the same
-    without viscosity for the difference."""
+    without viscosity, and with the ipp's IARTVIS=1 (NR_ARTIFICIAL_VISC) on
+    top, for the differences."""
     from ttcode_amd import ttgpu
     nx, ny, nz = a.nr, a.nphi, a.ntheta
     p = P.make_params(nx, ny, nz, epsmat=1e-10)
@@ -246,9 +247,9 @@ def run_forgas_visc(a):
     dev.pressg()
     dt = dev.schritt()
     out_us = {}
-    for name, ivis in (("viscous", 1), ("inviscid", 0)):
+    for name, ivis, iartvis in (("viscous", 1, 0), ("inviscid", 0, 0), ("ipp", 1, 1)):
         dev.load_state(st0)
-        dev.set_viscosity(ivis, 1, 4, 1, 1e9)
+        dev.set_viscosity(ivis, 1, 4, 1, 1e9, iartvis)
         dev.bounds()
         dev.qdens()
         for _ in range(a.warmup):
@@ -265,7 +266,9 @@ def run_forgas_visc(a):
            "dtype": "f64", "data": "synthetic (generator + 5% day-night contrast)",
            "config": {"workload": f"forgas_visc sph{nx}x{nz}x{ny}: IVIS=1 IVISTYP=1 IETA=4 IDISS=1 XNUE=1e9"},
            "inviscid_us": round(out_us["inviscid"], 1),
-           "viscosity_us": round(out_us["viscous"] - out_us["inviscid"], 1)}
+           "viscosity_us": round(out_us["viscous"] - out_us["inviscid"], 1),
+           "with_artvis_us": round(out_us["ipp"], 1),
+           "artvis_us": round(out_us["ipp"] - out_us["viscous"], 1)}
     print(json.dumps(out), flush=True)
 
 
