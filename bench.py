@@ -228,6 +228,27 @@ def run_kappa_table(a):
            "config": {"workload": f"kappa_table sph{nx}x{nz}x{ny}: PRESSG + FreedmanOPC over the ghosted box"},
            "algorithmic_bytes_per_call": alg, "alg_GBs_incl_pressg_and_sync": round(alg / (us * 1e-6) / 1e9, 1)}
     print(json.dumps(out), flush=True)
+    # the same for the Burrows tables (IKAPTYP 10: log10 of T and rho, no PRESSG)
+    p = P.make_params(nx, ny, nz, epsmat=1e-10, ikaptyp=10)
+    dev = ttgpu.Device(p)
+    dev.set_burrows_table(*opacity.synthetic_burrows_table(a.seed)[2:])
+    dev.load_state(st0)
+    dev.bounds()
+    for _ in range(a.warmup):
+        dev.kappa()
+    dev.sync()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        dev.kappa()
+    dev.sync()
+    us = (time.perf_counter() - t0) * 1e6 / a.steps
+    dev.close()
+    # T, RH read; xkapR/P/A, dxkapP/A_dT, sig written
+    out = dict(out, metric="KAPPA (IKAPTYP 10 Burrows tables) time per call", value=round(us, 2),
+               data="synthetic table (ttcode_amd.opacity.synthetic_burrows_table) and state",
+               config={"workload": f"kappa_table sph{nx}x{nz}x{ny}: BurrowsOPC over the ghosted box"},
+               alg_GBs_incl_pressg_and_sync=None, alg_GBs_incl_sync=round(alg / (us * 1e-6) / 1e9, 1))
+    print(json.dumps(out), flush=True)
 
 
 def run_forgas_visc(a):

@@ -46,7 +46,8 @@ typedef struct tt_params {
     int32_t ibdrad;          /* BOUNDXMAX(2) top-T version: 3, 4, 7 or 9      */
     int32_t iflimtyp;        /* flux limiter: 1 = 1/3, 2 = Levermore-Pomraning */
     int32_t f_incident;      /* 0 = no irradiation, 2 = implicit cos heating  */
-    int32_t ikaptyp;         /* 1 = constant ZKAPMULT, 26 = power law (new)   */
+    int32_t ikaptyp;         /* 1 = constant ZKAPMULT, 4/5 and 9-12/14 tables,
+                                26 = power law (new)                          */
     int32_t isigtyp;         /* 0 = no scattering, 1 = 0.4 cm^2/g             */
     int32_t isoth;           /* 0 = energy equation on                        */
     int32_t itmax;           /* ITMAX, shared by SOR and Newton (ipp:13)      */
@@ -385,9 +386,31 @@ int tt_sor_speculation_get(int64_t* spec_batches, int64_t* reruns);
  * (pprphy.f:944-1002).  The library keeps a copy, across tt_init and
  * tt_finalize, until the next call; it may be called before tt_init. */
 int tt_set_opacity_table(const double* kap_planck, const double* kap_ross, const double* kap_abs);
+
+/* ------------------------------------------------------------------------
+ * Table opacities, IKAPTYP 9, 10, 11, 12 and 14 (KAPPA, pprphy.f:199-234,
+ * 242-274; BurrowsOPC, pprphy.f:1005-1225): log10 of the Planck, Rosseland
+ * and absorption means on the reference's grid of 50 log10 T (1.69896 ...
+ * 3.69889) x 50 log10 rho (-11.9996 ... -1.99993) (ppmod.f:724-745, REAL
+ * values), temperature fastest: kap[it + 50 ir], as BurrowsXKAP_*(50,50).
+ * The tables are the ones READBURROWSOPCTABLE (pprphy.f:1227-1520) leaves:
+ * for IKAPTYP 12 and 14 the absorption table is log kP + 4 log10(gamma_OPC),
+ * for 14 the Rosseland table is log kP, for 11 the absorption table carries
+ * the planet's extra absorber below 0.03 bar (ttcode_amd/opacity.py
+ * read_burrows_table applies these).  IKAPTYP 9 reads the table with TiO/VO,
+ * the others the one without.  The wavelength-binned families 20-25 are not
+ * supported.
+ * ------------------------------------------------------------------------ */
+#define TT_BURROWS_NT 50
+#define TT_BURROWS_NR 50
+/* Replaces READBURROWSOPCTABLE's fill of BurrowsXKAP_plnk/_ross/_abs; kept
+ * like the IKAPTYP 4/5 tables, and may be called before tt_init. */
+int tt_set_burrows_table(const double* kap_planck, const double* kap_ross, const double* kap_abs);
+
 /* KAPPA + SUBSIG (pprphy.f:1-360,567-583) on the device state, as FLIM calls
  * them (pprphy.f:362-380).  IKAPTYP 4 and 5 call PRESSG first and stop with
- * a positive code on T <= 0 (pprphy.f:825-828). */
+ * a positive code on T <= 0 (pprphy.f:825-828); IKAPTYP 9-14 stop likewise
+ * on T <= 0 or a density outside the table's log range (pprphy.f:1040-1105). */
 int tt_kappa(void);
 
 /* ------------------------------------------------------------------------

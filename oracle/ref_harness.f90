@@ -10,7 +10,9 @@
 !   gitter | bounds | kappa | advect | forgas | energy | sor | adjust | step |
 !   diffusion | printout | restart
 ! For IKAPTYP 4 and 5 the reference's own READFREEDMANOPCTABLE reads its table
-! file (Freedman6117.dat or Freedman6117_noTIVO.dat) from the working directory.
+! file (Freedman6117.dat or Freedman6117_noTIVO.dat) from the working directory,
+! for IKAPTYP 9-12 and 14 READBURROWSOPCTABLE its (BURROWS6117_wTiVO.dat or
+! BURROWS6117_woutTiVO.dat, planet_num 1).
 ! then writes every array of interest to out_<name>.bin and the scalars to
 ! out_scalars.txt.  CALCMUSTAR lives in ppsub.f, which needs the absent hypre
 ! include; mu_star is therefore an input (in_mu_star.bin).
@@ -168,6 +170,7 @@ program tt_ref_harness
   end if
 
   if (IKAPTYP == 4 .or. IKAPTYP == 5) call READFREEDMANOPCTABLE
+  if ((IKAPTYP >= 9 .and. IKAPTYP <= 12) .or. IKAPTYP == 14) call READBURROWSOPCTABLE
 
   select case (trim(cmd))
   case ('gitter')

@@ -800,6 +800,85 @@ static void freedman_opc(double t_in, double pg_in, double* bkp, double* bkr, do
     *dka = (pow(10.0, k00[2]) - pow(10.0, k01[2])) / (FREEDMAN_T[it] - FREEDMAN_T[it1]);
 }
 
+/* Axes of the IKAPTYP 9-14 tables (ppmod.f:724-745): log10 T and log10 rho,
+ * REAL constants of DATA statements stored in double arrays (float values,
+ * widened). */
+static const double BURROWS_T[50] = {
+    1.69896f, 1.73978f, 1.78061f, 1.82143f, 1.86225f, 1.90308f, 1.94390f, 1.98473f, 2.02555f, 2.06637f,
+    2.10720f, 2.14802f, 2.18884f, 2.22967f, 2.27049f, 2.31132f, 2.35214f, 2.39296f, 2.43379f, 2.47461f,
+    2.51543f, 2.55626f, 2.59708f, 2.63790f, 2.67873f, 2.71955f, 2.76038f, 2.80120f, 2.84202f, 2.88285f,
+    2.92367f, 2.96406f, 3.00488f, 3.04571f, 3.08653f, 3.12735f, 3.16818f, 3.20900f, 3.24983f, 3.29065f,
+    3.33147f, 3.37230f, 3.41312f, 3.45394f, 3.49477f, 3.53559f, 3.57642f, 3.61724f, 3.65806f, 3.69889f};
+static const double BURROWS_RH[50] = {
+    -11.9996f, -11.7954f, -11.5913f, -11.3872f, -11.1831f, -10.9790f, -10.7748f, -10.5707f, -10.3666f,
+    -10.1625f, -9.95837f, -9.75425f, -9.55014f, -9.34602f, -9.14190f, -8.93778f, -8.73366f, -8.52954f,
+    -8.32543f, -8.12131f, -7.91719f, -7.71307f, -7.50895f, -7.30483f, -7.10071f, -6.89660f, -6.69248f,
+    -6.48836f, -6.28424f, -6.08012f, -5.87600f, -5.67189f, -5.46777f, -5.26365f, -5.05953f, -4.85541f,
+    -4.65129f, -4.44718f, -4.24479f, -4.04068f, -3.83656f, -3.63244f, -3.42875f, -3.22464f, -3.02052f,
+    -2.81640f, -2.61228f, -2.40816f, -2.20404f, -1.99993f};
+static double g_btab[3][50 * 50]; /* log10 Planck, Rosseland, absorption; T fastest */
+static int g_btab_set;
+
+int tor_set_burrows_table(const double* kp, const double* kr, const double* ka)
+{
+    memcpy(g_btab[0], kp, sizeof g_btab[0]);
+    memcpy(g_btab[1], kr, sizeof g_btab[1]);
+    memcpy(g_btab[2], ka, sizeof g_btab[2]);
+    g_btab_set = 1;
+    return 0;
+}
+
+/* BurrowsOPC, pprphy.f:1005-1225 (the grey part: the wavelength bins of
+ * IKAPTYP 20-25 are not evaluated): log10 T and log10 rho clamped into the
+ * table, the first node above each, bilinear interpolation of the log
+ * tables, 10^x; dk/dT the one-sided difference over 10^ of the T nodes.
+ * Returns 1 where the reference stops (T <= 0, or a NaN log10 rho, which
+ * finds no node above it). */
+static int burrows_opc(double t_in, double rh_in, double* bkp, double* bkr, double* bka, double* dkp,
+                       double* dka)
+{
+    if (t_in <= 0.0) return 1;
+    double rh1 = log10(rh_in);
+    double tm1 = log10(t_in);
+    if (tm1 < (double)1.69896f) tm1 = (double)1.69897f;
+    if (tm1 > (double)3.69889f) tm1 = (double)3.69888f;
+    if (rh1 < (double)-11.9996f) rh1 = (double)-11.9995f;
+    if (rh1 > (double)-1.99993f) rh1 = (double)-1.99994f;
+    int it = 0;
+    while (it < 50 && !(BURROWS_T[it] > tm1)) ++it;
+    if (it == 50 || it == 0) return 1;
+    const int it1 = it - 1;
+    double ws1 = BURROWS_T[it1] - tm1;
+    double ws = BURROWS_T[it1] - BURROWS_T[it];
+    ws1 = ws1 / ws;
+    ws = 1.0 - ws1;
+    int iw = 0;
+    while (iw < 50 && !(BURROWS_RH[iw] > rh1)) ++iw;
+    if (iw == 50 || iw == 0) return 1;
+    const int iw1 = iw - 1;
+    double ws2 = rh1 - BURROWS_RH[iw1];
+    double ws3 = BURROWS_RH[iw] - BURROWS_RH[iw1];
+    ws2 = ws2 / ws3;
+    ws3 = 1.0 - ws2;
+    double out[3], k00[3], k01[3];
+    for (int q = 0; q < 3; ++q) {
+        const double* t = g_btab[q];
+        const double a00 = t[it + 50 * iw], a10 = t[it + 50 * iw1], a11 = t[it1 + 50 * iw1], a01 = t[it1 + 50 * iw];
+        const double w4 = ws2 * (ws1 * a00 + ws * a01);
+        const double w5 = ws3 * (ws1 * a10 + ws * a11);
+        out[q] = pow(10.0, w4 + w5);
+        k00[q] = a00;
+        k01[q] = a01;
+    }
+    *bkp = out[0];
+    *bkr = out[1];
+    *bka = out[2];
+    const double dt = pow(10.0, BURROWS_T[it]) - pow(10.0, BURROWS_T[it1]);
+    *dkp = (pow(10.0, k00[0]) - pow(10.0, k01[0])) / dt;
+    *dka = (pow(10.0, k00[2]) - pow(10.0, k01[2])) / dt;
+    return 0;
+}
+
 static int kappa(void) /* KAPPA, pprphy.f:1-360; case 26 is new */
 {
     const tt_params* p = &O.p;
@@ -827,6 +906,27 @@ static int kappa(void) /* KAPPA, pprphy.f:1-360; case 26 is new */
         if (bad) return seterr(2, "NEG. T. IN FREEDMAN OPC");
         return 0;
     }
+    if ((p->ikaptyp >= 9 && p->ikaptyp <= 12) || p->ikaptyp == 14) { /* pprphy.f:199-234,242-274 */
+        if (!g_btab_set) return seterr(1, "KAPPA: IKAPTYP 9-12 and 14 need an opacity table");
+        int bad = 0;
+#pragma omp parallel for reduction(| : bad)
+        for (int k = -1; k <= NZ + 2; ++k)
+            for (int j = -1; j <= NY + 2; ++j) {
+                int ub = UBND(j, k);
+                for (int i = -1; i <= ub; ++i)
+                    bad |= burrows_opc(A(O.T, i, j, k), A(O.RH, i, j, k), &A(O.XKP, i, j, k), &A(O.XKR, i, j, k),
+                                       &A(O.XKA, i, j, k), &A(O.DKP, i, j, k), &A(O.DKA, i, j, k));
+                for (int i = ub + 1; i <= NX + 2; ++i) {
+                    A(O.XKR, i, j, k) = A(O.XKR, ub, j, k);
+                    A(O.XKP, i, j, k) = A(O.XKP, ub, j, k);
+                    A(O.XKA, i, j, k) = A(O.XKA, ub, j, k);
+                    A(O.DKP, i, j, k) = A(O.DKP, ub, j, k);
+                    A(O.DKA, i, j, k) = A(O.DKA, ub, j, k);
+                }
+            }
+        if (bad) return seterr(2, "NEG. T. IN BURROWS OPC (or a density outside the table)");
+        return 0;
+    }
     if (p->ikaptyp == 1) { /* pprphy.f:41-47 */
         for (size_t n = 0; n < O.n3; ++n) {
             O.XKR[n] = p->zkapmult; O.XKP[n] = p->zkapmult; O.XKA[n] = p->zkapmult;
@@ -834,7 +934,7 @@ static int kappa(void) /* KAPPA, pprphy.f:1-360; case 26 is new */
         }
         return 0;
     }
-    if (p->ikaptyp != 26) return seterr(1, "KAPPA: IKAPTYP must be 1, 4, 5 or 26");
+    if (p->ikaptyp != 26) return seterr(1, "KAPPA: IKAPTYP must be 1, 4, 5, 9-12, 14 or 26");
     /* Power law kappa_R = kappa_P = kappa_A = kap0 (rho/rho0)^a (T/T0)^b in the
      * loop structure of the IKAPTYP=8 power law (pprphy.f:176-198): cells up to
      * upperbnd are evaluated, cells above copy the upperbnd value.  Unlike

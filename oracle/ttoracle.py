@@ -50,6 +50,7 @@ class Oracle:
             L.tor_diffusion.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                         ctypes.POINTER(ctypes.c_int)]
             L.tor_set_opacity_table.argtypes = [ctypes.c_void_p] * 3
+            L.tor_set_burrows_table.argtypes = [ctypes.c_void_p] * 3
             L.tor_set_viscosity.argtypes = [ctypes.POINTER(P.Viscosity)]
             L.tor_get_viscosity.argtypes = [ctypes.POINTER(P.Viscosity)]
             L.tor_get_dt_artvis.argtypes = [ctypes.POINTER(ctypes.c_double)]
@@ -106,6 +107,14 @@ class Oracle:
         assert all(a.size == 42 * 18 for a in t)
         self._check(self.L.tor_set_opacity_table(*(a.ctypes.data_as(ctypes.c_void_p) for a in t)),
                     "set_opacity_table")
+
+    def set_burrows_table(self, kap_planck, kap_ross, kap_abs):
+        """The log10 tables of IKAPTYP 9-12/14 on the 50 x 50 (log T,
+        log rho) grid, temperature fastest."""
+        t = [np.ascontiguousarray(np.asarray(a, np.float64).reshape(-1)) for a in (kap_planck, kap_ross, kap_abs)]
+        assert all(a.size == 50 * 50 for a in t)
+        self._check(self.L.tor_set_burrows_table(*(a.ctypes.data_as(ctypes.c_void_p) for a in t)),
+                    "set_burrows_table")
 
     def set_viscosity(self, ivis=1, ivistyp=1, ieta=4, idiss=1, xnue=1e9, iartvis=0):
         """ETACALC + VISC inside FORGAS3D (ppfor.f:170-174), off by default;

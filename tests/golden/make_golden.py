@@ -347,6 +347,70 @@ def opacities():
          dict(tab, delt=dt, iii=2))
 
 
+def burrows_table_text(seed=5):
+    """A synthetic IKAPTYP 9-14 table in the reference's text format
+    (READBURROWSOPCTABLE, pprphy.f:1227-1520): ten header lines, T_star, then
+    2500 rows T rho log_kP log_kR log_kABS with T outer and rho inner, T and
+    rho the grid's nodes in linear units (the reader takes rows in order and
+    uses T and rho only for IKAPTYP 11's pressure test)."""
+    from ttcode_amd import opacity as O
+    lt, lr, kp, kr, ka = O.synthetic_burrows_table(seed)
+    rows = []
+    for it in range(O.BNT):
+        for ir in range(O.BNR):
+            q = it + O.BNT * ir
+            rows.append(f"{10.0 ** float(lt[it])!r} {10.0 ** float(lr[ir])!r} {float(kp[q])!r} {float(kr[q])!r} "
+                        f"{float(ka[q])!r}")
+    head = [f"synthetic Burrows table line {i + 1}" for i in range(10)]
+    return "\n".join(head + ["6117.0"] + rows) + "\n"
+
+
+def burrows():
+    """IKAPTYP 11 and 14 (Burrows tables, pprphy.f:199-274,1005-1520) on the
+    synthetic table: KAPPA alone on a scrambled state that reaches both
+    clamps in T and rho and the copy above upperbnd, 11 with the extra
+    absorber below 0.03 bar and 14 with the constant-gamma absorption and
+    Rosseland tables; and COUPLED_ENERGY_EQS with IKAPTYP 10 and a 5%
+    day-night contrast."""
+    import tempfile as _tf
+    from ttcode_amd import opacity as O
+    nx, ny, nz = 12, 16, 12
+    text = burrows_table_text()
+    o = Oracle(P.make_params(nx, ny, nz, ikaptyp=1, kap0=1e-2))
+    o.load_state(base_inputs(o.p))
+    o.bounds()
+    o.pressg()
+    dt = o.schritt()
+    for ikap in (11, 14, 10):
+        fname = O.BURROWS_FILES[ikap]
+        p = P.make_params(nx, ny, nz, ikaptyp=ikap, isigtyp=1 if ikap != 10 else 0, epsmat=1e-8, tirr=2000.0)
+        with _tf.TemporaryDirectory() as d:
+            fn = os.path.join(d, fname)
+            with open(fn, "w") as f:
+                f.write(text)
+            kp, kr, ka, tstar = O.read_burrows_table(fn, ikap, gamma_opc=0.1, mu_gas=p.mu_gas, planet_num=1)
+        tab = {"tab_planck": kp, "tab_ross": kr, "tab_abs": ka}
+        files = {fname: text}
+        if ikap != 10:
+            tab["tab_text"] = np.array(text)
+            inp = base_inputs(p, seed=61 + ikap)
+            rng = np.random.default_rng(61 + ikap)
+            T = 10.0 ** rng.uniform(1.5, 3.9, size=inp["T"].shape)
+            f = np.linspace(0.0, 1.0, inp["RH"].shape[-1])  # r is the fastest (last numpy) axis
+            RH = 10.0 ** (-1.0 - 11.8 * f + rng.uniform(-0.6, 0.6, size=inp["RH"].shape))
+            inp_k = dict(inp, T=T, RH=RH)
+            outs, sc = run_ref("kappa", p, inp_k, files=files)
+            save(f"kappa_burrows{ikap}_12x16x12", p, inp_k, outs, sc,
+                 ["T", "RH", "xkapR", "xkapP", "xkapA", "dxkapP_dT", "dxkapA_dT", "sig", "upperbnd"], tab)
+        else:
+            inp_e = base_inputs(p, seed=67, contrast=0.05)
+            outs, sc = run_ref("energy", p, inp_e, delt=dt, iii=2, files=files)
+            save("energy_burrows10_12x16x12", p, inp_e, outs, sc,
+                 ["T", "ER", "delta_ER", "PG", "xkapR", "xkapP", "xkapA", "dxkapP_dT", "dxkapA_dT",
+                  "STELLARINPUT", "FLIMX", "FLIMY", "FLIMZ", "FQ", "FR", "RHS_ER", "upperbnd"],
+                 dict(tab, delt=dt, iii=2))
+
+
 def viscosity():
     """FORGAS3D with viscosity (ppfor.f:170-174: ETACALC, ppvisc.f:375-484;
     VISC, ppvisc.f:2-373 with IVISTYP=1 and dissipative heating) after the
@@ -458,6 +522,8 @@ if __name__ == "__main__":
         viscosity()
     elif sys.argv[1:] == ["artvis"]:
         artvis()
+    elif sys.argv[1:] == ["burrows"]:
+        burrows()
     elif sys.argv[1:] == ["variants"]:
         variants()
     elif sys.argv[1:] == ["adjust"]:

@@ -244,6 +244,91 @@ def test_kappa_freedman_table_stops_on_nonpositive_temperature(ttgpu):
         d.close()
 
 
+def _burrows(d, g):
+    d.set_burrows_table(g["tab_planck"], g["tab_ross"], g["tab_abs"])
+
+
+@pytest.mark.parametrize("name", ["kappa_burrows11_12x16x12", "kappa_burrows14_12x16x12"])
+def test_kappa_burrows_table(ttgpu, name):
+    """KAPPA with IKAPTYP 11 and 14 (BurrowsOPC, pprphy.f:199-274,1005-1225)
+    and SUBSIG against the reference, on a state beyond both clamps in T and
+    rho (log10 and 10^x from the device math library: within 1e-12)."""
+    g = C.load(name)
+    p, d = _dev(ttgpu, g)
+    try:
+        _burrows(d, g)
+        d.bounds()
+        d.kappa()
+        _cmp(g, p, d, rel=1e-12)
+    finally:
+        d.close()
+
+
+def test_coupled_energy_burrows_table(ttgpu):
+    g = C.load("energy_burrows10_12x16x12")
+    p, d = _dev(ttgpu, g)
+    try:
+        _burrows(d, g)
+        st, _ = d.coupled_energy(float(g["delt"]), int(g["iii"]))
+        assert st.energy_iter == int(g["scal_energy_iter"])
+        assert st.sor_iter_last == int(g["scal_SOR_ITER"])
+        _cmp(g, p, d, rel=1e-12, scaled_tol=1e-10)
+    finally:
+        d.close()
+
+
+def test_kappa_burrows_table_stops_on_nonpositive_temperature(ttgpu):
+    """BurrowsOPC stops on T <= 0 (pprphy.f:1039-1045): a zero temperature
+    planted from the host makes KAPPA return a positive code; the kernels
+    still exit normally."""
+    g = C.load("kappa_burrows14_12x16x12")
+    p, d = _dev(ttgpu, g)
+    try:
+        _burrows(d, g)
+        d.bounds()
+        T = d.download("T")
+        T[3, 4, 5] = 0.0
+        d.upload("T", T)
+        with pytest.raises(ttgpu.TTError, match="BURROWS"):
+            d.kappa()
+    finally:
+        d.close()
+
+
+def test_step_burrows_table_32x64x32_vs_oracle(ttgpu):
+    """One full step with the Burrows tables (IKAPTYP 10, the synthetic
+    table) at 32x64x32 with a 5% day-night contrast, against the oracle."""
+    from oracle.ttoracle import Oracle
+    from ttcode_amd import opacity
+    p = P.make_params(32, 64, 32, epsmat=1e-8, ikaptyp=10)
+    st = P.generate_state(p, seed=71, contrast=0.05)
+    tab = opacity.synthetic_burrows_table(2)[2:]
+    o = Oracle(p, threads=1)
+    o.set_burrows_table(*tab)
+    o.load_state(st)
+    o.bounds()
+    o.pressg()
+    dt = o.schritt()
+    o.load_state(st)
+    so, _ = o.step(dt, 1)
+    d = ttgpu.Device(p)
+    try:
+        d.set_burrows_table(*tab)
+        d.load_state(st)
+        sg, _ = d.step(dt, 1)
+        assert sg.energy.energy_iter == so.energy.energy_iter
+        same = sg.energy.sor_iter_total == so.energy.sor_iter_total
+        assert same or abs(sg.energy.sor_iter_total - so.energy.sor_iter_total) <= so.energy.energy_iter
+        for k in ("RH", "V", "G", "H", "T", "ER", "xkapR", "dxkapP_dT"):
+            reg = C.compare_region(k, p.nx, p.ny, p.nz)
+            e = C.rel_err(d.download(k)[reg], o.download(k)[reg])
+            tol = 1e-12 if same or k in ("RH", "V", "G", "H") else 1e-7
+            print(f"  {k}: {e:.2e}")
+            assert e <= tol, (k, e)
+    finally:
+        d.close()
+
+
 def test_step_freedman_table_32x64x32_vs_oracle(ttgpu):
     """One full step with table opacities (IKAPTYP 4, the synthetic table)
     at 32x64x32 with a 5% day-night contrast, against the oracle."""

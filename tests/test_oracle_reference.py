@@ -334,3 +334,67 @@ def test_schritt_viscous_and_artificial_limits():
     assert zcf * p.fdelt < d_cfl  # the viscous bound is the one that acts
     np.testing.assert_allclose(d_vis, want, rtol=1e-14)
     np.testing.assert_allclose(d_all, min(d_vis / p.fdelt, dav) * p.fdelt, rtol=1e-15)
+
+
+def _burrows(o, g):
+    o.set_burrows_table(g["tab_planck"], g["tab_ross"], g["tab_abs"])
+
+
+def test_burrows_table_reader(tmp_path):
+    """READBURROWSOPCTABLE restated (ttcode_amd/opacity.py): rows in file
+    order, and the per-IKAPTYP rewrites: 12 and 14 absorption = Planck +
+    4 log10(gamma_OPC), 14 Rosseland = Planck, 11 the extra absorber where
+    the row's pressure is at most 0.03 bar.  The fixtures' tables (read for
+    IKAPTYP 11 and 14) are what the reference's KAPPA outputs were computed
+    from (the KAPPA tests below pin them bit for bit)."""
+    import math
+    from ttcode_amd import opacity as O
+    g = C.load("kappa_burrows11_12x16x12")
+    fn = tmp_path / "BURROWS6117_woutTiVO.dat"
+    fn.write_text(str(g["tab_text"]))
+    t10 = O.read_burrows_table(str(fn), 10, mu_gas=C.params_of(g).mu_gas)
+    t11 = O.read_burrows_table(str(fn), 11, mu_gas=C.params_of(g).mu_gas)
+    t12 = O.read_burrows_table(str(fn), 12, gamma_opc=0.1)
+    t14 = O.read_burrows_table(str(fn), 14, gamma_opc=0.1)
+    for q, k in enumerate(("tab_planck", "tab_ross", "tab_abs")):
+        np.testing.assert_array_equal(t11[q], g[k], err_msg=k)
+        np.testing.assert_array_equal(t14[q], C.load("kappa_burrows14_12x16x12")[k], err_msg=k)
+    assert t10[3] == 6117.0
+    np.testing.assert_array_equal(t11[0], t10[0])
+    np.testing.assert_array_equal(t11[1], t10[1])
+    changed = t11[2] != t10[2]
+    assert 0 < np.count_nonzero(changed) < t10[2].size  # both sides of the 0.03 bar test
+    np.testing.assert_allclose(10.0 ** t11[2][changed] - 10.0 ** t10[2][changed], float(np.float32(0.1)),
+                               rtol=1e-9)
+    np.testing.assert_array_equal(t12[1], t10[1])
+    np.testing.assert_array_equal(t12[2], t10[0] + 4.0 * math.log10(0.1))
+    np.testing.assert_array_equal(t14[1], t10[0])
+    np.testing.assert_array_equal(t14[2], t12[2])
+    with pytest.raises(ValueError, match="grey Burrows"):
+        O.read_burrows_table(str(fn), 21)
+
+
+@pytest.mark.parametrize("name", ["kappa_burrows11_12x16x12", "kappa_burrows14_12x16x12"])
+def test_kappa_burrows_table(name):
+    """KAPPA with IKAPTYP 11 and 14 (BurrowsOPC up to upperbnd and the
+    upperbnd values above it, pprphy.f:199-274,1005-1225) and SUBSIG, on a
+    state beyond both clamps in T and in rho: bit for bit."""
+    def act(o, g):
+        _burrows(o, g)
+        o.bounds()
+        o.kappa()
+    g, p, o, _ = _run(name, act)
+    T, RH = g["in_T"], g["in_RH"]
+    assert T.min() < 49.99 and T.max() > 5000.0
+    assert RH.min() < 1e-12 and RH.max() > 1.01e-2
+    _check_fields(g, p, o)
+
+
+def test_coupled_energy_burrows_table():
+    def act(o, g):
+        _burrows(o, g)
+        return o.coupled_energy(float(g["delt"]), int(g["iii"]))[0]
+    g, p, o, st = _run("energy_burrows10_12x16x12", act)
+    assert st.energy_iter == int(g["scal_energy_iter"])
+    assert st.sor_iter_last == int(g["scal_SOR_ITER"])
+    _check_fields(g, p, o)

@@ -99,6 +99,8 @@ struct Lib {
     double *DE1 = nullptr;          // first Newton correction of the last step (warm start)
     double* ftab = nullptr;         // IKAPTYP 4/5 tables on the device and 10^ of them (6 x 42 x 18)
     int ftab_version = -1;          // version of the host tables ftab holds
+    double* btab = nullptr;         // IKAPTYP 9-14 tables and log10 T axis, then 10^ of them (2 x (3 x 50 x 50 + 50))
+    int btab_version = -1;
     int have_de1 = 0;
     double* partial = nullptr;      // block partial sums (reductions)
     int npartial = 0;

@@ -12,6 +12,7 @@
 // Q/ER boundary blocks.
 #include <cmath>
 #include <cstring>
+#include <vector>
 
 #include "tt_common.h"
 
@@ -74,7 +75,7 @@ __constant__ double c_fp[TT_FREEDMAN_NP] = {0.001, 0.003, 0.01, 0.03, 0.1, 0.3, 
 // first node above each, bilinear weights as the reference forms them, the
 // log10 interpolants raised to 10^x, and dk/dT as the one-sided difference
 // of the upper T node's column.  tab: Planck, Rosseland, absorption, then
-// 10^ of each entry (k_pow10), which the difference reads.  Compute-bound:
+// 10^ of each entry (formed on the host), which the difference reads.  Compute-bound:
 // three 10^x per cell (the four of the derivative come from the 10^ table).
 __global__ void k_kappa_tab(Geo g, const int32_t* __restrict__ ub, const double* __restrict__ T,
                             const double* __restrict__ PG, const double* __restrict__ tab, double* XKR,
@@ -133,11 +134,107 @@ __global__ void k_kappa_tab(Geo g, const int32_t* __restrict__ ub, const double*
     DKA[t] = (a10[it + TT_FREEDMAN_NT * iw] - a10[it1 + TT_FREEDMAN_NT * iw]) / dt;
 }
 
-// 10^ of every table entry, once per table
-__global__ void k_pow10(const double* __restrict__ a, double* b, int n) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n) b[t] = pow(10.0, a[t]);
+// ---- IKAPTYP 9-12, 14: Burrows tables
+// Axes (ppmod.f:724-745): log10 T and log10 rho, REAL constants of DATA
+// statements into double arrays, so each is the float value widened.
+#define TT_BURROWS_LOGT \
+    1.69896f, 1.73978f, 1.78061f, 1.82143f, 1.86225f, 1.90308f, 1.94390f, 1.98473f, 2.02555f, 2.06637f, \
+    2.10720f, 2.14802f, 2.18884f, 2.22967f, 2.27049f, 2.31132f, 2.35214f, 2.39296f, 2.43379f, 2.47461f, \
+    2.51543f, 2.55626f, 2.59708f, 2.63790f, 2.67873f, 2.71955f, 2.76038f, 2.80120f, 2.84202f, 2.88285f, \
+    2.92367f, 2.96406f, 3.00488f, 3.04571f, 3.08653f, 3.12735f, 3.16818f, 3.20900f, 3.24983f, 3.29065f, \
+    3.33147f, 3.37230f, 3.41312f, 3.45394f, 3.49477f, 3.53559f, 3.57642f, 3.61724f, 3.65806f, 3.69889f
+#define TT_BURROWS_LOGRH \
+    -11.9996f, -11.7954f, -11.5913f, -11.3872f, -11.1831f, -10.9790f, -10.7748f, -10.5707f, -10.3666f, \
+    -10.1625f, -9.95837f, -9.75425f, -9.55014f, -9.34602f, -9.14190f, -8.93778f, -8.73366f, -8.52954f, \
+    -8.32543f, -8.12131f, -7.91719f, -7.71307f, -7.50895f, -7.30483f, -7.10071f, -6.89660f, -6.69248f, \
+    -6.48836f, -6.28424f, -6.08012f, -5.87600f, -5.67189f, -5.46777f, -5.26365f, -5.05953f, -4.85541f, \
+    -4.65129f, -4.44718f, -4.24479f, -4.04068f, -3.83656f, -3.63244f, -3.42875f, -3.22464f, -3.02052f, \
+    -2.81640f, -2.61228f, -2.40816f, -2.20404f, -1.99993f
+__constant__ double c_bt[TT_BURROWS_NT] = {TT_BURROWS_LOGT};
+__constant__ double c_brh[TT_BURROWS_NR] = {TT_BURROWS_LOGRH};
+static const double h_bt[TT_BURROWS_NT] = {TT_BURROWS_LOGT};
+
+// index of the first of the 50 strictly increasing nodes ax[] above x, given
+// x inside (ax[0], ax[49]) and a guess g of it - 1
+__device__ inline int first_above(const double* ax, double x, double g) {
+    int q = (int)g + 1;
+    q = q < 1 ? 1 : (q > 49 ? 49 : q);
+    while (q > 1 && ax[q - 1] > x) --q;
+    while (q < 49 && !(ax[q] > x)) ++q;
+    return q;
 }
+
+// KAPPA, IKAPTYP 9-12 and 14 (pprphy.f:199-234,242-274, MODTYP=2): BurrowsOPC
+// (pprphy.f:1005-1225) for cells up to upperbnd, whose values the cells
+// above take; SUBSIG.  One thread per cell evaluates at (min(i, upperbnd),
+// j, k): log10 T and log10 rho clamped into the table (the clamp bounds are
+// REAL constants too), the first node above each, the reference's bilinear
+// weights, 10^ of the interpolants, and dk/dT as the one-sided difference of
+// 10^ of the upper T node's column over 10^ of the two T nodes.  tab: the
+// Planck, Rosseland and absorption tables and the log10 T axis, then 10^ of
+// all of them (formed on the host).
+__global__ void k_kappa_bur(Geo g, const int32_t* __restrict__ ub, const double* __restrict__ T,
+                            const double* __restrict__ RH, const double* __restrict__ tab, double* XKR,
+                            double* XKP, double* XKA, double* DKP, double* DKA, double* SIG, double sig,
+                            Ctrl* ctrl) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.n3) return;
+    SIG[t] = sig;
+    int i, j, k;
+    box_ijk(g, t, i, j, k);
+    const int u = ub[g.idx2(j, k)];
+    const long long n = g.idx(i <= u ? i : u, j, k);
+    const double t_in = T[n];
+    if (!(t_in > 0.0)) {  // the reference stops (pprphy.f:1039-1045)
+        ctrl->bad_kappa = 1;
+        return;
+    }
+    double rh1 = log10(RH[n]);
+    double tm1 = log10(t_in);
+    if (tm1 < (double)1.69896f) tm1 = (double)1.69897f;
+    if (tm1 > (double)3.69889f) tm1 = (double)3.69888f;
+    if (rh1 < (double)-11.9996f) rh1 = (double)-11.9995f;
+    if (rh1 > (double)-1.99993f) rh1 = (double)-1.99994f;
+    if (rh1 != rh1) {  // rho < 0: no node above, the reference stops (pprphy.f:1091-1097)
+        ctrl->bad_kappa = 1;
+        return;
+    }
+    // the first node above each value (the clamps put it in 1..49): the axes
+    // are nearly uniform, so a guess from the mean spacing, then the step to
+    // the node the reference's scan finds (the axes increase strictly)
+    const int it = first_above(c_bt, tm1, (tm1 - c_bt[0]) * (49.0 / (c_bt[49] - c_bt[0])));
+    const int it1 = it - 1;
+    double ws1 = c_bt[it1] - tm1;
+    double ws = c_bt[it1] - c_bt[it];
+    ws1 = ws1 / ws;
+    ws = 1.0 - ws1;
+    const int iw = first_above(c_brh, rh1, (rh1 - c_brh[0]) * (49.0 / (c_brh[49] - c_brh[0])));
+    const int iw1 = iw - 1;
+    double ws2 = rh1 - c_brh[iw1];
+    double ws3 = c_brh[iw] - c_brh[iw1];
+    ws2 = ws2 / ws3;
+    ws3 = 1.0 - ws2;
+    const int TN = TT_BURROWS_NT * TT_BURROWS_NR;
+    double out[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const double* tq = tab + q * TN;
+        const double a00 = tq[it + TT_BURROWS_NT * iw], a10 = tq[it + TT_BURROWS_NT * iw1];
+        const double a11 = tq[it1 + TT_BURROWS_NT * iw1], a01 = tq[it1 + TT_BURROWS_NT * iw];
+        const double w4 = ws2 * (ws1 * a00 + ws * a01);
+        const double w5 = ws3 * (ws1 * a10 + ws * a11);
+        out[q] = exp10(w4 + w5);  // 10.d0**x, within a few ulp
+    }
+    XKP[t] = out[0];
+    XKR[t] = out[1];
+    XKA[t] = out[2];
+    const double* p10 = tab + 3 * TN + TT_BURROWS_NT;  // 10^ of the tables and of the T axis
+    const double* t10 = p10 + 3 * TN;
+    const double dt = t10[it] - t10[it1];
+    DKP[t] = (p10[it + TT_BURROWS_NT * iw] - p10[it1 + TT_BURROWS_NT * iw]) / dt;
+    DKA[t] = (p10[2 * TN + it + TT_BURROWS_NT * iw] - p10[2 * TN + it1 + TT_BURROWS_NT * iw]) / dt;
+}
+
 
 // CALCSTELLARTAU (pprad.f:191-221): tau* by a reverse scan along r, one
 // thread per (j,k) column, summed in the reference's order; the products
@@ -549,6 +646,22 @@ __global__ void k_energy_final(double* DER, double* T, const double* __restrict_
 // opc_init module; KAPPA copies them to the device when they have changed.
 static double g_ftab[3 * TT_FREEDMAN_NT * TT_FREEDMAN_NP];
 static int g_ftab_version = 0;  // 0: never set
+// IKAPTYP 9-14 tables likewise, followed by the log10 T axis
+static double g_btab[3 * TT_BURROWS_NT * TT_BURROWS_NR + TT_BURROWS_NT];
+static int g_btab_version = 0;
+
+// a table followed by 10^ of each entry, the powers from the host's libm
+// (the reference's 10.d0**x): the derivatives difference neighbouring 10^
+// values, which can cancel, so they are formed from the same powers the
+// reference forms
+static std::vector<double> with_pow10(const double* a, int n) {
+    std::vector<double> t(2 * (size_t)n);
+    for (int q = 0; q < n; ++q) {
+        t[q] = a[q];
+        t[n + q] = std::pow(10.0, a[q]);
+    }
+    return t;
+}
 
 static int kappa_impl() {
     double sig;
@@ -559,11 +672,9 @@ static int kappa_impl() {
     if (L.p.ikaptyp == 4 || L.p.ikaptyp == 5) {
         if (g_ftab_version == 0) return fail(1, "KAPPA: IKAPTYP 4 and 5 need tt_set_opacity_table first");
         if (L.ftab_version != g_ftab_version) {
-            const int n = 3 * TT_FREEDMAN_NT * TT_FREEDMAN_NP;
-            TT_HIP(hipMemcpyAsync(L.ftab, g_ftab, sizeof g_ftab, hipMemcpyHostToDevice, L.s));
-            k_pow10<<<nblk(n, 256), 256, 0, L.s>>>(L.ftab, L.ftab + n, n);
-            TT_LAUNCH();
-            TT_HIP(hipStreamSynchronize(L.s));  // g_ftab may change after this call
+            const std::vector<double> t = with_pow10(g_ftab, 3 * TT_FREEDMAN_NT * TT_FREEDMAN_NP);
+            TT_HIP(hipMemcpyAsync(L.ftab, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice, L.s));
+            TT_HIP(hipStreamSynchronize(L.s));  // t is released after this call
             L.ftab_version = g_ftab_version;
         }
         int rc = pressg_impl();  // pprphy.f:88
@@ -574,7 +685,21 @@ static int kappa_impl() {
         TT_LAUNCH();
         return 0;
     }
-    if (L.p.ikaptyp != 1 && L.p.ikaptyp != 26) return fail(1, "KAPPA: IKAPTYP must be 1, 4, 5 or 26");
+    if ((L.p.ikaptyp >= 9 && L.p.ikaptyp <= 12) || L.p.ikaptyp == 14) {
+        if (g_btab_version == 0) return fail(1, "KAPPA: IKAPTYP 9-12 and 14 need tt_set_burrows_table first");
+        if (L.btab_version != g_btab_version) {
+            const std::vector<double> t = with_pow10(g_btab, 3 * TT_BURROWS_NT * TT_BURROWS_NR + TT_BURROWS_NT);
+            TT_HIP(hipMemcpyAsync(L.btab, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice, L.s));
+            TT_HIP(hipStreamSynchronize(L.s));  // t is released after this call
+            L.btab_version = g_btab_version;
+        }
+        k_kappa_bur<<<nblk(L.g.n3, 256), 256, 0, L.s>>>(L.g, L.ub, f[TT_T], f[TT_RH], L.btab, f[TT_XKAPR],
+                                                       f[TT_XKAPP], f[TT_XKAPA], f[TT_DXKAPP_DT],
+                                                       f[TT_DXKAPA_DT], f[TT_SIG], sig, L.ctrl);
+        TT_LAUNCH();
+        return 0;
+    }
+    if (L.p.ikaptyp != 1 && L.p.ikaptyp != 26) return fail(1, "KAPPA: IKAPTYP must be 1, 4, 5, 9-12, 14 or 26");
     k_kappa<<<nblk(L.g.n3, 256), 256, 0, L.s>>>(L.g, L.p, L.ub, f[TT_RH], f[TT_T], f[TT_XKAPR], f[TT_XKAPP],
                                                 f[TT_XKAPA], f[TT_DXKAPP_DT], f[TT_DXKAPA_DT], f[TT_SIG], sig);
     TT_LAUNCH();
@@ -785,6 +910,17 @@ extern "C" int tt_set_opacity_table(const double* kap_planck, const double* kap_
     memcpy(g_ftab + n, kap_ross, n * sizeof(double));
     memcpy(g_ftab + 2 * n, kap_abs, n * sizeof(double));
     ++g_ftab_version;
+    return 0;
+}
+
+extern "C" int tt_set_burrows_table(const double* kap_planck, const double* kap_ross, const double* kap_abs) {
+    if (!kap_planck || !kap_ross || !kap_abs) return fail(1, "tt_set_burrows_table: null table");
+    const int n = TT_BURROWS_NT * TT_BURROWS_NR;
+    memcpy(g_btab, kap_planck, n * sizeof(double));
+    memcpy(g_btab + n, kap_ross, n * sizeof(double));
+    memcpy(g_btab + 2 * n, kap_abs, n * sizeof(double));
+    memcpy(g_btab + 3 * n, h_bt, sizeof h_bt);
+    ++g_btab_version;
     return 0;
 }
 

@@ -75,6 +75,11 @@ module ttgpu_c
        import :: c_int, c_double
        real(c_double), intent(in) :: kp(42, 18), kr(42, 18), ka(42, 18)
      end function tt_set_opacity_table
+     ! BurrowsXKAP_plnk/_ross/_abs(50,50) after READBURROWSOPCTABLE
+     integer(c_int) function tt_set_burrows_table(kp, kr, ka) bind(C, name='tt_set_burrows_table')
+       import :: c_int, c_double
+       real(c_double), intent(in) :: kp(50, 50), kr(50, 50), ka(50, 50)
+     end function tt_set_burrows_table
      integer(c_int) function tt_kappa() bind(C, name='tt_kappa')
        import :: c_int
      end function tt_kappa

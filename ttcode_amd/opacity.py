@@ -91,3 +91,94 @@ def synthetic_table(seed: int = 0):
     ka = kr - 0.5 + 0.2 * np.cos(2.0 * np.log10(np.asarray(FREEDMAN_T) / 100.0))[None, :] \
         + 0.02 * rng.standard_normal((NP, NT))
     return kp.reshape(-1), kr.reshape(-1), ka.reshape(-1)
+
+
+# ---- IKAPTYP 9-12 and 14: the Burrows tables
+# Axes (ppmod.f:724-745): log10 T and log10 rho on 50 nodes each, REAL
+# constants widened to double.
+BURROWS_LOGT = tuple(float(np.float32(v)) for v in (
+    1.69896, 1.73978, 1.78061, 1.82143, 1.86225, 1.90308, 1.94390, 1.98473, 2.02555, 2.06637,
+    2.10720, 2.14802, 2.18884, 2.22967, 2.27049, 2.31132, 2.35214, 2.39296, 2.43379, 2.47461,
+    2.51543, 2.55626, 2.59708, 2.63790, 2.67873, 2.71955, 2.76038, 2.80120, 2.84202, 2.88285,
+    2.92367, 2.96406, 3.00488, 3.04571, 3.08653, 3.12735, 3.16818, 3.20900, 3.24983, 3.29065,
+    3.33147, 3.37230, 3.41312, 3.45394, 3.49477, 3.53559, 3.57642, 3.61724, 3.65806, 3.69889))
+BURROWS_LOGRH = tuple(float(np.float32(v)) for v in (
+    -11.9996, -11.7954, -11.5913, -11.3872, -11.1831, -10.9790, -10.7748, -10.5707, -10.3666,
+    -10.1625, -9.95837, -9.75425, -9.55014, -9.34602, -9.14190, -8.93778, -8.73366, -8.52954,
+    -8.32543, -8.12131, -7.91719, -7.71307, -7.50895, -7.30483, -7.10071, -6.89660, -6.69248,
+    -6.48836, -6.28424, -6.08012, -5.87600, -5.67189, -5.46777, -5.26365, -5.05953, -4.85541,
+    -4.65129, -4.44718, -4.24479, -4.04068, -3.83656, -3.63244, -3.42875, -3.22464, -3.02052,
+    -2.81640, -2.61228, -2.40816, -2.20404, -1.99993))
+BNT = BNR = 50
+# planet_num 1 (HD209458b) reads these for IKAPTYP 9 and for 10-12, 14
+# (OPENOPACITYFILE_WITH_TIOVO / _WOTIOVO, ppplanets.f:90-213)
+BURROWS_FILES = {9: "BURROWS6117_wTiVO.dat", 10: "BURROWS6117_woutTiVO.dat", 11: "BURROWS6117_woutTiVO.dat",
+                 12: "BURROWS6117_woutTiVO.dat", 14: "BURROWS6117_woutTiVO.dat"}
+# SETPLANETKAPEXTRA (ppplanets.f:36-59), REAL constants: the extra absorber
+# of IKAPTYP 11 in cm^2/g by planet_num
+PLANET_KAP_EXTRA = {1: float(np.float32(0.1)), 2: float(np.float32(0.035)), 3: 0.0, 4: 0.0, 5: 0.0}
+RGAS = float(np.float32(8.315e7))  # Rgas (global_constants), a REAL constant
+
+
+def read_burrows_table(path, ikaptyp, gamma_opc=0.1, mu_gas=2.3, planet_num=1):
+    """READBURROWSOPCTABLE (pprphy.f:1227-1520) for IKAPTYP 9-12 and 14.
+
+    Ten header records, one with the stellar temperature, then 2500 rows
+    ``T rho log_kP log_kR log_kABS`` read in the order (T outer, rho inner)
+    into the tables whatever T and rho the row holds.  Then, per row, as the
+    reference does it: IKAPTYP 12 and 14 set log kABS = log kP +
+    4 log10(gamma_OPC); 14 sets log kR = log kP; 11 adds the planet's extra
+    absorber, log kABS = log10(10^log kABS + kappa_extra), where the row's
+    P = Rgas T rho / mu_gas 10^-6 bar is at most 0.03 (a REAL constant).
+
+    Returns ``(kap_planck, kap_ross, kap_abs, t_star)``: 50 x 50 values each,
+    temperature fastest (BurrowsXKAP_*(IT, IR) in Fortran order).
+    """
+    import math
+    if ikaptyp not in BURROWS_FILES:
+        raise ValueError(f"IKAPTYP {ikaptyp} is not a grey Burrows table (9-12, 14)")
+    recs = (r for r in _records(path) if r)
+    for _ in range(10):
+        next(recs)
+    t_star = _number(next(recs)[0])
+    tabs = np.full((3, BNR, BNT), MISSING)
+    kap_ext = PLANET_KAP_EXTRA[planet_num] if ikaptyp == 11 else 0.0
+    lim = float(np.float32(0.03))
+    for it in range(BNT):
+        for ir in range(BNR):
+            vals = []
+            while len(vals) < 5:
+                try:
+                    vals += [_number(t) for t in next(recs)]
+                except StopIteration:
+                    raise ValueError(f"{path}: {it * BNR + ir} of {BNT * BNR} table rows") from None
+            t, rh, kp, kr, ka = vals[:5]
+            if ikaptyp in (12, 14):
+                ka = kp + 4.0 * math.log10(gamma_opc)
+            if ikaptyp == 14:
+                kr = kp
+            if ikaptyp == 11:
+                pg_test = (RGAS * t * rh / mu_gas) * (10.0 ** (-6.0))
+                if pg_test <= lim:
+                    ka = math.log10(10.0 ** ka + kap_ext)
+            tabs[:, ir, it] = (kp, kr, ka)
+    kp, kr, ka = (tabs[q].reshape(-1) for q in range(3))
+    return kp, kr, ka, t_star
+
+
+def synthetic_burrows_table(seed: int = 0):
+    """A smooth synthetic table on the Burrows grid: log10 kappa_R = -1.5 +
+    0.5 log10(rho / 1e-6) + 1.5 log10(T / 1500 K) + 0.05 sin(2 log10 rho),
+    Planck 0.25 dex above it, absorption 0.4 dex below it with a ripple in
+    log T, each with seeded 0.02 dex noise.  Returns ``(log_t, log_rh,
+    kap_planck, kap_ross, kap_abs)``: the grid nodes and three tables,
+    temperature fastest."""
+    rng = np.random.default_rng(seed)
+    lt = np.asarray(BURROWS_LOGT)[None, :]
+    lr = np.asarray(BURROWS_LOGRH)[:, None]
+    kr = -1.5 + 0.5 * (lr + 6.0) + 1.5 * (lt - np.log10(1500.0)) + 0.05 * np.sin(2.0 * lr) \
+        + 0.02 * rng.standard_normal((BNR, BNT))
+    kp = kr + 0.25 + 0.02 * rng.standard_normal((BNR, BNT))
+    ka = kr - 0.4 + 0.15 * np.cos(3.0 * lt) + 0.02 * rng.standard_normal((BNR, BNT))
+    return (np.asarray(BURROWS_LOGT), np.asarray(BURROWS_LOGRH), kp.reshape(-1), kr.reshape(-1),
+            ka.reshape(-1))

@@ -55,6 +55,7 @@ def load_library():
     L.tt_varlimit.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double]
     L.tt_sphericalvellimit.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double]
     L.tt_set_opacity_table.argtypes = [ctypes.c_void_p] * 3
+    L.tt_set_burrows_table.argtypes = [ctypes.c_void_p] * 3
     L.tt_set_viscosity.argtypes = [ctypes.POINTER(P.Viscosity)]
     L.tt_get_viscosity.argtypes = [ctypes.POINTER(P.Viscosity)]
     L.tt_get_dt_artvis.argtypes = [ctypes.POINTER(ctypes.c_double)]
@@ -182,6 +183,16 @@ class Device:
             raise ValueError("opacity tables must hold 42 x 18 values")
         self._check(self.L.tt_set_opacity_table(*(a.ctypes.data_as(ctypes.c_void_p) for a in t)),
                     "tt_set_opacity_table")
+
+    def set_burrows_table(self, kap_planck, kap_ross, kap_abs):
+        """The log10 tables of IKAPTYP 9-12/14 on the 50 x 50 (log T,
+        log rho) grid, temperature fastest, as READBURROWSOPCTABLE leaves
+        them (``ttcode_amd.opacity.read_burrows_table``)."""
+        t = [np.ascontiguousarray(np.asarray(a, np.float64).reshape(-1)) for a in (kap_planck, kap_ross, kap_abs)]
+        if any(a.size != 50 * 50 for a in t):
+            raise ValueError("Burrows tables must hold 50 x 50 values")
+        self._check(self.L.tt_set_burrows_table(*(a.ctypes.data_as(ctypes.c_void_p) for a in t)),
+                    "tt_set_burrows_table")
 
     def set_viscosity(self, ivis=1, ivistyp=1, ieta=4, idiss=1, xnue=1e9, iartvis=0):
         """ETACALC + VISC inside FORGAS3D (ppfor.f:170-174), ivis 0 is off;
