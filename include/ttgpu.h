@@ -47,7 +47,7 @@ typedef struct tt_params {
     int32_t iflimtyp;        /* flux limiter: 1 = 1/3, 2 = Levermore-Pomraning */
     int32_t f_incident;      /* 0 = no irradiation, 2 = implicit cos heating  */
     int32_t ikaptyp;         /* 1 = constant ZKAPMULT, 4/5 and 9-12/14 tables,
-                                26 = power law (new)                          */
+                                13 = constant with gamma_opc, 26 = power law  */
     int32_t isigtyp;         /* 0 = no scattering, 1 = 0.4 cm^2/g             */
     int32_t isoth;           /* 0 = energy equation on                        */
     int32_t itmax;           /* ITMAX, shared by SOR and Newton (ipp:13)      */
@@ -68,6 +68,8 @@ typedef struct tt_params {
     double sorparam;         /* SORPARAM (omega)                              */
     double epsmat;           /* EPSMAT, L1 during-sweep tolerance             */
     double fdelt, dtmax;     /* FDELT (CFL factor), DTMAX                     */
+    double gamma_opc;        /* gamma_OPC = (kappa_A/kappa_P)^(1/4) of IKAPTYP 13
+                                (and of the 12/14 tables' rewrite, ipp:11)    */
 } tt_params;
 
 /* ------------------------------------------------------------------------

@@ -48,7 +48,7 @@ program tt_ref_harness
   integer :: h_nx, h_ny, h_nz, h_bcxmin, h_ibdrad, h_iflimtyp, h_f_incident
   integer :: h_ikaptyp, h_isigtyp, h_itmax, h_iii, h_nsteps
   double precision :: h_xmin, h_xmax, h_ymin, h_ymax, h_zmin, h_zmax
-  double precision :: h_gamma, h_mu_gas, h_zkapmult, h_tirr, h_fbottom, h_d_t
+  double precision :: h_gamma, h_mu_gas, h_zkapmult, h_tirr, h_fbottom, h_d_t, h_gamma_opc
   double precision :: h_omrot, h_sorparam, h_epsmat, h_fdelt, h_dtmax, h_delt
   double precision :: h_xnue, h_pr_num
   integer :: h_prefill_kappa, h_num_iter, h_ivis, h_ivistyp, h_ieta, h_idiss, h_iartvis
@@ -60,7 +60,7 @@ program tt_ref_harness
   namelist /ttin/ h_nx, h_ny, h_nz, h_bcxmin, h_ibdrad, h_iflimtyp, &
        h_f_incident, h_ikaptyp, h_isigtyp, h_itmax, h_iii, h_nsteps, &
        h_xmin, h_xmax, h_ymin, h_ymax, h_zmin, h_zmax, h_gamma, h_mu_gas, &
-       h_zkapmult, h_tirr, h_fbottom, h_d_t, h_omrot, h_sorparam, h_epsmat, &
+       h_zkapmult, h_tirr, h_fbottom, h_d_t, h_omrot, h_sorparam, h_epsmat, h_gamma_opc, &
        h_fdelt, h_dtmax, h_delt, h_xnue, h_pr_num, h_prefill_kappa, cmd, &
        h_zeit, h_num_iter, h_tinitbottom, h_tinittop, h_rhinittop, h_rhinitbottom, &
        h_hydrotime, h_maccreated, h_ivis, h_ivistyp, h_ieta, h_idiss, h_iartvis
@@ -71,7 +71,7 @@ program tt_ref_harness
   h_nsteps = 1
   h_zeit = 0.d0; h_num_iter = 0; h_tinitbottom = 0.d0; h_tinittop = 0.d0
   h_rhinittop = 0.d0; h_rhinitbottom = 0.d0; h_hydrotime = 0.d0; h_maccreated = 0.d0
-  h_ivis = 0; h_ivistyp = 1; h_ieta = 4; h_idiss = 0; h_iartvis = 0
+  h_ivis = 0; h_ivistyp = 1; h_ieta = 4; h_idiss = 0; h_iartvis = 0; h_gamma_opc = 0.1d0
   h_prefill_kappa = 0
   h_xnue = 1.d9
   h_pr_num = 1.d0
@@ -93,7 +93,7 @@ program tt_ref_harness
   IVEL = 0; VELMAX = 6.d5; VELDAMP = 0.d0; DAMP_STEPS = 100000
   IVISTYP = h_ivistyp; XNUE = h_xnue; IDISS = h_idiss; ALPHA = 1.d-4; IETA = h_ieta; IARTVIS = h_iartvis; PR_NUM = h_pr_num
   IGASTYP = 1; GAMMA = h_gamma; mu_gas = h_mu_gas; ICSTYP = 0; aspect = 0.04d0
-  IKAPTYP = h_ikaptyp; ISIGTYP = h_isigtyp; ZKAPMULT = h_zkapmult; gamma_OPC = 0.1d0
+  IKAPTYP = h_ikaptyp; ISIGTYP = h_isigtyp; ZKAPMULT = h_zkapmult; gamma_OPC = h_gamma_opc
   IROTATE = 0; PROT = 3.52d0; ramp_start = 0.d0; ramp_full = 0.d0; windAMP = 1.d0
   IBDRAD = h_ibdrad; IFLIMTYP = h_iflimtyp; F_INCIDENT = h_f_incident
   BCXMIN = h_bcxmin; Fbot = h_fbottom

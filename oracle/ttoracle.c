@@ -934,7 +934,15 @@ static int kappa(void) /* KAPPA, pprphy.f:1-360; case 26 is new */
         }
         return 0;
     }
-    if (p->ikaptyp != 26) return seterr(1, "KAPPA: IKAPTYP must be 1, 4, 5, 9-12, 14 or 26");
+    if (p->ikaptyp == 13) { /* pprphy.f:235-241 */
+        const double xa = pow(p->gamma_opc, 4.0) * p->zkapmult;
+        for (size_t n = 0; n < O.n3; ++n) {
+            O.XKR[n] = p->zkapmult; O.XKP[n] = p->zkapmult; O.XKA[n] = xa;
+            O.DKP[n] = 0.0; O.DKA[n] = 0.0;
+        }
+        return 0;
+    }
+    if (p->ikaptyp != 26) return seterr(1, "KAPPA: IKAPTYP must be 1, 4, 5, 9-14 or 26");
     /* Power law kappa_R = kappa_P = kappa_A = kap0 (rho/rho0)^a (T/T0)^b in the
      * loop structure of the IKAPTYP=8 power law (pprphy.f:176-198): cells up to
      * upperbnd are evaluated, cells above copy the upperbnd value.  Unlike

@@ -47,7 +47,7 @@ def _nml(p: P.TTParams, cmd, delt, iii, nsteps, prefill, xnue=1e9, pr_num=1.0, e
                h_ikaptyp=v["ikaptyp"], h_isigtyp=v["isigtyp"], h_itmax=v["itmax"],
                h_iii=iii, h_nsteps=nsteps, h_xmin=v["xmin"], h_xmax=v["xmax"],
                h_ymin=v["ymin"], h_ymax=v["ymax"], h_zmin=v["zmin"], h_zmax=v["zmax"],
-               h_gamma=v["gamma"], h_mu_gas=v["mu_gas"], h_zkapmult=v["zkapmult"],
+               h_gamma=v["gamma"], h_mu_gas=v["mu_gas"], h_zkapmult=v["zkapmult"], h_gamma_opc=v.get("gamma_opc", 0.1),
                h_tirr=v["tirr"], h_fbottom=v["fbottom"], h_d_t=v["d_t"], h_omrot=v["omrot"],
                h_sorparam=v["sorparam"], h_epsmat=v["epsmat"], h_fdelt=v["fdelt"],
                h_dtmax=v["dtmax"], h_delt=delt, h_xnue=xnue, h_pr_num=pr_num,
@@ -411,6 +411,25 @@ def burrows():
                  dict(tab, delt=dt, iii=2))
 
 
+def const13():
+    """COUPLED_ENERGY_EQS with IKAPTYP 13 (pprphy.f:235-241: kappa_R =
+    kappa_P = ZKAPMULT, kappa_A = gamma_OPC^4 ZKAPMULT), gamma_OPC = 0.3, so
+    that the absorption and Planck means differ."""
+    nx, ny, nz = 12, 16, 12
+    o = Oracle(P.make_params(nx, ny, nz, ikaptyp=1, kap0=1e-2))
+    o.load_state(base_inputs(o.p))
+    o.bounds()
+    o.pressg()
+    dt = o.schritt()
+    p = P.make_params(nx, ny, nz, ikaptyp=13, kap0=1e-1, epsmat=1e-8, tirr=2000.0)
+    p.gamma_opc = 0.3
+    inp = base_inputs(p, seed=73, contrast=0.05)
+    outs, sc = run_ref("energy", p, inp, delt=dt, iii=2)
+    save("energy_const13_12x16x12", p, inp, outs, sc,
+         ["T", "ER", "delta_ER", "xkapR", "xkapP", "xkapA", "dxkapP_dT", "dxkapA_dT", "STELLARINPUT",
+          "FLIMX", "FQ", "FR", "RHS_ER", "upperbnd"], {"delt": dt, "iii": 2})
+
+
 def viscosity():
     """FORGAS3D with viscosity (ppfor.f:170-174: ETACALC, ppvisc.f:375-484;
     VISC, ppvisc.f:2-373 with IVISTYP=1 and dissipative heating) after the
@@ -524,6 +543,8 @@ if __name__ == "__main__":
         artvis()
     elif sys.argv[1:] == ["burrows"]:
         burrows()
+    elif sys.argv[1:] == ["const13"]:
+        const13()
     elif sys.argv[1:] == ["variants"]:
         variants()
     elif sys.argv[1:] == ["adjust"]:

@@ -295,6 +295,19 @@ def test_kappa_burrows_table_stops_on_nonpositive_temperature(ttgpu):
         d.close()
 
 
+def test_coupled_energy_constant_gamma_opacity(ttgpu):
+    """IKAPTYP 13 (kappa_A = gamma_OPC^4 ZKAPMULT) against the reference."""
+    g = C.load("energy_const13_12x16x12")
+    p, d = _dev(ttgpu, g)
+    try:
+        st, _ = d.coupled_energy(float(g["delt"]), int(g["iii"]))
+        assert st.energy_iter == int(g["scal_energy_iter"])
+        assert st.sor_iter_last == int(g["scal_SOR_ITER"])
+        _cmp(g, p, d, rel=1e-12, scaled_tol=1e-10)
+    finally:
+        d.close()
+
+
 def test_step_burrows_table_32x64x32_vs_oracle(ttgpu):
     """One full step with the Burrows tables (IKAPTYP 10, the synthetic
     table) at 32x64x32 with a 5% day-night contrast, against the oracle."""

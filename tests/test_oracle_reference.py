@@ -398,3 +398,15 @@ def test_coupled_energy_burrows_table():
     assert st.energy_iter == int(g["scal_energy_iter"])
     assert st.sor_iter_last == int(g["scal_SOR_ITER"])
     _check_fields(g, p, o)
+
+
+def test_coupled_energy_constant_gamma_opacity():
+    """COUPLED_ENERGY_EQS with IKAPTYP 13 (kappa_A = gamma_OPC^4 ZKAPMULT,
+    pprphy.f:235-241), gamma_OPC = 0.3: bit for bit."""
+    g, p, o, st = _run("energy_const13_12x16x12", lambda o, g: o.coupled_energy(float(g["delt"]), int(g["iii"]))[0])
+    assert p.ikaptyp == 13 and p.gamma_opc == 0.3
+    assert st.energy_iter == int(g["scal_energy_iter"])
+    assert st.sor_iter_last == int(g["scal_SOR_ITER"])
+    xa = g["out_xkapA"][2:-2, 2:-2, 2:-2]
+    assert np.all(xa == 0.3 ** 4.0 * p.zkapmult)
+    _check_fields(g, p, o)

@@ -699,7 +699,17 @@ static int kappa_impl() {
         TT_LAUNCH();
         return 0;
     }
-    if (L.p.ikaptyp != 1 && L.p.ikaptyp != 26) return fail(1, "KAPPA: IKAPTYP must be 1, 4, 5, 9-12, 14 or 26");
+    if (L.p.ikaptyp == 13) {  // constants (pprphy.f:235-241); gamma_OPC**4.d0 from the host's libm
+        const double xa = std::pow(L.p.gamma_opc, 4.0) * L.p.zkapmult;
+        const double v[6] = {L.p.zkapmult, L.p.zkapmult, xa, 0.0, 0.0, sig};
+        const int id[6] = {TT_XKAPR, TT_XKAPP, TT_XKAPA, TT_DXKAPP_DT, TT_DXKAPA_DT, TT_SIG};
+        for (int q = 0; q < 6; ++q) {
+            k_fill_box<<<nblk(L.g.n3, 256), 256, 0, L.s>>>(f[id[q]], v[q], L.g.n3);
+            TT_LAUNCH();
+        }
+        return 0;
+    }
+    if (L.p.ikaptyp != 1 && L.p.ikaptyp != 26) return fail(1, "KAPPA: IKAPTYP must be 1, 4, 5, 9-14 or 26");
     k_kappa<<<nblk(L.g.n3, 256), 256, 0, L.s>>>(L.g, L.p, L.ub, f[TT_RH], f[TT_T], f[TT_XKAPR], f[TT_XKAPP],
                                                 f[TT_XKAPA], f[TT_DXKAPP_DT], f[TT_DXKAPA_DT], f[TT_SIG], sig);
     TT_LAUNCH();

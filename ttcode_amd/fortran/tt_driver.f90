@@ -29,7 +29,7 @@ program tt_driver
   p%isigtyp = 0; p%isoth = 0; p%itmax = h_itmax; p%pad0 = 0
   p%xmin = 7149199872.d0; p%xmax = 1.1d0 * 7149199872.d0
   p%ymin = 0.d0; p%ymax = 1.d0; p%zmin = -0.5d0; p%zmax = 0.5d0
-  p%gamma = 1.4d0; p%mu_gas = 2.3d0
+  p%gamma = 1.4d0; p%mu_gas = 2.3d0; p%gamma_opc = 0.1d0
   p%zkapmult = h_kap0; p%kap0 = h_kap0; p%dkap0 = h_dkap0; p%kap_rho0 = h_rho0; p%kap_t0 = h_t0
   p%kap_a = h_a; p%kap_b = h_b; p%tirr = h_tirr; p%fbottom = 0.d0; p%d_t = 0.d0; p%omrot = 0.d0
   p%sorparam = 1.7d0; p%epsmat = h_epsmat; p%fdelt = 0.4d0; p%dtmax = 1.d12

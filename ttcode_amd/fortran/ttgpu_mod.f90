@@ -19,6 +19,7 @@ module ttgpu_c
      real(c_double) :: xmin, xmax, ymin, ymax, zmin, zmax, gamma, mu_gas
      real(c_double) :: zkapmult, kap0, dkap0, kap_rho0, kap_t0, kap_a, kap_b
      real(c_double) :: tirr, fbottom, d_t, omrot, sorparam, epsmat, fdelt, dtmax
+     real(c_double) :: gamma_opc
   end type tt_params
 
   type, bind(C) :: tt_energy_stats

@@ -50,7 +50,7 @@ class TTParams(ctypes.Structure):
         ("fbottom", ctypes.c_double), ("d_t", ctypes.c_double),
         ("omrot", ctypes.c_double), ("sorparam", ctypes.c_double),
         ("epsmat", ctypes.c_double), ("fdelt", ctypes.c_double),
-        ("dtmax", ctypes.c_double),
+        ("dtmax", ctypes.c_double), ("gamma_opc", ctypes.c_double),
     ]
 
     def as_dict(self):
@@ -251,6 +251,7 @@ def make_params(nx, ny, nz, *, epsmat=1e-8, itmax=20000, sorparam=1.7,
     p.zmin, p.zmax = -0.5, 0.5
     p.gamma, p.mu_gas = GAMMA, MU_GAS
     p.zkapmult = kap0
+    p.gamma_opc = 0.1  # the shipped ipp's gamma_OPC (ipp:11)
     p.kap0 = kap0
     p.kap_rho0 = rho0
     p.kap_t0 = T0
