@@ -47,7 +47,8 @@ typedef struct tt_params {
     int32_t iflimtyp;        /* flux limiter: 1 = 1/3, 2 = Levermore-Pomraning */
     int32_t f_incident;      /* 0 = no irradiation, 2 = implicit cos heating  */
     int32_t ikaptyp;         /* 1 = constant ZKAPMULT, 4/5 and 9-12/14 tables,
-                                13 = constant with gamma_opc, 26 = power law  */
+                                6/7 column-depth tables, 13 = constant with
+                                gamma_opc, 26 = power law                     */
     int32_t isigtyp;         /* 0 = no scattering, 1 = 0.4 cm^2/g             */
     int32_t isoth;           /* 0 = energy equation on                        */
     int32_t itmax;           /* ITMAX, shared by SOR and Newton (ipp:13)      */
@@ -388,6 +389,15 @@ int tt_sor_speculation_get(int64_t* spec_batches, int64_t* reruns);
  * (pprphy.f:944-1002).  The library keeps a copy, across tt_init and
  * tt_finalize, until the next call; it may be called before tt_init. */
 int tt_set_opacity_table(const double* kap_planck, const double* kap_ross, const double* kap_abs);
+/* IKAPTYP 6 and 7 (pprphy.f:123-175, columnOPC pprphy.f:1543-1745) take the
+ * Planck and Rosseland means from the tables above and the absorption mean
+ * from a column-depth table: log10 kappa_abs on the same (T, P) grid x 8
+ * column levels log10 kappa_cut = -26 ... -19, kap[it + 42 (ip + 18 ic)], as
+ * ColumnXKAP_abs(42,18,8) after READCOLUMNDEPTHOPCTABLE (pprphy.f:1747-1833;
+ * ttcode_amd/opacity.py read_column_table reads the text format and caps
+ * each level at level 8's value, as the reference does).  Replaces that
+ * reader's fill; kept like the other tables. */
+int tt_set_column_table(const double* kap_abs);
 
 /* ------------------------------------------------------------------------
  * Table opacities, IKAPTYP 9, 10, 11, 12 and 14 (KAPPA, pprphy.f:199-234,

@@ -169,7 +169,8 @@ program tt_ref_harness
      if (have_sin) call load3(STELLARINPUT, 'STELLARINPUT')
   end if
 
-  if (IKAPTYP == 4 .or. IKAPTYP == 5) call READFREEDMANOPCTABLE
+  if (IKAPTYP >= 4 .and. IKAPTYP <= 7) call READFREEDMANOPCTABLE
+  if (IKAPTYP == 6 .or. IKAPTYP == 7) call READCOLUMNDEPTHOPCTABLE
   if ((IKAPTYP >= 9 .and. IKAPTYP <= 12) .or. IKAPTYP == 14) call READBURROWSOPCTABLE
 
   select case (trim(cmd))

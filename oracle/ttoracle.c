@@ -879,11 +879,78 @@ static int burrows_opc(double t_in, double rh_in, double* bkp, double* bkr, doub
     return 0;
 }
 
+static double g_ctab[8 * 42 * 18]; /* IKAPTYP 6/7 column table, T fastest, then P, then level */
+static int g_ctab_set;
+
+int tor_set_column_table(const double* ka)
+{
+    memcpy(g_ctab, ka, sizeof g_ctab);
+    g_ctab_set = 1;
+    return 0;
+}
+
+/* columnOPC's absorption mean, pprphy.f:1543-1745, after freedman_opc has
+ * set the Planck and Rosseland means and dkP/dT (the reference calls
+ * FreedmanOPC first and columnOPC then sets all five to the same P, R
+ * values): the column cut-off, its level weights (CS0 with the pressure
+ * node FreedmanPVAL(IC1), as the reference has it) and dkA/dT from the upper
+ * level.  Returns 1 where the reference stops. */
+static int column_opc(double t_in, double pg_in, double rh_in, double mu_in, double dr, double* bka, double* dka)
+{
+    double pg1 = pg_in * (0.001);
+    double tm1 = t_in;
+    double kc = (double)-23.9526937f + log10(mu_in) - log10(rh_in) - log10(dr + 1.0);
+    if (kc < -26.0) kc = -25.9999;
+    if (kc > -19.0) kc = -19.0001;
+    if (tm1 < 75.0) tm1 = 75.001;
+    if (tm1 > 4000.0) tm1 = 3999.9;
+    if (pg1 < 0.001) pg1 = 0.001001;
+    if (pg1 > 300000.0) pg1 = 299999.9;
+    int it = 0;
+    while (!(FREEDMAN_T[it] > tm1)) ++it;
+    const int it1 = it - 1;
+    double ws1 = FREEDMAN_T[it1] - tm1;
+    double ws = FREEDMAN_T[it1] - FREEDMAN_T[it];
+    ws1 = ws1 / ws;
+    ws = 1.0 - ws1;
+    int iw = 0;
+    while (!(FREEDMAN_P[iw] > pg1)) ++iw;
+    const int iw1 = iw - 1;
+    double ws2 = pg1 - FREEDMAN_P[iw1];
+    double ws3 = FREEDMAN_P[iw] - FREEDMAN_P[iw1];
+    ws2 = ws2 / ws3;
+    ws3 = 1.0 - ws2;
+    static const double CVAL[8] = {-26.0, -25.0, -24.0, -23.0, -22.0, -21.0, -20.0, -19.0};
+    int ic = 0;
+    while (ic < 8 && !(CVAL[ic] > kc)) ++ic;
+    if (ic == 8 || ic == 0) return 1;
+    const int ic1 = ic - 1;
+    double cs1 = kc - CVAL[ic1];
+    double cs0 = CVAL[ic] - FREEDMAN_P[ic1];
+    cs1 = cs1 / cs0;
+    cs0 = 1.0 - cs1;
+#define CA(a, b, c) g_ctab[(a) + 42 * ((b) + 18 * (c))]
+    const double a000 = CA(it, iw, ic), a010 = CA(it, iw1, ic), a011 = CA(it1, iw1, ic), a001 = CA(it1, iw, ic);
+    double w4 = ws2 * (ws1 * a000 + ws * a001);
+    double w5 = ws3 * (ws1 * a010 + ws * a011);
+    const double a_0 = cs1 * (w4 + w5);
+    const double a100 = CA(it, iw, ic1), a110 = CA(it, iw1, ic1), a111 = CA(it1, iw1, ic1), a101 = CA(it1, iw, ic1);
+    w4 = ws2 * (ws1 * a100 + ws * a101);
+    w5 = ws3 * (ws1 * a110 + ws * a111);
+    const double a_1 = cs0 * (w4 + w5);
+#undef CA
+    *bka = pow(10.0, a_0 + a_1);
+    *dka = (pow(10.0, a000) - pow(10.0, a001)) / (FREEDMAN_T[it] - FREEDMAN_T[it1]);
+    return 0;
+}
+
 static int kappa(void) /* KAPPA, pprphy.f:1-360; case 26 is new */
 {
     const tt_params* p = &O.p;
-    if (p->ikaptyp == 4 || p->ikaptyp == 5) { /* pprphy.f:84-120, MODTYP=2 */
-        if (!g_ftab_set) return seterr(1, "KAPPA: IKAPTYP 4/5 need an opacity table");
+    if (p->ikaptyp >= 4 && p->ikaptyp <= 7) { /* pprphy.f:84-175, MODTYP=2 */
+        if (!g_ftab_set) return seterr(1, "KAPPA: IKAPTYP 4-7 need an opacity table");
+        const int colm = p->ikaptyp >= 6;
+        if (colm && !g_ctab_set) return seterr(1, "KAPPA: IKAPTYP 6/7 need a column table");
         tor_pressg(); /* pprphy.f:88 */
         int bad = 0;
 #pragma omp parallel for reduction(| : bad)
@@ -894,6 +961,9 @@ static int kappa(void) /* KAPPA, pprphy.f:1-360; case 26 is new */
                     if (A(O.T, i, j, k) <= 0.0) { bad = 1; continue; }
                     freedman_opc(A(O.T, i, j, k), A(O.PG, i, j, k), &A(O.XKP, i, j, k), &A(O.XKR, i, j, k),
                                  &A(O.XKA, i, j, k), &A(O.DKP, i, j, k), &A(O.DKA, i, j, k));
+                    if (colm) /* XMUE = mu_gas (GASINIT, ppinit.f:1414) */
+                        bad |= column_opc(A(O.T, i, j, k), A(O.PG, i, j, k), A(O.RH, i, j, k), p->mu_gas,
+                                          XXB(ub) - XXB(i), &A(O.XKA, i, j, k), &A(O.DKA, i, j, k));
                 }
                 for (int i = ub + 1; i <= NX + 2; ++i) {
                     A(O.XKR, i, j, k) = A(O.XKR, ub, j, k);
@@ -942,7 +1012,7 @@ static int kappa(void) /* KAPPA, pprphy.f:1-360; case 26 is new */
         }
         return 0;
     }
-    if (p->ikaptyp != 26) return seterr(1, "KAPPA: IKAPTYP must be 1, 4, 5, 9-14 or 26");
+    if (p->ikaptyp != 26) return seterr(1, "KAPPA: IKAPTYP must be 1, 4-7, 9-14 or 26");
     /* Power law kappa_R = kappa_P = kappa_A = kap0 (rho/rho0)^a (T/T0)^b in the
      * loop structure of the IKAPTYP=8 power law (pprphy.f:176-198): cells up to
      * upperbnd are evaluated, cells above copy the upperbnd value.  Unlike

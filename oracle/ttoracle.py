@@ -51,6 +51,7 @@ class Oracle:
                                         ctypes.POINTER(ctypes.c_int)]
             L.tor_set_opacity_table.argtypes = [ctypes.c_void_p] * 3
             L.tor_set_burrows_table.argtypes = [ctypes.c_void_p] * 3
+            L.tor_set_column_table.argtypes = [ctypes.c_void_p]
             L.tor_set_viscosity.argtypes = [ctypes.POINTER(P.Viscosity)]
             L.tor_get_viscosity.argtypes = [ctypes.POINTER(P.Viscosity)]
             L.tor_get_dt_artvis.argtypes = [ctypes.POINTER(ctypes.c_double)]
@@ -107,6 +108,11 @@ class Oracle:
         assert all(a.size == 42 * 18 for a in t)
         self._check(self.L.tor_set_opacity_table(*(a.ctypes.data_as(ctypes.c_void_p) for a in t)),
                     "set_opacity_table")
+
+    def set_column_table(self, kap_abs):
+        a = np.ascontiguousarray(np.asarray(kap_abs, np.float64).reshape(-1))
+        assert a.size == 42 * 18 * 8
+        self._check(self.L.tor_set_column_table(a.ctypes.data_as(ctypes.c_void_p)), "set_column_table")
 
     def set_burrows_table(self, kap_planck, kap_ross, kap_abs):
         """The log10 tables of IKAPTYP 9-12/14 on the 50 x 50 (log T,

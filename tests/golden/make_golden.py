@@ -411,6 +411,75 @@ def burrows():
                  dict(tab, delt=dt, iii=2))
 
 
+def column_table_text(seed=9):
+    """A synthetic IKAPTYP 6 column-depth table in the reference's text format
+    (READCOLUMNDEPTHOPCTABLE, pprphy.f:1747-1833): ten header lines, T_star,
+    the level count and range, then 5888 rows T P level log_kABS in shuffled
+    order; the 160 entries at T >= 2300 K and 10 <= P <= 300 mbar are left
+    out (-7).  Lower levels rise with the level but a seeded fifth of them
+    lie above level 8, so the reader's cap acts."""
+    from ttcode_amd import opacity as O
+    rng = np.random.default_rng(seed)
+    rows = []
+    for ic, cv in enumerate(O.COLUMN_LEVELS):
+        for ip, pv in enumerate(O.FREEDMAN_P):
+            for it, tv in enumerate(O.FREEDMAN_T):
+                if it >= 37 and 8 <= ip <= 11:
+                    continue
+                lt, lp = np.log10(tv / 1500.0), np.log10(pv)
+                ka = -2.6 + 0.4 * lp + 1.1 * lt + 0.12 * (cv + 19.0) + 0.03 * rng.standard_normal()
+                if ic < 7 and rng.uniform() < 0.2:
+                    ka += 1.5
+                rows.append(f"{tv!r} {pv!r} {cv!r} {float(ka)!r}")
+    assert len(rows) == O.N_COLUMN_ENTRIES
+    order = rng.permutation(len(rows))
+    head = [f"synthetic column table line {i + 1}" for i in range(10)]
+    return "\n".join(head + ["6117.0", "8", "-26.0 -19.0"] + [rows[i] for i in order]) + "\n"
+
+
+def column():
+    """IKAPTYP 6 (pprphy.f:123-175, columnOPC 1543-1745) on the synthetic
+    Freedman and column tables: KAPPA alone on a scrambled state that
+    reaches the (T, P) clamps and both column cut-off clamps, and
+    COUPLED_ENERGY_EQS with a 5% day-night contrast."""
+    import tempfile as _tf
+    from ttcode_amd import opacity as O
+    nx, ny, nz = 12, 16, 12
+    ftext, ctext = freedman_table_text(), column_table_text()
+    with _tf.TemporaryDirectory() as d:
+        fn = os.path.join(d, "f.dat")
+        with open(fn, "w") as f:
+            f.write(ftext)
+        kp, kr, ka, _ = O.read_freedman_table(fn)
+        with open(fn, "w") as f:
+            f.write(ctext)
+        kcol, _ = O.read_column_table(fn)
+    tab = {"tab_planck": kp, "tab_ross": kr, "tab_abs": ka, "tab_col": kcol, "tab_col_text": np.array(ctext)}
+    files = {O.TABLE_FILES[4]: ftext, O.COLUMN_FILES[6]: ctext}
+    p = P.make_params(nx, ny, nz, ikaptyp=6, isigtyp=1, epsmat=1e-8, tirr=2000.0)
+    inp = base_inputs(p, seed=81)
+    rng = np.random.default_rng(81)
+    T = 10.0 ** rng.uniform(1.3, 3.85, size=inp["T"].shape)
+    f = np.linspace(0.0, 1.0, inp["RH"].shape[-1])  # r is the fastest (last numpy) axis
+    RH = 10.0 ** (-1.0 - 11.0 * f + rng.uniform(-0.5, 0.5, size=inp["RH"].shape))
+    inp_k = dict(inp, T=T, RH=RH)
+    outs, sc = run_ref("kappa", p, inp_k, files=files)
+    save("kappa_column6_12x16x12", p, inp_k, outs, sc,
+         ["T", "RH", "PG", "xkapR", "xkapP", "xkapA", "dxkapP_dT", "dxkapA_dT", "sig", "upperbnd"], tab)
+    o = Oracle(P.make_params(nx, ny, nz, ikaptyp=1, kap0=1e-2))
+    o.load_state(base_inputs(o.p))
+    o.bounds()
+    o.pressg()
+    dt = o.schritt()
+    pe = P.make_params(nx, ny, nz, ikaptyp=6, epsmat=1e-8, tirr=2000.0)
+    inp_e = base_inputs(pe, seed=83, contrast=0.05)
+    outs, sc = run_ref("energy", pe, inp_e, delt=dt, iii=2, files=files)
+    save("energy_column6_12x16x12", pe, inp_e, outs, sc,
+         ["T", "ER", "delta_ER", "xkapR", "xkapP", "xkapA", "dxkapP_dT", "dxkapA_dT", "STELLARINPUT",
+          "FLIMX", "FQ", "FR", "RHS_ER", "upperbnd"],
+         {k: v for k, v in dict(tab, delt=dt, iii=2).items() if k != "tab_col_text"})
+
+
 def const13():
     """COUPLED_ENERGY_EQS with IKAPTYP 13 (pprphy.f:235-241: kappa_R =
     kappa_P = ZKAPMULT, kappa_A = gamma_OPC^4 ZKAPMULT), gamma_OPC = 0.3, so
@@ -545,6 +614,8 @@ if __name__ == "__main__":
         burrows()
     elif sys.argv[1:] == ["const13"]:
         const13()
+    elif sys.argv[1:] == ["column"]:
+        column()
     elif sys.argv[1:] == ["variants"]:
         variants()
     elif sys.argv[1:] == ["adjust"]:

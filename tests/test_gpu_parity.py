@@ -295,6 +295,39 @@ def test_kappa_burrows_table_stops_on_nonpositive_temperature(ttgpu):
         d.close()
 
 
+def _column(d, g):
+    d.set_opacity_table(g["tab_planck"], g["tab_ross"], g["tab_abs"])
+    d.set_column_table(g["tab_col"])
+
+
+def test_kappa_column_table(ttgpu):
+    """KAPPA with IKAPTYP 6 (the column-depth absorption mean, pprphy.f:
+    123-175,1543-1745) against the reference (log10 and 10^x from the
+    device math library: within 1e-12)."""
+    g = C.load("kappa_column6_12x16x12")
+    p, d = _dev(ttgpu, g)
+    try:
+        _column(d, g)
+        d.bounds()
+        d.kappa()
+        _cmp(g, p, d, rel=1e-12)
+    finally:
+        d.close()
+
+
+def test_coupled_energy_column_table(ttgpu):
+    g = C.load("energy_column6_12x16x12")
+    p, d = _dev(ttgpu, g)
+    try:
+        _column(d, g)
+        st, _ = d.coupled_energy(float(g["delt"]), int(g["iii"]))
+        assert st.energy_iter == int(g["scal_energy_iter"])
+        assert st.sor_iter_last == int(g["scal_SOR_ITER"])
+        _cmp(g, p, d, rel=1e-12, scaled_tol=1e-10)
+    finally:
+        d.close()
+
+
 def test_coupled_energy_constant_gamma_opacity(ttgpu):
     """IKAPTYP 13 (kappa_A = gamma_OPC^4 ZKAPMULT) against the reference."""
     g = C.load("energy_const13_12x16x12")

@@ -410,3 +410,58 @@ def test_coupled_energy_constant_gamma_opacity():
     xa = g["out_xkapA"][2:-2, 2:-2, 2:-2]
     assert np.all(xa == 0.3 ** 4.0 * p.zkapmult)
     _check_fields(g, p, o)
+
+
+def _column(o, g):
+    o.set_opacity_table(g["tab_planck"], g["tab_ross"], g["tab_abs"])
+    o.set_column_table(g["tab_col"])
+
+
+def test_column_table_reader(tmp_path):
+    """READCOLUMNDEPTHOPCTABLE restated: rows placed by their grid values,
+    160 left out (-7), and levels 1-7 capped at level 8 (the reference's
+    pass after reading); the fixture's table is what its KAPPA ran on."""
+    from ttcode_amd import opacity as O
+    g = C.load("kappa_column6_12x16x12")
+    fn = tmp_path / O.COLUMN_FILES[6]
+    fn.write_text(str(g["tab_col_text"]))
+    ka, t_star = O.read_column_table(str(fn))
+    np.testing.assert_array_equal(ka, g["tab_col"])
+    assert t_star == 6117.0
+    t = ka.reshape(O.NC, O.NP, O.NT)
+    assert np.count_nonzero(t == O.MISSING) == O.NC * O.NT * O.NP - O.N_COLUMN_ENTRIES
+    assert np.all(t[:-1] <= t[-1])
+    # the cap acted: the text holds values above level 8 that the table does not
+    raw = np.full_like(t, O.MISSING)
+    for line in str(g["tab_col_text"]).splitlines()[13:]:
+        tv, pv, cv, kv = map(float, line.split())
+        raw[O.COLUMN_LEVELS.index(cv), O.FREEDMAN_P.index(pv), O.FREEDMAN_T.index(tv)] = kv
+    assert np.count_nonzero(raw[:-1] > raw[-1]) > 100
+    lines = str(g["tab_col_text"]).splitlines()
+    lines[20] = "1234.5 " + " ".join(lines[20].split()[1:])
+    fn.write_text("\n".join(lines) + "\n")
+    with pytest.raises(ValueError, match="not on the table grid"):
+        O.read_column_table(str(fn))
+
+
+def test_kappa_column_table():
+    """KAPPA with IKAPTYP 6 (FreedmanOPC, then columnOPC's absorption mean
+    from the column-depth table, pprphy.f:123-175,1543-1745) and SUBSIG, on
+    a scrambled state that reaches the (T, P) clamps: bit for bit, with the
+    reference's CS0 from the pressure axis."""
+    def act(o, g):
+        _column(o, g)
+        o.bounds()
+        o.kappa()
+    g, p, o, _ = _run("kappa_column6_12x16x12", act)
+    _check_fields(g, p, o)
+
+
+def test_coupled_energy_column_table():
+    def act(o, g):
+        _column(o, g)
+        return o.coupled_energy(float(g["delt"]), int(g["iii"]))[0]
+    g, p, o, st = _run("energy_column6_12x16x12", act)
+    assert st.energy_iter == int(g["scal_energy_iter"])
+    assert st.sor_iter_last == int(g["scal_SOR_ITER"])
+    _check_fields(g, p, o)

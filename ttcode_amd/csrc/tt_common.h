@@ -101,6 +101,8 @@ struct Lib {
     int ftab_version = -1;          // version of the host tables ftab holds
     double* btab = nullptr;         // IKAPTYP 9-14 tables and log10 T axis, then 10^ of them (2 x (3 x 50 x 50 + 50))
     int btab_version = -1;
+    double* ctab = nullptr;         // IKAPTYP 6/7 column table, then 10^ of it (2 x 8 x 42 x 18)
+    int ctab_version = -1;
     int have_de1 = 0;
     double* partial = nullptr;      // block partial sums (reductions)
     int npartial = 0;

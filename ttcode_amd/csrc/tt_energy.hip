@@ -77,17 +77,28 @@ __constant__ double c_fp[TT_FREEDMAN_NP] = {0.001, 0.003, 0.01, 0.03, 0.1, 0.3, 
 // of the upper T node's column.  tab: Planck, Rosseland, absorption, then
 // 10^ of each entry (formed on the host), which the difference reads.  Compute-bound:
 // three 10^x per cell (the four of the derivative come from the 10^ table).
-__global__ void k_kappa_tab(Geo g, const int32_t* __restrict__ ub, const double* __restrict__ T,
-                            const double* __restrict__ PG, const double* __restrict__ tab, double* XKR,
-                            double* XKP, double* XKA, double* DKP, double* DKA, double* SIG, double sig,
-                            Ctrl* ctrl) {
+//
+// IKAPTYP 6/7 (pprphy.f:123-175, columnOPC pprphy.f:1543-1745) with col set:
+// the absorption mean from the column-depth table instead, interpolated in
+// (T, P) on the two column levels around the cut-off log10(2 mu m_p / (3 rho
+// (r(upperbnd) - r + 1))) (the reference's -23.9526937 + log10 mu - log10
+// rho - log10(dr + 1), clamped into -26 ... -19), with the level weights as
+// the reference forms them (its CS0 takes the pressure node FreedmanPVAL(IC1),
+// kept for parity); dkA/dT from the upper level.  col: the 42 x 18 x 8 table
+// then 10^ of it.
+__global__ void k_kappa_tab(Geo g, Met m, const int32_t* __restrict__ ub, const double* __restrict__ T,
+                            const double* __restrict__ PG, const double* __restrict__ RH,
+                            const double* __restrict__ tab, const double* __restrict__ col, double logmu,
+                            double* XKR, double* XKP, double* XKA, double* DKP, double* DKA, double* SIG,
+                            double sig, Ctrl* ctrl) {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= g.n3) return;
     SIG[t] = sig;
     int i, j, k;
     box_ijk(g, t, i, j, k);
     const int u = ub[g.idx2(j, k)];
-    const long long n = g.idx(i <= u ? i : u, j, k);
+    const int ie = i <= u ? i : u;
+    const long long n = g.idx(ie, j, k);
     const double t_in = T[n];
     double pg1 = PG[n] * (0.001);
     if (!(t_in > 0.0) || pg1 != pg1) {  // the reference stops (pprphy.f:825-828,841-862)
@@ -126,12 +137,42 @@ __global__ void k_kappa_tab(Geo g, const int32_t* __restrict__ ub, const double*
     }
     XKP[t] = out[0];
     XKR[t] = out[1];
-    XKA[t] = out[2];
     const double dt = c_ft[it] - c_ft[it1];
     const double* p10 = tab + 3 * TN;  // 10^ of the Planck table
-    const double* a10 = tab + 5 * TN;  // and of the absorption table
     DKP[t] = (p10[it + TT_FREEDMAN_NT * iw] - p10[it1 + TT_FREEDMAN_NT * iw]) / dt;
-    DKA[t] = (a10[it + TT_FREEDMAN_NT * iw] - a10[it1 + TT_FREEDMAN_NT * iw]) / dt;
+    if (!col) {
+        const double* a10 = tab + 5 * TN;  // and of the absorption table
+        XKA[t] = out[2];
+        DKA[t] = (a10[it + TT_FREEDMAN_NT * iw] - a10[it1 + TT_FREEDMAN_NT * iw]) / dt;
+        return;
+    }
+    const double dr = m(TT_M_XXB, u) - m(TT_M_XXB, ie);
+    double kc = (double)-23.9526937f + logmu - log10(RH[n]) - log10(dr + 1.0);
+    if (kc < -26.0) kc = -25.9999;
+    if (kc > -19.0) kc = -19.0001;
+    if (kc != kc) {  // rho < 0: no level above, the reference stops (pprphy.f:1639-1651)
+        ctrl->bad_kappa = 1;
+        return;
+    }
+    int ic = 1;  // levels -26 ... -19: the first above kc is 1..7
+    while (ic < 7 && !(-26.0 + ic > kc)) ++ic;
+    const int ic1 = ic - 1;
+    double cs1 = kc - (-26.0 + ic1);
+    double cs0 = (-26.0 + ic) - c_fp[ic1];
+    cs1 = cs1 / cs0;
+    cs0 = 1.0 - cs1;
+    const int CN = TT_FREEDMAN_NT * TT_FREEDMAN_NP;
+    const double* c0 = col + ic * CN;
+    const double* c1 = col + ic1 * CN;
+    const double a000 = c0[it + TT_FREEDMAN_NT * iw], a010 = c0[it + TT_FREEDMAN_NT * iw1];
+    const double a011 = c0[it1 + TT_FREEDMAN_NT * iw1], a001 = c0[it1 + TT_FREEDMAN_NT * iw];
+    const double a_0 = cs1 * (ws2 * (ws1 * a000 + ws * a001) + ws3 * (ws1 * a010 + ws * a011));
+    const double a100 = c1[it + TT_FREEDMAN_NT * iw], a110 = c1[it + TT_FREEDMAN_NT * iw1];
+    const double a111 = c1[it1 + TT_FREEDMAN_NT * iw1], a101 = c1[it1 + TT_FREEDMAN_NT * iw];
+    const double a_1 = cs0 * (ws2 * (ws1 * a100 + ws * a101) + ws3 * (ws1 * a110 + ws * a111));
+    XKA[t] = exp10(a_0 + a_1);
+    const double* c10 = col + 8 * CN + ic * CN;  // 10^ of level ic
+    DKA[t] = (c10[it + TT_FREEDMAN_NT * iw] - c10[it1 + TT_FREEDMAN_NT * iw]) / dt;
 }
 
 // ---- IKAPTYP 9-12, 14: Burrows tables
@@ -649,6 +690,9 @@ static int g_ftab_version = 0;  // 0: never set
 // IKAPTYP 9-14 tables likewise, followed by the log10 T axis
 static double g_btab[3 * TT_BURROWS_NT * TT_BURROWS_NR + TT_BURROWS_NT];
 static int g_btab_version = 0;
+// IKAPTYP 6/7 column-depth absorption table (42 x 18 x 8)
+static double g_ctab[8 * TT_FREEDMAN_NT * TT_FREEDMAN_NP];
+static int g_ctab_version = 0;
 
 // a table followed by 10^ of each entry, the powers from the host's libm
 // (the reference's 10.d0**x): the derivatives difference neighbouring 10^
@@ -669,8 +713,16 @@ static int kappa_impl() {
     else if (L.p.isigtyp == 1) sig = 0.4;
     else return fail(1, "SUBSIG: ISIGTYP must be 0 or 1");
     double** f = L.f;
-    if (L.p.ikaptyp == 4 || L.p.ikaptyp == 5) {
-        if (g_ftab_version == 0) return fail(1, "KAPPA: IKAPTYP 4 and 5 need tt_set_opacity_table first");
+    if (L.p.ikaptyp >= 4 && L.p.ikaptyp <= 7) {
+        if (g_ftab_version == 0) return fail(1, "KAPPA: IKAPTYP 4-7 need tt_set_opacity_table first");
+        const bool colmode = L.p.ikaptyp >= 6;
+        if (colmode && g_ctab_version == 0) return fail(1, "KAPPA: IKAPTYP 6 and 7 need tt_set_column_table first");
+        if (colmode && L.ctab_version != g_ctab_version) {
+            const std::vector<double> t = with_pow10(g_ctab, 8 * TT_FREEDMAN_NT * TT_FREEDMAN_NP);
+            TT_HIP(hipMemcpyAsync(L.ctab, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice, L.s));
+            TT_HIP(hipStreamSynchronize(L.s));
+            L.ctab_version = g_ctab_version;
+        }
         if (L.ftab_version != g_ftab_version) {
             const std::vector<double> t = with_pow10(g_ftab, 3 * TT_FREEDMAN_NT * TT_FREEDMAN_NP);
             TT_HIP(hipMemcpyAsync(L.ftab, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice, L.s));
@@ -679,8 +731,10 @@ static int kappa_impl() {
         }
         int rc = pressg_impl();  // pprphy.f:88
         if (rc) return rc;
-        k_kappa_tab<<<nblk(L.g.n3, 256), 256, 0, L.s>>>(L.g, L.ub, f[TT_T], f[TT_PG], L.ftab, f[TT_XKAPR],
-                                                       f[TT_XKAPP], f[TT_XKAPA], f[TT_DXKAPP_DT],
+        // XMUE = mu_gas (GASINIT, ppinit.f:1414)
+        k_kappa_tab<<<nblk(L.g.n3, 256), 256, 0, L.s>>>(L.g, met(), L.ub, f[TT_T], f[TT_PG], f[TT_RH], L.ftab,
+                                                       colmode ? L.ctab : nullptr, std::log10(L.p.mu_gas),
+                                                       f[TT_XKAPR], f[TT_XKAPP], f[TT_XKAPA], f[TT_DXKAPP_DT],
                                                        f[TT_DXKAPA_DT], f[TT_SIG], sig, L.ctrl);
         TT_LAUNCH();
         return 0;
@@ -931,6 +985,13 @@ extern "C" int tt_set_burrows_table(const double* kap_planck, const double* kap_
     memcpy(g_btab + 2 * n, kap_abs, n * sizeof(double));
     memcpy(g_btab + 3 * n, h_bt, sizeof h_bt);
     ++g_btab_version;
+    return 0;
+}
+
+extern "C" int tt_set_column_table(const double* kap_abs) {
+    if (!kap_abs) return fail(1, "tt_set_column_table: null table");
+    memcpy(g_ctab, kap_abs, sizeof g_ctab);
+    ++g_ctab_version;
     return 0;
 }
 

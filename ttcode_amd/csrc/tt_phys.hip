@@ -376,7 +376,7 @@ int sync_check() {
     if (L.hctrl->bad_bound) return fail(3, "error in boundxmax: upperbnd above the domain");
     if (L.hctrl->bad_q) return fail(2, "ERROR IN DELTA_Q: non-positive value of Q");
     if (L.hctrl->bad_kappa)
-        return fail(2, L.p.ikaptyp == 4 || L.p.ikaptyp == 5
+        return fail(2, L.p.ikaptyp >= 4 && L.p.ikaptyp <= 7
                            ? "NEG. T. IN FREEDMAN OPC (or an undefined pressure)"
                            : "NEG. T. IN BURROWS OPC (or a density outside the table)");
     return 0;

@@ -76,6 +76,11 @@ module ttgpu_c
        import :: c_int, c_double
        real(c_double), intent(in) :: kp(42, 18), kr(42, 18), ka(42, 18)
      end function tt_set_opacity_table
+     ! ColumnXKAP_abs(42,18,8) after READCOLUMNDEPTHOPCTABLE
+     integer(c_int) function tt_set_column_table(ka) bind(C, name='tt_set_column_table')
+       import :: c_int, c_double
+       real(c_double), intent(in) :: ka(42, 18, 8)
+     end function tt_set_column_table
      ! BurrowsXKAP_plnk/_ross/_abs(50,50) after READBURROWSOPCTABLE
      integer(c_int) function tt_set_burrows_table(kp, kr, ka) bind(C, name='tt_set_burrows_table')
        import :: c_int, c_double

@@ -182,3 +182,51 @@ def synthetic_burrows_table(seed: int = 0):
     ka = kr - 0.4 + 0.15 * np.cos(3.0 * lt) + 0.02 * rng.standard_normal((BNR, BNT))
     return (np.asarray(BURROWS_LOGT), np.asarray(BURROWS_LOGRH), kp.reshape(-1), kr.reshape(-1),
             ka.reshape(-1))
+
+
+# ---- IKAPTYP 6 and 7: the column-depth absorption table
+COLUMN_LEVELS = (-26.0, -25.0, -24.0, -23.0, -22.0, -21.0, -20.0, -19.0)  # log10 kappa cut-off (ppmod.f:701-702)
+NC = len(COLUMN_LEVELS)
+N_COLUMN_ENTRIES = 5888  # rows the reference reads (pprphy.f:1763)
+COLUMN_FILES = {6: "Freedman6117_column.dat", 7: "Freedman6117_column_noTIVO.dat"}
+
+
+def read_column_table(path):
+    """READCOLUMNDEPTHOPCTABLE (pprphy.f:1747-1833).
+
+    Ten header records, T_star, the number of levels, the level range (two
+    values), then 5888 rows ``T P level log_kABS`` placed by their grid
+    values (T, P and the level must be grid values exactly); entries the file
+    does not list stay -7.  Then, as the reference does, every level 1-7 of
+    each (T, P) is capped at level 8's value.
+
+    Returns ``(kap_abs, t_star)``: 42 x 18 x 8 values, temperature fastest,
+    then pressure, then level (ColumnXKAP_abs(IT, IP, IC) in Fortran order).
+    """
+    recs = (r for r in _records(path) if r)
+    for _ in range(10):
+        next(recs)
+    t_star = _number(next(recs)[0])
+    next(recs)  # the number of levels
+    lv = []
+    while len(lv) < 2:  # the level range
+        lv += next(recs)
+    tab = np.full((NC, NP, NT), MISSING)
+    tidx = {t: i for i, t in enumerate(FREEDMAN_T)}
+    pidx = {p: i for i, p in enumerate(FREEDMAN_P)}
+    cidx = {c: i for i, c in enumerate(COLUMN_LEVELS)}
+    for row in range(N_COLUMN_ENTRIES):
+        vals = []
+        while len(vals) < 4:
+            try:
+                vals += [_number(t) for t in next(recs)]
+            except StopIteration:
+                raise ValueError(f"{path}: {row} of {N_COLUMN_ENTRIES} table rows") from None
+        t, p, c, ka = vals[:4]
+        if t not in tidx or p not in pidx or c not in cidx:
+            raise ValueError(f"{path}: row {row + 1} (T={t}, P={p}, level={c}) is not on the table grid")
+        tab[cidx[c], pidx[p], tidx[t]] = ka
+    top = tab[NC - 1]
+    for ic in range(NC - 2, -1, -1):
+        tab[ic] = np.where(tab[ic] > top, top, tab[ic])
+    return tab.reshape(-1), t_star

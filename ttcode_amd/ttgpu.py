@@ -56,6 +56,7 @@ def load_library():
     L.tt_sphericalvellimit.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double]
     L.tt_set_opacity_table.argtypes = [ctypes.c_void_p] * 3
     L.tt_set_burrows_table.argtypes = [ctypes.c_void_p] * 3
+    L.tt_set_column_table.argtypes = [ctypes.c_void_p]
     L.tt_set_viscosity.argtypes = [ctypes.POINTER(P.Viscosity)]
     L.tt_get_viscosity.argtypes = [ctypes.POINTER(P.Viscosity)]
     L.tt_get_dt_artvis.argtypes = [ctypes.POINTER(ctypes.c_double)]
@@ -183,6 +184,15 @@ class Device:
             raise ValueError("opacity tables must hold 42 x 18 values")
         self._check(self.L.tt_set_opacity_table(*(a.ctypes.data_as(ctypes.c_void_p) for a in t)),
                     "tt_set_opacity_table")
+
+    def set_column_table(self, kap_abs):
+        """The IKAPTYP 6/7 column-depth absorption table, 42 x 18 x 8 log10
+        values, temperature fastest then pressure then level
+        (``ttcode_amd.opacity.read_column_table``)."""
+        a = np.ascontiguousarray(np.asarray(kap_abs, np.float64).reshape(-1))
+        if a.size != 42 * 18 * 8:
+            raise ValueError("the column table must hold 42 x 18 x 8 values")
+        self._check(self.L.tt_set_column_table(a.ctypes.data_as(ctypes.c_void_p)), "tt_set_column_table")
 
     def set_burrows_table(self, kap_planck, kap_ross, kap_abs):
         """The log10 tables of IKAPTYP 9-12/14 on the 50 x 50 (log T,
