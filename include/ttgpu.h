@@ -312,8 +312,11 @@ int tt_sphericalvellimit(int ivel, double velmax, double hydrotime);
 typedef struct tt_viscosity {
     int32_t ivis;            /* > 0: viscosity on                             */
     int32_t ivistyp;         /* 0: divtx/y/z = 0; 1: the full stress tensor   */
-    int32_t ieta;            /* ETACALC: 0 and 5 eta = RH XNUE; 4 molecular,
-                                eta = RH min(CS mu_gas 1.67e-8 / RH, XNUE)    */
+    int32_t ieta;            /* ETACALC: 0 and 5 eta = RH XNUE; 2 XNUE ramped
+                                from 1e9 over days 282.5-292.5 of hydrotime
+                                (tt_set_limits); 3 raised at the bottom (nx >=
+                                13); 4 molecular, eta = RH min(CS mu_gas
+                                1.67e-8 / RH, XNUE)                           */
     int32_t idiss;           /* != 0: dissipative heating DISFN               */
     double xnue;             /* XNUE, kinematic viscosity, cm^2/s             */
     int32_t iartvis;         /* 0 off; 1 NR_ARTIFICIAL_VISC                   */

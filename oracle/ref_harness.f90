@@ -142,7 +142,7 @@ program tt_ref_harness
   D_T = h_d_t
   D_RH = 0.d0
   FBottom = h_fbottom
-  hydrotime = 0.d0
+  hydrotime = h_hydrotime
   call GITTER
   cv = 0.d0
   call GASINIT

@@ -417,7 +417,7 @@ int tor_set_viscosity(const tt_viscosity* v)
 {
     if (v->ivis > 0) {
         if (v->ivistyp != 0 && v->ivistyp != 1) return seterr(1, "VISC: IVISTYP must be 0 or 1");
-        if (v->ieta != 0 && v->ieta != 4 && v->ieta != 5) return seterr(1, "ETACALC: IETA must be 0, 4 or 5");
+        if (v->ieta < 0 || v->ieta == 1 || v->ieta > 5) return seterr(1, "ETACALC: IETA must be 0, 2, 3, 4 or 5");
     }
     if (v->iartvis > 1) return seterr(1, "unknown IARTVIS"); /* ppfor.f:388-390 */
     if (v->iartvis > 0 && g_visc.iartvis <= 0) g_dt_artvis = O.p.dtmax;
@@ -2089,7 +2089,30 @@ static int etacalc(void)
         for (size_t n = 0; n < O.n3; ++n) O.ETA[n] = O.RH[n] * xnue;
         return 0;
     }
-    if (g_visc.ieta != 4) return seterr(1, "ETACALC: IETA must be 0, 4 or 5");
+    if (g_visc.ieta == 2) { /* ramp in time, ppvisc.f:402-420 */
+        const double day = (double)8.64e4f, xs = 9.0;
+        const double t0 = (double)282.5f * day, t1 = t0 + 10.0 * day, ht = g_lim.hydrotime;
+        double xv;
+        if (ht < t0) xv = pow(10.0, xs);
+        else if (ht >= t0 && ht < t1) {
+            const double ms = (log10(xnue) - xs) / (t1 - t0);
+            xv = pow(10.0, ms * (ht - t0) + xs);
+        } else xv = xnue;
+        for (size_t n = 0; n < O.n3; ++n) O.ETA[n] = O.RH[n] * xv;
+        return 0;
+    }
+    if (g_visc.ieta == 3) { /* raised at the bottom boundary, ppvisc.f:427-441 */
+        if (NX < 13) return seterr(1, "ETACALC: IETA 3 needs nx >= 13");
+        for (size_t n = 0; n < O.n3; ++n) O.ETA[n] = O.RH[n] * xnue;
+        const double extra = 4.0, ms = extra / (1.0 - 1.0 * 15);
+        for (int k = -1; k <= NZ + 2; ++k)
+            for (int j = -1; j <= NY + 2; ++j) {
+                for (int i = 1; i <= 15; ++i) A(O.ETA, i, j, k) = A(O.RH, i, j, k) * pow(10.0, ms * (double)(i - 15) + log10(xnue));
+                for (int i = -1; i <= 0; ++i) A(O.ETA, i, j, k) = A(O.RH, i, j, k) * pow(10.0, extra + log10(xnue));
+            }
+        return 0;
+    }
+    if (g_visc.ieta != 4) return seterr(1, "ETACALC: IETA must be 0, 2, 3, 4 or 5");
     tor_sound();
     const double c = (double)1.67f * pow(10.0, -8.0); /* 1.67*10.d0**(-8.d0), 1.67 a REAL */
     for (size_t n = 0; n < O.n3; ++n) {

@@ -411,6 +411,27 @@ def burrows():
                  dict(tab, delt=dt, iii=2))
 
 
+def viscosity_ieta23():
+    """FORGAS3D with ETACALC's IETA 2 (the kinematic viscosity ramped from
+    1e9 to XNUE = 3e13 between days 282.5 and 292.5, here at day 287.5) and
+    IETA 3 (raised 10^4-fold below the bottom and ramped down over cells
+    1-15; 16 radial cells)."""
+    for ieta, nx, seed, ht in ((2, 12, 91, 287.5 * 86400.0), (3, 16, 93, 0.0)):
+        p = P.make_params(nx, 16, 12, ikaptyp=1, kap0=1e-2)
+        o = Oracle(p)
+        o.load_state(base_inputs(p))
+        o.bounds()
+        o.pressg()
+        dt = o.schritt()
+        inp = base_inputs(p, seed=seed, contrast=0.05)
+        vis = dict(h_ivis=1, h_ivistyp=1, h_ieta=ieta, h_idiss=1, h_hydrotime=ht)
+        outs, sc = run_ref("forgas", p, inp, delt=dt, xnue=3e13, extra=vis)
+        save(f"forgas_visc_ieta{ieta}_{nx}x16x12", p, inp, outs, sc,
+             ["T", "RH", "V", "G", "H", "PG", "eta", "divtx", "divty", "divtz", "DISFN"],
+             {"delt": dt, "vis_ivis": 1, "vis_ivistyp": 1, "vis_ieta": ieta, "vis_idiss": 1, "vis_xnue": 3e13,
+              "vis_hydrotime": ht})
+
+
 def column_table_text(seed=9):
     """A synthetic IKAPTYP 6 column-depth table in the reference's text format
     (READCOLUMNDEPTHOPCTABLE, pprphy.f:1747-1833): ten header lines, T_star,
@@ -616,6 +637,8 @@ if __name__ == "__main__":
         const13()
     elif sys.argv[1:] == ["column"]:
         column()
+    elif sys.argv[1:] == ["ieta23"]:
+        viscosity_ieta23()
     elif sys.argv[1:] == ["variants"]:
         variants()
     elif sys.argv[1:] == ["adjust"]:

@@ -409,7 +409,8 @@ def test_step_freedman_table_32x64x32_vs_oracle(ttgpu):
         d.close()
 
 
-@pytest.mark.parametrize("name", ["forgas_visc_ieta4_12x16x12", "forgas_visc_ieta0_12x16x12"])
+@pytest.mark.parametrize("name", ["forgas_visc_ieta4_12x16x12", "forgas_visc_ieta0_12x16x12",
+                                  "forgas_visc_ieta2_12x16x12", "forgas_visc_ieta3_16x16x12"])
 def test_forgas3d_viscosity(ttgpu, name):
     """FORGAS3D with ETACALC + VISC (IVISTYP=1, spherical) and dissipative
     heating against the reference (tan from the device math library)."""
@@ -418,6 +419,8 @@ def test_forgas3d_viscosity(ttgpu, name):
     try:
         d.set_viscosity(int(g["vis_ivis"]), int(g["vis_ivistyp"]), int(g["vis_ieta"]), int(g["vis_idiss"]),
                         float(g["vis_xnue"]))
+        if "vis_hydrotime" in g.files:  # IETA 2 ramps with the run's time
+            d.set_limits(hydrotime=float(g["vis_hydrotime"]))
         d.bounds()
         d.qdens()
         d.forgas3d(float(g["delt"]))

@@ -229,9 +229,12 @@ def _viscosity(o, g):
         return g[k].item() if k in g.files else d
     o.set_viscosity(int(get("vis_ivis", 0)), int(get("vis_ivistyp", 1)), int(get("vis_ieta", 4)),
                     int(get("vis_idiss", 0)), float(get("vis_xnue", 1e9)), int(get("vis_iartvis", 0)))
+    if "vis_hydrotime" in g.files:  # IETA 2 ramps with the run's time
+        o.set_limits(hydrotime=float(g["vis_hydrotime"]))
 
 
-@pytest.mark.parametrize("name", ["forgas_visc_ieta4_12x16x12", "forgas_visc_ieta0_12x16x12"])
+@pytest.mark.parametrize("name", ["forgas_visc_ieta4_12x16x12", "forgas_visc_ieta0_12x16x12",
+                                  "forgas_visc_ieta2_12x16x12", "forgas_visc_ieta3_16x16x12"])
 def test_forgas3d_viscosity(name):
     """FORGAS3D with ETACALC + VISC (IVISTYP=1, spherical) and dissipative
     heating: eta, divtx/y/z, DISFN and the updated fields, bit for bit."""

@@ -12,6 +12,7 @@
 // components in scratch arrays: a pass reads the neighbours of the one
 // before.  Every expression keeps the reference's operation order.
 #include <cmath>
+#include <vector>
 
 #include "tt_common.h"
 
@@ -24,6 +25,8 @@ namespace {
 // scratch: DIVV, TXX, TYY, TZZ, TXY, TXZ, TYZ, TXYAV, TYZAV, TXZAV
 enum { S_DIVV, S_TXX, S_TYY, S_TZZ, S_TXY, S_TXZ, S_TYZ, S_AXY, S_AYZ, S_AXZ, S_N };
 double* g_scr[S_N] = {};
+double* g_nu = nullptr;  // IETA 3: the radial profile of the kinematic viscosity
+double g_nu_xnue = 0.0;
 
 struct VArgs {
     const double *V, *G, *H, *ETA;
@@ -33,14 +36,20 @@ struct VArgs {
 
 // ETACALC: IETA 0 and 5, eta = RH XNUE; IETA 4, eta = RH min(CS mu_gas c /
 // RH, XNUE) with c = 1.67 10^-8 (the host forms it as the reference does)
-__global__ void k_eta(long long n, int ieta, double xnue, double mu, double c, const double* __restrict__ RH,
-                      const double* __restrict__ CS, double* ETA) {
+// IETA 2: eta = RH XNUE_VAL (the host forms the ramped value); IETA 3: eta =
+// RH nu(i) with the radial profile nu(i) from the host (nu_i, index i + 1)
+__global__ void k_eta(Geo g, int ieta, double xnue, double mu, double c, const double* __restrict__ nu_i,
+                      const double* __restrict__ RH, const double* __restrict__ CS, double* ETA) {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
+    if (t >= g.n3) return;
     if (ieta == 4) {
         double e = CS[t] * mu * c / RH[t];
         e = e < xnue ? e : xnue;
         ETA[t] = e * RH[t];
+    } else if (nu_i) {
+        int i, j, k;
+        box_ijk(g, t, i, j, k);
+        ETA[t] = RH[t] * nu_i[i + 1];
     } else {
         ETA[t] = RH[t] * xnue;
     }
@@ -316,6 +325,8 @@ void visc_free() {
         if (p) hipFree(p);
         p = nullptr;
     }
+    if (g_nu) hipFree(g_nu);
+    g_nu = nullptr;
     L.av_valid = 0;
 }
 
@@ -329,7 +340,31 @@ int visc_impl() {
         if (rc) return rc;
     }
     const double c = (double)1.67f * pow(10.0, -8.0);  // 1.67*10.d0**(-8.d0), 1.67 a REAL literal
-    k_eta<<<nblk(g.n3, 256), 256, 0, L.s>>>(g.n3, v.ieta, v.xnue, L.p.mu_gas, c, f[TT_RH], f[TT_CS], f[TT_ETA]);
+    double xnue = v.xnue;
+    const double* nu_i = nullptr;
+    if (v.ieta == 2) {  // ramp of the kinematic viscosity in time (ppvisc.f:402-420)
+        const double day = (double)8.64e4f, xs = 9.0;
+        const double t0 = (double)282.5f * day, t1 = t0 + 10.0 * day, ht = L.lim.hydrotime;
+        if (ht < t0) xnue = pow(10.0, xs);
+        else if (ht >= t0 && ht < t1) xnue = pow(10.0, (log10(v.xnue) - xs) / (t1 - t0) * (ht - t0) + xs);
+        else xnue = v.xnue;
+    } else if (v.ieta == 3) {  // raised at the bottom boundary (ppvisc.f:427-441)
+        if (!g_nu || g_nu_xnue != v.xnue) {
+            std::vector<double> h((size_t)g.nx + 4);
+            const double extra = 4.0, ms = extra / (1.0 - 1.0 * 15);
+            for (int i = -1; i <= g.nx + 2; ++i) {
+                double nu = v.xnue;
+                if (i >= 1 && i <= 15) nu = pow(10.0, ms * (double)(i - 15) + log10(v.xnue));
+                if (i <= 0) nu = pow(10.0, extra + log10(v.xnue));
+                h[i + 1] = nu;
+            }
+            if (!g_nu) TT_HIP(hipMalloc(&g_nu, h.size() * sizeof(double)));
+            TT_HIP(hipMemcpy(g_nu, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+            g_nu_xnue = v.xnue;
+        }
+        nu_i = g_nu;
+    }
+    k_eta<<<nblk(g.n3, 256), 256, 0, L.s>>>(g, v.ieta, xnue, L.p.mu_gas, c, nu_i, f[TT_RH], f[TT_CS], f[TT_ETA]);
     TT_LAUNCH();
     if (v.ivistyp == 0) {  // divtx/y/z = 0 (ppvisc.f:23-27)
         for (int q : {TT_DIVTX, TT_DIVTY, TT_DIVTZ}) TT_HIP(hipMemsetAsync(f[q], 0, (size_t)g.n3 * sizeof(double), L.s));
@@ -410,7 +445,8 @@ extern "C" int tt_set_viscosity(const tt_viscosity* v) {
     if (!v) return fail(1, "null viscosity options");
     if (v->ivis > 0) {
         if (v->ivistyp != 0 && v->ivistyp != 1) return fail(1, "VISC: IVISTYP must be 0 or 1");
-        if (v->ieta != 0 && v->ieta != 4 && v->ieta != 5) return fail(1, "ETACALC: IETA must be 0, 4 or 5");
+        if (v->ieta < 0 || v->ieta == 1 || v->ieta > 5) return fail(1, "ETACALC: IETA must be 0, 2, 3, 4 or 5");
+        if (v->ieta == 3 && L.g.nx < 13) return fail(1, "ETACALC: IETA 3 sets cells 1..15 and needs nx >= 13");
         if (!(v->xnue > 0.0)) return fail(1, "ETACALC: XNUE must be positive");
     }
     if (v->iartvis > 1) return fail(1, "unknown IARTVIS: only 1 (NR_ARTIFICIAL_VISC) is defined");
