@@ -47,7 +47,8 @@ typedef struct tt_params {
     int32_t iflimtyp;        /* flux limiter: 1 = 1/3, 2 = Levermore-Pomraning */
     int32_t f_incident;      /* 0 = no irradiation, 2 = implicit cos heating  */
     int32_t ikaptyp;         /* 1 = constant ZKAPMULT, 4/5 and 9-12/14 tables,
-                                6/7 column-depth tables, 13 = constant with
+                                6/7 column-depth tables, 8 = H- bound-free,
+                                13 = constant with
                                 gamma_opc, 26 = power law                     */
     int32_t isigtyp;         /* 0 = no scattering, 1 = 0.4 cm^2/g             */
     int32_t isoth;           /* 0 = energy equation on                        */

@@ -1004,6 +1004,24 @@ static int kappa(void) /* KAPPA, pprphy.f:1-360; case 26 is new */
         }
         return 0;
     }
+    if (p->ikaptyp == 8) { /* H- bound-free, pprphy.f:176-198: j from 1; 0.00342 a REAL */
+        for (int k = -1; k <= NZ + 2; ++k)
+            for (int j = 1; j <= NY + 2; ++j) {
+                int ub = UBND(j, k);
+                for (int i = -1; i <= ub; ++i) {
+                    A(O.XKR, i, j, k) = 0.95 * pow(A(O.RH, i, j, k), 0.5) * pow(A(O.T, i, j, k) / 2500.0, 9.0);
+                    A(O.DKP, i, j, k) = (double)0.00342f * pow(A(O.RH, i, j, k), 0.5) * pow(A(O.T, i, j, k) / 2500.0, 8.0);
+                    A(O.DKA, i, j, k) = (double)0.00342f * pow(A(O.RH, i, j, k), 0.5) * pow(A(O.T, i, j, k) / 2500.0, 8.0);
+                }
+                for (int i = ub + 1; i <= NX + 2; ++i) {
+                    A(O.XKR, i, j, k) = A(O.XKR, ub, j, k);
+                    A(O.DKP, i, j, k) = A(O.DKP, ub, j, k);
+                    A(O.DKA, i, j, k) = A(O.DKA, ub, j, k);
+                }
+            }
+        for (size_t n = 0; n < O.n3; ++n) { O.XKP[n] = O.XKR[n]; O.XKA[n] = O.XKR[n]; }
+        return 0;
+    }
     if (p->ikaptyp == 13) { /* pprphy.f:235-241 */
         const double xa = pow(p->gamma_opc, 4.0) * p->zkapmult;
         for (size_t n = 0; n < O.n3; ++n) {
@@ -1012,7 +1030,7 @@ static int kappa(void) /* KAPPA, pprphy.f:1-360; case 26 is new */
         }
         return 0;
     }
-    if (p->ikaptyp != 26) return seterr(1, "KAPPA: IKAPTYP must be 1, 4-7, 9-14 or 26");
+    if (p->ikaptyp != 26) return seterr(1, "KAPPA: IKAPTYP must be 1, 4-14 or 26");
     /* Power law kappa_R = kappa_P = kappa_A = kap0 (rho/rho0)^a (T/T0)^b in the
      * loop structure of the IKAPTYP=8 power law (pprphy.f:176-198): cells up to
      * upperbnd are evaluated, cells above copy the upperbnd value.  Unlike

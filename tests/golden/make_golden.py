@@ -411,6 +411,31 @@ def burrows():
                  dict(tab, delt=dt, iii=2))
 
 
+def hminus8():
+    """KAPPA and COUPLED_ENERGY_EQS with IKAPTYP 8 (H- bound-free, pprphy.f:
+    176-198), with the opacity arrays prefilled: the reference's j loop
+    starts at 1, so the j = -1, 0 ghost columns keep the prefilled values."""
+    nx, ny, nz = 12, 16, 12
+    p = P.make_params(nx, ny, nz, ikaptyp=8, isigtyp=1, epsmat=1e-8, tirr=2000.0)
+    inp = base_inputs(p, seed=97, contrast=0.05)
+    rng = np.random.default_rng(97)
+    for k in ("xkapR", "xkapP", "xkapA", "dxkapP_dT", "dxkapA_dT"):
+        inp[k] = 10.0 ** rng.uniform(-3.0, -1.0, size=inp["T"].shape)
+    outs, sc = run_ref("kappa", p, inp, prefill=True)
+    save("kappa_hminus8_12x16x12", p, inp, outs, sc,
+         ["xkapR", "xkapP", "xkapA", "dxkapP_dT", "dxkapA_dT", "sig", "upperbnd"])
+    o = Oracle(P.make_params(nx, ny, nz, ikaptyp=1, kap0=1e-2))
+    o.load_state(base_inputs(o.p))
+    o.bounds()
+    o.pressg()
+    dt = o.schritt()
+    pe = P.make_params(nx, ny, nz, ikaptyp=8, epsmat=1e-8, tirr=2000.0)
+    outs, sc = run_ref("energy", pe, inp, delt=dt, iii=2, prefill=True)
+    save("energy_hminus8_12x16x12", pe, inp, outs, sc,
+         ["T", "ER", "delta_ER", "xkapR", "xkapP", "xkapA", "dxkapP_dT", "dxkapA_dT", "FLIMX", "FQ", "FR",
+          "RHS_ER", "upperbnd"], {"delt": dt, "iii": 2})
+
+
 def viscosity_ieta23():
     """FORGAS3D with ETACALC's IETA 2 (the kinematic viscosity ramped from
     1e9 to XNUE = 3e13 between days 282.5 and 292.5, here at day 287.5) and
@@ -639,6 +664,8 @@ if __name__ == "__main__":
         column()
     elif sys.argv[1:] == ["ieta23"]:
         viscosity_ieta23()
+    elif sys.argv[1:] == ["hminus8"]:
+        hminus8()
     elif sys.argv[1:] == ["variants"]:
         variants()
     elif sys.argv[1:] == ["adjust"]:

@@ -73,7 +73,7 @@ def test_field_table_matches_header(f):
 
 
 @pytest.mark.parametrize("field,value,msg", [("bcxmin", 9, "BCXMIN"), ("ibdrad", 5, "IBDRAD"),
-                                             ("ikaptyp", 8, "IKAPTYP"), ("iflimtyp", 3, "IFLIMTYP"),
+                                             ("ikaptyp", 15, "IKAPTYP"), ("iflimtyp", 3, "IFLIMTYP"),
                                              ("f_incident", 1, "F_INCIDENT"), ("isigtyp", 2, "ISIGTYP"),
                                              ("nx", 31, "even")])
 def test_init_rejects_unsupported_switches(field, value, msg):

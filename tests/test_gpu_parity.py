@@ -328,6 +328,31 @@ def test_coupled_energy_column_table(ttgpu):
         d.close()
 
 
+def test_kappa_hminus(ttgpu):
+    """IKAPTYP 8 (H- bound-free) against the reference on prefilled opacity
+    arrays (pow from the device math library: within 1e-12)."""
+    g = C.load("kappa_hminus8_12x16x12")
+    p, d = _dev(ttgpu, g)
+    try:
+        d.bounds()
+        d.kappa()
+        _cmp(g, p, d, rel=1e-12)
+    finally:
+        d.close()
+
+
+def test_coupled_energy_hminus(ttgpu):
+    g = C.load("energy_hminus8_12x16x12")
+    p, d = _dev(ttgpu, g)
+    try:
+        st, _ = d.coupled_energy(float(g["delt"]), int(g["iii"]))
+        assert st.energy_iter == int(g["scal_energy_iter"])
+        assert st.sor_iter_last == int(g["scal_SOR_ITER"])
+        _cmp(g, p, d, rel=1e-12, scaled_tol=1e-10)
+    finally:
+        d.close()
+
+
 def test_coupled_energy_constant_gamma_opacity(ttgpu):
     """IKAPTYP 13 (kappa_A = gamma_OPC^4 ZKAPMULT) against the reference."""
     g = C.load("energy_const13_12x16x12")

@@ -468,3 +468,23 @@ def test_coupled_energy_column_table():
     assert st.energy_iter == int(g["scal_energy_iter"])
     assert st.sor_iter_last == int(g["scal_SOR_ITER"])
     _check_fields(g, p, o)
+
+
+def test_kappa_hminus():
+    """KAPPA with IKAPTYP 8 (H- bound-free, pprphy.f:176-198) on prefilled
+    opacity arrays: bit for bit, the j = -1, 0 columns keeping theirs."""
+    def act(o, g):
+        o.bounds()
+        o.kappa()
+    g, p, o, _ = _run("kappa_hminus8_12x16x12", act)
+    _check_fields(g, p, o)
+    xr = g["out_xkapR"]
+    np.testing.assert_array_equal(xr[:, :2, :], g["in_xkapR"][:, :2, :])  # j = -1, 0 untouched
+    np.testing.assert_array_equal(g["out_xkapP"], xr)
+
+
+def test_coupled_energy_hminus():
+    g, p, o, st = _run("energy_hminus8_12x16x12", lambda o, g: o.coupled_energy(float(g["delt"]), int(g["iii"]))[0])
+    assert st.energy_iter == int(g["scal_energy_iter"])
+    assert st.sor_iter_last == int(g["scal_SOR_ITER"])
+    _check_fields(g, p, o)

@@ -175,6 +175,33 @@ __global__ void k_kappa_tab(Geo g, Met m, const int32_t* __restrict__ ub, const 
     DKA[t] = (c10[it + TT_FREEDMAN_NT * iw] - c10[it1 + TT_FREEDMAN_NT * iw]) / dt;
 }
 
+// ---- IKAPTYP 8: H- bound-free absorption (pprphy.f:176-198).  The
+// reference's j loop starts at 1: cells at j < 1 keep their xkapR and dk/dT
+// from before, and kappa_P = kappa_A = kappa_R everywhere after it.  Cells
+// above upperbnd take the upperbnd values.  0.00342 is a REAL constant.
+__global__ void k_kappa_hminus(Geo g, const int32_t* __restrict__ ub, const double* __restrict__ RH,
+                               const double* __restrict__ T, double* XKR, double* XKP, double* XKA, double* DKP,
+                               double* DKA, double* SIG, double sig) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.n3) return;
+    SIG[t] = sig;
+    int i, j, k;
+    box_ijk(g, t, i, j, k);
+    double kr = XKR[t];
+    if (j >= 1) {
+        const int u = ub[g.idx2(j, k)];
+        const long long n = g.idx(i <= u ? i : u, j, k);
+        const double r5 = pow(RH[n], 0.5), tt = T[n] / 2500.0;
+        kr = 0.95 * r5 * pow(tt, 9.0);
+        const double dk = (double)0.00342f * r5 * pow(tt, 8.0);
+        XKR[t] = kr;
+        DKP[t] = dk;
+        DKA[t] = dk;
+    }
+    XKP[t] = kr;
+    XKA[t] = kr;
+}
+
 // ---- IKAPTYP 9-12, 14: Burrows tables
 // Axes (ppmod.f:724-745): log10 T and log10 rho, REAL constants of DATA
 // statements into double arrays, so each is the float value widened.
@@ -753,6 +780,13 @@ static int kappa_impl() {
         TT_LAUNCH();
         return 0;
     }
+    if (L.p.ikaptyp == 8) {
+        k_kappa_hminus<<<nblk(L.g.n3, 256), 256, 0, L.s>>>(L.g, L.ub, f[TT_RH], f[TT_T], f[TT_XKAPR], f[TT_XKAPP],
+                                                           f[TT_XKAPA], f[TT_DXKAPP_DT], f[TT_DXKAPA_DT],
+                                                           f[TT_SIG], sig);
+        TT_LAUNCH();
+        return 0;
+    }
     if (L.p.ikaptyp == 13) {  // constants (pprphy.f:235-241); gamma_OPC**4.d0 from the host's libm
         const double xa = std::pow(L.p.gamma_opc, 4.0) * L.p.zkapmult;
         const double v[6] = {L.p.zkapmult, L.p.zkapmult, xa, 0.0, 0.0, sig};
@@ -763,7 +797,7 @@ static int kappa_impl() {
         }
         return 0;
     }
-    if (L.p.ikaptyp != 1 && L.p.ikaptyp != 26) return fail(1, "KAPPA: IKAPTYP must be 1, 4, 5, 9-14 or 26");
+    if (L.p.ikaptyp != 1 && L.p.ikaptyp != 26) return fail(1, "KAPPA: IKAPTYP must be 1, 4-14 or 26");
     k_kappa<<<nblk(L.g.n3, 256), 256, 0, L.s>>>(L.g, L.p, L.ub, f[TT_RH], f[TT_T], f[TT_XKAPR], f[TT_XKAPP],
                                                 f[TT_XKAPA], f[TT_DXKAPP_DT], f[TT_DXKAPA_DT], f[TT_SIG], sig);
     TT_LAUNCH();

@@ -244,9 +244,8 @@ int tt_init(const tt_params* p) {
     if (p->bcxmin != 12) return fail(1, "BCXMIN must be 12");
     if (p->ibdrad != 3 && p->ibdrad != 4 && p->ibdrad != 7 && p->ibdrad != 9)
         return fail(1, "IBDRAD must be 3, 4, 7 or 9");
-    if (p->ikaptyp != 1 && !(p->ikaptyp >= 4 && p->ikaptyp <= 7) && !(p->ikaptyp >= 9 && p->ikaptyp <= 14) &&
-        p->ikaptyp != 26)
-        return fail(1, "IKAPTYP must be 1, 4-7, 9-14 or 26");
+    if (p->ikaptyp != 1 && !(p->ikaptyp >= 4 && p->ikaptyp <= 14) && p->ikaptyp != 26)
+        return fail(1, "IKAPTYP must be 1, 4-14 or 26");
     if (p->iflimtyp != 1 && p->iflimtyp != 2) return fail(1, "IFLIMTYP must be 1 or 2");
     if (p->f_incident != 0 && p->f_incident != 2) return fail(1, "F_INCIDENT must be 0 or 2");
     if (p->isigtyp != 0 && p->isigtyp != 1) return fail(1, "ISIGTYP must be 0 or 1");
