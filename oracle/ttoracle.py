@@ -91,6 +91,13 @@ class Oracle:
         self.L.tor_download_metric(m, out.ctypes.data_as(ctypes.c_void_p))
         return out
 
+    def upload_metric(self, m, a):
+        lo, hi = ctypes.c_int32(), ctypes.c_int32()
+        self.L.tor_metric_info(m, ctypes.byref(lo), ctypes.byref(hi))
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        assert a.shape == (hi.value - lo.value + 1,), (m, a.shape)
+        self._check(self.L.tor_upload_metric(m, a.ctypes.data_as(ctypes.c_void_p)), "upload_metric")
+
     def load_state(self, st: dict):
         for name, a in st.items():
             if name in P.FIELD_IDS:

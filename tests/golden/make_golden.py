@@ -649,8 +649,42 @@ def checkpoints():
     save("checkpoint_12x16x12", pe, inp, outs, scal, ["T", "ER"], extra)
 
 
+# The shipped ipp's frame rotation: IROTATE = 1, PROT = 3.52 days (ipp:11),
+# OMROT = 2 PI / (PROT DAY) (ppinit.f:501-502), DAY the REAL literal 8.64E4
+# (ppmod.f:1083).
+OMROT_IPP = 2.0 * P.PI / (3.52 * float(np.float32(8.64e4)))
+
+
+def rotating():
+    """ADVECT3D (GD = (G + OMROT) r^2 cos^2, ppadv.f:1082, and back,
+    :1518-1528), FORGAS3D (CENTRIFCORIOLIS with OMROT, ppfor.f:426-486) and
+    two full steps in the rotating frame of the shipped ipp."""
+    nx, ny, nz = 12, 16, 12
+    keep_state = ["T", "RH", "V", "G", "H", "ER", "upperbnd"]
+    p = P.make_params(nx, ny, nz, ikaptyp=1, kap0=1e-2, omrot=OMROT_IPP)
+    inp = base_inputs(p, seed=37)
+    o = Oracle(p)
+    o.load_state(inp)
+    o.bounds()
+    o.pressg()
+    dt = o.schritt()
+    outs, sc = run_ref("advect", p, inp, delt=dt)
+    save("advect_omrot_12x16x12", p, inp, outs, sc, keep_state + ["GD", "RHQX", "RHQY", "RHQZ"],
+         {"delt": dt})
+    outs, sc = run_ref("forgas", p, inp, delt=dt)
+    save("forgas_omrot_12x16x12", p, inp, outs, sc, keep_state + ["PG", "centcorx", "centcorz"],
+         {"delt": dt})
+    ps = P.make_params(nx, ny, nz, ikaptyp=1, kap0=1e-1, epsmat=1e-8, tirr=2000.0, omrot=OMROT_IPP)
+    inp_st = base_inputs(ps, seed=41)
+    outs, sc = run_ref("step", ps, inp_st, delt=dt, iii=2, nsteps=2)
+    save("step2_omrot_12x16x12", ps, inp_st, outs, sc, keep_state + ["delta_ER", "PG"],
+         {"delt": dt, "iii": 2, "nsteps": 2})
+
+
 if __name__ == "__main__":
-    if sys.argv[1:] == ["checkpoints"]:
+    if sys.argv[1:] == ["rotating"]:
+        rotating()
+    elif sys.argv[1:] == ["checkpoints"]:
         checkpoints()
     elif sys.argv[1:] == ["viscosity"]:
         viscosity()

@@ -68,11 +68,14 @@ def test_full_step_vs_oracle(setup):
         e = C.rel_err(gpu[k], o.download(k))
         print(f"{k}: max rel err {e:.2e}")
         assert e <= 1e-12, (k, e)
-    for k in ("T", "ER"):                     # after the FP64 SOR solve
+    # after the FP64 SOR solve: the same solve where the counts are equal,
+    # else within the solver tolerance
+    same = sg.energy.sor_iter_total == so.energy.sor_iter_total
+    for k in ("T", "ER"):
         reg = C.compare_region(k, NX, NY, NZ)
         e = C.rel_err(gpu[k][reg], o.download(k)[reg])
         print(f"{k}: max rel err {e:.2e}")
-        assert e <= 1e-9, (k, e)
+        assert e <= (1e-12 if same else p.epsmat), (k, e)
 
 
 def _residual_ratio(A, B, Cc, D, E, F, G, R, x):

@@ -2,8 +2,8 @@
 ppsub.f:316-391) on the GPU against the CPU oracle: bit-identical for the
 clamps themselves (comparisons and divisions by the same host-computed
 metrics); a step with every clamp switched on within the bars of the plain
-step test (1e-10 for the hydro fields, 1e-7 for T, whose energy solve stops
-on an L1 norm summed in another order)."""
+step test (1e-12 where the SOR counts are equal; where the L1 norm, summed in
+another order, moves the stop by one iteration, the solver tolerance)."""
 import numpy as np
 import pytest
 
@@ -25,16 +25,17 @@ def _run_oracle(p, st, fn):
     from oracle.ttoracle import Oracle
     o = Oracle(p)
     o.load_state(st)
-    fn(o)
-    return {k: o.download(k) for k in KEYS}
+    out = {"sor": fn(o)}
+    out.update({k: o.download(k) for k in KEYS})
+    return out
 
 
 def _run_gpu(p, st, fn):
     from ttcode_amd import ttgpu
     d = ttgpu.Device(p)
     d.load_state(st)
-    fn(d)
-    out = {k: d.download(k) for k in KEYS}
+    out = {"sor": fn(d)}
+    out.update({k: d.download(k) for k in KEYS})
     d.close()
     return out
 
@@ -71,12 +72,15 @@ def test_step_with_limits_matches_oracle():
 
     def fn(x):
         x.set_limits(**lim)
+        its = []
         for s in range(2):
-            x.step(dt, 2 + s)
+            its.append(x.step(dt, 2 + s)[0].energy.sor_iter_total)
         assert x.get_limits().hydrotime == 10 * DAY + dt + dt
+        return its
     want = _run_oracle(p, st, fn)
     got = _run_gpu(p, st, fn)
+    same = got["sor"] == want["sor"]
     for k in KEYS:
         e = C.rel_err(got[k], want[k])
-        print(f"{k}: max rel err {e:.2e}")
-        assert e <= (1e-7 if k == "T" else 1e-10), (k, e)
+        print(f"{k}: max rel err {e:.2e} (SOR {got['sor']} vs {want['sor']})")
+        assert e <= (1e-12 if same else p.epsmat), (k, e)

@@ -39,5 +39,6 @@ def test_mixed_precision_steps_vs_fp64_oracle():
         worst = max(worst, *errs.values())
     d.close()
     # the solve stops at an L1 residual ratio of 1e-10; the damped update
-    # (conv_parm = 0.1) and T = Q / (rho c_v) carry that into E and T
-    assert worst < 1e-9, worst
+    # (conv_parm = 0.1) and T = Q / (rho c_v) carry that into E and T: the
+    # bar is the solver tolerance (measured: below 1e-13)
+    assert worst < p.epsmat, worst

@@ -107,16 +107,19 @@ def test_bounds(ttgpu, name, rel):
     d.close()
 
 
-def test_advect3d(ttgpu):
-    g = C.load("advect_12x16x12")
+# *_omrot_*: the shipped ipp's rotating frame, OMROT = 2 PI / (3.52 days)
+@pytest.mark.parametrize("name", ["advect_12x16x12", "advect_omrot_12x16x12"])
+def test_advect3d(ttgpu, name):
+    g = C.load(name)
     p, d = _dev(ttgpu, g)
     d.advect3d(float(g["delt"]))
     _cmp(g, p, d)
     d.close()
 
 
-def test_forgas3d(ttgpu):
-    g = C.load("forgas_12x16x12")
+@pytest.mark.parametrize("name", ["forgas_12x16x12", "forgas_omrot_12x16x12"])
+def test_forgas3d(ttgpu, name):
+    g = C.load(name)
     p, d = _dev(ttgpu, g)
     d.bounds()
     d.qdens()
@@ -393,7 +396,7 @@ def test_step_burrows_table_32x64x32_vs_oracle(ttgpu):
         for k in ("RH", "V", "G", "H", "T", "ER", "xkapR", "dxkapP_dT"):
             reg = C.compare_region(k, p.nx, p.ny, p.nz)
             e = C.rel_err(d.download(k)[reg], o.download(k)[reg])
-            tol = 1e-12 if same or k in ("RH", "V", "G", "H") else 1e-7
+            tol = 1e-12 if same or k in ("RH", "V", "G", "H") else p.epsmat
             print(f"  {k}: {e:.2e}")
             assert e <= tol, (k, e)
     finally:
@@ -427,7 +430,7 @@ def test_step_freedman_table_32x64x32_vs_oracle(ttgpu):
         for k in ("RH", "V", "G", "H", "T", "ER", "xkapR", "dxkapP_dT"):
             reg = C.compare_region(k, p.nx, p.ny, p.nz)
             e = C.rel_err(d.download(k)[reg], o.download(k)[reg])
-            tol = 1e-12 if same or k in ("RH", "V", "G", "H") else 1e-7
+            tol = 1e-12 if same or k in ("RH", "V", "G", "H") else p.epsmat
             print(f"  {k}: {e:.2e}")
             assert e <= tol, (k, e)
     finally:
@@ -578,14 +581,14 @@ def test_step_viscosity_32x64x32_vs_oracle(ttgpu):
         for k in ("RH", "V", "G", "H", "T", "ER", "divtx", "DISFN"):
             reg = C.compare_region(k, p.nx, p.ny, p.nz)
             e = C.rel_err(d.download(k)[reg], o.download(k)[reg])
-            tol = 1e-12 if same or k not in ("T", "ER") else 1e-7
+            tol = 1e-12 if same or k not in ("T", "ER") else p.epsmat
             print(f"  {k}: {e:.2e}")
             assert e <= tol, (k, e)
     finally:
         d.close()
 
 
-@pytest.mark.parametrize("name", ["step2_12x16x12", "step2_ibdrad9_12x16x12"])
+@pytest.mark.parametrize("name", ["step2_12x16x12", "step2_ibdrad9_12x16x12", "step2_omrot_12x16x12"])
 def test_two_steps(ttgpu, name):
     g = C.load(name)
     p, d = _dev(ttgpu, g)
@@ -623,6 +626,26 @@ def test_schritt_matches_oracle(ttgpu):
     d.close()
 
 
+@pytest.mark.parametrize("grid", ["radial", "phi", "theta"])
+def test_schritt_closed_form(ttgpu, grid):
+    """k_schritt against SCHRITT's Courant bound written in numpy
+    (ppsub.f:49-78,129-131), on perturbed metrics and fast flows, on grids
+    where each of the three Courant terms binds in turn."""
+    from tests import closed_forms as CF
+    p = P.make_params(12, 16, 12, **CF.SCHRITT_GRIDS[grid])
+    st = P.generate_state(p, seed=11, vel_frac=0.5)
+    d = ttgpu.Device(p)
+    d.load_state(st)
+    d.pressg()
+    met = CF.perturbed_metrics(d.metric, np.random.default_rng(111))
+    for m, a in met.items():
+        d.upload_metric(m, a)
+    got = d.schritt()
+    want, _ = CF.schritt_cfl(p, st, met)
+    d.close()
+    assert got == want, (got, want)
+
+
 def test_step_32x64x32_vs_oracle(ttgpu):
     """One full step of the CPU-runnable workload (nr=32, ntheta=32, nphi=64)."""
     from oracle.ttoracle import Oracle
@@ -639,14 +662,19 @@ def test_step_32x64x32_vs_oracle(ttgpu):
     d.load_state(st)
     sg, _ = d.step(dt, 1)
     assert sg.energy.energy_iter == so.energy.energy_iter
-    assert abs(sg.energy.sor_iter_total - so.energy.sor_iter_total) <= so.energy.energy_iter
+    same = sg.energy.sor_iter_total == so.energy.sor_iter_total
+    assert same or abs(sg.energy.sor_iter_total - so.energy.sor_iter_total) <= so.energy.energy_iter
     for k in ("RH", "V", "G", "H"):
         e = C.rel_err(d.download(k), o.download(k))
-        assert e <= 1e-10, (k, e)
+        assert e <= 1e-12, (k, e)
+    # equal SOR counts: the same solve, to the north-star bar; a stop one
+    # iteration apart (the L1 norm summed in another order) moves E by less
+    # than the solver tolerance
     for k in ("T", "ER"):
         reg = C.compare_region(k, p.nx, p.ny, p.nz)
         e = C.rel_err(d.download(k)[reg], o.download(k)[reg])
-        assert e <= 1e-7, (k, e)
+        print(f"  {k}: {e:.2e} (SOR {sg.energy.sor_iter_total} vs {so.energy.sor_iter_total})")
+        assert e <= (1e-12 if same else p.epsmat), (k, e)
     d.close()
 
 
@@ -692,7 +720,7 @@ def test_ragged_shapes_vs_oracle(ttgpu, shape):
         for k in ("RH", "V", "G", "H", "T", "ER"):
             reg = C.compare_region(k, nx, ny, nz)
             e = C.rel_err(d.download(k)[reg], o.download(k)[reg])
-            tol = 1e-12 if same or k in ("RH", "V", "G", "H") else 1e-7
+            tol = 1e-12 if same or k in ("RH", "V", "G", "H") else p.epsmat
             print(f"  step {k}: {e:.2e} (SOR {sg.energy.sor_iter_total} vs {so.energy.sor_iter_total})")
             assert e <= tol, (k, e)
     finally:

@@ -48,18 +48,41 @@ def test_bounds(name):
     _check_fields(g, p, o)
 
 
-def test_advect3d():
-    g, p, o, _ = _run("advect_12x16x12", lambda o, g: o.advect3d(float(g["delt"])))
+# *_omrot_*: the shipped ipp's rotating frame, OMROT = 2 PI / (3.52 days)
+# (ipp:11, ppinit.f:501-502), which enters GD (ppadv.f:1082,1518-1528) and
+# CENTRIFCORIOLIS (ppfor.f:426-486)
+@pytest.mark.parametrize("name", ["advect_12x16x12", "advect_omrot_12x16x12"])
+def test_advect3d(name):
+    g, p, o, _ = _run(name, lambda o, g: o.advect3d(float(g["delt"])))
     _check_fields(g, p, o)
 
 
-def test_forgas3d():
+@pytest.mark.parametrize("name", ["forgas_12x16x12", "forgas_omrot_12x16x12"])
+def test_forgas3d(name):
     def act(o, g):
         o.bounds()
         o.L.tor_qdens()
         o.forgas3d(float(g["delt"]))
-    g, p, o, _ = _run("forgas_12x16x12", act)
+    g, p, o, _ = _run(name, act)
     _check_fields(g, p, o)
+
+
+def test_rotating_fixtures_rotate():
+    """The rotating fixtures differ from a non-rotating run of the same
+    inputs: OMROT is large against the flow (2.1e-5 rad/s against G of
+    order 1e-7), so a dropped OMROT term fails their comparisons."""
+    for name, act in (("advect_omrot_12x16x12", lambda o, g: o.advect3d(float(g["delt"]))),
+                      ("forgas_omrot_12x16x12", lambda o, g: (o.bounds(), o.L.tor_qdens(),
+                                                             o.forgas3d(float(g["delt"]))))):
+        g = C.load(name)
+        p = C.params_of(g)
+        assert p.omrot > 2e-5
+        p.omrot = 0.0
+        o = Oracle(p, threads=1)
+        o.load_state(C.inputs_of(g))
+        act(o, g)
+        moved = {k: C.rel_err(o.download(k), g["out_" + k]) for k in ("V", "G", "H")}
+        assert max(moved.values()) > 1e-6, (name, moved)
 
 
 @pytest.mark.parametrize("name", ["energy_const_12x16x12", "energy_pow8_12x16x12",
@@ -127,7 +150,7 @@ def test_coupled_energy_adjusts_omega_at_step_one():
     assert not np.array_equal(o2.download("T"), g["out_T"])
 
 
-@pytest.mark.parametrize("name", ["step2_12x16x12", "step2_ibdrad9_12x16x12"])
+@pytest.mark.parametrize("name", ["step2_12x16x12", "step2_ibdrad9_12x16x12", "step2_omrot_12x16x12"])
 def test_two_steps(name):
     def act(o, g):
         for s in range(int(g["nsteps"])):

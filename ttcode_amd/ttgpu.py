@@ -125,6 +125,15 @@ class Device:
         self._check(self.L.tt_download_metric(m, out.ctypes.data), "tt_download_metric")
         return out
 
+    def upload_metric(self, m, a):
+        """Replace one 1-D metric array (declared bounds, as tt_metric_info)."""
+        lo, hi = ctypes.c_int32(), ctypes.c_int32()
+        self.L.tt_metric_info(m, ctypes.byref(lo), ctypes.byref(hi))
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        if a.shape != (hi.value - lo.value + 1,):
+            raise ValueError(f"metric {m}: shape {a.shape} != ({hi.value - lo.value + 1},)")
+        self._check(self.L.tt_upload_metric(m, a.ctypes.data), "tt_upload_metric")
+
     def load_state(self, st: dict):
         for name, a in st.items():
             if name in P.FIELD_IDS:
