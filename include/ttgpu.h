@@ -119,7 +119,13 @@ enum {
     TT_DIVTY = 85,        /* divty (0:nx+1,1:ny+1,0:nz+1)     ppmod.f:440 */
     TT_DIVTZ = 86,        /* divtz (0:nx+1,0:ny+1,1:nz+1)     ppmod.f:441 */
     TT_DISFN = 87,        /* DISFN (0:nx+1,0:ny+1,0:nz+1)     ppmod.f:442 */
-    TT_FIELD_MAX = 90
+    /* conv_flux(c,:,:,:), c = 1..3: the thermal-energy advection fluxes
+     * FLX/FLY/FLZ of the last ADVECT3D through each cell's low face
+     * (ppadv.f:857-865; declared conv_flux(3,0:nx+1,0:ny+1,0:nz+1),
+     * ppmod.f:346).  The library holds the cells ADVECT3D's loops define and
+     * PRINTOUT writes (ppprt.f:51,78,96): (1:nx,1:ny,1:nz) per component. */
+    TT_CONV_FLUX1 = 88, TT_CONV_FLUX2 = 89, TT_CONV_FLUX3 = 90,
+    TT_FIELD_MAX = 92
 };
 
 /* Declared bounds of a field: lo[3], hi[3] (hi[2]=lo[2]=0 and rank 2 for

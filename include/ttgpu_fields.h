@@ -49,6 +49,7 @@ static inline int tt_field_bounds(int f, int nx, int ny, int nz,
     case TT_EVAL1: case TT_EVAL2: case TT_EVAL3: case TT_EVAL4: case TT_EVAL5:
     case TT_EVAL6: case TT_EVAL7: case TT_EVAL8: case TT_EVAL_1_6:
     case TT_JACOB_ETA: case TT_FQ: case TT_FR:
+    case TT_CONV_FLUX1: case TT_CONV_FLUX2: case TT_CONV_FLUX3:
         for (d = 0; d < 3; ++d) { lo[d] = 1; hi[d] = n[d]; }
         return 0;
     case TT_FLIMX: case TT_DIFRX: case TT_FLUX_X:

@@ -249,6 +249,26 @@ program tt_ref_harness
      open(12, file=trim(dir)//'/SNAP_000.0001', form='unformatted', status='replace')
      call PRINTOUT(1)
      close(12)
+  case ('stepprint')
+     ! one step in the pp.f order, then PRINTOUT(1) of the state it leaves:
+     ! conv_flux holds that step's ADVECT3D fluxes (ppadv.f:857-865) and
+     ! DISFN its FORGAS3D dissipation (ppvisc.f, with IVIS and IDISS); CS
+     ! as SCHRITT's SOUND leaves it before the next step prints (pp.f:565-598)
+     III = h_iii
+     call ADVECT3D
+     call FORGAS3D
+     call COUPLED_ENERGY_EQS
+     call BOUNDS
+     call BOUNDS
+     call PRESSG
+     call SOUND
+     ZEIT = h_zeit; NUM_ITER = h_num_iter
+     TinitBottom = h_tinitbottom; Tinittop = h_tinittop
+     rhinittop = h_rhinittop; rhinitbottom = h_rhinitbottom
+     hydrotime = h_hydrotime; Maccreated = h_maccreated
+     open(12, file=trim(dir)//'/SNAP_000.0001', form='unformatted', status='replace')
+     call PRINTOUT(1)
+     close(12)
   case ('restart')
      ! READRESTART (ppinit.f:1543-1600) of the file AKTTD in the working
      ! directory; the global arrays it fills are written out as read
@@ -290,6 +310,9 @@ program tt_ref_harness
      call dump3(SOR_D, 'SOR_D'); call dump3(SOR_E, 'SOR_E'); call dump3(SOR_F, 'SOR_F')
      call dump3(SOR_G, 'SOR_G')
      call dump3(centcorx, 'centcorx'); call dump3(centcorz, 'centcorz')
+     call dump3(conv_flux(1,1:locNX,1:locNY,1:locNZ), 'conv_flux1')
+     call dump3(conv_flux(2,1:locNX,1:locNY,1:locNZ), 'conv_flux2')
+     call dump3(conv_flux(3,1:locNX,1:locNY,1:locNZ), 'conv_flux3')
      call dump3(CS, 'CS')
      call dump2i(upperbnd, 'upperbnd')
   end if
@@ -332,7 +355,7 @@ contains
     RHQX = 0.d0; RHQY = 0.d0; RHQZ = 0.d0; STELLARINPUT = 0.d0; upperbnd = 0
     centcorx = 0.d0; centcory = 0.d0; centcorz = 0.d0
     gravx = 0.d0; gravy = 0.d0; gravz = 0.d0; divtx = 0.d0; divty = 0.d0; divtz = 0.d0
-    eta = 0.d0; DISFN = 0.d0
+    eta = 0.d0; DISFN = 0.d0; conv_flux = 0.d0
     FLIMX = 0.d0; FLIMY = 0.d0; FLIMZ = 0.d0; difrx = 0.d0; difry = 0.d0; difrz = 0.d0
     flux_x = 0.d0; flux_y = 0.d0; flux_z = 0.d0
     xkapR = 0.d0; xkapP = 0.d0; xkapA = 0.d0; gam_kap = 0.d0

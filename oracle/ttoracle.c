@@ -60,6 +60,7 @@ typedef struct {
     double *FLIMX, *FLIMY, *FLIMZ, *DIFRX, *DIFRY, *DIFRZ, *FLUXX, *FLUXY, *FLUXZ;
     double *CCX, *CCY, *CCZ, *Q;
     double *ETA, *DIVTX, *DIVTY, *DIVTZ, *DISFN;
+    double *CF1, *CF2, *CF3;  /* conv_flux(1..3,:,:,:) */
     /* 2-D */
     int32_t* UB;
     double* MUSTAR;
@@ -206,6 +207,9 @@ static double* field_ptr(int f)
     case TT_DIVTY: return O.DIVTY;
     case TT_DIVTZ: return O.DIVTZ;
     case TT_DISFN: return O.DISFN;
+    case TT_CONV_FLUX1: return O.CF1;
+    case TT_CONV_FLUX2: return O.CF2;
+    case TT_CONV_FLUX3: return O.CF3;
     case TT_CENTCORY: return O.CCY;
     case TT_CENTCORZ: return O.CCZ;
     case TT_Q: return O.Q;
@@ -395,7 +399,7 @@ void tor_free(void)
                      &O.EV4, &O.EV5, &O.EV6, &O.EV7, &O.EV8, &O.EV16, &O.JETA, &O.FQ, &O.FR,
                      &O.FLIMX, &O.FLIMY, &O.FLIMZ, &O.DIFRX, &O.DIFRY, &O.DIFRZ, &O.FLUXX,
                      &O.FLUXY, &O.FLUXZ, &O.CCX, &O.CCY, &O.CCZ, &O.Q, &O.ETA, &O.DIVTX,
-                     &O.DIVTY, &O.DIVTZ, &O.DISFN};
+                     &O.DIVTY, &O.DIVTZ, &O.DISFN, &O.CF1, &O.CF2, &O.CF3};
     for (size_t i = 0; i < sizeof f3 / sizeof f3[0]; ++i) { free(*f3[i]); *f3[i] = NULL; }
     free(O.UB); O.UB = NULL;
     free(O.MUSTAR); O.MUSTAR = NULL;
@@ -461,7 +465,7 @@ int tor_init(const tt_params* p)
                      &O.EV4, &O.EV5, &O.EV6, &O.EV7, &O.EV8, &O.EV16, &O.JETA, &O.FQ, &O.FR,
                      &O.FLIMX, &O.FLIMY, &O.FLIMZ, &O.DIFRX, &O.DIFRY, &O.DIFRZ, &O.FLUXX,
                      &O.FLUXY, &O.FLUXZ, &O.CCX, &O.CCY, &O.CCZ, &O.Q, &O.ETA, &O.DIVTX,
-                     &O.DIVTY, &O.DIVTZ, &O.DISFN};
+                     &O.DIVTY, &O.DIVTZ, &O.DISFN, &O.CF1, &O.CF2, &O.CF3};
     for (size_t i = 0; i < sizeof f3 / sizeof f3[0]; ++i) {
         *f3[i] = alloc3();
         if (!*f3[i]) return seterr(1, "out of memory");
@@ -1812,6 +1816,16 @@ int tor_advect3d(double delt) /* ADVECT3D, ppadv.f:525-1577 (IRHT=1, ISOTH=0, IE
                 A(FLZ, i, j, k) = (SURZA(k) - CCZA(k) * S[k]) * S[k] *
                                   (A(ED, i, j, IND[k]) + (XZA(k) - XZB(IND[k]) - half * S[k]) * GRAD[IND[k]]);
         }
+    /* the thermal advection flux kept (ppadv.f:857-865); PRINTOUT writes
+       cells 1..n of each component */
+#pragma omp parallel for
+    for (int k = 1; k <= NZ; ++k)
+        for (int j = 1; j <= NY; ++j)
+            for (int i = 1; i <= NX; ++i) {
+                A(O.CF1, i, j, k) = A(FLX, i, j, k);
+                A(O.CF2, i, j, k) = A(FLY, i, j, k);
+                A(O.CF3, i, j, k) = A(FLZ, i, j, k);
+            }
 #pragma omp parallel for
     for (int k = 1; k <= NZ; ++k)
         for (int j = 1; j <= NY; ++j)

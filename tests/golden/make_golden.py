@@ -649,6 +649,32 @@ def checkpoints():
     save("checkpoint_12x16x12", pe, inp, outs, scal, ["T", "ER"], extra)
 
 
+def checkpoint_step():
+    """PRINTOUT(1) after a full viscous step (ADVECT3D, FORGAS3D with
+    ETACALC IETA 0 + VISC + dissipative heating, COUPLED_ENERGY_EQS, BOUNDS x2,
+    PRESSG, SOUND): the file carries that step's conv_flux and DISFN."""
+    nx, ny, nz = 12, 16, 12
+    pe = P.make_params(nx, ny, nz, ikaptyp=1, kap0=1e-1, epsmat=1e-8, tirr=2000.0)
+    inp = base_inputs(pe, seed=43, contrast=0.05)
+    o = Oracle(pe)
+    o.load_state(inp)
+    o.bounds()
+    o.pressg()
+    dt = o.schritt()
+    sc = dict(CKPT_SCALARS)
+    vis = dict(h_ivis=1, h_ivistyp=1, h_ieta=0, h_idiss=1)
+    outs, scal = run_ref("stepprint", pe, inp, delt=dt, iii=2, files={"keep": ""}, xnue=3e14,
+                         collect=("SNAP_000.0001",), extra=dict(sc, **vis))
+    snap = np.frombuffer(outs["file:SNAP_000.0001"], dtype=np.uint8)
+    extra = {"delt": dt, "iii": 2, "snap_bytes": snap, "vis_xnue": 3e14}
+    for k, v in sc.items():
+        extra["ckpt_" + k] = v
+    for k, v in vis.items():
+        extra["vis_" + k] = v
+    save("checkpoint_visc_step_12x16x12", pe, inp, outs, scal,
+         ["T", "ER", "V", "DISFN", "conv_flux1", "conv_flux2", "conv_flux3"], extra)
+
+
 # The shipped ipp's frame rotation: IROTATE = 1, PROT = 3.52 days (ipp:11),
 # OMROT = 2 PI / (PROT DAY) (ppinit.f:501-502), DAY the REAL literal 8.64E4
 # (ppmod.f:1083).
@@ -684,6 +710,8 @@ def rotating():
 if __name__ == "__main__":
     if sys.argv[1:] == ["rotating"]:
         rotating()
+    elif sys.argv[1:] == ["checkpoint_step"]:
+        checkpoint_step()
     elif sys.argv[1:] == ["checkpoints"]:
         checkpoints()
     elif sys.argv[1:] == ["viscosity"]:

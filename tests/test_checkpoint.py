@@ -132,3 +132,34 @@ def test_restart_continues_the_run(case, tmp_path):
     o3.step(dt, 3)
     assert not np.array_equal(o3.download("RH"), after["RH"])
 
+
+
+def _viscous_step(x, g):
+    x.load_state(C.inputs_of(g))
+    x.set_viscosity(int(g["vis_h_ivis"]), int(g["vis_h_ivistyp"]), int(g["vis_h_ieta"]), int(g["vis_h_idiss"]),
+                    float(g["vis_xnue"]))
+    x.step(float(g["delt"]), int(g["iii"]))
+    x.pressg()
+    x.sound()
+
+
+def test_printout_after_a_viscous_step_is_the_references_file(tmp_path):
+    """PRINTOUT(1) after a full step with viscosity and dissipative heating:
+    the file carries that step's ADVECT3D fluxes (conv_flux, ppadv.f:857-865)
+    and FORGAS3D's dissipation (DISFN); the library's file equals the
+    reference's byte for byte, and neither record is zero."""
+    g = C.load("checkpoint_visc_step_12x16x12")
+    p = C.params_of(g)
+    o = Oracle(p, threads=1)
+    _viscous_step(o, g)
+    for c in (1, 2, 3):
+        np.testing.assert_array_equal(o.download(f"conv_flux{c}"), g[f"out_conv_flux{c}"])
+    sc = {k[5:]: g[k].item() for k in g.files if k.startswith("ckpt_")}
+    init = (sc["h_tinitbottom"], sc["h_tinittop"], sc["h_rhinittop"], sc["h_rhinitbottom"],
+            sc["h_hydrotime"], sc["h_maccreated"])
+    fn = tmp_path / "SNAP_000.0001"
+    K.write_printout(fn, o, p, sc["h_zeit"], float(g["delt"]), int(sc["h_num_iter"]), init)
+    assert fn.read_bytes() == g["snap_bytes"].tobytes()
+    d = K.read_printout(fn)
+    for k in ("DISFN", "conv_flux_1", "conv_flux_2", "conv_flux_3"):
+        assert np.count_nonzero(d[k]) > d[k].size // 2, k

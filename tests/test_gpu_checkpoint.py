@@ -90,3 +90,42 @@ def test_restart_through_a_second_device(tmp_path):
             assert e <= 1e-12, (k, e)
     finally:
         d2.close()
+
+
+def test_printout_after_a_viscous_step_from_the_device(tmp_path):
+    """PRINTOUT(1) from the device after a full viscous step: the records
+    of the reference's file, conv_flux (ADVECT3D's thermal fluxes) and DISFN
+    (FORGAS3D's dissipation) included, within 1e-12."""
+    from ttcode_amd import ttgpu
+    g = C.load("checkpoint_visc_step_12x16x12")
+    p = C.params_of(g)
+    sc, init = _scalars(g)
+    d = ttgpu.Device(p)
+    try:
+        d.load_state(C.inputs_of(g))
+        d.set_viscosity(int(g["vis_h_ivis"]), int(g["vis_h_ivistyp"]), int(g["vis_h_ieta"]),
+                        int(g["vis_h_idiss"]), float(g["vis_xnue"]))
+        st, _ = d.step(float(g["delt"]), int(g["iii"]))
+        assert st.energy.energy_iter == int(g["scal_energy_iter"])
+        assert st.energy.sor_iter_last == int(g["scal_SOR_ITER"])
+        d.pressg()
+        d.sound()
+        fn = tmp_path / "SNAP_gpu"
+        K.write_printout(fn, d, p, sc["h_zeit"], float(g["delt"]), int(sc["h_num_iter"]), init)
+    finally:
+        d.close()
+    ref_fn = tmp_path / "SNAP_ref"
+    ref_fn.write_bytes(g["snap_bytes"].tobytes())
+    mine, ref = K.read_printout(fn), K.read_printout(ref_fn)
+    assert mine.keys() == ref.keys()
+    for k, v in ref.items():
+        if isinstance(v, np.ndarray):
+            if v.dtype.kind == "i":
+                np.testing.assert_array_equal(mine[k], v, err_msg=k)
+                continue
+            e = C.rel_err(mine[k], v)
+            print(f"  {k}: {e:.2e}")
+            assert e <= 1e-12, (k, e)
+        else:
+            assert mine[k] == v, k
+    assert np.count_nonzero(ref["DISFN"]) and np.count_nonzero(ref["conv_flux_1"])

@@ -63,12 +63,11 @@ def _sub(src, name, nx, ny, nz, lo, hi):
 def write_printout(path, src, p: P.TTParams, zeit: float, delt: float, num_iter: int, init=(0.0,) * 6):
     """PRINTOUT(1) of ``src``'s state to ``path`` (ppprt.f:15-123): one rank
     holding the whole grid.  ``init`` = (TinitBottom, Tinittop, rhinittop,
-    rhinitbottom, hydrotime, Maccreated).  The convective flux and the
-    viscous dissipation, which this path does not compute, are written as
-    zeros (the reference's DISFN is zero without viscosity, pp.f:454)."""
+    rhinitbottom, hydrotime, Maccreated).  conv_flux is the last ADVECT3D's
+    thermal advection flux (ppadv.f:857-865) and DISFN the last FORGAS3D's
+    viscous dissipation (zero without viscosity, as pp.f:454 starts it)."""
     nx, ny, nz = p.nx, p.ny, p.nz
     inner = ((1, 1, 1), (nx, ny, nz))
-    zeros = np.zeros((nz, ny, nx))
     with open(path, "wb") as f:
         _record(f, _i4(nx, ny, nz))
         _record(f, _i4(nx, ny, nz, 1))
@@ -86,11 +85,11 @@ def write_printout(path, src, p: P.TTParams, zeit: float, delt: float, num_iter:
         _record(f, _i4(2))  # NCOSYS
         _record(f, _f8(_sub(src, "xkapR", nx, ny, nz, *inner)))
         _record(f, _f8(_sub(src, "FLIMX", nx, ny, nz, *inner)))
-        for _ in range(3):  # conv_flux(1..3, 1:nx, 1:ny, 1:nz)
-            _record(f, _f8(zeros))
+        for c in (1, 2, 3):  # conv_flux(c, 1:nx, 1:ny, 1:nz)
+            _record(f, _f8(src.download(f"conv_flux{c}")))
         _record(f, _f8(_sub(src, "xkapA", nx, ny, nz, *inner)))
         _record(f, _f8(_sub(src, "STELLARINPUT", nx, ny, nz, *inner)))
-        _record(f, _f8(zeros))  # DISFN
+        _record(f, _f8(_sub(src, "DISFN", nx, ny, nz, *inner)))
         _record(f, _f8(_sub(src, "xkapP", nx, ny, nz, *inner)))
 
 

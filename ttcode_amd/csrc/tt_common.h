@@ -134,6 +134,7 @@ struct Lib {
     int iii = 0;
     int have_prev_guess = 0;
     int64_t dev_bytes = 0;          // device memory held by the library
+    hipEvent_t step_ev[4] = {};     // tt_step's phase boundaries (created by tt_init)
 };
 
 extern Lib L;

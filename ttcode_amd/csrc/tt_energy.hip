@@ -734,7 +734,22 @@ static std::vector<double> with_pow10(const double* a, int n) {
     return t;
 }
 
+__global__ void k_clear_bad_kappa(Ctrl* c) {
+    if (threadIdx.x == 0) c->bad_kappa = 0;
+}
+
+// IKAPTYP 4-7 and 9-12, 14 interpolate tables, whose kernels flag states the
+// reference stops on (ctrl->bad_kappa)
+static bool table_kappa() {
+    const int t = L.p.ikaptyp;
+    return (t >= 4 && t <= 7) || (t >= 9 && t <= 12) || t == 14;
+}
+
 static int kappa_impl() {
+    if (table_kappa()) {
+        k_clear_bad_kappa<<<1, 64, 0, L.s>>>(L.ctrl);
+        TT_LAUNCH();
+    }
     double sig;
     if (L.p.isigtyp == 0) sig = 0.0;
     else if (L.p.isigtyp == 1) sig = 0.4;
@@ -863,6 +878,9 @@ int energy_impl(double delt, int iii, tt_energy_stats* st) {
     // KAPPA, SUBSIG, QDENS: T and RH stay fixed through the Newton loop, so the
     // opacities FLIM recomputes while energy_iter <= 1 are these
     if ((rc = kappa_impl())) return rc;
+    // a state outside the opacity tables stops here, before the Newton loop
+    // runs on partly stale opacities (pprphy.f:825-828,1040-1105 stop at once)
+    if (table_kappa() && (rc = sync_check())) return rc;
     if ((rc = qdens_impl())) return rc;
     if (L.p.f_incident > 1 && (rc = stellar_impl())) return rc;
     if (L.p.iflimtyp != 1 && L.p.iflimtyp != 2) return fail(1, "FLIM: IFLIMTYP must be 1 or 2");

@@ -144,6 +144,7 @@ struct AdvArgs {
     const double *RHQ;               // face density of the momentum (unused for energy)
     double* out;
     double dt;
+    double *cf1, *cf2, *cf3;         // energy only: conv_flux, the low-face fluxes (ppadv.f:857-865)
 };
 
 constexpr int ADV_TI = 32, ADV_TJ = 8, ADV_KC = 2;
@@ -204,6 +205,9 @@ struct AdvE {
                                   double fx1, double fy0, double fy1, double fz0, double fz1) {
         a.out[IX(i, j, k)] = a.A[IX(i, j, k)] - (fx1 - fx0) / m(TT_M_VOLXB, i) - (fy1 - fy0) / m(TT_M_VOLYB, j) -
                              (fz1 - fz0) / m(TT_M_VOLZB, k);
+        a.cf1[IX(i, j, k)] = fx0;
+        a.cf2[IX(i, j, k)] = fy0;
+        a.cf3[IX(i, j, k)] = fz0;
     }
 };
 
@@ -500,7 +504,8 @@ int advect_impl(double dt) {
     long long nc = (long long)g.nx * g.ny * g.nz;
     k_ed0<<<nblk(g.n3, 256), 256, 0, L.s>>>(f[TT_T], f[TT_RH], f[TT_CV], L.TOLD, g.n3);
     TT_LAUNCH();
-    AdvArgs aa{L.TOLD, f[TT_V], f[TT_G], f[TT_H], L.FLMX, L.FLMY, L.FLMZ, nullptr, L.ED, dt};
+    AdvArgs aa{L.TOLD, f[TT_V], f[TT_G], f[TT_H], L.FLMX, L.FLMY, L.FLMZ, nullptr, L.ED, dt,
+               f[TT_CONV_FLUX1], f[TT_CONV_FLUX2], f[TT_CONV_FLUX3]};
     if ((rc = launch_adv_tiled<AdvE>(aa))) return rc;
     aa.A = f[TT_V]; aa.RHQ = f[TT_RHQX]; aa.out = L.VD;
     if ((rc = launch_adv_tiled<AdvV>(aa))) return rc;
@@ -681,8 +686,7 @@ int tt_forgas3d(double delt) {
 int tt_step(double delt, int iii, tt_step_stats* st) {
     if (!L.ready) return fail(1, "tt_init has not been called");
     tt_step_stats s{};
-    hipEvent_t e[4];
-    for (auto& x : e) TT_HIP(hipEventCreate(&x));
+    hipEvent_t* e = L.step_ev;  // created once by tt_init, freed by tt_finalize
     L.iii = iii;
     TT_HIP(hipEventRecord(e[0], L.s));
     int rc = limits_impl(0, delt);
@@ -704,14 +708,13 @@ int tt_step(double delt, int iii, tt_step_stats* st) {
     rc = sync_check();
     if (rc) return rc;
     float a = 0, b = 0, c = 0;
-    hipEventElapsedTime(&a, e[0], e[1]);
-    hipEventElapsedTime(&b, e[1], e[2]);
-    hipEventElapsedTime(&c, e[2], e[3]);
+    TT_HIP(hipEventElapsedTime(&a, e[0], e[1]));
+    TT_HIP(hipEventElapsedTime(&b, e[1], e[2]));
+    TT_HIP(hipEventElapsedTime(&c, e[2], e[3]));
     s.ms_advect = a;
     s.ms_force = b;
     s.ms_energy = c;
     s.ms_total = (double)a + b + c;
-    for (auto& x : e) hipEventDestroy(x);
     if (st) *st = s;
     return erc;
 }

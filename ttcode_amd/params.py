@@ -125,6 +125,7 @@ FLIMX, FLIMY, FLIMZ, DIFRX, DIFRY, DIFRZ = 62, 63, 64, 65, 66, 67
 FLUX_X, FLUX_Y, FLUX_Z, CENTCORX, CENTCORY, CENTCORZ = 68, 69, 70, 71, 72, 73
 UPPERBND, MU_STAR, Q = 80, 81, 82
 ETA, DIVTX, DIVTY, DIVTZ, DISFN = 83, 84, 85, 86, 87
+CONV_FLUX1, CONV_FLUX2, CONV_FLUX3 = 88, 89, 90
 
 FIELD_NAMES = {
     T: "T", RH: "RH", V: "V", G: "G", H: "H", ER: "ER", DELTA_ER: "delta_ER", GD: "GD",
@@ -141,6 +142,7 @@ FIELD_NAMES = {
     FLUX_Y: "flux_y", FLUX_Z: "flux_z", CENTCORX: "centcorx", CENTCORY: "centcory",
     CENTCORZ: "centcorz", UPPERBND: "upperbnd", MU_STAR: "mu_star", Q: "Q",
     ETA: "eta", DIVTX: "divtx", DIVTY: "divty", DIVTZ: "divtz", DISFN: "DISFN",
+    CONV_FLUX1: "conv_flux1", CONV_FLUX2: "conv_flux2", CONV_FLUX3: "conv_flux3",
 }
 FIELD_IDS = {v: k for k, v in FIELD_NAMES.items()}
 
@@ -167,7 +169,8 @@ def field_bounds(f: int, nx: int, ny: int, nz: int):
     elif f in (STELLARTAU, DTAUS):
         lo[0], hi[0] = 1, nx + 1
     elif f in (RHS_ER, SOR_A, SOR_B, SOR_C, SOR_D, SOR_E, SOR_F, SOR_G, EVAL1, EVAL2,
-               EVAL3, EVAL4, EVAL5, EVAL6, EVAL7, EVAL8, EVAL_1_6, JACOB_ETA, FQ, FR):
+               EVAL3, EVAL4, EVAL5, EVAL6, EVAL7, EVAL8, EVAL_1_6, JACOB_ETA, FQ, FR,
+               CONV_FLUX1, CONV_FLUX2, CONV_FLUX3):
         lo, hi = [1, 1, 1], list(n)
     elif f in (FLIMX, DIFRX, FLUX_X):
         lo, hi = [1, 1, 1], [nx + 1, ny, nz]
