@@ -39,6 +39,27 @@ METRIC = "ms/step and SOR sweeps to 1e-10 residual: learned+FP32 SOR vs plain FP
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "sor_sweep_fp64_sph128x128x256_traffic.txt")
 CNN_WEIGHTS = os.path.join(ROOT, "ttcode_amd", "data", "cnn_guess.npz")
+# per-step device times (HIP events on the library's stream), reported as
+# medians over the timed steps: the step's phases, and the energy phase split
+# into the FLD build, the SOR solve and the update, summed over Newton
+PHASES = ("ms_advect", "ms_force", "ms_energy", "ms_total", "energy_ms_build", "energy_ms_sor",
+          "energy_ms_update")
+
+
+def host_info():
+    """The host the CPU baseline runs on: logical CPUs, the ones this
+    process may use, OMP_NUM_THREADS and the CPU model."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0)),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS"), "cpu_model": model}
 
 
 def read_traffic():
@@ -78,6 +99,7 @@ def run_mode(dev, st0, dt, steps, warmup, precision, guess, event_period=7):
     dev.sweep_timing(event_period)
     newton = sor = f32 = first = used = fall = skip = 0
     scales, r0s = [], []
+    phases = {k: [] for k in PHASES}
     t0 = time.perf_counter()
     for _ in range(steps):
         s, _ = dev.step(dt, iii)
@@ -92,6 +114,10 @@ def run_mode(dev, st0, dt, steps, warmup, precision, guess, event_period=7):
         skip += e.learned_skipped
         scales.append(e.guess_scale_first)
         r0s.append(e.guess_r0_first)
+        for k in ("ms_advect", "ms_force", "ms_energy", "ms_total"):
+            phases[k].append(getattr(s, k))
+        for k in ("ms_build", "ms_sor", "ms_update"):
+            phases["energy_" + k].append(getattr(e, k))
     dev.sync()
     t1 = time.perf_counter()
     n64, ms64, n32, ms32 = dev.sweep_timing_get()
@@ -108,6 +134,7 @@ def run_mode(dev, st0, dt, steps, warmup, precision, guess, event_period=7):
         "guess_scale_first": float(np.mean(scales)), "guess_r0_first": float(np.mean(r0s)),
         "sweep64_launches": n64, "sweep64_ms": ms64,
         "sweep32_launches": n32, "sweep32_ms": ms32,
+        "phase_median_ms": {k: round(float(np.median(v)), 4) for k, v in phases.items()},
     }
 
 
@@ -308,7 +335,8 @@ def main():
                     help="also run with omega from ADJUST_SOR_PARAM (the reference's step-1 search)")
     ap.add_argument("--event-period", type=int, default=7,
                     help="bracket every N-th SOR half-sweep launch with HIP events (0: none, no roofline)")
-    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    # every host CPU this process may use, up to the 16 a one-GPU share holds
+    ap.add_argument("--cpu-threads", type=int, default=min(16, len(os.sched_getaffinity(0))))
     ap.add_argument("--config", choices=("c2", "large", "kappa_table", "forgas_visc"), default="c2",
                     help="c2: the 128x128x256 headline workload; large: the 192x256x512 coupled run "
                          "(learned guess + FP32 SOR, 5%% day-night contrast); kappa_table: KAPPA with "
@@ -416,6 +444,7 @@ def main():
                    "nr": nx, "ntheta": nz, "nphi": ny, "dt_s": dt, "epsmat": 1e-10},
         "sor_sweeps_per_step": plain["sor_sweeps_per_step"],
         "newton_per_step": plain["newton_per_step"],
+        "phase_median_ms": plain["phase_median_ms"],
         "cell_updates_per_s": ncell / (plain["ms_per_step"] * 1e-3),
         "hbm_footprint_GB": round(hbm_bytes / 1e9, 3),
         "roofline": {"bound": "hbm", "kernel": "k_sweep<double> (red or black half sweep)",
@@ -429,6 +458,7 @@ def main():
         def summary(m):
             d = {"ms_per_step": round(m["ms_per_step"], 4),
                  "speedup_vs_fp64": round(plain["ms_per_step"] / m["ms_per_step"], 3),
+                 "phase_median_ms": m["phase_median_ms"],
                  "sor_sweeps_per_step": m["sor_sweeps_per_step"],
                  "first_solve_sweeps": m["first_solve_sweeps"]}
             errs = {}
@@ -478,6 +508,7 @@ def main():
     if not a.no_cpu:
         ms, s = cpu_baseline(p, st0, dt, a.cpu_threads)
         out["cpu_baseline"] = {"value": round(ms, 1), "unit": "ms/step", "cores": a.cpu_threads,
+                               "threads": a.cpu_threads, "host": host_info(),
                                "kind": "port",
                                "sample": f"one full step of the C FP64 oracle (OpenMP) at {nx}x{ny}x{nz}, "
                                          f"same state and dt; {s.energy.energy_iter} Newton, "

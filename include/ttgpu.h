@@ -251,6 +251,12 @@ typedef struct tt_energy_stats {
     int32_t first_start;     /* start the first solve used: 0 cold, 1 warm,
                                 2 learned                                     */
     int32_t pad0;
+    /* device time of the call's phases, summed over the Newton iterations,
+     * ms: the FLD build (FLUX_BC, CALCULATEFLUX with DIFF/FLIM, the Newton
+     * terms, FILL_EVALUES, FILL_SOR_MATRIX), SOLVE_WITH_SOR (with the
+     * initial guess and, when switched on, ADJUST_SOR_PARAM), and the
+     * delta_Q / Q, E update with the Q/E boundaries */
+    double ms_build, ms_sor, ms_update;
 } tt_energy_stats;
 
 /* COUPLED_ENERGY_EQS (ppenergy.f:2-791).  iii is the step counter III (read

@@ -135,6 +135,7 @@ struct Lib {
     int have_prev_guess = 0;
     int64_t dev_bytes = 0;          // device memory held by the library
     hipEvent_t step_ev[4] = {};     // tt_step's phase boundaries (created by tt_init)
+    hipEvent_t en_ev[4] = {};       // a Newton iteration's build / SOR / update boundaries
 };
 
 extern Lib L;

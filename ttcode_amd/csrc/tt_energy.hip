@@ -891,8 +891,10 @@ int energy_impl(double delt, int iii, tt_energy_stats* st) {
     TT_HIP(hipMemcpyAsync(f[TT_ER_OLD], f[TT_ER], (size_t)g.n3 * sizeof(double), hipMemcpyDeviceToDevice, L.s));
     TT_HIP(hipMemcpyAsync(L.VD, f[TT_Q], (size_t)g.n3 * sizeof(double), hipMemcpyDeviceToDevice, L.s));  // Q_old
     int srcode = 0;
+    hipEvent_t* ev = L.en_ev;  // phase boundaries of one Newton iteration
     for (;;) {
         L.energy_iter++;
+        TT_HIP(hipEventRecord(ev[0], L.s));
         if ((rc = planetshift_impl(f[TT_ER], 6))) return rc;
         if ((rc = flux_impl())) return rc;
         CellArgs ca{f[TT_FLUX_X], f[TT_FLUX_Y], f[TT_FLUX_Z], f[TT_DIFRX], f[TT_DIFRY], f[TT_DIFRZ], f[TT_Q], L.VD,
@@ -909,6 +911,7 @@ int energy_impl(double delt, int iii, tt_energy_stats* st) {
         k_newton_cell<<<nblk(nc, 256), 256, 0, L.s>>>(g, met(), ca);
         TT_LAUNCH();
         sor_sys_packed_written(with32);
+        TT_HIP(hipEventRecord(ev[1], L.s));
         // ADJUST_SOR_PARAM at the first Newton iteration of step 1 and of
         // every 10000th step (ppenergy.f:372-374), when switched on
         if (L.opts.adjust_omega && L.energy_iter == 1 && (iii % 10000 == 0 || iii == 1)) {
@@ -982,6 +985,7 @@ int energy_impl(double delt, int iii, tt_energy_stats* st) {
         if (L.have_prev_guess && ss.guess_used == 0) es.fallbacks++;
         if (ss.guess_used == 2) es.learned_used++;
         if (L.have_prev_guess == 2) L.learned_rejects = ss.guess_used == 2 ? 0 : L.learned_rejects + 1;
+        TT_HIP(hipEventRecord(ev[2], L.s));
         int nb = 1024;
         UpdArgs ua{f[TT_DELTA_ER], f[TT_EVAL1], f[TT_EVAL2], f[TT_EVAL3], f[TT_EVAL4], f[TT_EVAL5], f[TT_EVAL6],
                    f[TT_EVAL_1_6], L.DFQ, f[TT_FQ], f[TT_RH], f[TT_XKAPP], f[TT_Q], f[TT_ER], L.ub, delt, conv_parm};
@@ -990,7 +994,15 @@ int energy_impl(double delt, int iii, tt_energy_stats* st) {
         rc = fold_partials(L.partial, nb, 2, 1, &L.ctrl->maxdq, &L.ctrl->maxde);
         if (rc) return rc;
         if ((rc = energy_bcs(1))) return rc;
+        TT_HIP(hipEventRecord(ev[3], L.s));
         if ((rc = sync_check())) return rc;
+        float t01 = 0, t12 = 0, t23 = 0;
+        TT_HIP(hipEventElapsedTime(&t01, ev[0], ev[1]));
+        TT_HIP(hipEventElapsedTime(&t12, ev[1], ev[2]));
+        TT_HIP(hipEventElapsedTime(&t23, ev[2], ev[3]));
+        es.ms_build += t01;
+        es.ms_sor += t12;
+        es.ms_update += t23;
         es.max_dq = L.hctrl->maxdq;
         es.max_de = L.hctrl->maxde;
         es.energy_iter = L.energy_iter;

@@ -202,6 +202,10 @@ static void free_all() {
         if (e) hipEventDestroy(e);
         e = nullptr;
     }
+    for (auto& e : L.en_ev) {
+        if (e) hipEventDestroy(e);
+        e = nullptr;
+    }
     if (L.s) hipStreamDestroy(L.s);
     L.s = nullptr;
     L.ready = false;
@@ -266,6 +270,7 @@ int tt_init(const tt_params* p) {
     if (g.n3 >= (1LL << 31)) return fail(1, "grid too large: the ghosted box must hold fewer than 2^31 cells");
     TT_HIP(hipStreamCreateWithFlags(&L.s, hipStreamNonBlocking));
     for (auto& e : L.step_ev) TT_HIP(hipEventCreate(&e));
+    for (auto& e : L.en_ev) TT_HIP(hipEventCreate(&e));
     L.npartial = 8192;  // per-block partials: 2 x npartial doubles, allocated below
     // the SOR working set first: the hot loop's buffers get the first pages
     int rc = sor_alloc();

@@ -27,6 +27,7 @@ module ttgpu_c
      integer(c_int32_t) :: fp32_sweeps, fallbacks, learned_used, learned_skipped
      real(c_double) :: max_dq, max_de, guess_scale_first, guess_r0_first
      integer(c_int32_t) :: first_start, pad0
+     real(c_double) :: ms_build, ms_sor, ms_update
   end type tt_energy_stats
 
   type, bind(C) :: tt_viscosity
