@@ -53,7 +53,13 @@ typedef struct tt_params {
     int32_t isigtyp;         /* 0 = no scattering, 1 = 0.4 cm^2/g             */
     int32_t isoth;           /* 0 = energy equation on                        */
     int32_t itmax;           /* ITMAX, shared by SOR and Newton (ipp:13)      */
-    int32_t pad0;
+    int32_t igrav;           /* FGRAV as pp.f:461 calls it once at set-up
+                                (ppfor.f:646-867): 0 and 1 leave gravx/y/z
+                                zero for the host to upload (IGRAV 1 is the
+                                planet's constant SETPLANETXGRAV, ppplanets.f:
+                                3-33); 4 sets the point mass's radial gravity
+                                gravx(i) = -GRAV MSOL MSTAR2 / xxa(i)^2
+                                (ppfor.f:753-766)                             */
     /* doubles */
     double xmin, xmax;       /* r range, cm                                   */
     double ymin, ymax;       /* phi range in units of 2 pi (ipp YMIN,YMAX)    */
@@ -72,6 +78,8 @@ typedef struct tt_params {
     double fdelt, dtmax;     /* FDELT (CFL factor), DTMAX                     */
     double gamma_opc;        /* gamma_OPC = (kappa_A/kappa_P)^(1/4) of IKAPTYP 13
                                 (and of the 12/14 tables' rewrite, ipp:11)    */
+    double mstar2;           /* MSTAR2, the planet's mass in solar masses
+                                (IGRAV 4; ipp:23)                             */
 } tt_params;
 
 /* ------------------------------------------------------------------------

@@ -53,7 +53,8 @@ program tt_ref_harness
   double precision :: h_xnue, h_pr_num
   integer :: h_prefill_kappa, h_num_iter, h_ivis, h_ivistyp, h_ieta, h_idiss, h_iartvis
   double precision :: h_zeit, h_tinitbottom, h_tinittop, h_rhinittop, h_rhinitbottom
-  double precision :: h_hydrotime, h_maccreated
+  double precision :: h_hydrotime, h_maccreated, h_mstar2
+  integer :: h_igrav
   logical :: have_sin
   character(len=32) :: cmd
   character(len=512) :: dir
@@ -63,7 +64,8 @@ program tt_ref_harness
        h_zkapmult, h_tirr, h_fbottom, h_d_t, h_omrot, h_sorparam, h_epsmat, h_gamma_opc, &
        h_fdelt, h_dtmax, h_delt, h_xnue, h_pr_num, h_prefill_kappa, cmd, &
        h_zeit, h_num_iter, h_tinitbottom, h_tinittop, h_rhinittop, h_rhinitbottom, &
-       h_hydrotime, h_maccreated, h_ivis, h_ivistyp, h_ieta, h_idiss, h_iartvis
+       h_hydrotime, h_maccreated, h_ivis, h_ivistyp, h_ieta, h_idiss, h_iartvis, &
+       h_igrav, h_mstar2
   integer :: u, n, s
   integer :: adj_iter(50)
 
@@ -73,6 +75,7 @@ program tt_ref_harness
   h_rhinittop = 0.d0; h_rhinitbottom = 0.d0; h_hydrotime = 0.d0; h_maccreated = 0.d0
   h_ivis = 0; h_ivistyp = 1; h_ieta = 4; h_idiss = 0; h_iartvis = 0; h_gamma_opc = 0.1d0
   h_prefill_kappa = 0
+  h_igrav = 1; h_mstar2 = 0.d0
   h_xnue = 1.d9
   h_pr_num = 1.d0
   open(newunit=u, file=trim(dir)//'/params.nml', status='old')
@@ -87,7 +90,7 @@ program tt_ref_harness
   ISTEPMAX = 1; IREA = 10000; FDELT = h_fdelt; DTMAX = h_dtmax
   iprstep = 0; FILESTART = 0
   ietot = 0; isoth = 0; isoden = 0
-  IADV = 1; IFOR = 1; IGRAV = 1; IVIS = h_ivis; IRAD = 3; INUC = 0; ISCAL = 1; IDIF = 2
+  IADV = 1; IFOR = 1; IGRAV = h_igrav; IVIS = h_ivis; IRAD = 3; INUC = 0; ISCAL = 1; IDIF = 2
   IRHO = 0; RHMIN = 1.d-11
   ITEMP = 0; TEMPMAX = 1.d4; TEMPMIN = 1.d1
   IVEL = 0; VELMAX = 6.d5; VELDAMP = 0.d0; DAMP_STEPS = 100000
@@ -103,7 +106,7 @@ program tt_ref_harness
   iFbotramp = 0; Framp_steps = 3000; Fbot_final = 2000.d0
   NCOSYS = 2; inc_poles = 0; POLES = .false.
   MODTYP = 2; MODVER = 3; PLANET_NUM = 1
-  STARTYP = 0.d0; RSTAR = 1.155d0; TSTAR = 6070.d0; MSTAR1 = 1.119d0
+  STARTYP = 0.d0; RSTAR = 1.155d0; TSTAR = 6070.d0; MSTAR1 = 1.119d0; MSTAR2 = h_mstar2
   TIRR = h_tirr; RH0 = 0.d0
   NUMDIM = 3
   global_NX = h_nx; XMIN = h_xmin; XMAX = h_xmax; DXMIN = 0.d0
@@ -154,7 +157,12 @@ program tt_ref_harness
      call load3(G, 'G')
      call load3(H, 'H')
      call load3(ER, 'ER')
-     call load3(gravx, 'gravx')
+     if (IGRAV == 4) then
+        ! the point-mass gravity FGRAV computes (pp.f:461)
+        call FGRAV
+     else
+        call load3(gravx, 'gravx')
+     end if
      call load2(mu_star, 'mu_star')
      if (h_prefill_kappa /= 0) then
         call load3(xkapR, 'xkapR')
@@ -314,6 +322,7 @@ program tt_ref_harness
      call dump3(conv_flux(2,1:locNX,1:locNY,1:locNZ), 'conv_flux2')
      call dump3(conv_flux(3,1:locNX,1:locNY,1:locNZ), 'conv_flux3')
      call dump3(CS, 'CS')
+     call dump3(gravx, 'gravx')
      call dump2i(upperbnd, 'upperbnd')
   end if
   open(newunit=u, file=trim(dir)//'/out_scalars.txt', status='replace')

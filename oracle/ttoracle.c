@@ -480,6 +480,18 @@ int tor_init(const tt_params* p)
     /* GASINIT, ppinit.f:1414-1415 */
     double cv = RGAS / (p->mu_gas * (p->gamma - 1.0));
     for (size_t n = 0; n < O.n3; ++n) O.CV[n] = cv;
+    /* FGRAV, IGRAV 4 with NCOSYS 2 (ppfor.f:753-766): the point mass
+       MPLANET = MSOL MSTAR2 (GRAV, MSOL REAL literals, ppmod.f:1075,1079) */
+    if (p->igrav == 4) {
+        const double grav = (double)6.672e-8f, msol = (double)1.989e33f;
+        const double mplanet = msol * p->mstar2;
+        for (int k = -1; k <= NZ + 2; ++k)
+            for (int j = -1; j <= NY + 2; ++j)
+                for (int i = -1; i <= NX + 2; ++i) {
+                    const double x = i == -1 ? XXA(0) - DXA(1) : XXA(i); /* xxa(-1) is not defined */
+                    A(O.GRAVX, i, j, k) = -grav * mplanet / (x * x);
+                }
+    }
     return 0;
 }
 

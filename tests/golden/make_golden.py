@@ -51,7 +51,9 @@ def _nml(p: P.TTParams, cmd, delt, iii, nsteps, prefill, xnue=1e9, pr_num=1.0, e
                h_tirr=v["tirr"], h_fbottom=v["fbottom"], h_d_t=v["d_t"], h_omrot=v["omrot"],
                h_sorparam=v["sorparam"], h_epsmat=v["epsmat"], h_fdelt=v["fdelt"],
                h_dtmax=v["dtmax"], h_delt=delt, h_xnue=xnue, h_pr_num=pr_num,
-               h_prefill_kappa=1 if prefill else 0)
+               h_prefill_kappa=1 if prefill else 0,
+               # IGRAV 0 and 1 run alike here (gravx is an input); 4 is FGRAV's point mass
+               h_igrav=4 if v.get("igrav") == 4 else 1, h_mstar2=v.get("mstar2", 0.0))
     ent.update(extra or {})
     lines = ["&ttin"]
     for k, val in ent.items():
@@ -675,6 +677,31 @@ def checkpoint_step():
          ["T", "ER", "V", "DISFN", "conv_flux1", "conv_flux2", "conv_flux3"], extra)
 
 
+def gravity4():
+    """FGRAV with IGRAV 4 (ppfor.f:753-766, the point mass MSOL MSTAR2 with
+    the shipped ipp's MSTAR2 = 0.000654, ipp:23), computed at set-up as
+    pp.f:461 does: FORGAS3D, and two full steps.  gravx is an output here,
+    not an input."""
+    nx, ny, nz = 12, 16, 12
+    keep_state = ["T", "RH", "V", "G", "H", "ER", "upperbnd"]
+    p = P.make_params(nx, ny, nz, ikaptyp=1, kap0=1e-2, igrav=4, mstar2=0.000654)
+    inp = base_inputs(p, seed=47)
+    del inp["gravx"]
+    o = Oracle(p)
+    o.load_state(inp)
+    o.bounds()
+    o.pressg()
+    dt = o.schritt()
+    outs, sc = run_ref("forgas", p, inp, delt=dt)
+    save("forgas_igrav4_12x16x12", p, inp, outs, sc, keep_state + ["PG", "gravx", "centcorx"], {"delt": dt})
+    ps = P.make_params(nx, ny, nz, ikaptyp=1, kap0=1e-1, epsmat=1e-8, tirr=2000.0, igrav=4, mstar2=0.000654)
+    inp_st = base_inputs(ps, seed=53)
+    del inp_st["gravx"]
+    outs, sc = run_ref("step", ps, inp_st, delt=dt, iii=2, nsteps=2)
+    save("step2_igrav4_12x16x12", ps, inp_st, outs, sc, keep_state + ["delta_ER", "PG", "gravx"],
+         {"delt": dt, "iii": 2, "nsteps": 2})
+
+
 # The shipped ipp's frame rotation: IROTATE = 1, PROT = 3.52 days (ipp:11),
 # OMROT = 2 PI / (PROT DAY) (ppinit.f:501-502), DAY the REAL literal 8.64E4
 # (ppmod.f:1083).
@@ -710,6 +737,8 @@ def rotating():
 if __name__ == "__main__":
     if sys.argv[1:] == ["rotating"]:
         rotating()
+    elif sys.argv[1:] == ["gravity4"]:
+        gravity4()
     elif sys.argv[1:] == ["checkpoint_step"]:
         checkpoint_step()
     elif sys.argv[1:] == ["checkpoints"]:

@@ -117,7 +117,7 @@ def test_advect3d(ttgpu, name):
     d.close()
 
 
-@pytest.mark.parametrize("name", ["forgas_12x16x12", "forgas_omrot_12x16x12"])
+@pytest.mark.parametrize("name", ["forgas_12x16x12", "forgas_omrot_12x16x12", "forgas_igrav4_12x16x12"])
 def test_forgas3d(ttgpu, name):
     g = C.load(name)
     p, d = _dev(ttgpu, g)
@@ -588,7 +588,8 @@ def test_step_viscosity_32x64x32_vs_oracle(ttgpu):
         d.close()
 
 
-@pytest.mark.parametrize("name", ["step2_12x16x12", "step2_ibdrad9_12x16x12", "step2_omrot_12x16x12"])
+@pytest.mark.parametrize("name", ["step2_12x16x12", "step2_ibdrad9_12x16x12", "step2_omrot_12x16x12",
+                                  "step2_igrav4_12x16x12"])
 def test_two_steps(ttgpu, name):
     g = C.load(name)
     p, d = _dev(ttgpu, g)

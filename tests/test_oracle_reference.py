@@ -57,7 +57,7 @@ def test_advect3d(name):
     _check_fields(g, p, o)
 
 
-@pytest.mark.parametrize("name", ["forgas_12x16x12", "forgas_omrot_12x16x12"])
+@pytest.mark.parametrize("name", ["forgas_12x16x12", "forgas_omrot_12x16x12", "forgas_igrav4_12x16x12"])
 def test_forgas3d(name):
     def act(o, g):
         o.bounds()
@@ -65,6 +65,24 @@ def test_forgas3d(name):
         o.forgas3d(float(g["delt"]))
     g, p, o, _ = _run(name, act)
     _check_fields(g, p, o)
+
+
+def test_gravity4_closed_form():
+    """IGRAV 4 (ppfor.f:753-766): gravx = -GRAV MSOL MSTAR2 / xxa(i)^2 on
+    every (j, k), with xxa(0) - dxa(1) for i = -1; GRAV = 6.672E-8 and MSOL =
+    1.989E33 as REAL literals (ppmod.f:1075,1079).  The fixture's gravx is the
+    reference's FGRAV; it equals the closed form and the oracle's."""
+    g = C.load("forgas_igrav4_12x16x12")
+    p = C.params_of(g)
+    assert p.igrav == 4 and "in_gravx" not in g.files
+    xxa, dxa = g["out_m_xxa"], g["out_m_dxa"]   # xxa(0:nx+2), dxa(1:nx+2)
+    r = np.concatenate([[xxa[0] - dxa[0]], xxa])
+    gm = float(np.float32(6.672e-8)) * (float(np.float32(1.989e33)) * p.mstar2)
+    want = -(gm / (r * r))
+    gx = g["out_gravx"]
+    np.testing.assert_array_equal(gx, np.broadcast_to(want, gx.shape))
+    o = Oracle(p, threads=1)
+    np.testing.assert_array_equal(o.download("gravx"), gx)
 
 
 def test_rotating_fixtures_rotate():
@@ -150,7 +168,8 @@ def test_coupled_energy_adjusts_omega_at_step_one():
     assert not np.array_equal(o2.download("T"), g["out_T"])
 
 
-@pytest.mark.parametrize("name", ["step2_12x16x12", "step2_ibdrad9_12x16x12", "step2_omrot_12x16x12"])
+@pytest.mark.parametrize("name", ["step2_12x16x12", "step2_ibdrad9_12x16x12", "step2_omrot_12x16x12",
+                                  "step2_igrav4_12x16x12"])
 def test_two_steps(name):
     def act(o, g):
         for s in range(int(g["nsteps"])):

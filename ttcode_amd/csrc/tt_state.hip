@@ -141,6 +141,15 @@ __global__ void k_fill(double* a, double v, long long n) {
     if (t < n) a[t] = v;
 }
 
+// a radial profile col[i + 1], i = -1..nx+2, copied to every (j, k) of the box
+__global__ void k_fill_radial(Geo g, const double* __restrict__ col, double* a) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.n3) return;
+    int i, j, k;
+    box_ijk(g, t, i, j, k);
+    a[t] = col[i + 1];
+}
+
 static double* g_stage = nullptr;  // device staging buffer (one box)
 
 static int xfer(int f, void* host, bool to_host) {
@@ -258,6 +267,9 @@ int tt_init(const tt_params* p) {
     if (p->f_incident != 0 && p->f_incident != 2) return fail(1, "F_INCIDENT must be 0 or 2");
     if (p->isigtyp != 0 && p->isigtyp != 1) return fail(1, "ISIGTYP must be 0 or 1");
     if (p->isoth != 0) return fail(1, "ISOTH must be 0");
+    if (p->igrav != 0 && p->igrav != 1 && p->igrav != 4) return fail(1, "IGRAV must be 0, 1 or 4");
+    if (p->igrav == 4 && !(p->mstar2 > 0.0)) return fail(1, "IGRAV 4 needs MSTAR2 > 0");
+    if (p->nx + 4 > 16384) return fail(1, "nx must be below 16380");
     int ndev = 0;
     TT_HIP(hipGetDeviceCount(&ndev));
     if (ndev < 1) return fail(10, "no HIP device");
@@ -323,6 +335,25 @@ int tt_init(const tt_params* p) {
     double cv = RGAS / (p->mu_gas * (p->gamma - 1.0));
     k_fill<<<nblk(g.n3, 256), 256, 0, L.s>>>(L.f[TT_CV], cv, g.n3);
     TT_LAUNCH();
+    // FGRAV (pp.f:461).  IGRAV 4, NCOSYS 2 (ppfor.f:753-766): the point mass
+    // MPLANET = MSOL MSTAR2; gravx(-1) uses xxa(0) - dxa(1) (xxa(-1) is not
+    // defined), then gravx(i) = -GRAV MPLANET / xxa(i)^2, the same in every
+    // (j, k); gravy = gravz = 0 (zero since allocation).  GRAV and MSOL are
+    // REAL literals in DOUBLE PRECISION parameters (ppmod.f:1075,1079).
+    if (p->igrav == 4) {
+        const double grav = (double)6.672e-8f, msol = (double)1.989e33f;
+        const double mplanet = msol * p->mstar2;
+        const double* xxa = L.hmet + (size_t)TT_M_XXA * L.metlen + 2;  // xxa[i], i >= 0
+        const double* dxa = L.hmet + (size_t)TT_M_DXA * L.metlen + 2;
+        std::vector<double> col((size_t)g.gx);
+        const double r0 = xxa[0] - dxa[1];
+        col[0] = -(grav * mplanet / (r0 * r0));
+        for (int i = 0; i <= g.nx + 2; ++i) col[i + 1] = -(grav * mplanet / (xxa[i] * xxa[i]));
+        TT_HIP(hipMemcpyAsync(L.partial, col.data(), col.size() * sizeof(double), hipMemcpyHostToDevice, L.s));
+        k_fill_radial<<<nblk(g.n3, 256), 256, 0, L.s>>>(g, L.partial, L.f[TT_GRAVX]);
+        TT_LAUNCH();
+        TT_HIP(hipStreamSynchronize(L.s));  // col is released after this block
+    }
     L.opts.precision = 0;
     L.opts.guess = 0;
     L.opts.batch = 16;

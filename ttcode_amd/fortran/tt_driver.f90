@@ -26,7 +26,7 @@ program tt_driver
   nx = h_nx; ny = h_ny; nz = h_nz
   p%nx = nx; p%ny = ny; p%nz = nz
   p%bcxmin = 12; p%ibdrad = 7; p%iflimtyp = 2; p%f_incident = 2; p%ikaptyp = 26
-  p%isigtyp = 0; p%isoth = 0; p%itmax = h_itmax; p%pad0 = 0
+  p%isigtyp = 0; p%isoth = 0; p%itmax = h_itmax; p%igrav = 1; p%mstar2 = 0.d0
   p%xmin = 7149199872.d0; p%xmax = 1.1d0 * 7149199872.d0
   p%ymin = 0.d0; p%ymax = 1.d0; p%zmin = -0.5d0; p%zmax = 0.5d0
   p%gamma = 1.4d0; p%mu_gas = 2.3d0; p%gamma_opc = 0.1d0

@@ -15,11 +15,11 @@ module ttgpu_c
 
   type, bind(C) :: tt_params
      integer(c_int32_t) :: nx, ny, nz, bcxmin, ibdrad, iflimtyp, f_incident
-     integer(c_int32_t) :: ikaptyp, isigtyp, isoth, itmax, pad0
+     integer(c_int32_t) :: ikaptyp, isigtyp, isoth, itmax, igrav
      real(c_double) :: xmin, xmax, ymin, ymax, zmin, zmax, gamma, mu_gas
      real(c_double) :: zkapmult, kap0, dkap0, kap_rho0, kap_t0, kap_a, kap_b
      real(c_double) :: tirr, fbottom, d_t, omrot, sorparam, epsmat, fdelt, dtmax
-     real(c_double) :: gamma_opc
+     real(c_double) :: gamma_opc, mstar2
   end type tt_params
 
   type, bind(C) :: tt_energy_stats

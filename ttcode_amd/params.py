@@ -38,7 +38,7 @@ class TTParams(ctypes.Structure):
         ("bcxmin", ctypes.c_int32), ("ibdrad", ctypes.c_int32),
         ("iflimtyp", ctypes.c_int32), ("f_incident", ctypes.c_int32),
         ("ikaptyp", ctypes.c_int32), ("isigtyp", ctypes.c_int32),
-        ("isoth", ctypes.c_int32), ("itmax", ctypes.c_int32), ("pad0", ctypes.c_int32),
+        ("isoth", ctypes.c_int32), ("itmax", ctypes.c_int32), ("igrav", ctypes.c_int32),
         ("xmin", ctypes.c_double), ("xmax", ctypes.c_double),
         ("ymin", ctypes.c_double), ("ymax", ctypes.c_double),
         ("zmin", ctypes.c_double), ("zmax", ctypes.c_double),
@@ -51,6 +51,7 @@ class TTParams(ctypes.Structure):
         ("omrot", ctypes.c_double), ("sorparam", ctypes.c_double),
         ("epsmat", ctypes.c_double), ("fdelt", ctypes.c_double),
         ("dtmax", ctypes.c_double), ("gamma_opc", ctypes.c_double),
+        ("mstar2", ctypes.c_double),
     ]
 
     def as_dict(self):
@@ -256,6 +257,8 @@ def make_params(nx, ny, nz, *, epsmat=1e-8, itmax=20000, sorparam=1.7,
     p.gamma, p.mu_gas = GAMMA, MU_GAS
     p.zkapmult = kap0
     p.gamma_opc = 0.1  # the shipped ipp's gamma_OPC (ipp:11)
+    p.igrav = 1        # constant radial gravity, uploaded as gravx (generate_state)
+    p.mstar2 = 0.0
     p.kap0 = kap0
     p.kap_rho0 = rho0
     p.kap_t0 = T0
