@@ -42,7 +42,10 @@ extern "C" {
 typedef struct tt_params {
     /* integers (keep the order: a Fortran bind(C) type mirrors it) */
     int32_t nx, ny, nz;      /* locNX, locNY, locNZ  (r, phi, theta)          */
-    int32_t bcxmin;          /* BOUNDXMIN type at r_min; 12 supported        */
+    int32_t bcxmin;          /* BOUNDXMIN type at r_min: 12 (fixed dT/dr =
+                                D_T) or 9 (the flux FBottom carried by
+                                radiative diffusion; ppbou.f:1029-1066,
+                                ppenergy.f:150-167)                           */
     int32_t ibdrad;          /* BOUNDXMAX(2) top-T version: 3, 4, 7 or 9      */
     int32_t iflimtyp;        /* flux limiter: 1 = 1/3, 2 = Levermore-Pomraning */
     int32_t f_incident;      /* 0 = no irradiation, 2 = implicit cos heating  */
@@ -70,7 +73,7 @@ typedef struct tt_params {
                                 dkappa/dT = dkap0 (rho/kap_rho0)^a (T/kap_t0)^(b-1) */
     double kap_rho0, kap_t0, kap_a, kap_b;
     double tirr;             /* TIRR, K                                       */
-    double fbottom;          /* FBottom, bottom radiative flux                */
+    double fbottom;          /* FBottom, bottom radiative flux (FBOT, ipp:12) */
     double d_t;              /* D_T, fixed bottom dT/dr for BCXMIN=12         */
     double omrot;            /* OMROT, frame rotation rate (0 if IROTATE=0)   */
     double sorparam;         /* SORPARAM (omega)                              */
@@ -360,6 +363,14 @@ typedef struct tt_step_stats {
  * FORGAS3D, COUPLED_ENERGY_EQS, BOUNDS, BOUNDS, PRESSG (time step held fixed),
  * with the clamps tt_set_limits switched on at their places in that loop. */
 int tt_step(double delt, int iii, tt_step_stats* st);
+
+/* III of the host's step loop as BOUNDS reads it (BOUNDXMAX(2) IBDRAD 9,
+ * and BOUNDXMIN(9), which at III <= 0 -- the set-up calls of LOCALINIT and
+ * pp.f:419 -- evaluates BurrowsOPC at cell (1,1,1) and needs
+ * tt_set_burrows_table).  tt_init sets 0; tt_step and tt_coupled_energy set
+ * it from their argument.  No counterpart function: the reference's III is a
+ * module variable (ppmod.f, fluid_var_init). */
+int tt_set_step(int iii);
 
 /* Block until all queued device work is done. */
 int tt_sync(void);

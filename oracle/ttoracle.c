@@ -142,6 +142,8 @@ void tor_set_threads(int n)
 #endif
 }
 
+static int g_iii; /* III of the step loop, as BOUNDS reads it (set by tor_step, tor_coupled_energy, tor_set_iii) */
+
 static double* field_ptr(int f)
 {
     switch (f) {
@@ -447,6 +449,7 @@ int tor_init(const tt_params* p)
     g_adjust = 0;
     memset(&g_lim, 0, sizeof g_lim);
     memset(&g_visc, 0, sizeof g_visc);
+    g_iii = 0;
     free(g_qvx); free(g_qvy); free(g_qvz);
     g_qvx = g_qvy = g_qvz = NULL;
     int nt = O.nthreads ? O.nthreads : 1;
@@ -581,14 +584,40 @@ static void passupperbnd(void)
 }
 
 /* --------------------------------------------------------------- ppbou.f */
-static int boundxmin(int type) /* ppbou.f:734-1148; CASE(12) :1118-1139 */
+static int burrows_opc(double t_in, double rh_in, double* bkp, double* bkr, double* bka, double* dkp,
+                       double* dka);
+static int g_btab_set;
+
+/* ppbou.f:734-1148: CASE(12) :1118-1139 (T ghosts from the fixed gradient
+   D_T) and CASE(9) :1029-1066 (the gradient that carries FBottom through the
+   bottom by radiative diffusion); then V = 0 and stress-free G, H for both */
+static int boundxmin(int type, int iii)
 {
-    if (type != 12) return seterr(1, "BOUNDXMIN: only type 12 is supported");
-    const double D_T = O.p.d_t;
+    if (type != 12 && type != 9) return seterr(1, "BOUNDXMIN: only types 9 and 12 are supported");
+    if (type == 9 && g_visc.ieta == 5) return seterr(1, "BOUNDXMIN(9) with IETA 5 is not supported");
+    const double D_T = O.p.d_t, fb = O.p.fbottom;
+    if (type == 9 && iii <= 0) {
+        /* III = 0 (set-up): BurrowsOPC at cell (1,1,1) (writing its opacities
+           there), whose gradient every column then takes */
+        if (!g_btab_set) return seterr(1, "BOUNDXMIN(9) at III <= 0 evaluates BurrowsOPC: set a Burrows table");
+        if (burrows_opc(A(O.T, 1, 1, 1), A(O.RH, 1, 1, 1), &A(O.XKP, 1, 1, 1), &A(O.XKR, 1, 1, 1),
+                        &A(O.XKA, 1, 1, 1), &A(O.DKP, 1, 1, 1), &A(O.DKA, 1, 1, 1)))
+            return seterr(2, "NEG. T. IN BURROWS OPC (or a density outside the table)");
+    }
     for (int k = -1; k <= NZ + 2; ++k)
         for (int j = -1; j <= NY + 2; ++j) {
-            A(O.T, 0, j, k) = A(O.T, 1, j, k) + D_T * (XXB(0) - XXB(1));
-            A(O.T, -1, j, k) = A(O.T, 1, j, k) + D_T * (XXB(-1) - XXB(1));
+            if (type == 12) {
+                A(O.T, 0, j, k) = A(O.T, 1, j, k) + D_T * (XXB(0) - XXB(1));
+                A(O.T, -1, j, k) = A(O.T, 1, j, k) + D_T * (XXB(-1) - XXB(1));
+            } else {
+                /* T**3.d0: the reference's compiler multiplies, T*T*T (its
+                   fixtures match that, not pow) */
+                const int jj = iii > 0 ? j : 1, kk = iii > 0 ? k : 1;
+                const double dtdr = -3.0 * fb * A(O.XKR, 1, jj, kk) * A(O.RH, 1, jj, kk) /
+                                    (4.0 * ARAD * FD * (A(O.T, 1, jj, kk) * A(O.T, 1, jj, kk) * A(O.T, 1, jj, kk)));
+                A(O.T, 0, j, k) = A(O.T, 1, j, k) - dtdr * (XXB(1) - XXB(0));
+                A(O.T, -1, j, k) = A(O.T, 1, j, k) - dtdr * (XXB(1) - XXB(-1));
+            }
             A(O.V, 1, j, k) = 0.0;
             A(O.V, 0, j, k) = 0.0;
         }
@@ -715,7 +744,6 @@ static void boundzmax0(void) /* ppbou.f:1936-1988 */
         }
 }
 
-static int g_iii = 0; /* III as seen by BOUNDXMAX(2) IBDRAD=9 */
 
 static int boundplanet(void) /* ppbou.f:94-249, MODVER 0/2/3/4/5 branch */
 {
@@ -728,7 +756,7 @@ static int boundplanet(void) /* ppbou.f:94-249, MODVER 0/2/3/4/5 branch */
             UBND(j, k) = ub;
         }
     passupperbnd();
-    int rc = boundxmin(O.p.bcxmin);
+    int rc = boundxmin(O.p.bcxmin, g_iii);
     if (rc) return rc;
     rc = boundxmax2(g_iii);
     if (rc) return rc;
@@ -1513,8 +1541,23 @@ double tor_get_sorparam(void) { return O.p.sorparam; }
 
 /* COUPLED_ENERGY_EQS, ppenergy.f:2-791 (IRAD=3, theta=1, conv_parm=0.1,
  * delta_max=0.005, NCOSYS=2, BCXMIN=12, no poles). */
-static void energy_bottom_bc(void) /* ppenergy.f:184-195 / :640-651 (BCXMIN=12) */
+static void energy_bottom_bc(void) /* ppenergy.f:184-195 / :640-651 (BCXMIN=12); :150-167 / :621-638 (9) */
 {
+    if (O.p.bcxmin == 9) {
+        /* fixed flux, free T and Q: the radiative-diffusion gradient of Q that
+           carries FBottom, and the optically thick ER = a T^4 at i = 1, 0, -1 */
+        for (int k = -1; k <= NZ + 2; ++k)
+            for (int j = -1; j <= NY + 2; ++j) {
+                const double rh = A(O.RH, 1, j, k), cv = A(O.CV, 1, j, k); /* T**3.d0 as T*T*T */
+                const double dqdr = -rh * cv * 3.0 * O.p.fbottom * A(O.XKR, 1, j, k) * rh /
+                                    (4.0 * ARAD * FD * (A(O.T, 1, j, k) * A(O.T, 1, j, k) * A(O.T, 1, j, k)));
+                A(O.Q, 0, j, k) = A(O.Q, 1, j, k) - dqdr * (XXB(1) - XXB(0));
+                A(O.Q, -1, j, k) = A(O.Q, 1, j, k) - dqdr * (XXB(1) - XXB(-1));
+                for (int i = 1; i >= -1; --i)
+                    A(O.ER, i, j, k) = ARAD * pow(A(O.Q, i, j, k) / (A(O.RH, i, j, k) * A(O.CV, i, j, k)), 4.0);
+            }
+        return;
+    }
     for (int k = -1; k <= NZ + 2; ++k)
         for (int j = -1; j <= NY + 2; ++j) {
             double dqdr = A(O.CV, 1, j, k) * A(O.RH, 1, j, k) * O.p.d_t;
@@ -1566,7 +1609,8 @@ int tor_coupled_energy(double delt, int iii, tt_energy_stats* st)
      * is the nearest float, 0.100000001490116 */
     const double delta_max = 0.005, conv_parm = (double)0.1f, theta = 1.0;
     int rc;
-    if (O.p.bcxmin != 12) return seterr(1, "COUPLED_ENERGY_EQS: only BCXMIN=12 is supported");
+    if (O.p.bcxmin != 12 && O.p.bcxmin != 9) return seterr(1, "COUPLED_ENERGY_EQS: BCXMIN must be 9 or 12");
+    if (O.p.bcxmin == 9 && g_visc.ieta == 5) return seterr(1, "COUPLED_ENERGY_EQS: BCXMIN 9 with IETA 5 is not supported");
     g_iii = iii;
     double* dQ = alloc3();
     double* divF = alloc3();
@@ -1647,7 +1691,7 @@ int tor_coupled_energy(double delt, int iii, tt_energy_stats* st)
         int bad = 0;
         for (int k = 1; k <= NZ; ++k)
             for (int j = 1; j <= NY; ++j) {
-                int xmax = UBND(j, k) - 1, xmin = 1;
+                int xmax = UBND(j, k) - 1, xmin = O.p.bcxmin == 12 ? 1 : 2; /* ppenergy.f:390-396 */
                 for (int i = xmin; i <= xmax; ++i) {
                     size_t n = I3(i, j, k);
                     double Q1 = -delt * (1.0 - theta);
@@ -1668,7 +1712,7 @@ int tor_coupled_energy(double delt, int iii, tt_energy_stats* st)
         if (bad) { rc = seterr(2, "ERROR IN DELTA_Q: non-positive value of Q"); goto done; }
         for (int k = 1; k <= NZ; ++k)
             for (int j = 1; j <= NY; ++j) {
-                int xmax = UBND(j, k) - 1, xmin = 1;
+                int xmax = UBND(j, k) - 1, xmin = O.p.bcxmin == 12 ? 1 : 2; /* ppenergy.f:390-396 */
                 for (int i = xmin; i <= xmax; ++i) {
                     size_t n = I3(i, j, k);
                     O.ER[n] = O.ER[n] + conv_parm * O.DER[n];

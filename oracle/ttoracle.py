@@ -199,6 +199,11 @@ class Oracle:
     def sphericalvellimit(self, ivel, velmax, hydrotime=0.0):
         return self._check(self.L.tor_sphericalvellimit(ivel, velmax, hydrotime), "SPHERICALVELLIMIT")
 
+    def set_iii(self, iii):
+        """III of the step loop as BOUNDS reads it (step and coupled_energy
+        set it from their argument)."""
+        self.L.tor_set_iii(int(iii))
+
     def schritt(self):
         d = ctypes.c_double()
         self._check(self.L.tor_schritt(ctypes.byref(d)), "SCHRITT")

@@ -72,7 +72,8 @@ def test_field_table_matches_header(f):
     assert bool(isint) == (dt == __import__("numpy").int32)
 
 
-@pytest.mark.parametrize("field,value,msg", [("bcxmin", 9, "BCXMIN"), ("ibdrad", 5, "IBDRAD"),
+@pytest.mark.parametrize("field,value,msg", [("bcxmin", 8, "BCXMIN"), ("ibdrad", 5, "IBDRAD"),
+                                             ("igrav", 2, "IGRAV"),
                                              ("ikaptyp", 15, "IKAPTYP"), ("iflimtyp", 3, "IFLIMTYP"),
                                              ("f_incident", 1, "F_INCIDENT"), ("isigtyp", 2, "ISIGTYP"),
                                              ("nx", 31, "even")])

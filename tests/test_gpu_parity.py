@@ -107,6 +107,39 @@ def test_bounds(ttgpu, name, rel):
     d.close()
 
 
+def test_bounds_bcxmin9(ttgpu):
+    """BOUNDXMIN(9) at III = 1 (ppbou.f:1029-1066) on varying opacities."""
+    g = C.load("bounds_bcxmin9_12x16x12")
+    p, d = _dev(ttgpu, g)
+    d.set_iii(int(g["iii"]))
+    d.bounds()
+    _cmp(g, p, d)
+    d.close()
+
+
+def test_bounds_bcxmin9_iii0_burrows_cell(ttgpu):
+    """BOUNDXMIN(9) at III = 0: BurrowsOPC at cell (1,1,1) on the device."""
+    g = C.load("bounds_bcxmin9_iii0_12x16x12")
+    p, d = _dev(ttgpu, g)
+    d.set_burrows_table(g["tab_planck"], g["tab_ross"], g["tab_abs"])
+    d.set_iii(int(g["iii"]))
+    d.bounds()
+    _cmp(g, p, d)
+    d.close()
+
+
+def test_coupled_energy_bcxmin9(ttgpu):
+    """COUPLED_ENERGY_EQS with the BCXMIN 9 Q/E bottom blocks and delta_Q
+    from i = 2 (ppenergy.f:150-167,390-396,621-638)."""
+    g = C.load("energy_bcxmin9_12x16x12")
+    p, d = _dev(ttgpu, g)
+    st, _ = d.coupled_energy(float(g["delt"]), int(g["iii"]))
+    assert st.energy_iter == int(g["scal_energy_iter"])
+    assert st.sor_iter_last == int(g["scal_SOR_ITER"])
+    _cmp(g, p, d, rel=1e-12, scaled_tol=1e-10)
+    d.close()
+
+
 # *_omrot_*: the shipped ipp's rotating frame, OMROT = 2 PI / (3.52 days)
 @pytest.mark.parametrize("name", ["advect_12x16x12", "advect_omrot_12x16x12"])
 def test_advect3d(ttgpu, name):
@@ -589,7 +622,7 @@ def test_step_viscosity_32x64x32_vs_oracle(ttgpu):
 
 
 @pytest.mark.parametrize("name", ["step2_12x16x12", "step2_ibdrad9_12x16x12", "step2_omrot_12x16x12",
-                                  "step2_igrav4_12x16x12"])
+                                  "step2_igrav4_12x16x12", "step2_bcxmin9_12x16x12"])
 def test_two_steps(ttgpu, name):
     g = C.load(name)
     p, d = _dev(ttgpu, g)

@@ -48,6 +48,38 @@ def test_bounds(name):
     _check_fields(g, p, o)
 
 
+def test_bounds_bcxmin9():
+    """BOUNDXMIN(9) at III = 1 (ppbou.f:1029-1066): the bottom T ghosts
+    from the gradient -3 FBottom kappa_R rho / (4 a c T^3) of each column's
+    cell 1, on opacities that vary cell by cell."""
+    def act(o, g):
+        o.set_iii(int(g["iii"]))
+        o.bounds()
+    g, p, o, _ = _run("bounds_bcxmin9_12x16x12", act)
+    _check_fields(g, p, o)
+    # the gradient moved the ghosts: a dropped FBottom term fails this fixture
+    assert p.fbottom > 0 and not np.array_equal(g["out_T"][:, :, 0], g["out_T"][:, :, 2])
+
+
+def test_bounds_bcxmin9_iii0_burrows_cell():
+    """BOUNDXMIN(9) at III = 0 (set-up): BurrowsOPC at cell (1,1,1), whose
+    opacities it writes there and whose gradient every column takes."""
+    def act(o, g):
+        _burrows(o, g)
+        o.set_iii(int(g["iii"]))
+        o.bounds()
+    g, p, o, _ = _run("bounds_bcxmin9_iii0_12x16x12", act)
+    _check_fields(g, p, o)
+
+
+@pytest.mark.parametrize("name", ["energy_bcxmin9_12x16x12"])
+def test_coupled_energy_bcxmin9(name):
+    g, p, o, st = _run(name, lambda o, g: o.coupled_energy(float(g["delt"]), int(g["iii"]))[0])
+    assert st.energy_iter == int(g["scal_energy_iter"])
+    assert st.sor_iter_last == int(g["scal_SOR_ITER"])
+    _check_fields(g, p, o)
+
+
 # *_omrot_*: the shipped ipp's rotating frame, OMROT = 2 PI / (3.52 days)
 # (ipp:11, ppinit.f:501-502), which enters GD (ppadv.f:1082,1518-1528) and
 # CENTRIFCORIOLIS (ppfor.f:426-486)
@@ -169,7 +201,7 @@ def test_coupled_energy_adjusts_omega_at_step_one():
 
 
 @pytest.mark.parametrize("name", ["step2_12x16x12", "step2_ibdrad9_12x16x12", "step2_omrot_12x16x12",
-                                  "step2_igrav4_12x16x12"])
+                                  "step2_igrav4_12x16x12", "step2_bcxmin9_12x16x12"])
 def test_two_steps(name):
     def act(o, g):
         for s in range(int(g["nsteps"])):

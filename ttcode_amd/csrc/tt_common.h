@@ -73,6 +73,7 @@ struct Ctrl {
     double dtmin;
     double scratch[8];
     double av_min[3];  // NR_ARTIFICIAL_VISC: min dv/dx of the x, y and z passes
+    double bot_dtdr;   // BOUNDXMIN(9) at III <= 0: the gradient of cell (1,1,1)
     // stagnation watch of inner (FP32) solves
     double best;       // best L1 ratio seen
     int since;         // iterations since it last improved by 1%
@@ -251,6 +252,7 @@ void sor_sys_module_written();
 int sor_sys_before_download(int field);
 int diffusion_impl(double delt, double xnue, double pr, int* iters);
 int energy_impl(double delt, int iii, tt_energy_stats* st);
+int burrows_cell111_impl();        // BurrowsOPC at cell (1,1,1) (BOUNDXMIN(9), III <= 0)
 int advect_impl(double delt);
 int forgas_impl(double delt);
 int visc_impl();

@@ -241,32 +241,23 @@ __device__ inline int first_above(const double* ax, double x, double g) {
 // 10^ of the upper T node's column over 10^ of the two T nodes.  tab: the
 // Planck, Rosseland and absorption tables and the log10 T axis, then 10^ of
 // all of them (formed on the host).
-__global__ void k_kappa_bur(Geo g, const int32_t* __restrict__ ub, const double* __restrict__ T,
-                            const double* __restrict__ RH, const double* __restrict__ tab, double* XKR,
-                            double* XKP, double* XKA, double* DKP, double* DKA, double* SIG, double sig,
-                            Ctrl* ctrl) {
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= g.n3) return;
-    SIG[t] = sig;
-    int i, j, k;
-    box_ijk(g, t, i, j, k);
-    const int u = ub[g.idx2(j, k)];
-    const long long n = g.idx(i <= u ? i : u, j, k);
-    const double t_in = T[n];
-    if (!(t_in > 0.0)) {  // the reference stops (pprphy.f:1039-1045)
-        ctrl->bad_kappa = 1;
-        return;
-    }
-    double rh1 = log10(RH[n]);
+// BurrowsOPC (pprphy.f:1005-1225, its grey part) of one cell: log10 T and
+// log10 rho clamped into the table (the clamp bounds are REAL constants too),
+// the first node above each, the reference's bilinear weights, 10^ of the
+// interpolants, and dk/dT as the one-sided difference of 10^ of the upper T
+// node's column over 10^ of the two T nodes.  tab: the Planck, Rosseland and
+// absorption tables and the log10 T axis, then 10^ of all of them (formed on
+// the host).  out: kP, kR, kA, dkP/dT, dkA/dT.  Returns false where the
+// reference stops: T <= 0 (pprphy.f:1039-1045) or rho < 0 (:1091-1097).
+__device__ inline bool burrows_cell(double t_in, double rh_in, const double* __restrict__ tab, double out[5]) {
+    if (!(t_in > 0.0)) return false;
+    double rh1 = log10(rh_in);
     double tm1 = log10(t_in);
     if (tm1 < (double)1.69896f) tm1 = (double)1.69897f;
     if (tm1 > (double)3.69889f) tm1 = (double)3.69888f;
     if (rh1 < (double)-11.9996f) rh1 = (double)-11.9995f;
     if (rh1 > (double)-1.99993f) rh1 = (double)-1.99994f;
-    if (rh1 != rh1) {  // rho < 0: no node above, the reference stops (pprphy.f:1091-1097)
-        ctrl->bad_kappa = 1;
-        return;
-    }
+    if (rh1 != rh1) return false;  // rho < 0: no node above
     // the first node above each value (the clamps put it in 1..49): the axes
     // are nearly uniform, so a guess from the mean spacing, then the step to
     // the node the reference's scan finds (the axes increase strictly)
@@ -283,7 +274,6 @@ __global__ void k_kappa_bur(Geo g, const int32_t* __restrict__ ub, const double*
     ws2 = ws2 / ws3;
     ws3 = 1.0 - ws2;
     const int TN = TT_BURROWS_NT * TT_BURROWS_NR;
-    double out[3];
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
         const double* tq = tab + q * TN;
@@ -293,14 +283,63 @@ __global__ void k_kappa_bur(Geo g, const int32_t* __restrict__ ub, const double*
         const double w5 = ws3 * (ws1 * a10 + ws * a11);
         out[q] = exp10(w4 + w5);  // 10.d0**x, within a few ulp
     }
-    XKP[t] = out[0];
-    XKR[t] = out[1];
-    XKA[t] = out[2];
     const double* p10 = tab + 3 * TN + TT_BURROWS_NT;  // 10^ of the tables and of the T axis
     const double* t10 = p10 + 3 * TN;
     const double dt = t10[it] - t10[it1];
-    DKP[t] = (p10[it + TT_BURROWS_NT * iw] - p10[it1 + TT_BURROWS_NT * iw]) / dt;
-    DKA[t] = (p10[2 * TN + it + TT_BURROWS_NT * iw] - p10[2 * TN + it1 + TT_BURROWS_NT * iw]) / dt;
+    out[3] = (p10[it + TT_BURROWS_NT * iw] - p10[it1 + TT_BURROWS_NT * iw]) / dt;
+    out[4] = (p10[2 * TN + it + TT_BURROWS_NT * iw] - p10[2 * TN + it1 + TT_BURROWS_NT * iw]) / dt;
+    return true;
+}
+
+// KAPPA, IKAPTYP 9-12 and 14 (pprphy.f:199-234,242-274, MODTYP=2): BurrowsOPC
+// for cells up to upperbnd, whose values the cells above take; SUBSIG.  One
+// thread per cell evaluates at (min(i, upperbnd), j, k).
+__global__ void k_kappa_bur(Geo g, const int32_t* __restrict__ ub, const double* __restrict__ T,
+                            const double* __restrict__ RH, const double* __restrict__ tab, double* XKR,
+                            double* XKP, double* XKA, double* DKP, double* DKA, double* SIG, double sig,
+                            Ctrl* ctrl) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.n3) return;
+    SIG[t] = sig;
+    int i, j, k;
+    box_ijk(g, t, i, j, k);
+    const int u = ub[g.idx2(j, k)];
+    const long long n = g.idx(i <= u ? i : u, j, k);
+    double o[5];
+    if (!burrows_cell(T[n], RH[n], tab, o)) {
+        ctrl->bad_kappa = 1;
+        return;
+    }
+    XKP[t] = o[0];
+    XKR[t] = o[1];
+    XKA[t] = o[2];
+    DKP[t] = o[3];
+    DKA[t] = o[4];
+}
+
+// BOUNDXMIN(9) at III <= 0 (ppbou.f:1036-1046): BurrowsOPC at cell (1,1,1),
+// written there (the reference passes those array elements)
+// written there (the reference passes those array elements), and the
+// bottom gradient -3 FBottom kappa_R rho / (4 a c T^3) every column then
+// takes, into ctrl->bot_dtdr (T**3.d0 as T*T*T, as the reference's compiler
+// evaluates it)
+__global__ void k_bur_cell111(Geo g, const double* __restrict__ T, const double* __restrict__ RH,
+                              const double* __restrict__ tab, double* XKR, double* XKP, double* XKA, double* DKP,
+                              double* DKA, double fbottom, Ctrl* ctrl) {
+    if (threadIdx.x != 0) return;
+    const long long n = g.idx(1, 1, 1);
+    double o[5];
+    if (!burrows_cell(T[n], RH[n], tab, o)) {
+        ctrl->bad_kappa = 1;
+        return;
+    }
+    XKP[n] = o[0];
+    XKR[n] = o[1];
+    XKA[n] = o[2];
+    DKP[n] = o[3];
+    DKA[n] = o[4];
+    const double t1 = T[n];
+    ctrl->bot_dtdr = -3.0 * fbottom * o[1] * RH[n] / (4.0 * ARAD * FD * (t1 * t1 * t1));
 }
 
 
@@ -596,6 +635,7 @@ struct UpdArgs {
     double *Q, *ER;
     const int32_t* ub;
     double delt, conv;
+    int imin;  // first cell of the update: 1 (BCXMIN 12), 2 (else; ppenergy.f:390-396)
 };
 
 // delta_Q from the delta_ER stencil (ppenergy.f:379-408), the positivity
@@ -610,7 +650,7 @@ __global__ __launch_bounds__(256) void k_dq_update(Geo g, UpdArgs a, double* par
         int i, j, k;
         sub_ijk(t, g.nx, g.ny, 1, 1, 1, i, j, k);
         int u = a.ub[g.idx2(j, k)];
-        if (i > u - 1) continue;
+        if (i > u - 1 || i < a.imin) continue;
         long long c = IX(i, j, k);
         const double theta = 1.0;
         double Q1 = -a.delt * (1.0 - theta);
@@ -653,10 +693,12 @@ __device__ inline double flux_x_alias(const Geo& g, const double* FX, int i, int
 }
 
 // Q/ER r boundaries per (j,k) column: upper (ppenergy.f:80-104, :681-723) and
-// BCXMIN=12 bottom (:184-195, :640-651).  order 0: top first, 1: bottom first.
+// the bottom, BCXMIN=12 (:184-195, :640-651) or 9 (:150-167, :621-638).
+// order 0: top first, 1: bottom first.
 __global__ void k_energy_bc_x(Geo g, Met m, const int32_t* __restrict__ ub, double* Q, double* ER,
                               const double* __restrict__ RH, const double* __restrict__ CV,
-                              const double* __restrict__ T, const double* __restrict__ FX, double d_t, int order) {
+                              const double* __restrict__ T, const double* __restrict__ FX,
+                              const double* __restrict__ XKR, int bcxmin, double d_t, double fbottom, int order) {
     int nj = g.ny + 4, nk = g.nz + 4;
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nj * nk) return;
@@ -670,6 +712,20 @@ __global__ void k_energy_bc_x(Geo g, Met m, const int32_t* __restrict__ ub, doub
                 Q[IX(i, j, k)] = qup;
                 ER[IX(i, j, k)] = fabs(2.0 * flux_x_alias(g, FX, i, j, k) / FD);
             }
+        } else if (bcxmin == 9) {
+            // fixed flux, free T and Q: dQ/dr of the radiative diffusion that
+            // carries FBottom (T**3.d0 as T*T*T), optically thick ER = a T^4
+            const long long c1 = IX(1, j, k), c0 = IX(0, j, k), cm = IX(-1, j, k);
+            const double rh = RH[c1], cv = CV[c1], t1 = T[c1];
+            const double dq = -rh * cv * 3.0 * fbottom * XKR[c1] * rh / (4.0 * ARAD * FD * (t1 * t1 * t1));
+            const double q1 = Q[c1];
+            const double q0 = q1 - dq * (m(TT_M_XXB, 1) - m(TT_M_XXB, 0));
+            const double qm = q1 - dq * (m(TT_M_XXB, 1) - m(TT_M_XXB, -1));
+            Q[c0] = q0;
+            Q[cm] = qm;
+            ER[c1] = ARAD * pow(q1 / (rh * cv), 4.0);
+            ER[c0] = ARAD * pow(q0 / (RH[c0] * CV[c0]), 4.0);
+            ER[cm] = ARAD * pow(qm / (RH[cm] * CV[cm]), 4.0);
         } else {
             double dq = CV[IX(1, j, k)] * RH[IX(1, j, k)] * d_t;
             double q1 = Q[IX(1, j, k)];
@@ -745,6 +801,31 @@ static bool table_kappa() {
     return (t >= 4 && t <= 7) || (t >= 9 && t <= 12) || t == 14;
 }
 
+// the Burrows tables on the device, refreshed when the host's have changed
+static int btab_sync() {
+    if (L.btab_version != g_btab_version) {
+        const std::vector<double> t = with_pow10(g_btab, 3 * TT_BURROWS_NT * TT_BURROWS_NR + TT_BURROWS_NT);
+        TT_HIP(hipMemcpyAsync(L.btab, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice, L.s));
+        TT_HIP(hipStreamSynchronize(L.s));  // t is released after this call
+        L.btab_version = g_btab_version;
+    }
+    return 0;
+}
+
+int burrows_cell111_impl() {
+    if (g_btab_version == 0)
+        return fail(1, "BOUNDXMIN(9) at III <= 0 evaluates BurrowsOPC: call tt_set_burrows_table first");
+    int rc = btab_sync();
+    if (rc) return rc;
+    double** f = L.f;
+    k_clear_bad_kappa<<<1, 64, 0, L.s>>>(L.ctrl);
+    TT_LAUNCH();
+    k_bur_cell111<<<1, 64, 0, L.s>>>(L.g, f[TT_T], f[TT_RH], L.btab, f[TT_XKAPR], f[TT_XKAPP], f[TT_XKAPA],
+                                     f[TT_DXKAPP_DT], f[TT_DXKAPA_DT], L.p.fbottom, L.ctrl);
+    TT_LAUNCH();
+    return 0;
+}
+
 static int kappa_impl() {
     if (table_kappa()) {
         k_clear_bad_kappa<<<1, 64, 0, L.s>>>(L.ctrl);
@@ -783,12 +864,8 @@ static int kappa_impl() {
     }
     if ((L.p.ikaptyp >= 9 && L.p.ikaptyp <= 12) || L.p.ikaptyp == 14) {
         if (g_btab_version == 0) return fail(1, "KAPPA: IKAPTYP 9-12 and 14 need tt_set_burrows_table first");
-        if (L.btab_version != g_btab_version) {
-            const std::vector<double> t = with_pow10(g_btab, 3 * TT_BURROWS_NT * TT_BURROWS_NR + TT_BURROWS_NT);
-            TT_HIP(hipMemcpyAsync(L.btab, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice, L.s));
-            TT_HIP(hipStreamSynchronize(L.s));  // t is released after this call
-            L.btab_version = g_btab_version;
-        }
+        int rc = btab_sync();
+        if (rc) return rc;
         k_kappa_bur<<<nblk(L.g.n3, 256), 256, 0, L.s>>>(L.g, L.ub, f[TT_T], f[TT_RH], L.btab, f[TT_XKAPR],
                                                        f[TT_XKAPP], f[TT_XKAPA], f[TT_DXKAPP_DT],
                                                        f[TT_DXKAPA_DT], f[TT_SIG], sig, L.ctrl);
@@ -855,7 +932,8 @@ static int energy_bcs(int order) {
     double** f = L.f;
     int ncol = (g.ny + 4) * (g.nz + 4), nline = (g.nx + 4) * (g.ny + 4);
     k_energy_bc_x<<<nblk(ncol, 128), 128, 0, L.s>>>(g, met(), L.ub, f[TT_Q], f[TT_ER], f[TT_RH], f[TT_CV], f[TT_T],
-                                                    f[TT_FLUX_X], L.p.d_t, order);
+                                                    f[TT_FLUX_X], f[TT_XKAPR], L.p.bcxmin, L.p.d_t, L.p.fbottom,
+                                                    order);
     TT_LAUNCH();
     k_energy_bc_z<<<nblk(nline, 256), 256, 0, L.s>>>(g, f[TT_Q], f[TT_ER]);
     TT_LAUNCH();
@@ -867,7 +945,9 @@ int energy_impl(double delt, int iii, tt_energy_stats* st) {
     double** f = L.f;
     const double delta_max = 0.005, conv_parm = (double)0.1f;  // ppenergy.f:42,49 (0.1 is a REAL literal)
     tt_energy_stats es{};
-    if (L.p.bcxmin != 12) return fail(1, "COUPLED_ENERGY_EQS: only BCXMIN=12 is supported");
+    if (L.p.bcxmin != 12 && L.p.bcxmin != 9) return fail(1, "COUPLED_ENERGY_EQS: BCXMIN must be 9 or 12");
+    if (L.p.bcxmin == 9 && L.visc.ieta == 5)
+        return fail(1, "COUPLED_ENERGY_EQS: BCXMIN 9 with IETA 5 is not supported");
     L.iii = iii;
     int rc;
     L.energy_iter = 0;
@@ -988,7 +1068,8 @@ int energy_impl(double delt, int iii, tt_energy_stats* st) {
         TT_HIP(hipEventRecord(ev[2], L.s));
         int nb = 1024;
         UpdArgs ua{f[TT_DELTA_ER], f[TT_EVAL1], f[TT_EVAL2], f[TT_EVAL3], f[TT_EVAL4], f[TT_EVAL5], f[TT_EVAL6],
-                   f[TT_EVAL_1_6], L.DFQ, f[TT_FQ], f[TT_RH], f[TT_XKAPP], f[TT_Q], f[TT_ER], L.ub, delt, conv_parm};
+                   f[TT_EVAL_1_6], L.DFQ, f[TT_FQ], f[TT_RH], f[TT_XKAPP], f[TT_Q], f[TT_ER], L.ub, delt, conv_parm,
+                   L.p.bcxmin == 12 ? 1 : 2};
         k_dq_update<<<nb, 256, 0, L.s>>>(g, ua, L.partial, nb, L.ctrl);
         TT_LAUNCH();
         rc = fold_partials(L.partial, nb, 2, 1, &L.ctrl->maxdq, &L.ctrl->maxde);

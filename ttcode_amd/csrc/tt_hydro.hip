@@ -683,6 +683,12 @@ int tt_forgas3d(double delt) {
     return rc ? rc : sync_check();
 }
 
+int tt_set_step(int iii) {
+    if (!L.ready) return fail(1, "tt_init has not been called");
+    L.iii = iii;
+    return 0;
+}
+
 int tt_step(double delt, int iii, tt_step_stats* st) {
     if (!L.ready) return fail(1, "tt_init has not been called");
     tt_step_stats s{};

@@ -85,17 +85,32 @@ __global__ __launch_bounds__(256) void k_shift5_upperbnd(Geo g, double* T, doubl
     else upperbnd_body(g, RH, ub, blockIdx.x - nshift);
 }
 
-// BOUNDXMIN(12) (ppbou.f:1118-1139) then BOUNDXMAX(2) (ppbou.f:1269-1395)
+// BOUNDXMIN(12) (ppbou.f:1118-1139) or (9) (:1029-1066), then BOUNDXMAX(2)
+// (ppbou.f:1269-1395)
 __global__ void k_xbc(Geo g, Met m, const int32_t* __restrict__ ub, double* T, double* RH, double* V, double* G,
                       double* H, const double* __restrict__ ER, const double* __restrict__ SIN,
-                      const double* __restrict__ XKP, double d_t, int ibdrad, int iii, double tirr, Ctrl* ctrl) {
+                      const double* __restrict__ XKP, const double* __restrict__ XKR, int bcxmin, double d_t,
+                      double fbottom, int ibdrad, int iii, double tirr, Ctrl* ctrl) {
     int nj = g.ny + 4, nk = g.nz + 4;
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nj * nk) return;
     int j = t % nj - 1, k = t / nj - 1;
-    // BOUNDXMIN(12)
-    T[g.idx(0, j, k)] = T[g.idx(1, j, k)] + d_t * (m(TT_M_XXB, 0) - m(TT_M_XXB, 1));
-    T[g.idx(-1, j, k)] = T[g.idx(1, j, k)] + d_t * (m(TT_M_XXB, -1) - m(TT_M_XXB, 1));
+    if (bcxmin == 12) {  // fixed dT/dr = D_T
+        T[g.idx(0, j, k)] = T[g.idx(1, j, k)] + d_t * (m(TT_M_XXB, 0) - m(TT_M_XXB, 1));
+        T[g.idx(-1, j, k)] = T[g.idx(1, j, k)] + d_t * (m(TT_M_XXB, -1) - m(TT_M_XXB, 1));
+    } else {  // 9: the radiative-diffusion gradient that carries FBottom
+        double dtdr;
+        if (iii > 0) {
+            const long long c = g.idx(1, j, k);
+            const double t1 = T[c];
+            dtdr = -3.0 * fbottom * XKR[c] * RH[c] / (4.0 * ARAD * FD * (t1 * t1 * t1));  // T**3.d0 as T*T*T
+        } else {
+            dtdr = ctrl->bot_dtdr;  // cell (1,1,1)'s, from BurrowsOPC (k_bur_cell111)
+        }
+        const double t1 = T[g.idx(1, j, k)];
+        T[g.idx(0, j, k)] = t1 - dtdr * (m(TT_M_XXB, 1) - m(TT_M_XXB, 0));
+        T[g.idx(-1, j, k)] = t1 - dtdr * (m(TT_M_XXB, 1) - m(TT_M_XXB, -1));
+    }
     V[g.idx(1, j, k)] = 0.0;
     V[g.idx(0, j, k)] = 0.0;
     if (j >= 0) {
@@ -199,7 +214,8 @@ __global__ void k_zbc(Geo g, Met m, double* T, double* RH, double* V, double* G,
 int bounds_impl() {
     Geo& g = L.g;
     double** f = L.f;
-    if (L.p.bcxmin != 12) return fail(1, "BOUNDXMIN: only type 12 is supported");
+    if (L.p.bcxmin != 12 && L.p.bcxmin != 9) return fail(1, "BOUNDXMIN: only types 9 and 12 are supported");
+    if (L.p.bcxmin == 9 && L.visc.ieta == 5) return fail(1, "BOUNDXMIN(9) with IETA 5 is not supported");
     if (L.p.ibdrad != 3 && L.p.ibdrad != 4 && L.p.ibdrad != 7 && L.p.ibdrad != 9)
         return fail(1, "BOUNDXMAX(2): IBDRAD must be 3, 4, 7 or 9");
     long long ns = (long long)(g.nx + 4) * g.nz;
@@ -208,9 +224,13 @@ int bounds_impl() {
     k_shift5_upperbnd<<<nshift + nblk(ncol, 4), 256, 0, L.s>>>(g, f[TT_T], f[TT_RH], f[TT_V], f[TT_G], f[TT_H],
                                                                L.ub, (int)nshift);
     TT_LAUNCH();
+    if (L.p.bcxmin == 9 && L.iii <= 0) {
+        int rc = burrows_cell111_impl();
+        if (rc) return rc;
+    }
     k_xbc<<<nblk(ncol, 128), 128, 0, L.s>>>(g, met(), L.ub, f[TT_T], f[TT_RH], f[TT_V], f[TT_G], f[TT_H],
-                                            f[TT_ER], f[TT_STELLARINPUT], f[TT_XKAPP], L.p.d_t, L.p.ibdrad,
-                                            L.iii, L.p.tirr, L.ctrl);
+                                            f[TT_ER], f[TT_STELLARINPUT], f[TT_XKAPP], f[TT_XKAPR], L.p.bcxmin,
+                                            L.p.d_t, L.p.fbottom, L.p.ibdrad, L.iii, L.p.tirr, L.ctrl);
     TT_LAUNCH();
     k_zbc<<<nblk(nline, 256), 256, 0, L.s>>>(g, met(), f[TT_T], f[TT_RH], f[TT_V], f[TT_G], f[TT_H]);
     TT_LAUNCH();

@@ -258,7 +258,7 @@ int tt_init(const tt_params* p) {
     if (p->nx % 2 || p->ny % 2) return fail(1, "nx and ny must be even (red-black colouring across the phi wrap)");
     // the switch values the kernels implement (everything else is rejected,
     // not silently treated as one of these)
-    if (p->bcxmin != 12) return fail(1, "BCXMIN must be 12");
+    if (p->bcxmin != 12 && p->bcxmin != 9) return fail(1, "BCXMIN must be 9 or 12");
     if (p->ibdrad != 3 && p->ibdrad != 4 && p->ibdrad != 7 && p->ibdrad != 9)
         return fail(1, "IBDRAD must be 3, 4, 7 or 9");
     if (p->ikaptyp != 1 && !(p->ikaptyp >= 4 && p->ikaptyp <= 14) && p->ikaptyp != 26)

@@ -107,6 +107,10 @@ module ttgpu_c
        import :: c_int, c_double
        real(c_double), value :: delt
      end function tt_forgas3d
+     integer(c_int) function tt_set_step(iii) bind(C, name='tt_set_step')
+       import :: c_int
+       integer(c_int), value :: iii
+     end function tt_set_step
      integer(c_int) function tt_coupled_energy(delt, iii, st) bind(C, name='tt_coupled_energy')
        import :: c_int, c_double, tt_energy_stats
        real(c_double), value :: delt
@@ -151,6 +155,7 @@ contains
   end subroutine tt_check
 
   subroutine BOUNDS
+    call tt_check(tt_set_step(int(III, c_int)), 'BOUNDS')
     call tt_check(tt_bounds(), 'BOUNDS')
   end subroutine BOUNDS
 

@@ -259,6 +259,10 @@ class Device:
         rc = self._check(self.L.tt_solve_sor(1 if fill else 0, ctypes.byref(st)), "SOLVE_WITH_SOR")
         return st, rc
 
+    def set_iii(self, iii):
+        """III of the step loop as BOUNDS reads it (tt_set_step)."""
+        self._check(self.L.tt_set_step(int(iii)), "tt_set_step")
+
     def schritt(self):
         d = ctypes.c_double()
         self._check(self.L.tt_schritt(ctypes.byref(d)), "SCHRITT")
