@@ -51,8 +51,8 @@ typedef struct tt_params {
     int32_t f_incident;      /* 0 = no irradiation, 2 = implicit cos heating  */
     int32_t ikaptyp;         /* 1 = constant ZKAPMULT, 4/5 and 9-12/14 tables,
                                 6/7 column-depth tables, 8 = H- bound-free,
-                                13 = constant with
-                                gamma_opc, 26 = power law                     */
+                                13 = constant with gamma_opc, 20/21/23-25
+                                wavelength-binned tables, 26 = power law      */
     int32_t isigtyp;         /* 0 = no scattering, 1 = 0.4 cm^2/g             */
     int32_t isoth;           /* 0 = energy equation on                        */
     int32_t itmax;           /* ITMAX, shared by SOR and Newton (ipp:13)      */
@@ -445,14 +445,40 @@ int tt_set_column_table(const double* kap_abs);
  * for 14 the Rosseland table is log kP, for 11 the absorption table carries
  * the planet's extra absorber below 0.03 bar (ttcode_amd/opacity.py
  * read_burrows_table applies these).  IKAPTYP 9 reads the table with TiO/VO,
- * the others the one without.  The wavelength-binned families 20-25 are not
- * supported.
+ * the others the one without.  The wavelength-binned families 20-25 take
+ * their means from these tables too (below).
  * ------------------------------------------------------------------------ */
 #define TT_BURROWS_NT 50
 #define TT_BURROWS_NR 50
 /* Replaces READBURROWSOPCTABLE's fill of BurrowsXKAP_plnk/_ross/_abs; kept
  * like the IKAPTYP 4/5 tables, and may be called before tt_init. */
 int tt_set_burrows_table(const double* kap_planck, const double* kap_ross, const double* kap_abs);
+
+/* ------------------------------------------------------------------------
+ * IKAPTYP 20, 21, 23, 24 and 25: the Burrows tables with 30 wavelength bins
+ * (KAPPA, pprphy.f:321-349; BurrowsOPC, pprphy.f:1005-1225).  The Planck,
+ * Rosseland and absorption means come from tt_set_burrows_table as for 9-14
+ * (READBURROWSOPCTABLE's columns 3-5), and the cells from upperbnd up take
+ * upperbnd-1's values.  For 20, 21, 23 and 24 STELLARINPUT_HEATING sums the
+ * stellar flux over the bins (pprad.f:429-447): per bin the optical depth of
+ * xkapW rho DR from upperbnd down (WAVELENGTH_CALCSTELLARTAU, pprad.f:
+ * 519-547), its central difference (WAVELENGTH_STELLAR_DTAU, :549-569), and
+ * STELLARINPUT = sum of BSTAR_BIN exp(-tau/mu*) dtau, times the dilution,
+ * over dxb; 25 uses the grey irradiation.  xkapW, the bins' opacity, is
+ * formed where that sum needs it and not kept.  The two-stream bins of
+ * 22-25 (xkapLW, local_BB) feed only the wavelength radiative transfer of
+ * IRAD 5-7 (ppwavelth.f) and are not formed.  IKAPTYP 22 is refused: the
+ * reference's reader opens no file for it and stops (pprphy.f:1265-1275).
+ * ------------------------------------------------------------------------ */
+#define TT_NWAVE 30
+/* Replaces READBURROWSOPCTABLE's fill of BurrowsXKAP_waveopac and BSTAR_BIN
+ * (pprphy.f:1297-1360,1438-1469,1509-1538): kap_wave = log10 of the 30 bins'
+ * irradiation opacity, kap[it + 50 (ir + 50 l)] as BurrowsXKAP_waveopac(50,
+ * 50,30) (with the planet's extra absorber for 20, 24 and 25), bstar[30] =
+ * BSTAR_BIN, dilution = (RSTAR RSOL / semi)^2 with semi from pp.f:204-206
+ * (ttcode_amd/opacity.py reads both files and computes it).  Kept like the
+ * other tables; may be called before tt_init. */
+int tt_set_burrows_wave_table(const double* kap_wave, const double* bstar, double dilution);
 
 /* KAPPA + SUBSIG (pprphy.f:1-360,567-583) on the device state, as FLIM calls
  * them (pprphy.f:362-380).  IKAPTYP 4 and 5 call PRESSG first and stop with

@@ -41,7 +41,7 @@ program tt_ref_harness
   use global_var_alloc
   use artvis_var_init
   use artvis_var_alloc
-  use global_constants, only: FD_c => FD, ARAD_c => ARAD
+  use global_constants, only: FD_c => FD, ARAD_c => ARAD, GRAV, MSOL, DAY, PI
   implicit none
   include "mpif.h"
 
@@ -68,6 +68,8 @@ program tt_ref_harness
        h_igrav, h_mstar2
   integer :: u, n, s
   integer :: adj_iter(50)
+  integer(8) :: clk0, clk1, clk_rate
+  double precision :: step_ms(1000)
 
   call get_command_argument(1, dir)
   h_nsteps = 1
@@ -146,6 +148,9 @@ program tt_ref_harness
   D_RH = 0.d0
   FBottom = h_fbottom
   hydrotime = h_hydrotime
+  ! the semi-major axis as pp.f:204-206 computes it (the wavelength-binned
+  ! STELLARINPUT_HEATING divides by it)
+  semi = (GRAV*MSOL*(mstar1+mstar2)*((Porb*DAY)**2.d0)/(4.0*PI*PI))**(1.d0/3.d0)
   call GITTER
   cv = 0.d0
   call GASINIT
@@ -179,7 +184,8 @@ program tt_ref_harness
 
   if (IKAPTYP >= 4 .and. IKAPTYP <= 7) call READFREEDMANOPCTABLE
   if (IKAPTYP == 6 .or. IKAPTYP == 7) call READCOLUMNDEPTHOPCTABLE
-  if ((IKAPTYP >= 9 .and. IKAPTYP <= 12) .or. IKAPTYP == 14) call READBURROWSOPCTABLE
+  if ((IKAPTYP >= 9 .and. IKAPTYP <= 12) .or. IKAPTYP == 14 .or. &
+      (IKAPTYP >= 20 .and. IKAPTYP <= 25)) call READBURROWSOPCTABLE
 
   select case (trim(cmd))
   case ('gitter')
@@ -226,14 +232,19 @@ program tt_ref_harness
      SORPARAM = h_sorparam
      call ADJUST_SOR_PARAM
   case ('step')
+     ! each step's wall time (system_clock) goes to out_scalars.txt
+     call system_clock(count_rate=clk_rate)
      do s = 1, h_nsteps
         III = h_iii + s - 1
+        call system_clock(clk0)
         call ADVECT3D
         call FORGAS3D
         call COUPLED_ENERGY_EQS
         call BOUNDS
         call BOUNDS
         call PRESSG
+        call system_clock(clk1)
+        if (s <= 1000) step_ms(s) = 1.d3 * dble(clk1 - clk0) / dble(clk_rate)
      end do
   case ('diffusion')
      call BOUNDS
@@ -333,6 +344,11 @@ program tt_ref_harness
   write(u, '(A,1X,ES25.17)') 'FD', FD_c
   write(u, '(A,1X,ES25.17)') 'ARAD', ARAD_c
   write(u, '(A,1X,ES25.17)') 'SORPARAM', SORPARAM
+  if (trim(cmd) == 'step') then
+     do s = 1, min(h_nsteps, 1000)
+        write(u, '(A,I0,1X,ES25.17)') 'step_ms_', s, step_ms(s)
+     end do
+  end if
   if (IARTVIS > 0) write(u, '(A,1X,ES25.17)') 'dt_artvis', dt_artvis
   if (trim(cmd) == 'restart') then
      write(u, '(A,1X,ES25.17)') 'ZEIT', ZEIT

@@ -878,6 +878,72 @@ int tor_set_burrows_table(const double* kp, const double* kr, const double* ka)
  * tables, 10^x; dk/dT the one-sided difference over 10^ of the T nodes.
  * Returns 1 where the reference stops (T <= 0, or a NaN log10 rho, which
  * finds no node above it). */
+/* BurrowsOPC's table nodes and bilinear weights (pprphy.f:1040-1105) */
+typedef struct { int it, it1, iw, iw1; double ws1, ws, ws2, ws3; } bur_nodes;
+
+static int burrows_nodes(double t_in, double rh_in, bur_nodes* b)
+{
+    if (t_in <= 0.0) return 1;
+    double rh1 = log10(rh_in);
+    double tm1 = log10(t_in);
+    if (tm1 < (double)1.69896f) tm1 = (double)1.69897f;
+    if (tm1 > (double)3.69889f) tm1 = (double)3.69888f;
+    if (rh1 < (double)-11.9996f) rh1 = (double)-11.9995f;
+    if (rh1 > (double)-1.99993f) rh1 = (double)-1.99994f;
+    int it = 0;
+    while (it < 50 && !(BURROWS_T[it] > tm1)) ++it;
+    if (it == 50 || it == 0) return 1;
+    b->it = it;
+    b->it1 = it - 1;
+    b->ws1 = BURROWS_T[b->it1] - tm1;
+    b->ws = BURROWS_T[b->it1] - BURROWS_T[it];
+    b->ws1 = b->ws1 / b->ws;
+    b->ws = 1.0 - b->ws1;
+    int iw = 0;
+    while (iw < 50 && !(BURROWS_RH[iw] > rh1)) ++iw;
+    if (iw == 50 || iw == 0) return 1;
+    b->iw = iw;
+    b->iw1 = iw - 1;
+    b->ws2 = rh1 - BURROWS_RH[b->iw1];
+    b->ws3 = BURROWS_RH[iw] - BURROWS_RH[b->iw1];
+    b->ws2 = b->ws2 / b->ws3;
+    b->ws3 = 1.0 - b->ws2;
+    return 0;
+}
+
+/* IKAPTYP 20-25: log10 of the 30 wavelength bins' absorption opacity of the
+   stellar irradiation, BurrowsXKAP_waveopac(50,50,30) (T fastest, then rho,
+   then bin), the bins' stellar flux BSTAR_BIN and the dilution
+   (RSTAR RSOL / semi)^2 */
+static double g_wtab[30 * 2500], g_bstar[30], g_wdil;
+static int g_wtab_set;
+
+int tor_set_burrows_wave_table(const double* kw, const double* bstar, double dil)
+{
+    memcpy(g_wtab, kw, sizeof g_wtab);
+    memcpy(g_bstar, bstar, sizeof g_bstar);
+    g_wdil = dil;
+    g_wtab_set = 1;
+    return 0;
+}
+
+/* BKAP_W of BurrowsOPC (pprphy.f:1138-1151): the bins' 10^ of the bilinear
+   interpolant */
+static int burrows_wave(double t_in, double rh_in, double kw[30])
+{
+    bur_nodes b;
+    if (burrows_nodes(t_in, rh_in, &b)) return 1;
+    for (int l = 0; l < 30; ++l) {
+        const double* t = g_wtab + 2500 * l;
+        const double w00 = t[b.it + 50 * b.iw], w10 = t[b.it + 50 * b.iw1], w11 = t[b.it1 + 50 * b.iw1],
+                     w01 = t[b.it1 + 50 * b.iw];
+        const double w4 = b.ws2 * (b.ws1 * w00 + b.ws * w01);
+        const double w5 = b.ws3 * (b.ws1 * w10 + b.ws * w11);
+        kw[l] = pow(10.0, w4 + w5);
+    }
+    return 0;
+}
+
 static int burrows_opc(double t_in, double rh_in, double* bkp, double* bkr, double* bka, double* dkp,
                        double* dka)
 {
@@ -1041,6 +1107,29 @@ static int kappa(void) /* KAPPA, pprphy.f:1-360; case 26 is new */
         if (bad) return seterr(2, "NEG. T. IN BURROWS OPC (or a density outside the table)");
         return 0;
     }
+    if (p->ikaptyp >= 20 && p->ikaptyp <= 25) { /* pprphy.f:321-349: the cells from upperbnd up take
+                                                   upperbnd-1's values (the wavelength bins are formed where
+                                                   STELLARINPUT_HEATING uses them) */
+        if (!g_btab_set) return seterr(1, "KAPPA: IKAPTYP 20-25 need an opacity table");
+        int bad = 0;
+#pragma omp parallel for reduction(| : bad)
+        for (int k = -1; k <= NZ + 2; ++k)
+            for (int j = -1; j <= NY + 2; ++j) {
+                int ub = UBND(j, k);
+                for (int i = -1; i <= ub; ++i)
+                    bad |= burrows_opc(A(O.T, i, j, k), A(O.RH, i, j, k), &A(O.XKP, i, j, k), &A(O.XKR, i, j, k),
+                                       &A(O.XKA, i, j, k), &A(O.DKP, i, j, k), &A(O.DKA, i, j, k));
+                for (int i = ub; i <= NX + 2; ++i) {
+                    A(O.XKR, i, j, k) = A(O.XKR, ub - 1, j, k);
+                    A(O.XKP, i, j, k) = A(O.XKP, ub - 1, j, k);
+                    A(O.XKA, i, j, k) = A(O.XKA, ub - 1, j, k);
+                    A(O.DKP, i, j, k) = A(O.DKP, ub - 1, j, k);
+                    A(O.DKA, i, j, k) = A(O.DKA, ub - 1, j, k);
+                }
+            }
+        if (bad) return seterr(2, "NEG. T. IN BURROWS OPC (or a density outside the table)");
+        return 0;
+    }
     if (p->ikaptyp == 1) { /* pprphy.f:41-47 */
         for (size_t n = 0; n < O.n3; ++n) {
             O.XKR[n] = p->zkapmult; O.XKP[n] = p->zkapmult; O.XKA[n] = p->zkapmult;
@@ -1074,7 +1163,7 @@ static int kappa(void) /* KAPPA, pprphy.f:1-360; case 26 is new */
         }
         return 0;
     }
-    if (p->ikaptyp != 26) return seterr(1, "KAPPA: IKAPTYP must be 1, 4-14 or 26");
+    if (p->ikaptyp != 26) return seterr(1, "KAPPA: IKAPTYP must be 1, 4-14, 20-25 or 26");
     /* Power law kappa_R = kappa_P = kappa_A = kap0 (rho/rho0)^a (T/T0)^b in the
      * loop structure of the IKAPTYP=8 power law (pprphy.f:176-198): cells up to
      * upperbnd are evaluated, cells above copy the upperbnd value.  Unlike
@@ -1271,6 +1360,72 @@ static double calc_tincident(int j, int k) /* CALC_TINCIDENT, pprad.f:296-405, F
     return 0.0;
 }
 
+/* The wavelength-binned branch of STELLARINPUT_HEATING (pprad.f:429-447) for
+ * IKAPTYP 20, 21, 23 and 24: per bin, the optical depth by a reverse scan of
+ * xkapW rho DR from upperbnd (WAVELENGTH_CALCSTELLARTAU, pprad.f:519-547;
+ * xkapW at upperbnd is upperbnd-1's, as KAPPA copies it), its central
+ * difference (WAVELENGTH_STELLAR_DTAU, :549-569: dtau(1) = dtau(2),
+ * dtau(nx+1) = 0), and STELLARINPUT = sum over the bins in order of
+ * BSTAR_BIN exp(-tau/mu*) dtau, times the dilution, over dxb.
+ * The reference's i loop runs to nx+2 over arrays declared 1:nx (STELLARINPUT)
+ * and 1:nx+1 (tau, dtau), and STELLARINPUT is zeroed once and then summed
+ * into, so the column before in Fortran order (j-1, or ny+2 of k-1) leaves
+ * at i = nx+2 a sum over its own mu* of this column's tau(1), dtau(1) (times
+ * the dilution over dxb(nx+2)) in this column's STELLARINPUT(2), to which
+ * this column's own terms are then added; at i = nx+1 it adds zeros
+ * (dtau(nx+1) = 0).  The first column of the box gets nothing. */
+static int stellarinput_wave(void)
+{
+    if (!g_wtab_set) return seterr(1, "STELLARINPUT_HEATING: IKAPTYP 20-24 need the wavelength table");
+    const double dr = XXB(2) - XXB(1);
+    const int nt = NX + 2; /* tau(1..nx+1), tau[l * nt + i] */
+    int bad = 0;
+#pragma omp parallel for reduction(| : bad)
+    for (int k = -1; k <= NZ + 2; ++k) {
+        double* tau = (double*)calloc((size_t)30 * nt, sizeof(double));
+        double* dtau = (double*)calloc((size_t)30 * nt, sizeof(double));
+        for (int j = -1; j <= NY + 2; ++j) {
+            const int ub = UBND(j, k);
+            memset(tau, 0, sizeof(double) * 30 * nt);
+            double tot[30], kw[30];
+            for (int l = 0; l < 30; ++l) tot[l] = 0.0;
+            for (int i = ub; i >= 1; --i) {
+                const int c = i < ub ? i : ub - 1;
+                if (burrows_wave(A(O.T, c, j, k), A(O.RH, c, j, k), kw)) { bad = 1; continue; }
+                for (int l = 0; l < 30; ++l) {
+                    tot[l] = tot[l] + (kw[l] * A(O.RH, i, j, k) * dr);
+                    tau[l * nt + i] = tot[l];
+                }
+            }
+            for (int l = 0; l < 30; ++l) {
+                double* t = tau + l * nt;
+                double* d = dtau + l * nt;
+                for (int i = 2; i <= NX; ++i) d[i] = (t[i - 1] - t[i + 1]) / 2.0;
+                d[1] = d[2];
+                d[NX + 1] = 0.0;
+            }
+            const double mu = O.MUSTAR[I2(j, k)];
+            double carry = 0.0; /* the previous column's i = nx+2 sum */
+            if (j > -1 || k > -1) {
+                const double mup = j > -1 ? O.MUSTAR[I2(j - 1, k)] : O.MUSTAR[I2(NY + 2, k - 1)];
+                for (int l = 0; l < 30; ++l)
+                    carry = carry + g_bstar[l] * (exp(-tau[l * nt + 1] / mup)) * dtau[l * nt + 1];
+                carry = carry * g_wdil / DXB(NX + 2);
+            }
+            for (int i = 1; i <= NX; ++i) {
+                double s = i == 2 ? carry : 0.0;
+                for (int l = 0; l < 30; ++l)
+                    s = s + g_bstar[l] * (exp(-tau[l * nt + i] / mu)) * dtau[l * nt + i];
+                A(O.SIN, i, j, k) = s * g_wdil / DXB(i);
+            }
+        }
+        free(tau);
+        free(dtau);
+    }
+    if (bad) return seterr(2, "NEG. T. IN BURROWS OPC (or a density outside the table)");
+    return 0;
+}
+
 static int stellarinput_heating(void) /* STELLARINPUT_HEATING, pprad.f:408-516, grey */
 {
     if (O.p.f_incident == 0) {
@@ -1278,6 +1433,8 @@ static int stellarinput_heating(void) /* STELLARINPUT_HEATING, pprad.f:408-516, 
         return 0;
     }
     if (O.p.f_incident != 2) return seterr(1, "STELLARINPUT_HEATING: F_INCIDENT must be 0 or 2");
+    const int ik = O.p.ikaptyp;
+    if (ik == 20 || ik == 21 || ik == 23 || ik == 24) return stellarinput_wave();
     stellar_tau();
     /* The reference's i loop runs to nx+2 over an array declared 1:nx; the
      * two extra writes per column land on (1:2, j+1, k) and are overwritten

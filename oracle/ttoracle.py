@@ -51,6 +51,7 @@ class Oracle:
                                         ctypes.POINTER(ctypes.c_int)]
             L.tor_set_opacity_table.argtypes = [ctypes.c_void_p] * 3
             L.tor_set_burrows_table.argtypes = [ctypes.c_void_p] * 3
+            L.tor_set_burrows_wave_table.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double]
             L.tor_set_column_table.argtypes = [ctypes.c_void_p]
             L.tor_set_viscosity.argtypes = [ctypes.POINTER(P.Viscosity)]
             L.tor_get_viscosity.argtypes = [ctypes.POINTER(P.Viscosity)]
@@ -128,6 +129,17 @@ class Oracle:
         assert all(a.size == 50 * 50 for a in t)
         self._check(self.L.tor_set_burrows_table(*(a.ctypes.data_as(ctypes.c_void_p) for a in t)),
                     "set_burrows_table")
+
+    def set_burrows_wave_table(self, kap_wave, bstar, dilution):
+        """IKAPTYP 20-25: log10 of the 30 bins' irradiation opacity on the
+        Burrows grid (T fastest, then rho, then bin), the bins' BSTAR_BIN and
+        the dilution (RSTAR RSOL / semi)^2."""
+        kw = np.ascontiguousarray(np.asarray(kap_wave, np.float64).reshape(-1))
+        bs = np.ascontiguousarray(np.asarray(bstar, np.float64).reshape(-1))
+        assert kw.size == 50 * 50 * 30 and bs.size == 30
+        self._check(self.L.tor_set_burrows_wave_table(kw.ctypes.data_as(ctypes.c_void_p),
+                                                      bs.ctypes.data_as(ctypes.c_void_p), float(dilution)),
+                    "set_burrows_wave_table")
 
     def set_viscosity(self, ivis=1, ivistyp=1, ieta=4, idiss=1, xnue=1e9, iartvis=0):
         """ETACALC + VISC inside FORGAS3D (ppfor.f:170-174), off by default;

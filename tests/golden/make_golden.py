@@ -702,6 +702,86 @@ def gravity4():
          {"delt": dt, "iii": 2, "nsteps": 2})
 
 
+def burrows_wave_table_text(ikaptyp, seed=5, wseed=7):
+    """A synthetic IKAPTYP 20-25 table in the reference's text format: the
+    grey table's header and means, then per row the 30 irradiation bins (20,
+    21: 35 values a row) and, for 23-25, 60 more values (the two-stream bins
+    only IRAD 5-7 read)."""
+    from ttcode_amd import opacity as O
+    lt, lr, kp, kr, ka = O.synthetic_burrows_table(seed)
+    kw, _ = O.synthetic_wave_table(wseed)
+    kw = kw.reshape(O.NWAVE, O.BNR, O.BNT)
+    rng = np.random.default_rng(wseed + 1)
+    rows = []
+    for it in range(O.BNT):
+        for ir in range(O.BNR):
+            q = it + O.BNT * ir
+            vals = [10.0 ** float(lt[it]), 10.0 ** float(lr[ir]), float(kp[q]), float(kr[q]), float(ka[q])]
+            vals += [float(v) for v in kw[:, ir, it]]
+            if ikaptyp not in (20, 21):
+                vals += [float(v) for v in rng.uniform(-3.0, 1.0, 60)]
+            rows.append(" ".join(repr(v) for v in vals))
+    head = [f"synthetic Burrows wavelength table line {i + 1}" for i in range(10)]
+    return "\n".join(head + ["6070.0"] + rows) + "\n"
+
+
+def bstar_text(seed=7):
+    from ttcode_amd import opacity as O
+    _, bstar = O.synthetic_wave_table(seed)
+    lines = ["synthetic BSTAR bins line 1", "line 2", "line 3", "6070.0", "30"]
+    lines += [f"{l + 1} {float(b)!r}" for l, b in enumerate(bstar)]
+    return "\n".join(lines) + "\n"
+
+
+def burrows_wave():
+    """IKAPTYP 21 (the shipped ipp's) and 24: KAPPA's means with the cells
+    from upperbnd up taking upperbnd-1's (pprphy.f:321-349) on a scrambled
+    state, and COUPLED_ENERGY_EQS with the wavelength-binned STELLARINPUT
+    (pprad.f:429-447,519-569) on synthetic tables read by the reference's
+    own READBURROWSOPCTABLE (24: 95-value rows, the extra absorber on every
+    bin)."""
+    import tempfile as _tf
+    from ttcode_amd import opacity as O
+    nx, ny, nz = 12, 16, 12
+    o = Oracle(P.make_params(nx, ny, nz, ikaptyp=1, kap0=1e-2))
+    o.load_state(base_inputs(o.p))
+    o.bounds()
+    o.pressg()
+    dt = o.schritt()
+    btext = bstar_text()
+    for ikap in (21, 24):
+        text = burrows_wave_table_text(ikap)
+        fname = O.BURROWS_WAVE_FILES[ikap]
+        p = P.make_params(nx, ny, nz, ikaptyp=ikap, epsmat=1e-8, mstar2=0.000654)
+        with _tf.TemporaryDirectory() as d:
+            fn = os.path.join(d, fname)
+            with open(fn, "w") as f:
+                f.write(text)
+            kp, kr, ka, kw, _ = O.read_burrows_wave_table(fn, ikap, planet_num=1)
+            bn = os.path.join(d, "b.dat")
+            with open(bn, "w") as f:
+                f.write(btext)
+            bstar = O.read_bstar(bn)
+        tab = {"tab_planck": kp, "tab_ross": kr, "tab_abs": ka, "tab_wave": kw, "tab_bstar": bstar,
+               "wave_dilution": O.wave_dilution(mstar2=0.000654)}
+        files = {fname: text, O.BSTAR_FILES[1]: btext}
+        if ikap == 21:
+            inp = base_inputs(p, seed=81)
+            rng = np.random.default_rng(81)
+            T = 10.0 ** rng.uniform(1.5, 3.9, size=inp["T"].shape)
+            f = np.linspace(0.0, 1.0, inp["RH"].shape[-1])
+            RH = 10.0 ** (-1.0 - 11.8 * f + rng.uniform(-0.6, 0.6, size=inp["RH"].shape))
+            inp_k = dict(inp, T=T, RH=RH)
+            outs, sc = run_ref("kappa", p, inp_k, files=files, extra={"h_mstar2": 0.000654})
+            save("kappa_burrows21_12x16x12", p, inp_k, outs, sc,
+                 ["T", "RH", "xkapR", "xkapP", "xkapA", "dxkapP_dT", "dxkapA_dT", "sig", "upperbnd"], tab)
+        inp_e = base_inputs(p, seed=83 + ikap, contrast=0.05)
+        outs, sc = run_ref("energy", p, inp_e, delt=dt, iii=2, files=files, extra={"h_mstar2": 0.000654})
+        save(f"energy_burrows{ikap}_12x16x12", p, inp_e, outs, sc,
+             ["T", "ER", "delta_ER", "xkapR", "xkapP", "xkapA", "dxkapP_dT", "dxkapA_dT", "STELLARINPUT",
+              "FQ", "FR", "RHS_ER", "upperbnd"], dict(tab, delt=dt, iii=2))
+
+
 FBOT_IPP = 15844.230  # FBOT of the shipped ipp (ipp:12), erg cm^-2 s^-1
 
 
@@ -789,6 +869,8 @@ def rotating():
 if __name__ == "__main__":
     if sys.argv[1:] == ["rotating"]:
         rotating()
+    elif sys.argv[1:] == ["burrows_wave"]:
+        burrows_wave()
     elif sys.argv[1:] == ["bcxmin9"]:
         bcxmin9()
     elif sys.argv[1:] == ["gravity4"]:

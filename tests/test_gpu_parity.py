@@ -107,6 +107,36 @@ def test_bounds(ttgpu, name, rel):
     d.close()
 
 
+def _burrows_wave(d, g):
+    d.set_burrows_table(g["tab_planck"], g["tab_ross"], g["tab_abs"])
+    d.set_burrows_wave_table(g["tab_wave"], g["tab_bstar"], float(g["wave_dilution"]))
+
+
+def test_kappa_burrows21(ttgpu):
+    """KAPPA, IKAPTYP 21: the means, and upperbnd-1's values from upperbnd up."""
+    g = C.load("kappa_burrows21_12x16x12")
+    p, d = _dev(ttgpu, g)
+    _burrows_wave(d, g)
+    d.bounds()
+    d.kappa()
+    _cmp(g, p, d)
+    d.close()
+
+
+@pytest.mark.parametrize("name", ["energy_burrows21_12x16x12", "energy_burrows24_12x16x12"])
+def test_coupled_energy_burrows_wave(ttgpu, name):
+    """COUPLED_ENERGY_EQS with IKAPTYP 21 and 24: the wavelength-binned
+    STELLARINPUT (pprad.f:429-447,519-569) within 1e-12 of the reference."""
+    g = C.load(name)
+    p, d = _dev(ttgpu, g)
+    _burrows_wave(d, g)
+    st, _ = d.coupled_energy(float(g["delt"]), int(g["iii"]))
+    assert st.energy_iter == int(g["scal_energy_iter"])
+    assert st.sor_iter_last == int(g["scal_SOR_ITER"])
+    _cmp(g, p, d, rel=1e-12, scaled_tol=1e-10)
+    d.close()
+
+
 def test_bounds_bcxmin9(ttgpu):
     """BOUNDXMIN(9) at III = 1 (ppbou.f:1029-1066) on varying opacities."""
     g = C.load("bounds_bcxmin9_12x16x12")

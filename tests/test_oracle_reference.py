@@ -417,6 +417,68 @@ def _burrows(o, g):
     o.set_burrows_table(g["tab_planck"], g["tab_ross"], g["tab_abs"])
 
 
+def _burrows_wave(o, g):
+    o.set_burrows_table(g["tab_planck"], g["tab_ross"], g["tab_abs"])
+    o.set_burrows_wave_table(g["tab_wave"], g["tab_bstar"], float(g["wave_dilution"]))
+
+
+def test_kappa_burrows21():
+    """KAPPA with IKAPTYP 21 (pprphy.f:321-349): BurrowsOPC's means up to
+    upperbnd, and the cells from upperbnd up take upperbnd-1's values (not
+    upperbnd's, as 9-14 do), on a state that reaches both table clamps."""
+    def act(o, g):
+        _burrows_wave(o, g)
+        o.bounds()
+        o.kappa()
+    g, p, o, _ = _run("kappa_burrows21_12x16x12", act)
+    _check_fields(g, p, o)
+
+
+@pytest.mark.parametrize("name", ["energy_burrows21_12x16x12", "energy_burrows24_12x16x12"])
+def test_coupled_energy_burrows_wave(name):
+    """COUPLED_ENERGY_EQS with IKAPTYP 21 and 24: the wavelength-binned
+    STELLARINPUT (pprad.f:429-447,519-569) from the synthetic tables the
+    reference's READBURROWSOPCTABLE read (24: 95-value rows, the extra
+    absorber on every bin) and the BSTAR bins, bit for bit."""
+    def act(o, g):
+        _burrows_wave(o, g)
+        return o.coupled_energy(float(g["delt"]), int(g["iii"]))[0]
+    g, p, o, st = _run(name, act)
+    assert st.energy_iter == int(g["scal_energy_iter"])
+    assert st.sor_iter_last == int(g["scal_SOR_ITER"])
+    _check_fields(g, p, o)
+    assert np.count_nonzero(g["out_STELLARINPUT"]) > g["out_STELLARINPUT"].size // 4
+
+
+def test_burrows_wave_reader_and_dilution(tmp_path):
+    """READBURROWSOPCTABLE's rows of 35 (IKAPTYP 21) and 95 values (24, with
+    the planet's extra absorber on every bin), the BSTAR bins, and the
+    dilution (RSTAR RSOL / semi)^2 with semi from pp.f:204-206: the fixtures'
+    STELLARINPUT (above) was computed by the reference from the same files
+    and these values reproduce it; here the reader is checked against the
+    table it was written from."""
+    import math
+    from tests.golden import make_golden as M
+    from ttcode_amd import opacity as O
+    kw0, bstar0 = O.synthetic_wave_table(7)
+    for ikap in (21, 24):
+        fn = tmp_path / f"t{ikap}.dat"
+        fn.write_text(M.burrows_wave_table_text(ikap))
+        kp, kr, ka, kw, tstar = O.read_burrows_wave_table(str(fn), ikap)
+        kw = kw.reshape(O.NWAVE, O.BNR, O.BNT)
+        want = kw0.reshape(O.NWAVE, O.BNR, O.BNT)
+        if ikap == 24:
+            ext = float(np.float32(0.1))
+            want = np.vectorize(lambda x: math.log10(10.0 ** x + ext))(want)
+        np.testing.assert_array_equal(kw, want)
+        assert tstar == 6070.0
+    fn = tmp_path / "b.dat"
+    fn.write_text(M.bstar_text())
+    np.testing.assert_array_equal(O.read_bstar(str(fn)), bstar0)
+    g = C.load("energy_burrows21_12x16x12")
+    assert float(g["wave_dilution"]) == O.wave_dilution(mstar2=0.000654)
+
+
 def test_burrows_table_reader(tmp_path):
     """READBURROWSOPCTABLE restated (ttcode_amd/opacity.py): rows in file
     order, and the per-IKAPTYP rewrites: 12 and 14 absorption = Planck +

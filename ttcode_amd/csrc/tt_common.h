@@ -104,6 +104,8 @@ struct Lib {
     int btab_version = -1;
     double* ctab = nullptr;         // IKAPTYP 6/7 column table, then 10^ of it (2 x 8 x 42 x 18)
     int ctab_version = -1;
+    double* wtab = nullptr;         // IKAPTYP 20-25 irradiation bins (50 x 50 x 30, log10)
+    int wtab_version = -1;
     int have_de1 = 0;
     double* partial = nullptr;      // block partial sums (reductions)
     int npartial = 0;

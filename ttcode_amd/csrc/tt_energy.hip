@@ -293,18 +293,21 @@ __device__ inline bool burrows_cell(double t_in, double rh_in, const double* __r
 
 // KAPPA, IKAPTYP 9-12 and 14 (pprphy.f:199-234,242-274, MODTYP=2): BurrowsOPC
 // for cells up to upperbnd, whose values the cells above take; SUBSIG.  One
-// thread per cell evaluates at (min(i, upperbnd), j, k).
+// thread per cell evaluates at (min(i, upperbnd), j, k).  IKAPTYP 20-25
+// (below != 0, pprphy.f:321-349): the cells from upperbnd up take
+// upperbnd-1's values.
 __global__ void k_kappa_bur(Geo g, const int32_t* __restrict__ ub, const double* __restrict__ T,
                             const double* __restrict__ RH, const double* __restrict__ tab, double* XKR,
                             double* XKP, double* XKA, double* DKP, double* DKA, double* SIG, double sig,
-                            Ctrl* ctrl) {
+                            int below, Ctrl* ctrl) {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= g.n3) return;
     SIG[t] = sig;
     int i, j, k;
     box_ijk(g, t, i, j, k);
     const int u = ub[g.idx2(j, k)];
-    const long long n = g.idx(i <= u ? i : u, j, k);
+    const int src = below ? (i < u ? i : u - 1) : (i <= u ? i : u);
+    const long long n = g.idx(src, j, k);
     double o[5];
     if (!burrows_cell(T[n], RH[n], tab, o)) {
         ctrl->bad_kappa = 1;
@@ -401,6 +404,115 @@ __global__ void k_sin(Geo g, Met m, tt_params p, const double* __restrict__ STAU
     double mu = MU[g.idx2(j, k)];
     double t4 = pow(tinc, 4.0);
     SIN[IX(i, j, k)] = SBCONST * (t4 / mu) * (exp(-STAU[IX(i, j, k)] / mu)) * dt / m(TT_M_DXB, i);
+}
+
+// The wavelength-binned STELLARINPUT_HEATING (pprad.f:429-447) for IKAPTYP
+// 20, 21, 23 and 24, one 64-lane wavefront per (j, k) column:
+//  1. lanes over the cells: each cell's 30 bins of xkapW (BurrowsOPC's
+//     bilinear interpolant of the irradiation table, pprphy.f:1138-1151, at
+//     the cell, or at upperbnd-1 for the cell upperbnd, as KAPPA copies it)
+//     times RH, into LDS, tw[l][i];
+//  2. lanes over the bins: the reverse scan from upperbnd, tau = tau +
+//     (xkapW RH) DR, in the reference's order (WAVELENGTH_CALCSTELLARTAU,
+//     pprad.f:519-547), in place; tau = 0 above upperbnd;
+//  3. lanes over the cells: dtau = (tau(i-1) - tau(i+1)) / 2, dtau(1) =
+//     dtau(2) (WAVELENGTH_STELLAR_DTAU), and the sum over the bins in order
+//     of BSTAR_BIN exp(-tau/mu*) dtau, times the dilution, over dxb.
+// The reference zeroes STELLARINPUT once and sums into it over i = 1..nx+2,
+// while the array is declared 1:nx and tau, dtau 1:nx+1: the column before
+// (in Fortran order) leaves at its i = nx+2 a sum of this column's tau(1),
+// dtau(1) over its own mu* (times the dilution over dxb(nx+2)) in this
+// column's STELLARINPUT(2), to which the own terms are added (its i = nx+1
+// adds zeros: dtau(nx+1) = 0).  Reproduced as `carry`.
+struct WaveArgs {
+    double bstar[TT_NWAVE];
+    double dil;
+};
+
+__global__ __launch_bounds__(64) void k_sin_wave(Geo g, Met m, const int32_t* __restrict__ ub,
+                                                 const double* __restrict__ T, const double* __restrict__ RH,
+                                                 const double* __restrict__ wtab, const double* __restrict__ MU,
+                                                 WaveArgs w, double* SIN, Ctrl* ctrl) {
+    extern __shared__ double tw[];  // tw[l * nt + i], i = 0..nx+1
+    const int nt = g.nx + 2;
+    const int nj = g.ny + 4;
+    const int j = (int)(blockIdx.x % nj) - 1, k = (int)(blockIdx.x / nj) - 1;
+    const int lane = threadIdx.x;
+    const int u = ub[g.idx2(j, k)];
+    const double dr = m(TT_M_XXB, 2) - m(TT_M_XXB, 1);
+    const int TN = TT_BURROWS_NT * TT_BURROWS_NR;
+    for (int i = lane; i <= g.nx + 1; i += 64) {
+        if (i >= 1 && i <= u) {
+            const int c = i < u ? i : u - 1;
+            const double t_in = T[IX(c, j, k)], rh_in = RH[IX(c, j, k)];
+            const double rh = RH[IX(i, j, k)];
+            bool ok = t_in > 0.0;
+            double rh1 = log10(rh_in), tm1 = log10(t_in);
+            if (tm1 < (double)1.69896f) tm1 = (double)1.69897f;
+            if (tm1 > (double)3.69889f) tm1 = (double)3.69888f;
+            if (rh1 < (double)-11.9996f) rh1 = (double)-11.9995f;
+            if (rh1 > (double)-1.99993f) rh1 = (double)-1.99994f;
+            if (rh1 != rh1) ok = false;
+            if (!ok) {
+                ctrl->bad_kappa = 1;
+                for (int l = 0; l < TT_NWAVE; ++l) tw[l * nt + i] = 0.0;
+                continue;
+            }
+            const int it = first_above(c_bt, tm1, (tm1 - c_bt[0]) * (49.0 / (c_bt[49] - c_bt[0])));
+            const int it1 = it - 1;
+            double ws1 = c_bt[it1] - tm1;
+            double ws = c_bt[it1] - c_bt[it];
+            ws1 = ws1 / ws;
+            ws = 1.0 - ws1;
+            const int iw = first_above(c_brh, rh1, (rh1 - c_brh[0]) * (49.0 / (c_brh[49] - c_brh[0])));
+            const int iw1 = iw - 1;
+            double ws2 = rh1 - c_brh[iw1];
+            double ws3 = c_brh[iw] - c_brh[iw1];
+            ws2 = ws2 / ws3;
+            ws3 = 1.0 - ws2;
+            for (int l = 0; l < TT_NWAVE; ++l) {
+                const double* tq = wtab + l * TN;
+                const double a00 = tq[it + TT_BURROWS_NT * iw], a10 = tq[it + TT_BURROWS_NT * iw1];
+                const double a11 = tq[it1 + TT_BURROWS_NT * iw1], a01 = tq[it1 + TT_BURROWS_NT * iw];
+                const double w4 = ws2 * (ws1 * a00 + ws * a01);
+                const double w5 = ws3 * (ws1 * a10 + ws * a11);
+                tw[l * nt + i] = exp10(w4 + w5) * rh;
+            }
+        } else {
+            for (int l = 0; l < TT_NWAVE; ++l) tw[l * nt + i] = 0.0;
+        }
+    }
+    __syncthreads();
+    if (lane < TT_NWAVE) {
+        double* t = tw + lane * nt;
+        double tot = 0.0;
+        for (int i = u; i >= 1; --i) {
+            tot = tot + (t[i] * dr);
+            t[i] = tot;
+        }
+    }
+    __syncthreads();
+    const double mu = MU[g.idx2(j, k)];
+    for (int i = 1 + lane; i <= g.nx; i += 64) {
+        double s = 0.0;
+        if (i == 2 && (j > -1 || k > -1)) {
+            const double mup = j > -1 ? MU[g.idx2(j - 1, k)] : MU[g.idx2(g.ny + 2, k - 1)];
+            double carry = 0.0;
+            for (int l = 0; l < TT_NWAVE; ++l) {
+                const double* t = tw + l * nt;
+                const double d = (t[1] - t[3]) / 2.0;  // dtau(1) = dtau(2)
+                carry = carry + w.bstar[l] * (exp(-t[1] / mup)) * d;
+            }
+            s = carry * w.dil / m(TT_M_DXB, g.nx + 2);
+        }
+        const int ic = i == 1 ? 2 : i;
+        for (int l = 0; l < TT_NWAVE; ++l) {
+            const double* t = tw + l * nt;
+            const double d = (t[ic - 1] - t[ic + 1]) / 2.0;
+            s = s + w.bstar[l] * (exp(-t[i] / mu)) * d;
+        }
+        SIN[IX(i, j, k)] = s * w.dil / m(TT_M_DXB, i);
+    }
 }
 
 __global__ void k_fill_box(double* a, double v, long long n) {
@@ -773,6 +885,12 @@ static int g_ftab_version = 0;  // 0: never set
 // IKAPTYP 9-14 tables likewise, followed by the log10 T axis
 static double g_btab[3 * TT_BURROWS_NT * TT_BURROWS_NR + TT_BURROWS_NT];
 static int g_btab_version = 0;
+// IKAPTYP 20-25 irradiation bins (log10, T fastest, then rho, then bin), the
+// bins' stellar flux and the dilution
+static double g_wtab[TT_NWAVE * TT_BURROWS_NT * TT_BURROWS_NR];
+static double g_bstar[TT_NWAVE];
+static double g_wdil = 0.0;
+static int g_wtab_version = 0;
 // IKAPTYP 6/7 column-depth absorption table (42 x 18 x 8)
 static double g_ctab[8 * TT_FREEDMAN_NT * TT_FREEDMAN_NP];
 static int g_ctab_version = 0;
@@ -798,7 +916,7 @@ __global__ void k_clear_bad_kappa(Ctrl* c) {
 // reference stops on (ctrl->bad_kappa)
 static bool table_kappa() {
     const int t = L.p.ikaptyp;
-    return (t >= 4 && t <= 7) || (t >= 9 && t <= 12) || t == 14;
+    return (t >= 4 && t <= 7) || (t >= 9 && t <= 12) || t == 14 || (t >= 20 && t <= 25);
 }
 
 // the Burrows tables on the device, refreshed when the host's have changed
@@ -862,13 +980,14 @@ static int kappa_impl() {
         TT_LAUNCH();
         return 0;
     }
-    if ((L.p.ikaptyp >= 9 && L.p.ikaptyp <= 12) || L.p.ikaptyp == 14) {
-        if (g_btab_version == 0) return fail(1, "KAPPA: IKAPTYP 9-12 and 14 need tt_set_burrows_table first");
+    const bool wave = L.p.ikaptyp >= 20 && L.p.ikaptyp <= 25;
+    if ((L.p.ikaptyp >= 9 && L.p.ikaptyp <= 12) || L.p.ikaptyp == 14 || wave) {
+        if (g_btab_version == 0) return fail(1, "KAPPA: IKAPTYP 9-14 and 20-25 need tt_set_burrows_table first");
         int rc = btab_sync();
         if (rc) return rc;
         k_kappa_bur<<<nblk(L.g.n3, 256), 256, 0, L.s>>>(L.g, L.ub, f[TT_T], f[TT_RH], L.btab, f[TT_XKAPR],
                                                        f[TT_XKAPP], f[TT_XKAPA], f[TT_DXKAPP_DT],
-                                                       f[TT_DXKAPA_DT], f[TT_SIG], sig, L.ctrl);
+                                                       f[TT_DXKAPA_DT], f[TT_SIG], sig, wave ? 1 : 0, L.ctrl);
         TT_LAUNCH();
         return 0;
     }
@@ -905,6 +1024,25 @@ static int stellar_impl() {
     }
     if (L.p.f_incident != 2) return fail(1, "STELLARINPUT_HEATING: F_INCIDENT must be 0 or 2");
     int ncol = (L.g.ny + 4) * (L.g.nz + 4);
+    const int ik = L.p.ikaptyp;
+    if (ik == 20 || ik == 21 || ik == 23 || ik == 24) {  // the wavelength-binned branch (pprad.f:429-447)
+        if (g_wtab_version == 0)
+            return fail(1, "STELLARINPUT_HEATING: IKAPTYP 20-24 need tt_set_burrows_wave_table first");
+        if (L.wtab_version != g_wtab_version) {
+            TT_HIP(hipMemcpyAsync(L.wtab, g_wtab, sizeof g_wtab, hipMemcpyHostToDevice, L.s));
+            TT_HIP(hipStreamSynchronize(L.s));
+            L.wtab_version = g_wtab_version;
+        }
+        const size_t lds = (size_t)TT_NWAVE * (L.g.nx + 2) * sizeof(double);
+        if (lds > 64 * 1024) return fail(1, "STELLARINPUT_HEATING: the wavelength branch needs nx <= 270");
+        WaveArgs w;
+        for (int l = 0; l < TT_NWAVE; ++l) w.bstar[l] = g_bstar[l];
+        w.dil = g_wdil;
+        k_sin_wave<<<ncol, 64, lds, L.s>>>(L.g, met(), L.ub, f[TT_T], f[TT_RH], L.wtab, L.mustar, w,
+                                          f[TT_STELLARINPUT], L.ctrl);
+        TT_LAUNCH();
+        return 0;
+    }
     k_stau<<<nblk(ncol, 64), 64, 0, L.s>>>(L.g, met(), L.ub, f[TT_XKAPA], f[TT_RH], f[TT_STELLARTAU]);
     TT_LAUNCH();
     long long ncell = (long long)(L.g.nx + 1) * ncol;
@@ -1130,6 +1268,15 @@ extern "C" int tt_set_burrows_table(const double* kap_planck, const double* kap_
     memcpy(g_btab + 2 * n, kap_abs, n * sizeof(double));
     memcpy(g_btab + 3 * n, h_bt, sizeof h_bt);
     ++g_btab_version;
+    return 0;
+}
+
+extern "C" int tt_set_burrows_wave_table(const double* kap_wave, const double* bstar, double dilution) {
+    if (!kap_wave || !bstar) return fail(1, "tt_set_burrows_wave_table: null table");
+    memcpy(g_wtab, kap_wave, sizeof g_wtab);
+    memcpy(g_bstar, bstar, sizeof g_bstar);
+    g_wdil = dilution;
+    ++g_wtab_version;
     return 0;
 }
 

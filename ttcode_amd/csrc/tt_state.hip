@@ -194,7 +194,7 @@ static void free_all() {
         L.f[f] = nullptr;
     }
     double** tmp[] = {&L.ED,  &L.VD,      &L.GDN,    &L.HDN, &L.FLMX,   &L.FLMY, &L.FLMZ, &L.TOLD, &L.VH,
-                      &L.GH,  &L.HH,      &L.DFQ,    &L.DE1, &L.partial, &L.mustar, &L.met, &L.ftab, &L.btab, &L.ctab, &g_stage};
+                      &L.GH,  &L.HH,      &L.DFQ,    &L.DE1, &L.partial, &L.mustar, &L.met, &L.ftab, &L.btab, &L.ctab, &L.wtab, &g_stage};
     for (auto p : tmp) {
         if (*p) hipFree(*p);
         *p = nullptr;
@@ -261,8 +261,9 @@ int tt_init(const tt_params* p) {
     if (p->bcxmin != 12 && p->bcxmin != 9) return fail(1, "BCXMIN must be 9 or 12");
     if (p->ibdrad != 3 && p->ibdrad != 4 && p->ibdrad != 7 && p->ibdrad != 9)
         return fail(1, "IBDRAD must be 3, 4, 7 or 9");
-    if (p->ikaptyp != 1 && !(p->ikaptyp >= 4 && p->ikaptyp <= 14) && p->ikaptyp != 26)
-        return fail(1, "IKAPTYP must be 1, 4-14 or 26");
+    if (p->ikaptyp != 1 && !(p->ikaptyp >= 4 && p->ikaptyp <= 14) && p->ikaptyp != 26 &&
+        !(p->ikaptyp >= 20 && p->ikaptyp <= 25 && p->ikaptyp != 22))
+        return fail(1, "IKAPTYP must be 1, 4-14, 20, 21, 23-25 or 26");
     if (p->iflimtyp != 1 && p->iflimtyp != 2) return fail(1, "IFLIMTYP must be 1 or 2");
     if (p->f_incident != 0 && p->f_incident != 2) return fail(1, "F_INCIDENT must be 0 or 2");
     if (p->isigtyp != 0 && p->isigtyp != 1) return fail(1, "ISIGTYP must be 0 or 1");
@@ -307,6 +308,7 @@ int tt_init(const tt_params* p) {
     TT_HIP(dev_alloc(&L.ftab, (size_t)6 * TT_FREEDMAN_NT * TT_FREEDMAN_NP * sizeof(double)));
     TT_HIP(dev_alloc(&L.ctab, (size_t)16 * TT_FREEDMAN_NT * TT_FREEDMAN_NP * sizeof(double)));
     TT_HIP(dev_alloc(&L.btab, (size_t)2 * (3 * TT_BURROWS_NT * TT_BURROWS_NR + TT_BURROWS_NT) * sizeof(double)));
+    TT_HIP(dev_alloc(&L.wtab, (size_t)TT_NWAVE * TT_BURROWS_NT * TT_BURROWS_NR * sizeof(double)));
     TT_HIP(dev_alloc(&L.ctrl, sizeof(Ctrl)));
     TT_HIP(hipMemsetAsync(L.ctrl, 0, sizeof(Ctrl), L.s));
     TT_HIP(hipHostMalloc((void**)&L.hctrl, sizeof(Ctrl), hipHostMallocDefault));

@@ -184,6 +184,110 @@ def synthetic_burrows_table(seed: int = 0):
             ka.reshape(-1))
 
 
+# ---- IKAPTYP 20-25: the Burrows tables with 30 wavelength bins
+NWAVE = 30  # nwave_bins (ppmod.f:489)
+# OPENOPACITYFILE_WITH_TIOVO / _WOTIOVO (ppplanets.f:90-217) for planet_num 1
+# (HD209458b).  IKAPTYP 22 opens neither (the reference's reader stops on it,
+# pprphy.f:1265-1275); 25 falls through to the 5-column grey file for planet 1.
+BURROWS_WAVE_FILES = {20: "BURROWS6070_wTiVO_wavelth.dat", 21: "BURROWS6070_woTiVO_wavelth.dat",
+                      23: "BURROWS6070_woTiVO_radtransfer.dat", 24: "BURROWS6070_woTiVO_radtransfer.dat"}
+BSTAR_FILES = {1: "BSTAR_WAVEBINS_TS6070.dat", 2: "BSTAR_WAVEBINS_TS5040.dat", 5: "BSTAR_WAVEBINS_TS6300.dat"}
+
+
+def read_burrows_wave_table(path, ikaptyp, planet_num=1):
+    """READBURROWSOPCTABLE (pprphy.f:1227-1520) for IKAPTYP 20-25.
+
+    Ten header records, T_star, then 2500 rows in the order (T outer, rho
+    inner): ``T rho log_kP log_kR log_kABS`` and the 30 bins' log opacity of
+    the stellar irradiation (35 values, IKAPTYP 20 and 21), or those and 60
+    more (the local re-radiation opacity and Planck bins of the two-stream
+    transfer, which only IRAD 5-7 read; 95 values, 22-25).  IKAPTYP 20, 24 and
+    25 add the planet's extra absorber to every irradiation bin, log k =
+    log10(10^log k + kappa_extra) (pprphy.f:1438-1469).
+
+    Returns ``(kap_planck, kap_ross, kap_abs, kap_wave, t_star)``: three
+    50 x 50 tables, temperature fastest, and 50 x 50 x 30 bins, temperature
+    fastest, then rho, then bin (BurrowsXKAP_waveopac(IT, IR, L))."""
+    import math
+    if ikaptyp not in (20, 21, 22, 23, 24, 25):
+        raise ValueError(f"IKAPTYP {ikaptyp} is not a wavelength-binned Burrows table (20-25)")
+    if ikaptyp == 22:
+        raise ValueError("IKAPTYP 22: the reference's READBURROWSOPCTABLE opens no file for it and stops")
+    ncol = 35 if ikaptyp in (20, 21) else 95
+    recs = (r for r in _records(path) if r)
+    for _ in range(10):
+        next(recs)
+    t_star = _number(next(recs)[0])
+    tabs = np.full((3, BNR, BNT), MISSING)
+    wave = np.full((NWAVE, BNR, BNT), MISSING)
+    kap_ext = PLANET_KAP_EXTRA[planet_num] if ikaptyp in (20, 24, 25) else 0.0
+    for it in range(BNT):
+        for ir in range(BNR):
+            vals = []
+            while len(vals) < ncol:
+                try:
+                    vals += [_number(t) for t in next(recs)]
+                except StopIteration:
+                    raise ValueError(f"{path}: {it * BNR + ir} of {BNT * BNR} table rows") from None
+            tabs[:, ir, it] = vals[2:5]
+            w = vals[5:5 + NWAVE]
+            if kap_ext:
+                w = [math.log10(10.0 ** x + kap_ext) for x in w]
+            wave[:, ir, it] = w
+    kp, kr, ka = (tabs[q].reshape(-1) for q in range(3))
+    return kp, kr, ka, wave.reshape(-1), t_star
+
+
+def read_bstar(path):
+    """The bins' stellar flux BSTAR_BIN read after the table (pprphy.f:
+    1509-1538): three header records, the stellar temperature, the number of
+    bins (30), then one row per bin whose second value is BSTAR_BIN."""
+    recs = (r for r in _records(path) if r)
+    for _ in range(3):
+        next(recs)
+    next(recs)
+    n = int(_number(next(recs)[0]))
+    if n != NWAVE:
+        raise ValueError(f"{path}: {n} bins, the reference needs {NWAVE}")
+    out = []
+    while len(out) < NWAVE:
+        vals = []
+        while len(vals) < 2:
+            vals += [_number(t) for t in next(recs)]
+        out.append(vals[1])
+    return np.asarray(out)
+
+
+# global_constants (ppmod.f:1074-1083): REAL literals into DOUBLE parameters
+_GRAV, _MSOL, _RSOL, _DAY = (float(np.float32(v)) for v in (6.672e-8, 1.989e33, 6.9598e10, 8.64e4))
+_PI = 3.141592653589793
+
+
+def wave_dilution(rstar=1.155, porb=3.52, mstar1=1.119, mstar2=0.000654):
+    """(RSTAR RSOL / semi)^2 of the wavelength-binned STELLARINPUT_HEATING
+    (pprad.f:444), with semi the semi-major axis of pp.f:204-206 from the
+    orbital period (days) and the masses (solar); defaults: the shipped ipp
+    (ipp:18, 22-23)."""
+    semi = (_GRAV * _MSOL * (mstar1 + mstar2) * ((porb * _DAY) ** 2.0) / (4.0 * _PI * _PI)) ** (1.0 / 3.0)
+    return (rstar * _RSOL / semi) ** 2.0
+
+
+def synthetic_wave_table(seed: int = 0):
+    """30 smooth synthetic irradiation bins on the Burrows grid (log10 k of
+    bin l: -2.5 + 2 l/29 + 0.3 log10(rho / 1e-6) + 0.2 log10(T / 1500 K), with
+    seeded 0.02 dex noise) and a stellar spectrum over the bins whose total
+    is sigma (6070 K)^4.  Returns ``(kap_wave, bstar)``."""
+    rng = np.random.default_rng(seed)
+    lt = np.asarray(BURROWS_LOGT)[None, None, :]
+    lr = np.asarray(BURROWS_LOGRH)[None, :, None]
+    lb = np.arange(NWAVE)[:, None, None]
+    kw = -2.5 + 2.0 * lb / 29.0 + 0.3 * (lr + 6.0) + 0.2 * (lt - np.log10(1500.0)) \
+        + 0.02 * rng.standard_normal((NWAVE, BNR, BNT))
+    shape = np.exp(-0.5 * ((np.arange(NWAVE) - 12.0) / 6.0) ** 2)
+    bstar = shape / shape.sum() * float(np.float32(5.6705e-5)) * 6070.0 ** 4
+    return kw.reshape(-1), bstar
+
+
 # ---- IKAPTYP 6 and 7: the column-depth absorption table
 COLUMN_LEVELS = (-26.0, -25.0, -24.0, -23.0, -22.0, -21.0, -20.0, -19.0)  # log10 kappa cut-off (ppmod.f:701-702)
 NC = len(COLUMN_LEVELS)

@@ -56,6 +56,7 @@ def load_library():
     L.tt_sphericalvellimit.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double]
     L.tt_set_opacity_table.argtypes = [ctypes.c_void_p] * 3
     L.tt_set_burrows_table.argtypes = [ctypes.c_void_p] * 3
+    L.tt_set_burrows_wave_table.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double]
     L.tt_set_column_table.argtypes = [ctypes.c_void_p]
     L.tt_set_viscosity.argtypes = [ctypes.POINTER(P.Viscosity)]
     L.tt_get_viscosity.argtypes = [ctypes.POINTER(P.Viscosity)]
@@ -212,6 +213,19 @@ class Device:
             raise ValueError("Burrows tables must hold 50 x 50 values")
         self._check(self.L.tt_set_burrows_table(*(a.ctypes.data_as(ctypes.c_void_p) for a in t)),
                     "tt_set_burrows_table")
+
+    def set_burrows_wave_table(self, kap_wave, bstar, dilution):
+        """IKAPTYP 20-25: log10 of the 30 irradiation bins on the Burrows grid
+        (T fastest, then rho, then bin), BSTAR_BIN and the dilution
+        (``ttcode_amd.opacity.read_burrows_wave_table``, ``read_bstar``,
+        ``wave_dilution``)."""
+        kw = np.ascontiguousarray(np.asarray(kap_wave, np.float64).reshape(-1))
+        bs = np.ascontiguousarray(np.asarray(bstar, np.float64).reshape(-1))
+        if kw.size != 50 * 50 * 30 or bs.size != 30:
+            raise ValueError("the wavelength table holds 50 x 50 x 30 values and BSTAR_BIN 30")
+        self._check(self.L.tt_set_burrows_wave_table(kw.ctypes.data_as(ctypes.c_void_p),
+                                                     bs.ctypes.data_as(ctypes.c_void_p), float(dilution)),
+                    "tt_set_burrows_wave_table")
 
     def set_viscosity(self, ivis=1, ivistyp=1, ieta=4, idiss=1, xnue=1e9, iartvis=0):
         """ETACALC + VISC inside FORGAS3D (ppfor.f:170-174), ivis 0 is off;
