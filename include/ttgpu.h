@@ -279,6 +279,29 @@ int tt_coupled_energy(double delt, int iii, tt_energy_stats* st);
  * returns 2 when rho, T or PG is non-positive). */
 int tt_schritt(double* delt);
 
+/* SCHRITT's Courant bound on caller-owned device arrays, allocating nothing.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc) and hold their own device state; it needs no tt_init.  rh, t,
+ * pg, v, g, h: device arrays over the ghosted box (-1:nx+2, -1:ny+2,
+ * -1:nz+2), column-major (r fastest), V, G and H with their unused low ghost
+ * layer present; dxa, dya, dza (declared 1:n+2), geoxg, geoxh (-1:nx+2),
+ * geozg (-1:nz+2): device arrays of the GITTER metric (tt_download_metric
+ * gives them).  Computes SOUND's CS = sqrt(gamma PG / RH) in place and
+ * writes to device memory out[0] = FDELT min(DTMAX, min over the interior of
+ * the three Courant terms) (ppsub.f:49-78,129-131; IVIS = 0, IARTVIS = 0) and
+ * out[1] = 1 where the reference stops on a non-positive RH, T or PG
+ * (ppsub.f:26-46), 2 where the bound is <= 0, else 0.  Work is queued on
+ * `stream` (a hipStream_t; null: the null stream); the caller synchronises.
+ * The reduction's block partials live in a fixed 4 x 1024-double array in
+ * device memory (32 KB per GPU, held until the process ends); calls that
+ * overlap on one GPU share it and race.  Returns 1 for nx, ny or nz < 1, a
+ * box of 2^31 cells or more, or a null pointer; 0 when queued. */
+int tt_schritt_noalloc(int nx, int ny, int nz, const double* rh, const double* t, const double* pg,
+                       const double* v, const double* g, const double* h, const double* dxa, const double* dya,
+                       const double* dza, const double* geoxg, const double* geoxh, const double* geozg,
+                       double gamma, double fdelt, double dtmax, double* out, void* stream);
+
 /* DIFFUSION (ppdif.f:1-167) applied to T with constant diffusivity
  * xnue/pr_num (the IDIF=2 call of pp.f:765-770); returns SOR iterations in
  * *iters.  ITMIN=5, SORITMAX=500 as in ppdif.f:14-15. */

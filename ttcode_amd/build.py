@@ -27,7 +27,8 @@ FLAGS = [
     "-I", os.path.join(HERE, "..", "include"),
 ]
 
-SOURCES = ["tt_state.hip", "tt_sor.hip", "tt_phys.hip", "tt_energy.hip", "tt_hydro.hip", "tt_cnn.hip", "tt_visc.hip"]
+SOURCES = ["tt_state.hip", "tt_sor.hip", "tt_phys.hip", "tt_energy.hip", "tt_hydro.hip", "tt_cnn.hip", "tt_visc.hip",
+           "tt_noalloc.hip"]
 
 
 def _compile(src: str) -> str:
