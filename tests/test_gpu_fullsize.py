@@ -126,3 +126,56 @@ def test_mixed_step_vs_fp64_step(setup):
         e = C.rel_err(out[1][k][reg], out[0][k][reg])
         print(f"{k}: mixed vs FP64 max rel err {e:.2e}")
         assert e <= 1e-12, (k, e)
+
+
+CNN_WEIGHTS = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "ttcode_amd", "data",
+                           "cnn_guess.npz")
+
+
+def test_learned_guess_config_c4(setup):
+    """The learned-guess configuration at 128x128x256: three steps with the
+    shipped network, FP32 sweeps in FP64 defect correction.  T and ER within
+    1e-12 of the plain FP64 steps (the FP64 solve stops at its during-sweep
+    ratio 1e-10, the mixed one at the true residual 1e-10; measured 4e-14);
+    the network runs on every first solve (fewer than three rejections in a
+    row) and its guess is either kept or replaced by the cold start; the
+    re-solved last system meets the true FP64 residual ratio 1e-10."""
+    from ttcode_amd import cnn_train, ttgpu
+    p, st = setup
+    out = {}
+    for mode in ("fp64", "learned"):
+        d, dt = _dt(ttgpu, p, st)
+        try:
+            d.load_state(st)
+            if mode == "learned":
+                cnn_train.load(d, CNN_WEIGHTS)
+                d.set_solver(precision=1, guess=2)
+            for s in range(3):
+                sg, rc = d.step(dt, 2 + s)
+                e = sg.energy
+                print(f"{mode} step {s}: Newton {e.energy_iter}, SOR {e.sor_iter_total}, FP32 {e.fp32_sweeps}, "
+                      f"learned used {e.learned_used}, fallbacks {e.fallbacks}, skipped {e.learned_skipped}, "
+                      f"first start {e.first_start}", flush=True)
+                if mode == "learned":
+                    assert e.learned_skipped == 0
+                    assert e.first_start in (0, 2)
+                    assert (e.first_start == 2) == (e.learned_used >= 1)
+                    assert e.first_start == 2 or e.fallbacks >= 1
+            out[mode] = {k: d.download(k) for k in ("T", "ER")}
+            if mode == "learned":
+                d.set_solver(precision=1, guess=0)
+                ss, _ = d.solve_sor(False)
+                names = ("SOR_A", "SOR_B", "SOR_C", "SOR_D", "SOR_E", "SOR_F", "SOR_G", "RHS_ER")
+                A, B, Cc, D, E, F, G, R = (d.download(k) for k in names)
+                ratio = _residual_ratio(A, B, Cc, D, E, F, G, R, d.download("delta_ER"))
+                print(f"re-solved last system: true residual ratio {ratio:.3e}")
+                assert ratio <= p.epsmat, ratio
+        finally:
+            if mode == "learned":
+                d.cnn_release()
+            d.close()
+    for k in ("T", "ER"):
+        reg = C.compare_region(k, NX, NY, NZ)
+        e = C.rel_err(out["learned"][k][reg], out["fp64"][k][reg])
+        print(f"{k}: learned + FP32 vs FP64 max rel err {e:.2e}")
+        assert e <= 1e-12, (k, e)
