@@ -89,6 +89,7 @@ struct Lib {
     double* f[TT_FIELD_MAX] = {};   // ghosted device arrays, by field id
     int32_t* ub = nullptr;          // upperbnd (-1:ny+2,-1:nz+2)
     double* mustar = nullptr;
+    double* sincoef = nullptr;      // per (j,k) column: SBCONST Tinc^4 / mu* of the grey irradiation
     double* met = nullptr;          // device metric buffer
     double* hmet = nullptr;         // host copy
     int metlen = 0;

@@ -346,64 +346,79 @@ __global__ void k_bur_cell111(Geo g, const double* __restrict__ T, const double*
 }
 
 
-// CALCSTELLARTAU (pprad.f:191-221): tau* by a reverse scan along r, one
-// thread per (j,k) column, summed in the reference's order; the products
-// are loaded sixteen cells at a time so the loads overlap.
-__global__ void k_stau(Geo g, Met m, const int32_t* __restrict__ ub, const double* __restrict__ XKA,
-                       const double* __restrict__ RH, double* STAU) {
-    int nj = g.ny + 4, nk = g.nz + 4;
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nj * nk) return;
-    int j = t % nj - 1, k = t / nj - 1;
-    int u = ub[g.idx2(j, k)];
+// The grey irradiation, one 64-lane wavefront per (j, k) column (ghost
+// columns included): CALCSTELLARTAU (pprad.f:191-221), STELLAR_DTAU
+// (:223-242) and STELLARINPUT_HEATING's grey branch (:490-502).
+//  1. lanes over the cells load xkapA and RH (coalesced along r) and form
+//     (xkapA RH) DR into LDS;
+//  2. one lane sums them from upperbnd down in the reference's order, tau =
+//     tau + (xkapA RH) DR, sixteen LDS values per batch so the reads overlap;
+//     above upperbnd tau is zeroed when upperbnd < nx, else STELLARTAU(nx+1)
+//     keeps its value, as the reference's zeroing covers upperbnd+1..nx+1;
+//  3. lanes over the cells write STELLARTAU, DTAUS = (tau(i-1) - tau(i+1)) /
+//     2 with DTAUS(1) = DTAUS(2) and DTAUS(nx+1) = DTAUS(nx), and
+//     STELLARINPUT = SBCONST (Tinc^4 / mu*) exp(-tau / mu*) DTAUS / dxb for
+//     i = 1..nx (the reference's i = nx+1, nx+2 land in the next column and
+//     are overwritten there); the column's SBCONST (Tinc^4 / mu*) comes from
+//     k_sin_coef.
+// CALC_TINCIDENT (F_INCIDENT 2, pprad.f:321-330) and the column factor
+// SBCONST (Tinc^4 / mu*) of the grey STELLARINPUT, once per (j, k) column
+__global__ void k_sin_coef(Geo g, Met m, double tirr, const double* __restrict__ MU, double* COEF) {
+    const int nj = g.ny + 4, ncol = nj * (g.nz + 4);
+    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= ncol) return;
+    const int j = col % nj - 1, k = col / nj - 1;
+    const double angle = cos(m(TT_M_XYB, j)) * cos(m(TT_M_XZB, k));
+    double tinc = 0.0;
+    if (angle >= 0.0) {
+        tinc = tirr * pow(angle, 0.25);
+        tinc = tinc > 0.0 ? tinc : 0.0;
+    }
+    COEF[col] = SBCONST * (pow(tinc, 4.0) / MU[g.idx2(j, k)]);
+}
+
+__global__ __launch_bounds__(64) void k_stau_sin(Geo g, Met m, const int32_t* __restrict__ ub,
+                                                 const double* __restrict__ XKA, const double* __restrict__ RH,
+                                                 const double* __restrict__ MU, const double* __restrict__ COEF,
+                                                 double* STAU, double* DTAUS, double* SIN) {
+    extern __shared__ double tau[];  // tau[i], i = 0..nx+1
+    const int nj = g.ny + 4;
+    const int j = (int)(blockIdx.x % nj) - 1, k = (int)(blockIdx.x / nj) - 1;
+    const int lane = threadIdx.x;
+    const int u = ub[g.idx2(j, k)];
     const double dr = m(TT_M_XXB, 2) - m(TT_M_XXB, 1);
-    if (u < g.nx)
-        for (int i = u + 1; i <= g.nx + 1; ++i) STAU[IX(i, j, k)] = 0.0;
-    double tot = 0.0;
-    for (int i0 = u; i0 >= 1; i0 -= 16) {
-        double ka[16], rh[16];
+    for (int i = lane; i <= g.nx + 1; i += 64) {
+        double v = 0.0;
+        if (i >= 1 && i <= u) v = XKA[IX(i, j, k)] * RH[IX(i, j, k)] * dr;
+        else if (i == g.nx + 1 && u >= g.nx) v = STAU[IX(i, j, k)];
+        tau[i] = v;
+    }
+    __syncthreads();
+    if (lane == 0) {
+        double tot = 0.0;
+        for (int i0 = u; i0 >= 1; i0 -= 16) {
+            double v[16];
 #pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            int i = i0 - q;
-            ka[q] = i >= 1 ? XKA[IX(i, j, k)] : 0.0;
-            rh[q] = i >= 1 ? RH[IX(i, j, k)] : 0.0;
-        }
+            for (int q = 0; q < 16; ++q) v[q] = i0 - q >= 1 ? tau[i0 - q] : 0.0;
 #pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            int i = i0 - q;
-            if (i >= 1) {
-                tot = tot + (ka[q] * rh[q] * dr);
-                STAU[IX(i, j, k)] = tot;
+            for (int q = 0; q < 16; ++q) {
+                if (i0 - q >= 1) {
+                    tot = tot + v[q];
+                    tau[i0 - q] = tot;
+                }
             }
         }
     }
-}
-
-// STELLAR_DTAU + STELLARINPUT_HEATING grey branch (pprad.f:223-242,
-// 408-516), per cell i = 1..nx+1 of every column incl. ghosts
-__global__ void k_sin(Geo g, Met m, tt_params p, const double* __restrict__ STAU, const double* __restrict__ MU,
-                      double* DTAUS, double* SIN) {
-    int nj = g.ny + 4, nk = g.nz + 4;
-    long long n = (long long)(g.nx + 1) * nj * nk;
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    int i = (int)(t % (g.nx + 1)) + 1;
-    int col = (int)(t / (g.nx + 1));
-    int j = col % nj - 1, k = col / nj - 1;
-    // DTAUS(1) = DTAUS(2) and DTAUS(nx+1) = DTAUS(nx)
-    int ic = i == 1 ? 2 : (i == g.nx + 1 ? g.nx : i);
-    double dt = (STAU[IX(ic - 1, j, k)] - STAU[IX(ic + 1, j, k)]) / 2.0;
-    DTAUS[IX(i, j, k)] = dt;
-    if (i > g.nx) return;
-    double angle = cos(m(TT_M_XYB, j)) * cos(m(TT_M_XZB, k));
-    double tinc = 0.0;
-    if (angle >= 0.0) {
-        tinc = p.tirr * pow(angle, 0.25);
-        tinc = tinc > 0.0 ? tinc : 0.0;
+    __syncthreads();
+    const double mu = MU[g.idx2(j, k)];
+    const double coef = COEF[blockIdx.x];
+    for (int i = 1 + lane; i <= g.nx + 1; i += 64) {
+        if (i <= u || u < g.nx) STAU[IX(i, j, k)] = tau[i];
+        const int ic = i == 1 ? 2 : (i == g.nx + 1 ? g.nx : i);
+        const double dt = (tau[ic - 1] - tau[ic + 1]) / 2.0;
+        DTAUS[IX(i, j, k)] = dt;
+        if (i <= g.nx) SIN[IX(i, j, k)] = coef * (exp(-tau[i] / mu)) * dt / m(TT_M_DXB, i);
     }
-    double mu = MU[g.idx2(j, k)];
-    double t4 = pow(tinc, 4.0);
-    SIN[IX(i, j, k)] = SBCONST * (t4 / mu) * (exp(-STAU[IX(i, j, k)] / mu)) * dt / m(TT_M_DXB, i);
 }
 
 // The wavelength-binned STELLARINPUT_HEATING (pprad.f:429-447) for IKAPTYP
@@ -1043,11 +1058,13 @@ static int stellar_impl() {
         TT_LAUNCH();
         return 0;
     }
-    k_stau<<<nblk(ncol, 64), 64, 0, L.s>>>(L.g, met(), L.ub, f[TT_XKAPA], f[TT_RH], f[TT_STELLARTAU]);
+    const size_t lds = (size_t)(L.g.nx + 2) * sizeof(double);
+    if (lds > 64 * 1024) return fail(1, "STELLARINPUT_HEATING: nx too large for the column kernel");
+    double* coef = L.sincoef;  // mu_star may have been uploaded since the last call: formed each call
+    k_sin_coef<<<nblk(ncol, 256), 256, 0, L.s>>>(L.g, met(), L.p.tirr, L.mustar, coef);
     TT_LAUNCH();
-    long long ncell = (long long)(L.g.nx + 1) * ncol;
-    k_sin<<<nblk(ncell, 256), 256, 0, L.s>>>(L.g, met(), L.p, f[TT_STELLARTAU], L.mustar, f[TT_DTAUS],
-                                             f[TT_STELLARINPUT]);
+    k_stau_sin<<<ncol, 64, lds, L.s>>>(L.g, met(), L.ub, f[TT_XKAPA], f[TT_RH], L.mustar, coef, f[TT_STELLARTAU],
+                                       f[TT_DTAUS], f[TT_STELLARINPUT]);
     TT_LAUNCH();
     return 0;
 }

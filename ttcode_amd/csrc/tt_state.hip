@@ -194,7 +194,7 @@ static void free_all() {
         L.f[f] = nullptr;
     }
     double** tmp[] = {&L.ED,  &L.VD,      &L.GDN,    &L.HDN, &L.FLMX,   &L.FLMY, &L.FLMZ, &L.TOLD, &L.VH,
-                      &L.GH,  &L.HH,      &L.DFQ,    &L.DE1, &L.partial, &L.mustar, &L.met, &L.ftab, &L.btab, &L.ctab, &L.wtab, &g_stage};
+                      &L.GH,  &L.HH,      &L.DFQ,    &L.DE1, &L.partial, &L.mustar, &L.sincoef, &L.met, &L.ftab, &L.btab, &L.ctab, &L.wtab, &g_stage};
     for (auto p : tmp) {
         if (*p) hipFree(*p);
         *p = nullptr;
@@ -305,6 +305,7 @@ int tt_init(const tt_params* p) {
     TT_HIP(dev_alloc(&L.ub, (size_t)g.gy * g.gz * sizeof(int32_t)));
     TT_HIP(hipMemsetAsync(L.ub, 0, (size_t)g.gy * g.gz * sizeof(int32_t), L.s));
     TT_HIP(dev_alloc(&L.mustar, (size_t)g.gy * g.gz * sizeof(double)));
+    TT_HIP(dev_alloc(&L.sincoef, (size_t)g.gy * g.gz * sizeof(double)));
     TT_HIP(dev_alloc(&L.ftab, (size_t)6 * TT_FREEDMAN_NT * TT_FREEDMAN_NP * sizeof(double)));
     TT_HIP(dev_alloc(&L.ctab, (size_t)16 * TT_FREEDMAN_NT * TT_FREEDMAN_NP * sizeof(double)));
     TT_HIP(dev_alloc(&L.btab, (size_t)2 * (3 * TT_BURROWS_NT * TT_BURROWS_NR + TT_BURROWS_NT) * sizeof(double)));
