@@ -88,9 +88,9 @@ def adjust_omega(dev, st0, dt, precision):
     return dev.get_sorparam(), time.perf_counter() - t0
 
 
-def run_mode(dev, st0, dt, steps, warmup, precision, guess, event_period=7):
+def run_mode(dev, st0, dt, steps, warmup, precision, guess, event_period=7, inner=0):
     dev.load_state(st0)
-    dev.set_solver(precision=precision, guess=guess, batch=16, inner_tol=1e-5, inner_max=20000)
+    dev.set_solver(precision=precision, guess=guess, batch=16, inner_tol=1e-5, inner_max=20000, inner=inner)
     iii = 2
     for _ in range(warmup):
         dev.step(dt, iii)
@@ -166,8 +166,9 @@ def run_large(a):
     dt = dev.schritt()
     dev.load_state(st0)
     cnn_train.load(dev, CNN_WEIGHTS)
-    dev.set_solver(precision=1, guess=2, batch=16, inner_tol=1e-5, inner_max=20000)
-    print(f"# large workload {nx}x{ny}x{nz} dt={dt:.4g}s", file=sys.stderr, flush=True)
+    inner = 1 if a.inner == "line" else 0
+    dev.set_solver(precision=1, guess=2, batch=16, inner_tol=1e-5, inner_max=20000, inner=inner)
+    print(f"# large workload {nx}x{ny}x{nz} dt={dt:.4g}s, {a.inner} sweeps", file=sys.stderr, flush=True)
     iii = 2
     for _ in range(a.warmup):
         dev.step(dt, iii)
@@ -202,8 +203,8 @@ def run_large(a):
         "warmup": a.warmup, "ms_per_step": round(ms, 3), "higher_is_better": False, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32 sweeps + f64", "data": "synthetic (generator + 5% day-night contrast)",
         "config": {"workload": f"sph{nx}x{nz}x{ny}: n_r={nx} n_theta={nz} n_phi={ny}, learned guess + FP32 "
-                               f"SOR in FP64 defect correction to 1e-10, fixed CFL=0.4 dt",
-                   "nr": nx, "ntheta": nz, "nphi": ny, "dt_s": dt},
+                               f"{a.inner} SOR in FP64 defect correction to 1e-10, fixed CFL=0.4 dt",
+                   "nr": nx, "ntheta": nz, "nphi": ny, "dt_s": dt, "inner_sweeps": a.inner},
         "cells": nx * ny * nz, "cell_updates_per_s": nx * ny * nz / (ms * 1e-3),
         "fp32_sweeps_per_step": sweeps / a.steps,
         # the network runs on first Newton solves only; each run's guess is
@@ -342,6 +343,8 @@ def main():
                          "(learned guess + FP32 SOR, 5%% day-night contrast); kappa_table: KAPPA with "
                          "table opacities (IKAPTYP 4) at 128x128x256; forgas_visc: FORGAS3D with viscosity "
                          "at 128x128x256")
+    ap.add_argument("--inner", choices=("line", "point"), default="line",
+                    help="--config large: FP32 inner sweeps, radial-line or point red-black SOR")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy 1 GiB device to device (reference for rocprofv3 byte counters)")
     a = ap.parse_args()
@@ -388,10 +391,13 @@ def main():
     if not a.no_mixed:
         modes["fp32_cold"] = run_mode(dev, st0, dt, a.steps, a.warmup, 1, 0, a.event_period)
         modes["fp32_warm"] = run_mode(dev, st0, dt, a.steps, a.warmup, 1, 1, a.event_period)
+        modes["line_cold"] = run_mode(dev, st0, dt, a.steps, a.warmup, 1, 0, a.event_period, inner=1)
+        modes["line_warm"] = run_mode(dev, st0, dt, a.steps, a.warmup, 1, 1, a.event_period, inner=1)
         if os.path.exists(CNN_WEIGHTS):
             from ttcode_amd import cnn_train
             cnn_train.load(dev, CNN_WEIGHTS)
             modes["fp32_learned"] = run_mode(dev, st0, dt, a.steps, a.warmup, 1, 2, a.event_period)
+            modes["line_learned"] = run_mode(dev, st0, dt, a.steps, a.warmup, 1, 2, a.event_period, inner=1)
             dev.cnn_release()
         for k, v in modes.items():
             print(f"# {k}: { {q: w for q, w in v.items() if q != 'final'} }", file=sys.stderr, flush=True)
@@ -487,6 +493,20 @@ def main():
                                    "learned_skipped": m["learned_skipped"],
                                    "guess_scale": round(m["guess_scale_first"], 4),
                                    "guess_r0_over_cold": round(m["guess_r0_first"], 4)}
+        line_method = ("FP32 red-black radial-line SOR (each r column solved exactly as a tridiagonal system, "
+                       "columns coloured by (j+k) parity, omega 1.1) inside FP64 defect correction to the same "
+                       "1e-10 FP64 residual")
+        out["fp32_line"] = {"method": line_method + "; cold start", **summary(modes["line_cold"])}
+        w = modes["line_warm"]
+        out["fp32_line_warm"] = {"method": line_method + "; warm start as fp32_warm", **summary(w),
+                                 "guess_r0_over_cold": round(w["guess_r0_first"], 4)}
+        if "line_learned" in modes:
+            m = modes["line_learned"]
+            out["learned_fp32_line"] = {"method": line_method + "; initial guess from the CNN as learned_fp32",
+                                        **summary(m), "learned_used": m["learned_used"],
+                                        "fallbacks": m["fallbacks"], "learned_skipped": m["learned_skipped"],
+                                        "guess_scale": round(m["guess_scale_first"], 4),
+                                        "guess_r0_over_cold": round(m["guess_r0_first"], 4)}
     if adj:
         # speedups against the headline (omega = 1.7, speedup_vs_fp64) and
         # against FP64 at the omega its own search chose

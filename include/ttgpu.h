@@ -185,6 +185,13 @@ int tt_metric_info(int metric, int32_t* lo, int32_t* hi);
  * iterations on recent first solves -- a moving average with weight 1/4 --
  * trying the other on every 16th); a warm or learned guess is kept only if its
  * FP64 residual does not exceed the cold start's.
+ * inner (precision 1 only) picks the FP32 sweeps of the inner solves: 0 =
+ * point red-black SOR with omega = SORPARAM, the reference's own sweep in
+ * FP32; 1 = red-black radial-line SOR with omega = line_omega: each radial
+ * column (j,k) is solved exactly as a tridiagonal system in r, the columns
+ * coloured by (j+k) parity, so the strong radial couplings of the FLD
+ * operator are inside the line solves and the sweep count to the same FP64
+ * tolerance falls by more than an order of magnitude.
  * ------------------------------------------------------------------------ */
 typedef struct tt_solver_opts {
     int32_t precision;       /* 0 = FP64 SOR, 1 = FP32 SOR + FP64 correction */
@@ -194,9 +201,10 @@ typedef struct tt_solver_opts {
     int32_t adjust_omega;    /* 1: ADJUST_SOR_PARAM at the first Newton
                                 iteration of steps III = 1 and III % 10000 = 0
                                 (ppenergy.f:372-374); 0 (default): omega fixed */
-    int32_t pad0;
+    int32_t inner;           /* precision 1: 0 point sweeps, 1 radial lines  */
     double inner_tol;        /* FP32 inner L1 ratio target (precision 1)     */
     double warm_scale;       /* scale of the previous delta_ER for guess 1   */
+    double line_omega;       /* over-relaxation of the line sweeps (0, 2)    */
 } tt_solver_opts;
 
 typedef struct tt_sor_stats {

@@ -88,13 +88,13 @@ def _residual_ratio(A, B, Cc, D, E, F, G, R, x):
     return float(np.sum(np.abs(R - ax)) / np.sum(np.abs(R)))
 
 
-@pytest.mark.parametrize("precision", [0, 1])
-def test_sor_solution_satisfies_its_system(setup, precision):
+@pytest.mark.parametrize("precision,inner", [(0, 0), (1, 0), (1, 1)], ids=["fp64", "fp32_point", "fp32_line"])
+def test_sor_solution_satisfies_its_system(setup, precision, inner):
     from ttcode_amd import ttgpu
     p, st = setup
     d, dt = _dt(ttgpu, p, st)
     d.load_state(st)
-    d.set_solver(precision=precision)
+    d.set_solver(precision=precision, inner=inner)
     d.coupled_energy(dt, 2)                  # leaves the last Newton system on the device
     ss, _ = d.solve_sor(False)               # solve it again from a cold start
     names = ("SOR_A", "SOR_B", "SOR_C", "SOR_D", "SOR_E", "SOR_F", "SOR_G", "RHS_ER")
@@ -102,8 +102,8 @@ def test_sor_solution_satisfies_its_system(setup, precision):
     x = d.download("delta_ER")
     d.close()
     ratio = _residual_ratio(A, B, Cc, D, E, F, G, R, x)
-    print(f"precision {precision}: iterations {ss.sor_iter}, reported ratio {ss.rnorm / ss.rhsnorm:.3e}, "
-          f"numpy true residual ratio {ratio:.3e}")
+    print(f"precision {precision} inner {inner}: iterations {ss.sor_iter}, outer {ss.outer_iter}, "
+          f"reported ratio {ss.rnorm / ss.rhsnorm:.3e}, numpy true residual ratio {ratio:.3e}")
     assert ss.rnorm / ss.rhsnorm <= p.epsmat
     # the reported FP64 path ratio is the during-sweep norm; the true
     # residual after the last sweep is of the same size
@@ -111,21 +111,28 @@ def test_sor_solution_satisfies_its_system(setup, precision):
 
 
 def test_mixed_step_vs_fp64_step(setup):
+    """One step with FP32 point sweeps and one with FP32 radial-line sweeps,
+    each inside FP64 defect correction to the same 1e-10 residual, against the
+    FP64 step; the line solver needs far fewer iterations."""
     from ttcode_amd import ttgpu
     p, st = setup
-    out = {}
-    for prec in (0, 1):
+    out, sweeps = {}, {}
+    for mode in ((0, 0), (1, 0), (1, 1)):
         d, dt = _dt(ttgpu, p, st)
         d.load_state(st)
-        d.set_solver(precision=prec)
-        d.step(dt, 2)
-        out[prec] = {k: d.download(k) for k in ("T", "ER")}
+        d.set_solver(precision=mode[0], inner=mode[1])
+        ss, _ = d.step(dt, 2)
+        out[mode] = {k: d.download(k) for k in ("T", "ER")}
+        sweeps[mode] = ss.energy.sor_iter_total
         d.close()
-    for k in ("T", "ER"):
-        reg = C.compare_region(k, NX, NY, NZ)
-        e = C.rel_err(out[1][k][reg], out[0][k][reg])
-        print(f"{k}: mixed vs FP64 max rel err {e:.2e}")
-        assert e <= 1e-12, (k, e)
+    print(f"SOR iterations: FP64 {sweeps[(0, 0)]}, FP32 point {sweeps[(1, 0)]}, FP32 line {sweeps[(1, 1)]}")
+    assert sweeps[(1, 1)] * 10 < sweeps[(0, 0)]
+    for mode in ((1, 0), (1, 1)):
+        for k in ("T", "ER"):
+            reg = C.compare_region(k, NX, NY, NZ)
+            e = C.rel_err(out[mode][k][reg], out[(0, 0)][k][reg])
+            print(f"{mode} {k}: vs FP64 max rel err {e:.2e}")
+            assert e <= 1e-12, (mode, k, e)
 
 
 CNN_WEIGHTS = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "ttcode_amd", "data",

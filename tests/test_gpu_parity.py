@@ -845,6 +845,61 @@ def test_mixed_precision_solve(ttgpu):
     d.close()
 
 
+def test_line_sor_solve(ttgpu):
+    """FP32 red-black radial-line sweeps inside FP64 defect correction reach
+    the FP64 residual target, agree with the FP64 solve within it, and take
+    far fewer iterations; the same system solved twice gives the same bits."""
+    p = P.make_params(32, 64, 32, epsmat=1e-10)
+    st = P.generate_state(p, seed=5)
+    d = ttgpu.Device(p)
+    try:
+        d.load_state(st)
+        d.bounds()
+        d.pressg()
+        dt = d.schritt()
+        d.coupled_energy(dt, 2)                 # leaves its last system on the device
+        st64, _ = d.solve_sor(False)
+        x64 = d.download("delta_ER")
+        d.set_solver(precision=1, inner=1)
+        stl, _ = d.solve_sor(False)
+        xl = d.download("delta_ER")
+        stl2, _ = d.solve_sor(False)
+        xl2 = d.download("delta_ER")
+        print(f"FP64 {st64.sor_iter} iterations; line {stl.sor_iter} in {stl.outer_iter} corrections, "
+              f"ratio {stl.rnorm / stl.rhsnorm:.2e}")
+        assert stl.rnorm / stl.rhsnorm <= 1e-10
+        assert stl.outer_iter >= 1 and 0 < stl.fp32_sweeps and stl.sor_iter * 5 < st64.sor_iter
+        assert np.max(np.abs(xl - x64)) <= 1e-6 * np.max(np.abs(x64))
+        assert np.array_equal(xl, xl2) and stl2.sor_iter == stl.sor_iter
+    finally:
+        d.close()
+
+
+def test_line_sor_fixture_system(ttgpu):
+    """The line solver on the reference's SOR fixture system (12x16x12, whose
+    identity rows above upperbnd cut the lines): the FP64 residual target is
+    met and the result agrees with the FP64 solve."""
+    g = C.load("sor_12x16x12")
+    p = C.params_of(g)
+    p.epsmat = 1e-10
+    d = ttgpu.Device(p)
+    try:
+        d.load_state(C.inputs_of(g))
+        for k in ("EVAL1", "EVAL2", "EVAL3", "EVAL4", "EVAL5", "EVAL6", "EVAL7", "EVAL8",
+                  "EVAL_1_6", "RHS_ER", "upperbnd"):
+            d.upload(k, g["in_" + k])
+        st64, _ = d.solve_sor(True)
+        x64 = d.download("delta_ER")
+        d.set_solver(precision=1, inner=1)
+        stl, _ = d.solve_sor(True)
+        xl = d.download("delta_ER")
+        print(f"FP64 {st64.sor_iter}, line {stl.sor_iter} ({stl.outer_iter} corrections)")
+        assert stl.rnorm / stl.rhsnorm <= 1e-10
+        assert np.max(np.abs(xl - x64)) <= 1e-6 * np.max(np.abs(x64))
+    finally:
+        d.close()
+
+
 def test_energy_totals_match_host_integral(ttgpu):
     """tt_energy_totals against the same volume integrals in numpy from the
     downloaded fields and the GITTER volume metrics."""

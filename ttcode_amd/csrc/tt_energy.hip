@@ -1139,7 +1139,7 @@ int energy_impl(double delt, int iii, tt_energy_stats* st) {
                     f[TT_FR], f[TT_RHS_ER], L.DFQ, nullptr, nullptr, nullptr, nullptr, 0, L.ub, L.p.f_incident,
                     delt};
         const PackedSys ps = sor_packed();
-        const bool with32 = L.opts.precision == 1;
+        const bool with32 = L.opts.precision == 1 && L.opts.inner == 0;  // line sweeps build their own FP32 system
         ca.P0 = ps.c64[0]; ca.P1 = ps.c64[1]; ca.hx = ps.hx;
         if (with32) { ca.Q0 = ps.c32[0]; ca.Q1 = ps.c32[1]; }
         long long nc = (long long)g.nx * g.ny * g.nz;

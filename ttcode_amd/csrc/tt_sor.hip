@@ -67,6 +67,27 @@ static long long g_bid = 0;  // batches enqueued so far
 
 static long long padded_len() { return (long long)g_cs.px * g_cs.ny * (g_cs.nz + 2); }
 
+// Radial-line inner solves (tt_solver_opts.inner = 1).  Line (j,k) is the
+// whole radial column i = 1..nx; lines are coloured by (j+k) parity, so a
+// line's phi and theta neighbours are lines of the other colour and the
+// lines of one colour are solved independently.  Row r = (j-1) + ny (k-1)
+// of g_lco holds eight FP32 vectors of nx values: the couplings D, E (phi
+// -/+), F, G (theta -/+), then the LU factors of the line's tridiagonal
+// part, m (the multiplier of row i-1) and u (the pivot), the super-diagonal
+// C, and the right-hand side.  The unknowns are rows of nx values over
+// (j, k = 0..nz+1); the theta ghost rows k = 0, nz+1 and the r ghosts stay
+// zero (the correction's boundary values).
+struct LS {
+    int nx, ny, nz;
+};
+static LS g_ls;
+static float* g_lco = nullptr;
+static float* g_lx = nullptr;
+static int g_lchunk = 0;  // lines of a colour per logical block
+static int g_line = 0;    // FP32 sweeps launch the line kernel
+static bool g_line_ok = false;  // g_lco's factors match the FP64 system
+static long long line_x_len() { return (long long)g_ls.nx * g_ls.ny * (g_ls.nz + 2); }
+
 void sor_free() {
     for (int c = 0; c < 2; ++c) {
         if (c64[c]) hipFree(c64[c]);
@@ -78,6 +99,11 @@ void sor_free() {
         x64[c] = nullptr;
         x32[c] = nullptr;
     }
+    if (g_lco) hipFree(g_lco);
+    if (g_lx) hipFree(g_lx);
+    g_lco = nullptr;
+    g_lx = nullptr;
+    g_line_ok = false;
     for (auto e : ev_pool) hipEventDestroy(e);
     for (auto e : ev_pend) hipEventDestroy(e);
     ev_pool.clear();
@@ -124,6 +150,22 @@ int sor_alloc() {
     if (4 * nb > L.npartial || 2 * nb > 16 * 256) return fail(1, "SOR: partial buffer too small");
     g_nb = nb;
     g_chunk = (nrows + nb - 1) / nb;  // rows per block
+    return 0;
+}
+
+// The line system and unknowns, on the first line solve (134 + 17 MB at
+// 128 x 256 x 128), kept to the end.
+static int line_alloc() {
+    if (g_lco) return 0;
+    if (L.g.nx > 512) return fail(1, "line sweeps: nx must be <= 512 (eight cells per lane)");
+    g_ls = LS{L.g.nx, L.g.ny, L.g.nz};
+    const long long ncell = (long long)g_ls.nx * g_ls.ny * g_ls.nz;
+    TT_HIP(dev_alloc(&g_lco, 8 * ncell * sizeof(float)));
+    TT_HIP(dev_alloc(&g_lx, line_x_len() * sizeof(float)));
+    TT_HIP(hipMemsetAsync(g_lx, 0, line_x_len() * sizeof(float), L.s));
+    const int nl = (g_ls.ny / 2) * g_ls.nz;  // lines of a colour
+    g_lchunk = (nl + g_nb - 1) / g_nb;
+    g_line_ok = false;
     return 0;
 }
 
@@ -325,6 +367,182 @@ __global__ __launch_bounds__(256) void k_sweep_f32x2(int c, const float* __restr
     if (threadIdx.x == 0 && !done) part[c * nb + lb] = bs;
 }
 
+// ------------------------------------------------------ radial-line sweeps
+// One half sweep of red-black line SOR: every line of colour c (lines
+// (j,k) with (j+k) parity c) is solved exactly for the current values of its
+// four neighbour lines,
+//   T y = f,  f = b - D x(j-1) - E x(j+1) - F x(k-1) - G x(k+1),
+// with T the line's tridiagonal part (B, A, C) in its LU form (m, u, C):
+// z_i = f_i - m_i z_{i-1}, y_i = (z_i - C_i y_{i+1}) / u_i; then
+// x <- x + omega (y - x).  One wavefront per line: lane l holds the Q cells
+// l Q .. l Q + Q - 1 (0-based i - 1), so both recurrences are affine maps a
+// lane composes over its own cells and the wavefront combines with a
+// 6-step shuffle scan (forward over lanes, then backward).  The residual of
+// the line before its update, f - L U x, is summed in |.| for the
+// convergence test, as the point sweep sums its residuals.  Cells past nx
+// act as identity rows with zero right-hand side.  Rows, blocks, the test
+// block and the partials are laid out as in k_sweep.
+template <int Q>
+__global__ __launch_bounds__(256) void k_line_sweep(int c, const float* __restrict__ coef, float* __restrict__ x, LS ls,
+                                                    float omega, double* part, int nb, int lines_per_block, Ctrl* ctrl,
+                                                    int rev, FinArgs fa) {
+    __shared__ double sh[4];
+    if ((int)blockIdx.x == nb) {
+        finish_block(ctrl, fa.part, fa.nparts, fa.step, sh, 1);
+        return;
+    }
+    const int done = ctrl->done;
+    int lb = xcd_block(nb);
+    if (rev) lb = nb - 1 - lb;
+    const int lane = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const int nx = ls.nx, ny = ls.ny, hy = ny >> 1;
+    const int nl = hy * ls.nz;
+    const int q0 = lb * lines_per_block;
+    const int q1 = min(q0 + lines_per_block, nl);
+    const int i0 = lane * Q;
+    double acc = 0.0;
+    for (int q = done ? q1 : q0 + ty; q < q1; q += 4) {
+        const int kk = q / hy;
+        const int jj = 2 * (q - kk * hy) + ((kk + c) & 1);  // (jj + kk) & 1 == c
+        const int jm = jj == 0 ? ny - 1 : jj - 1;
+        const int jp = jj == ny - 1 ? 0 : jj + 1;
+        const float* __restrict__ cr = coef + (long long)(jj + ny * kk) * 8 * nx + i0;
+        float* __restrict__ xw = x + (long long)(jj + ny * (kk + 1)) * nx + i0;
+        const float* __restrict__ xjm = x + (long long)(jm + ny * (kk + 1)) * nx + i0;
+        const float* __restrict__ xjp = x + (long long)(jp + ny * (kk + 1)) * nx + i0;
+        const float* __restrict__ xkm = x + (long long)(jj + ny * kk) * nx + i0;
+        const float* __restrict__ xkp = x + (long long)(jj + ny * (kk + 2)) * nx + i0;
+        float cD[Q], cE[Q], cF[Q], cG[Q], cM[Q], cU[Q], cC[Q], cB[Q], X[Q], n1[Q], n2[Q], n3[Q], n4[Q];
+#pragma unroll
+        for (int t = 0; t < Q; ++t) {
+            const bool in = i0 + t < nx;
+            cD[t] = in ? cr[t] : 0.f;
+            cE[t] = in ? cr[nx + t] : 0.f;
+            cF[t] = in ? cr[2 * nx + t] : 0.f;
+            cG[t] = in ? cr[3 * nx + t] : 0.f;
+            cM[t] = in ? cr[4 * nx + t] : 0.f;
+            cU[t] = in ? cr[5 * nx + t] : 1.f;
+            cC[t] = in ? cr[6 * nx + t] : 0.f;
+            cB[t] = in ? cr[7 * nx + t] : 0.f;
+            X[t] = in ? xw[t] : 0.f;
+            n1[t] = in ? xjm[t] : 0.f;
+            n2[t] = in ? xjp[t] : 0.f;
+            n3[t] = in ? xkm[t] : 0.f;
+            n4[t] = in ? xkp[t] : 0.f;
+        }
+        float f[Q], g[Q];
+#pragma unroll
+        for (int t = 0; t < Q; ++t) f[t] = cB[t] - cD[t] * n1[t] - cE[t] * n2[t] - cF[t] * n3[t] - cG[t] * n4[t];
+        // residual before the update: f - L (U x), (U x)_i = u_i x_i + C_i x_{i+1},
+        // (L w)_i = w_i + m_i w_{i-1}; the r ghosts are zero
+        float xnext = __shfl_down(X[0], 1);
+        if (lane == 63) xnext = 0.f;
+#pragma unroll
+        for (int t = 0; t < Q; ++t) g[t] = cU[t] * X[t] + cC[t] * (t + 1 < Q ? X[t + 1] : xnext);
+        float gprev = __shfl_up(g[Q - 1], 1);
+        if (lane == 0) gprev = 0.f;
+        float rs = 0.f;
+#pragma unroll
+        for (int t = 0; t < Q; ++t) rs += fabsf(f[t] - (g[t] + cM[t] * (t ? g[t - 1] : gprev)));
+        acc += (double)rs;
+        // forward substitution z_i = f_i - m_i z_{i-1}: the lane's map
+        // z_out = pa z_in + pb, composed with the lanes before it
+        float pa = 1.f, pb = 0.f;
+#pragma unroll
+        for (int t = 0; t < Q; ++t) {
+            pb = f[t] - cM[t] * pb;
+            pa = -cM[t] * pa;
+        }
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const float ua = __shfl_up(pa, d), ub = __shfl_up(pb, d);
+            if (lane >= d) {
+                pb = pa * ub + pb;
+                pa = pa * ua;
+            }
+        }
+        float zin = __shfl_up(pb, 1);
+        if (lane == 0) zin = 0.f;
+        float ca[Q], cb[Q];
+#pragma unroll
+        for (int t = 0; t < Q; ++t) {
+            zin = f[t] - cM[t] * zin;
+            const float w = __builtin_amdgcn_rcpf(cU[t]);
+            ca[t] = -cC[t] * w;
+            cb[t] = zin * w;
+        }
+        // back substitution y_i = ca_i y_{i+1} + cb_i: the lane's map
+        // y_first = qa y_in + qb, composed with the lanes after it
+        float qa = 1.f, qb = 0.f;
+#pragma unroll
+        for (int t = Q - 1; t >= 0; --t) {
+            qb = ca[t] * qb + cb[t];
+            qa = ca[t] * qa;
+        }
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const float va = __shfl_down(qa, d), vb = __shfl_down(qb, d);
+            if (lane + d < 64) {
+                qb = qa * vb + qb;
+                qa = qa * va;
+            }
+        }
+        float yin = __shfl_down(qb, 1);
+        if (lane == 63) yin = 0.f;
+#pragma unroll
+        for (int t = Q - 1; t >= 0; --t) {
+            yin = ca[t] * yin + cb[t];
+            if (i0 + t < nx) xw[t] = X[t] + omega * (yin - X[t]);
+        }
+    }
+    double bs = block_sum256(acc, sh);
+    if (threadIdx.x == 0 && !done) part[c * nb + lb] = bs;
+}
+
+// LU factors of every line's tridiagonal part from the FP64 colour-split
+// system: u_1 = A_1, m_i = B_i / u_{i-1}, u_i = A_i - m_i C_{i-1} (FP64,
+// stored FP32).  B_1 and C_nx couple to the r ghosts, whose corrections are
+// zero, and drop out.  One thread per line, i ascending.
+__global__ void k_line_factor(CS cs, LS ls, const double* __restrict__ C0, const double* __restrict__ C1,
+                              float* __restrict__ lco) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= ls.ny * ls.nz) return;
+    const int kk = r / ls.ny, jj = r - kk * ls.ny;
+    const int hx = cs.hx, nx = ls.nx;
+    const double* __restrict__ b0 = C0 + (long long)r * 8 * hx;
+    const double* __restrict__ b1 = C1 + (long long)r * 8 * hx;
+    float* __restrict__ o = lco + (long long)r * 8 * nx;
+    double up = 1.0, cp = 0.0;
+    for (int i = 1; i <= nx; ++i) {
+        const double* __restrict__ bb = (((i + jj + kk) & 1) ? b1 : b0) + ((i - 1) >> 1);  // colour (i+j+k)&1
+        const double a = bb[0], b = bb[hx], cc = bb[2 * hx];
+        const double m = i == 1 ? 0.0 : b / up;
+        const double u = a - m * cp;
+        o[4 * nx + i - 1] = (float)m;
+        o[5 * nx + i - 1] = (float)u;
+        o[6 * nx + i - 1] = (float)cc;
+        up = u;
+        cp = cc;
+    }
+}
+
+// The phi and theta couplings D..G of every cell into the line layout (FP32).
+__global__ void k_line_angular(CS cs, LS ls, const double* __restrict__ C0, const double* __restrict__ C1,
+                               float* __restrict__ lco) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int nx = ls.nx;
+    if (t >= (long long)nx * ls.ny * ls.nz) return;
+    const int r = (int)(t / nx), i = (int)(t - (long long)r * nx) + 1;
+    const int kk = r / ls.ny, jj = r - kk * ls.ny;
+    const int hx = cs.hx;
+    const double* __restrict__ bb = (((i + jj + kk) & 1) ? C1 : C0) + (long long)r * 8 * hx + ((i - 1) >> 1);
+    float* __restrict__ o = lco + (long long)r * 8 * nx + i - 1;
+    o[0] = (float)bb[3 * hx];
+    o[nx] = (float)bb[4 * hx];
+    o[2 * nx] = (float)bb[5 * hx];
+    o[3 * nx] = (float)bb[6 * hx];
+}
+
 // ---------------------------------------------------------- matrix fill
 // FILL_SOR_MATRIX (ppsor.f:189-277) + B_VECTOR_BOUNDARIES (ppsor.f:280-329)
 // for one rank without poles: identity rows at i >= upperbnd, i = nx, k = 1,
@@ -455,7 +673,7 @@ int planetshift_impl(double* a, int f) {
 // optionally the residual in FP32 into the RHS slot of the FP32 operator.
 __global__ __launch_bounds__(256) void k_resid64(CS cs, const double* __restrict__ C0, const double* __restrict__ C1,
                                                   const double* x0, const double* x1, float* r32_0, float* r32_1,
-                                                  double* part, int chunk) {
+                                                  float* rline, int nx, double* part, int chunk) {
     __shared__ double sh[4];
     const int lb = blockIdx.x;
     long long start = (long long)lb * chunk, end = start + chunk;
@@ -483,6 +701,7 @@ __global__ __launch_bounds__(256) void k_resid64(CS cs, const double* __restrict
         double res = q[7 * hx] - ax;
         acc += fabs(res);
         if (r32_0) (c ? r32_1 : r32_0)[(long long)rr * 8 * hx + 7 * hx + h] = (float)res;
+        if (rline) rline[((long long)rr * 8 + 7) * nx + 2 * h + s] = (float)res;  // cell i = 2h + 1 + s
     }
     double bs = block_sum256(acc, sh);
     if (threadIdx.x == 0) part[lb] = bs;
@@ -546,6 +765,21 @@ __global__ void k_add_corr(CS cs, double* x, const float* __restrict__ c) {
     x[p] = x[p] + (double)c[p];
 }
 
+// x64 (padded, colour cc) += the line unknowns over the interior cells
+__global__ void k_add_corr_line(CS cs, int cc, int nx, double* x, const float* __restrict__ xl) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cs.ncell) return;
+    int tt = (int)t;
+    int r = tt / cs.hx;
+    int h = tt - r * cs.hx;
+    int kk = r / cs.ny;
+    int jj = r - kk * cs.ny;
+    int s = (jj + kk + cc + 1) & 1;
+    long long row = jj + (long long)cs.ny * (kk + 1);
+    long long p = (long long)cs.px * row + h + 2;
+    x[p] = x[p] + (double)xl[row * nx + 2 * h + s];
+}
+
 __global__ void k_scale(double* a, double s, long long n) {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) a[t] = a[t] * s;
@@ -597,6 +831,43 @@ static void ev_drain(const Ctrl* st, long long upto) {
     ev_bid.resize(keep);
 }
 
+template <int Q>
+static void line_go(bool timed, hipEvent_t e0, hipEvent_t e1, int grid, int c, float omega, int rev, double* part,
+                    FinArgs fa) {
+    if (!timed)
+        k_line_sweep<Q><<<grid, 256, 0, L.s>>>(c, g_lco, g_lx, g_ls, omega, part, g_nb, g_lchunk, L.ctrl, rev, fa);
+    else
+        hipExtLaunchKernelGGL(k_line_sweep<Q>, dim3(grid), dim3(256), 0, L.s, e0, e1, 0, c, (const float*)g_lco,
+                              g_lx, g_ls, omega, part, g_nb, g_lchunk, L.ctrl, rev, fa);
+}
+
+// A half sweep of the line solver (same partials, test block and timing as
+// the point sweeps); Q = cells per lane, nx <= 64 Q.
+static int launch_line(int c, double omega, int rev, int iter, double* part, FinArgs fa, int grid, bool timed) {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (timed) {
+        e0 = ev_get();
+        e1 = ev_get();
+    }
+    const float w = (float)omega;
+    const int q = (g_ls.nx + 63) / 64;
+    if (q <= 1) line_go<1>(timed, e0, e1, grid, c, w, rev, part, fa);
+    else if (q == 2) line_go<2>(timed, e0, e1, grid, c, w, rev, part, fa);
+    else if (q == 3) line_go<3>(timed, e0, e1, grid, c, w, rev, part, fa);
+    else if (q == 4) line_go<4>(timed, e0, e1, grid, c, w, rev, part, fa);
+    else if (q <= 6) line_go<6>(timed, e0, e1, grid, c, w, rev, part, fa);
+    else line_go<8>(timed, e0, e1, grid, c, w, rev, part, fa);
+    TT_LAUNCH();
+    if (timed) {
+        ev_pend.push_back(e0);
+        ev_pend.push_back(e1);
+        ev_kind.push_back(32);
+        ev_iter.push_back(iter);
+        ev_bid.push_back(g_bid);
+    }
+    return 0;
+}
+
 // Half sweep c of iteration iter (0-based within the solve), in place.  The
 // partials of iteration iter go to slot iter % 2 (two colours x nb each).
 // fin: the red sweep also runs the convergence test of iteration iter - 1.
@@ -611,6 +882,7 @@ static int launch_sweep(int c, T* coef0, T* coef1, T* xa, T* xb, double omega, i
     const int grid = g_nb + (fin ? 1 : 0);
     const bool timed = L.timing && (L.timing_count++ % L.timing_period) == 0;
     const bool pairs = sizeof(T) == 4 && g_cs.hx % 2 == 0;
+    if (sizeof(T) == 4 && g_line) return launch_line(c, omega, rev, iter, part, fa, grid, timed);
     if (!timed) {
         // the plain launch path: the host's cost per launch bounds how fast
         // queued sweeps that return at once (after convergence) drain
@@ -738,8 +1010,9 @@ static int take_batch(int q, SolveView& v, bool* done) {
 
 template <typename T>
 static int run_sweeps(T* coef0, T* coef1, T* xa, T* xb, double omega, int* iters, double* rnorm,
-                      int check_every, bool deferred, bool speculate = false) {
+                      int check_every, bool deferred, bool speculate = false, int max_batch = 0) {
     int batch = L.opts.batch > 0 ? L.opts.batch : 16;
+    if (max_batch > 0 && batch > max_batch) batch = max_batch;
     batch = ((batch + 3) / 4) * 4;  // a multiple of 4: directions alternate and checks stay aligned
     long long launched = 0;
     int slot = 0, pend = -1, last = 0;
@@ -835,10 +1108,12 @@ PackedSys sor_packed() { return PackedSys{{c64[0], c64[1]}, {c32[0], c32[1]}, g_
 void sor_sys_packed_written(bool with32) {
     g_sys = 1;
     g_c32_ok = with32;
+    g_line_ok = false;
 }
 void sor_sys_module_written() {
     g_sys = 0;
     g_c32_ok = false;
+    g_line_ok = false;
 }
 
 // sum |RHS| over the packed copy, pair by pair in k_pack's order (same
@@ -904,6 +1179,7 @@ int sor_pack_impl() {
                                     f[TT_SOR_F], f[TT_SOR_G], f[TT_RHS_ER], c64[0], c64[1], part);
         g_sys = 2;
         g_c32_ok = false;
+        g_line_ok = false;
     } else {
         k_rhsnorm<<<nb, 256, 0, L.s>>>(g_cs, c64[0], c64[1], part);
     }
@@ -911,15 +1187,17 @@ int sor_pack_impl() {
     return fold_partials(part, nb, 1, 0, &L.ctrl->rhsnorm);
 }
 
-static int resid64(double* out_norm, bool to32) {
+// to: 0 the norm only, 1 also the FP32 residual into the point system's RHS
+// slots, 2 into the line system's
+static int resid64(double* out_norm, int to) {
     int nb = L.npartial;
     long long tot = 2 * g_cs.ncell;
     int chunk = (int)((tot + nb - 1) / nb);
     chunk = ((chunk + 255) / 256) * 256;
     nb = (int)((tot + chunk - 1) / chunk);
     double* part = L.partial + L.npartial;
-    k_resid64<<<nb, 256, 0, L.s>>>(g_cs, c64[0], c64[1], x64[0], x64[1], to32 ? c32[0] : nullptr,
-                                   to32 ? c32[1] : nullptr, part, chunk);
+    k_resid64<<<nb, 256, 0, L.s>>>(g_cs, c64[0], c64[1], x64[0], x64[1], to == 1 ? c32[0] : nullptr,
+                                   to == 1 ? c32[1] : nullptr, to == 2 ? g_lco : nullptr, g_ls.nx, part, chunk);
     TT_LAUNCH();
     int rc = fold_partials(part, nb, 1, 0, &L.ctrl->scratch[0]);
     if (rc) return rc;
@@ -1016,7 +1294,7 @@ int sor_solve_impl(int fill_from_evals, tt_sor_stats* st, int mode) {
                     TT_LAUNCH();
                 }
                 double r0;
-                rc = resid64(&r0, false);
+                rc = resid64(&r0, 0);
                 if (rc) return rc;
                 s.r0_guess = r0;
                 if (r0 <= s.r0_cold) s.guess_used = L.have_prev_guess;  // 1 warm, 2 learned
@@ -1051,15 +1329,28 @@ int sor_solve_impl(int fill_from_evals, tt_sor_stats* st, int mode) {
         s.rnorm = rnorm;
     } else {
         // FP32 SOR inside FP64 defect correction
-        if (!g_c32_ok) {
+        const bool line = L.opts.inner == 1;
+        if (line) {
+            if ((rc = line_alloc())) return rc;
+            if (!g_line_ok) {
+                const int nl = g_ls.ny * g_ls.nz;
+                k_line_factor<<<nblk(nl, 64), 64, 0, L.s>>>(g_cs, g_ls, c64[0], c64[1], g_lco);
+                TT_LAUNCH();
+                const long long nc = (long long)g_ls.nx * nl;
+                k_line_angular<<<nblk(nc, 256), 256, 0, L.s>>>(g_cs, g_ls, c64[0], c64[1], g_lco);
+                TT_LAUNCH();
+                g_line_ok = true;
+            }
+        } else if (!g_c32_ok) {
             for (int c = 0; c < 2; ++c) {
                 k_to_f32<<<nblk(8 * g_cs.ncell, 256), 256, 0, L.s>>>(c64[c], c32[c], 8 * g_cs.ncell);
                 TT_LAUNCH();
             }
             g_c32_ok = true;
         }
+        const int to = line ? 2 : 1;
         double rn;
-        rc = resid64(&rn, true);  // reads the control block: rhsnorm is current
+        rc = resid64(&rn, to);  // reads the control block: rhsnorm is current
         if (rc) return rc;
         const double rhs_norm = L.hctrl->rhsnorm;
         int outer = 0, total32 = 0;
@@ -1067,8 +1358,12 @@ int sor_solve_impl(int fill_from_evals, tt_sor_stats* st, int mode) {
         while (!(rn / rhs_norm <= L.p.epsmat) && outer < max_outer && total32 < L.p.itmax) {
             ++outer;
             // inner: A c = r in FP32 from c = 0, L1 during-sweep ratio vs sum|r|
-            TT_HIP(hipMemsetAsync(x32[0], 0, pl * sizeof(float), L.s));
-            TT_HIP(hipMemsetAsync(x32[1], 0, pl * sizeof(float), L.s));
+            if (line) {
+                TT_HIP(hipMemsetAsync(g_lx, 0, line_x_len() * sizeof(float), L.s));
+            } else {
+                TT_HIP(hipMemsetAsync(x32[0], 0, pl * sizeof(float), L.s));
+                TT_HIP(hipMemsetAsync(x32[1], 0, pl * sizeof(float), L.s));
+            }
             Ctrl c = *L.hctrl;  // current: resid64 has just read it
             c.done = 0; c.it = 0; c.counter = 0; c.counter2 = 0;
             c.itmin = 1;
@@ -1076,9 +1371,10 @@ int sor_solve_impl(int fill_from_evals, tt_sor_stats* st, int mode) {
             // FP32 sweeps stop reducing the residual at their rounding floor:
             // hand over to the FP64 correction when the ratio has not fallen
             // by 1% over 256 iterations
+            // (8 iterations for line sweeps, which gain a decade in a few)
             c.best = 1e300;
             c.since = 0;
-            c.stag = 256;
+            c.stag = line ? 8 : 256;
             // the inner solve only has to take the FP64 residual below the
             // outer target (with a factor 2 margin); inner_tol is the floor,
             // what FP32 sweeps can resolve
@@ -1089,15 +1385,26 @@ int sor_solve_impl(int fill_from_evals, tt_sor_stats* st, int mode) {
             if ((rc = ctrl_put())) return rc;
             int it32 = 0;
             double rn32 = 0;
-            rc = run_sweeps<float>(c32[0], c32[1], x32[0], x32[1], L.p.sorparam, &it32, &rn32, 4, true);
+            if (line) {
+                // every iteration tested (inside the next red sweep), batches of 4
+                g_line = 1;
+                rc = run_sweeps<float>(c32[0], c32[1], x32[0], x32[1], L.opts.line_omega, &it32, &rn32, 1, true,
+                                       false, 4);
+                g_line = 0;
+            } else {
+                rc = run_sweeps<float>(c32[0], c32[1], x32[0], x32[1], L.p.sorparam, &it32, &rn32, 4, true);
+            }
             if (rc) return rc;
             total32 += it32;
             for (int cc = 0; cc < 2; ++cc) {
-                k_add_corr<<<nblk(g_cs.ncell, 256), 256, 0, L.s>>>(g_cs, x64[cc], x32[cc]);
+                if (line)
+                    k_add_corr_line<<<nblk(g_cs.ncell, 256), 256, 0, L.s>>>(g_cs, cc, g_ls.nx, x64[cc], g_lx);
+                else
+                    k_add_corr<<<nblk(g_cs.ncell, 256), 256, 0, L.s>>>(g_cs, x64[cc], x32[cc]);
                 TT_LAUNCH();
             }
             double rn_prev = rn;
-            rc = resid64(&rn, true);
+            rc = resid64(&rn, to);
             if (rc) return rc;
             if (!(rn < rn_prev)) break;  // no progress: report the cap below
         }

@@ -369,6 +369,8 @@ int tt_init(const tt_params* p) {
     L.start_picks = 0;
     L.opts.inner_tol = 1e-5;
     L.opts.warm_scale = 0.9;
+    L.opts.inner = 0;
+    L.opts.line_omega = 1.1;
     TT_HIP(hipStreamSynchronize(L.s));
     L.ready = true;
     return 0;
@@ -405,9 +407,11 @@ int tt_sync(void) {
 int tt_set_solver(const tt_solver_opts* o) {
     if (!o) return fail(1, "null options");
     if (o->precision < 0 || o->precision > 1 || o->guess < 0 || o->guess > 2 || o->batch < 1 ||
-        o->adjust_omega < 0 || o->adjust_omega > 1)
+        o->adjust_omega < 0 || o->adjust_omega > 1 || o->inner < 0 || o->inner > 1)
         return fail(1, "invalid solver options");
-    if (o->guess != L.opts.guess || o->precision != L.opts.precision) {
+    if (o->inner == 1 && !(o->line_omega > 0.0 && o->line_omega < 2.0))
+        return fail(1, "invalid solver options: line_omega must lie in (0, 2)");
+    if (o->guess != L.opts.guess || o->precision != L.opts.precision || o->inner != L.opts.inner) {
         // sweep counts of another mode say nothing about this one
         L.start_avg[0] = L.start_avg[1] = 0.0;
         L.start_picks = 0;

@@ -63,8 +63,9 @@ class SolverOpts(ctypes.Structure):
 
     _fields_ = [("precision", ctypes.c_int32), ("guess", ctypes.c_int32),
                 ("batch", ctypes.c_int32), ("inner_max", ctypes.c_int32),
-                ("adjust_omega", ctypes.c_int32), ("pad0", ctypes.c_int32),
-                ("inner_tol", ctypes.c_double), ("warm_scale", ctypes.c_double)]
+                ("adjust_omega", ctypes.c_int32), ("inner", ctypes.c_int32),
+                ("inner_tol", ctypes.c_double), ("warm_scale", ctypes.c_double),
+                ("line_omega", ctypes.c_double)]
 
 
 class Viscosity(ctypes.Structure):

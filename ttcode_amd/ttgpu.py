@@ -142,8 +142,8 @@ class Device:
 
     # ---- solver options
     def set_solver(self, precision=0, guess=0, batch=16, inner_max=4000, inner_tol=1e-5, warm_scale=0.9,
-                   adjust_omega=0):
-        o = P.SolverOpts(precision, guess, batch, inner_max, adjust_omega, 0, inner_tol, warm_scale)
+                   adjust_omega=0, inner=0, line_omega=1.1):
+        o = P.SolverOpts(precision, guess, batch, inner_max, adjust_omega, inner, inner_tol, warm_scale, line_omega)
         self._check(self.L.tt_set_solver(ctypes.byref(o)), "tt_set_solver")
 
     def adjust_sor_param(self, fill_matrix=False):
