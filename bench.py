@@ -343,6 +343,8 @@ def main():
                          "(learned guess + FP32 SOR, 5%% day-night contrast); kappa_table: KAPPA with "
                          "table opacities (IKAPTYP 4) at 128x128x256; forgas_visc: FORGAS3D with viscosity "
                          "at 128x128x256")
+    ap.add_argument("--modes", default="fp32_cold,fp32_warm,line_cold,line_warm,fp32_learned,line_learned",
+                    help="mixed-precision modes to run after the FP64 headline (comma list)")
     ap.add_argument("--inner", choices=("line", "point"), default="line",
                     help="--config large: FP32 inner sweeps, radial-line or point red-black SOR")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -388,16 +390,18 @@ def main():
     plain = run_mode(dev, st0, dt, a.steps, a.warmup, 0, 0, a.event_period)
     print(f"# fp64: { {k: v for k, v in plain.items() if k != 'final'} }", file=sys.stderr, flush=True)
     modes = {}
+    want = set(a.modes.split(","))
+    spec = {"fp32_cold": (0, 0), "fp32_warm": (1, 0), "line_cold": (0, 1), "line_warm": (1, 1)}
     if not a.no_mixed:
-        modes["fp32_cold"] = run_mode(dev, st0, dt, a.steps, a.warmup, 1, 0, a.event_period)
-        modes["fp32_warm"] = run_mode(dev, st0, dt, a.steps, a.warmup, 1, 1, a.event_period)
-        modes["line_cold"] = run_mode(dev, st0, dt, a.steps, a.warmup, 1, 0, a.event_period, inner=1)
-        modes["line_warm"] = run_mode(dev, st0, dt, a.steps, a.warmup, 1, 1, a.event_period, inner=1)
-        if os.path.exists(CNN_WEIGHTS):
+        for name, (guess, inner) in spec.items():
+            if name in want:
+                modes[name] = run_mode(dev, st0, dt, a.steps, a.warmup, 1, guess, a.event_period, inner=inner)
+        if os.path.exists(CNN_WEIGHTS) and want & {"fp32_learned", "line_learned"}:
             from ttcode_amd import cnn_train
             cnn_train.load(dev, CNN_WEIGHTS)
-            modes["fp32_learned"] = run_mode(dev, st0, dt, a.steps, a.warmup, 1, 2, a.event_period)
-            modes["line_learned"] = run_mode(dev, st0, dt, a.steps, a.warmup, 1, 2, a.event_period, inner=1)
+            for name, inner in (("fp32_learned", 0), ("line_learned", 1)):
+                if name in want:
+                    modes[name] = run_mode(dev, st0, dt, a.steps, a.warmup, 1, 2, a.event_period, inner=inner)
             dev.cnn_release()
         for k, v in modes.items():
             print(f"# {k}: { {q: w for q, w in v.items() if q != 'final'} }", file=sys.stderr, flush=True)
@@ -478,12 +482,14 @@ def main():
                 d["fp32_sweep_GBs"] = round(22.0 * ncell / (l32 * 1e-3) / 1e9, 1)
             return d
     if modes:
-        out["fp32"] = {"method": "FP32 red-black SOR inside FP64 defect correction to the same 1e-10 FP64 "
-                                 "residual; cold start", **summary(modes["fp32_cold"])}
-        w = modes["fp32_warm"]
-        out["fp32_warm"] = {"method": "as fp32, initial guess = previous step's first Newton correction, "
-                                      "scaled to its least-residual multiple", **summary(w),
-                            "guess_r0_over_cold": round(w["guess_r0_first"], 4)}
+        if "fp32_cold" in modes:
+            out["fp32"] = {"method": "FP32 red-black SOR inside FP64 defect correction to the same 1e-10 FP64 "
+                                     "residual; cold start", **summary(modes["fp32_cold"])}
+        if "fp32_warm" in modes:
+            w = modes["fp32_warm"]
+            out["fp32_warm"] = {"method": "as fp32, initial guess = previous step's first Newton correction, "
+                                          "scaled to its least-residual multiple", **summary(w),
+                                "guess_r0_over_cold": round(w["guess_r0_first"], 4)}
         if "fp32_learned" in modes:
             m = modes["fp32_learned"]
             out["learned_fp32"] = {"method": "as fp32, initial guess from the BF16 CNN trained on 2000 "
@@ -496,10 +502,12 @@ def main():
         line_method = ("FP32 red-black radial-line SOR (each r column solved exactly as a tridiagonal system, "
                        "columns coloured by (j+k) parity, omega 1.1) inside FP64 defect correction to the same "
                        "1e-10 FP64 residual")
-        out["fp32_line"] = {"method": line_method + "; cold start", **summary(modes["line_cold"])}
-        w = modes["line_warm"]
-        out["fp32_line_warm"] = {"method": line_method + "; warm start as fp32_warm", **summary(w),
-                                 "guess_r0_over_cold": round(w["guess_r0_first"], 4)}
+        if "line_cold" in modes:
+            out["fp32_line"] = {"method": line_method + "; cold start", **summary(modes["line_cold"])}
+        if "line_warm" in modes:
+            w = modes["line_warm"]
+            out["fp32_line_warm"] = {"method": line_method + "; warm start as fp32_warm", **summary(w),
+                                     "guess_r0_over_cold": round(w["guess_r0_first"], 4)}
         if "line_learned" in modes:
             m = modes["line_learned"]
             out["learned_fp32_line"] = {"method": line_method + "; initial guess from the CNN as learned_fp32",

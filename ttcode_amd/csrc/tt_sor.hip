@@ -368,6 +368,82 @@ __global__ __launch_bounds__(256) void k_sweep_f32x2(int c, const float* __restr
 }
 
 // ------------------------------------------------------ radial-line sweeps
+// Lane moves on the vector unit (DPP), for the scans of the line sweep.
+// dpp<CTRL>(old, v): v from the lane CTRL selects; lanes whose source lies
+// outside the wavefront or the 16-lane row keep old.  Controls: 0x111 + n - 1
+// row_shr:n (lane l - n of the row), 0x101 + n - 1 row_shl:n (lane l + n),
+// 0x138 wave_shr:1 (lane l - 1), 0x130 wave_shl:1 (lane l + 1).
+template <int CTRL>
+__device__ inline float dpp(float old, float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+__device__ inline float rdlane(float v, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l)); }
+
+// Inclusive scan of affine maps z -> a z + b over the lanes in ascending
+// order (lane l ends with the composition of lanes 0..l, lane 0 applied
+// first): within each 16-lane row by four shifted compositions, then the
+// totals of the rows before, read from lanes 15, 31, 47.
+__device__ inline void scan_affine_up(float& a, float& b, int lane) {
+#define TT_UP(N)                                                                                    \
+    {                                                                                               \
+        const float ua = dpp<0x110 + N>(1.f, a), ub = dpp<0x110 + N>(0.f, b);                       \
+        b = a * ub + b;                                                                             \
+        a = a * ua;                                                                                 \
+    }
+    TT_UP(1) TT_UP(2) TT_UP(4) TT_UP(8)
+#undef TT_UP
+    const float a0 = rdlane(a, 15), b0 = rdlane(b, 15), a1 = rdlane(a, 31), b1 = rdlane(b, 31);
+    const float a2 = rdlane(a, 47), b2 = rdlane(b, 47);
+    // rows 0..r-1 composed: r = 1: T0; r = 2: T1 T0; r = 3: T2 T1 T0
+    const float a01 = a1 * a0, b01 = a1 * b0 + b1;
+    const float a012 = a2 * a01, b012 = a2 * b01 + b2;
+    const int row = lane >> 4;
+    const float ca = row == 0 ? 1.f : row == 1 ? a0 : row == 2 ? a01 : a012;
+    const float cb = row == 0 ? 0.f : row == 1 ? b0 : row == 2 ? b01 : b012;
+    b = a * cb + b;
+    a = a * ca;
+}
+
+// The same in descending order (lane l ends with lanes l..63, lane 63
+// applied first); row totals from lanes 48, 32, 16.
+__device__ inline void scan_affine_down(float& a, float& b, int lane) {
+#define TT_DN(N)                                                                                    \
+    {                                                                                               \
+        const float va = dpp<0x100 + N>(1.f, a), vb = dpp<0x100 + N>(0.f, b);                       \
+        b = a * vb + b;                                                                             \
+        a = a * va;                                                                                 \
+    }
+    TT_DN(1) TT_DN(2) TT_DN(4) TT_DN(8)
+#undef TT_DN
+    const float a3 = rdlane(a, 48), b3 = rdlane(b, 48), a2 = rdlane(a, 32), b2 = rdlane(b, 32);
+    const float a1 = rdlane(a, 16), b1 = rdlane(b, 16);
+    // rows r+1..3 composed: r = 2: T3; r = 1: T2 T3; r = 0: T1 T2 T3
+    const float a23 = a2 * a3, b23 = a2 * b3 + b2;
+    const float a123 = a1 * a23, b123 = a1 * b23 + b1;
+    const int row = lane >> 4;
+    const float ca = row == 3 ? 1.f : row == 2 ? a3 : row == 1 ? a23 : a123;
+    const float cb = row == 3 ? 0.f : row == 2 ? b3 : row == 1 ? b23 : b123;
+    b = a * cb + b;
+    a = a * ca;
+}
+
+// Q cells of a lane from p (FULL: all 64 Q cells of the line exist, so the
+// lane's cells load as 8-byte pairs where Q is even); cells past nx read fill.
+template <int Q, bool FULL>
+__device__ inline void load_cells(float (&v)[Q], const float* __restrict__ p, int i0, int nx, float fill) {
+    if (FULL && Q % 2 == 0) {
+#pragma unroll
+        for (int t = 0; t < Q; t += 2) {
+            const float2 w = *(const float2*)(p + t);
+            v[t] = w.x;
+            v[t + 1] = w.y;
+        }
+    } else {
+#pragma unroll
+        for (int t = 0; t < Q; ++t) v[t] = (FULL || i0 + t < nx) ? p[t] : fill;
+    }
+}
+
 // One half sweep of red-black line SOR: every line of colour c (lines
 // (j,k) with (j+k) parity c) is solved exactly for the current values of its
 // four neighbour lines,
@@ -376,13 +452,60 @@ __global__ __launch_bounds__(256) void k_sweep_f32x2(int c, const float* __restr
 // z_i = f_i - m_i z_{i-1}, y_i = (z_i - C_i y_{i+1}) / u_i; then
 // x <- x + omega (y - x).  One wavefront per line: lane l holds the Q cells
 // l Q .. l Q + Q - 1 (0-based i - 1), so both recurrences are affine maps a
-// lane composes over its own cells and the wavefront combines with a
-// 6-step shuffle scan (forward over lanes, then backward).  The residual of
-// the line before its update, f - L U x, is summed in |.| for the
-// convergence test, as the point sweep sums its residuals.  Cells past nx
-// act as identity rows with zero right-hand side.  Rows, blocks, the test
-// block and the partials are laid out as in k_sweep.
+// lane composes over its own cells and the wavefront combines with a scan
+// (forward over lanes, then backward).  The residual of the line before its
+// update, f - L U x, is summed in |.| for the convergence test, as the point
+// sweep sums its residuals.  Cells past nx act as identity rows with zero
+// right-hand side.  Rows, blocks, the test block and the partials are laid
+// out as in k_sweep.
+// The operands of one line for a lane: eight coefficient vectors, the own
+// unknowns and the four neighbour lines' (Q cells each).
 template <int Q>
+struct LineOps {
+    float cD[Q], cE[Q], cF[Q], cG[Q], cM[Q], cU[Q], cC[Q], cB[Q], X[Q], n1[Q], n2[Q], n3[Q], n4[Q];
+};
+
+template <int Q, bool FULL>
+__device__ inline void line_load(LineOps<Q>& o, const float* __restrict__ coef, const float* __restrict__ x, LS ls,
+                                 int c, int q, int i0) {
+    const int nx = ls.nx, ny = ls.ny, hy = ny >> 1;
+    const int kk = q / hy;
+    const int jj = 2 * (q - kk * hy) + ((kk + c) & 1);  // (jj + kk) & 1 == c
+    const int jm = jj == 0 ? ny - 1 : jj - 1;
+    const int jp = jj == ny - 1 ? 0 : jj + 1;
+    const float* __restrict__ cr = coef + (long long)(jj + ny * kk) * 8 * nx + i0;
+    load_cells<Q, FULL>(o.cD, cr, i0, nx, 0.f);
+    load_cells<Q, FULL>(o.cE, cr + nx, i0, nx, 0.f);
+    load_cells<Q, FULL>(o.cF, cr + 2 * nx, i0, nx, 0.f);
+    load_cells<Q, FULL>(o.cG, cr + 3 * nx, i0, nx, 0.f);
+    load_cells<Q, FULL>(o.cM, cr + 4 * nx, i0, nx, 0.f);
+    load_cells<Q, FULL>(o.cU, cr + 5 * nx, i0, nx, 1.f);
+    load_cells<Q, FULL>(o.cC, cr + 6 * nx, i0, nx, 0.f);
+    load_cells<Q, FULL>(o.cB, cr + 7 * nx, i0, nx, 0.f);
+    load_cells<Q, FULL>(o.X, x + (long long)(jj + ny * (kk + 1)) * nx + i0, i0, nx, 0.f);
+    load_cells<Q, FULL>(o.n1, x + (long long)(jm + ny * (kk + 1)) * nx + i0, i0, nx, 0.f);
+    load_cells<Q, FULL>(o.n2, x + (long long)(jp + ny * (kk + 1)) * nx + i0, i0, nx, 0.f);
+    load_cells<Q, FULL>(o.n3, x + (long long)(jj + ny * kk) * nx + i0, i0, nx, 0.f);
+    load_cells<Q, FULL>(o.n4, x + (long long)(jj + ny * (kk + 2)) * nx + i0, i0, nx, 0.f);
+}
+
+// One half sweep of red-black line SOR: every line of colour c (lines
+// (j,k) with (j+k) parity c) is solved exactly for the current values of its
+// four neighbour lines,
+//   T y = f,  f = b - D x(j-1) - E x(j+1) - F x(k-1) - G x(k+1),
+// with T the line's tridiagonal part (B, A, C) in its LU form (m, u, C):
+// z_i = f_i - m_i z_{i-1}, y_i = (z_i - C_i y_{i+1}) / u_i; then
+// x <- x + omega (y - x).  One wavefront per line: lane l holds the Q cells
+// l Q .. l Q + Q - 1 (0-based i - 1), so both recurrences are affine maps a
+// lane composes over its own cells and the wavefront combines with a scan
+// (forward over lanes, then backward).  The residual of the line before its
+// update, f - L U x, is summed in |.| for the convergence test, as the point
+// sweep sums its residuals.  Cells past nx act as identity rows with zero
+// right-hand side.  A wavefront's lines are software-pipelined: the next
+// line's operands load while the current one is solved (lines of one colour
+// never read each other).  Rows, blocks, the test block and the partials are
+// laid out as in k_sweep.
+template <int Q, bool FULL>
 __global__ __launch_bounds__(256) void k_line_sweep(int c, const float* __restrict__ coef, float* __restrict__ x, LS ls,
                                                     float omega, double* part, int nb, int lines_per_block, Ctrl* ctrl,
                                                     int rev, FinArgs fa) {
@@ -401,74 +524,45 @@ __global__ __launch_bounds__(256) void k_line_sweep(int c, const float* __restri
     const int q1 = min(q0 + lines_per_block, nl);
     const int i0 = lane * Q;
     double acc = 0.0;
-    for (int q = done ? q1 : q0 + ty; q < q1; q += 4) {
+    int q = done ? q1 : q0 + ty;
+    LineOps<Q> o;
+    if (q < q1) line_load<Q, FULL>(o, coef, x, ls, c, q, i0);
+    for (; q < q1; q += 4) {
+        LineOps<Q> nxt;
+        if (q + 4 < q1) line_load<Q, FULL>(nxt, coef, x, ls, c, q + 4, i0);
         const int kk = q / hy;
-        const int jj = 2 * (q - kk * hy) + ((kk + c) & 1);  // (jj + kk) & 1 == c
-        const int jm = jj == 0 ? ny - 1 : jj - 1;
-        const int jp = jj == ny - 1 ? 0 : jj + 1;
-        const float* __restrict__ cr = coef + (long long)(jj + ny * kk) * 8 * nx + i0;
+        const int jj = 2 * (q - kk * hy) + ((kk + c) & 1);
         float* __restrict__ xw = x + (long long)(jj + ny * (kk + 1)) * nx + i0;
-        const float* __restrict__ xjm = x + (long long)(jm + ny * (kk + 1)) * nx + i0;
-        const float* __restrict__ xjp = x + (long long)(jp + ny * (kk + 1)) * nx + i0;
-        const float* __restrict__ xkm = x + (long long)(jj + ny * kk) * nx + i0;
-        const float* __restrict__ xkp = x + (long long)(jj + ny * (kk + 2)) * nx + i0;
-        float cD[Q], cE[Q], cF[Q], cG[Q], cM[Q], cU[Q], cC[Q], cB[Q], X[Q], n1[Q], n2[Q], n3[Q], n4[Q];
-#pragma unroll
-        for (int t = 0; t < Q; ++t) {
-            const bool in = i0 + t < nx;
-            cD[t] = in ? cr[t] : 0.f;
-            cE[t] = in ? cr[nx + t] : 0.f;
-            cF[t] = in ? cr[2 * nx + t] : 0.f;
-            cG[t] = in ? cr[3 * nx + t] : 0.f;
-            cM[t] = in ? cr[4 * nx + t] : 0.f;
-            cU[t] = in ? cr[5 * nx + t] : 1.f;
-            cC[t] = in ? cr[6 * nx + t] : 0.f;
-            cB[t] = in ? cr[7 * nx + t] : 0.f;
-            X[t] = in ? xw[t] : 0.f;
-            n1[t] = in ? xjm[t] : 0.f;
-            n2[t] = in ? xjp[t] : 0.f;
-            n3[t] = in ? xkm[t] : 0.f;
-            n4[t] = in ? xkp[t] : 0.f;
-        }
         float f[Q], g[Q];
 #pragma unroll
-        for (int t = 0; t < Q; ++t) f[t] = cB[t] - cD[t] * n1[t] - cE[t] * n2[t] - cF[t] * n3[t] - cG[t] * n4[t];
+        for (int t = 0; t < Q; ++t)
+            f[t] = o.cB[t] - o.cD[t] * o.n1[t] - o.cE[t] * o.n2[t] - o.cF[t] * o.n3[t] - o.cG[t] * o.n4[t];
         // residual before the update: f - L (U x), (U x)_i = u_i x_i + C_i x_{i+1},
         // (L w)_i = w_i + m_i w_{i-1}; the r ghosts are zero
-        float xnext = __shfl_down(X[0], 1);
-        if (lane == 63) xnext = 0.f;
+        const float xnext = dpp<0x130>(0.f, o.X[0]);
 #pragma unroll
-        for (int t = 0; t < Q; ++t) g[t] = cU[t] * X[t] + cC[t] * (t + 1 < Q ? X[t + 1] : xnext);
-        float gprev = __shfl_up(g[Q - 1], 1);
-        if (lane == 0) gprev = 0.f;
+        for (int t = 0; t < Q; ++t) g[t] = o.cU[t] * o.X[t] + o.cC[t] * (t + 1 < Q ? o.X[t + 1] : xnext);
+        const float gprev = dpp<0x138>(0.f, g[Q - 1]);
         float rs = 0.f;
 #pragma unroll
-        for (int t = 0; t < Q; ++t) rs += fabsf(f[t] - (g[t] + cM[t] * (t ? g[t - 1] : gprev)));
+        for (int t = 0; t < Q; ++t) rs += fabsf(f[t] - (g[t] + o.cM[t] * (t ? g[t - 1] : gprev)));
         acc += (double)rs;
         // forward substitution z_i = f_i - m_i z_{i-1}: the lane's map
         // z_out = pa z_in + pb, composed with the lanes before it
         float pa = 1.f, pb = 0.f;
 #pragma unroll
         for (int t = 0; t < Q; ++t) {
-            pb = f[t] - cM[t] * pb;
-            pa = -cM[t] * pa;
+            pb = f[t] - o.cM[t] * pb;
+            pa = -o.cM[t] * pa;
         }
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const float ua = __shfl_up(pa, d), ub = __shfl_up(pb, d);
-            if (lane >= d) {
-                pb = pa * ub + pb;
-                pa = pa * ua;
-            }
-        }
-        float zin = __shfl_up(pb, 1);
-        if (lane == 0) zin = 0.f;
+        scan_affine_up(pa, pb, lane);
+        float zin = dpp<0x138>(0.f, pb);  // z before this lane's first cell (0 before cell 1)
         float ca[Q], cb[Q];
 #pragma unroll
         for (int t = 0; t < Q; ++t) {
-            zin = f[t] - cM[t] * zin;
-            const float w = __builtin_amdgcn_rcpf(cU[t]);
-            ca[t] = -cC[t] * w;
+            zin = f[t] - o.cM[t] * zin;
+            const float w = __builtin_amdgcn_rcpf(o.cU[t]);
+            ca[t] = -o.cC[t] * w;
             cb[t] = zin * w;
         }
         // back substitution y_i = ca_i y_{i+1} + cb_i: the lane's map
@@ -479,68 +573,150 @@ __global__ __launch_bounds__(256) void k_line_sweep(int c, const float* __restri
             qb = ca[t] * qb + cb[t];
             qa = ca[t] * qa;
         }
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const float va = __shfl_down(qa, d), vb = __shfl_down(qb, d);
-            if (lane + d < 64) {
-                qb = qa * vb + qb;
-                qa = qa * va;
-            }
-        }
-        float yin = __shfl_down(qb, 1);
-        if (lane == 63) yin = 0.f;
+        scan_affine_down(qa, qb, lane);
+        float yin = dpp<0x130>(0.f, qb);  // y after this lane's last cell (0 after cell nx)
+        float y[Q];
 #pragma unroll
         for (int t = Q - 1; t >= 0; --t) {
             yin = ca[t] * yin + cb[t];
-            if (i0 + t < nx) xw[t] = X[t] + omega * (yin - X[t]);
+            y[t] = o.X[t] + omega * (yin - o.X[t]);
         }
+        if (FULL && Q % 2 == 0) {
+#pragma unroll
+            for (int t = 0; t < Q; t += 2) *(float2*)(xw + t) = make_float2(y[t], y[t + 1]);
+        } else {
+#pragma unroll
+            for (int t = 0; t < Q; ++t)
+                if (FULL || i0 + t < nx) xw[t] = y[t];
+        }
+        o = nxt;
     }
     double bs = block_sum256(acc, sh);
     if (threadIdx.x == 0 && !done) part[c * nb + lb] = bs;
 }
 
-// LU factors of every line's tridiagonal part from the FP64 colour-split
-// system: u_1 = A_1, m_i = B_i / u_{i-1}, u_i = A_i - m_i C_{i-1} (FP64,
-// stored FP32).  B_1 and C_nx couple to the r ghosts, whose corrections are
-// zero, and drop out.  One thread per line, i ascending.
-__global__ void k_line_factor(CS cs, LS ls, const double* __restrict__ C0, const double* __restrict__ C1,
-                              float* __restrict__ lco) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= ls.ny * ls.nz) return;
+// FP64 lane moves and reads (two 32-bit halves each).
+template <int CTRL>
+__device__ inline double dppd(double old, double v) {
+    const long long o = __double_as_longlong(old), x = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp((int)o, (int)x, CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(x >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ inline double rdlaned(double v, int l) {
+    const long long x = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)x, l), hi = __builtin_amdgcn_readlane((int)(x >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// 2x2 matrices [[a, b], [c, d]] for the pivot recurrence, kept scaled by a
+// power of two (exact) so that long products neither overflow nor underflow.
+struct M2 {
+    double a, b, c, d;
+};
+__device__ inline M2 mmul(const M2& x, const M2& y) {
+    return M2{x.a * y.a + x.b * y.c, x.a * y.b + x.b * y.d, x.c * y.a + x.d * y.c, x.c * y.b + x.d * y.d};
+}
+__device__ inline M2 mnorm(const M2& x) {
+    const double s = fmax(fmax(fabs(x.a), fabs(x.b)), fmax(fabs(x.c), fabs(x.d)));
+    const int e = ilogb(s);
+    return M2{ldexp(x.a, -e), ldexp(x.b, -e), ldexp(x.c, -e), ldexp(x.d, -e)};
+}
+template <int CTRL>
+__device__ inline M2 mdpp(const M2& x) {  // identity where the source lane is outside
+    return M2{dppd<CTRL>(1.0, x.a), dppd<CTRL>(0.0, x.b), dppd<CTRL>(0.0, x.c), dppd<CTRL>(1.0, x.d)};
+}
+__device__ inline M2 mread(const M2& x, int l) {
+    return M2{rdlaned(x.a, l), rdlaned(x.b, l), rdlaned(x.c, l), rdlaned(x.d, l)};
+}
+
+// The line system from the FP64 colour-split one, one wavefront per line
+// (lane l holds cells l Q .. l Q + Q - 1): the couplings D..G, and the LU
+// factors of the tridiagonal part, u_1 = A_1, m_i = B_i / u_{i-1},
+// u_i = A_i - m_i C_{i-1}, all stored FP32.  B_1 and C_nx couple to the r
+// ghosts, whose corrections are zero, and drop out.  The pivots are ratios
+// of leading principal minors, u_i = p_i / p_{i-1}, with
+// (p_i, p_{i-1}) = M_i (p_{i-1}, p_{i-2}), M_i = [[A_i, -B_i C_{i-1}], [1, 0]],
+// so each is read from a prefix product of the M_i: a lane multiplies its
+// own cells', the wavefront scans the lanes' products (four shifted
+// multiplications in each 16-lane row, then the row carries), and the lane
+// steps through its cells from the product before them.  In FP64 the
+// pivots agree with the sequential recurrence to about 1e-14 relative.
+template <int Q, bool FULL>
+__global__ __launch_bounds__(256) void k_line_factor(CS cs, LS ls, const double* __restrict__ C0,
+                                                     const double* __restrict__ C1, float* __restrict__ lco) {
+    const int lane = threadIdx.x & 63;
+    const int r = blockIdx.x * 4 + (threadIdx.x >> 6);  // line (jj, kk), r = jj + ny kk
+    if (r >= ls.ny * ls.nz) return;                      // whole wavefronts leave together
     const int kk = r / ls.ny, jj = r - kk * ls.ny;
     const int hx = cs.hx, nx = ls.nx;
+    const int i0 = lane * Q;
     const double* __restrict__ b0 = C0 + (long long)r * 8 * hx;
     const double* __restrict__ b1 = C1 + (long long)r * 8 * hx;
-    float* __restrict__ o = lco + (long long)r * 8 * nx;
-    double up = 1.0, cp = 0.0;
-    for (int i = 1; i <= nx; ++i) {
-        const double* __restrict__ bb = (((i + jj + kk) & 1) ? b1 : b0) + ((i - 1) >> 1);  // colour (i+j+k)&1
-        const double a = bb[0], b = bb[hx], cc = bb[2 * hx];
-        const double m = i == 1 ? 0.0 : b / up;
-        const double u = a - m * cp;
-        o[4 * nx + i - 1] = (float)m;
-        o[5 * nx + i - 1] = (float)u;
-        o[6 * nx + i - 1] = (float)cc;
-        up = u;
-        cp = cc;
+    double va[Q], vb[Q], vc[Q], cpl[7][Q];
+#pragma unroll
+    for (int t = 0; t < Q; ++t) {
+        const int idx = i0 + t;  // cell i = idx + 1, colour (i + j + k) & 1
+        const bool in = FULL || idx < nx;
+        const double* __restrict__ bb = (((idx + 1 + jj + kk) & 1) ? b1 : b0) + (idx >> 1);
+        va[t] = in ? bb[0] : 1.0;
+        vb[t] = in ? bb[hx] : 0.0;
+        vc[t] = in ? bb[2 * hx] : 0.0;
+#pragma unroll
+        for (int v = 3; v < 7; ++v) cpl[v][t] = in ? bb[v * hx] : 0.0;
+    }
+    // C of the cell before this lane's first
+    const double cbefore = dppd<0x138>(0.0, vc[Q - 1]);
+    M2 P{1.0, 0.0, 0.0, 1.0};
+#pragma unroll
+    for (int t = 0; t < Q; ++t) {
+        const double cprev = t ? vc[t - 1] : cbefore;
+        const M2 m{va[t], i0 + t == 0 ? 0.0 : -vb[t] * cprev, 1.0, 0.0};
+        P = mnorm(mmul(m, P));
+    }
+    // inclusive scan: lane l gets M(lane l) ... M(lane 0)
+#define TT_MS(N) P = mnorm(mmul(P, mdpp<0x110 + N>(P)));
+    TT_MS(1) TT_MS(2) TT_MS(4) TT_MS(8)
+#undef TT_MS
+    {
+        const M2 t0 = mread(P, 15), t1 = mread(P, 31), t2 = mread(P, 47);
+        const M2 t10 = mnorm(mmul(t1, t0));
+        const M2 t210 = mnorm(mmul(t2, t10));
+        const int row = lane >> 4;
+        const M2 cy = row == 0 ? M2{1.0, 0.0, 0.0, 1.0} : row == 1 ? t0 : row == 2 ? t10 : t210;
+        P = mnorm(mmul(P, cy));
+    }
+    // the product over the cells before this lane's first (identity on lane 0)
+    M2 cur = mdpp<0x138>(P);
+    double uprev = cur.a / cur.c;  // u of the cell before (unused for cell 1)
+    float* __restrict__ o = lco + (long long)r * 8 * nx + i0;
+#pragma unroll
+    for (int t = 0; t < Q; ++t) {
+        const double cprev = t ? vc[t - 1] : cbefore;
+        const M2 m{va[t], i0 + t == 0 ? 0.0 : -vb[t] * cprev, 1.0, 0.0};
+        cur = mnorm(mmul(m, cur));
+        const double u = cur.a / cur.c;
+        const double mm = i0 + t == 0 ? 0.0 : vb[t] / uprev;
+        uprev = u;
+        if (FULL || i0 + t < nx) {
+            o[t] = (float)cpl[3][t];
+            o[nx + t] = (float)cpl[4][t];
+            o[2 * nx + t] = (float)cpl[5][t];
+            o[3 * nx + t] = (float)cpl[6][t];
+            o[4 * nx + t] = (float)mm;
+            o[5 * nx + t] = (float)u;
+            o[6 * nx + t] = (float)vc[t];
+        }
     }
 }
 
-// The phi and theta couplings D..G of every cell into the line layout (FP32).
-__global__ void k_line_angular(CS cs, LS ls, const double* __restrict__ C0, const double* __restrict__ C1,
-                               float* __restrict__ lco) {
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int nx = ls.nx;
-    if (t >= (long long)nx * ls.ny * ls.nz) return;
-    const int r = (int)(t / nx), i = (int)(t - (long long)r * nx) + 1;
-    const int kk = r / ls.ny, jj = r - kk * ls.ny;
-    const int hx = cs.hx;
-    const double* __restrict__ bb = (((i + jj + kk) & 1) ? C1 : C0) + (long long)r * 8 * hx + ((i - 1) >> 1);
-    float* __restrict__ o = lco + (long long)r * 8 * nx + i - 1;
-    o[0] = (float)bb[3 * hx];
-    o[nx] = (float)bb[4 * hx];
-    o[2 * nx] = (float)bb[5 * hx];
-    o[3 * nx] = (float)bb[6 * hx];
+template <int Q>
+static void line_factor_q(int nl) {
+    const int grid = (nl + 3) / 4;
+    if (g_ls.nx == 64 * Q)
+        k_line_factor<Q, true><<<grid, 256, 0, L.s>>>(g_cs, g_ls, c64[0], c64[1], g_lco);
+    else
+        k_line_factor<Q, false><<<grid, 256, 0, L.s>>>(g_cs, g_ls, c64[0], c64[1], g_lco);
 }
 
 // ---------------------------------------------------------- matrix fill
@@ -765,6 +941,23 @@ __global__ void k_add_corr(CS cs, double* x, const float* __restrict__ c) {
     x[p] = x[p] + (double)c[p];
 }
 
+// The residual of a zero start, RHS itself, in FP32 into the point system's
+// RHS slots (r32_0/1) or the line system's (rline).
+__global__ void k_rhs_f32(CS cs, const double* __restrict__ C0, const double* __restrict__ C1, float* r32_0,
+                          float* r32_1, float* rline, int nx) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 2 * cs.ncell) return;
+    const int c = t >= cs.ncell;
+    const int e = (int)(t - c * cs.ncell);
+    const int hx = cs.hx;
+    const int rr = e / hx, h = e - rr * hx;
+    const int kk = rr / cs.ny, jj = rr - kk * cs.ny;
+    const long long o = (long long)rr * 8 * hx + 7 * hx + h;
+    const double v = (c ? C1 : C0)[o];
+    if (r32_0) (c ? r32_1 : r32_0)[o] = (float)v;
+    if (rline) rline[((long long)rr * 8 + 7) * nx + 2 * h + ((jj + kk + c + 1) & 1)] = (float)v;
+}
+
 // x64 (padded, colour cc) += the line unknowns over the interior cells
 __global__ void k_add_corr_line(CS cs, int cc, int nx, double* x, const float* __restrict__ xl) {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -831,14 +1024,22 @@ static void ev_drain(const Ctrl* st, long long upto) {
     ev_bid.resize(keep);
 }
 
-template <int Q>
+template <int Q, bool FULL>
 static void line_go(bool timed, hipEvent_t e0, hipEvent_t e1, int grid, int c, float omega, int rev, double* part,
                     FinArgs fa) {
     if (!timed)
-        k_line_sweep<Q><<<grid, 256, 0, L.s>>>(c, g_lco, g_lx, g_ls, omega, part, g_nb, g_lchunk, L.ctrl, rev, fa);
+        k_line_sweep<Q, FULL><<<grid, 256, 0, L.s>>>(c, g_lco, g_lx, g_ls, omega, part, g_nb, g_lchunk, L.ctrl, rev,
+                                                     fa);
     else
-        hipExtLaunchKernelGGL(k_line_sweep<Q>, dim3(grid), dim3(256), 0, L.s, e0, e1, 0, c, (const float*)g_lco,
-                              g_lx, g_ls, omega, part, g_nb, g_lchunk, L.ctrl, rev, fa);
+        hipExtLaunchKernelGGL(k_line_sweep<Q, FULL>, dim3(grid), dim3(256), 0, L.s, e0, e1, 0, c,
+                              (const float*)g_lco, g_lx, g_ls, omega, part, g_nb, g_lchunk, L.ctrl, rev, fa);
+}
+
+template <int Q>
+static void line_go_q(bool timed, hipEvent_t e0, hipEvent_t e1, int grid, int c, float omega, int rev, double* part,
+                      FinArgs fa) {
+    if (g_ls.nx == 64 * Q) line_go<Q, true>(timed, e0, e1, grid, c, omega, rev, part, fa);
+    else line_go<Q, false>(timed, e0, e1, grid, c, omega, rev, part, fa);
 }
 
 // A half sweep of the line solver (same partials, test block and timing as
@@ -851,12 +1052,12 @@ static int launch_line(int c, double omega, int rev, int iter, double* part, Fin
     }
     const float w = (float)omega;
     const int q = (g_ls.nx + 63) / 64;
-    if (q <= 1) line_go<1>(timed, e0, e1, grid, c, w, rev, part, fa);
-    else if (q == 2) line_go<2>(timed, e0, e1, grid, c, w, rev, part, fa);
-    else if (q == 3) line_go<3>(timed, e0, e1, grid, c, w, rev, part, fa);
-    else if (q == 4) line_go<4>(timed, e0, e1, grid, c, w, rev, part, fa);
-    else if (q <= 6) line_go<6>(timed, e0, e1, grid, c, w, rev, part, fa);
-    else line_go<8>(timed, e0, e1, grid, c, w, rev, part, fa);
+    if (q <= 1) line_go_q<1>(timed, e0, e1, grid, c, w, rev, part, fa);
+    else if (q == 2) line_go_q<2>(timed, e0, e1, grid, c, w, rev, part, fa);
+    else if (q == 3) line_go_q<3>(timed, e0, e1, grid, c, w, rev, part, fa);
+    else if (q == 4) line_go_q<4>(timed, e0, e1, grid, c, w, rev, part, fa);
+    else if (q <= 6) line_go_q<6>(timed, e0, e1, grid, c, w, rev, part, fa);
+    else line_go_q<8>(timed, e0, e1, grid, c, w, rev, part, fa);
     TT_LAUNCH();
     if (timed) {
         ev_pend.push_back(e0);
@@ -1012,8 +1213,9 @@ template <typename T>
 static int run_sweeps(T* coef0, T* coef1, T* xa, T* xb, double omega, int* iters, double* rnorm,
                       int check_every, bool deferred, bool speculate = false, int max_batch = 0) {
     int batch = L.opts.batch > 0 ? L.opts.batch : 16;
-    if (max_batch > 0 && batch > max_batch) batch = max_batch;
     batch = ((batch + 3) / 4) * 4;  // a multiple of 4: directions alternate and checks stay aligned
+    // (even suffices where every iteration is tested)
+    if (max_batch > 0 && batch > max_batch) batch = max_batch;
     long long launched = 0;
     int slot = 0, pend = -1, last = 0;
     SolveView view;
@@ -1334,10 +1536,13 @@ int sor_solve_impl(int fill_from_evals, tt_sor_stats* st, int mode) {
             if ((rc = line_alloc())) return rc;
             if (!g_line_ok) {
                 const int nl = g_ls.ny * g_ls.nz;
-                k_line_factor<<<nblk(nl, 64), 64, 0, L.s>>>(g_cs, g_ls, c64[0], c64[1], g_lco);
-                TT_LAUNCH();
-                const long long nc = (long long)g_ls.nx * nl;
-                k_line_angular<<<nblk(nc, 256), 256, 0, L.s>>>(g_cs, g_ls, c64[0], c64[1], g_lco);
+                const int q = (g_ls.nx + 63) / 64;
+                if (q <= 1) line_factor_q<1>(nl);
+                else if (q == 2) line_factor_q<2>(nl);
+                else if (q == 3) line_factor_q<3>(nl);
+                else if (q == 4) line_factor_q<4>(nl);
+                else if (q <= 6) line_factor_q<6>(nl);
+                else line_factor_q<8>(nl);
                 TT_LAUNCH();
                 g_line_ok = true;
             }
@@ -1350,7 +1555,17 @@ int sor_solve_impl(int fill_from_evals, tt_sor_stats* st, int mode) {
         }
         const int to = line ? 2 : 1;
         double rn;
-        rc = resid64(&rn, to);  // reads the control block: rhsnorm is current
+        if (s.guess_used) {
+            rc = resid64(&rn, to);  // reads the control block: rhsnorm is current
+        } else {
+            // zero start: the residual is RHS, its norm sum|RHS|
+            k_rhs_f32<<<nblk(2 * g_cs.ncell, 256), 256, 0, L.s>>>(g_cs, c64[0], c64[1], line ? nullptr : c32[0],
+                                                                  line ? nullptr : c32[1], line ? g_lco : nullptr,
+                                                                  g_ls.nx);
+            TT_LAUNCH();
+            rc = read_ctrl();
+            rn = L.hctrl->rhsnorm;
+        }
         if (rc) return rc;
         const double rhs_norm = L.hctrl->rhsnorm;
         int outer = 0, total32 = 0;
@@ -1389,7 +1604,7 @@ int sor_solve_impl(int fill_from_evals, tt_sor_stats* st, int mode) {
                 // every iteration tested (inside the next red sweep), batches of 4
                 g_line = 1;
                 rc = run_sweeps<float>(c32[0], c32[1], x32[0], x32[1], L.opts.line_omega, &it32, &rn32, 1, true,
-                                       false, 4);
+                                       false, 2);
                 g_line = 0;
             } else {
                 rc = run_sweeps<float>(c32[0], c32[1], x32[0], x32[1], L.p.sorparam, &it32, &rn32, 4, true);

@@ -412,9 +412,11 @@ int tt_set_solver(const tt_solver_opts* o) {
     if (o->inner == 1 && !(o->line_omega > 0.0 && o->line_omega < 2.0))
         return fail(1, "invalid solver options: line_omega must lie in (0, 2)");
     if (o->guess != L.opts.guess || o->precision != L.opts.precision || o->inner != L.opts.inner) {
-        // sweep counts of another mode say nothing about this one
+        // sweep counts and rejections of another mode say nothing about this one
         L.start_avg[0] = L.start_avg[1] = 0.0;
         L.start_picks = 0;
+        L.learned_rejects = 0;
+        L.learned_idle = 0;
     }
     L.opts = *o;
     return 0;
