@@ -324,7 +324,7 @@ def run_forgas_visc(a):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--nr", type=int, default=128)
