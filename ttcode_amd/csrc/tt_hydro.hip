@@ -6,7 +6,7 @@
 // the cells within two of it along its line, so here the fluxes are formed
 // per face directly from the state, each face once: the mass fluxes by
 // k_adv_mass (stored, the momentum sweeps reuse them through face
-// averages), the transported quantities by k_adv_tiled, which shares each
+// averages), the transported quantities by k_adv_t, which shares each
 // face between its two cells through LDS.  The arithmetic is the
 // reference's, face by face.
 #include <cmath>
@@ -125,18 +125,21 @@ __global__ void k_hd_init(Geo g, Met m, const double* __restrict__ H, double* HD
     HD[t] = H[t] * (gh * gh);
 }
 
-// ---- face-shared sweeps.  The four transport updates above evaluate, per
-// cell, the fluxes through all six of its faces, so every face flux is
-// formed twice.  k_adv_tiled forms each face flux once: a block takes a tile
-// of 32 (i) x 8 (j) cells and marches through 2 planes of k (longer marches
-// share more z faces but leave too few blocks to hide the loads' latency:
-// 2 planes measured best at 128x128x256).  Per
-// plane each cell lane forms the x face on its low side and the y face on its
-// low side into LDS (double-buffered, one barrier per plane) and the z face
-// on its high side in a register, which is the low z face of the next plane;
-// a fifth wavefront forms the tile's extra high x faces (8) and high y faces
-// (32).  The flux expressions are the reference's, face by face, so the
-// results equal the per-cell form bit for bit.
+// ---- face-shared sweeps.  The four transport updates evaluate, per cell,
+// the fluxes through all six of its faces, so every face flux would be
+// formed twice.  k_adv_t forms each face flux once: a block takes a tile of
+// 32 (i) x 8 (j) cells and marches through KC planes of k.  Per plane each
+// cell lane forms the x face on its low side and the y face on its low side
+// into LDS (double-buffered, one barrier per plane) and the z face on its
+// high side in a register, which is the low z face of the next plane; a
+// fifth wavefront forms the tile's extra high x faces (8) and high y faces
+// (32).  Addressing and metrics are arranged for few instructions per face:
+// one 32-bit index per cell and plane with fixed offsets along the lines
+// (+-1 in i, +-gx in j, +-gx*gy in k); the 1-D metrics a face or cell needs
+// are loaded once per lane (x faces and i, y faces and j: before the march)
+// or once per plane (z faces and k).  The flux expressions are the
+// reference's, face by face, so the results equal the per-cell form bit for
+// bit.
 struct AdvArgs {
     const double *A;                 // advected quantity: rho c_v T, V, GD or HD
     const double *V, *G, *H;         // velocities
@@ -147,67 +150,77 @@ struct AdvArgs {
     double *cf1, *cf2, *cf3;         // energy only: conv_flux, the low-face fluxes (ppadv.f:857-865)
 };
 
-constexpr int ADV_TI = 32, ADV_TJ = 8, ADV_KC = 2;
+constexpr int ADV_TI = 32, ADV_TJ = 8;
 
-// KIND 0: thermal energy (ppadv.f:741-891), cell faces I = i, i+1 etc.; the
-// upwind cell is I-1 for a positive velocity, else I
+// Offsets of a face's neighbours: one step along i, j, k.
+struct Step {
+    int y, z;  // gx, gx*gy (x is 1)
+};
+
+// The metrics of one face along its line: the spacings of the cells around
+// it (d0..d2 or d_1..d1), two cell coordinates and a face coordinate.
+struct FaceMet {
+    double d0, d1, d2, x0, x1, xa;
+};
+// the metrics of a face whose low candidate cell is I-1 (centred quantities:
+// DXB, XXB spacings; d_1, d0, d1 in d0..d2, x_1, x0 in x0, x1)
+__device__ inline FaceMet fm_b(const Met& m, int D, int X, int XA, int I) {
+    return FaceMet{m(D, I - 1), m(D, I), m(D, I + 1), m(X, I - 1), m(X, I), m(XA, I)};
+}
+// a face of a staggered quantity whose candidate cells are I and I+1 (DXA,
+// XXA spacings: d0..d2 at I..I+2, x0, x1 at I, I+1, xa the cell's XXB(I))
+__device__ inline FaceMet fm_a(const Met& m, int D, int X, int XB, int I) {
+    return FaceMet{m(D, I), m(D, I + 1), m(D, I + 2), m(X, I), m(X, I + 1), m(XB, I)};
+}
+
+// KIND 0: thermal energy (ppadv.f:741-891), faces I = i, i+1 etc.; the
+// upwind cell is I-1 for a positive velocity, else I.  All three face
+// families have the same form.
 struct AdvE {
     static constexpr int ox = 0, oy = 0, oz = 0;
-    __device__ static int ilo(const Geo&) { return 1; }
-    __device__ static int ihi(const Geo& g) { return g.nx; }
-    __device__ static int jlo(const Geo&) { return 1; }
-    __device__ static int jhi(const Geo& g) { return g.ny; }
-    __device__ static int klo(const Geo&) { return 1; }
-    __device__ static int khi(const Geo& g) { return g.nz; }
-    __device__ static double fx(const Geo& g, const Met& m, const AdvArgs& a, int I, int j, int k) {
+    static constexpr int ilo = 1, jlo = 1, klo = 1;
+    struct XF {
+        FaceMet f;
+        double sa, ca;
+    };
+    __device__ static XF xmet(const Met& m, int I, int) {
+        return XF{fm_b(m, TT_M_DXB, TT_M_XXB, TT_M_XXA, I), m(TT_M_SURXA, I), m(TT_M_CCXA, I)};
+    }
+    __device__ static XF ymet(const Met& m, int J, int) {
+        return XF{fm_b(m, TT_M_DYB, TT_M_XYB, TT_M_XYA, J), m(TT_M_SURYA, J), m(TT_M_CCYA, J)};
+    }
+    __device__ static XF zmet(const Met& m, int K) {
+        return XF{fm_b(m, TT_M_DZB, TT_M_XZB, TT_M_XZA, K), m(TT_M_SURZA, K), m(TT_M_CCZA, K)};
+    }
+    struct PM {
+        double volz;
+    };
+    __device__ static PM pmet(const Met& m, int k) { return PM{m(TT_M_VOLZB, k)}; }
+    __device__ static double vol_x(const Met& m, int i) { return m(TT_M_VOLXB, i); }
+    __device__ static double vol_y(const Met& m, int j) { return m(TT_M_VOLYB, j); }
+    // face flux along a line with step s through the face's own cell n
+    __device__ static double face(const AdvArgs& a, const double* __restrict__ vel, int n, int s, const XF& x) {
         const double half = 0.5;
-        Box3 E{a.A, g};
-        const double v = a.V[IX(I, j, k)];
-        const double e_2 = E(I - 2, j, k), e_1 = E(I - 1, j, k), e0 = E(I, j, k), e1 = E(I + 1, j, k);
-        const double d_1 = m(TT_M_DXB, I - 1), d0 = m(TT_M_DXB, I), d1 = m(TT_M_DXB, I + 1);
-        const double x_1 = m(TT_M_XXB, I - 1), x0 = m(TT_M_XXB, I);
-        const double sa = m(TT_M_SURXA, I), ca = m(TT_M_CCXA, I), xa = m(TT_M_XXA, I);
+        const double* __restrict__ E = a.A;
+        const double v = vel[n];
+        const double e_2 = E[n - 2 * s], e_1 = E[n - s], e0 = E[n], e1 = E[n + s];
         double S = v * a.dt;
         const bool up = v > 0.0;
         double eu;
-        double gr = vl_face(up, e_2, e_1, e0, e1, d_1, d0, d1, 0, &eu);
-        return (sa - ca * S) * S * (eu + (xa - (up ? x_1 : x0) - half * S) * gr);
+        double gr = vl_face(up, e_2, e_1, e0, e1, x.f.d0, x.f.d1, x.f.d2, 0, &eu);
+        return (x.sa - x.ca * S) * S * (eu + (x.f.xa - (up ? x.f.x0 : x.f.x1) - half * S) * gr);
     }
-    __device__ static double fy(const Geo& g, const Met& m, const AdvArgs& a, int i, int J, int k) {
-        const double half = 0.5;
-        Box3 E{a.A, g};
-        const double v = a.G[IX(i, J, k)];
-        const double e_2 = E(i, J - 2, k), e_1 = E(i, J - 1, k), e0 = E(i, J, k), e1 = E(i, J + 1, k);
-        const double d_1 = m(TT_M_DYB, J - 1), d0 = m(TT_M_DYB, J), d1 = m(TT_M_DYB, J + 1);
-        const double x_1 = m(TT_M_XYB, J - 1), x0 = m(TT_M_XYB, J);
-        const double sa = m(TT_M_SURYA, J), ca = m(TT_M_CCYA, J), xa = m(TT_M_XYA, J);
-        double S = v * a.dt;
-        const bool up = v > 0.0;
-        double eu;
-        double gr = vl_face(up, e_2, e_1, e0, e1, d_1, d0, d1, 0, &eu);
-        return (sa - ca * S) * S * (eu + (xa - (up ? x_1 : x0) - half * S) * gr);
+    __device__ static double fx(const AdvArgs& a, int n, Step, const XF& x, const PM&) { return face(a, a.V, n, 1, x); }
+    __device__ static double fy(const AdvArgs& a, int n, Step s, const XF& y, const PM&) { return face(a, a.G, n, s.y, y); }
+    __device__ static double fz(const AdvArgs& a, int n, Step s, const XF& z, int, double, double) {
+        return face(a, a.H, n, s.z, z);
     }
-    __device__ static double fz(const Geo& g, const Met& m, const AdvArgs& a, int i, int j, int K) {
-        const double half = 0.5;
-        Box3 E{a.A, g};
-        const double v = a.H[IX(i, j, K)];
-        const double e_2 = E(i, j, K - 2), e_1 = E(i, j, K - 1), e0 = E(i, j, K), e1 = E(i, j, K + 1);
-        const double d_1 = m(TT_M_DZB, K - 1), d0 = m(TT_M_DZB, K), d1 = m(TT_M_DZB, K + 1);
-        const double x_1 = m(TT_M_XZB, K - 1), x0 = m(TT_M_XZB, K);
-        const double sa = m(TT_M_SURZA, K), ca = m(TT_M_CCZA, K), xa = m(TT_M_XZA, K);
-        double S = v * a.dt;
-        const bool up = v > 0.0;
-        double eu;
-        double gr = vl_face(up, e_2, e_1, e0, e1, d_1, d0, d1, 0, &eu);
-        return (sa - ca * S) * S * (eu + (xa - (up ? x_1 : x0) - half * S) * gr);
-    }
-    __device__ static void finish(const Geo& g, const Met& m, const AdvArgs& a, int i, int j, int k, double fx0,
-                                  double fx1, double fy0, double fy1, double fz0, double fz1) {
-        a.out[IX(i, j, k)] = a.A[IX(i, j, k)] - (fx1 - fx0) / m(TT_M_VOLXB, i) - (fy1 - fy0) / m(TT_M_VOLYB, j) -
-                             (fz1 - fz0) / m(TT_M_VOLZB, k);
-        a.cf1[IX(i, j, k)] = fx0;
-        a.cf2[IX(i, j, k)] = fy0;
-        a.cf3[IX(i, j, k)] = fz0;
+    __device__ static void finish(const AdvArgs& a, int n, double vx, double vy, const PM& pm, double fx0, double fx1,
+                                  double fy0, double fy1, double fz0, double fz1) {
+        a.out[n] = a.A[n] - (fx1 - fx0) / vx - (fy1 - fy0) / vy - (fz1 - fz0) / pm.volz;
+        a.cf1[n] = fx0;
+        a.cf2[n] = fy0;
+        a.cf3[n] = fz0;
     }
 };
 
@@ -215,59 +228,58 @@ struct AdvE {
 // FLX(I) at I = i-1, i (upwind cell I for a positive S, else I+1)
 struct AdvV {
     static constexpr int ox = -1, oy = 0, oz = 0;
-    __device__ static int ilo(const Geo&) { return 2; }
-    __device__ static int ihi(const Geo& g) { return g.nx; }
-    __device__ static int jlo(const Geo&) { return 1; }
-    __device__ static int jhi(const Geo& g) { return g.ny; }
-    __device__ static int klo(const Geo&) { return 1; }
-    __device__ static int khi(const Geo& g) { return g.nz; }
-    __device__ static double fx(const Geo& g, const Met& m, const AdvArgs& a, int I, int j, int k) {
+    static constexpr int ilo = 2, jlo = 1, klo = 1;
+    struct XF {
+        FaceMet f;
+        double p1, p0;  // DDX1, DDX0 of the face's cell column (y and z faces)
+    };
+    __device__ static XF xmet(const Met& m, int I, int) {
+        return XF{fm_a(m, TT_M_DXA, TT_M_XXA, TT_M_XXB, I), 0.0, 0.0};
+    }
+    __device__ static XF ymet(const Met& m, int J, int i) {
+        return XF{fm_b(m, TT_M_DYB, TT_M_XYB, TT_M_XYA, J), m(TT_M_DDX1, i), m(TT_M_DDX0, i)};
+    }
+    __device__ static XF zmet(const Met& m, int K) { return XF{fm_b(m, TT_M_DZB, TT_M_XZB, TT_M_XZA, K), 0.0, 0.0}; }
+    struct PM {
+        double volz;
+    };
+    __device__ static PM pmet(const Met& m, int k) { return PM{m(TT_M_VOLZB, k)}; }
+    __device__ static double vol_x(const Met& m, int i) { return m(TT_M_VOLXA, i); }
+    __device__ static double vol_y(const Met& m, int j) { return m(TT_M_VOLYB, j); }
+    __device__ static double fx(const AdvArgs& a, int n, Step, const XF& x, const PM&) {
         const double half = 0.5;
-        Box3 Vb{a.V, g};
-        const double v_1 = Vb(I - 1, j, k), v0 = Vb(I, j, k), v1 = Vb(I + 1, j, k), v2 = Vb(I + 2, j, k);
-        const double f0 = a.FX[IX(I, j, k)], f1 = a.FX[IX(I + 1, j, k)];
-        const double d0 = m(TT_M_DXA, I), d1 = m(TT_M_DXA, I + 1), d2 = m(TT_M_DXA, I + 2);
-        const double x0 = m(TT_M_XXA, I), x1 = m(TT_M_XXA, I + 1), xb = m(TT_M_XXB, I);
+        const double* __restrict__ Vb = a.V;
+        const double v_1 = Vb[n - 1], v0 = Vb[n], v1 = Vb[n + 1], v2 = Vb[n + 2];
+        const double f0 = a.FX[n], f1 = a.FX[n + 1];
         double S = (v0 + v1) * a.dt / 2.0;
         const bool up = S > 0.0;
         double vu;
-        double gr = vl_face(up, v_1, v0, v1, v2, d0, d1, d2, 1, &vu);
-        return half * (f0 + f1) * (vu + (xb - (up ? x0 : x1) - half * S) * gr);
+        double gr = vl_face(up, v_1, v0, v1, v2, x.f.d0, x.f.d1, x.f.d2, 1, &vu);
+        return half * (f0 + f1) * (vu + (x.f.xa - (up ? x.f.x0 : x.f.x1) - half * S) * gr);
     }
-    __device__ static double fy(const Geo& g, const Met& m, const AdvArgs& a, int i, int J, int k) {
+    // a y or z face at n along step s, with the column factors of the cell
+    __device__ static double fyz(const AdvArgs& a, const double* __restrict__ vel, const double* __restrict__ fl,
+                                 int n, int s, const FaceMet& f, double p1, double p0) {
         const double half = 0.5;
-        Box3 Vb{a.V, g};
-        const double p1 = m(TT_M_DDX1, i), p0 = m(TT_M_DDX0, i);
-        const double g1 = a.G[IX(i, J, k)], g0 = a.G[IX(i - 1, J, k)];
-        const double f1 = a.FY[IX(i, J, k)], f0 = a.FY[IX(i - 1, J, k)];
-        const double v_2 = Vb(i, J - 2, k), v_1 = Vb(i, J - 1, k), v0 = Vb(i, J, k), v1 = Vb(i, J + 1, k);
-        const double d_1 = m(TT_M_DYB, J - 1), d0 = m(TT_M_DYB, J), d1 = m(TT_M_DYB, J + 1);
-        const double x_1 = m(TT_M_XYB, J - 1), x0 = m(TT_M_XYB, J), xa = m(TT_M_XYA, J);
+        const double* __restrict__ Vb = a.V;
+        const double g1 = vel[n], g0 = vel[n - 1];
+        const double f1 = fl[n], f0 = fl[n - 1];
+        const double v_2 = Vb[n - 2 * s], v_1 = Vb[n - s], v0 = Vb[n], v1 = Vb[n + s];
         double S = (p1 * g1 + p0 * g0) * a.dt;
         const bool up = S > 0.0;
         double vu;
-        double gr = vl_face(up, v_2, v_1, v0, v1, d_1, d0, d1, 1, &vu);
-        return (p1 * f1 + p0 * f0) * (vu + (xa - (up ? x_1 : x0) - half * S) * gr);
+        double gr = vl_face(up, v_2, v_1, v0, v1, f.d0, f.d1, f.d2, 1, &vu);
+        return (p1 * f1 + p0 * f0) * (vu + (f.xa - (up ? f.x0 : f.x1) - half * S) * gr);
     }
-    __device__ static double fz(const Geo& g, const Met& m, const AdvArgs& a, int i, int j, int K) {
-        const double half = 0.5;
-        Box3 Vb{a.V, g};
-        const double p1 = m(TT_M_DDX1, i), p0 = m(TT_M_DDX0, i);
-        const double h1 = a.H[IX(i, j, K)], h0 = a.H[IX(i - 1, j, K)];
-        const double f1 = a.FZ[IX(i, j, K)], f0 = a.FZ[IX(i - 1, j, K)];
-        const double v_2 = Vb(i, j, K - 2), v_1 = Vb(i, j, K - 1), v0 = Vb(i, j, K), v1 = Vb(i, j, K + 1);
-        const double d_1 = m(TT_M_DZB, K - 1), d0 = m(TT_M_DZB, K), d1 = m(TT_M_DZB, K + 1);
-        const double x_1 = m(TT_M_XZB, K - 1), x0 = m(TT_M_XZB, K), xa = m(TT_M_XZA, K);
-        double S = (p1 * h1 + p0 * h0) * a.dt;
-        const bool up = S > 0.0;
-        double vu;
-        double gr = vl_face(up, v_2, v_1, v0, v1, d_1, d0, d1, 1, &vu);
-        return (p1 * f1 + p0 * f0) * (vu + (xa - (up ? x_1 : x0) - half * S) * gr);
+    __device__ static double fy(const AdvArgs& a, int n, Step s, const XF& y, const PM&) {
+        return fyz(a, a.G, a.FY, n, s.y, y.f, y.p1, y.p0);
     }
-    __device__ static void finish(const Geo& g, const Met& m, const AdvArgs& a, int i, int j, int k, double fx0,
-                                  double fx1, double fy0, double fy1, double fz0, double fz1) {
-        a.out[IX(i, j, k)] = a.V[IX(i, j, k)] * a.RHQ[IX(i, j, k)] - (fx1 - fx0) / m(TT_M_VOLXA, i) -
-                             (fy1 - fy0) / m(TT_M_VOLYB, j) - (fz1 - fz0) / m(TT_M_VOLZB, k);
+    __device__ static double fz(const AdvArgs& a, int n, Step s, const XF& z, int, double p1, double p0) {
+        return fyz(a, a.H, a.FZ, n, s.z, z.f, p1, p0);
+    }
+    __device__ static void finish(const AdvArgs& a, int n, double vx, double vy, const PM& pm, double fx0, double fx1,
+                                  double fy0, double fy1, double fz0, double fz1) {
+        a.out[n] = a.V[n] * a.RHQ[n] - (fx1 - fx0) / vx - (fy1 - fy0) / vy - (fz1 - fz0) / pm.volz;
     }
 };
 
@@ -276,62 +288,61 @@ struct AdvV {
 // copy of FLY(ny)
 struct AdvG {
     static constexpr int ox = 0, oy = -1, oz = 0;
-    __device__ static int ilo(const Geo&) { return 1; }
-    __device__ static int ihi(const Geo& g) { return g.nx; }
-    __device__ static int jlo(const Geo&) { return 1; }
-    __device__ static int jhi(const Geo& g) { return g.ny; }
-    __device__ static int klo(const Geo&) { return 1; }
-    __device__ static int khi(const Geo& g) { return g.nz; }
-    __device__ static double fx(const Geo& g, const Met& m, const AdvArgs& a, int I, int j, int k) {
+    static constexpr int ilo = 1, jlo = 1, klo = 1;
+    struct XF {
+        FaceMet f;
+        double p1, p0;  // x and z faces: DDY1, DDY0 of the row; y faces: DCY0, DCY1 of J+1
+    };
+    __device__ static XF xmet(const Met& m, int I, int j) {
+        return XF{fm_b(m, TT_M_DXB, TT_M_XXB, TT_M_XXA, I), m(TT_M_DDY1, j), m(TT_M_DDY0, j)};
+    }
+    __device__ static XF ymet(const Met& m, int J, int) {
+        return XF{fm_a(m, TT_M_DYA, TT_M_XYA, TT_M_XYB, J), m(TT_M_DCY0, J + 1), m(TT_M_DCY1, J + 1)};
+    }
+    __device__ static XF zmet(const Met& m, int K) { return XF{fm_b(m, TT_M_DZB, TT_M_XZB, TT_M_XZA, K), 0.0, 0.0}; }
+    struct PM {
+        double volz;
+    };
+    __device__ static PM pmet(const Met& m, int k) { return PM{m(TT_M_VOLZB, k)}; }
+    __device__ static double vol_x(const Met& m, int i) { return m(TT_M_VOLXB, i); }
+    __device__ static double vol_y(const Met& m, int j) { return m(TT_M_VOLYA, j); }
+    // x or z face at n along step s (s1: the step to j-1)
+    __device__ static double fxz(const AdvArgs& a, const double* __restrict__ vel, const double* __restrict__ fl,
+                                 int n, int s, int s1, const FaceMet& f, double p1, double p0) {
         const double half = 0.5;
-        Box3 D{a.A, g};
-        const double p1 = m(TT_M_DDY1, j), p0 = m(TT_M_DDY0, j);
-        const double u1 = a.V[IX(I, j, k)], u0 = a.V[IX(I, j - 1, k)];
-        const double f1 = a.FX[IX(I, j, k)], f0 = a.FX[IX(I, j - 1, k)];
-        const double q_2 = D(I - 2, j, k), q_1 = D(I - 1, j, k), q0 = D(I, j, k), q1 = D(I + 1, j, k);
-        const double d_1 = m(TT_M_DXB, I - 1), d0 = m(TT_M_DXB, I), d1 = m(TT_M_DXB, I + 1);
-        const double x_1 = m(TT_M_XXB, I - 1), x0 = m(TT_M_XXB, I), xa = m(TT_M_XXA, I);
+        const double* __restrict__ D = a.A;
+        const double u1 = vel[n], u0 = vel[n - s1];
+        const double f1 = fl[n], f0 = fl[n - s1];
+        const double q_2 = D[n - 2 * s], q_1 = D[n - s], q0 = D[n], q1 = D[n + s];
         double S = (p1 * u1 + p0 * u0) * a.dt;
         const bool up = S > 0.0;
         double qu;
-        double gr = vl_face(up, q_2, q_1, q0, q1, d_1, d0, d1, 1, &qu);
-        return (p1 * f1 + p0 * f0) * (qu + (xa - (up ? x_1 : x0) - half * S) * gr);
+        double gr = vl_face(up, q_2, q_1, q0, q1, f.d0, f.d1, f.d2, 1, &qu);
+        return (p1 * f1 + p0 * f0) * (qu + (f.xa - (up ? f.x0 : f.x1) - half * S) * gr);
     }
-    __device__ static double fy(const Geo& g, const Met& m, const AdvArgs& a, int i, int J, int k) {
+    __device__ static double fx(const AdvArgs& a, int n, Step s, const XF& x, const PM&) {
+        return fxz(a, a.V, a.FX, n, 1, s.y, x.f, x.p1, x.p0);
+    }
+    __device__ static double fy(const AdvArgs& a, int n, Step s, const XF& y, const PM&) {
         const double half = 0.5;
-        Box3 D{a.A, g};
-        if (J == 0) J = g.ny;
-        const double c0 = m(TT_M_DCY0, J + 1), c1 = m(TT_M_DCY1, J + 1);
-        const double g0 = a.G[IX(i, J, k)], g1 = a.G[IX(i, J + 1, k)];
-        const double f0 = a.FY[IX(i, J, k)], f1 = a.FY[IX(i, J + 1, k)];
-        const double q_1 = D(i, J - 1, k), q0 = D(i, J, k), q1 = D(i, J + 1, k), q2 = D(i, J + 2, k);
-        const double d0 = m(TT_M_DYA, J), d1 = m(TT_M_DYA, J + 1), d2 = m(TT_M_DYA, J + 2);
-        const double y0 = m(TT_M_XYA, J), y1 = m(TT_M_XYA, J + 1), yb = m(TT_M_XYB, J);
+        const double* __restrict__ D = a.A;
+        const int sy = s.y;
+        const double c0 = y.p1, c1 = y.p0;
+        const double g0 = a.G[n], g1 = a.G[n + sy];
+        const double f0 = a.FY[n], f1 = a.FY[n + sy];
+        const double q_1 = D[n - sy], q0 = D[n], q1 = D[n + sy], q2 = D[n + 2 * sy];
         double S = (c0 * g0 + c1 * g1) * a.dt;
         const bool up = S > 0.0;
         double qu;
-        double gr = vl_face(up, q_1, q0, q1, q2, d0, d1, d2, 1, &qu);
-        return (c0 * f0 + c1 * f1) * (qu + (yb - (up ? y0 : y1) - half * S) * gr);
+        double gr = vl_face(up, q_1, q0, q1, q2, y.f.d0, y.f.d1, y.f.d2, 1, &qu);
+        return (c0 * f0 + c1 * f1) * (qu + (y.f.xa - (up ? y.f.x0 : y.f.x1) - half * S) * gr);
     }
-    __device__ static double fz(const Geo& g, const Met& m, const AdvArgs& a, int i, int j, int K) {
-        const double half = 0.5;
-        Box3 D{a.A, g};
-        const double p1 = m(TT_M_DDY1, j), p0 = m(TT_M_DDY0, j);
-        const double h1 = a.H[IX(i, j, K)], h0 = a.H[IX(i, j - 1, K)];
-        const double f1 = a.FZ[IX(i, j, K)], f0 = a.FZ[IX(i, j - 1, K)];
-        const double q_2 = D(i, j, K - 2), q_1 = D(i, j, K - 1), q0 = D(i, j, K), q1 = D(i, j, K + 1);
-        const double d_1 = m(TT_M_DZB, K - 1), d0 = m(TT_M_DZB, K), d1 = m(TT_M_DZB, K + 1);
-        const double x_1 = m(TT_M_XZB, K - 1), x0 = m(TT_M_XZB, K), xa = m(TT_M_XZA, K);
-        double S = (p1 * h1 + p0 * h0) * a.dt;
-        const bool up = S > 0.0;
-        double qu;
-        double gr = vl_face(up, q_2, q_1, q0, q1, d_1, d0, d1, 1, &qu);
-        return (p1 * f1 + p0 * f0) * (qu + (xa - (up ? x_1 : x0) - half * S) * gr);
+    __device__ static double fz(const AdvArgs& a, int n, Step s, const XF& z, int, double p1, double p0) {
+        return fxz(a, a.H, a.FZ, n, s.z, s.y, z.f, p1, p0);
     }
-    __device__ static void finish(const Geo& g, const Met& m, const AdvArgs& a, int i, int j, int k, double fx0,
-                                  double fx1, double fy0, double fy1, double fz0, double fz1) {
-        a.out[IX(i, j, k)] = a.A[IX(i, j, k)] * a.RHQ[IX(i, j, k)] - (fx1 - fx0) / m(TT_M_VOLXB, i) -
-                             (fy1 - fy0) / m(TT_M_VOLYA, j) - (fz1 - fz0) / m(TT_M_VOLZB, k);
+    __device__ static void finish(const AdvArgs& a, int n, double vx, double vy, const PM& pm, double fx0, double fx1,
+                                  double fy0, double fy1, double fz0, double fz1) {
+        a.out[n] = a.A[n] * a.RHQ[n] - (fx1 - fx0) / vx - (fy1 - fy0) / vy - (fz1 - fz0) / pm.volz;
     }
 };
 
@@ -339,122 +350,167 @@ struct AdvG {
 // z faces FLZ(K) at K = k-1, k (upwind K for a positive S, else K+1)
 struct AdvH {
     static constexpr int ox = 0, oy = 0, oz = -1;
-    __device__ static int ilo(const Geo&) { return 1; }
-    __device__ static int ihi(const Geo& g) { return g.nx; }
-    __device__ static int jlo(const Geo&) { return 1; }
-    __device__ static int jhi(const Geo& g) { return g.ny; }
-    __device__ static int klo(const Geo&) { return 2; }
-    __device__ static int khi(const Geo& g) { return g.nz; }
-    __device__ static double fx(const Geo& g, const Met& m, const AdvArgs& a, int I, int j, int k) {
+    static constexpr int ilo = 1, jlo = 1, klo = 2;
+    struct XF {
+        FaceMet f;
+        double p1, p0;  // unused: the x and y faces take DDZ1, DDZ0 of the plane
+    };
+    __device__ static XF xmet(const Met& m, int I, int) {
+        return XF{fm_b(m, TT_M_DXB, TT_M_XXB, TT_M_XXA, I), 0.0, 0.0};
+    }
+    __device__ static XF ymet(const Met& m, int J, int) {
+        return XF{fm_b(m, TT_M_DYB, TT_M_XYB, TT_M_XYA, J), 0.0, 0.0};
+    }
+    __device__ static XF zmet(const Met& m, int K) { return XF{fm_a(m, TT_M_DZA, TT_M_XZA, TT_M_XZB, K), 0.0, 0.0}; }
+    struct PM {
+        double volz, p1, p0;
+    };
+    __device__ static PM pmet(const Met& m, int k) { return PM{m(TT_M_VOLZA, k), m(TT_M_DDZ1, k), m(TT_M_DDZ0, k)}; }
+    __device__ static double vol_x(const Met& m, int i) { return m(TT_M_VOLXB, i); }
+    __device__ static double vol_y(const Met& m, int j) { return m(TT_M_VOLYB, j); }
+    // x or y face at n along step s, with the plane factors (k-1: step sz)
+    __device__ static double fxy(const AdvArgs& a, const double* __restrict__ vel, const double* __restrict__ fl,
+                                 int n, int s, int sz, const FaceMet& f, double p1, double p0) {
         const double half = 0.5;
-        Box3 D{a.A, g};
-        const double p1 = m(TT_M_DDZ1, k), p0 = m(TT_M_DDZ0, k);
-        const double u1 = a.V[IX(I, j, k)], u0 = a.V[IX(I, j, k - 1)];
-        const double f1 = a.FX[IX(I, j, k)], f0 = a.FX[IX(I, j, k - 1)];
-        const double q_2 = D(I - 2, j, k), q_1 = D(I - 1, j, k), q0 = D(I, j, k), q1 = D(I + 1, j, k);
-        const double d_1 = m(TT_M_DXB, I - 1), d0 = m(TT_M_DXB, I), d1 = m(TT_M_DXB, I + 1);
-        const double x_1 = m(TT_M_XXB, I - 1), x0 = m(TT_M_XXB, I), xa = m(TT_M_XXA, I);
+        const double* __restrict__ D = a.A;
+        const double u1 = vel[n], u0 = vel[n - sz];
+        const double f1 = fl[n], f0 = fl[n - sz];
+        const double q_2 = D[n - 2 * s], q_1 = D[n - s], q0 = D[n], q1 = D[n + s];
         double S = (p1 * u1 + p0 * u0) * a.dt;
         const bool up = S > 0.0;
         double qu;
-        double gr = vl_face(up, q_2, q_1, q0, q1, d_1, d0, d1, 1, &qu);
-        return (p1 * f1 + p0 * f0) * (qu + (xa - (up ? x_1 : x0) - half * S) * gr);
+        double gr = vl_face(up, q_2, q_1, q0, q1, f.d0, f.d1, f.d2, 1, &qu);
+        return (p1 * f1 + p0 * f0) * (qu + (f.xa - (up ? f.x0 : f.x1) - half * S) * gr);
     }
-    __device__ static double fy(const Geo& g, const Met& m, const AdvArgs& a, int i, int J, int k) {
-        const double half = 0.5;
-        Box3 D{a.A, g};
-        const double p1 = m(TT_M_DDZ1, k), p0 = m(TT_M_DDZ0, k);
-        const double g1 = a.G[IX(i, J, k)], g0 = a.G[IX(i, J, k - 1)];
-        const double f1 = a.FY[IX(i, J, k)], f0 = a.FY[IX(i, J, k - 1)];
-        const double q_2 = D(i, J - 2, k), q_1 = D(i, J - 1, k), q0 = D(i, J, k), q1 = D(i, J + 1, k);
-        const double d_1 = m(TT_M_DYB, J - 1), d0 = m(TT_M_DYB, J), d1 = m(TT_M_DYB, J + 1);
-        const double x_1 = m(TT_M_XYB, J - 1), x0 = m(TT_M_XYB, J), xa = m(TT_M_XYA, J);
-        double S = (p1 * g1 + p0 * g0) * a.dt;
-        const bool up = S > 0.0;
-        double qu;
-        double gr = vl_face(up, q_2, q_1, q0, q1, d_1, d0, d1, 1, &qu);
-        return (p1 * f1 + p0 * f0) * (qu + (xa - (up ? x_1 : x0) - half * S) * gr);
+    __device__ static double fx(const AdvArgs& a, int n, Step s, const XF& x, const PM& pm) {
+        return fxy(a, a.V, a.FX, n, 1, s.z, x.f, pm.p1, pm.p0);
     }
-    __device__ static double fz(const Geo& g, const Met& m, const AdvArgs& a, int i, int j, int K) {
+    __device__ static double fy(const AdvArgs& a, int n, Step s, const XF& y, const PM& pm) {
+        return fxy(a, a.G, a.FY, n, s.y, s.z, y.f, pm.p1, pm.p0);
+    }
+    __device__ static double fz(const AdvArgs& a, int n, Step s, const XF& z, int, double, double) {
         const double half = 0.5;
-        Box3 D{a.A, g};
-        const double h0 = a.H[IX(i, j, K)], h1 = a.H[IX(i, j, K + 1)];
-        const double f0 = a.FZ[IX(i, j, K)], f1 = a.FZ[IX(i, j, K + 1)];
-        const double q_1 = D(i, j, K - 1), q0 = D(i, j, K), q1 = D(i, j, K + 1), q2 = D(i, j, K + 2);
-        const double d0 = m(TT_M_DZA, K), d1 = m(TT_M_DZA, K + 1), d2 = m(TT_M_DZA, K + 2);
-        const double z0 = m(TT_M_XZA, K), z1 = m(TT_M_XZA, K + 1), zb = m(TT_M_XZB, K);
+        const double* __restrict__ D = a.A;
+        const int sz = s.z;
+        const double h0 = a.H[n], h1 = a.H[n + sz];
+        const double f0 = a.FZ[n], f1 = a.FZ[n + sz];
+        const double q_1 = D[n - sz], q0 = D[n], q1 = D[n + sz], q2 = D[n + 2 * sz];
         double S = half * (h0 + h1) * a.dt;
         const bool up = S > 0.0;
         double qu;
-        double gr = vl_face(up, q_1, q0, q1, q2, d0, d1, d2, 1, &qu);
-        return half * (f0 + f1) * (qu + (zb - (up ? z0 : z1) - half * S) * gr);
+        double gr = vl_face(up, q_1, q0, q1, q2, z.f.d0, z.f.d1, z.f.d2, 1, &qu);
+        return half * (f0 + f1) * (qu + (z.f.xa - (up ? z.f.x0 : z.f.x1) - half * S) * gr);
     }
-    __device__ static void finish(const Geo& g, const Met& m, const AdvArgs& a, int i, int j, int k, double fx0,
-                                  double fx1, double fy0, double fy1, double fz0, double fz1) {
-        a.out[IX(i, j, k)] = a.A[IX(i, j, k)] * a.RHQ[IX(i, j, k)] - (fx1 - fx0) / m(TT_M_VOLXB, i) -
-                             (fy1 - fy0) / m(TT_M_VOLYB, j) - (fz1 - fz0) / m(TT_M_VOLZA, k);
+    __device__ static void finish(const AdvArgs& a, int n, double vx, double vy, const PM& pm, double fx0, double fx1,
+                                  double fy0, double fy1, double fz0, double fz1) {
+        a.out[n] = a.A[n] * a.RHQ[n] - (fx1 - fx0) / vx - (fy1 - fy0) / vy - (fz1 - fz0) / pm.volz;
     }
 };
 
-template <class K>
-__global__ __launch_bounds__(320) void k_adv_tiled(Geo g, Met m, AdvArgs a) {
+// the z face's column factors, by kind (V: DDX of the cell's i; G: DDY of
+// its j; E, H: none)
+template <class K> __device__ inline void zcol(const Met&, int, int, double& p1, double& p0) { p1 = p0 = 0.0; }
+template <> __device__ inline void zcol<AdvV>(const Met& m, int i, int, double& p1, double& p0) {
+    p1 = m(TT_M_DDX1, i);
+    p0 = m(TT_M_DDX0, i);
+}
+template <> __device__ inline void zcol<AdvG>(const Met& m, int, int j, double& p1, double& p0) {
+    p1 = m(TT_M_DDY1, j);
+    p0 = m(TT_M_DDY0, j);
+}
+
+template <class K, int KC>
+__global__ __launch_bounds__(320) void k_adv_t(Geo g, Met m, AdvArgs a) {
     __shared__ double sx[2][ADV_TJ][ADV_TI + 1];  // x faces of the tile rows, plus the high one
     __shared__ double sy[2][ADV_TJ + 1][ADV_TI];  // y faces of the tile columns, plus the high row
-    const int ilo = K::ilo(g), ihi = K::ihi(g), jlo = K::jlo(g), jhi = K::jhi(g);
-    const int klo = K::klo(g), khi = K::khi(g);
+    const int ilo = K::ilo, ihi = g.nx, jlo = K::jlo, jhi = g.ny, klo = K::klo, khi = g.nz;
     const int nti = (ihi - ilo + ADV_TI) / ADV_TI, ntj = (jhi - jlo + ADV_TJ) / ADV_TJ;
     int b = blockIdx.x;
     const int ti = b % nti;
     b /= nti;
     const int tj = b % ntj, kc = b / ntj;
     const int ci0 = ilo + ti * ADV_TI, cj0 = jlo + tj * ADV_TJ;
-    const int k0 = klo + kc * ADV_KC;
-    const int k1 = min(khi, k0 + ADV_KC - 1);
+    const int k0 = klo + kc * KC;
+    const int k1 = min(khi, k0 + KC - 1);
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
     const bool helper = w == 4;
     const int r = 2 * w + (l >> 5), c = l & 31;
-    const int i = ci0 + c, j = cj0 + r;
-    const bool vi = i <= ihi, vj = j <= jhi;
-    const bool cell = !helper && vi && vj;
     // the highest face of each family
     const int Ihi = ihi + K::ox + 1, Jhi = jhi + K::oy + 1;
+    const Step st{g.gx, g.gx * g.gy};
+    // the lane's cell (i, j) and faces: cell lanes own a cell; the helper's
+    // lanes 0..7 the tile's high x faces (row cj0 + l), lanes 8..39 its high
+    // y faces (column ci0 + l - 8)
+    int i, j;
+    bool dox, doy, cell;
+    if (!helper) {
+        i = ci0 + c;
+        j = cj0 + r;
+        dox = j <= jhi && i + K::ox <= Ihi;
+        doy = i <= ihi && j + K::oy <= Jhi;
+        cell = i <= ihi && j <= jhi;
+    } else if (l < ADV_TJ) {
+        i = ci0 + ADV_TI;
+        j = cj0 + l;
+        dox = j <= jhi && i + K::ox <= Ihi;
+        doy = cell = false;
+    } else if (l < ADV_TJ + ADV_TI) {
+        i = ci0 + (l - ADV_TJ);
+        j = cj0 + ADV_TJ;
+        doy = i <= ihi && j + K::oy <= Jhi;
+        dox = cell = false;
+    } else {
+        i = ci0;
+        j = cj0;
+        dox = doy = cell = false;
+    }
+    // faces and metrics of lanes without work are clamped into the box
+    const int ic = min(i, ihi + 1), jc = min(j, jhi + 1);
+    int Jf = jc + K::oy;
+    if (Jf == 0) Jf = g.ny;  // FLY(0) is the periodic copy of FLY(ny) (AdvG)
+    const typename K::XF xm = K::xmet(m, ic + K::ox, jc);
+    const typename K::XF ym = K::ymet(m, Jf, ic);
+    const double volx = K::vol_x(m, min(i, ihi)), voly = K::vol_y(m, min(j, jhi));
+    double zp1, zp0;
+    zcol<K>(m, ic, jc, zp1, zp0);
+    int n = (int)g.idx(ic, jc, k0);
+    const int ny_wrap = (jc + K::oy == 0) ? g.ny * st.y : 0;  // the y face's index shift for the wrap
     double zlo = 0.0;
+    if (cell) zlo = K::fz(a, n + K::oz * st.z, st, K::zmet(m, k0 + K::oz), k0 + K::oz, zp1, zp0);
     int p = 0;
-    for (int k = k0; k <= k1; ++k, p ^= 1) {
+    for (int k = k0; k <= k1; ++k, p ^= 1, n += st.z) {
+        const typename K::PM pm = K::pmet(m, k);
         double zhi = 0.0;
-        if (!helper) {
-            // the low x face of the lane's cell is also the high face of the
-            // cell before it, and of the last valid cell when this one is
-            // past the end of the row
-            if (vj && i + K::ox <= Ihi) sx[p][r][c] = K::fx(g, m, a, i + K::ox, j, k);
-            if (vi && j + K::oy <= Jhi) sy[p][r][c] = K::fy(g, m, a, i, j + K::oy, k);
-            if (cell) {
-                zhi = K::fz(g, m, a, i, j, k + K::oz + 1);
-                if (k == k0) zlo = K::fz(g, m, a, i, j, k + K::oz);
-            }
-        } else if (l < ADV_TJ) {
-            const int jr = cj0 + l, I = ci0 + ADV_TI + K::ox;
-            if (jr <= jhi && I <= Ihi) sx[p][l][ADV_TI] = K::fx(g, m, a, I, jr, k);
-        } else if (l < ADV_TJ + ADV_TI) {
-            const int ic = ci0 + (l - ADV_TJ), J = cj0 + ADV_TJ + K::oy;
-            if (ic <= ihi && J <= Jhi) sy[p][ADV_TJ][l - ADV_TJ] = K::fy(g, m, a, ic, J, k);
+        if (dox) {
+            const double fx = K::fx(a, n + K::ox, st, xm, pm);
+            if (!helper) sx[p][r][c] = fx;
+            else sx[p][l][ADV_TI] = fx;
         }
+        if (doy) {
+            const double fy = K::fy(a, n + K::oy * st.y + ny_wrap, st, ym, pm);
+            if (!helper) sy[p][r][c] = fy;
+            else sy[p][ADV_TJ][l - ADV_TJ] = fy;
+        }
+        if (cell) zhi = K::fz(a, n + (K::oz + 1) * st.z, st, K::zmet(m, k + K::oz + 1), k + K::oz + 1, zp1, zp0);
         __syncthreads();
         if (cell) {
-            K::finish(g, m, a, i, j, k, sx[p][r][c], sx[p][r][c + 1], sy[p][r][c], sy[p][r + 1][c], zlo, zhi);
+            K::finish(a, n, volx, voly, pm, sx[p][r][c], sx[p][r][c + 1], sy[p][r][c], sy[p][r + 1][c], zlo, zhi);
             zlo = zhi;
         }
     }
 }
 
+// planes per block: at 128x128x256 the four kernels take 561, 535 and 544 us
+// together with 2, 4 and 8
+constexpr int ADV_KC = 4;
+
 template <class K>
-static int launch_adv_tiled(const AdvArgs& a) {
+static int launch_adv(const AdvArgs& a) {
     Geo& g = L.g;
-    // host copies of the cell ranges (the device functions above)
-    const int ni = K::ox == -1 ? g.nx - 1 : g.nx, nj = g.ny, nk = K::oz == -1 ? g.nz - 1 : g.nz;
+    const int ni = g.nx - K::ilo + 1, nj = g.ny - K::jlo + 1, nk = g.nz - K::klo + 1;
     const long long nb = (long long)((ni + ADV_TI - 1) / ADV_TI) * ((nj + ADV_TJ - 1) / ADV_TJ) *
                          ((nk + ADV_KC - 1) / ADV_KC);
-    k_adv_tiled<K><<<(unsigned)nb, 320, 0, L.s>>>(g, met(), a);
+    k_adv_t<K, ADV_KC><<<(unsigned)nb, 320, 0, L.s>>>(g, met(), a);
     TT_LAUNCH();
     return 0;
 }
@@ -506,17 +562,17 @@ int advect_impl(double dt) {
     TT_LAUNCH();
     AdvArgs aa{L.TOLD, f[TT_V], f[TT_G], f[TT_H], L.FLMX, L.FLMY, L.FLMZ, nullptr, L.ED, dt,
                f[TT_CONV_FLUX1], f[TT_CONV_FLUX2], f[TT_CONV_FLUX3]};
-    if ((rc = launch_adv_tiled<AdvE>(aa))) return rc;
+    if ((rc = launch_adv<AdvE>(aa))) return rc;
     aa.A = f[TT_V]; aa.RHQ = f[TT_RHQX]; aa.out = L.VD;
-    if ((rc = launch_adv_tiled<AdvV>(aa))) return rc;
+    if ((rc = launch_adv<AdvV>(aa))) return rc;
     k_gd_init<<<nblk(g.n3, 256), 256, 0, L.s>>>(g, met(), L.p.omrot, f[TT_G], f[TT_GD]);
     TT_LAUNCH();
     aa.A = f[TT_GD]; aa.RHQ = f[TT_RHQY]; aa.out = L.GDN;
-    if ((rc = launch_adv_tiled<AdvG>(aa))) return rc;
+    if ((rc = launch_adv<AdvG>(aa))) return rc;
     k_hd_init<<<nblk(g.n3, 256), 256, 0, L.s>>>(g, met(), f[TT_H], L.HH);
     TT_LAUNCH();
     aa.A = L.HH; aa.RHQ = f[TT_RHQZ]; aa.out = L.HDN;
-    if ((rc = launch_adv_tiled<AdvH>(aa))) return rc;
+    if ((rc = launch_adv<AdvH>(aa))) return rc;
     k_adv_rho<<<nblk(nc, 256), 256, 0, L.s>>>(g, met(), f[TT_RH], L.FLMX, L.FLMY, L.FLMZ);
     TT_LAUNCH();
     if ((rc = bounds_impl())) return rc;
