@@ -112,6 +112,7 @@ struct Lib {
     int npartial = 0;
     Ctrl* ctrl = nullptr;           // device
     Ctrl* hctrl = nullptr;          // pinned host mirror
+    Ctrl* hctrl_d = nullptr;        // its device-side address (kernels write it directly)
     tt_solver_opts opts{};
     tt_limits lim{};                // clamps applied by tt_step
     // learned guess: consecutive rejections, and first solves since the
@@ -264,5 +265,9 @@ int artvis_impl(double dt);       // NR_ARTIFICIAL_VISC, ppfor.f:1032-1230
 double dt_artvis();               // dt_artvis after the last artvis_impl (host; syncs)
 int sphericalvellimit_impl(int ivel, double velmax, double hydrotime);
 int sync_check();
+// The control block into its pinned host mirror (a one-wavefront kernel that
+// writes the mapped host memory, then a stream sync): no copy engine or blit
+// kernel per read.
+int ctrl_to_host();
 
 }  // namespace tt

@@ -1221,7 +1221,10 @@ int energy_impl(double delt, int iii, tt_energy_stats* st) {
         if (ss.guess_used == 2) es.learned_used++;
         if (L.have_prev_guess == 2) L.learned_rejects = ss.guess_used == 2 ? 0 : L.learned_rejects + 1;
         TT_HIP(hipEventRecord(ev[2], L.s));
-        int nb = 1024;
+        // 4 blocks per CU keep the grid-stride loop's loads in flight (max
+        // reductions: the result does not depend on the block count)
+        const long long ncell = (long long)g.nx * g.ny * g.nz;
+        int nb = (int)std::min<long long>(4096, (ncell + 255) / 256);
         UpdArgs ua{f[TT_DELTA_ER], f[TT_EVAL1], f[TT_EVAL2], f[TT_EVAL3], f[TT_EVAL4], f[TT_EVAL5], f[TT_EVAL6],
                    f[TT_EVAL_1_6], L.DFQ, f[TT_FQ], f[TT_RH], f[TT_XKAPP], f[TT_Q], f[TT_ER], L.ub, delt, conv_parm,
                    L.p.bcxmin == 12 ? 1 : 2};

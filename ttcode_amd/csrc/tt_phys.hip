@@ -391,8 +391,8 @@ __global__ void k_dif_matrix(Geo g, Met m, double delt, double c, const double* 
 }
 
 int sync_check() {
-    TT_HIP(hipMemcpyAsync(L.hctrl, L.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, L.s));
-    TT_HIP(hipStreamSynchronize(L.s));
+    int rc0 = ctrl_to_host();
+    if (rc0) return rc0;
     if (L.hctrl->bad_bound) return fail(3, "error in boundxmax: upperbnd above the domain");
     if (L.hctrl->bad_q) return fail(2, "ERROR IN DELTA_Q: non-positive value of Q");
     if (L.hctrl->bad_kappa)
@@ -653,8 +653,8 @@ extern "C" int tt_energy_totals(double* out3) {
     TT_LAUNCH();
     int rc = fold_partials(L.partial, nb, 3, 0, &L.ctrl->scratch[5], &L.ctrl->scratch[6], &L.ctrl->scratch[7]);
     if (rc) return rc;
-    TT_HIP(hipMemcpyAsync(L.hctrl, L.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, L.s));
-    TT_HIP(hipStreamSynchronize(L.s));
+    rc = ctrl_to_host();
+    if (rc) return rc;
     out3[0] = L.hctrl->scratch[5];
     out3[1] = L.hctrl->scratch[6];
     out3[2] = L.hctrl->scratch[7];

@@ -70,8 +70,8 @@ static long long padded_len() { return (long long)g_cs.px * g_cs.ny * (g_cs.nz +
 // Radial-line inner solves (tt_solver_opts.inner = 1).  Line (j,k) is the
 // whole radial column i = 1..nx; lines are coloured by (j+k) parity, so a
 // line's phi and theta neighbours are lines of the other colour and the
-// lines of one colour are solved independently.  Row r = (j-1) + ny (k-1)
-// of g_lco holds eight FP32 vectors of nx values: the couplings D, E (phi
+// lines of one colour are solved independently.  Row line_row(j-1, k-1) of
+// g_lco (below) holds eight FP32 vectors of nx values: the couplings D, E (phi
 // -/+), F, G (theta -/+), then the LU factors of the line's tridiagonal
 // part, m (the multiplier of row i-1) and u (the pivot), the super-diagonal
 // C, and the right-hand side.  The unknowns are rows of nx values over
@@ -81,9 +81,22 @@ struct LS {
     int nx, ny, nz;
 };
 static LS g_ls;
+
+// Row of line (jj, kk) (0-based j-1, k-1) in the line system: the lines of
+// each colour (j+k parity) are stored together, colour 0 first, in (k, j)
+// order within it, so that a half sweep streams one contiguous block.
+__host__ __device__ inline long long line_row(int jj, int kk, int ny, int nz) {
+    const int hy = ny >> 1;
+    return (long long)((jj + kk) & 1) * hy * nz + (long long)kk * hy + (jj >> 1);
+}
 static float* g_lco = nullptr;
 static float* g_lx = nullptr;
 static int g_lchunk = 0;  // lines of a colour per logical block
+static int g_lnb = 0;     // line-sweep blocks
+// lines per wavefront in a line half sweep (2, 4 and 8 measure the same,
+// 15.2-15.3 us at 128x256x128)
+constexpr int LINE_LPW = 4;
+static int g_pn = 0;      // partials per colour of the sweeps being queued (g_nb or g_lnb)
 static int g_line = 0;    // FP32 sweeps launch the line kernel
 static bool g_line_ok = false;  // g_lco's factors match the FP64 system
 static long long line_x_len() { return (long long)g_ls.nx * g_ls.ny * (g_ls.nz + 2); }
@@ -149,6 +162,7 @@ int sor_alloc() {
     // half of the partial buffer; the fold reads at most 16 per thread of 256
     if (4 * nb > L.npartial || 2 * nb > 16 * 256) return fail(1, "SOR: partial buffer too small");
     g_nb = nb;
+    g_pn = nb;
     g_chunk = (nrows + nb - 1) / nb;  // rows per block
     return 0;
 }
@@ -164,7 +178,12 @@ static int line_alloc() {
     TT_HIP(dev_alloc(&g_lx, line_x_len() * sizeof(float)));
     TT_HIP(hipMemsetAsync(g_lx, 0, line_x_len() * sizeof(float), L.s));
     const int nl = (g_ls.ny / 2) * g_ls.nz;  // lines of a colour
-    g_lchunk = (nl + g_nb - 1) / g_nb;
+    g_lchunk = 4 * LINE_LPW;
+    g_lnb = (nl + g_lchunk - 1) / g_lchunk;
+    if (g_lnb > g_nb) {  // the partial slots hold g_nb per colour
+        g_lnb = g_nb;
+        g_lchunk = (nl + g_nb - 1) / g_nb;
+    }
     g_line_ok = false;
     return 0;
 }
@@ -264,7 +283,8 @@ __global__ __launch_bounds__(256) void k_sweep(int c, const T* __restrict__ coef
     const int done = ctrl->done;
     int lb = xcd_block(nb);
     if (rev) lb = nb - 1 - lb;
-    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    // the wavefront index, uniform: the row arithmetic runs on the scalar unit
+    const int tx = threadIdx.x & 63, ty = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int hx = cs.hx, ny = cs.ny, px = cs.px;
     const int nrows = ny * cs.nz;
     const int r0 = lb * rows_per_block;
@@ -319,16 +339,22 @@ __global__ __launch_bounds__(256) void k_sweep_f32x2(int c, const float* __restr
     const int done = ctrl->done;
     int lb = xcd_block(nb);
     if (rev) lb = nb - 1 - lb;
-    const int lane = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    // the wavefront's row pair starts at a uniform row re (scalar arithmetic);
+    // the upper half-wavefront takes re + 1
+    const int lane = threadIdx.x & 63, ty = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int half = lane >> 5, tp = lane & 31;
     const int hx = cs.hx, hp = hx >> 1, ny = cs.ny, px = cs.px;
     const int nrows = ny * cs.nz;
     const int r0 = lb * rows_per_block;
     const int r1 = min(r0 + rows_per_block, nrows);
     double acc = 0.0;
-    for (int r = done ? r1 : r0 + 2 * ty + half; r < r1; r += 8) {
-        const int kk = r / ny;
-        const int jj = r - kk * ny;
+    for (int re = done ? r1 : r0 + 2 * ty; re < r1; re += 8) {
+        const int r = re + half;
+        if (r >= r1) continue;
+        const int ke = re / ny, je = re - ke * ny;
+        const bool wrap = je + half == ny;
+        const int kk = wrap ? ke + 1 : ke;
+        const int jj = wrap ? 0 : je + half;
         const int s = (jj + kk + c + 1) & 1;
         const int jm = jj == 0 ? ny - 1 : jj - 1;
         const int jp = jj == ny - 1 ? 0 : jj + 1;
@@ -473,7 +499,7 @@ __device__ inline void line_load(LineOps<Q>& o, const float* __restrict__ coef, 
     const int jj = 2 * (q - kk * hy) + ((kk + c) & 1);  // (jj + kk) & 1 == c
     const int jm = jj == 0 ? ny - 1 : jj - 1;
     const int jp = jj == ny - 1 ? 0 : jj + 1;
-    const float* __restrict__ cr = coef + (long long)(jj + ny * kk) * 8 * nx + i0;
+    const float* __restrict__ cr = coef + ((long long)c * hy * ls.nz + q) * 8 * nx + i0;  // line_row(jj, kk)
     load_cells<Q, FULL>(o.cD, cr, i0, nx, 0.f);
     load_cells<Q, FULL>(o.cE, cr + nx, i0, nx, 0.f);
     load_cells<Q, FULL>(o.cF, cr + 2 * nx, i0, nx, 0.f);
@@ -489,6 +515,71 @@ __device__ inline void line_load(LineOps<Q>& o, const float* __restrict__ coef, 
     load_cells<Q, FULL>(o.n4, x + (long long)(jj + ny * (kk + 2)) * nx + i0, i0, nx, 0.f);
 }
 
+// One line's solve for a lane (see k_line_sweep): returns the lane's part
+// of the residual sum and leaves the new unknowns in y.
+template <int Q>
+__device__ inline float line_solve(const LineOps<Q>& o, float omega, int lane, float (&y)[Q]) {
+    float f[Q], g[Q];
+#pragma unroll
+    for (int t = 0; t < Q; ++t)
+        f[t] = o.cB[t] - o.cD[t] * o.n1[t] - o.cE[t] * o.n2[t] - o.cF[t] * o.n3[t] - o.cG[t] * o.n4[t];
+    // residual before the update: f - L (U x), (U x)_i = u_i x_i + C_i x_{i+1},
+    // (L w)_i = w_i + m_i w_{i-1}; the r ghosts are zero
+    const float xnext = dpp<0x130>(0.f, o.X[0]);
+#pragma unroll
+    for (int t = 0; t < Q; ++t) g[t] = o.cU[t] * o.X[t] + o.cC[t] * (t + 1 < Q ? o.X[t + 1] : xnext);
+    const float gprev = dpp<0x138>(0.f, g[Q - 1]);
+    float rs = 0.f;
+#pragma unroll
+    for (int t = 0; t < Q; ++t) rs += fabsf(f[t] - (g[t] + o.cM[t] * (t ? g[t - 1] : gprev)));
+    // forward substitution z_i = f_i - m_i z_{i-1}: the lane's map
+    // z_out = pa z_in + pb, composed with the lanes before it
+    float pa = 1.f, pb = 0.f;
+#pragma unroll
+    for (int t = 0; t < Q; ++t) {
+        pb = f[t] - o.cM[t] * pb;
+        pa = -o.cM[t] * pa;
+    }
+    scan_affine_up(pa, pb, lane);
+    float zin = dpp<0x138>(0.f, pb);  // z before this lane's first cell (0 before cell 1)
+    float ca[Q], cb[Q];
+#pragma unroll
+    for (int t = 0; t < Q; ++t) {
+        zin = f[t] - o.cM[t] * zin;
+        const float w = __builtin_amdgcn_rcpf(o.cU[t]);
+        ca[t] = -o.cC[t] * w;
+        cb[t] = zin * w;
+    }
+    // back substitution y_i = ca_i y_{i+1} + cb_i: the lane's map
+    // y_first = qa y_in + qb, composed with the lanes after it
+    float qa = 1.f, qb = 0.f;
+#pragma unroll
+    for (int t = Q - 1; t >= 0; --t) {
+        qb = ca[t] * qb + cb[t];
+        qa = ca[t] * qa;
+    }
+    scan_affine_down(qa, qb, lane);
+    float yin = dpp<0x130>(0.f, qb);  // y after this lane's last cell (0 after cell nx)
+#pragma unroll
+    for (int t = Q - 1; t >= 0; --t) {
+        yin = ca[t] * yin + cb[t];
+        y[t] = o.X[t] + omega * (yin - o.X[t]);
+    }
+    return rs;
+}
+
+template <int Q, bool FULL>
+__device__ inline void line_store(float* __restrict__ xw, const float (&y)[Q], int i0, int nx) {
+    if (FULL && Q % 2 == 0) {
+#pragma unroll
+        for (int t = 0; t < Q; t += 2) *(float2*)(xw + t) = make_float2(y[t], y[t + 1]);
+    } else {
+#pragma unroll
+        for (int t = 0; t < Q; ++t)
+            if (FULL || i0 + t < nx) xw[t] = y[t];
+    }
+}
+
 // One half sweep of red-black line SOR: every line of colour c (lines
 // (j,k) with (j+k) parity c) is solved exactly for the current values of its
 // four neighbour lines,
@@ -501,9 +592,9 @@ __device__ inline void line_load(LineOps<Q>& o, const float* __restrict__ coef, 
 // (forward over lanes, then backward).  The residual of the line before its
 // update, f - L U x, is summed in |.| for the convergence test, as the point
 // sweep sums its residuals.  Cells past nx act as identity rows with zero
-// right-hand side.  A wavefront's lines are software-pipelined: the next
-// line's operands load while the current one is solved (lines of one colour
-// never read each other).  Rows, blocks, the test block and the partials are
+// right-hand side.  A wavefront solves two of its lines at once (lines of
+// one colour never read each other), so that their two dependent chains of
+// lane moves overlap.  Rows, blocks, the test block and the partials are
 // laid out as in k_sweep.
 template <int Q, bool FULL>
 __global__ __launch_bounds__(256) void k_line_sweep(int c, const float* __restrict__ coef, float* __restrict__ x, LS ls,
@@ -517,79 +608,32 @@ __global__ __launch_bounds__(256) void k_line_sweep(int c, const float* __restri
     const int done = ctrl->done;
     int lb = xcd_block(nb);
     if (rev) lb = nb - 1 - lb;
-    const int lane = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    // the wavefront index, uniform: the line arithmetic and the loop run on
+    // the scalar unit
+    const int lane = threadIdx.x & 63, ty = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int nx = ls.nx, ny = ls.ny, hy = ny >> 1;
     const int nl = hy * ls.nz;
     const int q0 = lb * lines_per_block;
     const int q1 = min(q0 + lines_per_block, nl);
     const int i0 = lane * Q;
     double acc = 0.0;
-    int q = done ? q1 : q0 + ty;
-    LineOps<Q> o;
-    if (q < q1) line_load<Q, FULL>(o, coef, x, ls, c, q, i0);
-    for (; q < q1; q += 4) {
-        LineOps<Q> nxt;
-        if (q + 4 < q1) line_load<Q, FULL>(nxt, coef, x, ls, c, q + 4, i0);
-        const int kk = q / hy;
-        const int jj = 2 * (q - kk * hy) + ((kk + c) & 1);
-        float* __restrict__ xw = x + (long long)(jj + ny * (kk + 1)) * nx + i0;
-        float f[Q], g[Q];
-#pragma unroll
-        for (int t = 0; t < Q; ++t)
-            f[t] = o.cB[t] - o.cD[t] * o.n1[t] - o.cE[t] * o.n2[t] - o.cF[t] * o.n3[t] - o.cG[t] * o.n4[t];
-        // residual before the update: f - L (U x), (U x)_i = u_i x_i + C_i x_{i+1},
-        // (L w)_i = w_i + m_i w_{i-1}; the r ghosts are zero
-        const float xnext = dpp<0x130>(0.f, o.X[0]);
-#pragma unroll
-        for (int t = 0; t < Q; ++t) g[t] = o.cU[t] * o.X[t] + o.cC[t] * (t + 1 < Q ? o.X[t + 1] : xnext);
-        const float gprev = dpp<0x138>(0.f, g[Q - 1]);
-        float rs = 0.f;
-#pragma unroll
-        for (int t = 0; t < Q; ++t) rs += fabsf(f[t] - (g[t] + o.cM[t] * (t ? g[t - 1] : gprev)));
-        acc += (double)rs;
-        // forward substitution z_i = f_i - m_i z_{i-1}: the lane's map
-        // z_out = pa z_in + pb, composed with the lanes before it
-        float pa = 1.f, pb = 0.f;
-#pragma unroll
-        for (int t = 0; t < Q; ++t) {
-            pb = f[t] - o.cM[t] * pb;
-            pa = -o.cM[t] * pa;
+    for (int q = done ? q1 : q0 + ty; q < q1; q += 8) {
+        const bool two = q + 4 < q1;  // wavefront-uniform
+        const int qb = two ? q + 4 : q;
+        LineOps<Q> oa, ob;
+        line_load<Q, FULL>(oa, coef, x, ls, c, q, i0);
+        line_load<Q, FULL>(ob, coef, x, ls, c, qb, i0);
+        float ya[Q], yb[Q];
+        const float ra = line_solve<Q>(oa, omega, lane, ya);
+        const float rb = line_solve<Q>(ob, omega, lane, yb);
+        const int ka = q / hy, ja = 2 * (q - ka * hy) + ((ka + c) & 1);
+        line_store<Q, FULL>(x + (long long)(ja + ny * (ka + 1)) * nx + i0, ya, i0, nx);
+        acc += (double)ra;
+        if (two) {
+            const int kb = qb / hy, jb = 2 * (qb - kb * hy) + ((kb + c) & 1);
+            line_store<Q, FULL>(x + (long long)(jb + ny * (kb + 1)) * nx + i0, yb, i0, nx);
+            acc += (double)rb;
         }
-        scan_affine_up(pa, pb, lane);
-        float zin = dpp<0x138>(0.f, pb);  // z before this lane's first cell (0 before cell 1)
-        float ca[Q], cb[Q];
-#pragma unroll
-        for (int t = 0; t < Q; ++t) {
-            zin = f[t] - o.cM[t] * zin;
-            const float w = __builtin_amdgcn_rcpf(o.cU[t]);
-            ca[t] = -o.cC[t] * w;
-            cb[t] = zin * w;
-        }
-        // back substitution y_i = ca_i y_{i+1} + cb_i: the lane's map
-        // y_first = qa y_in + qb, composed with the lanes after it
-        float qa = 1.f, qb = 0.f;
-#pragma unroll
-        for (int t = Q - 1; t >= 0; --t) {
-            qb = ca[t] * qb + cb[t];
-            qa = ca[t] * qa;
-        }
-        scan_affine_down(qa, qb, lane);
-        float yin = dpp<0x130>(0.f, qb);  // y after this lane's last cell (0 after cell nx)
-        float y[Q];
-#pragma unroll
-        for (int t = Q - 1; t >= 0; --t) {
-            yin = ca[t] * yin + cb[t];
-            y[t] = o.X[t] + omega * (yin - o.X[t]);
-        }
-        if (FULL && Q % 2 == 0) {
-#pragma unroll
-            for (int t = 0; t < Q; t += 2) *(float2*)(xw + t) = make_float2(y[t], y[t + 1]);
-        } else {
-#pragma unroll
-            for (int t = 0; t < Q; ++t)
-                if (FULL || i0 + t < nx) xw[t] = y[t];
-        }
-        o = nxt;
     }
     double bs = block_sum256(acc, sh);
     if (threadIdx.x == 0 && !done) part[c * nb + lb] = bs;
@@ -646,7 +690,7 @@ template <int Q, bool FULL>
 __global__ __launch_bounds__(256) void k_line_factor(CS cs, LS ls, const double* __restrict__ C0,
                                                      const double* __restrict__ C1, float* __restrict__ lco) {
     const int lane = threadIdx.x & 63;
-    const int r = blockIdx.x * 4 + (threadIdx.x >> 6);  // line (jj, kk), r = jj + ny kk
+    const int r = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // line (jj, kk), r = jj + ny kk
     if (r >= ls.ny * ls.nz) return;                      // whole wavefronts leave together
     const int kk = r / ls.ny, jj = r - kk * ls.ny;
     const int hx = cs.hx, nx = ls.nx;
@@ -689,7 +733,7 @@ __global__ __launch_bounds__(256) void k_line_factor(CS cs, LS ls, const double*
     // the product over the cells before this lane's first (identity on lane 0)
     M2 cur = mdpp<0x138>(P);
     double uprev = cur.a / cur.c;  // u of the cell before (unused for cell 1)
-    float* __restrict__ o = lco + (long long)r * 8 * nx + i0;
+    float* __restrict__ o = lco + line_row(jj, kk, ls.ny, ls.nz) * 8 * nx + i0;
 #pragma unroll
     for (int t = 0; t < Q; ++t) {
         const double cprev = t ? vc[t - 1] : cbefore;
@@ -877,7 +921,7 @@ __global__ __launch_bounds__(256) void k_resid64(CS cs, const double* __restrict
         double res = q[7 * hx] - ax;
         acc += fabs(res);
         if (r32_0) (c ? r32_1 : r32_0)[(long long)rr * 8 * hx + 7 * hx + h] = (float)res;
-        if (rline) rline[((long long)rr * 8 + 7) * nx + 2 * h + s] = (float)res;  // cell i = 2h + 1 + s
+        if (rline) rline[(line_row(jj, kk, ny, cs.nz) * 8 + 7) * nx + 2 * h + s] = (float)res;  // cell i = 2h + 1 + s
     }
     double bs = block_sum256(acc, sh);
     if (threadIdx.x == 0) part[lb] = bs;
@@ -955,7 +999,7 @@ __global__ void k_rhs_f32(CS cs, const double* __restrict__ C0, const double* __
     const long long o = (long long)rr * 8 * hx + 7 * hx + h;
     const double v = (c ? C1 : C0)[o];
     if (r32_0) (c ? r32_1 : r32_0)[o] = (float)v;
-    if (rline) rline[((long long)rr * 8 + 7) * nx + 2 * h + ((jj + kk + c + 1) & 1)] = (float)v;
+    if (rline) rline[(line_row(jj, kk, cs.ny, cs.nz) * 8 + 7) * nx + 2 * h + ((jj + kk + c + 1) & 1)] = (float)v;
 }
 
 // x64 (padded, colour cc) += the line unknowns over the interior cells
@@ -1028,11 +1072,11 @@ template <int Q, bool FULL>
 static void line_go(bool timed, hipEvent_t e0, hipEvent_t e1, int grid, int c, float omega, int rev, double* part,
                     FinArgs fa) {
     if (!timed)
-        k_line_sweep<Q, FULL><<<grid, 256, 0, L.s>>>(c, g_lco, g_lx, g_ls, omega, part, g_nb, g_lchunk, L.ctrl, rev,
+        k_line_sweep<Q, FULL><<<grid, 256, 0, L.s>>>(c, g_lco, g_lx, g_ls, omega, part, g_lnb, g_lchunk, L.ctrl, rev,
                                                      fa);
     else
         hipExtLaunchKernelGGL(k_line_sweep<Q, FULL>, dim3(grid), dim3(256), 0, L.s, e0, e1, 0, c,
-                              (const float*)g_lco, g_lx, g_ls, omega, part, g_nb, g_lchunk, L.ctrl, rev, fa);
+                              (const float*)g_lco, g_lx, g_ls, omega, part, g_lnb, g_lchunk, L.ctrl, rev, fa);
 }
 
 template <int Q>
@@ -1079,8 +1123,8 @@ static int launch_sweep(int c, T* coef0, T* coef1, T* xa, T* xb, double omega, i
     T* xc = c ? xb : xa;
     const T* xo = c ? xa : xb;
     double* part = L.partial + (iter & 1) * 2 * g_nb;
-    FinArgs fa{L.partial + ((iter + 1) & 1) * 2 * g_nb, 2 * g_nb, step, fin};
-    const int grid = g_nb + (fin ? 1 : 0);
+    FinArgs fa{L.partial + ((iter + 1) & 1) * 2 * g_nb, 2 * g_pn, step, fin};
+    const int grid = (g_line && sizeof(T) == 4 ? g_lnb : g_nb) + (fin ? 1 : 0);
     const bool timed = L.timing && (L.timing_count++ % L.timing_period) == 0;
     const bool pairs = sizeof(T) == 4 && g_cs.hx % 2 == 0;
     if (sizeof(T) == 4 && g_line) return launch_line(c, omega, rev, iter, part, fa, grid, timed);
@@ -1116,8 +1160,8 @@ static int launch_sweep(int c, T* coef0, T* coef1, T* xa, T* xb, double omega, i
 }
 
 static int read_ctrl() {
-    TT_HIP(hipMemcpyAsync(L.hctrl, L.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, L.s));
-    TT_HIP(hipStreamSynchronize(L.s));
+    int rc = ctrl_to_host();
+    if (rc) return rc;
     if (L.timing) ev_drain(L.hctrl, g_bid);
     return 0;
 }
@@ -1271,7 +1315,7 @@ static int run_sweeps(T* coef0, T* coef1, T* xa, T* xb, double omega, int* iters
             if (defer ? b == bsize - 1 : (b + 1) % check_every == 0) {
                 // the batch's last test also leaves the control block in the
                 // host's pinned copy for this batch (bsize is a multiple of 4)
-                k_finish<<<1, 256, 0, L.s>>>(L.ctrl, L.partial + (iter & 1) * 2 * g_nb, 2 * g_nb, check_every,
+                k_finish<<<1, 256, 0, L.s>>>(L.ctrl, L.partial + (iter & 1) * 2 * g_nb, 2 * g_pn, check_every,
                                              b == bsize - 1 ? g_hcd[slot] : nullptr);
                 TT_LAUNCH();
             }
@@ -1603,9 +1647,11 @@ int sor_solve_impl(int fill_from_evals, tt_sor_stats* st, int mode) {
             if (line) {
                 // every iteration tested (inside the next red sweep), batches of 4
                 g_line = 1;
+                g_pn = g_lnb;
                 rc = run_sweeps<float>(c32[0], c32[1], x32[0], x32[1], L.opts.line_omega, &it32, &rn32, 1, true,
                                        false, 2);
                 g_line = 0;
+                g_pn = g_nb;
             } else {
                 rc = run_sweeps<float>(c32[0], c32[1], x32[0], x32[1], L.p.sorparam, &it32, &rn32, 4, true);
             }

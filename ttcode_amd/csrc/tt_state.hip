@@ -27,6 +27,22 @@ __global__ __launch_bounds__(256) void k_fold(const double* __restrict__ part, i
     }
 }
 
+// the control block word by word into the mapped host mirror
+__global__ void k_ctrl_snap(const Ctrl* __restrict__ src, Ctrl* dst) {
+    constexpr int nw = (int)(sizeof(Ctrl) / sizeof(unsigned int));
+    const unsigned int* s = (const unsigned int*)src;
+    unsigned int* d = (unsigned int*)dst;
+    for (int q = threadIdx.x; q < nw; q += blockDim.x) d[q] = s[q];
+}
+
+int ctrl_to_host() {
+    static_assert(sizeof(Ctrl) % sizeof(unsigned int) == 0, "control block of whole words");
+    k_ctrl_snap<<<1, 64, 0, L.s>>>(L.ctrl, L.hctrl_d);
+    TT_LAUNCH();
+    TT_HIP(hipStreamSynchronize(L.s));
+    return 0;
+}
+
 int fold_partials(const double* part, int nb, int nvec, int op, double* out0, double* out1, double* out2) {
     k_fold<<<1, 256, 0, L.s>>>(part, nb, nvec, op, out0, out1, out2);
     TT_LAUNCH();
@@ -205,6 +221,7 @@ static void free_all() {
     L.ctrl = nullptr;
     if (L.hctrl) hipHostFree(L.hctrl);
     L.hctrl = nullptr;
+    L.hctrl_d = nullptr;
     delete[] L.hmet;
     L.hmet = nullptr;
     for (auto& e : L.step_ev) {
@@ -313,6 +330,7 @@ int tt_init(const tt_params* p) {
     TT_HIP(dev_alloc(&L.ctrl, sizeof(Ctrl)));
     TT_HIP(hipMemsetAsync(L.ctrl, 0, sizeof(Ctrl), L.s));
     TT_HIP(hipHostMalloc((void**)&L.hctrl, sizeof(Ctrl), hipHostMallocDefault));
+    TT_HIP(hipHostGetDevicePointer((void**)&L.hctrl_d, L.hctrl, 0));
     int nmax = g.nx > g.ny ? g.nx : g.ny;
     if (g.nz > nmax) nmax = g.nz;
     L.metlen = nmax + 6;
