@@ -288,6 +288,13 @@ int tt_init(const tt_params* p) {
     if (p->igrav != 0 && p->igrav != 1 && p->igrav != 4) return fail(1, "IGRAV must be 0, 1 or 4");
     if (p->igrav == 4 && !(p->mstar2 > 0.0)) return fail(1, "IGRAV 4 needs MSTAR2 > 0");
     if (p->nx + 4 > 16384) return fail(1, "nx must be below 16380");
+    // The host waits on the device many times per step (SOR batches, the
+    // defect correction's residuals, Newton and SCHRITT reads); a spinning
+    // wait wakes in microseconds where a sleeping one takes tens.  The flag
+    // only takes effect before the process's first HIP context, so its
+    // refusal afterwards is ignored.
+    (void)hipSetDeviceFlags(hipDeviceScheduleSpin);
+    (void)hipGetLastError();
     int ndev = 0;
     TT_HIP(hipGetDeviceCount(&ndev));
     if (ndev < 1) return fail(10, "no HIP device");
