@@ -78,6 +78,13 @@ struct Ctrl {
     double best;       // best L1 ratio seen
     int since;         // iterations since it last improved by 1%
     int stag;          // window (0: off)
+    // the FP64 defect correction's outer loop, decided on the device
+    double rhs64;      // sum |RHS| of the FP64 system
+    double rn64;       // FP64 L1 residual after the latest correction
+    int outer;         // corrections started
+    int outer_done;    // 0 running, 1 converged, 2 no progress, 3 a cap reached
+    int total;         // inner iterations of the finished corrections
+    int pad_c;
 };
 
 // Library state (one device, one stream).

@@ -425,10 +425,16 @@ __global__ __launch_bounds__(320) void k_adv_t(Geo g, Met m, AdvArgs a) {
     __shared__ double sy[2][ADV_TJ + 1][ADV_TI];  // y faces of the tile columns, plus the high row
     const int ilo = K::ilo, ihi = g.nx, jlo = K::jlo, jhi = g.ny, klo = K::klo, khi = g.nz;
     const int nti = (ihi - ilo + ADV_TI) / ADV_TI, ntj = (jhi - jlo + ADV_TJ) / ADV_TJ;
-    int b = blockIdx.x;
-    const int ti = b % nti;
-    b /= nti;
-    const int tj = b % ntj, kc = b / ntj;
+    const int nkc = (khi - klo + KC) / KC;
+    // Hardware blocks b, b+8, ... run on one XCD; each XCD takes a contiguous
+    // range of logical blocks, ordered with the k chunk fastest, so the tiles
+    // that share halos (the next k chunk, then the next tile) are read
+    // through one L2.
+    const int nb = nti * ntj * nkc;
+    int b = (nb % 8 == 0) ? (int)(blockIdx.x % 8) * (nb / 8) + (int)(blockIdx.x / 8) : (int)blockIdx.x;
+    const int kc = b % nkc;
+    b /= nkc;
+    const int ti = b % nti, tj = b / nti;
     const int ci0 = ilo + ti * ADV_TI, cj0 = jlo + tj * ADV_TJ;
     const int k0 = klo + kc * KC;
     const int k1 = min(khi, k0 + KC - 1);

@@ -1022,6 +1022,53 @@ __global__ void k_scale(double* a, double s, long long n) {
     if (t < n) a[t] = a[t] * s;
 }
 
+// One step of the defect correction's outer loop, on the device, in the
+// stream after the FP64 residual of the latest correction (folded into
+// scratch[0]; first = 1: the zero start, whose residual is RHS with norm
+// rhsnorm; first = 2: a guess, residual in scratch[0]).  It stops the loop
+// when the FP64 L1 ratio meets eps64, when a correction made no progress or
+// at a cap, and otherwise sets up the next inner solve: its tolerance is
+// what takes the FP64 residual below the target with a factor 2 margin,
+// max(inner_tol, min(0.5 eps64 sum|RHS| / r, 0.5)).  When the loop stops,
+// done is set, so the inner solve's sweeps the host has queued return at once.
+__global__ void k_outer_step(Ctrl* c, int first, double eps64, double inner_tol, int inner_max, int stag,
+                             int max_outer, int itmax) {
+    if (threadIdx.x != 0) return;
+    if (first) {
+        c->rhs64 = c->rhsnorm;
+        c->outer = 0;
+        c->total = 0;
+        c->outer_done = 0;
+    } else {
+        c->total += c->it;  // the correction just finished
+    }
+    const double rn = first == 1 ? c->rhs64 : c->scratch[0];
+    int stop = 0;
+    if (rn / c->rhs64 <= eps64) stop = 1;
+    else if (!first && !(rn < c->rn64)) stop = 2;
+    else if (c->outer >= max_outer || c->total >= itmax) stop = 3;
+    c->rn64 = rn;
+    if (stop) {
+        c->outer_done = stop;
+        c->done = 1;
+        return;
+    }
+    c->outer++;
+    c->done = 0;
+    c->it = 0;
+    c->counter = 0;
+    c->counter2 = 0;
+    c->itmin = 1;
+    c->itmax = inner_max;
+    c->best = 1e300;
+    c->since = 0;
+    c->stag = stag;
+    c->spec_hit = 0;
+    const double need = 0.5 * eps64 * c->rhs64 / rn;
+    c->eps = need > inner_tol ? (need < 0.5 ? need : 0.5) : inner_tol;
+    c->rhsnorm = rn;
+}
+
 // ------------------------------------------------------------- host side
 static hipEvent_t ev_get() {
     hipEvent_t e;
@@ -1228,6 +1275,7 @@ static const int SOR_SPEC_MISS = -100;
 struct SolveView {
     int nknown = 0, it_a = 0, it_b = 0;
     double lr_a = 0.0, lr_b = 0.0;
+    double leps = 0.0;  // log of the solve's tolerance, from the snapshots (the device may set it)
     // log-ratio fall per iteration over the two reads (0: unknown)
     double rate() const { return nknown >= 2 && it_b > it_a ? (lr_a - lr_b) / (double)(it_b - it_a) : 0.0; }
 };
@@ -1243,6 +1291,7 @@ static int take_batch(int q, SolveView& v, bool* done) {
         *L.hctrl = *h;
         return 0;
     }
+    v.leps = log(h->eps);
     if (h->it > v.it_b && h->rnorm > 0.0 && h->rhsnorm > 0.0) {
         v.it_a = v.it_b;
         v.lr_a = v.lr_b;
@@ -1263,7 +1312,7 @@ static int run_sweeps(T* coef0, T* coef1, T* xa, T* xb, double omega, int* iters
     long long launched = 0;
     int slot = 0, pend = -1, last = 0;
     SolveView view;
-    const double leps = log(L.hctrl->eps);
+    view.leps = log(L.hctrl->eps);
     bool done = false;
     for (;;) {
         // the control block's iteration cap ends the loop; a solve that has
@@ -1278,7 +1327,7 @@ static int run_sweeps(T* coef0, T* coef1, T* xa, T* xb, double omega, int* iters
         const double rate = view.rate();
         int bsize = batch;
         if (rate > 0.0) {
-            const double remain = (double)view.it_b + (view.lr_b - leps) / rate - (double)launched;
+            const double remain = (double)view.it_b + (view.lr_b - view.leps) / rate - (double)launched;
             if (remain <= 0.0 && pend >= 0) {
                 int rc = take_batch(pend, view, &done);
                 if (rc) return rc;
@@ -1300,7 +1349,7 @@ static int run_sweeps(T* coef0, T* coef1, T* xa, T* xb, double omega, int* iters
             // iterations from the newer read to the end of this batch
             const double ahead = (double)(launched - view.it_b);
             const double m = L.spec_margin;
-            defer = m == 0.0 || (m > 0.0 && rate > 0.0 && view.lr_b - leps > m * rate * ahead);
+            defer = m == 0.0 || (m > 0.0 && rate > 0.0 && view.lr_b - view.leps > m * rate * ahead);
             if (defer) ++L.spec_batches;
         }
         for (int b = 0; b < bsize; ++b) {
@@ -1447,6 +1496,7 @@ static int resid64(double* out_norm, int to) {
     TT_LAUNCH();
     int rc = fold_partials(part, nb, 1, 0, &L.ctrl->scratch[0]);
     if (rc) return rc;
+    if (!out_norm) return 0;  // stays on the device, for k_outer_step
     rc = read_ctrl();
     if (rc) return rc;
     *out_norm = L.hctrl->scratch[0];
@@ -1598,24 +1648,28 @@ int sor_solve_impl(int fill_from_evals, tt_sor_stats* st, int mode) {
             g_c32_ok = true;
         }
         const int to = line ? 2 : 1;
-        double rn;
         if (s.guess_used) {
-            rc = resid64(&rn, to);  // reads the control block: rhsnorm is current
+            rc = resid64(nullptr, to);  // the guess's residual, folded into scratch[0]
+            if (rc) return rc;
         } else {
             // zero start: the residual is RHS, its norm sum|RHS|
             k_rhs_f32<<<nblk(2 * g_cs.ncell, 256), 256, 0, L.s>>>(g_cs, c64[0], c64[1], line ? nullptr : c32[0],
                                                                   line ? nullptr : c32[1], line ? g_lco : nullptr,
                                                                   g_ls.nx);
             TT_LAUNCH();
-            rc = read_ctrl();
-            rn = L.hctrl->rhsnorm;
         }
-        if (rc) return rc;
-        const double rhs_norm = L.hctrl->rhsnorm;
-        int outer = 0, total32 = 0;
-        const int max_outer = 200;
-        while (!(rn / rhs_norm <= L.p.epsmat) && outer < max_outer && total32 < L.p.itmax) {
-            ++outer;
+        // The outer loop's decisions run on the device (k_outer_step): the
+        // host queues a correction's kernels and the next inner solve without
+        // reading the FP64 residual first, and learns that the loop has
+        // ended from the inner solve's control-block snapshots.  FP32 sweeps
+        // stop reducing the residual at their rounding floor: an inner solve
+        // hands over when its ratio has not fallen by 1% over 256 iterations
+        // (8 for line sweeps, which gain a decade in a few).
+        const int stag = line ? 8 : 256, max_outer = 200;
+        k_outer_step<<<1, 64, 0, L.s>>>(L.ctrl, s.guess_used ? 2 : 1, L.p.epsmat, L.opts.inner_tol, L.opts.inner_max,
+                                        stag, max_outer, L.p.itmax);
+        TT_LAUNCH();
+        for (;;) {
             // inner: A c = r in FP32 from c = 0, L1 during-sweep ratio vs sum|r|
             if (line) {
                 TT_HIP(hipMemsetAsync(g_lx, 0, line_x_len() * sizeof(float), L.s));
@@ -1623,29 +1677,11 @@ int sor_solve_impl(int fill_from_evals, tt_sor_stats* st, int mode) {
                 TT_HIP(hipMemsetAsync(x32[0], 0, pl * sizeof(float), L.s));
                 TT_HIP(hipMemsetAsync(x32[1], 0, pl * sizeof(float), L.s));
             }
-            Ctrl c = *L.hctrl;  // current: resid64 has just read it
-            c.done = 0; c.it = 0; c.counter = 0; c.counter2 = 0;
-            c.itmin = 1;
-            c.itmax = L.opts.inner_max;
-            // FP32 sweeps stop reducing the residual at their rounding floor:
-            // hand over to the FP64 correction when the ratio has not fallen
-            // by 1% over 256 iterations
-            // (8 iterations for line sweeps, which gain a decade in a few)
-            c.best = 1e300;
-            c.since = 0;
-            c.stag = line ? 8 : 256;
-            // the inner solve only has to take the FP64 residual below the
-            // outer target (with a factor 2 margin); inner_tol is the floor,
-            // what FP32 sweeps can resolve
-            double need = 0.5 * L.p.epsmat * rhs_norm / rn;
-            c.eps = need > L.opts.inner_tol ? (need < 0.5 ? need : 0.5) : L.opts.inner_tol;
-            c.rhsnorm = rn;
-            *L.hctrl = c;
-            if ((rc = ctrl_put())) return rc;
+            L.hctrl->itmax = L.opts.inner_max;  // the host's guard in run_sweeps
             int it32 = 0;
             double rn32 = 0;
             if (line) {
-                // every iteration tested (inside the next red sweep), batches of 4
+                // every iteration tested (inside the next red sweep), batches of 2
                 g_line = 1;
                 g_pn = g_lnb;
                 rc = run_sweeps<float>(c32[0], c32[1], x32[0], x32[1], L.opts.line_omega, &it32, &rn32, 1, true,
@@ -1656,7 +1692,7 @@ int sor_solve_impl(int fill_from_evals, tt_sor_stats* st, int mode) {
                 rc = run_sweeps<float>(c32[0], c32[1], x32[0], x32[1], L.p.sorparam, &it32, &rn32, 4, true);
             }
             if (rc) return rc;
-            total32 += it32;
+            if (L.hctrl->outer_done) break;  // the loop had ended: these sweeps returned at once
             for (int cc = 0; cc < 2; ++cc) {
                 if (line)
                     k_add_corr_line<<<nblk(g_cs.ncell, 256), 256, 0, L.s>>>(g_cs, cc, g_ls.nx, x64[cc], g_lx);
@@ -1664,14 +1700,17 @@ int sor_solve_impl(int fill_from_evals, tt_sor_stats* st, int mode) {
                     k_add_corr<<<nblk(g_cs.ncell, 256), 256, 0, L.s>>>(g_cs, x64[cc], x32[cc]);
                 TT_LAUNCH();
             }
-            double rn_prev = rn;
-            rc = resid64(&rn, to);
+            rc = resid64(nullptr, to);
             if (rc) return rc;
-            if (!(rn < rn_prev)) break;  // no progress: report the cap below
+            k_outer_step<<<1, 64, 0, L.s>>>(L.ctrl, 0, L.p.epsmat, L.opts.inner_tol, L.opts.inner_max, stag,
+                                            max_outer, L.p.itmax);
+            TT_LAUNCH();
         }
+        const double rhs_norm = L.hctrl->rhs64, rn = L.hctrl->rn64;
+        const int total32 = L.hctrl->total;
         s.sor_iter = total32;
         s.fp32_sweeps = total32;
-        s.outer_iter = outer;
+        s.outer_iter = L.hctrl->outer;
         s.rnorm = rn;
         // restore the FP64 control block's solve fields
         L.hctrl->rhsnorm = rhs_norm;
