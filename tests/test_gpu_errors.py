@@ -93,3 +93,32 @@ def test_zero_rhs_stops_after_itmin(ttgpu):
         assert not d.download("delta_ER").any()
     finally:
         d.close()
+
+
+def test_itmax_cap_in_the_defect_correction(ttgpu):
+    """With FP32 line sweeps the outer loop's caps are decided on the device:
+    an ITMAX below what the solve needs ends the corrections early, the
+    return is the informational cap (negative) and the state stays finite;
+    the same solve uncapped converges."""
+    g = C.load("energy_const_12x16x12")
+    p = C.params_of(g)
+    d = ttgpu.Device(p)
+    try:
+        d.set_solver(precision=1, inner=1)
+        d.load_state(C.inputs_of(g))
+        sf, rcf = d.coupled_energy(float(g["delt"]), int(g["iii"]))
+        d.load_state(C.inputs_of(g))
+        p2 = C.params_of(g)
+        p2.itmax = 2
+        d.close()
+        d = ttgpu.Device(p2)
+        d.set_solver(precision=1, inner=1)
+        d.load_state(C.inputs_of(g))
+        sc, rcc = d.coupled_energy(float(g["delt"]), int(g["iii"]))
+        print(f"uncapped rc {rcf}, Newton {sf.energy_iter}, FP32 iterations {sf.fp32_sweeps}; "
+              f"capped rc {rcc}, Newton {sc.energy_iter}, FP32 iterations {sc.fp32_sweeps}")
+        assert rcf == 0 and rcc < 0
+        for k in ("T", "ER"):
+            assert np.isfinite(d.download(k)).all()
+    finally:
+        d.close()

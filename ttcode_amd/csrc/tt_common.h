@@ -39,10 +39,17 @@ struct Geo {
 };
 
 // 1-D metric arrays: one buffer, metric m at base[m*len + i + 2].
+// The table is written only by host copies, never while a kernel runs, so it
+// is read through the constant address space: a metric whose index is
+// uniform across the wavefront (a plane's k, a block's row) becomes a scalar
+// load into scalar registers, which no later store in the kernel can alias.
+using cdouble_p = const __attribute__((address_space(4))) double*;
 struct Met {
     const double* base;
     int len;
-    __device__ inline double operator()(int m, int i) const { return base[(long long)m * len + i + 2]; }
+    __device__ inline double operator()(int m, int i) const {
+        return ((cdouble_p)base)[(long long)m * len + i + 2];
+    }
 };
 
 // div(v) of a cell in the reference's operation order (the p dV work,

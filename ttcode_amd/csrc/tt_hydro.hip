@@ -427,14 +427,17 @@ __global__ __launch_bounds__(320) void k_adv_t(Geo g, Met m, AdvArgs a) {
     const int nti = (ihi - ilo + ADV_TI) / ADV_TI, ntj = (jhi - jlo + ADV_TJ) / ADV_TJ;
     const int nkc = (khi - klo + KC) / KC;
     // Hardware blocks b, b+8, ... run on one XCD; each XCD takes a contiguous
-    // range of logical blocks, ordered with the k chunk fastest, so the tiles
-    // that share halos (the next k chunk, then the next tile) are read
-    // through one L2.
+    // range of logical blocks, ordered with the tiles of a k chunk fastest,
+    // so the blocks resident on one XCD at a time cover whole planes of a few
+    // consecutive k chunks: a tile's x and y halos (its neighbour tiles) and
+    // its z halo (the next chunk) are then read through that XCD's L2 rather
+    // than fetched again (at 128x128x256 the fetched bytes fell from up to
+    // 2.3x to 1.0-1.1x of the arrays' size).
     const int nb = nti * ntj * nkc;
     int b = (nb % 8 == 0) ? (int)(blockIdx.x % 8) * (nb / 8) + (int)(blockIdx.x / 8) : (int)blockIdx.x;
-    const int kc = b % nkc;
-    b /= nkc;
-    const int ti = b % nti, tj = b / nti;
+    const int ti = b % nti;
+    b /= nti;
+    const int tj = b % ntj, kc = b / ntj;
     const int ci0 = ilo + ti * ADV_TI, cj0 = jlo + tj * ADV_TJ;
     const int k0 = klo + kc * KC;
     const int k1 = min(khi, k0 + KC - 1);

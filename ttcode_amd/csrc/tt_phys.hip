@@ -86,7 +86,11 @@ __global__ __launch_bounds__(256) void k_shift5_upperbnd(Geo g, double* T, doubl
 }
 
 // BOUNDXMIN(12) (ppbou.f:1118-1139) or (9) (:1029-1066), then BOUNDXMAX(2)
-// (ppbou.f:1269-1395)
+// (ppbou.f:1269-1395), one thread per (j,k) column.  The column's reads are
+// issued first, in two batches (the bottom cells and upperbnd, then the cells
+// at upperbnd), and the writes follow in the reference's order.  No read
+// sees a cell this column writes earlier in that order, except T(u-1) with
+// u = 1, which is the T(0) BOUNDXMIN has just set and is taken from it.
 __global__ void k_xbc(Geo g, Met m, const int32_t* __restrict__ ub, double* T, double* RH, double* V, double* G,
                       double* H, const double* __restrict__ ER, const double* __restrict__ SIN,
                       const double* __restrict__ XKP, const double* __restrict__ XKR, int bcxmin, double d_t,
@@ -95,35 +99,51 @@ __global__ void k_xbc(Geo g, Met m, const int32_t* __restrict__ ub, double* T, d
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nj * nk) return;
     int j = t % nj - 1, k = t / nj - 1;
+    const long long c1 = g.idx(1, j, k);
+    const bool inner = j >= 1 && j <= g.ny && k >= 1 && k <= g.nz;
+    // batch 1
+    const int u = ub[g.idx2(j, k)];
+    const double t1 = T[c1];
+    const double g1 = j >= 0 ? G[c1] : 0.0, h1 = k >= 0 ? H[c1] : 0.0;
+    const bool flux9 = bcxmin != 12 && iii > 0;
+    const double xkr1 = flux9 ? XKR[c1] : 0.0, rh1 = flux9 ? RH[c1] : 0.0;
+    // batch 2: the cells at upperbnd (u is within 1..nx+1 here; bad columns stop below)
+    const int uc = u < 1 ? 1 : (u > g.nx + 2 ? g.nx + 2 : u);
+    const long long q = g.idx(uc - 1, j, k);
+    double tum1 = T[q];
+    const double gu = j >= 0 ? G[g.idx(uc, j, k)] : 0.0, hu = k >= 0 ? H[g.idx(uc, j, k)] : 0.0;
+    const bool rad9 = inner && ibdrad == 9 && iii > 1;
+    const double eq = rad9 ? ER[q] : 0.0, sq = rad9 ? SIN[q] : 0.0, rq = rad9 ? RH[q] : 0.0, kq = rad9 ? XKP[q] : 0.0;
+    // BOUNDXMIN
+    double t0;
     if (bcxmin == 12) {  // fixed dT/dr = D_T
-        T[g.idx(0, j, k)] = T[g.idx(1, j, k)] + d_t * (m(TT_M_XXB, 0) - m(TT_M_XXB, 1));
-        T[g.idx(-1, j, k)] = T[g.idx(1, j, k)] + d_t * (m(TT_M_XXB, -1) - m(TT_M_XXB, 1));
+        t0 = t1 + d_t * (m(TT_M_XXB, 0) - m(TT_M_XXB, 1));
+        T[g.idx(0, j, k)] = t0;
+        T[g.idx(-1, j, k)] = t1 + d_t * (m(TT_M_XXB, -1) - m(TT_M_XXB, 1));
     } else {  // 9: the radiative-diffusion gradient that carries FBottom
         double dtdr;
         if (iii > 0) {
-            const long long c = g.idx(1, j, k);
-            const double t1 = T[c];
-            dtdr = -3.0 * fbottom * XKR[c] * RH[c] / (4.0 * ARAD * FD * (t1 * t1 * t1));  // T**3.d0 as T*T*T
+            dtdr = -3.0 * fbottom * xkr1 * rh1 / (4.0 * ARAD * FD * (t1 * t1 * t1));  // T**3.d0 as T*T*T
         } else {
             dtdr = ctrl->bot_dtdr;  // cell (1,1,1)'s, from BurrowsOPC (k_bur_cell111)
         }
-        const double t1 = T[g.idx(1, j, k)];
-        T[g.idx(0, j, k)] = t1 - dtdr * (m(TT_M_XXB, 1) - m(TT_M_XXB, 0));
+        t0 = t1 - dtdr * (m(TT_M_XXB, 1) - m(TT_M_XXB, 0));
+        T[g.idx(0, j, k)] = t0;
         T[g.idx(-1, j, k)] = t1 - dtdr * (m(TT_M_XXB, 1) - m(TT_M_XXB, -1));
     }
-    V[g.idx(1, j, k)] = 0.0;
+    if (uc == 1) tum1 = t0;  // T(u-1) = the T(0) just set
+    V[c1] = 0.0;
     V[g.idx(0, j, k)] = 0.0;
     if (j >= 0) {
-        G[g.idx(0, j, k)] = G[g.idx(1, j, k)];
-        G[g.idx(-1, j, k)] = G[g.idx(1, j, k)];
+        G[g.idx(0, j, k)] = g1;
+        G[g.idx(-1, j, k)] = g1;
     }
     if (k >= 0) {
-        H[g.idx(0, j, k)] = H[g.idx(1, j, k)];
-        H[g.idx(-1, j, k)] = H[g.idx(1, j, k)];
+        H[g.idx(0, j, k)] = h1;
+        H[g.idx(-1, j, k)] = h1;
     }
     // BOUNDXMAX(2): temperature, density and radial velocity above upperbnd
-    int u = ub[g.idx2(j, k)];
-    if (j >= 1 && j <= g.ny && k >= 1 && k <= g.nz) {
+    if (inner) {
         if (u >= g.nx + 1) {
             ctrl->bad_bound = 1;
             return;
@@ -134,11 +154,10 @@ __global__ void k_xbc(Geo g, Met m, const int32_t* __restrict__ ub, double* T, d
             TJ = TJ > 100.0 ? TJ : 100.0;
         } else if (ibdrad == 4) {
             TJ = tirr;
-        } else if (ibdrad == 9 && iii > 1) {
-            long long q = g.idx(u - 1, j, k);
-            TJ = pow((ER[q] / ARAD) + (SIN[q] / (4.0 * SBCONST * RH[q] * XKP[q])), 0.25);
+        } else if (rad9) {
+            TJ = pow((eq / ARAD) + (sq / (4.0 * SBCONST * rq * kq)), 0.25);
         } else {  // 7, or 9 at III <= 1
-            TJ = T[g.idx(u - 1, j, k)];
+            TJ = tum1;
         }
         T[g.idx(u, j, k)] = TJ;
         const double rh_top = (double)1.0e-10f;
@@ -154,14 +173,10 @@ __global__ void k_xbc(Geo g, Met m, const int32_t* __restrict__ ub, double* T, d
         }
     }
     // stress free above upperbnd
-    if (j >= 0) {
-        double gv = G[g.idx(u, j, k)];
-        for (int i = u + 1; i <= g.nx + 2; ++i) G[g.idx(i, j, k)] = gv;
-    }
-    if (k >= 0) {
-        double hv = H[g.idx(u, j, k)];
-        for (int i = u + 1; i <= g.nx + 2; ++i) H[g.idx(i, j, k)] = hv;
-    }
+    if (j >= 0)
+        for (int i = u + 1; i <= g.nx + 2; ++i) G[g.idx(i, j, k)] = gu;
+    if (k >= 0)
+        for (int i = u + 1; i <= g.nx + 2; ++i) H[g.idx(i, j, k)] = hu;
 }
 
 // BOUNDZMIN(0) (ppbou.f:1833-1878) then BOUNDZMAX(0) (ppbou.f:1946-1988)
