@@ -16,7 +16,10 @@ namespace tt {
 int planetshift_impl(double* a, int f);
 
 // PLANETSHIFT of T, RH, V, G, H (phi wrap of the two ghost planes), one
-// thread per (i, k) line; b is the block's index within this work
+// thread per (i, k) line; b is the block's index within this work.  The
+// ghost planes j = -1, 0, ny+1, ny+2 and their sources j = ny-1, ny, 1, 2 are
+// disjoint (tt_init takes ny >= 4), so all the line's sources are read
+// first, in one batch, and then written.
 __device__ inline void shift5_body(const Geo& g, double* T, double* RH, double* V, double* G, double* H, int b) {
     int ni = g.nx + 4;
     long long n = (long long)ni * g.nz;
@@ -25,24 +28,43 @@ __device__ inline void shift5_body(const Geo& g, double* T, double* RH, double* 
     int i = (int)(t % ni) - 1, k = (int)(t / ni) + 1;
     const int ny = g.ny;
     double* f3[3] = {T, RH, H};
+    const bool hv = i >= 0;  // V(0:nx+2,...)
+    double s3[3][4], sv[4], sg[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const double* a = f3[q];
+        s3[q][0] = a[g.idx(i, ny, k)];
+        s3[q][1] = a[g.idx(i, ny - 1, k)];
+        s3[q][2] = a[g.idx(i, 1, k)];
+        s3[q][3] = a[g.idx(i, 2, k)];
+    }
+    if (hv) {
+        sv[0] = V[g.idx(i, ny, k)];
+        sv[1] = V[g.idx(i, ny - 1, k)];
+        sv[2] = V[g.idx(i, 1, k)];
+        sv[3] = V[g.idx(i, 2, k)];
+    }
+    sg[0] = G[g.idx(i, ny, k)];
+    sg[1] = G[g.idx(i, 1, k)];
+    sg[2] = G[g.idx(i, 2, k)];
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
         double* a = f3[q];
-        a[g.idx(i, 0, k)] = a[g.idx(i, ny, k)];
-        a[g.idx(i, -1, k)] = a[g.idx(i, ny - 1, k)];
-        a[g.idx(i, ny + 1, k)] = a[g.idx(i, 1, k)];
-        a[g.idx(i, ny + 2, k)] = a[g.idx(i, 2, k)];
+        a[g.idx(i, 0, k)] = s3[q][0];
+        a[g.idx(i, -1, k)] = s3[q][1];
+        a[g.idx(i, ny + 1, k)] = s3[q][2];
+        a[g.idx(i, ny + 2, k)] = s3[q][3];
     }
-    if (i >= 0) {  // V(0:nx+2,...)
-        V[g.idx(i, 0, k)] = V[g.idx(i, ny, k)];
-        V[g.idx(i, -1, k)] = V[g.idx(i, ny - 1, k)];
-        V[g.idx(i, ny + 1, k)] = V[g.idx(i, 1, k)];
-        V[g.idx(i, ny + 2, k)] = V[g.idx(i, 2, k)];
+    if (hv) {
+        V[g.idx(i, 0, k)] = sv[0];
+        V[g.idx(i, -1, k)] = sv[1];
+        V[g.idx(i, ny + 1, k)] = sv[2];
+        V[g.idx(i, ny + 2, k)] = sv[3];
     }
     // G(:,0:ny+2,:) has no j = -1 plane
-    G[g.idx(i, 0, k)] = G[g.idx(i, ny, k)];
-    G[g.idx(i, ny + 1, k)] = G[g.idx(i, 1, k)];
-    G[g.idx(i, ny + 2, k)] = G[g.idx(i, 2, k)];
+    G[g.idx(i, 0, k)] = sg[0];
+    G[g.idx(i, ny + 1, k)] = sg[1];
+    G[g.idx(i, ny + 2, k)] = sg[2];
 }
 
 __global__ __launch_bounds__(256) void k_shift5(Geo g, double* T, double* RH, double* V, double* G, double* H) {
@@ -179,50 +201,60 @@ __global__ void k_xbc(Geo g, Met m, const int32_t* __restrict__ ub, double* T, d
         for (int i = u + 1; i <= g.nx + 2; ++i) H[g.idx(i, j, k)] = hu;
 }
 
-// BOUNDZMIN(0) (ppbou.f:1833-1878) then BOUNDZMAX(0) (ppbou.f:1946-1988)
+// BOUNDZMIN(0) (ppbou.f:1833-1878) then BOUNDZMAX(0) (ppbou.f:1946-1988),
+// one thread per (i, j) line.  Each written plane takes its value from k = 2
+// or nz-1 or from a plane written just before (k = 1, nz), and neither half
+// writes the other's source plane (tt_init takes nz >= 4), so the line reads
+// its two source planes first, in one batch, and forms every written value
+// from them in the reference's order.
 __global__ void k_zbc(Geo g, Met m, double* T, double* RH, double* V, double* G, double* H) {
     int ni = g.nx + 4, nj = g.ny + 4;
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ni * nj) return;
     int i = t % ni - 1, j = t / ni - 1;
     const int nz = g.nz;
-    RH[g.idx(i, j, 1)] = RH[g.idx(i, j, 2)];
-    RH[g.idx(i, j, 0)] = RH[g.idx(i, j, 1)];
-    RH[g.idx(i, j, -1)] = RH[g.idx(i, j, 1)];
-    T[g.idx(i, j, 1)] = T[g.idx(i, j, 2)];
-    T[g.idx(i, j, 0)] = T[g.idx(i, j, 1)];
-    T[g.idx(i, j, -1)] = T[g.idx(i, j, 1)];
+    const bool hv = i >= 0, hg = j >= 0;
+    const double rh2 = RH[g.idx(i, j, 2)], t2 = T[g.idx(i, j, 2)], g2 = hg ? G[g.idx(i, j, 2)] : 0.0;
+    const double rhn = RH[g.idx(i, j, nz - 1)], tn = T[g.idx(i, j, nz - 1)], gn = hg ? G[g.idx(i, j, nz - 1)] : 0.0;
+    RH[g.idx(i, j, 1)] = rh2;
+    RH[g.idx(i, j, 0)] = rh2;
+    RH[g.idx(i, j, -1)] = rh2;
+    T[g.idx(i, j, 1)] = t2;
+    T[g.idx(i, j, 0)] = t2;
+    T[g.idx(i, j, -1)] = t2;
     H[g.idx(i, j, 2)] = 0.0;
     H[g.idx(i, j, 1)] = 0.0;
     H[g.idx(i, j, 0)] = 0.0;
-    if (i >= 0) {
+    if (hv) {
         V[g.idx(i, j, 1)] = 0.0;
-        V[g.idx(i, j, 0)] = V[g.idx(i, j, 1)];
-        V[g.idx(i, j, -1)] = V[g.idx(i, j, 1)];
+        V[g.idx(i, j, 0)] = 0.0;
+        V[g.idx(i, j, -1)] = 0.0;
     }
-    if (j >= 0) {
-        G[g.idx(i, j, 1)] = G[g.idx(i, j, 2)] * m(TT_M_SURZB, 2) / m(TT_M_SURZB, 1);
-        G[g.idx(i, j, 0)] = G[g.idx(i, j, 1)] * m(TT_M_SURZB, 1) / m(TT_M_SURZB, 0);
-        G[g.idx(i, j, -1)] = G[g.idx(i, j, 1)] * m(TT_M_SURZB, 1) / cos(m(TT_M_XZB, -1));
+    if (hg) {
+        const double g1 = g2 * m(TT_M_SURZB, 2) / m(TT_M_SURZB, 1);
+        G[g.idx(i, j, 1)] = g1;
+        G[g.idx(i, j, 0)] = g1 * m(TT_M_SURZB, 1) / m(TT_M_SURZB, 0);
+        G[g.idx(i, j, -1)] = g1 * m(TT_M_SURZB, 1) / cos(m(TT_M_XZB, -1));
     }
-    RH[g.idx(i, j, nz)] = RH[g.idx(i, j, nz - 1)];
-    RH[g.idx(i, j, nz + 1)] = RH[g.idx(i, j, nz)];
-    RH[g.idx(i, j, nz + 2)] = RH[g.idx(i, j, nz)];
-    T[g.idx(i, j, nz)] = T[g.idx(i, j, nz - 1)];
-    T[g.idx(i, j, nz + 1)] = T[g.idx(i, j, nz)];
-    T[g.idx(i, j, nz + 2)] = T[g.idx(i, j, nz)];
+    RH[g.idx(i, j, nz)] = rhn;
+    RH[g.idx(i, j, nz + 1)] = rhn;
+    RH[g.idx(i, j, nz + 2)] = rhn;
+    T[g.idx(i, j, nz)] = tn;
+    T[g.idx(i, j, nz + 1)] = tn;
+    T[g.idx(i, j, nz + 2)] = tn;
     H[g.idx(i, j, nz)] = 0.0;
     H[g.idx(i, j, nz + 1)] = 0.0;
     H[g.idx(i, j, nz + 2)] = 0.0;
-    if (i >= 0) {
+    if (hv) {
         V[g.idx(i, j, nz)] = 0.0;
-        V[g.idx(i, j, nz + 1)] = V[g.idx(i, j, nz)];
-        V[g.idx(i, j, nz + 2)] = V[g.idx(i, j, nz)];
+        V[g.idx(i, j, nz + 1)] = 0.0;
+        V[g.idx(i, j, nz + 2)] = 0.0;
     }
-    if (j >= 0) {
-        G[g.idx(i, j, nz)] = G[g.idx(i, j, nz - 1)] * m(TT_M_SURZB, nz - 1) / m(TT_M_SURZB, nz);
-        G[g.idx(i, j, nz + 1)] = G[g.idx(i, j, nz)] * m(TT_M_SURZB, nz) / m(TT_M_SURZB, nz + 1);
-        G[g.idx(i, j, nz + 2)] = G[g.idx(i, j, nz)] * m(TT_M_SURZB, nz) / m(TT_M_SURZB, nz + 2);
+    if (hg) {
+        const double gz = gn * m(TT_M_SURZB, nz - 1) / m(TT_M_SURZB, nz);
+        G[g.idx(i, j, nz)] = gz;
+        G[g.idx(i, j, nz + 1)] = gz * m(TT_M_SURZB, nz) / m(TT_M_SURZB, nz + 1);
+        G[g.idx(i, j, nz + 2)] = gz * m(TT_M_SURZB, nz) / m(TT_M_SURZB, nz + 2);
     }
 }
 
