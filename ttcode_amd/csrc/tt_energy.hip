@@ -31,6 +31,18 @@ __global__ void k_q_init(double* Q, const double* __restrict__ RH, const double*
     if (t < n) Q[t] = RH[t] * CV[t] * T[t];
 }
 
+// x**y for the power law's exponents: 1/2, 1, 2 and 3 by sqrt and products
+// (correctly rounded, or at most two roundings, where the general pow of the
+// device library is within one unit in the last place), anything else by
+// pow.  y is a kernel argument, so the choice is uniform.
+__device__ inline double pow_exp(double x, double y) {
+    if (y == 0.5) return sqrt(x);
+    if (y == 1.0) return x;
+    if (y == 2.0) return x * x;
+    if (y == 3.0) return x * x * x;
+    return pow(x, y);
+}
+
 // KAPPA (pprphy.f:1-360): IKAPTYP=1 constant; IKAPTYP=26 (new) power law
 // evaluated up to upperbnd, cells above take the upperbnd value; SUBSIG.
 __global__ void k_kappa(Geo g, tt_params p, const int32_t* __restrict__ ub, const double* __restrict__ RH,
@@ -50,8 +62,9 @@ __global__ void k_kappa(Geo g, tt_params p, const int32_t* __restrict__ ub, cons
     int ii = i <= u ? i : u;
     long long n = g.idx(ii, j, k);
     double rr = RH[n] / p.kap_rho0, tt = T[n] / p.kap_t0;
-    double kap = p.kap0 * pow(rr, p.kap_a) * pow(tt, p.kap_b);
-    double dk = p.dkap0 * pow(rr, p.kap_a) * pow(tt, p.kap_b - 1.0);
+    const double pa = pow_exp(rr, p.kap_a);
+    double kap = p.kap0 * pa * pow_exp(tt, p.kap_b);
+    double dk = p.dkap0 * pa * pow_exp(tt, p.kap_b - 1.0);
     XKR[t] = kap; XKP[t] = kap; XKA[t] = kap;
     DKP[t] = dk; DKA[t] = dk;
 }
