@@ -434,6 +434,90 @@ __global__ __launch_bounds__(64) void k_stau_sin(Geo g, Met m, const int32_t* __
     }
 }
 
+// The same three steps for C columns per 64-lane wavefront (consecutive
+// (j, k) columns), which spreads the serial scan's instructions over C
+// columns: step 1 loads the C columns' cells, NCH = ceil((nx + 2) / 64)
+// per lane and column, in one batch (the addresses do not depend on
+// upperbnd; the selection does), lanes 0..C-1 then scan one column each, in
+// the reference's order, and step 3 writes the columns one after another.
+// LDS rows are nx + 3 doubles apart, so the scanning lanes fall in distinct
+// banks.  The last block's missing columns repeat the last column and are
+// not written.
+template <int C, int NCH>
+__global__ __launch_bounds__(64) void k_stau_sin_c(Geo g, Met m, const int32_t* __restrict__ ub,
+                                                   const double* __restrict__ XKA, const double* __restrict__ RH,
+                                                   const double* __restrict__ MU, const double* __restrict__ COEF,
+                                                   double* STAU, double* DTAUS, double* SIN) {
+    extern __shared__ double tau[];  // tau[c * ld + i], i = 0..nx+1
+    const int nj = g.ny + 4, ncol = nj * (g.nz + 4), ld = g.nx + 3;
+    const int lane = threadIdx.x, c0 = blockIdx.x * C;
+    const double dr = m(TT_M_XXB, 2) - m(TT_M_XXB, 1);
+    int uc[C];
+    double va[C][NCH], vb[C][NCH];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        const int col = min(c0 + c, ncol - 1);
+        const int j = col % nj - 1, k = col / nj - 1;
+        uc[c] = ub[g.idx2(j, k)];
+#pragma unroll
+        for (int h = 0; h < NCH; ++h) {
+            const int i = min(lane + 64 * h, g.nx + 1);
+            const long long n = IX(i, j, k);
+            va[c][h] = (i == g.nx + 1 ? STAU : XKA)[n];
+            vb[c][h] = RH[n];
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        const int u = uc[c];
+#pragma unroll
+        for (int h = 0; h < NCH; ++h) {
+            const int i = lane + 64 * h;
+            double v = 0.0;
+            if (i >= 1 && i <= u) v = va[c][h] * vb[c][h] * dr;
+            else if (i == g.nx + 1 && u >= g.nx) v = va[c][h];
+            if (i <= g.nx + 1) tau[c * ld + i] = v;
+        }
+    }
+    __syncthreads();
+    if (lane < C) {
+        const int col = min(c0 + lane, ncol - 1);
+        const int u = ub[g.idx2(col % nj - 1, col / nj - 1)];
+        double* t = tau + lane * ld;
+        double tot = 0.0;
+        for (int i0 = u; i0 >= 1; i0 -= 16) {
+            double v[16];
+#pragma unroll
+            for (int q = 0; q < 16; ++q) v[q] = i0 - q >= 1 ? t[i0 - q] : 0.0;
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                if (i0 - q >= 1) {
+                    tot = tot + v[q];
+                    t[i0 - q] = tot;
+                }
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        const int col = c0 + c;
+        if (col >= ncol) break;
+        const int j = col % nj - 1, k = col / nj - 1;
+        const int u = uc[c];
+        const double* t = tau + c * ld;
+        const double mu = MU[g.idx2(j, k)];
+        const double coef = COEF[col];
+        for (int i = 1 + lane; i <= g.nx + 1; i += 64) {
+            if (i <= u || u < g.nx) STAU[IX(i, j, k)] = t[i];
+            const int ic = i == 1 ? 2 : (i == g.nx + 1 ? g.nx : i);
+            const double dt = (t[ic - 1] - t[ic + 1]) / 2.0;
+            DTAUS[IX(i, j, k)] = dt;
+            if (i <= g.nx) SIN[IX(i, j, k)] = coef * (exp(-tau[c * ld + i] / mu)) * dt / m(TT_M_DXB, i);
+        }
+    }
+}
+
 // The wavelength-binned STELLARINPUT_HEATING (pprad.f:429-447) for IKAPTYP
 // 20, 21, 23 and 24, one 64-lane wavefront per (j, k) column:
 //  1. lanes over the cells: each cell's 30 bins of xkapW (BurrowsOPC's
@@ -1043,6 +1127,10 @@ static int kappa_impl() {
     return 0;
 }
 
+#ifndef STAU_COLS
+#define STAU_COLS 4
+#endif
+
 static int stellar_impl() {
     double** f = L.f;
     if (L.p.f_incident == 0) {
@@ -1076,8 +1164,22 @@ static int stellar_impl() {
     double* coef = L.sincoef;  // mu_star may have been uploaded since the last call: formed each call
     k_sin_coef<<<nblk(ncol, 256), 256, 0, L.s>>>(L.g, met(), L.p.tirr, L.mustar, coef);
     TT_LAUNCH();
-    k_stau_sin<<<ncol, 64, lds, L.s>>>(L.g, met(), L.ub, f[TT_XKAPA], f[TT_RH], L.mustar, coef, f[TT_STELLARTAU],
-                                       f[TT_DTAUS], f[TT_STELLARINPUT]);
+    constexpr int SC = STAU_COLS;
+    const int nch = (L.g.nx + 2 + 63) / 64;
+    const size_t ldsc = (size_t)SC * (L.g.nx + 3) * sizeof(double);
+    const unsigned nbc = (unsigned)((ncol + SC - 1) / SC);
+#define TT_STAU_C(N)                                                                                         \
+    case N:                                                                                                  \
+        k_stau_sin_c<SC, N><<<nbc, 64, ldsc, L.s>>>(L.g, met(), L.ub, f[TT_XKAPA], f[TT_RH], L.mustar, coef, \
+                                                   f[TT_STELLARTAU], f[TT_DTAUS], f[TT_STELLARINPUT]);      \
+        break;
+    switch (nch) {
+        TT_STAU_C(1) TT_STAU_C(2) TT_STAU_C(3) TT_STAU_C(4) TT_STAU_C(5) TT_STAU_C(6) TT_STAU_C(7) TT_STAU_C(8)
+        default:  // nx > 510: one column per wavefront
+            k_stau_sin<<<ncol, 64, lds, L.s>>>(L.g, met(), L.ub, f[TT_XKAPA], f[TT_RH], L.mustar, coef,
+                                               f[TT_STELLARTAU], f[TT_DTAUS], f[TT_STELLARINPUT]);
+    }
+#undef TT_STAU_C
     TT_LAUNCH();
     return 0;
 }
