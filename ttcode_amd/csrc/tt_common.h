@@ -92,6 +92,7 @@ struct Ctrl {
     int outer_done;    // 0 running, 1 converged, 2 no progress, 3 a cap reached
     int total;         // inner iterations of the finished corrections
     int pad_c;
+    int hist[16];      // inner iterations of each of the first 16 corrections (0: not run)
 };
 
 // Library state (one device, one stream).

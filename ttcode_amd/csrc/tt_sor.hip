@@ -57,6 +57,10 @@ static std::vector<hipEvent_t> ev_pool, ev_pend;
 static std::vector<int> ev_kind;
 static std::vector<int> ev_iter;  // SOR iteration (0-based) of each timed launch
 static std::vector<long long> ev_bid;  // batch of each timed launch
+// correction of each timed launch in a queued line defect correction (-1:
+// another solve), and the correction being queued
+static std::vector<int> ev_outer;
+static int g_outer_tag = -1;
 // Two pinned copies of the control block, so that the host reads batch k's
 // convergence state while batch k+1 already runs on the device.
 static Ctrl* g_hc[2] = {nullptr, nullptr};
@@ -124,6 +128,7 @@ void sor_free() {
     ev_kind.clear();
     ev_iter.clear();
     ev_bid.clear();
+    ev_outer.clear();
     for (int q = 0; q < 2; ++q) {
         if (g_hc[q]) hipHostFree(g_hc[q]);
         if (g_cev[q]) hipEventDestroy(g_cev[q]);
@@ -891,10 +896,13 @@ int planetshift_impl(double* a, int f) {
 // FP64 true residual r = RHS - A x for both colours (defect correction and
 // the guess check); per-block sums of |r| (folded into ctrl->scratch[0]);
 // optionally the residual in FP32 into the RHS slot of the FP32 operator.
+// With guard set it does nothing once the outer loop has ended.
 __global__ __launch_bounds__(256) void k_resid64(CS cs, const double* __restrict__ C0, const double* __restrict__ C1,
                                                   const double* x0, const double* x1, float* r32_0, float* r32_1,
-                                                  float* rline, int nx, double* part, int chunk) {
+                                                  float* rline, int nx, double* part, int chunk,
+                                                  const Ctrl* guard) {
     __shared__ double sh[4];
+    if (guard && guard->outer_done) return;  // a queued correction after the loop has ended
     const int lb = blockIdx.x;
     long long start = (long long)lb * chunk, end = start + chunk;
     if (end > 2 * cs.ncell) end = 2 * cs.ncell;
@@ -1002,8 +1010,11 @@ __global__ void k_rhs_f32(CS cs, const double* __restrict__ C0, const double* __
     if (rline) rline[(line_row(jj, kk, cs.ny, cs.nz) * 8 + 7) * nx + 2 * h + ((jj + kk + c + 1) & 1)] = (float)v;
 }
 
-// x64 (padded, colour cc) += the line unknowns over the interior cells
-__global__ void k_add_corr_line(CS cs, int cc, int nx, double* x, const float* __restrict__ xl) {
+// x64 (padded, colour cc) += the line unknowns over the interior cells,
+// which are then zeroed: the next correction's inner solve starts from 0.
+// Nothing once the outer loop has ended.
+__global__ void k_add_corr_line(CS cs, int cc, int nx, double* x, float* xl, const Ctrl* guard) {
+    if (guard->outer_done) return;
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= cs.ncell) return;
     int tt = (int)t;
@@ -1014,7 +1025,9 @@ __global__ void k_add_corr_line(CS cs, int cc, int nx, double* x, const float* _
     int s = (jj + kk + cc + 1) & 1;
     long long row = jj + (long long)cs.ny * (kk + 1);
     long long p = (long long)cs.px * row + h + 2;
-    x[p] = x[p] + (double)xl[row * nx + 2 * h + s];
+    const long long q = row * nx + 2 * h + s;
+    x[p] = x[p] + (double)xl[q];
+    xl[q] = 0.0f;
 }
 
 __global__ void k_scale(double* a, double s, long long n) {
@@ -1031,16 +1044,17 @@ __global__ void k_scale(double* a, double s, long long n) {
 // what takes the FP64 residual below the target with a factor 2 margin,
 // max(inner_tol, min(0.5 eps64 sum|RHS| / r, 0.5)).  When the loop stops,
 // done is set, so the inner solve's sweeps the host has queued return at once.
-__global__ void k_outer_step(Ctrl* c, int first, double eps64, double inner_tol, int inner_max, int stag,
-                             int max_outer, int itmax) {
-    if (threadIdx.x != 0) return;
+__device__ inline void outer_step_body(Ctrl* c, int first, double eps64, double inner_tol, int inner_max, int stag,
+                                       int max_outer, int itmax) {
     if (first) {
         c->rhs64 = c->rhsnorm;
         c->outer = 0;
         c->total = 0;
         c->outer_done = 0;
+        for (int q = 0; q < 16; ++q) c->hist[q] = 0;
     } else {
         c->total += c->it;  // the correction just finished
+        if (c->outer - 1 < 16) c->hist[c->outer - 1] = c->it;
     }
     const double rn = first == 1 ? c->rhs64 : c->scratch[0];
     int stop = 0;
@@ -1067,6 +1081,28 @@ __global__ void k_outer_step(Ctrl* c, int first, double eps64, double inner_tol,
     const double need = 0.5 * eps64 * c->rhs64 / rn;
     c->eps = need > inner_tol ? (need < 0.5 ? need : 0.5) : inner_tol;
     c->rhsnorm = rn;
+}
+
+__global__ void k_outer_step(Ctrl* c, int first, double eps64, double inner_tol, int inner_max, int stag,
+                             int max_outer, int itmax) {
+    if (threadIdx.x != 0) return;
+    outer_step_body(c, first, eps64, inner_tol, inner_max, stag, max_outer, itmax);
+}
+
+// The fold of k_resid64's block sums into scratch[0] (k_fold's order, so
+// the same bits) and the outer step after it, in one launch; nothing once
+// the loop has ended, so corrections queued beyond its end cost a launch.
+__global__ __launch_bounds__(256) void k_outer_fold(Ctrl* c, const double* __restrict__ part, int nb, double eps64,
+                                                    double inner_tol, int inner_max, int stag, int max_outer,
+                                                    int itmax) {
+    __shared__ double sh[4];
+    if (c->outer_done) return;
+    double s = 0.0;
+    for (int q = threadIdx.x; q < nb; q += 256) s = s + part[q];
+    s = block_reduce256(s, sh, 0);
+    if (threadIdx.x != 0) return;
+    c->scratch[0] = s;
+    outer_step_body(c, 0, eps64, inner_tol, inner_max, stag, max_outer, itmax);
 }
 
 // ------------------------------------------------------------- host side
@@ -1097,10 +1133,15 @@ static void ev_drain(const Ctrl* st, long long upto) {
             ev_kind[keep] = ev_kind[u];
             ev_iter[keep] = ev_iter[u];
             ev_bid[keep] = ev_bid[u];
+            ev_outer[keep] = ev_outer[u];
             ++keep;
             continue;
         }
-        if (!(fin && ev_iter[u] >= itc)) {
+        // a queued correction's sweep ran if its iteration is below that
+        // correction's count (corrections queued after the loop ended have 0)
+        const int o = ev_outer[u];
+        const bool ran = o >= 0 ? (st && o < 16 && ev_iter[u] < st->hist[o]) : !(fin && ev_iter[u] >= itc);
+        if (ran) {
             float ms = 0.f;
             hipEventElapsedTime(&ms, a, b);
             if (ev_kind[u] == 64) { L.ms64 += ms; L.n64++; }
@@ -1113,6 +1154,7 @@ static void ev_drain(const Ctrl* st, long long upto) {
     ev_kind.resize(keep);
     ev_iter.resize(keep);
     ev_bid.resize(keep);
+    ev_outer.resize(keep);
 }
 
 template <int Q, bool FULL>
@@ -1156,6 +1198,7 @@ static int launch_line(int c, double omega, int rev, int iter, double* part, Fin
         ev_kind.push_back(32);
         ev_iter.push_back(iter);
         ev_bid.push_back(g_bid);
+        ev_outer.push_back(g_outer_tag);
     }
     return 0;
 }
@@ -1203,6 +1246,7 @@ static int launch_sweep(int c, T* coef0, T* coef1, T* xa, T* xb, double omega, i
     ev_kind.push_back(sizeof(T) == 8 ? 64 : 32);
     ev_iter.push_back(iter);
     ev_bid.push_back(g_bid);
+    ev_outer.push_back(g_outer_tag);
     return 0;
 }
 
@@ -1492,7 +1536,8 @@ static int resid64(double* out_norm, int to) {
     nb = (int)((tot + chunk - 1) / chunk);
     double* part = L.partial + L.npartial;
     k_resid64<<<nb, 256, 0, L.s>>>(g_cs, c64[0], c64[1], x64[0], x64[1], to == 1 ? c32[0] : nullptr,
-                                   to == 1 ? c32[1] : nullptr, to == 2 ? g_lco : nullptr, g_ls.nx, part, chunk);
+                                   to == 1 ? c32[1] : nullptr, to == 2 ? g_lco : nullptr, g_ls.nx, part, chunk,
+                                   nullptr);
     TT_LAUNCH();
     int rc = fold_partials(part, nb, 1, 0, &L.ctrl->scratch[0]);
     if (rc) return rc;
@@ -1500,6 +1545,78 @@ static int resid64(double* out_norm, int to) {
     rc = read_ctrl();
     if (rc) return rc;
     *out_norm = L.hctrl->scratch[0];
+    return 0;
+}
+
+// Inner iterations of each correction in the previous line solve (0: none
+// yet), the host's forecast for the next solve.
+static int g_hist_prev[16];
+static int g_nouter_prev = 0;
+
+// The line-sweep defect correction with no host read inside it.  The host
+// queues whole corrections -- the inner solve's iterations, the test of its
+// last one, the correction added (which zeroes the line unknowns for the
+// next inner solve), the FP64 residual, and its fold with the outer step --
+// and reads the control block once after each round of them.  An inner
+// solve gets the iterations the same correction took in the previous solve
+// plus one (6 for corrections it did not reach): its sweeps return at once
+// after its own test ends it, and if it is not over by then its correction
+// is added as it stands and the device's outer step judges the result as
+// for any other.  The first round queues as many corrections as the
+// previous solve took (a correction queued after the loop has ended costs a
+// launch per kernel, about as much as one more read), later rounds one.
+static int line_chain(int stag, int max_outer) {
+    int nb = L.npartial;
+    const long long tot = 2 * g_cs.ncell;
+    int chunk = (int)((tot + nb - 1) / nb);
+    chunk = ((chunk + 255) / 256) * 256;
+    nb = (int)((tot + chunk - 1) / chunk);
+    double* part = L.partial + L.npartial;
+    TT_HIP(hipMemsetAsync(g_lx, 0, line_x_len() * sizeof(float), L.s));
+    g_line = 1;
+    g_pn = g_lnb;
+    int rc = 0, o = 0;
+    for (int round = 0; rc == 0; ++round) {
+        const int nq = round == 0 ? (g_nouter_prev > 0 ? g_nouter_prev : 2) : 1;
+        ++g_bid;
+        for (int q = 0; q < nq && rc == 0; ++q, ++o) {
+            const int ni = (o < 16 && g_hist_prev[o] > 0) ? g_hist_prev[o] + 1 : 6;
+            g_outer_tag = o;
+            for (int b = 0; b < ni && rc == 0; ++b) {
+                // every iteration tested, inside the next red sweep
+                rc = launch_sweep<float>(0, c32[0], c32[1], x32[0], x32[1], L.opts.line_omega, b & 1, b, b > 0 ? 1 : 0,
+                                         1);
+                if (rc == 0)
+                    rc = launch_sweep<float>(1, c32[0], c32[1], x32[0], x32[1], L.opts.line_omega, b & 1, b, 0, 1);
+            }
+            if (rc) break;
+            k_finish<<<1, 256, 0, L.s>>>(L.ctrl, L.partial + ((ni - 1) & 1) * 2 * g_nb, 2 * g_pn, 1, nullptr);
+            TT_LAUNCH();
+            for (int cc = 0; cc < 2; ++cc) {
+                k_add_corr_line<<<nblk(g_cs.ncell, 256), 256, 0, L.s>>>(g_cs, cc, g_ls.nx, x64[cc], g_lx, L.ctrl);
+                TT_LAUNCH();
+            }
+            k_resid64<<<nb, 256, 0, L.s>>>(g_cs, c64[0], c64[1], x64[0], x64[1], nullptr, nullptr, g_lco, g_ls.nx,
+                                           part, chunk, L.ctrl);
+            TT_LAUNCH();
+            k_outer_fold<<<1, 256, 0, L.s>>>(L.ctrl, part, nb, L.p.epsmat, L.opts.inner_tol, L.opts.inner_max, stag,
+                                             max_outer, L.p.itmax);
+            TT_LAUNCH();
+        }
+        g_outer_tag = -1;
+        if (rc) break;
+        rc = ctrl_to_host();
+        if (rc) break;
+        if (L.timing) ev_drain(L.hctrl, g_bid);
+        if (L.hctrl->outer_done) break;
+        if (o > max_outer + 4) rc = fail(12, "line defect correction: the device did not end the outer loop");
+    }
+    g_outer_tag = -1;
+    g_line = 0;
+    g_pn = g_nb;
+    if (rc) return rc;
+    g_nouter_prev = L.hctrl->outer;
+    for (int q = 0; q < 16; ++q) g_hist_prev[q] = L.hctrl->hist[q];
     return 0;
 }
 
@@ -1661,50 +1778,40 @@ int sor_solve_impl(int fill_from_evals, tt_sor_stats* st, int mode) {
         // The outer loop's decisions run on the device (k_outer_step): the
         // host queues a correction's kernels and the next inner solve without
         // reading the FP64 residual first, and learns that the loop has
-        // ended from the inner solve's control-block snapshots.  FP32 sweeps
-        // stop reducing the residual at their rounding floor: an inner solve
-        // hands over when its ratio has not fallen by 1% over 256 iterations
-        // (8 for line sweeps, which gain a decade in a few).
+        // ended from the inner solve's control-block snapshots (point
+        // sweeps) or from one read after all the corrections it queued
+        // (line sweeps, line_chain).  FP32 sweeps stop reducing the residual
+        // at their rounding floor: an inner solve hands over when its ratio
+        // has not fallen by 1% over 256 iterations (8 for line sweeps, which
+        // gain a decade in a few).
         const int stag = line ? 8 : 256, max_outer = 200;
         k_outer_step<<<1, 64, 0, L.s>>>(L.ctrl, s.guess_used ? 2 : 1, L.p.epsmat, L.opts.inner_tol, L.opts.inner_max,
                                         stag, max_outer, L.p.itmax);
         TT_LAUNCH();
-        for (;;) {
-            // inner: A c = r in FP32 from c = 0, L1 during-sweep ratio vs sum|r|
-            if (line) {
-                TT_HIP(hipMemsetAsync(g_lx, 0, line_x_len() * sizeof(float), L.s));
-            } else {
+        if (line) {
+            rc = line_chain(stag, max_outer);
+            if (rc) return rc;
+        } else {
+            for (;;) {
+                // inner: A c = r in FP32 from c = 0, L1 during-sweep ratio vs sum|r|
                 TT_HIP(hipMemsetAsync(x32[0], 0, pl * sizeof(float), L.s));
                 TT_HIP(hipMemsetAsync(x32[1], 0, pl * sizeof(float), L.s));
-            }
-            L.hctrl->itmax = L.opts.inner_max;  // the host's guard in run_sweeps
-            int it32 = 0;
-            double rn32 = 0;
-            if (line) {
-                // every iteration tested (inside the next red sweep), batches of 2
-                g_line = 1;
-                g_pn = g_lnb;
-                rc = run_sweeps<float>(c32[0], c32[1], x32[0], x32[1], L.opts.line_omega, &it32, &rn32, 1, true,
-                                       false, 2);
-                g_line = 0;
-                g_pn = g_nb;
-            } else {
+                L.hctrl->itmax = L.opts.inner_max;  // the host's guard in run_sweeps
+                int it32 = 0;
+                double rn32 = 0;
                 rc = run_sweeps<float>(c32[0], c32[1], x32[0], x32[1], L.p.sorparam, &it32, &rn32, 4, true);
-            }
-            if (rc) return rc;
-            if (L.hctrl->outer_done) break;  // the loop had ended: these sweeps returned at once
-            for (int cc = 0; cc < 2; ++cc) {
-                if (line)
-                    k_add_corr_line<<<nblk(g_cs.ncell, 256), 256, 0, L.s>>>(g_cs, cc, g_ls.nx, x64[cc], g_lx);
-                else
+                if (rc) return rc;
+                if (L.hctrl->outer_done) break;  // the loop had ended: these sweeps returned at once
+                for (int cc = 0; cc < 2; ++cc) {
                     k_add_corr<<<nblk(g_cs.ncell, 256), 256, 0, L.s>>>(g_cs, x64[cc], x32[cc]);
+                    TT_LAUNCH();
+                }
+                rc = resid64(nullptr, to);
+                if (rc) return rc;
+                k_outer_step<<<1, 64, 0, L.s>>>(L.ctrl, 0, L.p.epsmat, L.opts.inner_tol, L.opts.inner_max, stag,
+                                                max_outer, L.p.itmax);
                 TT_LAUNCH();
             }
-            rc = resid64(nullptr, to);
-            if (rc) return rc;
-            k_outer_step<<<1, 64, 0, L.s>>>(L.ctrl, 0, L.p.epsmat, L.opts.inner_tol, L.opts.inner_max, stag,
-                                            max_outer, L.p.itmax);
-            TT_LAUNCH();
         }
         const double rhs_norm = L.hctrl->rhs64, rn = L.hctrl->rn64;
         const int total32 = L.hctrl->total;
