@@ -870,6 +870,22 @@ __global__ void k_store_x(Geo g, CS cs, const double* __restrict__ x0, const dou
     fld[g.idx(i, j, k)] = x[(long long)cs.px * ((j - 1) + (long long)cs.ny * k) + h + 2];
 }
 
+// unknowns of both colours -> a whole ghosted field: the interior from x,
+// every other cell 0 (a zero fill and k_store_x in one pass)
+__global__ void k_store_x_box(Geo g, CS cs, const double* __restrict__ x0, const double* __restrict__ x1,
+                              double* fld) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.n3) return;
+    int i, j, k;
+    box_ijk(g, t, i, j, k);
+    double v = 0.0;
+    if (i >= 1 && i <= g.nx && j >= 1 && j <= g.ny && k >= 1 && k <= g.nz) {
+        const double* x = ((i + j + k) & 1) ? x1 : x0;
+        v = x[(long long)cs.px * ((j - 1) + (long long)cs.ny * k) + ((i - 1) >> 1) + 2];
+    }
+    fld[t] = v;
+}
+
 // PLANETSHIFT for one rank (ppshift.f:1009-1214): periodic phi ghosts for
 // k = 1..nz, i = low..nx+2; skipm1 for arrays without a j = -1 plane.
 __global__ void k_planetshift(double* a, Geo g, int low, int skipm1) {
@@ -1045,7 +1061,7 @@ __global__ void k_scale(double* a, double s, long long n) {
 // max(inner_tol, min(0.5 eps64 sum|RHS| / r, 0.5)).  When the loop stops,
 // done is set, so the inner solve's sweeps the host has queued return at once.
 __device__ inline void outer_step_body(Ctrl* c, int first, double eps64, double inner_tol, int inner_max, int stag,
-                                       int max_outer, int itmax) {
+                                       int max_outer, int itmax, int restore) {
     if (first) {
         c->rhs64 = c->rhsnorm;
         c->outer = 0;
@@ -1065,6 +1081,11 @@ __device__ inline void outer_step_body(Ctrl* c, int first, double eps64, double 
     if (stop) {
         c->outer_done = stop;
         c->done = 1;
+        if (restore) {  // the FP64 solve's fields, as the reference's solve leaves them
+            c->rhsnorm = c->rhs64;
+            c->it = c->total;
+            c->done = (rn / c->rhs64 <= eps64) ? 1 : 2;
+        }
         return;
     }
     c->outer++;
@@ -1084,9 +1105,9 @@ __device__ inline void outer_step_body(Ctrl* c, int first, double eps64, double 
 }
 
 __global__ void k_outer_step(Ctrl* c, int first, double eps64, double inner_tol, int inner_max, int stag,
-                             int max_outer, int itmax) {
+                             int max_outer, int itmax, int restore) {
     if (threadIdx.x != 0) return;
-    outer_step_body(c, first, eps64, inner_tol, inner_max, stag, max_outer, itmax);
+    outer_step_body(c, first, eps64, inner_tol, inner_max, stag, max_outer, itmax, restore);
 }
 
 // The fold of k_resid64's block sums into scratch[0] (k_fold's order, so
@@ -1102,7 +1123,7 @@ __global__ __launch_bounds__(256) void k_outer_fold(Ctrl* c, const double* __res
     s = block_reduce256(s, sh, 0);
     if (threadIdx.x != 0) return;
     c->scratch[0] = s;
-    outer_step_body(c, 0, eps64, inner_tol, inner_max, stag, max_outer, itmax);
+    outer_step_body(c, 0, eps64, inner_tol, inner_max, stag, max_outer, itmax, 1);
 }
 
 // ------------------------------------------------------------- host side
@@ -1786,7 +1807,7 @@ int sor_solve_impl(int fill_from_evals, tt_sor_stats* st, int mode) {
         // gain a decade in a few).
         const int stag = line ? 8 : 256, max_outer = 200;
         k_outer_step<<<1, 64, 0, L.s>>>(L.ctrl, s.guess_used ? 2 : 1, L.p.epsmat, L.opts.inner_tol, L.opts.inner_max,
-                                        stag, max_outer, L.p.itmax);
+                                        stag, max_outer, L.p.itmax, line ? 1 : 0);
         TT_LAUNCH();
         if (line) {
             rc = line_chain(stag, max_outer);
@@ -1809,7 +1830,7 @@ int sor_solve_impl(int fill_from_evals, tt_sor_stats* st, int mode) {
                 rc = resid64(nullptr, to);
                 if (rc) return rc;
                 k_outer_step<<<1, 64, 0, L.s>>>(L.ctrl, 0, L.p.epsmat, L.opts.inner_tol, L.opts.inner_max, stag,
-                                                max_outer, L.p.itmax);
+                                                max_outer, L.p.itmax, 0);
                 TT_LAUNCH();
             }
         }
@@ -1819,19 +1840,24 @@ int sor_solve_impl(int fill_from_evals, tt_sor_stats* st, int mode) {
         s.fp32_sweeps = total32;
         s.outer_iter = L.hctrl->outer;
         s.rnorm = rn;
-        // restore the FP64 control block's solve fields
-        L.hctrl->rhsnorm = rhs_norm;
-        L.hctrl->it = total32;
-        L.hctrl->done = (rn / rhs_norm <= L.p.epsmat) ? 1 : 2;
-        if ((rc = ctrl_put())) return rc;
+        // the FP64 control block's solve fields: the line path's outer step
+        // restored them on the device when it ended the loop (the snapshot
+        // holds them); the point path restores them here
+        if (!line) {
+            L.hctrl->rhsnorm = rhs_norm;
+            L.hctrl->it = total32;
+            L.hctrl->done = (rn / rhs_norm <= L.p.epsmat) ? 1 : 2;
+            if ((rc = ctrl_put())) return rc;
+        }
     }
     s.rhsnorm = L.hctrl->rhsnorm;
     if (mode == 0 && !s.guess_used && s.r0_cold == 0.0) s.r0_cold = s.rhsnorm;  // zero start: residual = RHS
-    // unknowns back into the ghosted field, then the phi wrap
-    if (mode == 0) {
-        TT_HIP(hipMemsetAsync(target, 0, (size_t)g.n3 * sizeof(double), L.s));
-    }
-    k_store_x<<<nblk(2 * g_cs.ncell, 256), 256, 0, L.s>>>(g, g_cs, x64[0], x64[1], target);
+    // unknowns back into the ghosted field (delta_ER: zero outside the
+    // interior), then the phi wrap
+    if (mode == 0)
+        k_store_x_box<<<nblk(g.n3, 256), 256, 0, L.s>>>(g, g_cs, x64[0], x64[1], target);
+    else
+        k_store_x<<<nblk(2 * g_cs.ncell, 256), 256, 0, L.s>>>(g, g_cs, x64[0], x64[1], target);
     TT_LAUNCH();
     rc = planetshift_impl(target, mode == 1 ? 10 : 8);
     if (rc) return rc;
