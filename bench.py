@@ -88,7 +88,7 @@ def adjust_omega(dev, st0, dt, precision):
     return dev.get_sorparam(), time.perf_counter() - t0
 
 
-def run_mode(dev, st0, dt, steps, warmup, precision, guess, event_period=50, inner=0):
+def run_mode(dev, st0, dt, steps, warmup, precision, guess, event_period=51, inner=0):
     dev.load_state(st0)
     dev.set_solver(precision=precision, guess=guess, batch=16, inner_tol=1e-5, inner_max=20000, inner=inner)
     iii = 2
@@ -335,8 +335,9 @@ def main():
     ap.add_argument("--adjust", action=argparse.BooleanOptionalAction, default=True,
                     help="also run with omega from ADJUST_SOR_PARAM (the reference's step-1 search)")
     # an event-bracketed launch leaves the queue idle for ~10 us around it:
-    # one launch in 50 keeps that below 1% of a step in every mode
-    ap.add_argument("--event-period", type=int, default=50,
+    # one launch in 51 keeps that below 1% of a step in every mode (odd, so
+    # the samples alternate red and black half sweeps)
+    ap.add_argument("--event-period", type=int, default=51,
                     help="bracket every N-th SOR half-sweep launch with HIP events (0: none, no roofline)")
     # every host CPU this process may use, up to the 16 a one-GPU share holds
     ap.add_argument("--cpu-threads", type=int, default=min(16, len(os.sched_getaffinity(0))))
