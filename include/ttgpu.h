@@ -269,13 +269,17 @@ typedef struct tt_energy_stats {
     double guess_r0_first;   /* its L1 residual / the cold start's (0: no guess) */
     int32_t first_start;     /* start the first solve used: 0 cold, 1 warm,
                                 2 learned                                     */
-    int32_t pad0;
+    int32_t first_choice;    /* start it was given (a rejected guess falls
+                                back to cold): 0 cold, 1 warm, 2 learned      */
     /* device time of the call's phases, summed over the Newton iterations,
      * ms: the FLD build (FLUX_BC, CALCULATEFLUX with DIFF/FLIM, the Newton
      * terms, FILL_EVALUES, FILL_SOR_MATRIX), SOLVE_WITH_SOR (with the
      * initial guess and, when switched on, ADJUST_SOR_PARAM), and the
      * delta_Q / Q, E update with the Q/E boundaries */
     double ms_build, ms_sor, ms_update;
+    double ms_sor_first;     /* SOLVE_WITH_SOR phase of the first Newton
+                                iteration, ms (guess 2 picks warm or cold by
+                                its moving average)                           */
 } tt_energy_stats;
 
 /* COUPLED_ENERGY_EQS (ppenergy.f:2-791).  iii is the step counter III (read

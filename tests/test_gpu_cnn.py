@@ -120,10 +120,11 @@ def test_adversarial_guess_falls_back_to_cold_start():
 def test_rejected_network_is_skipped_then_retried():
     """After 3 first solves in a row reject the learned guess, the network
     runs only on every 16th first solve.  The others start warm or cold,
-    whichever has taken fewer SOR iterations on recent first solves (moving
-    averages with weight 1/4; an untried warm start is tried first; every
-    16th such solve takes the other start).  The policy is replayed here from
-    the per-call statistics.  A rejected guess never changes the solve."""
+    whichever has taken less time on recent first solves (moving averages,
+    weight 1/4, of the first Newton iteration's SOR phase, by the start the
+    solve was given; an untried warm start is tried first; every 16th such
+    solve takes the other start).  The policy is replayed here from the
+    per-call statistics.  A rejected guess never changes the solve."""
     p = P.make_params(16, 32, 16, epsmat=1e-8)
     st = P.generate_state(p, seed=9)
     w, b = init_params(3)
@@ -158,8 +159,12 @@ def test_rejected_network_is_skipped_then_retried():
             else:
                 assert es.first_start == 0  # the wild network's guess is rejected
             starts.append(es.first_start)
-            k = es.first_start
-            avg[k] = float(es.sor_iter_first) if avg[k] == 0.0 else 0.75 * avg[k] + 0.25 * es.sor_iter_first
+            assert es.first_choice == (2 if not es.learned_skipped else (1 if staged else 0))
+            k = es.first_choice
+            if k < 2:
+                t = es.ms_sor_first
+                assert t > 0.0
+                avg[k] = t if avg[k] == 0.0 else 0.75 * avg[k] + 0.25 * t
     finally:
         d.close()
         ttgpu.load_library().tt_cnn_release()

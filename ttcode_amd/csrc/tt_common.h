@@ -136,8 +136,9 @@ struct Lib {
     tt_viscosity visc{};            // ETACALC/VISC in FORGAS3D (off: ivis 0)
     int av_valid = 0;               // ctrl->av_min holds a NR_ARTIFICIAL_VISC result
     // guess 2 without the network: moving averages of the first solve's SOR
-    // iterations from the cold [0] and the warm [1] start (0: none yet), and
-    // the count of such solves (every 16th tries the start not preferred)
+    // phase time (ms: the guess's check and the solve) from the cold [0] and
+    // the warm [1] start (0: none yet), and the count of such solves (every
+    // 16th tries the start not preferred)
     double start_avg[2] = {0.0, 0.0};
     int start_picks = 0;
     std::string err;

@@ -1241,6 +1241,7 @@ int energy_impl(double delt, int iii, tt_energy_stats* st) {
     TT_HIP(hipMemcpyAsync(f[TT_ER_OLD], f[TT_ER], (size_t)g.n3 * sizeof(double), hipMemcpyDeviceToDevice, L.s));
     TT_HIP(hipMemcpyAsync(L.VD, f[TT_Q], (size_t)g.n3 * sizeof(double), hipMemcpyDeviceToDevice, L.s));  // Q_old
     int srcode = 0;
+    int first_choice = 0;      // the first solve's start: 0 cold, 1 warm, 2 learned
     hipEvent_t* ev = L.en_ev;  // phase boundaries of one Newton iteration
     for (;;) {
         L.energy_iter++;
@@ -1272,10 +1273,11 @@ int energy_impl(double delt, int iii, tt_energy_stats* st) {
         // first correction of the previous step (guess 1), or the learned
         // prediction (guess 2, when a network is loaded).  When guess 2 does
         // not run the network, the first solve starts warm or cold, whichever
-        // has taken fewer SOR iterations on recent first solves; every 16th
-        // such solve takes the other, so its average stays current.  The SOR
-        // solve keeps a guess only if its residual is not above the cold
-        // start's.
+        // has taken less time on recent first solves (the SOR phase: the
+        // guess's check and the solve; with line sweeps both starts take the
+        // same iterations and the check is the difference); every 16th such
+        // solve takes the other, so its average stays current.  The SOR solve
+        // keeps a guess only if its residual is not above the cold start's.
         L.have_prev_guess = 0;
         if (L.opts.guess != 0) {
             if (L.energy_iter > 1) {
@@ -1309,6 +1311,7 @@ int energy_impl(double delt, int iii, tt_energy_stats* st) {
                 }
             }
         }
+        if (L.energy_iter == 1) first_choice = L.have_prev_guess;
         tt_sor_stats ss{};
         int src = sor_solve_impl(0, &ss, 0);
         if (src > 0) return src;
@@ -1325,10 +1328,6 @@ int energy_impl(double delt, int iii, tt_energy_stats* st) {
         if (L.energy_iter == 1) {
             es.sor_iter_first = ss.sor_iter;
             es.first_start = ss.guess_used;
-            if (L.opts.guess == 2 && ss.guess_used < 2) {
-                double& a = L.start_avg[ss.guess_used];
-                a = a == 0.0 ? (double)ss.sor_iter : 0.75 * a + 0.25 * (double)ss.sor_iter;
-            }
             es.guess_scale_first = ss.guess_scale;
             es.guess_r0_first = (L.have_prev_guess && ss.r0_cold > 0.0) ? ss.r0_guess / ss.r0_cold : 0.0;
         }
@@ -1357,6 +1356,14 @@ int energy_impl(double delt, int iii, tt_energy_stats* st) {
         es.ms_build += t01;
         es.ms_sor += t12;
         es.ms_update += t23;
+        if (L.energy_iter == 1) {
+            es.first_choice = first_choice;
+            es.ms_sor_first = t12;
+            if (L.opts.guess == 2 && first_choice < 2) {  // the start the first solve was given
+                double& a = L.start_avg[first_choice];
+                a = a == 0.0 ? (double)t12 : 0.75 * a + 0.25 * (double)t12;
+            }
+        }
         es.max_dq = L.hctrl->maxdq;
         es.max_de = L.hctrl->maxde;
         es.energy_iter = L.energy_iter;

@@ -26,8 +26,8 @@ module ttgpu_c
      integer(c_int32_t) :: energy_iter, sor_iter_total, sor_iter_last, sor_iter_first
      integer(c_int32_t) :: fp32_sweeps, fallbacks, learned_used, learned_skipped
      real(c_double) :: max_dq, max_de, guess_scale_first, guess_r0_first
-     integer(c_int32_t) :: first_start, pad0
-     real(c_double) :: ms_build, ms_sor, ms_update
+     integer(c_int32_t) :: first_start, first_choice
+     real(c_double) :: ms_build, ms_sor, ms_update, ms_sor_first
   end type tt_energy_stats
 
   type, bind(C) :: tt_viscosity

@@ -103,8 +103,9 @@ class EnergyStats(ctypes.Structure):
                 ("learned_used", ctypes.c_int32), ("learned_skipped", ctypes.c_int32),
                 ("max_dq", ctypes.c_double), ("max_de", ctypes.c_double),
                 ("guess_scale_first", ctypes.c_double), ("guess_r0_first", ctypes.c_double),
-                ("first_start", ctypes.c_int32), ("pad0", ctypes.c_int32),
-                ("ms_build", ctypes.c_double), ("ms_sor", ctypes.c_double), ("ms_update", ctypes.c_double)]
+                ("first_start", ctypes.c_int32), ("first_choice", ctypes.c_int32),
+                ("ms_build", ctypes.c_double), ("ms_sor", ctypes.c_double), ("ms_update", ctypes.c_double),
+                ("ms_sor_first", ctypes.c_double)]
 
 
 class StepStats(ctypes.Structure):
