@@ -855,7 +855,7 @@ __global__ void k_newton_cell(Geo g, Met m, CellArgs a) {
 }
 
 struct UpdArgs {
-    const double *DER, *E1, *E2, *E3, *E4, *E5, *E6, *E16, *DFQ, *FQ, *RH, *XKP;
+    const double *DER, *DFQ, *FQ, *RH, *XKP;
     double *Q, *ER;
     const int32_t* ub;
     double delt, conv;
@@ -864,6 +864,12 @@ struct UpdArgs {
 
 // delta_Q from the delta_ER stencil (ppenergy.f:379-408), the positivity
 // test, the two max reductions and the damped update (ppenergy.f:551-573).
+// The stencil's seven terms are each multiplied by Q1 = -delt*(1 - theta),
+// and theta is fixed at 1 (ppenergy.f:60): each term is a signed zero, and a
+// signed zero added to Q2*d - FQ changes neither the result nor, where that
+// is zero, anything computed from it (q + conv*dq, |dq/q|).  So the kernel
+// forms dq = (Q2*d - FQ)/DFQ and reads neither EVAL1..6, EVAL_1_6 nor the
+// neighbouring delta_ER values (as the reference does for finite ones).
 __global__ __launch_bounds__(256) void k_dq_update(Geo g, UpdArgs a, double* part, int nb, Ctrl* ctrl) {
     __shared__ double sh[2][4];
     double mq = 0.0, me = 0.0;
@@ -876,14 +882,9 @@ __global__ __launch_bounds__(256) void k_dq_update(Geo g, UpdArgs a, double* par
         int u = a.ub[g.idx2(j, k)];
         if (i > u - 1 || i < a.imin) continue;
         long long c = IX(i, j, k);
-        const double theta = 1.0;
-        double Q1 = -a.delt * (1.0 - theta);
         double Q2 = a.delt * a.RH[c] * a.XKP[c] * FD;
         double d = a.DER[c];
-        double dq = (1.0 / a.DFQ[c]) *
-                    (Q1 * a.E16[c] * d - Q1 * a.E1[c] * a.DER[IX(i + 1, j, k)] - Q1 * a.E2[c] * a.DER[IX(i - 1, j, k)] -
-                     Q1 * a.E3[c] * a.DER[IX(i, j + 1, k)] - Q1 * a.E4[c] * a.DER[IX(i, j - 1, k)] -
-                     Q1 * a.E5[c] * a.DER[IX(i, j, k + 1)] - Q1 * a.E6[c] * a.DER[IX(i, j, k - 1)] + Q2 * d - a.FQ[c]);
+        double dq = (1.0 / a.DFQ[c]) * (Q2 * d - a.FQ[c]);
         double q = a.Q[c], er = a.ER[c];
         if ((q + a.conv * dq) <= 0.0) bad = 1;
         mq = fmax(mq, fabs(dq / q));
@@ -1339,8 +1340,7 @@ int energy_impl(double delt, int iii, tt_energy_stats* st) {
         // reductions: the result does not depend on the block count)
         const long long ncell = (long long)g.nx * g.ny * g.nz;
         int nb = (int)std::min<long long>(4096, (ncell + 255) / 256);
-        UpdArgs ua{f[TT_DELTA_ER], f[TT_EVAL1], f[TT_EVAL2], f[TT_EVAL3], f[TT_EVAL4], f[TT_EVAL5], f[TT_EVAL6],
-                   f[TT_EVAL_1_6], L.DFQ, f[TT_FQ], f[TT_RH], f[TT_XKAPP], f[TT_Q], f[TT_ER], L.ub, delt, conv_parm,
+        UpdArgs ua{f[TT_DELTA_ER], L.DFQ, f[TT_FQ], f[TT_RH], f[TT_XKAPP], f[TT_Q], f[TT_ER], L.ub, delt, conv_parm,
                    L.p.bcxmin == 12 ? 1 : 2};
         k_dq_update<<<nb, 256, 0, L.s>>>(g, ua, L.partial, nb, L.ctrl);
         TT_LAUNCH();

@@ -10,6 +10,7 @@
 // face between its two cells through LDS.  The arithmetic is the
 // reference's, face by face.
 #include <cmath>
+#include <utility>
 
 #include "tt_common.h"
 
@@ -620,10 +621,21 @@ __global__ void k_centcor(Geo g, Met m, double om, const double* __restrict__ G,
     CCZ[c] = -m(TT_M_XXB, i) * GQ * GQ * sin(za) * cos(za);
 }
 
-// p dV work: T -= fac*dt PG/RH/CV div(v) over cells lo..hi in each axis;
-// with DISFN (dissipative heating, IDISS) then T += fac*dt DISFN/CV/RH on the
-// interior, as the reference's following loop does cell by cell
-// (ppfor.f:202-210,335-343)
+// p dV work of cell (i, j, k): T - fac*dt PG/RH/CV div(v); with DISFN
+// (dissipative heating, IDISS) then + fac*dt DISFN/CV/RH on the interior, as
+// the reference's following loop does cell by cell (ppfor.f:190-210,335-343)
+__device__ inline double pdv_cell(const Geo& g, const Met& m, double dtf, double t, long long c, int i, int j, int k,
+                                  const double* __restrict__ PG, const double* __restrict__ RH,
+                                  const double* __restrict__ CV, const double* __restrict__ V,
+                                  const double* __restrict__ G, const double* __restrict__ H,
+                                  const double* __restrict__ DISFN) {
+    double tn = t - dtf * PG[c] / RH[c] / CV[c] * divv(g, m, V, G, H, i, j, k);
+    if (DISFN && i >= 1 && i <= g.nx && j >= 1 && j <= g.ny && k >= 1 && k <= g.nz)
+        tn = tn + dtf * DISFN[c] / CV[c] / RH[c];
+    return tn;
+}
+
+// p dV work in place over cells lo..hi in each axis (hi = n + hiadd)
 __global__ void k_pdv(Geo g, Met m, double dtf, int lo, int hiadd, double* T, const double* __restrict__ PG,
                       const double* __restrict__ RH, const double* __restrict__ CV, const double* __restrict__ V,
                       const double* __restrict__ G, const double* __restrict__ H, const double* __restrict__ DISFN) {
@@ -634,16 +646,31 @@ __global__ void k_pdv(Geo g, Met m, double dtf, int lo, int hiadd, double* T, co
     int i, j, k;
     sub_ijk(t, ni, nj, lo, lo, lo, i, j, k);
     long long c = IX(i, j, k);
-    double tn = T[c] - dtf * PG[c] / RH[c] / CV[c] * divv(g, m, V, G, H, i, j, k);
-    if (DISFN && i >= 1 && i <= g.nx && j >= 1 && j <= g.ny && k >= 1 && k <= g.nz)
-        tn = tn + dtf * DISFN[c] / CV[c] / RH[c];
-    T[c] = tn;
+    T[c] = pdv_cell(g, m, dtf, T[c], c, i, j, k, PG, RH, CV, V, G, H, DISFN);
+}
+
+// The same over cells lo..hi, written out of place: TN over the whole box
+// holds T with the work done there and T's own value elsewhere, so TN is
+// exactly what the in-place form leaves in T while T keeps its old values.
+__global__ void k_pdv_box(Geo g, Met m, double dtf, int lo, int hiadd, const double* __restrict__ T,
+                          double* __restrict__ TN, const double* __restrict__ PG, const double* __restrict__ RH,
+                          const double* __restrict__ CV, const double* __restrict__ V, const double* __restrict__ G,
+                          const double* __restrict__ H, const double* __restrict__ DISFN) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.n3) return;
+    int i, j, k;
+    box_ijk(g, t, i, j, k);
+    const bool in = i >= lo && i <= g.nx + hiadd && j >= lo && j <= g.ny + hiadd && k >= lo && k <= g.nz + hiadd;
+    TN[t] = in ? pdv_cell(g, m, dtf, T[t], t, i, j, k, PG, RH, CV, V, G, H, DISFN) : T[t];
 }
 
 // VH, GH, HH (ppfor.f:217-278); DX/DY/DZ: divtx/y/z, null without viscosity
-// (the reference's zero arrays)
-__global__ void k_force_vh(Geo g, Met m, double dt, const double* __restrict__ V, const double* __restrict__ G,
-                           const double* __restrict__ H, const double* __restrict__ PG, const double* __restrict__ QX,
+// (the reference's zero arrays).  Each cell reads only its own V, G, H, so
+// the kernel also applies the update that follows in place: avg != 0 sets
+// V = (V+VH)/2 etc. and stores VH, GH, HH for the second half (ppfor.f:
+// 322-328); avg == 0 sets V = VH etc. (the isothermal case, :346-352).
+__global__ void k_force_vh(Geo g, Met m, double dt, int avg, double* V, double* G, double* H,
+                           const double* __restrict__ PG, const double* __restrict__ QX,
                            const double* __restrict__ QY, const double* __restrict__ QZ, const double* __restrict__ CCX,
                            const double* __restrict__ CCY, const double* __restrict__ CCZ,
                            const double* __restrict__ GX, const double* __restrict__ GY, const double* __restrict__ GZ,
@@ -657,28 +684,22 @@ __global__ void k_force_vh(Geo g, Met m, double dt, const double* __restrict__ V
     long long c = IX(i, j, k);
     double gx = m(TT_M_GEOXG, i), gz = m(TT_M_GEOZG, k), gh = m(TT_M_GEOXH, i);
     const double dx = DX ? DX[c] : 0.0, dy = DY ? DY[c] : 0.0, dz = DZ ? DZ[c] : 0.0;
-    VH[c] = V[c] + dt * ((-(PG[c] - PG[IX(i - 1, j, k)]) / m(TT_M_DXB, i) + dx) / QX[c] + CCX[c] + GX[c]);
-    GH[c] = G[c] + dt * (((-(PG[c] - PG[IX(i, j - 1, k)]) / m(TT_M_DYB, j) / gx / gz + dy) / QY[c] + CCY[c] + GY[c]) / gx / gz);
-    HH[c] = H[c] + dt * (((-(PG[c] - PG[IX(i, j, k - 1)]) / m(TT_M_DZB, k) / gh + dz) / QZ[c] + CCZ[c] + GZ[c]) / gh);
-}
-
-// mode 0: V = (V+VH)/2 etc.; mode 1: V = VH etc.
-__global__ void k_force_set(Geo g, int mode, double* V, double* G, double* H, const double* __restrict__ VH,
-                            const double* __restrict__ GH, const double* __restrict__ HH) {
-    long long n = (long long)g.nx * g.ny * g.nz;
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    int i, j, k;
-    sub_ijk(t, g.nx, g.ny, 1, 1, 1, i, j, k);
-    long long c = IX(i, j, k);
-    if (mode == 0) {
-        V[c] = (V[c] + VH[c]) / 2.0;
-        G[c] = (G[c] + GH[c]) / 2.0;
-        H[c] = (H[c] + HH[c]) / 2.0;
+    const double v = V[c], gv = G[c], hv = H[c];
+    const double vh = v + dt * ((-(PG[c] - PG[IX(i - 1, j, k)]) / m(TT_M_DXB, i) + dx) / QX[c] + CCX[c] + GX[c]);
+    const double gvh =
+        gv + dt * (((-(PG[c] - PG[IX(i, j - 1, k)]) / m(TT_M_DYB, j) / gx / gz + dy) / QY[c] + CCY[c] + GY[c]) / gx / gz);
+    const double hvh = hv + dt * (((-(PG[c] - PG[IX(i, j, k - 1)]) / m(TT_M_DZB, k) / gh + dz) / QZ[c] + CCZ[c] + GZ[c]) / gh);
+    if (avg) {
+        VH[c] = vh;
+        GH[c] = gvh;
+        HH[c] = hvh;
+        V[c] = (v + vh) / 2.0;
+        G[c] = (gv + gvh) / 2.0;
+        H[c] = (hv + hvh) / 2.0;
     } else {
-        V[c] = VH[c];
-        G[c] = GH[c];
-        H[c] = HH[c];
+        V[c] = vh;
+        G[c] = gvh;
+        H[c] = hvh;
     }
 }
 
@@ -695,32 +716,45 @@ int forgas_impl(double dt) {
     k_centcor<<<nblk(nc, 256), 256, 0, L.s>>>(g, met(), L.p.omrot, f[TT_G], f[TT_H], f[TT_CENTCORX], f[TT_CENTCORY],
                                               f[TT_CENTCORZ]);
     TT_LAUNCH();
-    TT_HIP(hipMemcpyAsync(L.TOLD, f[TT_T], (size_t)g.n3 * sizeof(double), hipMemcpyDeviceToDevice, L.s));
-    if (L.p.isoth == 0) {
-        long long nh = (long long)(g.nx + 2) * (g.ny + 2) * (g.nz + 2);
-        k_pdv<<<nblk(nh, 256), 256, 0, L.s>>>(g, met(), dt / 2.0, 0, 1, f[TT_T], f[TT_PG], f[TT_RH], f[TT_CV], f[TT_V],
-                                              f[TT_G], f[TT_H], disfn);
+    const bool adiab = L.p.isoth == 0;
+    if (adiab) {
+        // TOLD = T, then T at t(n+1/2) by the p dV work: formed out of place
+        // into TOLD's buffer, after which the two buffers trade places, so T
+        // holds the half-step temperature and TOLD the saved one without a
+        // copy; they trade back where the reference sets T = TOLD
+        k_pdv_box<<<nblk(g.n3, 256), 256, 0, L.s>>>(g, met(), dt / 2.0, 0, 1, f[TT_T], L.TOLD, f[TT_PG], f[TT_RH],
+                                                    f[TT_CV], f[TT_V], f[TT_G], f[TT_H], disfn);
         TT_LAUNCH();
-        if ((rc = bounds_impl())) return rc;
-        if ((rc = pressg_impl())) return rc;
+        std::swap(f[TT_T], L.TOLD);
+        if ((rc = bounds_impl()) || (rc = pressg_impl())) {
+            std::swap(f[TT_T], L.TOLD);
+            return rc;
+        }
     }
-    k_force_vh<<<nblk(nc, 256), 256, 0, L.s>>>(g, met(), dt, f[TT_V], f[TT_G], f[TT_H], f[TT_PG], f[TT_RHQX], f[TT_RHQY],
-                                               f[TT_RHQZ], f[TT_CENTCORX], f[TT_CENTCORY], f[TT_CENTCORZ], f[TT_GRAVX],
-                                               f[TT_GRAVY], f[TT_GRAVZ], vis ? f[TT_DIVTX] : nullptr,
-                                               vis ? f[TT_DIVTY] : nullptr, vis ? f[TT_DIVTZ] : nullptr, L.VH, L.GH,
-                                               L.HH);
+    // VH, GH, HH, and V = (V+VH)/2 etc. (adiabatic) or V = VH etc.
+    k_force_vh<<<nblk(nc, 256), 256, 0, L.s>>>(g, met(), dt, adiab ? 1 : 0, f[TT_V], f[TT_G], f[TT_H], f[TT_PG],
+                                               f[TT_RHQX], f[TT_RHQY], f[TT_RHQZ], f[TT_CENTCORX], f[TT_CENTCORY],
+                                               f[TT_CENTCORZ], f[TT_GRAVX], f[TT_GRAVY], f[TT_GRAVZ],
+                                               vis ? f[TT_DIVTX] : nullptr, vis ? f[TT_DIVTY] : nullptr,
+                                               vis ? f[TT_DIVTZ] : nullptr, L.VH, L.GH, L.HH);
     TT_LAUNCH();
-    TT_HIP(hipMemcpyAsync(f[TT_T], L.TOLD, (size_t)g.n3 * sizeof(double), hipMemcpyDeviceToDevice, L.s));
-    if (L.p.isoth == 0) {
-        k_force_set<<<nblk(nc, 256), 256, 0, L.s>>>(g, 0, f[TT_V], f[TT_G], f[TT_H], L.VH, L.GH, L.HH);
-        TT_LAUNCH();
+    if (adiab) {
+        std::swap(f[TT_T], L.TOLD);  // T = TOLD
+        // the velocity ceiling on the intermediate velocities (ppfor.f:330-342)
+        if (L.lim.ivel != 0 && (rc = sphericalvellimit_impl(L.lim.ivel, L.lim.velmax, L.lim.hydrotime))) return rc;
         if ((rc = bounds_impl())) return rc;
         k_pdv<<<nblk(nc, 256), 256, 0, L.s>>>(g, met(), dt, 1, 0, f[TT_T], f[TT_PG], f[TT_RH], f[TT_CV], f[TT_V], f[TT_G],
                                               f[TT_H], disfn);
         TT_LAUNCH();
+        // V = VH etc. on the interior: the buffers trade places.  The ghost
+        // cells V, G and H then hold are not read before the BOUNDS below,
+        // which rewrites every one of them from interior values and
+        // constants; the velocity ceiling, if on, clamps cell by cell in
+        // between.
+        std::swap(f[TT_V], L.VH);
+        std::swap(f[TT_G], L.GH);
+        std::swap(f[TT_H], L.HH);
     }
-    k_force_set<<<nblk(nc, 256), 256, 0, L.s>>>(g, 1, f[TT_V], f[TT_G], f[TT_H], L.VH, L.GH, L.HH);
-    TT_LAUNCH();
     // the velocity ceiling (ppfor.f:361-371: SPHERICALVELLIMIT for NCOSYS=2)
     if (L.lim.ivel != 0 && (rc = sphericalvellimit_impl(L.lim.ivel, L.lim.velmax, L.lim.hydrotime))) return rc;
     if (L.visc.iartvis > 0) {  // ppfor.f:374-395
