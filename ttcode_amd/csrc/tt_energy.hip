@@ -28,7 +28,7 @@ int cnn_collect_sample(double delt);
 __global__ void k_q_init(double* Q, const double* __restrict__ RH, const double* __restrict__ CV,
                          const double* __restrict__ T, long long n) {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n) Q[t] = RH[t] * CV[t] * T[t];
+    if (t < n) st_nt(&Q[t], RH[t] * CV[t] * T[t]);
 }
 
 // x**y for the power law's exponents: 1/2, 1, 2 and 3 by sqrt and products
@@ -50,10 +50,10 @@ __global__ void k_kappa(Geo g, tt_params p, const int32_t* __restrict__ ub, cons
                         double* DKA, double* SIG, double sig) {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= g.n3) return;
-    SIG[t] = sig;
+    st_nt(&SIG[t], sig);
     if (p.ikaptyp == 1) {
-        XKR[t] = p.zkapmult; XKP[t] = p.zkapmult; XKA[t] = p.zkapmult;
-        DKP[t] = 0.0; DKA[t] = 0.0;
+        st_nt(&XKR[t], p.zkapmult); st_nt(&XKP[t], p.zkapmult); st_nt(&XKA[t], p.zkapmult);
+        st_nt(&DKP[t], 0.0); st_nt(&DKA[t], 0.0);
         return;
     }
     int i, j, k;
@@ -65,8 +65,8 @@ __global__ void k_kappa(Geo g, tt_params p, const int32_t* __restrict__ ub, cons
     const double pa = pow_exp(rr, p.kap_a);
     double kap = p.kap0 * pa * pow_exp(tt, p.kap_b);
     double dk = p.dkap0 * pa * pow_exp(tt, p.kap_b - 1.0);
-    XKR[t] = kap; XKP[t] = kap; XKA[t] = kap;
-    DKP[t] = dk; DKA[t] = dk;
+    st_nt(&XKR[t], kap); st_nt(&XKP[t], kap); st_nt(&XKA[t], kap);
+    st_nt(&DKP[t], dk); st_nt(&DKA[t], dk);
 }
 
 // ---- IKAPTYP 4/5: table opacities
@@ -747,9 +747,9 @@ __global__ __launch_bounds__(256) void k_faces(Geo g, Met m, FaceArgs a) {
                        (m(TT_M_DDZ1, k) * (ki + si) + m(TT_M_DDZ0, k) * (kiz + siz)));
     double fz = -dz * (ei - eiz);
     if (k <= g.nz && i >= u) fz = 0.0;
-    if (hx) { a.FLIM[0][c] = flx; a.DIFR[0][c] = dx; a.FLUX[0][c] = fx; }
-    if (hy) { a.FLIM[1][c] = fly; a.DIFR[1][c] = dy; a.FLUX[1][c] = fy; }
-    if (hz) { a.FLIM[2][c] = flz; a.DIFR[2][c] = dz; a.FLUX[2][c] = fz; }
+    if (hx) { st_nt(&a.FLIM[0][c], flx); st_nt(&a.DIFR[0][c], dx); st_nt(&a.FLUX[0][c], fx); }
+    if (hy) { st_nt(&a.FLIM[1][c], fly); st_nt(&a.DIFR[1][c], dy); st_nt(&a.FLUX[1][c], fy); }
+    if (hz) { st_nt(&a.FLIM[2][c], flz); st_nt(&a.DIFR[2][c], dz); st_nt(&a.FLUX[2][c], fz); }
 }
 
 // The CALCULATEFLUX + FLUX_BC pair before the Newton loop (ppenergy.f:78-79)
@@ -827,8 +827,9 @@ __global__ void k_newton_cell(Geo g, Met m, CellArgs a) {
     double rhs = -fr - eta * fq;
     double e7 = 1.0 + (1.0 - eta) * delt * rh * kp * FD;
     double e8 = (eta * (1.0 - theta) - 1.0) * delt;
-    a.E1[c] = e1; a.E2[c] = e2; a.E3[c] = e3; a.E4[c] = e4; a.E5[c] = e5; a.E6[c] = e6;
-    a.E7[c] = e7; a.E8[c] = e8; a.E16[c] = e16; a.ETA[c] = eta; a.FQ[c] = fq; a.FR[c] = fr; a.DFQ[c] = dfq;
+    st_nt(&a.E1[c], e1); st_nt(&a.E2[c], e2); st_nt(&a.E3[c], e3); st_nt(&a.E4[c], e4); st_nt(&a.E5[c], e5);
+    st_nt(&a.E6[c], e6); st_nt(&a.E7[c], e7); st_nt(&a.E8[c], e8); st_nt(&a.E16[c], e16); st_nt(&a.ETA[c], eta);
+    st_nt(&a.FQ[c], fq); st_nt(&a.FR[c], fr); st_nt(&a.DFQ[c], dfq);
     (void)a.RHS;
     // FILL_SOR_MATRIX + B_VECTOR_BOUNDARIES
     double sa = e7 + e8 * e16, sb = -e8 * e2, sc = -e8 * e1, sd = -e8 * e4, se = -e8 * e3, sf = -e8 * e6,
@@ -889,8 +890,8 @@ __global__ __launch_bounds__(256) void k_dq_update(Geo g, UpdArgs a, double* par
         if ((q + a.conv * dq) <= 0.0) bad = 1;
         mq = fmax(mq, fabs(dq / q));
         me = fmax(me, fabs(d / er));
-        a.ER[c] = er + a.conv * d;
-        a.Q[c] = q + a.conv * dq;
+        st_nt(&a.ER[c], er + a.conv * d);
+        st_nt(&a.Q[c], q + a.conv * dq);
     }
     if (bad) ctrl->bad_q = 1;
     double x = wave_max(mq), y = wave_max(me);
@@ -986,8 +987,8 @@ __global__ void k_energy_final(double* DER, double* T, const double* __restrict_
                                const double* __restrict__ CV, long long n) {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
-    DER[t] = ER[t] - EROLD[t];
-    T[t] = Q[t] / (RH[t] * CV[t]);
+    st_nt(&DER[t], ER[t] - EROLD[t]);
+    st_nt(&T[t], Q[t] / (RH[t] * CV[t]));
 }
 
 // IKAPTYP 4/5 tables on the host (log10 Planck, Rosseland, absorption;
@@ -1222,7 +1223,7 @@ int energy_impl(double delt, int iii, tt_energy_stats* st) {
     L.iii = iii;
     int rc;
     L.energy_iter = 0;
-    TT_HIP(hipMemsetAsync(f[TT_DELTA_ER], 0, (size_t)g.n3 * sizeof(double), L.s));
+    if ((rc = dev_zero(f[TT_DELTA_ER], (size_t)g.n3 * sizeof(double)))) return rc;
     if ((rc = bounds_impl())) return rc;
     k_q_init<<<nblk(g.n3, 256), 256, 0, L.s>>>(f[TT_Q], f[TT_RH], f[TT_CV], f[TT_T], g.n3);
     TT_LAUNCH();
@@ -1239,8 +1240,8 @@ int energy_impl(double delt, int iii, tt_energy_stats* st) {
     TT_LAUNCH();
     if ((rc = energy_bcs(0))) return rc;
     if ((rc = planetshift_impl(f[TT_ER], 6))) return rc;
-    TT_HIP(hipMemcpyAsync(f[TT_ER_OLD], f[TT_ER], (size_t)g.n3 * sizeof(double), hipMemcpyDeviceToDevice, L.s));
-    TT_HIP(hipMemcpyAsync(L.VD, f[TT_Q], (size_t)g.n3 * sizeof(double), hipMemcpyDeviceToDevice, L.s));  // Q_old
+    if ((rc = dev_copy(f[TT_ER_OLD], f[TT_ER], (size_t)g.n3 * sizeof(double)))) return rc;
+    if ((rc = dev_copy(L.VD, f[TT_Q], (size_t)g.n3 * sizeof(double)))) return rc;  // Q_old
     int srcode = 0;
     int first_choice = 0;      // the first solve's start: 0 cold, 1 warm, 2 learned
     hipEvent_t* ev = L.en_ev;  // phase boundaries of one Newton iteration
@@ -1305,8 +1306,7 @@ int energy_impl(double delt, int iii, tt_energy_stats* st) {
                         warm = (++L.start_picks % 16 == 0) ? !prefer_warm : prefer_warm;
                     }
                     if (warm && L.have_de1) {
-                        TT_HIP(hipMemcpyAsync(f[TT_DELTA_ER], L.DE1, (size_t)g.n3 * sizeof(double),
-                                              hipMemcpyDeviceToDevice, L.s));
+                        if ((rc = dev_copy(f[TT_DELTA_ER], L.DE1, (size_t)g.n3 * sizeof(double)))) return rc;
                         L.have_prev_guess = 1;
                     }
                 }
@@ -1318,8 +1318,7 @@ int energy_impl(double delt, int iii, tt_energy_stats* st) {
         if (src > 0) return src;
         if (src < 0) srcode = src;
         if (L.energy_iter == 1) {
-            TT_HIP(hipMemcpyAsync(L.DE1, f[TT_DELTA_ER], (size_t)g.n3 * sizeof(double), hipMemcpyDeviceToDevice,
-                                  L.s));
+            if ((rc = dev_copy(L.DE1, f[TT_DELTA_ER], (size_t)g.n3 * sizeof(double)))) return rc;
             L.have_de1 = 1;
             if ((rc = cnn_collect_sample(delt))) return rc;  // training set of the learned guess, when collecting
         }

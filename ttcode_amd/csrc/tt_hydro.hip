@@ -70,8 +70,8 @@ __global__ void k_adv_mass(Geo g, Met m, double dt, const double* __restrict__ R
             double S = v * dt;
             int ind = v > 0.0 ? i - 1 : i;
             double gr = vl_slope(R(ind - 1, jj, k), R(ind, jj, k), R(ind + 1, jj, k), m(TT_M_DXB, ind), m(TT_M_DXB, ind + 1), 0);
-            FLMX[IX(i, j, k)] = (m(TT_M_SURXA, i) - m(TT_M_CCXA, i) * S) * S *
-                                (R(ind, jj, k) + (m(TT_M_XXA, i) - m(TT_M_XXB, ind) - half * S) * gr);
+            st_nt(&FLMX[IX(i, j, k)], (m(TT_M_SURXA, i) - m(TT_M_CCXA, i) * S) * S *
+                                          (R(ind, jj, k) + (m(TT_M_XXA, i) - m(TT_M_XXB, ind) - half * S) * gr));
         }
     }
     // y faces: i = 1..nx, j = 1..ny+1, k = 1..nz
@@ -80,8 +80,8 @@ __global__ void k_adv_mass(Geo g, Met m, double dt, const double* __restrict__ R
         double S = gv * dt;
         int ind = gv > 0.0 ? j - 1 : j;
         double gr = vl_slope(R(i, ind - 1, k), R(i, ind, k), R(i, ind + 1, k), m(TT_M_DYB, ind), m(TT_M_DYB, ind + 1), 0);
-        FLMY[IX(i, j, k)] = S * (m(TT_M_SURYA, j) + m(TT_M_CCYA, j) * S) *
-                            (R(i, ind, k) + (m(TT_M_XYA, j) - m(TT_M_XYB, ind) - half * S) * gr);
+        st_nt(&FLMY[IX(i, j, k)], S * (m(TT_M_SURYA, j) + m(TT_M_CCYA, j) * S) *
+                                      (R(i, ind, k) + (m(TT_M_XYA, j) - m(TT_M_XYB, ind) - half * S) * gr));
     }
     // z faces: i = 1..nx, j = 0..ny (j = 0 periodic copy), k = 1..nz+1
     if (i <= g.nx && j <= g.ny) {
@@ -90,8 +90,8 @@ __global__ void k_adv_mass(Geo g, Met m, double dt, const double* __restrict__ R
         double S = hv * dt;
         int ind = hv > 0.0 ? k - 1 : k;
         double gr = vl_slope(R(i, jj, ind - 1), R(i, jj, ind), R(i, jj, ind + 1), m(TT_M_DZB, ind), m(TT_M_DZB, ind + 1), 0);
-        FLMZ[IX(i, j, k)] = (m(TT_M_SURZA, k) + m(TT_M_CCZA, k) * S) * S *
-                            (R(i, jj, ind) + (m(TT_M_XZA, k) - m(TT_M_XZB, ind) - half * S) * gr);
+        st_nt(&FLMZ[IX(i, j, k)], (m(TT_M_SURZA, k) + m(TT_M_CCZA, k) * S) * S *
+                                      (R(i, jj, ind) + (m(TT_M_XZA, k) - m(TT_M_XZB, ind) - half * S) * gr));
     }
 }
 
@@ -101,7 +101,7 @@ __global__ void k_adv_mass(Geo g, Met m, double dt, const double* __restrict__ R
 __global__ void k_ed0(const double* __restrict__ T, const double* __restrict__ RH, const double* __restrict__ CV,
                       double* E0, long long n) {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n) E0[t] = T[t] * RH[t] * CV[t];
+    if (t < n) st_nt(&E0[t], T[t] * RH[t] * CV[t]);
 }
 
 // GD = (G + OMROT) r^2 cos^2 (ppadv.f:1078-1084); the phi ghosts equal the
@@ -113,7 +113,7 @@ __global__ void k_gd_init(Geo g, Met m, double om, const double* __restrict__ G,
     box_ijk(g, t, i, j, k);
     if (j < 0) return;
     double gx = m(TT_M_GEOXG, i), gz = m(TT_M_GEOZG, k);
-    GD[t] = (G[t] + om) * (gx * gx) * (gz * gz);
+    st_nt(&GD[t], (G[t] + om) * (gx * gx) * (gz * gz));
 }
 
 __global__ void k_hd_init(Geo g, Met m, const double* __restrict__ H, double* HD) {
@@ -123,7 +123,7 @@ __global__ void k_hd_init(Geo g, Met m, const double* __restrict__ H, double* HD
     box_ijk(g, t, i, j, k);
     if (k < 0) return;
     double gh = m(TT_M_GEOXH, i);
-    HD[t] = H[t] * (gh * gh);
+    st_nt(&HD[t], H[t] * (gh * gh));
 }
 
 // ---- face-shared sweeps.  The four transport updates evaluate, per cell,
@@ -218,10 +218,10 @@ struct AdvE {
     }
     __device__ static void finish(const AdvArgs& a, int n, double vx, double vy, const PM& pm, double fx0, double fx1,
                                   double fy0, double fy1, double fz0, double fz1) {
-        a.out[n] = a.A[n] - (fx1 - fx0) / vx - (fy1 - fy0) / vy - (fz1 - fz0) / pm.volz;
-        a.cf1[n] = fx0;
-        a.cf2[n] = fy0;
-        a.cf3[n] = fz0;
+        st_nt(&a.out[n], a.A[n] - (fx1 - fx0) / vx - (fy1 - fy0) / vy - (fz1 - fz0) / pm.volz);
+        st_nt(&a.cf1[n], fx0);
+        st_nt(&a.cf2[n], fy0);
+        st_nt(&a.cf3[n], fz0);
     }
 };
 
@@ -280,7 +280,7 @@ struct AdvV {
     }
     __device__ static void finish(const AdvArgs& a, int n, double vx, double vy, const PM& pm, double fx0, double fx1,
                                   double fy0, double fy1, double fz0, double fz1) {
-        a.out[n] = a.V[n] * a.RHQ[n] - (fx1 - fx0) / vx - (fy1 - fy0) / vy - (fz1 - fz0) / pm.volz;
+        st_nt(&a.out[n], a.V[n] * a.RHQ[n] - (fx1 - fx0) / vx - (fy1 - fy0) / vy - (fz1 - fz0) / pm.volz);
     }
 };
 
@@ -343,7 +343,7 @@ struct AdvG {
     }
     __device__ static void finish(const AdvArgs& a, int n, double vx, double vy, const PM& pm, double fx0, double fx1,
                                   double fy0, double fy1, double fz0, double fz1) {
-        a.out[n] = a.A[n] * a.RHQ[n] - (fx1 - fx0) / vx - (fy1 - fy0) / vy - (fz1 - fz0) / pm.volz;
+        st_nt(&a.out[n], a.A[n] * a.RHQ[n] - (fx1 - fx0) / vx - (fy1 - fy0) / vy - (fz1 - fz0) / pm.volz);
     }
 };
 
@@ -404,7 +404,7 @@ struct AdvH {
     }
     __device__ static void finish(const AdvArgs& a, int n, double vx, double vy, const PM& pm, double fx0, double fx1,
                                   double fy0, double fy1, double fz0, double fz1) {
-        a.out[n] = a.A[n] * a.RHQ[n] - (fx1 - fx0) / vx - (fy1 - fy0) / vy - (fz1 - fz0) / pm.volz;
+        st_nt(&a.out[n], a.A[n] * a.RHQ[n] - (fx1 - fx0) / vx - (fy1 - fy0) / vy - (fz1 - fz0) / pm.volz);
     }
 };
 
@@ -533,8 +533,9 @@ __global__ void k_adv_rho(Geo g, Met m, double* RH, const double* __restrict__ F
     int i, j, k;
     sub_ijk(t, g.nx, g.ny, 1, 1, 1, i, j, k);
     long long c = IX(i, j, k);
-    RH[c] = RH[c] - (FLMX[IX(i + 1, j, k)] - FLMX[c]) / m(TT_M_VOLXB, i) -
-            (FLMY[IX(i, j + 1, k)] - FLMY[c]) / m(TT_M_VOLYB, j) - (FLMZ[IX(i, j, k + 1)] - FLMZ[c]) / m(TT_M_VOLZB, k);
+    st_nt(&RH[c], RH[c] - (FLMX[IX(i + 1, j, k)] - FLMX[c]) / m(TT_M_VOLXB, i) -
+                      (FLMY[IX(i, j + 1, k)] - FLMY[c]) / m(TT_M_VOLYB, j) -
+                      (FLMZ[IX(i, j, k + 1)] - FLMZ[c]) / m(TT_M_VOLZB, k));
 }
 
 // new V, G, H, T (ppadv.f:1498-1575); GD keeps the updated momentum density
@@ -550,12 +551,12 @@ __global__ void k_adv_final(Geo g, Met m, double om, double* V, double* G, doubl
     int i, j, k;
     sub_ijk(t, g.nx, g.ny, 1, 1, 1, i, j, k);
     long long c = IX(i, j, k);
-    if (i >= 2) V[c] = VD[c] / RHQX[c];
+    if (i >= 2) st_nt(&V[c], VD[c] / RHQX[c]);
     double gx = m(TT_M_GEOXG, i), gz = m(TT_M_GEOZG, k), gh = m(TT_M_GEOXH, i);
-    GD[c] = GDN[c];
-    G[c] = GDN[c] / (RHQY[c] * (gx * gx) * (gz * gz)) - om;
-    if (k >= 2) H[c] = HDN[c] / (RHQZ[c] * (gh * gh));
-    T[c] = ED[c] / (RH[c] * CV[c]);
+    st_nt(&GD[c], GDN[c]);
+    st_nt(&G[c], GDN[c] / (RHQY[c] * (gx * gx) * (gz * gz)) - om);
+    if (k >= 2) st_nt(&H[c], HDN[c] / (RHQZ[c] * (gh * gh)));
+    st_nt(&T[c], ED[c] / (RH[c] * CV[c]));
 }
 
 int advect_impl(double dt) {
@@ -612,13 +613,13 @@ __global__ void k_centcor(Geo g, Met m, double om, const double* __restrict__ G,
     double xa = m(TT_M_XXA, i);
     double tq = GQk / xa * m(TT_M_GEOZG, k);
     long long c = IX(i, j, k);
-    CCX[c] = tq * tq + xa * (HQ * HQ);
-    CCY[c] = 0.0;
+    st_nt(&CCX[c], tq * tq + xa * (HQ * HQ));
+    st_nt(&CCY[c], 0.0);
     G1 = m(TT_M_DCY0, j + 1) * G[IX(i, j, k - 1)] + m(TT_M_DCY1, j + 1) * G[IX(i, j + 1, k - 1)];
     G2 = m(TT_M_DCY0, j + 1) * G[IX(i, j, k)] + m(TT_M_DCY1, j + 1) * G[IX(i, j + 1, k)];
     double GQ = m(TT_M_DDZ1, k) * G2 + m(TT_M_DDZ0, k) * G1 + om;
     double za = m(TT_M_XZA, k);
-    CCZ[c] = -m(TT_M_XXB, i) * GQ * GQ * sin(za) * cos(za);
+    st_nt(&CCZ[c], -m(TT_M_XXB, i) * GQ * GQ * sin(za) * cos(za));
 }
 
 // p dV work of cell (i, j, k): T - fac*dt PG/RH/CV div(v); with DISFN
@@ -646,7 +647,7 @@ __global__ void k_pdv(Geo g, Met m, double dtf, int lo, int hiadd, double* T, co
     int i, j, k;
     sub_ijk(t, ni, nj, lo, lo, lo, i, j, k);
     long long c = IX(i, j, k);
-    T[c] = pdv_cell(g, m, dtf, T[c], c, i, j, k, PG, RH, CV, V, G, H, DISFN);
+    st_nt(&T[c], pdv_cell(g, m, dtf, T[c], c, i, j, k, PG, RH, CV, V, G, H, DISFN));
 }
 
 // The same over cells lo..hi, written out of place: TN over the whole box
@@ -661,7 +662,7 @@ __global__ void k_pdv_box(Geo g, Met m, double dtf, int lo, int hiadd, const dou
     int i, j, k;
     box_ijk(g, t, i, j, k);
     const bool in = i >= lo && i <= g.nx + hiadd && j >= lo && j <= g.ny + hiadd && k >= lo && k <= g.nz + hiadd;
-    TN[t] = in ? pdv_cell(g, m, dtf, T[t], t, i, j, k, PG, RH, CV, V, G, H, DISFN) : T[t];
+    st_nt(&TN[t], in ? pdv_cell(g, m, dtf, T[t], t, i, j, k, PG, RH, CV, V, G, H, DISFN) : T[t]);
 }
 
 // VH, GH, HH (ppfor.f:217-278); DX/DY/DZ: divtx/y/z, null without viscosity
@@ -690,16 +691,16 @@ __global__ void k_force_vh(Geo g, Met m, double dt, int avg, double* V, double* 
         gv + dt * (((-(PG[c] - PG[IX(i, j - 1, k)]) / m(TT_M_DYB, j) / gx / gz + dy) / QY[c] + CCY[c] + GY[c]) / gx / gz);
     const double hvh = hv + dt * (((-(PG[c] - PG[IX(i, j, k - 1)]) / m(TT_M_DZB, k) / gh + dz) / QZ[c] + CCZ[c] + GZ[c]) / gh);
     if (avg) {
-        VH[c] = vh;
-        GH[c] = gvh;
-        HH[c] = hvh;
-        V[c] = (v + vh) / 2.0;
-        G[c] = (gv + gvh) / 2.0;
-        H[c] = (hv + hvh) / 2.0;
+        st_nt(&VH[c], vh);
+        st_nt(&GH[c], gvh);
+        st_nt(&HH[c], hvh);
+        st_nt(&V[c], (v + vh) / 2.0);
+        st_nt(&G[c], (gv + gvh) / 2.0);
+        st_nt(&H[c], (hv + hvh) / 2.0);
     } else {
-        V[c] = vh;
-        G[c] = gvh;
-        H[c] = hvh;
+        st_nt(&V[c], vh);
+        st_nt(&G[c], gvh);
+        st_nt(&H[c], hvh);
     }
 }
 

@@ -305,9 +305,9 @@ __global__ void k_qdens(Geo g, Met m, const double* __restrict__ RH, double* QX,
     int i, j, k;
     box_ijk(g, t, i, j, k);
     double r = RH[t];
-    if (i >= 0) QX[t] = m(TT_M_DDX1, i) * r + m(TT_M_DDX0, i) * RH[g.idx(i - 1, j, k)];
-    if (j >= 0) QY[t] = m(TT_M_DDY1, j) * r + m(TT_M_DDY0, j) * RH[g.idx(i, j - 1, k)];
-    if (k >= 0) QZ[t] = m(TT_M_DDZ1, k) * r + m(TT_M_DDZ0, k) * RH[g.idx(i, j, k - 1)];
+    if (i >= 0) st_nt(&QX[t], m(TT_M_DDX1, i) * r + m(TT_M_DDX0, i) * RH[g.idx(i - 1, j, k)]);
+    if (j >= 0) st_nt(&QY[t], m(TT_M_DDY1, j) * r + m(TT_M_DDY0, j) * RH[g.idx(i, j - 1, k)]);
+    if (k >= 0) st_nt(&QZ[t], m(TT_M_DDZ1, k) * r + m(TT_M_DDZ0, k) * RH[g.idx(i, j, k - 1)]);
 }
 
 int pressg_impl() {

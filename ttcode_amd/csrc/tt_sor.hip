@@ -1593,7 +1593,7 @@ static int line_chain(int stag, int max_outer) {
     chunk = ((chunk + 255) / 256) * 256;
     nb = (int)((tot + chunk - 1) / chunk);
     double* part = L.partial + L.npartial;
-    TT_HIP(hipMemsetAsync(g_lx, 0, line_x_len() * sizeof(float), L.s));
+    if (int e = dev_zero(g_lx, line_x_len() * sizeof(float))) return e;
     g_line = 1;
     g_pn = g_lnb;
     int rc = 0, o = 0;
@@ -1678,8 +1678,8 @@ int sor_solve_impl(int fill_from_evals, tt_sor_stats* st, int mode) {
                     k_scale<<<nblk(pl, 256), 256, 0, L.s>>>(x64[c], alpha, pl);
                     TT_LAUNCH();
                 }
-            } else {
-                TT_HIP(hipMemsetAsync(x64[c], 0, pl * sizeof(double), L.s));
+            } else if (int e = dev_zero(x64[c], pl * sizeof(double))) {
+                return e;
             }
         }
         return 0;
@@ -1735,8 +1735,7 @@ int sor_solve_impl(int fill_from_evals, tt_sor_stats* st, int mode) {
             }
         }
         if (!s.guess_used) {
-            TT_HIP(hipMemsetAsync(x64[0], 0, pl * sizeof(double), L.s));
-            TT_HIP(hipMemsetAsync(x64[1], 0, pl * sizeof(double), L.s));
+            if ((rc = dev_zero(x64[0], pl * sizeof(double))) || (rc = dev_zero(x64[1], pl * sizeof(double)))) return rc;
         }
     }
     int iters = 0;
@@ -1815,8 +1814,7 @@ int sor_solve_impl(int fill_from_evals, tt_sor_stats* st, int mode) {
         } else {
             for (;;) {
                 // inner: A c = r in FP32 from c = 0, L1 during-sweep ratio vs sum|r|
-                TT_HIP(hipMemsetAsync(x32[0], 0, pl * sizeof(float), L.s));
-                TT_HIP(hipMemsetAsync(x32[1], 0, pl * sizeof(float), L.s));
+                if ((rc = dev_zero(x32[0], pl * sizeof(float))) || (rc = dev_zero(x32[1], pl * sizeof(float)))) return rc;
                 L.hctrl->itmax = L.opts.inner_max;  // the host's guard in run_sweeps
                 int it32 = 0;
                 double rn32 = 0;

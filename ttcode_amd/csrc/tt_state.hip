@@ -157,6 +157,62 @@ __global__ void k_fill(double* a, double v, long long n) {
     if (t < n) a[t] = v;
 }
 
+// Device copies and zero fills inside the step, as plain kernels: between
+// two kernels a hipMemcpyAsync/hipMemsetAsync left the queue idle 5-20 us
+// (the runtime's own blit kernel and its host path), a launch 1-2 us.
+// 16-byte words where both ends are aligned to them, else 4-byte words.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+__global__ void k_copy16(u32x4* __restrict__ d, const u32x4* __restrict__ s, long long n) {
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x)
+        st_nt(&d[t], s[t]);
+}
+__global__ void k_copy4(uint32_t* __restrict__ d, const uint32_t* __restrict__ s, long long n) {
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x)
+        st_nt(&d[t], s[t]);
+}
+__global__ void k_zero16(u32x4* d, long long n) {
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x)
+        st_nt(&d[t], u32x4{0u, 0u, 0u, 0u});
+}
+__global__ void k_zero4(uint32_t* d, long long n) {
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x)
+        st_nt(&d[t], 0u);
+}
+
+static unsigned copy_blocks(long long n) {
+    const long long cap = 8192;  // 32 per CU; each thread then takes a few words
+    long long b = (n + 255) / 256;
+    return (unsigned)(b < cap ? (b > 0 ? b : 1) : cap);
+}
+
+int dev_copy(void* dst, const void* src, size_t bytes) {
+    if (bytes == 0) return 0;
+    if (bytes % 4) return fail(1, "dev_copy: size not a multiple of 4 bytes");
+    if (((uintptr_t)dst | (uintptr_t)src | bytes) % 16 == 0) {
+        const long long n = (long long)(bytes / 16);
+        k_copy16<<<copy_blocks(n), 256, 0, L.s>>>((u32x4*)dst, (const u32x4*)src, n);
+    } else {
+        const long long n = (long long)(bytes / 4);
+        k_copy4<<<copy_blocks(n), 256, 0, L.s>>>((uint32_t*)dst, (const uint32_t*)src, n);
+    }
+    TT_LAUNCH();
+    return 0;
+}
+
+int dev_zero(void* dst, size_t bytes) {
+    if (bytes == 0) return 0;
+    if (bytes % 4) return fail(1, "dev_zero: size not a multiple of 4 bytes");
+    if (((uintptr_t)dst | bytes) % 16 == 0) {
+        const long long n = (long long)(bytes / 16);
+        k_zero16<<<copy_blocks(n), 256, 0, L.s>>>((u32x4*)dst, n);
+    } else {
+        const long long n = (long long)(bytes / 4);
+        k_zero4<<<copy_blocks(n), 256, 0, L.s>>>((uint32_t*)dst, n);
+    }
+    TT_LAUNCH();
+    return 0;
+}
+
 // a radial profile col[i + 1], i = -1..nx+2, copied to every (j, k) of the box
 __global__ void k_fill_radial(Geo g, const double* __restrict__ col, double* a) {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
