@@ -288,6 +288,13 @@ void sor_sys_module_written();
 int sor_sys_before_download(int field);
 int diffusion_impl(double delt, double xnue, double pr, int* iters);
 int energy_impl(double delt, int iii, tt_energy_stats* st);
+// EVAL1..6 / EVAL_1_6 are written to their module arrays on demand
+// (tt_energy.hip): evals_materialize() writes them if they are pending;
+// evals_before_access(field) does so before a host transfer of a field they
+// derive from or are; evals_reset() drops the pending state (tt_init)
+int evals_materialize();
+int evals_before_access(int field);
+void evals_reset();
 int burrows_cell111_impl();        // BurrowsOPC at cell (1,1,1) (BOUNDXMIN(9), III <= 0)
 int advect_impl(double delt);
 int forgas_impl(double delt);

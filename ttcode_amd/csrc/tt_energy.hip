@@ -771,6 +771,44 @@ __global__ void k_flux_x_top(Geo g, const double* __restrict__ ER, const int32_t
     FX[IX(1, j, k)] = fbottom;
 }
 
+// FILL_EVALUES' six face terms and their sum (ppenergy.f:984-1030), from the
+// diffusion coefficients DIFR and the metrics alone
+struct Evals {
+    double e1, e2, e3, e4, e5, e6, e16;
+};
+__device__ inline Evals fill_evalues(const Geo& g, const Met& m, const double* __restrict__ DX,
+                                     const double* __restrict__ DY, const double* __restrict__ DZ, int i, int j,
+                                     int k) {
+    const long long c = IX(i, j, k);
+    const double sxb = m(TT_M_SURXB, i), dxa = m(TT_M_DXA, i), xxb = m(TT_M_XXB, i), szb = m(TT_M_SURZB, k);
+    const double dya = m(TT_M_DYA, j), dza = m(TT_M_DZA, k), sxa0 = m(TT_M_SURXA, i), sxa1 = m(TT_M_SURXA, i + 1);
+    const double sza0 = m(TT_M_SURZA, k), sza1 = m(TT_M_SURZA, k + 1);
+    Evals e;
+    e.e1 = -sxa1 * DX[IX(i + 1, j, k)] / (sxb * dxa);
+    e.e2 = -sxa0 * DX[c] / (sxb * dxa);
+    e.e3 = -DY[IX(i, j + 1, k)] / (xxb * szb * dya);
+    e.e4 = -DY[c] / (xxb * szb * dya);
+    e.e5 = -DZ[IX(i, j, k + 1)] * sza1 / (xxb * szb * dza);
+    e.e6 = -DZ[c] * sza0 / (xxb * szb * dza);
+    e.e16 = e.e1 + e.e2 + e.e3 + e.e4 + e.e5 + e.e6;
+    return e;
+}
+
+// EVAL1..6 and EVAL_1_6 into their module arrays over the interior, from the
+// same expressions k_newton_cell evaluates
+__global__ void k_evals(Geo g, Met m, const double* __restrict__ DX, const double* __restrict__ DY,
+                        const double* __restrict__ DZ, double* E1, double* E2, double* E3, double* E4, double* E5,
+                        double* E6, double* E16) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)g.nx * g.ny * g.nz) return;
+    int i, j, k;
+    sub_ijk(t, g.nx, g.ny, 1, 1, 1, i, j, k);
+    const long long c = IX(i, j, k);
+    const Evals e = fill_evalues(g, m, DX, DY, DZ, i, j, k);
+    st_nt(&E1[c], e.e1); st_nt(&E2[c], e.e2); st_nt(&E3[c], e.e3); st_nt(&E4[c], e.e4);
+    st_nt(&E5[c], e.e5); st_nt(&E6[c], e.e6); st_nt(&E16[c], e.e16);
+}
+
 struct CellArgs {
     const double *FX, *FY, *FZ, *DX, *DY, *DZ, *Q, *QOLD, *ER, *EROLD, *RH, *CV, *XKP, *XKA, *DKP, *DKA, *SIN;
     double *E1, *E2, *E3, *E4, *E5, *E6, *E7, *E8, *E16, *ETA, *FQ, *FR, *RHS, *DFQ;
@@ -798,13 +836,8 @@ __global__ void k_newton_cell(Geo g, Met m, CellArgs a) {
     double divf = ((sxa1 / (sxb * dxa)) * a.FX[IX(i + 1, j, k)] - (sxa0 / (sxb * dxa)) * a.FX[c]) +
                   ((1.0 / (xxb * szb * dya)) * a.FY[IX(i, j + 1, k)] - (1.0 / (xxb * szb * dya)) * a.FY[c]) +
                   ((sza1 / (xxb * szb * dza)) * a.FZ[IX(i, j, k + 1)] - (sza0 / (xxb * szb * dza)) * a.FZ[c]);
-    double e1 = -sxa1 * a.DX[IX(i + 1, j, k)] / (sxb * dxa);
-    double e2 = -sxa0 * a.DX[c] / (sxb * dxa);
-    double e3 = -a.DY[IX(i, j + 1, k)] / (xxb * szb * dya);
-    double e4 = -a.DY[c] / (xxb * szb * dya);
-    double e5 = -a.DZ[IX(i, j, k + 1)] * sza1 / (xxb * szb * dza);
-    double e6 = -a.DZ[c] * sza0 / (xxb * szb * dza);
-    double e16 = e1 + e2 + e3 + e4 + e5 + e6;
+    const Evals ev = fill_evalues(g, m, a.DX, a.DY, a.DZ, i, j, k);
+    const double e1 = ev.e1, e2 = ev.e2, e3 = ev.e3, e4 = ev.e4, e5 = ev.e5, e6 = ev.e6, e16 = ev.e16;
     double rh = a.RH[c], cv = a.CV[c], q = a.Q[c], er = a.ER[c], kp = a.XKP[c];
     double B = ARAD * pow(q / (rh * cv), 4.0);
     double fq, heat = 0.0;
@@ -827,8 +860,8 @@ __global__ void k_newton_cell(Geo g, Met m, CellArgs a) {
     double rhs = -fr - eta * fq;
     double e7 = 1.0 + (1.0 - eta) * delt * rh * kp * FD;
     double e8 = (eta * (1.0 - theta) - 1.0) * delt;
-    st_nt(&a.E1[c], e1); st_nt(&a.E2[c], e2); st_nt(&a.E3[c], e3); st_nt(&a.E4[c], e4); st_nt(&a.E5[c], e5);
-    st_nt(&a.E6[c], e6); st_nt(&a.E7[c], e7); st_nt(&a.E8[c], e8); st_nt(&a.E16[c], e16); st_nt(&a.ETA[c], eta);
+    // EVAL1..6 and EVAL_1_6 are left to k_evals (they depend on DIFR only)
+    st_nt(&a.E7[c], e7); st_nt(&a.E8[c], e8); st_nt(&a.ETA[c], eta);
     st_nt(&a.FQ[c], fq); st_nt(&a.FR[c], fr); st_nt(&a.DFQ[c], dfq);
     (void)a.RHS;
     // FILL_SOR_MATRIX + B_VECTOR_BOUNDARIES
@@ -1186,6 +1219,34 @@ static int stellar_impl() {
     return 0;
 }
 
+// EVAL1..6 and EVAL_1_6 after a Newton iteration: k_newton_cell forms them
+// in registers and does not store them (seven arrays fewer to write, about a
+// quarter of its time).  They depend on DIFR and the metrics only, so their
+// module arrays are written from those (k_evals) when something reads them
+// (a download, a solve that fills its matrix from them) or before DIFR, an
+// EVAL array or a metric is overwritten from the host.
+static bool g_evals_lazy = false;
+
+int evals_materialize() {
+    if (!g_evals_lazy) return 0;
+    double** f = L.f;
+    const long long n = (long long)L.g.nx * L.g.ny * L.g.nz;
+    k_evals<<<nblk(n, 256), 256, 0, L.s>>>(L.g, met(), f[TT_DIFRX], f[TT_DIFRY], f[TT_DIFRZ], f[TT_EVAL1], f[TT_EVAL2],
+                                            f[TT_EVAL3], f[TT_EVAL4], f[TT_EVAL5], f[TT_EVAL6], f[TT_EVAL_1_6]);
+    TT_LAUNCH();
+    g_evals_lazy = false;
+    return 0;
+}
+
+int evals_before_access(int field) {
+    if (field == TT_DIFRX || field == TT_DIFRY || field == TT_DIFRZ || (field >= TT_EVAL1 && field <= TT_EVAL6) ||
+        field == TT_EVAL_1_6)
+        return evals_materialize();
+    return 0;
+}
+
+void evals_reset() { g_evals_lazy = false; }
+
 static int flux_impl() {
     if (L.p.iflimtyp != 1 && L.p.iflimtyp != 2) return fail(1, "FLIM: IFLIMTYP must be 1 or 2");
     Geo& g = L.g;
@@ -1263,6 +1324,7 @@ int energy_impl(double delt, int iii, tt_energy_stats* st) {
         long long nc = (long long)g.nx * g.ny * g.nz;
         k_newton_cell<<<nblk(nc, 256), 256, 0, L.s>>>(g, met(), ca);
         TT_LAUNCH();
+        g_evals_lazy = true;
         sor_sys_packed_written(with32);
         TT_HIP(hipEventRecord(ev[1], L.s));
         // ADJUST_SOR_PARAM at the first Newton iteration of step 1 and of

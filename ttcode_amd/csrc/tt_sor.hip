@@ -1649,6 +1649,7 @@ int sor_solve_impl(int fill_from_evals, tt_sor_stats* st, int mode) {
     double** f = L.f;
     tt_sor_stats s{};
     if (fill_from_evals) {
+        if (int e = evals_materialize()) return e;
         long long n = (long long)g.nx * g.ny * g.nz;
         k_fill_matrix<<<nblk(n, 256), 256, 0, L.s>>>(g, L.ub, f[TT_EVAL1], f[TT_EVAL2], f[TT_EVAL3], f[TT_EVAL4],
                                                      f[TT_EVAL5], f[TT_EVAL6], f[TT_EVAL7], f[TT_EVAL8],

@@ -239,6 +239,7 @@ static int xfer(int f, void* host, bool to_host) {
     }
     double* a = L.f[f];
     if (!a) return fail(1, "field not held on the device");
+    if (int rc = evals_before_access(f)) return rc;
     if (to_host) {
         int rc = sor_sys_before_download(f);
         if (rc) return rc;
@@ -453,6 +454,7 @@ int tt_init(const tt_params* p) {
     L.opts.inner = 0;
     L.opts.line_omega = 1.1;
     TT_HIP(hipStreamSynchronize(L.s));
+    evals_reset();
     L.ready = true;
     return 0;
 }
@@ -464,6 +466,7 @@ int tt_upload_metric(int metric, const double* host) {
     if (!L.ready) return fail(1, "tt_init has not been called");
     int32_t lo, hi;
     if (tt_metric_bounds(metric, L.g.nx, L.g.ny, L.g.nz, &lo, &hi)) return fail(1, "unknown metric");
+    if (int rc = evals_materialize()) return rc;  // the pending EVALs are those of the old metrics
     for (int i = lo; i <= hi; ++i) L.hmet[(size_t)metric * L.metlen + i + 2] = host[i - lo];
     TT_HIP(hipMemcpyAsync(L.met + (size_t)metric * L.metlen, L.hmet + (size_t)metric * L.metlen,
                           L.metlen * sizeof(double), hipMemcpyHostToDevice, L.s));
