@@ -165,6 +165,10 @@ struct Lib {
     int energy_iter = 0;
     int iii = 0;
     int have_prev_guess = 0;
+    // upperbnd holds the search result of the last BOUNDS and RH has changed
+    // since only through that BOUNDS (cleared by every other writer of RH or
+    // upperbnd): the next BOUNDS may take its search from those results
+    bool ub_repeat = false;
     int64_t dev_bytes = 0;          // device memory held by the library
     hipEvent_t step_ev[4] = {};     // tt_step's phase boundaries (created by tt_init)
     hipEvent_t en_ev[4] = {};       // a Newton iteration's build / SOR / update boundaries

@@ -586,6 +586,7 @@ int advect_impl(double dt) {
     if ((rc = launch_adv<AdvH>(aa))) return rc;
     k_adv_rho<<<nblk(nc, 256), 256, 0, L.s>>>(g, met(), f[TT_RH], L.FLMX, L.FLMY, L.FLMZ);
     TT_LAUNCH();
+    L.ub_repeat = false;
     if ((rc = bounds_impl())) return rc;
     if ((rc = qdens_impl())) return rc;
     k_adv_final<<<nblk(nc, 256), 256, 0, L.s>>>(g, met(), L.p.omrot, f[TT_V], f[TT_G], f[TT_H], f[TT_T], f[TT_GD], L.VD,

@@ -98,12 +98,38 @@ __device__ inline void upperbnd_body(const Geo& g, const double* __restrict__ RH
     if (lane == 0) ub[g.idx2(j, k)] = u;
 }
 
+// upperbnd again, right after a BOUNDS that searched it, with RH changed
+// since only by that BOUNDS.  For an interior column (j, k), 2 <= k <= nz-1,
+// that BOUNDS left RH(1..u-1) as they were, all at or above the threshold
+// (u is the first cell below it, or nx-1 where none is), kept RH(u) unless
+// u = nx, and set RH to 1e-10 above u and at nx when u = nx (BOUNDXMAX).  So
+// the search now finds u if RH(u) is below the threshold and nx otherwise
+// (u = nx-1 with no cell below: RH(nx) is now 1e-10); reading RH(u) is
+// enough.  Columns k = 1 and nz hold copies of k = 2 and nz-1 (BOUNDZMIN/
+// ZMAX copy all of them), and the ghost columns search the interior columns
+// they wrap to, so each column takes the result of interior column (jw, ks).
+__device__ inline void upperbnd_repeat_body(const Geo& g, const double* __restrict__ RH, int32_t* ub, int t) {
+    int nj = g.ny + 4, nk = g.nz + 4;
+    if (t >= nj * nk) return;
+    int j = t % nj - 1, k = t / nj - 1;
+    int jw = j <= 0 ? j + g.ny : (j > g.ny ? j - g.ny : j);
+    int ks = k < 2 ? 2 : (k > g.nz - 1 ? g.nz - 1 : k);
+    const double thr = (double)1.0e-9f;
+    const int up = ub[g.idx2(jw, ks)];
+    // the source column's own thread may have stored its result already:
+    // this map is idempotent (u -> u where RH(u) is below, nx -> nx), so
+    // either value gives the same
+    const int u = RH[g.idx(up, jw, ks)] < thr ? up : g.nx;
+    ub[g.idx2(j, k)] = u;
+}
+
 // BOUNDS' first PLANETSHIFT and upperbnd in one launch: upperbnd reads only
 // the interior columns it wraps to, which the shift does not write, and the
 // shift does not read upperbnd.  Blocks [0, nshift) shift, the rest search.
 __global__ __launch_bounds__(256) void k_shift5_upperbnd(Geo g, double* T, double* RH, double* V, double* G,
-                                                         double* H, int32_t* ub, int nshift) {
+                                                         double* H, int32_t* ub, int nshift, int repeat) {
     if ((int)blockIdx.x < nshift) shift5_body(g, T, RH, V, G, H, blockIdx.x);
+    else if (repeat) upperbnd_repeat_body(g, RH, ub, (blockIdx.x - nshift) * 256 + threadIdx.x);
     else upperbnd_body(g, RH, ub, blockIdx.x - nshift);
 }
 
@@ -268,8 +294,9 @@ int bounds_impl() {
     long long ns = (long long)(g.nx + 4) * g.nz;
     int ncol = (g.ny + 4) * (g.nz + 4), nline = (g.nx + 4) * (g.ny + 4);
     const unsigned nshift = nblk(ns, 256);
-    k_shift5_upperbnd<<<nshift + nblk(ncol, 4), 256, 0, L.s>>>(g, f[TT_T], f[TT_RH], f[TT_V], f[TT_G], f[TT_H],
-                                                               L.ub, (int)nshift);
+    const int rep = L.ub_repeat ? 1 : 0;
+    k_shift5_upperbnd<<<nshift + (rep ? nblk(ncol, 256) : nblk(ncol, 4)), 256, 0, L.s>>>(
+        g, f[TT_T], f[TT_RH], f[TT_V], f[TT_G], f[TT_H], L.ub, (int)nshift, rep);
     TT_LAUNCH();
     if (L.p.bcxmin == 9 && L.iii <= 0) {
         int rc = burrows_cell111_impl();
@@ -283,6 +310,7 @@ int bounds_impl() {
     TT_LAUNCH();
     k_shift5<<<nblk(ns, 256), 256, 0, L.s>>>(g, f[TT_T], f[TT_RH], f[TT_V], f[TT_G], f[TT_H]);
     TT_LAUNCH();
+    L.ub_repeat = true;
     return 0;
 }
 
@@ -520,6 +548,7 @@ static int varlimit_impl(int f, double vmin, double vmax) {
     long long n = (long long)n0 * n1 * (hi[2] - lo[2] + 1);
     k_clamp<<<nblk(n, 256), 256, 0, L.s>>>(g, L.f[f], lo[0], lo[1], lo[2], n0, n1, n, vmin, vmax);
     TT_LAUNCH();
+    if (f == TT_RH) L.ub_repeat = false;
     return 0;
 }
 
