@@ -116,6 +116,7 @@ struct Lib {
     int32_t* ub = nullptr;          // upperbnd (-1:ny+2,-1:nz+2)
     double* mustar = nullptr;
     double* sincoef = nullptr;      // per (j,k) column: SBCONST Tinc^4 / mu* of the grey irradiation
+    double* scza = nullptr;         // sin and cos of XZA(k), k = -2 .. (metric slots), from the device's functions
     double* met = nullptr;          // device metric buffer
     double* hmet = nullptr;         // host copy
     int metlen = 0;
@@ -169,6 +170,10 @@ struct Lib {
     // since only through that BOUNDS (cleared by every other writer of RH or
     // upperbnd): the next BOUNDS may take its search from those results
     bool ub_repeat = false;
+    // GRAVY / GRAVZ hold +0.0 in every cell (as tt_init leaves them; cleared
+    // by an upload of anything else): the force kernel then adds the
+    // constant instead of reading the array, the same arithmetic
+    bool gy_zero = true, gz_zero = true;
     int64_t dev_bytes = 0;          // device memory held by the library
     hipEvent_t step_ev[4] = {};     // tt_step's phase boundaries (created by tt_init)
     hipEvent_t en_ev[4] = {};       // a Newton iteration's build / SOR / update boundaries
@@ -297,6 +302,7 @@ int energy_impl(double delt, int iii, tt_energy_stats* st);
 // evals_before_access(field) does so before a host transfer of a field they
 // derive from or are; evals_reset() drops the pending state (tt_init)
 int evals_materialize();
+int scza_impl();                  // L.scza from the metric table (tt_state.hip)
 int evals_before_access(int field);
 void evals_reset();
 int burrows_cell111_impl();        // BurrowsOPC at cell (1,1,1) (BOUNDXMIN(9), III <= 0)

@@ -51,15 +51,44 @@ struct Box3 {
     __device__ double operator()(int i, int j, int k) const { return a[g.idx(i, j, k)]; }
 };
 
-// ---- mass fluxes (ppadv.f:559-644) and their phi plane exchange (:654-711)
-__global__ void k_adv_mass(Geo g, Met m, double dt, const double* __restrict__ RH, const double* __restrict__ V,
-                           const double* __restrict__ G, const double* __restrict__ H, double* FLMX, double* FLMY,
-                           double* FLMZ) {
-    long long n = (long long)(g.nx + 1) * (g.ny + 2) * (g.nz + 1);
+// ---- mass fluxes (ppadv.f:559-644) and their phi plane exchange (:654-711),
+// one thread per cell of the box.  The same pass forms the transported
+// quantities the later sweeps read over the whole box: the thermal energy
+// E0 = T*RH*CV (ppadv.f:741-891), GD = (G + OMROT) r^2 cos^2 (:1078-1084;
+// the phi ghosts equal the formula at the wrapped column because G's ghosts
+// were wrapped by BOUNDS) and HD = H r^2 (:1278-1283), as the reference's
+// expressions.
+struct MassArgs {
+    const double *RH, *V, *G, *H, *T, *CV;
+    double *FLMX, *FLMY, *FLMZ, *E0, *GD, *HD;
+    double dt, om;
+};
+__global__ void k_adv_mass(Geo g, Met m, MassArgs a) {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
+    if (t >= g.n3) return;
     int i, j, k;
-    sub_ijk(t, g.nx + 1, g.ny + 2, 1, 0, 1, i, j, k);
+    box_ijk(g, t, i, j, k);
+    {
+        const double r = a.RH[t];
+        st_nt(&a.E0[t], a.T[t] * r * a.CV[t]);
+        if (j >= 0) {
+            const double gx = m(TT_M_GEOXG, i), gz = m(TT_M_GEOZG, k);
+            st_nt(&a.GD[t], (a.G[t] + a.om) * (gx * gx) * (gz * gz));
+        }
+        if (k >= 0) {
+            const double gh = m(TT_M_GEOXH, i);
+            st_nt(&a.HD[t], a.H[t] * (gh * gh));
+        }
+    }
+    if (i < 1 || i > g.nx + 1 || j < 0 || j > g.ny + 1 || k < 1 || k > g.nz + 1) return;
+    const double dt = a.dt;
+    const double* __restrict__ RH = a.RH;
+    const double* __restrict__ V = a.V;
+    const double* __restrict__ G = a.G;
+    const double* __restrict__ H = a.H;
+    double* FLMX = a.FLMX;
+    double* FLMY = a.FLMY;
+    double* FLMZ = a.FLMZ;
     const double half = 0.5;
     Box3 R{RH, g};
     // x faces: i = 1..nx+1, j = 0..ny (j = 0 is the periodic copy of j = ny)
@@ -93,37 +122,6 @@ __global__ void k_adv_mass(Geo g, Met m, double dt, const double* __restrict__ R
         st_nt(&FLMZ[IX(i, j, k)], (m(TT_M_SURZA, k) + m(TT_M_CCZA, k) * S) * S *
                                       (R(i, jj, ind) + (m(TT_M_XZA, k) - m(TT_M_XZB, ind) - half * S) * gr));
     }
-}
-
-// ---- thermal energy ED = T*RH*CV (ppadv.f:741-891).  The product is formed
-// once per cell of the box (k_ed0, the reference's expression), so the
-// sweep reads one value per stencil point instead of three.
-__global__ void k_ed0(const double* __restrict__ T, const double* __restrict__ RH, const double* __restrict__ CV,
-                      double* E0, long long n) {
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n) st_nt(&E0[t], T[t] * RH[t] * CV[t]);
-}
-
-// GD = (G + OMROT) r^2 cos^2 (ppadv.f:1078-1084); the phi ghosts equal the
-// formula at the wrapped column because G's ghosts were wrapped by BOUNDS.
-__global__ void k_gd_init(Geo g, Met m, double om, const double* __restrict__ G, double* GD) {
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= g.n3) return;
-    int i, j, k;
-    box_ijk(g, t, i, j, k);
-    if (j < 0) return;
-    double gx = m(TT_M_GEOXG, i), gz = m(TT_M_GEOZG, k);
-    st_nt(&GD[t], (G[t] + om) * (gx * gx) * (gz * gz));
-}
-
-__global__ void k_hd_init(Geo g, Met m, const double* __restrict__ H, double* HD) {
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= g.n3) return;
-    int i, j, k;
-    box_ijk(g, t, i, j, k);
-    if (k < 0) return;
-    double gh = m(TT_M_GEOXH, i);
-    st_nt(&HD[t], H[t] * (gh * gh));
 }
 
 // ---- face-shared sweeps.  The four transport updates evaluate, per cell,
@@ -564,24 +562,20 @@ int advect_impl(double dt) {
     double** f = L.f;
     int rc;
     if ((rc = bounds_impl())) return rc;
-    long long nm = (long long)(g.nx + 1) * (g.ny + 2) * (g.nz + 1);
-    k_adv_mass<<<nblk(nm, 256), 256, 0, L.s>>>(g, met(), dt, f[TT_RH], f[TT_V], f[TT_G], f[TT_H], L.FLMX, L.FLMY, L.FLMZ);
+    // mass fluxes, and E0 (into TOLD), GD and HD (into HH) for the sweeps
+    MassArgs ma{f[TT_RH], f[TT_V], f[TT_G], f[TT_H], f[TT_T], f[TT_CV], L.FLMX, L.FLMY, L.FLMZ, L.TOLD, f[TT_GD],
+                L.HH, dt, L.p.omrot};
+    k_adv_mass<<<nblk(g.n3, 256), 256, 0, L.s>>>(g, met(), ma);
     TT_LAUNCH();
     if ((rc = qdens_impl())) return rc;
     long long nc = (long long)g.nx * g.ny * g.nz;
-    k_ed0<<<nblk(g.n3, 256), 256, 0, L.s>>>(f[TT_T], f[TT_RH], f[TT_CV], L.TOLD, g.n3);
-    TT_LAUNCH();
     AdvArgs aa{L.TOLD, f[TT_V], f[TT_G], f[TT_H], L.FLMX, L.FLMY, L.FLMZ, nullptr, L.ED, dt,
                f[TT_CONV_FLUX1], f[TT_CONV_FLUX2], f[TT_CONV_FLUX3]};
     if ((rc = launch_adv<AdvE>(aa))) return rc;
     aa.A = f[TT_V]; aa.RHQ = f[TT_RHQX]; aa.out = L.VD;
     if ((rc = launch_adv<AdvV>(aa))) return rc;
-    k_gd_init<<<nblk(g.n3, 256), 256, 0, L.s>>>(g, met(), L.p.omrot, f[TT_G], f[TT_GD]);
-    TT_LAUNCH();
     aa.A = f[TT_GD]; aa.RHQ = f[TT_RHQY]; aa.out = L.GDN;
     if ((rc = launch_adv<AdvG>(aa))) return rc;
-    k_hd_init<<<nblk(g.n3, 256), 256, 0, L.s>>>(g, met(), f[TT_H], L.HH);
-    TT_LAUNCH();
     aa.A = L.HH; aa.RHQ = f[TT_RHQZ]; aa.out = L.HDN;
     if ((rc = launch_adv<AdvH>(aa))) return rc;
     k_adv_rho<<<nblk(nc, 256), 256, 0, L.s>>>(g, met(), f[TT_RH], L.FLMX, L.FLMY, L.FLMZ);
@@ -598,8 +592,10 @@ int advect_impl(double dt) {
 
 // ------------------------------------------------------------- FORGAS3D
 // CENTRIFCORIOLIS, NCOSYS=2 (ppfor.f:435-486)
+// sin and cos of XZA(k) come from SCZ (k_scza: the same device functions,
+// once per theta face instead of once per cell)
 __global__ void k_centcor(Geo g, Met m, double om, const double* __restrict__ G, const double* __restrict__ H,
-                          double* CCX, double* CCY, double* CCZ) {
+                          const double* __restrict__ SCZ, double* CCX, double* CCY, double* CCZ) {
     long long n = (long long)g.nx * g.ny * g.nz;
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
@@ -619,8 +615,7 @@ __global__ void k_centcor(Geo g, Met m, double om, const double* __restrict__ G,
     G1 = m(TT_M_DCY0, j + 1) * G[IX(i, j, k - 1)] + m(TT_M_DCY1, j + 1) * G[IX(i, j + 1, k - 1)];
     G2 = m(TT_M_DCY0, j + 1) * G[IX(i, j, k)] + m(TT_M_DCY1, j + 1) * G[IX(i, j + 1, k)];
     double GQ = m(TT_M_DDZ1, k) * G2 + m(TT_M_DDZ0, k) * G1 + om;
-    double za = m(TT_M_XZA, k);
-    st_nt(&CCZ[c], -m(TT_M_XXB, i) * GQ * GQ * sin(za) * cos(za));
+    st_nt(&CCZ[c], -m(TT_M_XXB, i) * GQ * GQ * SCZ[2 * (k + 2)] * SCZ[2 * (k + 2) + 1]);
 }
 
 // p dV work of cell (i, j, k): T - fac*dt PG/RH/CV div(v); with DISFN
@@ -667,7 +662,10 @@ __global__ void k_pdv_box(Geo g, Met m, double dtf, int lo, int hiadd, const dou
 }
 
 // VH, GH, HH (ppfor.f:217-278); DX/DY/DZ: divtx/y/z, null without viscosity
-// (the reference's zero arrays).  Each cell reads only its own V, G, H, so
+// (the reference's zero arrays); CCY (CENTRIFCORIOLIS sets it to zero), GY
+// and GZ null where they hold +0.0 throughout, which is then added as a
+// constant: the same operations on the same values.  Each cell reads only
+// its own V, G, H, so
 // the kernel also applies the update that follows in place: avg != 0 sets
 // V = (V+VH)/2 etc. and stores VH, GH, HH for the second half (ppfor.f:
 // 322-328); avg == 0 sets V = VH etc. (the isothermal case, :346-352).
@@ -688,9 +686,10 @@ __global__ void k_force_vh(Geo g, Met m, double dt, int avg, double* V, double* 
     const double dx = DX ? DX[c] : 0.0, dy = DY ? DY[c] : 0.0, dz = DZ ? DZ[c] : 0.0;
     const double v = V[c], gv = G[c], hv = H[c];
     const double vh = v + dt * ((-(PG[c] - PG[IX(i - 1, j, k)]) / m(TT_M_DXB, i) + dx) / QX[c] + CCX[c] + GX[c]);
+    const double ccy = CCY ? CCY[c] : 0.0, gyc = GY ? GY[c] : 0.0, gzc = GZ ? GZ[c] : 0.0;
     const double gvh =
-        gv + dt * (((-(PG[c] - PG[IX(i, j - 1, k)]) / m(TT_M_DYB, j) / gx / gz + dy) / QY[c] + CCY[c] + GY[c]) / gx / gz);
-    const double hvh = hv + dt * (((-(PG[c] - PG[IX(i, j, k - 1)]) / m(TT_M_DZB, k) / gh + dz) / QZ[c] + CCZ[c] + GZ[c]) / gh);
+        gv + dt * (((-(PG[c] - PG[IX(i, j - 1, k)]) / m(TT_M_DYB, j) / gx / gz + dy) / QY[c] + ccy + gyc) / gx / gz);
+    const double hvh = hv + dt * (((-(PG[c] - PG[IX(i, j, k - 1)]) / m(TT_M_DZB, k) / gh + dz) / QZ[c] + CCZ[c] + gzc) / gh);
     if (avg) {
         st_nt(&VH[c], vh);
         st_nt(&GH[c], gvh);
@@ -715,8 +714,8 @@ int forgas_impl(double dt) {
     if (vis && (rc = visc_impl())) return rc;  // ETACALC + VISC, ppfor.f:170-174
     const double* disfn = vis && L.visc.idiss != 0 ? f[TT_DISFN] : nullptr;
     long long nc = (long long)g.nx * g.ny * g.nz;
-    k_centcor<<<nblk(nc, 256), 256, 0, L.s>>>(g, met(), L.p.omrot, f[TT_G], f[TT_H], f[TT_CENTCORX], f[TT_CENTCORY],
-                                              f[TT_CENTCORZ]);
+    k_centcor<<<nblk(nc, 256), 256, 0, L.s>>>(g, met(), L.p.omrot, f[TT_G], f[TT_H], L.scza, f[TT_CENTCORX],
+                                              f[TT_CENTCORY], f[TT_CENTCORZ]);
     TT_LAUNCH();
     const bool adiab = L.p.isoth == 0;
     if (adiab) {
@@ -735,8 +734,9 @@ int forgas_impl(double dt) {
     }
     // VH, GH, HH, and V = (V+VH)/2 etc. (adiabatic) or V = VH etc.
     k_force_vh<<<nblk(nc, 256), 256, 0, L.s>>>(g, met(), dt, adiab ? 1 : 0, f[TT_V], f[TT_G], f[TT_H], f[TT_PG],
-                                               f[TT_RHQX], f[TT_RHQY], f[TT_RHQZ], f[TT_CENTCORX], f[TT_CENTCORY],
-                                               f[TT_CENTCORZ], f[TT_GRAVX], f[TT_GRAVY], f[TT_GRAVZ],
+                                               f[TT_RHQX], f[TT_RHQY], f[TT_RHQZ], f[TT_CENTCORX], nullptr,
+                                               f[TT_CENTCORZ], f[TT_GRAVX], L.gy_zero ? nullptr : f[TT_GRAVY],
+                                               L.gz_zero ? nullptr : f[TT_GRAVZ],
                                                vis ? f[TT_DIVTX] : nullptr, vis ? f[TT_DIVTY] : nullptr,
                                                vis ? f[TT_DIVTZ] : nullptr, L.VH, L.GH, L.HH);
     TT_LAUNCH();

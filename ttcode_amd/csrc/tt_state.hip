@@ -179,6 +179,22 @@ __global__ void k_zero4(uint32_t* d, long long n) {
         st_nt(&d[t], 0u);
 }
 
+// sin and cos of every theta face coordinate XZA(k), by the device's
+// functions: CENTRIFCORIOLIS's cell loop evaluated them per cell
+__global__ void k_scza(Met m, double* sc) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m.len) return;
+    const double za = m(TT_M_XZA, t - 2);
+    sc[2 * t] = sin(za);
+    sc[2 * t + 1] = cos(za);
+}
+
+int scza_impl() {
+    k_scza<<<nblk(L.metlen, 256), 256, 0, L.s>>>(met(), L.scza);
+    TT_LAUNCH();
+    return 0;
+}
+
 static unsigned copy_blocks(long long n) {
     const long long cap = 8192;  // 32 per CU; each thread then takes a few words
     long long b = (n + 255) / 256;
@@ -249,6 +265,12 @@ static int xfer(int f, void* host, bool to_host) {
     }
     int ni = hi[0] - lo[0] + 1, nj = hi[1] - lo[1] + 1, nk = hi[2] - lo[2] + 1;
     long long n = (long long)ni * nj * nk;
+    if (!to_host && (f == TT_GRAVY || f == TT_GRAVZ)) {
+        bool zero = true;  // +0.0 everywhere, bit for bit
+        const uint64_t* w = (const uint64_t*)host;
+        for (long long t = 0; t < n && zero; ++t) zero = w[t] == 0;
+        (f == TT_GRAVY ? L.gy_zero : L.gz_zero) = zero;
+    }
     if (to_host) {
         k_gather<<<nblk(n, 256), 256, 0, L.s>>>(a, g_stage, L.g, lo[0], lo[1], lo[2], ni, nj, n);
         TT_LAUNCH();
@@ -268,7 +290,7 @@ static void free_all() {
         L.f[f] = nullptr;
     }
     double** tmp[] = {&L.ED,  &L.VD,      &L.GDN,    &L.HDN, &L.FLMX,   &L.FLMY, &L.FLMZ, &L.TOLD, &L.VH,
-                      &L.GH,  &L.HH,      &L.DFQ,    &L.DE1, &L.partial, &L.mustar, &L.sincoef, &L.met, &L.ftab, &L.btab, &L.ctab, &L.wtab, &g_stage};
+                      &L.GH,  &L.HH,      &L.DFQ,    &L.DE1, &L.partial, &L.mustar, &L.sincoef, &L.scza, &L.met, &L.ftab, &L.btab, &L.ctab, &L.wtab, &g_stage};
     for (auto p : tmp) {
         if (*p) hipFree(*p);
         *p = nullptr;
@@ -403,6 +425,8 @@ int tt_init(const tt_params* p) {
     gitter_host(L.hmet, L.metlen, *p);
     TT_HIP(dev_alloc(&L.met, (size_t)TT_M_COUNT * L.metlen * sizeof(double)));
     TT_HIP(hipMemcpyAsync(L.met, L.hmet, (size_t)TT_M_COUNT * L.metlen * sizeof(double), hipMemcpyHostToDevice, L.s));
+    TT_HIP(dev_alloc(&L.scza, (size_t)2 * L.metlen * sizeof(double)));
+    if ((rc = scza_impl())) return rc;
     // CALCMUSTAR (ppsub.f:156-179)
     std::vector<double> mu((size_t)g.gy * g.gz);
     for (int k = -1; k <= g.nz + 2; ++k)
@@ -472,6 +496,8 @@ int tt_upload_metric(int metric, const double* host) {
     for (int i = lo; i <= hi; ++i) L.hmet[(size_t)metric * L.metlen + i + 2] = host[i - lo];
     TT_HIP(hipMemcpyAsync(L.met + (size_t)metric * L.metlen, L.hmet + (size_t)metric * L.metlen,
                           L.metlen * sizeof(double), hipMemcpyHostToDevice, L.s));
+    if (metric == TT_M_XZA)
+        if (int rc = scza_impl()) return rc;
     TT_HIP(hipStreamSynchronize(L.s));
     return 0;
 }
