@@ -651,6 +651,7 @@ static int check_grid(int nx, int ny, int nz) {
 }
 
 bool cnn_ready() { return M.have_w && M.have_norm; }
+bool cnn_collecting() { return M.collect != 0; }
 
 void cnn_free() {}  // the model and the training set outlive tt_init/tt_finalize
 

@@ -12,6 +12,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <functional>
 #include <string>
 
 #include "../../include/ttgpu.h"
@@ -104,6 +105,9 @@ struct Ctrl {
     int total;         // inner iterations of the finished corrections
     int pad_c;
     int hist[16];      // inner iterations of each of the first 16 corrections (0: not run)
+    // the Newton loop's decision after an update, on the device (k_newton_decide)
+    int newton_fin;    // 1: the loop ends here (converged, the iteration cap, or a fatal flag)
+    int pad_n;
 };
 
 // Library state (one device, one stream).
@@ -135,6 +139,7 @@ struct Lib {
     double* wtab = nullptr;         // IKAPTYP 20-25 irradiation bins (50 x 50 x 30, log10)
     int wtab_version = -1;
     int have_de1 = 0;
+    double* de1_out = nullptr;      // set: the next energy solve also stores its correction there (DE1)
     double* partial = nullptr;      // block partial sums (reductions)
     int npartial = 0;
     Ctrl* ctrl = nullptr;           // device
@@ -174,6 +179,19 @@ struct Lib {
     // by an upload of anything else): the force kernel then adds the
     // constant instead of reading the array, the same arithmetic
     bool gy_zero = true, gz_zero = true;
+    // non-null: the kernels the *_impl helpers launch run only where this
+    // device flag is non-zero (work queued ahead of a control-block read,
+    // which returns at once when the flag was not set in stream order)
+    const int* run_if = nullptr;
+    // the work after a line-sweep energy solve, queued ahead of its control-
+    // block read under run_if = outer_done (set by energy_impl around the
+    // solve; sor_solve_impl then leaves storing the result to it)
+    std::function<int()> sor_tail;
+    // tt_step's work after COUPLED_ENERGY_EQS (BOUNDS, BOUNDS, PRESSG),
+    // queued by energy_impl ahead of its last read under newton_fin; set by
+    // tt_step, and trailing_done reports whether that read found it done
+    std::function<int()> step_trailing;
+    bool trailing_done = false;
     int64_t dev_bytes = 0;          // device memory held by the library
     hipEvent_t step_ev[4] = {};     // tt_step's phase boundaries (created by tt_init)
     hipEvent_t en_ev[4] = {};       // a Newton iteration's build / SOR / update boundaries
@@ -272,6 +290,9 @@ __device__ inline double block_sum256(double v, double* sh) { return block_reduc
 int fold_partials(const double* part, int nb, int nvec, int op, double* out0, double* out1 = nullptr,
                   double* out2 = nullptr);
 
+// A kernel launched under L.run_if returns at once unless the flag is set.
+__device__ inline bool skip_run(const int* run) { return run && *run == 0; }
+
 // Internal entry points shared between translation units
 int bounds_impl();
 int pressg_impl();
@@ -279,6 +300,7 @@ int qdens_impl();
 int sound_impl();
 int planetshift_impl(double* a, int f);
 int sor_solve_impl(int fill_from_evals, tt_sor_stats* st, int fieldmode);
+int sor_store_result(int mode);   // the solution into delta_ER (mode 0) or T (1), phi ghosts wrapped
 int sor_adjust_impl(int fill_from_evals, double* omega_out, int32_t* iters);
 int limits_impl(int stage, double delt);
 int sor_pack_impl();
@@ -314,6 +336,7 @@ int artvis_impl(double dt);       // NR_ARTIFICIAL_VISC, ppfor.f:1032-1230
 double dt_artvis();               // dt_artvis after the last artvis_impl (host; syncs)
 int sphericalvellimit_impl(int ivel, double velmax, double hydrotime);
 int sync_check();
+int check_flags();                // sync_check's tests on the host copy already read
 // The control block into its pinned host mirror (a one-wavefront kernel that
 // writes the mapped host memory, then a stream sync): no copy engine or blit
 // kernel per read.

@@ -22,6 +22,7 @@ int sor_scale_guess(double sc);
 int cnn_predict_guess(double delt);
 bool cnn_ready();
 int cnn_collect_sample(double delt);
+bool cnn_collecting();
 
 #define IX(i, j, k) g.idx(i, j, k)
 
@@ -45,11 +46,14 @@ __device__ inline double pow_exp(double x, double y) {
 
 // KAPPA (pprphy.f:1-360): IKAPTYP=1 constant; IKAPTYP=26 (new) power law
 // evaluated up to upperbnd, cells above take the upperbnd value; SUBSIG.
+// Q (non-null) receives COUPLED_ENERGY_EQS's Q = RH*CV*T (ppenergy.f:63-69)
+// in the same pass.
 __global__ void k_kappa(Geo g, tt_params p, const int32_t* __restrict__ ub, const double* __restrict__ RH,
                         const double* __restrict__ T, double* XKR, double* XKP, double* XKA, double* DKP,
-                        double* DKA, double* SIG, double sig) {
+                        double* DKA, double* SIG, double sig, double* Q, const double* __restrict__ CV) {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= g.n3) return;
+    if (Q) st_nt(&Q[t], RH[t] * CV[t] * T[t]);
     st_nt(&SIG[t], sig);
     if (p.ikaptyp == 1) {
         st_nt(&XKR[t], p.zkapmult); st_nt(&XKP[t], p.zkapmult); st_nt(&XKA[t], p.zkapmult);
@@ -904,8 +908,10 @@ struct UpdArgs {
 // is zero, anything computed from it (q + conv*dq, |dq/q|).  So the kernel
 // forms dq = (Q2*d - FQ)/DFQ and reads neither EVAL1..6, EVAL_1_6 nor the
 // neighbouring delta_ER values (as the reference does for finite ones).
-__global__ __launch_bounds__(256) void k_dq_update(Geo g, UpdArgs a, double* part, int nb, Ctrl* ctrl) {
+__global__ __launch_bounds__(256) void k_dq_update(Geo g, UpdArgs a, double* part, int nb, Ctrl* ctrl,
+                                                    const int* run) {
     __shared__ double sh[2][4];
+    if (skip_run(run)) return;
     double mq = 0.0, me = 0.0;
     int bad = 0;
     long long n = (long long)g.nx * g.ny * g.nz;
@@ -957,7 +963,9 @@ __device__ inline double flux_x_alias(const Geo& g, const double* FX, int i, int
 __global__ void k_energy_bc_x(Geo g, Met m, const int32_t* __restrict__ ub, double* Q, double* ER,
                               const double* __restrict__ RH, const double* __restrict__ CV,
                               const double* __restrict__ T, const double* __restrict__ FX,
-                              const double* __restrict__ XKR, int bcxmin, double d_t, double fbottom, int order) {
+                              const double* __restrict__ XKR, int bcxmin, double d_t, double fbottom, int order,
+                              const int* run) {
+    if (skip_run(run)) return;
     int nj = g.ny + 4, nk = g.nz + 4;
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nj * nk) return;
@@ -1000,7 +1008,8 @@ __global__ void k_energy_bc_x(Geo g, Met m, const int32_t* __restrict__ ub, doub
 }
 
 // theta edges: zero gradient (ppenergy.f:204-227, :727-757) per (i,j)
-__global__ void k_energy_bc_z(Geo g, double* Q, double* ER) {
+__global__ void k_energy_bc_z(Geo g, double* Q, double* ER, const int* run) {
+    if (skip_run(run)) return;
     int ni = g.nx + 4, nj = g.ny + 4;
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ni * nj) return;
@@ -1017,7 +1026,8 @@ __global__ void k_energy_bc_z(Geo g, double* Q, double* ER) {
 
 __global__ void k_energy_final(double* DER, double* T, const double* __restrict__ ER, const double* __restrict__ EROLD,
                                const double* __restrict__ Q, const double* __restrict__ RH,
-                               const double* __restrict__ CV, long long n) {
+                               const double* __restrict__ CV, long long n, const int* run) {
+    if (skip_run(run)) return;
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     st_nt(&DER[t], ER[t] - EROLD[t]);
@@ -1091,7 +1101,10 @@ int burrows_cell111_impl() {
     return 0;
 }
 
-static int kappa_impl() {
+// Q: also form Q = RH*CV*T where the opacity kernel passes over every cell
+// anyway (the power law and the constant); *q_done tells the caller
+static int kappa_impl(double* Q = nullptr, bool* q_done = nullptr) {
+    if (q_done) *q_done = false;
     if (table_kappa()) {
         k_clear_bad_kappa<<<1, 64, 0, L.s>>>(L.ctrl);
         TT_LAUNCH();
@@ -1157,8 +1170,10 @@ static int kappa_impl() {
     }
     if (L.p.ikaptyp != 1 && L.p.ikaptyp != 26) return fail(1, "KAPPA: IKAPTYP must be 1, 4-14 or 26");
     k_kappa<<<nblk(L.g.n3, 256), 256, 0, L.s>>>(L.g, L.p, L.ub, f[TT_RH], f[TT_T], f[TT_XKAPR], f[TT_XKAPP],
-                                                f[TT_XKAPA], f[TT_DXKAPP_DT], f[TT_DXKAPA_DT], f[TT_SIG], sig);
+                                                f[TT_XKAPA], f[TT_DXKAPP_DT], f[TT_DXKAPA_DT], f[TT_SIG], sig, Q,
+                                                f[TT_CV]);
     TT_LAUNCH();
+    if (q_done) *q_done = Q != nullptr;
     return 0;
 }
 
@@ -1266,11 +1281,26 @@ static int energy_bcs(int order) {
     int ncol = (g.ny + 4) * (g.nz + 4), nline = (g.nx + 4) * (g.ny + 4);
     k_energy_bc_x<<<nblk(ncol, 128), 128, 0, L.s>>>(g, met(), L.ub, f[TT_Q], f[TT_ER], f[TT_RH], f[TT_CV], f[TT_T],
                                                     f[TT_FLUX_X], f[TT_XKAPR], L.p.bcxmin, L.p.d_t, L.p.fbottom,
-                                                    order);
+                                                    order, L.run_if);
     TT_LAUNCH();
-    k_energy_bc_z<<<nblk(nline, 256), 256, 0, L.s>>>(g, f[TT_Q], f[TT_ER]);
+    k_energy_bc_z<<<nblk(nline, 256), 256, 0, L.s>>>(g, f[TT_Q], f[TT_ER], L.run_if);
     TT_LAUNCH();
     return 0;
+}
+
+// The Newton loop's test after an update, on the device: the loop ends when
+// max|dQ/Q| is not above delta_max or at the iteration cap (ppenergy.f:
+// 750-791), and not where a fatal flag is set (the host stops there).
+// newton_fin is written whatever the run flag says, so it never holds an
+// earlier solve's value.
+__global__ void k_newton_decide(Ctrl* c, double delta_max, int iter, int itmax, const int* run) {
+    if (threadIdx.x) return;
+    if (run && *run == 0) {
+        c->newton_fin = 0;
+        return;
+    }
+    const bool fatal = c->bad_q || c->bad_bound || c->bad_kappa;
+    c->newton_fin = !fatal && (iter == itmax || !(c->maxdq > delta_max)) ? 1 : 0;
 }
 
 int energy_impl(double delt, int iii, tt_energy_stats* st) {
@@ -1286,11 +1316,15 @@ int energy_impl(double delt, int iii, tt_energy_stats* st) {
     L.energy_iter = 0;
     if ((rc = dev_zero(f[TT_DELTA_ER], (size_t)g.n3 * sizeof(double)))) return rc;
     if ((rc = bounds_impl())) return rc;
-    k_q_init<<<nblk(g.n3, 256), 256, 0, L.s>>>(f[TT_Q], f[TT_RH], f[TT_CV], f[TT_T], g.n3);
-    TT_LAUNCH();
-    // KAPPA, SUBSIG, QDENS: T and RH stay fixed through the Newton loop, so the
-    // opacities FLIM recomputes while energy_iter <= 1 are these
-    if ((rc = kappa_impl())) return rc;
+    // Q = RH*CV*T, and KAPPA, SUBSIG, QDENS: T and RH stay fixed through the
+    // Newton loop, so the opacities FLIM recomputes while energy_iter <= 1
+    // are these.  The opacity pass forms Q too where it covers every cell.
+    bool q_done = false;
+    if ((rc = kappa_impl(f[TT_Q], &q_done))) return rc;
+    if (!q_done) {
+        k_q_init<<<nblk(g.n3, 256), 256, 0, L.s>>>(f[TT_Q], f[TT_RH], f[TT_CV], f[TT_T], g.n3);
+        TT_LAUNCH();
+    }
     // a state outside the opacity tables stops here, before the Newton loop
     // runs on partly stale opacities (pprphy.f:825-828,1040-1105 stop at once)
     if (table_kappa() && (rc = sync_check())) return rc;
@@ -1304,6 +1338,14 @@ int energy_impl(double delt, int iii, tt_energy_stats* st) {
     if ((rc = dev_copy(f[TT_ER_OLD], f[TT_ER], (size_t)g.n3 * sizeof(double)))) return rc;
     if ((rc = dev_copy(L.VD, f[TT_Q], (size_t)g.n3 * sizeof(double)))) return rc;  // Q_old
     int srcode = 0;
+    bool final_done = false;   // k_energy_final has run (queued ahead, k_newton_decide set newton_fin)
+    L.trailing_done = false;
+    auto final_impl = [&]() -> int {
+        k_energy_final<<<nblk(g.n3, 256), 256, 0, L.s>>>(f[TT_DELTA_ER], f[TT_T], f[TT_ER], f[TT_ER_OLD], f[TT_Q],
+                                                         f[TT_RH], f[TT_CV], g.n3, L.run_if);
+        TT_LAUNCH();
+        return 0;
+    };
     int first_choice = 0;      // the first solve's start: 0 cold, 1 warm, 2 learned
     hipEvent_t* ev = L.en_ev;  // phase boundaries of one Newton iteration
     for (;;) {
@@ -1376,11 +1418,53 @@ int energy_impl(double delt, int iii, tt_energy_stats* st) {
         }
         if (L.energy_iter == 1) first_choice = L.have_prev_guess;
         tt_sor_stats ss{};
+        // delta_Q, the damped update and the energy blocks after the solve
+        auto update = [&]() -> int {
+            TT_HIP(hipEventRecord(ev[2], L.s));
+            // 4 blocks per CU keep the grid-stride loop's loads in flight (max
+            // reductions: the result does not depend on the block count)
+            const long long ncell = (long long)g.nx * g.ny * g.nz;
+            int nb = (int)std::min<long long>(4096, (ncell + 255) / 256);
+            UpdArgs ua{f[TT_DELTA_ER], L.DFQ, f[TT_FQ], f[TT_RH], f[TT_XKAPP], f[TT_Q], f[TT_ER], L.ub, delt, conv_parm,
+                       L.p.bcxmin == 12 ? 1 : 2};
+            k_dq_update<<<nb, 256, 0, L.s>>>(g, ua, L.partial, nb, L.ctrl, L.run_if);
+            TT_LAUNCH();
+            int r = fold_partials(L.partial, nb, 2, 1, &L.ctrl->maxdq, &L.ctrl->maxde);
+            if (r) return r;
+            if ((r = energy_bcs(1))) return r;
+            TT_HIP(hipEventRecord(ev[3], L.s));
+            return 0;
+        };
+        // With line sweeps the solve's one control-block read comes after
+        // everything that follows it in the common case, queued ahead on
+        // device flags: the result's store, the update, the Newton test
+        // (k_newton_decide) and, where that test ends the loop, the final
+        // T and delta_ER and tt_step's trailing BOUNDS/PRESSG.  Every kernel
+        // of it returns at once where its flag is not set in stream order
+        // (the solve not ended, the loop going on), and the host then queues
+        // what is still to do as before.
+        const bool tail = L.opts.precision == 1 && L.opts.inner == 1 && !cnn_collecting();
+        if (tail)
+            L.sor_tail = [&]() -> int {
+                int r;
+                if ((r = sor_store_result(0))) return r;
+                if ((r = update())) return r;
+                k_newton_decide<<<1, 64, 0, L.s>>>(L.ctrl, delta_max, L.energy_iter, L.p.itmax, L.run_if);
+                TT_LAUNCH();
+                const int* keep = L.run_if;
+                L.run_if = &L.ctrl->newton_fin;
+                r = final_impl();
+                if (!r && L.step_trailing) r = L.step_trailing();
+                L.run_if = keep;
+                return r;
+            };
+        L.de1_out = L.energy_iter == 1 ? L.DE1 : nullptr;  // the first correction, for the next warm start
         int src = sor_solve_impl(0, &ss, 0);
+        L.de1_out = nullptr;
+        L.sor_tail = nullptr;
         if (src > 0) return src;
         if (src < 0) srcode = src;
         if (L.energy_iter == 1) {
-            if ((rc = dev_copy(L.DE1, f[TT_DELTA_ER], (size_t)g.n3 * sizeof(double)))) return rc;
             L.have_de1 = 1;
             if ((rc = cnn_collect_sample(delt))) return rc;  // training set of the learned guess, when collecting
         }
@@ -1396,20 +1480,15 @@ int energy_impl(double delt, int iii, tt_energy_stats* st) {
         if (L.have_prev_guess && ss.guess_used == 0) es.fallbacks++;
         if (ss.guess_used == 2) es.learned_used++;
         if (L.have_prev_guess == 2) L.learned_rejects = ss.guess_used == 2 ? 0 : L.learned_rejects + 1;
-        TT_HIP(hipEventRecord(ev[2], L.s));
-        // 4 blocks per CU keep the grid-stride loop's loads in flight (max
-        // reductions: the result does not depend on the block count)
-        const long long ncell = (long long)g.nx * g.ny * g.nz;
-        int nb = (int)std::min<long long>(4096, (ncell + 255) / 256);
-        UpdArgs ua{f[TT_DELTA_ER], L.DFQ, f[TT_FQ], f[TT_RH], f[TT_XKAPP], f[TT_Q], f[TT_ER], L.ub, delt, conv_parm,
-                   L.p.bcxmin == 12 ? 1 : 2};
-        k_dq_update<<<nb, 256, 0, L.s>>>(g, ua, L.partial, nb, L.ctrl);
-        TT_LAUNCH();
-        rc = fold_partials(L.partial, nb, 2, 1, &L.ctrl->maxdq, &L.ctrl->maxde);
-        if (rc) return rc;
-        if ((rc = energy_bcs(1))) return rc;
-        TT_HIP(hipEventRecord(ev[3], L.s));
-        if ((rc = sync_check())) return rc;
+        if (tail) {
+            // the solve's read came after the update (and the rest): its
+            // flags are in the host copy already
+            if ((rc = check_flags())) return rc;
+            final_done = L.hctrl->newton_fin != 0;
+        } else {
+            if ((rc = update())) return rc;
+            if ((rc = sync_check())) return rc;
+        }
         float t01 = 0, t12 = 0, t23 = 0;
         TT_HIP(hipEventElapsedTime(&t01, ev[0], ev[1]));
         TT_HIP(hipEventElapsedTime(&t12, ev[1], ev[2]));
@@ -1434,9 +1513,10 @@ int energy_impl(double delt, int iii, tt_energy_stats* st) {
         }
         if (!(es.max_dq > delta_max)) break;
     }
-    k_energy_final<<<nblk(g.n3, 256), 256, 0, L.s>>>(f[TT_DELTA_ER], f[TT_T], f[TT_ER], f[TT_ER_OLD], f[TT_Q],
-                                                     f[TT_RH], f[TT_CV], g.n3);
-    TT_LAUNCH();
+    // the final T and delta_ER, unless queued ahead and run already; then
+    // tt_step's trailing work has run too, where it gave some
+    if (!final_done && (rc = final_impl())) return rc;
+    L.trailing_done = final_done && L.step_trailing;
     if (st) *st = es;
     return srcode;
 }

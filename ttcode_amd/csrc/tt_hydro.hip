@@ -806,14 +806,24 @@ int tt_step(double delt, int iii, tt_step_stats* st) {
     if (rc) return rc;
     if ((rc = limits_impl(2, delt))) return rc;
     TT_HIP(hipEventRecord(e[2], L.s));
+    // BOUNDS, BOUNDS, PRESSG after COUPLED_ENERGY_EQS; the energy solve may
+    // queue them ahead of its last control-block read (then that read has
+    // checked their flags)
+    auto trailing = [&]() -> int {
+        int r;
+        if ((r = bounds_impl()) || (r = bounds_impl()) || (r = pressg_impl())) return r;
+        TT_HIP(hipEventRecord(e[3], L.s));
+        return 0;
+    };
+    const bool queue_ahead = !(L.p.bcxmin == 9 && iii <= 0);  // BurrowsOPC at III <= 0 is not queued ahead
+    if (queue_ahead) L.step_trailing = trailing;
     int erc = energy_impl(delt, iii, &s.energy);
+    L.step_trailing = nullptr;
     if (erc > 0) return erc;
-    if ((rc = bounds_impl())) return rc;
-    if ((rc = bounds_impl())) return rc;
-    if ((rc = pressg_impl())) return rc;
-    TT_HIP(hipEventRecord(e[3], L.s));
-    rc = sync_check();
-    if (rc) return rc;
+    if (!L.trailing_done) {
+        if ((rc = trailing())) return rc;
+        if ((rc = sync_check())) return rc;
+    }
     float a = 0, b = 0, c = 0;
     TT_HIP(hipEventElapsedTime(&a, e[0], e[1]));
     TT_HIP(hipEventElapsedTime(&b, e[1], e[2]));

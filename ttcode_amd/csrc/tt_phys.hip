@@ -67,7 +67,9 @@ __device__ inline void shift5_body(const Geo& g, double* T, double* RH, double* 
     G[g.idx(i, ny + 2, k)] = sg[2];
 }
 
-__global__ __launch_bounds__(256) void k_shift5(Geo g, double* T, double* RH, double* V, double* G, double* H) {
+__global__ __launch_bounds__(256) void k_shift5(Geo g, double* T, double* RH, double* V, double* G, double* H,
+                                                const int* run) {
+    if (skip_run(run)) return;
     shift5_body(g, T, RH, V, G, H, blockIdx.x);
 }
 
@@ -127,7 +129,9 @@ __device__ inline void upperbnd_repeat_body(const Geo& g, const double* __restri
 // the interior columns it wraps to, which the shift does not write, and the
 // shift does not read upperbnd.  Blocks [0, nshift) shift, the rest search.
 __global__ __launch_bounds__(256) void k_shift5_upperbnd(Geo g, double* T, double* RH, double* V, double* G,
-                                                         double* H, int32_t* ub, int nshift, int repeat) {
+                                                         double* H, int32_t* ub, int nshift, int repeat,
+                                                         const int* run) {
+    if (skip_run(run)) return;
     if ((int)blockIdx.x < nshift) shift5_body(g, T, RH, V, G, H, blockIdx.x);
     else if (repeat) upperbnd_repeat_body(g, RH, ub, (blockIdx.x - nshift) * 256 + threadIdx.x);
     else upperbnd_body(g, RH, ub, blockIdx.x - nshift);
@@ -142,7 +146,8 @@ __global__ __launch_bounds__(256) void k_shift5_upperbnd(Geo g, double* T, doubl
 __global__ void k_xbc(Geo g, Met m, const int32_t* __restrict__ ub, double* T, double* RH, double* V, double* G,
                       double* H, const double* __restrict__ ER, const double* __restrict__ SIN,
                       const double* __restrict__ XKP, const double* __restrict__ XKR, int bcxmin, double d_t,
-                      double fbottom, int ibdrad, int iii, double tirr, Ctrl* ctrl) {
+                      double fbottom, int ibdrad, int iii, double tirr, Ctrl* ctrl, const int* run) {
+    if (skip_run(run)) return;
     int nj = g.ny + 4, nk = g.nz + 4;
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nj * nk) return;
@@ -233,7 +238,8 @@ __global__ void k_xbc(Geo g, Met m, const int32_t* __restrict__ ub, double* T, d
 // writes the other's source plane (tt_init takes nz >= 4), so the line reads
 // its two source planes first, in one batch, and forms every written value
 // from them in the reference's order.
-__global__ void k_zbc(Geo g, Met m, double* T, double* RH, double* V, double* G, double* H) {
+__global__ void k_zbc(Geo g, Met m, double* T, double* RH, double* V, double* G, double* H, const int* run) {
+    if (skip_run(run)) return;
     int ni = g.nx + 4, nj = g.ny + 4;
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ni * nj) return;
@@ -296,26 +302,28 @@ int bounds_impl() {
     const unsigned nshift = nblk(ns, 256);
     const int rep = L.ub_repeat ? 1 : 0;
     k_shift5_upperbnd<<<nshift + (rep ? nblk(ncol, 256) : nblk(ncol, 4)), 256, 0, L.s>>>(
-        g, f[TT_T], f[TT_RH], f[TT_V], f[TT_G], f[TT_H], L.ub, (int)nshift, rep);
+        g, f[TT_T], f[TT_RH], f[TT_V], f[TT_G], f[TT_H], L.ub, (int)nshift, rep, L.run_if);
     TT_LAUNCH();
     if (L.p.bcxmin == 9 && L.iii <= 0) {
+        if (L.run_if) return fail(1, "BOUNDS: BurrowsOPC at III <= 0 cannot be queued ahead");
         int rc = burrows_cell111_impl();
         if (rc) return rc;
     }
     k_xbc<<<nblk(ncol, 128), 128, 0, L.s>>>(g, met(), L.ub, f[TT_T], f[TT_RH], f[TT_V], f[TT_G], f[TT_H],
                                             f[TT_ER], f[TT_STELLARINPUT], f[TT_XKAPP], f[TT_XKAPR], L.p.bcxmin,
-                                            L.p.d_t, L.p.fbottom, L.p.ibdrad, L.iii, L.p.tirr, L.ctrl);
+                                            L.p.d_t, L.p.fbottom, L.p.ibdrad, L.iii, L.p.tirr, L.ctrl, L.run_if);
     TT_LAUNCH();
-    k_zbc<<<nblk(nline, 256), 256, 0, L.s>>>(g, met(), f[TT_T], f[TT_RH], f[TT_V], f[TT_G], f[TT_H]);
+    k_zbc<<<nblk(nline, 256), 256, 0, L.s>>>(g, met(), f[TT_T], f[TT_RH], f[TT_V], f[TT_G], f[TT_H], L.run_if);
     TT_LAUNCH();
-    k_shift5<<<nblk(ns, 256), 256, 0, L.s>>>(g, f[TT_T], f[TT_RH], f[TT_V], f[TT_G], f[TT_H]);
+    k_shift5<<<nblk(ns, 256), 256, 0, L.s>>>(g, f[TT_T], f[TT_RH], f[TT_V], f[TT_G], f[TT_H], L.run_if);
     TT_LAUNCH();
     L.ub_repeat = true;
     return 0;
 }
 
 __global__ void k_pressg(double* PG, const double* __restrict__ T, const double* __restrict__ RH, double mu,
-                         long long n) {
+                         long long n, const int* run) {
+    if (skip_run(run)) return;
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) PG[t] = RGAS * T[t] * RH[t] / mu;
 }
@@ -339,7 +347,7 @@ __global__ void k_qdens(Geo g, Met m, const double* __restrict__ RH, double* QX,
 }
 
 int pressg_impl() {
-    k_pressg<<<nblk(L.g.n3, 256), 256, 0, L.s>>>(L.f[TT_PG], L.f[TT_T], L.f[TT_RH], L.p.mu_gas, L.g.n3);
+    k_pressg<<<nblk(L.g.n3, 256), 256, 0, L.s>>>(L.f[TT_PG], L.f[TT_T], L.f[TT_RH], L.p.mu_gas, L.g.n3, L.run_if);
     TT_LAUNCH();
     return 0;
 }
@@ -468,6 +476,10 @@ __global__ void k_dif_matrix(Geo g, Met m, double delt, double c, const double* 
 int sync_check() {
     int rc0 = ctrl_to_host();
     if (rc0) return rc0;
+    return check_flags();
+}
+
+int check_flags() {
     if (L.hctrl->bad_bound) return fail(3, "error in boundxmax: upperbnd above the domain");
     if (L.hctrl->bad_q) return fail(2, "ERROR IN DELTA_Q: non-positive value of Q");
     if (L.hctrl->bad_kappa)

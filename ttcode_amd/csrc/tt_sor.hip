@@ -872,8 +872,11 @@ __global__ void k_store_x(Geo g, CS cs, const double* __restrict__ x0, const dou
 
 // unknowns of both colours -> a whole ghosted field: the interior from x,
 // every other cell 0 (a zero fill and k_store_x in one pass)
+// (fld2, when set, receives the same values: the first Newton correction
+// kept for the next step's warm start, whose phi ghosts no reader uses)
 __global__ void k_store_x_box(Geo g, CS cs, const double* __restrict__ x0, const double* __restrict__ x1,
-                              double* fld) {
+                              double* fld, double* fld2, const int* run) {
+    if (skip_run(run)) return;
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= g.n3) return;
     int i, j, k;
@@ -883,12 +886,14 @@ __global__ void k_store_x_box(Geo g, CS cs, const double* __restrict__ x0, const
         const double* x = ((i + j + k) & 1) ? x1 : x0;
         v = x[(long long)cs.px * ((j - 1) + (long long)cs.ny * k) + ((i - 1) >> 1) + 2];
     }
-    fld[t] = v;
+    st_nt(&fld[t], v);
+    if (fld2) st_nt(&fld2[t], v);
 }
 
 // PLANETSHIFT for one rank (ppshift.f:1009-1214): periodic phi ghosts for
 // k = 1..nz, i = low..nx+2; skipm1 for arrays without a j = -1 plane.
-__global__ void k_planetshift(double* a, Geo g, int low, int skipm1) {
+__global__ void k_planetshift(double* a, Geo g, int low, int skipm1, const int* run) {
+    if (skip_run(run)) return;
     int ni = g.nx + 2 - low + 1;
     long long n = (long long)ni * g.nz;
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -904,7 +909,7 @@ int planetshift_impl(double* a, int f) {
     int low = (f == 3) ? 0 : -1;
     int skip = (f == 4 || f == 11) ? 1 : 0;
     long long n = (long long)(L.g.nx + 3 - low) * L.g.nz;
-    k_planetshift<<<nblk(n, 256), 256, 0, L.s>>>(a, L.g, low, skip);
+    k_planetshift<<<nblk(n, 256), 256, 0, L.s>>>(a, L.g, low, skip, L.run_if);
     TT_LAUNCH();
     return 0;
 }
@@ -1626,6 +1631,12 @@ static int line_chain(int stag, int max_outer) {
         }
         g_outer_tag = -1;
         if (rc) break;
+        if (L.sor_tail) {  // what follows the solve, on the device once it has ended
+            L.run_if = &L.ctrl->outer_done;
+            rc = L.sor_tail();
+            L.run_if = nullptr;
+            if (rc) break;
+        }
         rc = ctrl_to_host();
         if (rc) break;
         if (L.timing) ev_drain(L.hctrl, g_bid);
@@ -1694,10 +1705,11 @@ int sor_solve_impl(int fill_from_evals, tt_sor_stats* st, int mode) {
         // cold start: zero; a guess (warm/learned) is staged in DELTA_ER by
         // the caller and accepted only if its residual is not larger
         bool use_guess = L.opts.guess != 0 && L.have_prev_guess != 0;
-        for (int c = 0; c < 2; ++c) {
-            k_load_x<<<nblk(pl, 256), 256, 0, L.s>>>(g, g_cs, c, f[TT_DELTA_ER], x64[c], 1);
-            TT_LAUNCH();
-        }
+        if (use_guess)
+            for (int c = 0; c < 2; ++c) {
+                k_load_x<<<nblk(pl, 256), 256, 0, L.s>>>(g, g_cs, c, f[TT_DELTA_ER], x64[c], 1);
+                TT_LAUNCH();
+            }
         s.guess_used = 0;
         if (use_guess) {
             rc = read_ctrl();
@@ -1852,14 +1864,10 @@ int sor_solve_impl(int fill_from_evals, tt_sor_stats* st, int mode) {
     s.rhsnorm = L.hctrl->rhsnorm;
     if (mode == 0 && !s.guess_used && s.r0_cold == 0.0) s.r0_cold = s.rhsnorm;  // zero start: residual = RHS
     // unknowns back into the ghosted field (delta_ER: zero outside the
-    // interior), then the phi wrap
-    if (mode == 0)
-        k_store_x_box<<<nblk(g.n3, 256), 256, 0, L.s>>>(g, g_cs, x64[0], x64[1], target);
-    else
-        k_store_x<<<nblk(2 * g_cs.ncell, 256), 256, 0, L.s>>>(g, g_cs, x64[0], x64[1], target);
-    TT_LAUNCH();
-    rc = planetshift_impl(target, mode == 1 ? 10 : 8);
-    if (rc) return rc;
+    // interior), then the phi wrap; after line sweeps with a tail, the tail
+    // has queued that ahead of the solve's read
+    const bool tail = prec == 1 && L.opts.inner == 1 && L.sor_tail;
+    if (!tail && (rc = sor_store_result(mode))) return rc;
     if (st) *st = s;
     if (L.hctrl->done == 2) {
         fail(-1, mode == 1 ? "Max iteration reached: DIF SOR matrix solver" : "MAX COUPLED SOR ITERATION");
@@ -1869,6 +1877,17 @@ int sor_solve_impl(int fill_from_evals, tt_sor_stats* st, int mode) {
 }
 
 // scale the last delta_ER in place (warm start: delta_ER <- s * delta_ER)
+int sor_store_result(int mode) {
+    Geo& g = L.g;
+    double* target = mode == 1 ? L.f[TT_T] : L.f[TT_DELTA_ER];
+    if (mode == 0)
+        k_store_x_box<<<nblk(g.n3, 256), 256, 0, L.s>>>(g, g_cs, x64[0], x64[1], target, L.de1_out, L.run_if);
+    else
+        k_store_x<<<nblk(2 * g_cs.ncell, 256), 256, 0, L.s>>>(g, g_cs, x64[0], x64[1], target);
+    TT_LAUNCH();
+    return planetshift_impl(target, mode == 1 ? 10 : 8);
+}
+
 int sor_scale_guess(double sc) {
     k_scale<<<nblk(L.g.n3, 256), 256, 0, L.s>>>(L.f[TT_DELTA_ER], sc, L.g.n3);
     TT_LAUNCH();

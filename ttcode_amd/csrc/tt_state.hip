@@ -13,7 +13,8 @@
 namespace tt {
 
 __global__ __launch_bounds__(256) void k_fold(const double* __restrict__ part, int nb, int nvec, int op, double* o0,
-                                              double* o1, double* o2) {
+                                              double* o1, double* o2, const int* run) {
+    if (skip_run(run)) return;
     __shared__ double sh[4];
     for (int v = 0; v < nvec; ++v) {
         const double id = op == 0 ? 0.0 : (op == 1 ? -1e308 : 1e308);
@@ -44,7 +45,7 @@ int ctrl_to_host() {
 }
 
 int fold_partials(const double* part, int nb, int nvec, int op, double* out0, double* out1, double* out2) {
-    k_fold<<<1, 256, 0, L.s>>>(part, nb, nvec, op, out0, out1, out2);
+    k_fold<<<1, 256, 0, L.s>>>(part, nb, nvec, op, out0, out1, out2, L.run_if);
     TT_LAUNCH();
     return 0;
 }
