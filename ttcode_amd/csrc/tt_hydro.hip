@@ -214,9 +214,11 @@ struct AdvE {
     __device__ static double fz(const AdvArgs& a, int n, Step s, const XF& z, int, double, double) {
         return face(a, a.H, n, s.z, z);
     }
-    __device__ static void finish(const AdvArgs& a, int n, double vx, double vy, const PM& pm, double fx0, double fx1,
-                                  double fy0, double fy1, double fz0, double fz1) {
-        st_nt(&a.out[n], a.A[n] - (fx1 - fx0) / vx - (fy1 - fy0) / vy - (fz1 - fz0) / pm.volz);
+    // the cell's own transported amount the update starts from
+    __device__ static double own(const AdvArgs& a, int n) { return a.A[n]; }
+    __device__ static void finish(const AdvArgs& a, int n, double q0, double vx, double vy, const PM& pm, double fx0,
+                                  double fx1, double fy0, double fy1, double fz0, double fz1) {
+        st_nt(&a.out[n], q0 - (fx1 - fx0) / vx - (fy1 - fy0) / vy - (fz1 - fz0) / pm.volz);
         st_nt(&a.cf1[n], fx0);
         st_nt(&a.cf2[n], fy0);
         st_nt(&a.cf3[n], fz0);
@@ -276,9 +278,10 @@ struct AdvV {
     __device__ static double fz(const AdvArgs& a, int n, Step s, const XF& z, int, double p1, double p0) {
         return fyz(a, a.H, a.FZ, n, s.z, z.f, p1, p0);
     }
-    __device__ static void finish(const AdvArgs& a, int n, double vx, double vy, const PM& pm, double fx0, double fx1,
-                                  double fy0, double fy1, double fz0, double fz1) {
-        st_nt(&a.out[n], a.V[n] * a.RHQ[n] - (fx1 - fx0) / vx - (fy1 - fy0) / vy - (fz1 - fz0) / pm.volz);
+    __device__ static double own(const AdvArgs& a, int n) { return a.V[n] * a.RHQ[n]; }
+    __device__ static void finish(const AdvArgs& a, int n, double q0, double vx, double vy, const PM& pm, double fx0,
+                                  double fx1, double fy0, double fy1, double fz0, double fz1) {
+        st_nt(&a.out[n], q0 - (fx1 - fx0) / vx - (fy1 - fy0) / vy - (fz1 - fz0) / pm.volz);
     }
 };
 
@@ -339,9 +342,10 @@ struct AdvG {
     __device__ static double fz(const AdvArgs& a, int n, Step s, const XF& z, int, double p1, double p0) {
         return fxz(a, a.H, a.FZ, n, s.z, s.y, z.f, p1, p0);
     }
-    __device__ static void finish(const AdvArgs& a, int n, double vx, double vy, const PM& pm, double fx0, double fx1,
-                                  double fy0, double fy1, double fz0, double fz1) {
-        st_nt(&a.out[n], a.A[n] * a.RHQ[n] - (fx1 - fx0) / vx - (fy1 - fy0) / vy - (fz1 - fz0) / pm.volz);
+    __device__ static double own(const AdvArgs& a, int n) { return a.A[n] * a.RHQ[n]; }
+    __device__ static void finish(const AdvArgs& a, int n, double q0, double vx, double vy, const PM& pm, double fx0,
+                                  double fx1, double fy0, double fy1, double fz0, double fz1) {
+        st_nt(&a.out[n], q0 - (fx1 - fx0) / vx - (fy1 - fy0) / vy - (fz1 - fz0) / pm.volz);
     }
 };
 
@@ -400,9 +404,10 @@ struct AdvH {
         double gr = vl_face(up, q_1, q0, q1, q2, z.f.d0, z.f.d1, z.f.d2, 1, &qu);
         return half * (f0 + f1) * (qu + (z.f.xa - (up ? z.f.x0 : z.f.x1) - half * S) * gr);
     }
-    __device__ static void finish(const AdvArgs& a, int n, double vx, double vy, const PM& pm, double fx0, double fx1,
-                                  double fy0, double fy1, double fz0, double fz1) {
-        st_nt(&a.out[n], a.A[n] * a.RHQ[n] - (fx1 - fx0) / vx - (fy1 - fy0) / vy - (fz1 - fz0) / pm.volz);
+    __device__ static double own(const AdvArgs& a, int n) { return a.A[n] * a.RHQ[n]; }
+    __device__ static void finish(const AdvArgs& a, int n, double q0, double vx, double vy, const PM& pm, double fx0,
+                                  double fx1, double fy0, double fy1, double fz0, double fz1) {
+        st_nt(&a.out[n], q0 - (fx1 - fx0) / vx - (fy1 - fy0) / vy - (fz1 - fz0) / pm.volz);
     }
 };
 
@@ -500,9 +505,12 @@ __global__ __launch_bounds__(320) void k_adv_t(Geo g, Met m, AdvArgs a) {
             else sy[p][ADV_TJ][l - ADV_TJ] = fy;
         }
         if (cell) zhi = K::fz(a, n + (K::oz + 1) * st.z, st, K::zmet(m, k + K::oz + 1), k + K::oz + 1, zp1, zp0);
+        // the cell's own amount is read before the barrier, with the faces'
+        // loads, so the update after it waits on LDS only
+        const double q0 = cell ? K::own(a, n) : 0.0;
         __syncthreads();
         if (cell) {
-            K::finish(a, n, volx, voly, pm, sx[p][r][c], sx[p][r][c + 1], sy[p][r][c], sy[p][r + 1][c], zlo, zhi);
+            K::finish(a, n, q0, volx, voly, pm, sx[p][r][c], sx[p][r][c + 1], sy[p][r][c], sy[p][r + 1][c], zlo, zhi);
             zlo = zhi;
         }
     }

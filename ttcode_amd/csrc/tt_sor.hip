@@ -1031,14 +1031,16 @@ __global__ void k_rhs_f32(CS cs, const double* __restrict__ C0, const double* __
     if (rline) rline[(line_row(jj, kk, cs.ny, cs.nz) * 8 + 7) * nx + 2 * h + ((jj + kk + c + 1) & 1)] = (float)v;
 }
 
-// x64 (padded, colour cc) += the line unknowns over the interior cells,
+// x64 (padded, both colours) += the line unknowns over the interior cells,
 // which are then zeroed: the next correction's inner solve starts from 0.
 // Nothing once the outer loop has ended.
-__global__ void k_add_corr_line(CS cs, int cc, int nx, double* x, float* xl, const Ctrl* guard) {
+__global__ void k_add_corr_line(CS cs, int nx, double* x0, double* x1, float* xl, const Ctrl* guard) {
     if (guard->outer_done) return;
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= cs.ncell) return;
-    int tt = (int)t;
+    if (t >= 2 * cs.ncell) return;
+    const int cc = t >= cs.ncell;
+    double* x = cc ? x1 : x0;
+    int tt = (int)(t - cc * cs.ncell);
     int r = tt / cs.hx;
     int h = tt - r * cs.hx;
     int kk = r / cs.ny;
@@ -1618,10 +1620,8 @@ static int line_chain(int stag, int max_outer) {
             if (rc) break;
             k_finish<<<1, 256, 0, L.s>>>(L.ctrl, L.partial + ((ni - 1) & 1) * 2 * g_nb, 2 * g_pn, 1, nullptr);
             TT_LAUNCH();
-            for (int cc = 0; cc < 2; ++cc) {
-                k_add_corr_line<<<nblk(g_cs.ncell, 256), 256, 0, L.s>>>(g_cs, cc, g_ls.nx, x64[cc], g_lx, L.ctrl);
-                TT_LAUNCH();
-            }
+            k_add_corr_line<<<nblk(2 * g_cs.ncell, 256), 256, 0, L.s>>>(g_cs, g_ls.nx, x64[0], x64[1], g_lx, L.ctrl);
+            TT_LAUNCH();
             k_resid64<<<nb, 256, 0, L.s>>>(g_cs, c64[0], c64[1], x64[0], x64[1], nullptr, nullptr, g_lco, g_ls.nx,
                                            part, chunk, L.ctrl);
             TT_LAUNCH();
