@@ -15,7 +15,7 @@ per step, cell-updates/s; the same for FP32 SOR inside FP64 defect
 correction with a cold start (`fp32`), a warm start (`fp32_warm`) and the
 learned initial guess (`learned_fp32`, weights from ttcode_amd/data); the HBM
 roofline of the dominant kernel (the FP64 SOR
-half sweep; every 7th launch in the timed region carries HIP start/stop
+half sweep; every 51st launch in the timed region carries HIP start/stop
 events recorded by its own dispatch packet on the library's stream, which
 alternates red and black sweeps and keeps the events' cost out of ms/step),
 and the CPU oracle timed on the host cores as the reported CPU baseline.

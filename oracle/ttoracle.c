@@ -2622,7 +2622,7 @@ int tor_forgas3d(double delt) /* FORGAS3D, ppfor.f:142-397 (IGRAV=1) */
                     A(O.G, i, j, k) = (A(O.G, i, j, k) + A(GH, i, j, k)) / 2.0;
                     A(O.H, i, j, k) = (A(O.H, i, j, k) + A(HH, i, j, k)) / 2.0;
                 }
-        /* the velocity ceiling on the intermediate velocities, ppfor.f:330-342 */
+        /* the velocity ceiling on the intermediate velocities, ppfor.f:303-315 */
         if (g_lim.ivel != 0 && (rc = tor_sphericalvellimit(g_lim.ivel, g_lim.velmax, g_lim.hydrotime))) goto out;
         rc = tor_bounds();
         if (rc) goto out;

@@ -750,7 +750,7 @@ int forgas_impl(double dt) {
     TT_LAUNCH();
     if (adiab) {
         std::swap(f[TT_T], L.TOLD);  // T = TOLD
-        // the velocity ceiling on the intermediate velocities (ppfor.f:330-342)
+        // the velocity ceiling on the intermediate velocities (ppfor.f:303-315)
         if (L.lim.ivel != 0 && (rc = sphericalvellimit_impl(L.lim.ivel, L.lim.velmax, L.lim.hydrotime))) return rc;
         if ((rc = bounds_impl())) return rc;
         k_pdv<<<nblk(nc, 256), 256, 0, L.s>>>(g, met(), dt, 1, 0, f[TT_T], f[TT_PG], f[TT_RH], f[TT_CV], f[TT_V], f[TT_G],
