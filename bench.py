@@ -92,9 +92,16 @@ def run_mode(dev, st0, dt, steps, warmup, precision, guess, event_period=51, inn
     dev.load_state(st0)
     dev.set_solver(precision=precision, guess=guess, batch=16, inner_tol=1e-5, inner_max=20000, inner=inner)
     iii = 2
-    for _ in range(warmup):
+    # line sweeps: the first solve after set_solver chooses their
+    # over-relaxation by trial solves, so at least one step stays untimed
+    search_s = None
+    for w in range(max(warmup, 1) if inner == 1 else warmup):
+        t0 = time.perf_counter()
         dev.step(dt, iii)
         iii += 1
+        if inner == 1 and w == 0:
+            dev.sync()
+            search_s = time.perf_counter() - t0
     dev.sync()
     dev.sweep_timing(event_period)
     newton = sor = f32 = first = used = fall = skip = 0
@@ -123,7 +130,12 @@ def run_mode(dev, st0, dt, steps, warmup, precision, guess, event_period=51, inn
     n64, ms64, n32, ms32 = dev.sweep_timing_get()
     dev.sweep_timing(0)
     final = {k: dev.download(k) for k in ("ER", "T")}
+    line = {}
+    if inner == 1:
+        om, trials = dev.get_line_omega()
+        line = {"line_omega": om, "line_omega_trials": trials, "first_step_with_search_s": round(search_s, 4)}
     return {
+        **line,
         "final": final,
         "ms_per_step": (t1 - t0) * 1e3 / steps,
         "newton_per_step": newton / steps,
@@ -170,9 +182,12 @@ def run_large(a):
     dev.set_solver(precision=1, guess=2, batch=16, inner_tol=1e-5, inner_max=20000, inner=inner)
     print(f"# large workload {nx}x{ny}x{nz} dt={dt:.4g}s, {a.inner} sweeps", file=sys.stderr, flush=True)
     iii = 2
-    for _ in range(a.warmup):
+    # line sweeps: the first solve chooses their over-relaxation by trial
+    # solves, so at least one step stays untimed
+    for _ in range(max(a.warmup, 1) if inner == 1 else a.warmup):
         dev.step(dt, iii)
         iii += 1
+    line_om = dev.get_line_omega() if inner == 1 else None
     dev.bounds()
     e0 = dev.energy_totals()
     dev.sync()
@@ -207,6 +222,7 @@ def run_large(a):
                    "nr": nx, "ntheta": nz, "nphi": ny, "dt_s": dt, "inner_sweeps": a.inner},
         "cells": nx * ny * nz, "cell_updates_per_s": nx * ny * nz / (ms * 1e-3),
         "fp32_sweeps_per_step": sweeps / a.steps,
+        **({"line_omega": line_om[0], "line_omega_trials": line_om[1]} if line_om else {}),
         # the network runs on first Newton solves only; each run's guess is
         # kept or falls back to the plain (cold/warm) solve
         "learned_network_ran": a.steps - skipped, "learned_guess_used": learned,
@@ -479,6 +495,9 @@ def main():
                 a_, b_ = m["final"][k][2:-2, 2:-2, 2:-2], plain["final"][k][2:-2, 2:-2, 2:-2]
                 errs[k] = float(np.max(np.abs(a_ - b_) / np.abs(b_)))
             d["max_rel_err_vs_fp64_after_steps"] = {k: float(f"{v:.3e}") for k, v in errs.items()}
+            for k in ("line_omega", "line_omega_trials", "first_step_with_search_s"):
+                if k in m:
+                    d[k] = m[k]
             if m["sweep32_launches"]:
                 l32 = m["sweep32_ms"] / m["sweep32_launches"]
                 d["fp32_sweep_us"] = round(l32 * 1e3, 3)
@@ -503,8 +522,8 @@ def main():
                                    "guess_scale": round(m["guess_scale_first"], 4),
                                    "guess_r0_over_cold": round(m["guess_r0_first"], 4)}
         line_method = ("FP32 red-black radial-line SOR (each r column solved exactly as a tridiagonal system, "
-                       "columns coloured by (j+k) parity, omega 1.1) inside FP64 defect correction to the same "
-                       "1e-10 FP64 residual")
+                       "columns coloured by (j+k) parity, omega chosen by trial solves of the first system) inside "
+                       "FP64 defect correction to the same 1e-10 FP64 residual")
         if "line_cold" in modes:
             out["fp32_line"] = {"method": line_method + "; cold start", **summary(modes["line_cold"])}
         if "line_warm" in modes:

@@ -187,7 +187,8 @@ int tt_metric_info(int metric, int32_t* lo, int32_t* hi);
  * FP64 residual does not exceed the cold start's.
  * inner (precision 1 only) picks the FP32 sweeps of the inner solves: 0 =
  * point red-black SOR with omega = SORPARAM, the reference's own sweep in
- * FP32; 1 = red-black radial-line SOR with omega = line_omega: each radial
+ * FP32; 1 = red-black radial-line SOR with omega = line_omega (0: chosen by
+ * a search on the first line solve, see tt_get_line_omega): each radial
  * column (j,k) is solved exactly as a tridiagonal system in r, the columns
  * coloured by (j+k) parity, so the strong radial couplings of the FLD
  * operator are inside the line solves and the sweep count to the same FP64
@@ -204,7 +205,8 @@ typedef struct tt_solver_opts {
     int32_t inner;           /* precision 1: 0 point sweeps, 1 radial lines  */
     double inner_tol;        /* FP32 inner L1 ratio target (precision 1)     */
     double warm_scale;       /* scale of the previous delta_ER for guess 1   */
-    double line_omega;       /* over-relaxation of the line sweeps (0, 2)    */
+    double line_omega;       /* line sweeps' over-relaxation in (0, 2); 0 =
+                                chosen by a search (default)                 */
 } tt_solver_opts;
 
 typedef struct tt_sor_stats {
@@ -242,6 +244,16 @@ int tt_adjust_sor_param(int fill_matrix, double* omega, int32_t* iters);
  * changed by ADJUST_SOR_PARAM). */
 int tt_set_sorparam(double omega);
 double tt_get_sorparam(void);
+
+/* The over-relaxation the radial-line sweeps use (tt_solver_opts.inner 1):
+ * line_omega where it is set; where it is 0 (the default), the value a search
+ * chose on the first line solve after tt_init or tt_set_solver -- cold-start
+ * solves of that system to the FP64 target over omega in [1, 1.95], the
+ * fewest line iterations winning, as ADJUST_SOR_PARAM (ppsor.f:332-354)
+ * chooses SORPARAM -- or 0 before that solve.  *trials: the search's trial
+ * solves (0 with line_omega set).  No counterpart in the reference: the line
+ * sweeps are this library's own inner solver. */
+int tt_get_line_omega(double* omega, int32_t* trials);
 
 /* ------------------------------------------------------------------------
  * Physics entry points, in the step order of pp.f:483-821.

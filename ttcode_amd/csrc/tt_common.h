@@ -160,6 +160,8 @@ struct Lib {
     int start_picks = 0;
     std::string err;
     // sweep timing
+    double line_om_auto = 0.0;  // the line sweeps' omega a search chose (tt_solver_opts.line_omega 0)
+    int line_om_trials = 0;     // that search's trial solves
     bool timing = false;
     int timing_period = 1;
     int64_t timing_count = 0;

@@ -29,8 +29,12 @@
 // would write the XCD's L2 back once per block).  The host queues `batch`
 // iterations between reads of the control block, so the iteration at which
 // the solve stops is exactly the first one that meets the tolerance.
+#include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
+#include <functional>
+#include <utility>
 #include <vector>
 
 #include <hip/hip_ext.h>
@@ -1593,6 +1597,10 @@ static int g_nouter_prev = 0;
 // for any other.  The first round queues as many corrections as the
 // previous solve took (a correction queued after the loop has ended costs a
 // launch per kernel, about as much as one more read), later rounds one.
+// The line sweeps' over-relaxation: tt_solver_opts.line_omega where it is
+// set, else the value the search below chose (line_om_auto)
+static double line_om() { return L.opts.line_omega > 0.0 ? L.opts.line_omega : L.line_om_auto; }
+
 static int line_chain(int stag, int max_outer) {
     int nb = L.npartial;
     const long long tot = 2 * g_cs.ncell;
@@ -1612,10 +1620,8 @@ static int line_chain(int stag, int max_outer) {
             g_outer_tag = o;
             for (int b = 0; b < ni && rc == 0; ++b) {
                 // every iteration tested, inside the next red sweep
-                rc = launch_sweep<float>(0, c32[0], c32[1], x32[0], x32[1], L.opts.line_omega, b & 1, b, b > 0 ? 1 : 0,
-                                         1);
-                if (rc == 0)
-                    rc = launch_sweep<float>(1, c32[0], c32[1], x32[0], x32[1], L.opts.line_omega, b & 1, b, 0, 1);
+                rc = launch_sweep<float>(0, c32[0], c32[1], x32[0], x32[1], line_om(), b & 1, b, b > 0 ? 1 : 0, 1);
+                if (rc == 0) rc = launch_sweep<float>(1, c32[0], c32[1], x32[0], x32[1], line_om(), b & 1, b, 0, 1);
             }
             if (rc) break;
             k_finish<<<1, 256, 0, L.s>>>(L.ctrl, L.partial + ((ni - 1) & 1) * 2 * g_nb, 2 * g_pn, 1, nullptr);
@@ -1652,6 +1658,116 @@ static int line_chain(int stag, int max_outer) {
     return 0;
 }
 
+// the line system of the FP64 system on the device: allocated on first use,
+// factored once per system
+static int line_prepare() {
+    if (int rc = line_alloc()) return rc;
+    if (!g_line_ok) {
+        const int nl = g_ls.ny * g_ls.nz;
+        const int q = (g_ls.nx + 63) / 64;
+        if (q <= 1) line_factor_q<1>(nl);
+        else if (q == 2) line_factor_q<2>(nl);
+        else if (q == 3) line_factor_q<3>(nl);
+        else if (q == 4) line_factor_q<4>(nl);
+        else if (q <= 6) line_factor_q<6>(nl);
+        else line_factor_q<8>(nl);
+        TT_LAUNCH();
+        g_line_ok = true;
+    }
+    return 0;
+}
+
+// The over-relaxation of the line sweeps, where tt_solver_opts.line_omega is
+// 0: chosen on the first system a line solve meets after tt_init or
+// tt_set_solver, as ADJUST_SOR_PARAM (ppsor.f:332-354) chooses SORPARAM for
+// the point sweeps: cold-start solves of that system, each the whole defect
+// correction to the FP64 target, with omega = 1.0, 1.1, ... (until the count
+// exceeds 1.5 times the best, past it), then in steps of 0.02 from 0.08 below
+// the fastest coarse value to 0.08 above (below the lowest and above the
+// highest where several tie); the fewest line iterations win, the middle
+// value of a tie.  The chosen omega's corrections then keep the budget its
+// trial needed, as a solve keeps the previous solve's.  The best omega grows with the weight of the phi and theta
+// couplings: 1.13-1.16 at 128x256x128, about 1.55 at 192x512x256.  Nothing
+// here writes delta_ER; the unknowns are the caller's to set afterwards.
+static int line_omega_search() {
+    if (int rc = line_prepare()) return rc;
+    const bool timing = L.timing;
+    std::function<int()> tail;
+    tail.swap(L.sor_tail);  // what follows the solve waits for the solve itself
+    L.timing = false;
+    int best_no = 0, best_hist[16] = {0};
+    std::vector<std::pair<double, int>> tried;
+    long long pl = padded_len();
+    auto trial = [&](double om) -> int {
+        for (auto& t : tried)
+            if (fabs(t.first - om) < 1e-9) return 0;
+        int rc = set_ctrl_solve(L.p.itmax, 1, L.p.epsmat);
+        if (!rc) rc = dev_zero(x64[0], pl * sizeof(double));
+        if (!rc) rc = dev_zero(x64[1], pl * sizeof(double));
+        if (rc) return rc;
+        k_rhs_f32<<<nblk(2 * g_cs.ncell, 256), 256, 0, L.s>>>(g_cs, c64[0], c64[1], nullptr, nullptr, g_lco, g_ls.nx);
+        TT_LAUNCH();
+        k_outer_step<<<1, 64, 0, L.s>>>(L.ctrl, 1, L.p.epsmat, L.opts.inner_tol, L.opts.inner_max, 8, 200, L.p.itmax,
+                                        1);
+        TT_LAUNCH();
+        L.line_om_auto = om;
+        // each correction may run until its own test ends it (64 line
+        // iterations queued per correction, two corrections per round), so
+        // the count is omega's and not the previous solve's budget
+        g_nouter_prev = 2;
+        for (int q = 0; q < 16; ++q) g_hist_prev[q] = 63;
+        if ((rc = line_chain(8, 200))) return rc;
+        const int n = L.hctrl->done == 1 ? L.hctrl->total : INT_MAX;  // not converged: never chosen
+        int bestn = INT_MAX;
+        for (auto& t : tried) bestn = std::min(bestn, t.second);
+        if (n < bestn) {
+            best_no = g_nouter_prev;
+            for (int q = 0; q < 16; ++q) best_hist[q] = g_hist_prev[q];
+        }
+        tried.push_back({om, n});
+        return 0;
+    };
+    auto fewest = [&]() {
+        int bestn = INT_MAX;
+        for (auto& t : tried) bestn = std::min(bestn, t.second);
+        return bestn;
+    };
+    int rc = 0;
+    double best_om = 1.0;
+    for (int c = 0; c <= 9 && rc == 0; ++c) {
+        const double om = 1.0 + 0.1 * c;
+        rc = trial(om);
+        const int bn = fewest();
+        if (!rc && tried.back().second == bn) best_om = om;
+        if (!rc && om > best_om + 0.15 && (double)tried.back().second > 1.5 * bn) break;
+    }
+    if (!rc) {
+        const int bn = fewest();
+        double lo = 2.0, hi = 0.0;
+        for (auto& t : tried)
+            if (t.second == bn) lo = std::min(lo, t.first), hi = std::max(hi, t.first);
+        for (double om = std::max(1.0, lo - 0.08); om <= std::min(1.95, hi + 0.08) + 1e-9 && rc == 0; om += 0.02)
+            rc = trial(std::round(om * 100.0) / 100.0);
+    }
+    if (!rc) {
+        const int bn = fewest();
+        std::vector<double> ties;
+        for (auto& t : tried)
+            if (t.second == bn) ties.push_back(t.first);
+        std::sort(ties.begin(), ties.end());
+        if (bn == INT_MAX) rc = fail(1, "line sweeps: no over-relaxation in [1, 1.95] reached the FP64 target");
+        else L.line_om_auto = ties[ties.size() / 2];
+        L.line_om_trials = (int)tried.size();
+        g_nouter_prev = best_no;  // the budget of the chosen omega's corrections
+        for (int q = 0; q < 16; ++q) g_hist_prev[q] = best_hist[q];
+    }
+    if (rc) L.line_om_auto = 0.0;
+    L.timing = timing;
+    tail.swap(L.sor_tail);
+    if (rc) return rc;
+    return set_ctrl_solve(L.p.itmax, 1, L.p.epsmat);
+}
+
 // mode 0: delta_ER (x from 0 or a guess held in x64, zero ghosts, ITMIN=1,
 // ITMAX from the parameters); mode 1: DIFFUSION of T (x and ghosts from T,
 // ITMIN=5, SORITMAX=500, ppdif.f:14-15).
@@ -1674,6 +1790,8 @@ int sor_solve_impl(int fill_from_evals, tt_sor_stats* st, int mode) {
     if (rc) return rc;
     rc = sor_pack_impl();
     if (rc) return rc;
+    if (mode == 0 && L.opts.precision == 1 && L.opts.inner == 1 && !(line_om() > 0.0))
+        if ((rc = line_omega_search())) return rc;
     long long pl = padded_len();
     double* target = mode == 1 ? f[TT_T] : f[TT_DELTA_ER];
     int prec = mode == 1 ? 0 : L.opts.precision;
@@ -1777,19 +1895,7 @@ int sor_solve_impl(int fill_from_evals, tt_sor_stats* st, int mode) {
         // FP32 SOR inside FP64 defect correction
         const bool line = L.opts.inner == 1;
         if (line) {
-            if ((rc = line_alloc())) return rc;
-            if (!g_line_ok) {
-                const int nl = g_ls.ny * g_ls.nz;
-                const int q = (g_ls.nx + 63) / 64;
-                if (q <= 1) line_factor_q<1>(nl);
-                else if (q == 2) line_factor_q<2>(nl);
-                else if (q == 3) line_factor_q<3>(nl);
-                else if (q == 4) line_factor_q<4>(nl);
-                else if (q <= 6) line_factor_q<6>(nl);
-                else line_factor_q<8>(nl);
-                TT_LAUNCH();
-                g_line_ok = true;
-            }
+            if ((rc = line_prepare())) return rc;
         } else if (!g_c32_ok) {
             for (int c = 0; c < 2; ++c) {
                 k_to_f32<<<nblk(8 * g_cs.ncell, 256), 256, 0, L.s>>>(c64[c], c32[c], 8 * g_cs.ncell);
@@ -1961,6 +2067,13 @@ int tt_set_sorparam(double omega) {
 }
 
 double tt_get_sorparam(void) { return L.ready ? L.p.sorparam : 0.0; }
+
+int tt_get_line_omega(double* omega, int32_t* trials) {
+    if (!L.ready) return fail(1, "tt_init has not been called");
+    if (omega) *omega = line_om();
+    if (trials) *trials = L.opts.line_omega > 0.0 ? 0 : L.line_om_trials;
+    return 0;
+}
 
 int tt_sweep_timing(int period) {
     if (L.s) hipStreamSynchronize(L.s);

@@ -478,7 +478,9 @@ int tt_init(const tt_params* p) {
     L.opts.inner_tol = 1e-5;
     L.opts.warm_scale = 0.9;
     L.opts.inner = 0;
-    L.opts.line_omega = 1.1;
+    L.opts.line_omega = 0.0;  // chosen by a search on the first line solve
+    L.line_om_auto = 0.0;
+    L.line_om_trials = 0;
     TT_HIP(hipStreamSynchronize(L.s));
     evals_reset();
     L.ub_repeat = false;
@@ -522,8 +524,8 @@ int tt_set_solver(const tt_solver_opts* o) {
     if (o->precision < 0 || o->precision > 1 || o->guess < 0 || o->guess > 2 || o->batch < 1 ||
         o->adjust_omega < 0 || o->adjust_omega > 1 || o->inner < 0 || o->inner > 1)
         return fail(1, "invalid solver options");
-    if (o->inner == 1 && !(o->line_omega > 0.0 && o->line_omega < 2.0))
-        return fail(1, "invalid solver options: line_omega must lie in (0, 2)");
+    if (o->inner == 1 && !(o->line_omega == 0.0 || (o->line_omega > 0.0 && o->line_omega < 2.0)))
+        return fail(1, "invalid solver options: line_omega must be 0 (chosen by a search) or lie in (0, 2)");
     if (o->guess != L.opts.guess || o->precision != L.opts.precision || o->inner != L.opts.inner) {
         // sweep counts and rejections of another mode say nothing about this one
         L.start_avg[0] = L.start_avg[1] = 0.0;
@@ -532,6 +534,8 @@ int tt_set_solver(const tt_solver_opts* o) {
         L.learned_idle = 0;
     }
     L.opts = *o;
+    L.line_om_auto = 0.0;  // line_omega 0: a new search on the next line solve
+    L.line_om_trials = 0;
     return 0;
 }
 

@@ -50,6 +50,7 @@ def load_library():
     L.tt_sor_speculation_get.argtypes = [ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
     L.tt_adjust_sor_param.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_void_p]
     L.tt_set_sorparam.argtypes = [ctypes.c_double]
+    L.tt_get_line_omega.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32)]
     L.tt_set_limits.argtypes = [ctypes.POINTER(P.Limits)]
     L.tt_get_limits.argtypes = [ctypes.POINTER(P.Limits)]
     L.tt_varlimit.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double]
@@ -142,7 +143,9 @@ class Device:
 
     # ---- solver options
     def set_solver(self, precision=0, guess=0, batch=16, inner_max=4000, inner_tol=1e-5, warm_scale=0.9,
-                   adjust_omega=0, inner=0, line_omega=1.1):
+                   adjust_omega=0, inner=0, line_omega=0.0):
+        """tt_set_solver; line_omega 0 lets the first line solve choose the
+        line sweeps' over-relaxation by a search (get_line_omega)."""
         o = P.SolverOpts(precision, guess, batch, inner_max, adjust_omega, inner, inner_tol, warm_scale, line_omega)
         self._check(self.L.tt_set_solver(ctypes.byref(o)), "tt_set_solver")
 
@@ -180,6 +183,13 @@ class Device:
 
     def get_sorparam(self):
         return self.L.tt_get_sorparam()
+
+    def get_line_omega(self):
+        """(omega of the line sweeps, trial solves of the search that chose it)"""
+        om = ctypes.c_double()
+        n = ctypes.c_int32()
+        self._check(self.L.tt_get_line_omega(ctypes.byref(om), ctypes.byref(n)), "tt_get_line_omega")
+        return om.value, n.value
 
     # ---- the reference's routines
     def bounds(self):
