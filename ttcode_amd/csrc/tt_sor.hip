@@ -1685,7 +1685,8 @@ static int line_prepare() {
 // exceeds 1.5 times the best, past it), then in steps of 0.02 from 0.08 below
 // the fastest coarse value to 0.08 above (below the lowest and above the
 // highest where several tie); the fewest line iterations win, the middle
-// value of a tie.  The chosen omega's corrections then keep the budget its
+// value of a tie (where ITMAX ends every trial first, the smallest FP64
+// residual ratio reached).  The chosen omega's corrections then keep the budget its
 // trial needed, as a solve keeps the previous solve's.  The best omega grows with the weight of the phi and theta
 // couplings: 1.13-1.16 at 128x256x128, about 1.55 at 192x512x256.  Nothing
 // here writes delta_ER; the unknowns are the caller's to set afterwards.
@@ -1696,11 +1697,23 @@ static int line_omega_search() {
     tail.swap(L.sor_tail);  // what follows the solve waits for the solve itself
     L.timing = false;
     int best_no = 0, best_hist[16] = {0};
-    std::vector<std::pair<double, int>> tried;
+    struct Trial {
+        double om;
+        int n;         // line iterations to the FP64 target; INT_MAX where a cap ended it first
+        double ratio;  // the FP64 residual ratio it reached
+    };
+    std::vector<Trial> tried;
+    auto better = [](const Trial& a, const Trial& b) { return a.n != b.n ? a.n < b.n : a.ratio < b.ratio; };
+    auto best = [&]() -> const Trial& {
+        size_t k = 0;
+        for (size_t q = 1; q < tried.size(); ++q)
+            if (better(tried[q], tried[k])) k = q;
+        return tried[k];
+    };
     long long pl = padded_len();
     auto trial = [&](double om) -> int {
         for (auto& t : tried)
-            if (fabs(t.first - om) < 1e-9) return 0;
+            if (fabs(t.om - om) < 1e-9) return 0;
         int rc = set_ctrl_solve(L.p.itmax, 1, L.p.epsmat);
         if (!rc) rc = dev_zero(x64[0], pl * sizeof(double));
         if (!rc) rc = dev_zero(x64[1], pl * sizeof(double));
@@ -1717,46 +1730,39 @@ static int line_omega_search() {
         g_nouter_prev = 2;
         for (int q = 0; q < 16; ++q) g_hist_prev[q] = 63;
         if ((rc = line_chain(8, 200))) return rc;
-        const int n = L.hctrl->done == 1 ? L.hctrl->total : INT_MAX;  // not converged: never chosen
-        int bestn = INT_MAX;
-        for (auto& t : tried) bestn = std::min(bestn, t.second);
-        if (n < bestn) {
+        const Ctrl& c = *L.hctrl;
+        Trial t{om, c.done == 1 ? c.total : INT_MAX, c.rhs64 > 0.0 ? c.rn64 / c.rhs64 : 0.0};
+        if (tried.empty() || better(t, best())) {
             best_no = g_nouter_prev;
             for (int q = 0; q < 16; ++q) best_hist[q] = g_hist_prev[q];
         }
-        tried.push_back({om, n});
+        tried.push_back(t);
         return 0;
     };
-    auto fewest = [&]() {
-        int bestn = INT_MAX;
-        for (auto& t : tried) bestn = std::min(bestn, t.second);
-        return bestn;
-    };
     int rc = 0;
-    double best_om = 1.0;
     for (int c = 0; c <= 9 && rc == 0; ++c) {
         const double om = 1.0 + 0.1 * c;
-        rc = trial(om);
-        const int bn = fewest();
-        if (!rc && tried.back().second == bn) best_om = om;
-        if (!rc && om > best_om + 0.15 && (double)tried.back().second > 1.5 * bn) break;
+        if ((rc = trial(om))) break;
+        const Trial& b = best();
+        if (om > b.om + 0.15 && (tried.back().n == INT_MAX || (double)tried.back().n > 1.5 * b.n)) break;
     }
     if (!rc) {
-        const int bn = fewest();
-        double lo = 2.0, hi = 0.0;
+        const Trial b = best();
+        double lo = b.om, hi = b.om;
         for (auto& t : tried)
-            if (t.second == bn) lo = std::min(lo, t.first), hi = std::max(hi, t.first);
+            if (b.n != INT_MAX && t.n == b.n) lo = std::min(lo, t.om), hi = std::max(hi, t.om);
         for (double om = std::max(1.0, lo - 0.08); om <= std::min(1.95, hi + 0.08) + 1e-9 && rc == 0; om += 0.02)
             rc = trial(std::round(om * 100.0) / 100.0);
     }
     if (!rc) {
-        const int bn = fewest();
+        // the fewest iterations, the middle omega of a tie; where a cap
+        // (ITMAX) ended every trial, the smallest residual ratio
+        const Trial b = best();
         std::vector<double> ties;
         for (auto& t : tried)
-            if (t.second == bn) ties.push_back(t.first);
+            if (b.n != INT_MAX ? t.n == b.n : t.om == b.om) ties.push_back(t.om);
         std::sort(ties.begin(), ties.end());
-        if (bn == INT_MAX) rc = fail(1, "line sweeps: no over-relaxation in [1, 1.95] reached the FP64 target");
-        else L.line_om_auto = ties[ties.size() / 2];
+        L.line_om_auto = ties[ties.size() / 2];
         L.line_om_trials = (int)tried.size();
         g_nouter_prev = best_no;  // the budget of the chosen omega's corrections
         for (int q = 0; q < 16; ++q) g_hist_prev[q] = best_hist[q];
