@@ -64,3 +64,37 @@ def test_evals_survive_host_writes_of_their_inputs(ttgpu):
         assert C.rel_err(d.download("EVAL1"), want["EVAL1"]) <= 1e-12
     finally:
         d.close()
+
+
+def test_centcory_after_an_upload(ttgpu):
+    """CENTRIFCORIOLIS sets CENTCORY to zero (ppfor.f:435-486); the library
+    writes those zeros only after the host has put something else there, so
+    an upload of CENTCORY is undone by the next FORGAS3D, as in the
+    reference, and FORGAS3D's fields are the same either way."""
+    g = C.load("forgas_12x16x12")
+    p = C.params_of(g)
+    d = ttgpu.Device(p)
+    try:
+        d.load_state(C.inputs_of(g))
+        d.bounds()
+        d.qdens()                 # RHQX/Y/Z, as the parity test of FORGAS3D prepares them
+        d.forgas3d(float(g["delt"]))
+        want = {k: d.download(k) for k in ("V", "G", "H", "T", "centcory")}
+        assert not np.any(want["centcory"])
+        assert all(np.all(np.isfinite(want[k])) for k in ("V", "G", "H", "T"))
+        d.load_state(C.inputs_of(g))
+        d.upload("centcory", np.ones_like(want["centcory"]))
+        d.bounds()
+        d.qdens()
+        d.forgas3d(float(g["delt"]))
+        for k, v in want.items():
+            got = d.download(k)
+            if k == "centcory":
+                # declared (1:nx+1, 1:ny+1, 1:nz+1); CENTRIFCORIOLIS sets
+                # the cells 1..n and leaves the last plane of each axis
+                assert not np.any(got[:-1, :-1, :-1]), k
+                assert np.all(got[-1] == 1.0) and np.all(got[:, -1] == 1.0) and np.all(got[:, :, -1] == 1.0), k
+            else:
+                assert np.array_equal(got, v), k
+    finally:
+        d.close()

@@ -180,6 +180,7 @@ struct Lib {
     // GRAVY / GRAVZ hold +0.0 in every cell (as tt_init leaves them; cleared
     // by an upload of anything else): the force kernel then adds the
     // constant instead of reading the array, the same arithmetic
+    bool ccy_zero = true;  // CENTCORY is +0.0 on the interior (zeroed at tt_init, not uploaded since)
     bool gy_zero = true, gz_zero = true;
     // non-null: the kernels the *_impl helpers launch run only where this
     // device flag is non-zero (work queued ahead of a control-block read,

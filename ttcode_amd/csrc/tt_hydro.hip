@@ -619,7 +619,7 @@ __global__ void k_centcor(Geo g, Met m, double om, const double* __restrict__ G,
     double tq = GQk / xa * m(TT_M_GEOZG, k);
     long long c = IX(i, j, k);
     st_nt(&CCX[c], tq * tq + xa * (HQ * HQ));
-    st_nt(&CCY[c], 0.0);
+    if (CCY) st_nt(&CCY[c], 0.0);  // null: CCY holds these zeros already
     G1 = m(TT_M_DCY0, j + 1) * G[IX(i, j, k - 1)] + m(TT_M_DCY1, j + 1) * G[IX(i, j + 1, k - 1)];
     G2 = m(TT_M_DCY0, j + 1) * G[IX(i, j, k)] + m(TT_M_DCY1, j + 1) * G[IX(i, j + 1, k)];
     double GQ = m(TT_M_DDZ1, k) * G2 + m(TT_M_DDZ0, k) * G1 + om;
@@ -723,8 +723,9 @@ int forgas_impl(double dt) {
     const double* disfn = vis && L.visc.idiss != 0 ? f[TT_DISFN] : nullptr;
     long long nc = (long long)g.nx * g.ny * g.nz;
     k_centcor<<<nblk(nc, 256), 256, 0, L.s>>>(g, met(), L.p.omrot, f[TT_G], f[TT_H], L.scza, f[TT_CENTCORX],
-                                              f[TT_CENTCORY], f[TT_CENTCORZ]);
+                                              L.ccy_zero ? nullptr : f[TT_CENTCORY], f[TT_CENTCORZ]);
     TT_LAUNCH();
+    L.ccy_zero = true;
     const bool adiab = L.p.isoth == 0;
     if (adiab) {
         // TOLD = T, then T at t(n+1/2) by the p dV work: formed out of place

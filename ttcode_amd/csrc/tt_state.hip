@@ -246,6 +246,7 @@ static int xfer(int f, void* host, bool to_host) {
     int32_t lo[3], hi[3], rank, isint;
     if (tt_field_bounds(f, L.g.nx, L.g.ny, L.g.nz, lo, hi, &rank, &isint)) return fail(1, "unknown field id");
     if (!to_host && (f == TT_RH || isint)) L.ub_repeat = false;  // RH or upperbnd from the host
+    if (!to_host && f == TT_CENTCORY) L.ccy_zero = false;
     if (rank == 2) {
         size_t n = (size_t)(hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1);
         void* d = isint ? (void*)L.ub : (void*)L.mustar;
@@ -484,6 +485,7 @@ int tt_init(const tt_params* p) {
     TT_HIP(hipStreamSynchronize(L.s));
     evals_reset();
     L.ub_repeat = false;
+    L.ccy_zero = true;  // every field was zeroed above
     L.ready = true;
     return 0;
 }
